@@ -1,0 +1,231 @@
+/*
+ * mastg — MI355X-native element assembly, solve and sensitivity for MAST.
+ *
+ * C ABI of the drop-in boundary.  Plain pointers and sizes only; every entry
+ * point returns an int status (MASTG_OK = 0, negative on error) and never
+ * throws.  mastg_last_error() returns the message of the last failure on a
+ * context (or of the last failed create when ctx is NULL).
+ *
+ * The reference path this replaces (MAST, files relative to the reference root):
+ *   - MAST::NonlinearImplicitAssembly::residual_and_jacobian
+ *       src/base/nonlinear_implicit_assembly.cpp:64-266       -> mastg_assemble
+ *   - MAST::NonlinearImplicitAssembly::sensitivity_assemble +
+ *     AssemblyBase::calculate_output_adjoint_sensitivity_multiple_parameters_no_direct
+ *       src/base/nonlinear_implicit_assembly.cpp:515-686,
+ *       src/base/assembly_base.cpp:594-639                     -> mastg_sensitivity
+ *   - MAST::NonlinearSystem::solve / adjoint_solve (libMesh SNES + PETSc KSP)
+ *       src/base/nonlinear_system.cpp:209-235, 798-847         -> mastg_solve
+ *   - MAST::ComplianceOutput::evaluate / output_total
+ *       src/elasticity/compliance_output.cpp:71-102, 239-248   -> mastg_output
+ *   - the element-operation contract
+ *       src/base/nonlinear_implicit_assembly_elem_operations.h:35-116,
+ *       src/elasticity/structural_nonlinear_assembly.cpp:112-197,
+ *       src/heat_conduction/heat_conduction_nonlinear_assembly.cpp:81-118
+ *     is realised per physics by the element kernels selected with
+ *     mastg_problem_desc.physics.
+ *
+ * Vectors passed across the boundary use the caller's DofMap numbering
+ * (DOF(node, var) = node_dof_base[node] + var, as libMesh's var-major numbering
+ * with one variable group gives).  The global matrix is exported over the
+ * "active" variables (hex8 elasticity: ux uy uz; Mindlin plate: u v w tx ty;
+ * heat: T), with rows and columns in the caller's order: it is the caller's
+ * full-coupling sparsity pattern restricted to those variables.
+ */
+#ifndef MASTG_MASTG_H
+#define MASTG_MASTG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status -- */
+#define MASTG_OK 0
+#define MASTG_ERR_INVALID (-1)        /* bad argument / unsupported combination   */
+#define MASTG_ERR_OOM (-2)            /* device allocation failed                  */
+#define MASTG_ERR_NOT_CONVERGED (-3)  /* iterative solve hit its iteration limit  */
+#define MASTG_ERR_NAN (-4)            /* NaN / Inf detected in inputs or iterates  */
+#define MASTG_ERR_DEVICE (-5)         /* HIP runtime error                         */
+#define MASTG_ERR_ADMISSION (-6)      /* request exceeds the server's HBM budget   */
+#define MASTG_ERR_BUSY (-7)           /* server queue full; resubmit later         */
+
+typedef struct mastg_ctx mastg_ctx;
+
+/* physics of the element kernels (the elem-ops class the caller would attach) */
+#define MASTG_HEX8_ELASTIC 1   /* MAST::StructuralElement3D, linear strain, SIMP E_e = rho_e^p E */
+#define MASTG_QUAD4_MINDLIN 2  /* MAST::StructuralElement2D + MindlinBendingOperator, param h_e */
+#define MASTG_HEX8_HEAT 3      /* MAST::HeatConductionElementBase, k(T) = k0(1+alpha T), param q_e */
+
+/* Problem description.  Arrays are read during mastg_ctx_create only; the
+ * context keeps its own device-resident copies (mesh, DofMap, CSR structure,
+ * element->CSR position maps).  Replaces what MAST reads from the libMesh
+ * MeshBase / DofMap / boundary info and the property cards
+ * (src/elasticity/structural_system_initialization.cpp:37-55,
+ *  src/base/physics_discipline_base.cpp:45-121, 218-363). */
+typedef struct {
+    int32_t physics;             /* MASTG_* physics                                   */
+    int64_t n_nodes;
+    int64_t n_elems;
+    const double* xyz;           /* [n_nodes*3]                                        */
+    const int32_t* conn;         /* [n_elems*npe] libMesh local node order (npe 8 / 4) */
+    int32_t n_vars;              /* variables per node of the caller's system (6 or 1) */
+    const int64_t* node_dof_base;/* [n_nodes] or NULL: libMesh var-major first-touch  */
+    int64_t n_constrained;       /* homogeneous Dirichlet DOFs (caller numbering)      */
+    const int64_t* constrained_dofs;
+    int64_t n_sides;             /* boundary sides: (elem, libMesh side, boundary id)  */
+    const int32_t* side_elem;
+    const int32_t* side_index;
+    const int32_t* side_id;
+    const double* point_loads;   /* [n_dofs] nodal loads F (R -= F) or NULL            */
+    /* material / section / boundary data */
+    double E, nu, penalty;       /* elasticity; SIMP penalty p (E_e = rho^p E)         */
+    double kappa, offset;        /* Mindlin section shear factor, offset (0 supported) */
+    double pressure;             /* Mindlin uniform pressure along local -z            */
+    double k0, alpha;            /* conduction k(T) = k0 (1 + alpha T)                 */
+    double conv_h, conv_tinf;    /* convection coefficient / ambient temperature       */
+    uint32_t conv_bid_mask;      /* boundary ids carrying convection (bit mask)        */
+    int32_t flags;               /* reserved, 0                                        */
+} mastg_problem_desc;
+
+typedef struct {
+    int64_t n_dofs;         /* caller DOFs                                */
+    int64_t n_active;       /* DOFs of the active variables (matrix size) */
+    int64_t nnz;            /* nonzeros of the active CSR                 */
+    int64_t n_blocks;       /* node-pair blocks (nnz / nav^2)             */
+    int32_t n_active_vars;  /* nav                                        */
+    int32_t nodes_per_elem;
+    int64_t device_bytes;   /* resident device footprint of the context   */
+} mastg_sizes;
+
+/* pointer-location flags for vectors passed to the entry points */
+#define MASTG_HOST 0
+#define MASTG_DEVICE 1
+
+int mastg_ctx_create(const mastg_problem_desc* desc, mastg_ctx** out);
+int mastg_ctx_destroy(mastg_ctx* ctx);
+const char* mastg_last_error(const mastg_ctx* ctx);
+int mastg_get_sizes(const mastg_ctx* ctx, mastg_sizes* out);
+
+/* Sparsity of the active-variable matrix: row_ptr [n_active+1], col_idx [nnz],
+ * rows/cols numbered in the caller's order restricted to active variables.
+ * Replaces libMesh SparsityPattern::Build as attached by
+ * MAST::NonlinearSystem::init_data (src/base/nonlinear_system.cpp:114-159). */
+int mastg_get_csr_pattern(const mastg_ctx* ctx, int64_t* row_ptr, int32_t* col_idx);
+/* active index of each caller DOF (-1 for inactive variables), [n_dofs] */
+int mastg_get_active_map(const mastg_ctx* ctx, int64_t* active_of_dof);
+
+/* Per-element design parameter (rho_e / h_e / q_e), [n_elems]; `where` =
+ * MASTG_HOST or MASTG_DEVICE.  Replaces the per-element FieldFunction
+ * (examples/structural/example_6/example_6.cpp:100-137). */
+int mastg_set_params(mastg_ctx* ctx, const double* params, int where);
+
+/* element precision for the Jacobian build */
+#define MASTG_ELEM_FP64 0
+#define MASTG_ELEM_FP32 1
+
+/* NonlinearImplicitAssembly::residual_and_jacobian.  X, R: [n_dofs] caller
+ * numbering (X NULL = zero state); J_vals: [nnz] in mastg_get_csr_pattern
+ * order (NULL = residual only).  Element loop, libMesh constraint
+ * application (constrained rows/cols zeroed, unit diagonal per element),
+ * deterministic segmented-reduction scatter (no atomics) and point loads.
+ * The assembled matrix also stays resident for mastg_solve. */
+int mastg_assemble(mastg_ctx* ctx, const double* X, double* R, double* J_vals,
+                   int where, int elem_precision);
+
+/* ------------------------------------------------------------ solve ---- */
+#define MASTG_SOLVER_CG_JACOBI 0        /* FP64 CSR CG + Jacobi (plain path)           */
+#define MASTG_SOLVER_MIXED 1            /* FP32-SpMV PCG + FP64 reliable updates        */
+#define MASTG_SOLVER_BICGSTAB_JACOBI 2  /* FP64 BiCGStab + Jacobi (non-symmetric J)     */
+
+typedef struct {
+    int32_t solver;       /* MASTG_SOLVER_*                                   */
+    int32_t max_iters;    /* 0 -> 100000                                      */
+    double rtol;          /* ||b - A x||_2 / ||b||_2 target; 0 -> 1e-10        */
+    double reliable_delta;/* mixed: FP64 residual replacement trigger, 0 -> 0.1 */
+    int32_t transpose;    /* solve A^T x = b (adjoint_solve)                  */
+    int32_t check_every;  /* convergence polling interval in iterations, 0->32*/
+} mastg_solve_opts;
+
+typedef struct {
+    int32_t iters;          /* Krylov iterations                     */
+    int32_t replacements;   /* FP64 residual replacements (mixed)    */
+    double rel_res;         /* final true relative residual (FP64)   */
+    double seconds;         /* device time of the solve              */
+} mastg_solve_stats;
+
+/* Solve J x = rhs with the matrix of the last mastg_assemble; rhs, x in the
+ * caller numbering (inactive variables of x are set to 0). */
+int mastg_solve(mastg_ctx* ctx, const double* rhs, double* x, int where,
+                const mastg_solve_opts* opts, mastg_solve_stats* stats);
+
+/* g_e = lambda_e . dR_e/dp_e for every element (constrained), the batched form of
+ * sensitivity_assemble + the per-parameter dot of
+ * calculate_output_adjoint_sensitivity_multiple_parameters_no_direct. */
+int mastg_sensitivity(mastg_ctx* ctx, const double* X, const double* lambda, double* g,
+                      int where);
+
+/* outputs */
+#define MASTG_OUTPUT_COMPLIANCE 1   /* C = F . X (external loads, incl. point loads) */
+#define MASTG_OUTPUT_MEAN_T 2       /* (1/V) int T dV                                */
+int mastg_output(mastg_ctx* ctx, int kind, const double* X, int where, double* value,
+                 double* dvalue_dX /* [n_dofs] or NULL */);
+
+/* ------------------------------------------- one design evaluation ---- */
+/* assemble + solve + output + adjoint sensitivities with everything resident
+ * on the device: compliance and dC/dp for the elastic physics, mean
+ * temperature and d(mean T)/dq for heat (Newton for the k(T) nonlinearity).
+ * params / dobj are [n_elems]; `where` applies to both. */
+typedef struct {
+    double objective;
+    int32_t iters;          /* linear iterations (sum over Newton steps)    */
+    int32_t newton_iters;
+    double rel_res;
+    double t_assemble, t_solve, t_sens;   /* seconds (device events)       */
+} mastg_design_result;
+
+int mastg_design_eval(mastg_ctx* ctx, const double* params, double* dobj, int where,
+                      const mastg_solve_opts* opts, mastg_design_result* out);
+
+/* device vector access for callers that keep data resident */
+int mastg_device_alloc(mastg_ctx* ctx, size_t bytes, void** ptr);
+int mastg_device_free(mastg_ctx* ctx, void* ptr);
+int mastg_copy(mastg_ctx* ctx, void* dst, const void* src, size_t bytes, int kind /*0 h2d 1 d2h 2 d2d*/);
+int mastg_synchronize(mastg_ctx* ctx);
+
+/* Device time (milliseconds) of named kernels over the context's lifetime,
+ * measured with HIP events on the stream the kernels run on. kind:
+ * 0 element kernel, 1 scatter, 2 SpMV (FP64 CSR), 3 SpMV (mixed inner),
+ * 4 sensitivity; returns total ms and launch count. */
+int mastg_kernel_time(const mastg_ctx* ctx, int kind, double* total_ms, int64_t* launches);
+int mastg_enable_kernel_timing(mastg_ctx* ctx, int on);
+/* Replays one kernel of the resident state n_launch times back to back on the
+ * context's stream and returns the average duration from HIP events (same
+ * kinds as mastg_kernel_time).  For roofline measurement. */
+int mastg_bench_kernel(mastg_ctx* ctx, int kind, int n_launch, double* avg_ms);
+
+/* ------------------------------------------------ benchmark workloads -- */
+/* Structured problems of the benchmark configurations, built the way libMesh
+ * MeshTools::Generation::build_cube / build_square number meshes (what MAST's
+ * examples use, e.g. examples/structural/example_6/example_6.cpp), with the
+ * configurations' boundary conditions, loads and SplitMix64-seeded parameters:
+ *   kind 1: hex8 cantilever 1.0 x 0.5 x 0.5 m, nx x ny x nz, x=0 clamped,
+ *           traction -1e6 Pa (z) on x=1 as consistent nodal loads, rho ~ U[lo,hi)
+ *   kind 2: Mindlin plate 1 x 1 m, nx x ny quad4, edges u=v=w=0, p = 1e3 Pa,
+ *           h ~ U[0.002, 0.01)
+ *   kind 3: heat unit cube nx^3, T=0 on z=0, convection on the other faces,
+ *           q ~ U[0, 1e6)
+ * mast_exact_vars selects the MAST variable count (6 structural) over the
+ * active one.  The workload owns the arrays its description points to. */
+typedef struct mastg_workload mastg_workload;
+int mastg_workload_create(int kind, int nx, int ny, int nz, uint64_t seed, double lo, double hi,
+                          int mast_exact_vars, mastg_workload** out);
+const mastg_problem_desc* mastg_workload_desc(const mastg_workload* w);
+const double* mastg_workload_params(const mastg_workload* w);
+int mastg_workload_destroy(mastg_workload* w);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MASTG_MASTG_H */

@@ -1,0 +1,624 @@
+// C ABI of the mastg library (include/mastg/mastg.h): context set-up, the
+// device-resident assembly / solve / sensitivity pipeline and its timing.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "kernels.h"
+#include "mastg/mastg.h"
+#include "structure.h"
+
+using namespace mastg;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+struct KTimer {   // accumulated device time of one kernel kind
+    double ms = 0;
+    int64_t launches = 0;
+};
+
+}  // namespace
+
+struct mastg_ctx {
+    mastg_problem_desc d{};          // scalars of the description (pointers cleared)
+    Structure S;
+    std::string err;
+    hipStream_t stream = nullptr;
+    int nparts = 1024;
+    int64_t device_bytes = 0;
+    // device structure
+    DevBuf xyz, econn, blk_ptr, blk_col, row_ptr, col, cptr, cidx, diag_pos, con, dof_of_active;
+    DevBuf ninc_ptr, ninc;
+    // device data
+    DevBuf param, Kel, Rel, val, dinv, F;
+    DevBuf vx, vr, vz, vp, vq, vb, vtmp, vcaller, vg, part0, part1, scal, dscalar;
+    CGScalars* h_scal = nullptr;      // pinned mirror of the CG scalars
+    double* h_pin = nullptr;          // pinned scratch for scalar read-back
+    double Dhat[36];
+    bool assembled = false;
+    bool params_set = false;
+    // captured CG iterations
+    hipGraphExec_t cg_exec = nullptr;
+    int cg_exec_len = 0;
+    // timing
+    bool timing = false;
+    KTimer kt[8];
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace {
+
+int fail(mastg_ctx* c, int code, const std::string& m) {
+    if (c) c->err = m;
+    else g_create_error = m;
+    return code;
+}
+
+#define HIPC(c, expr)                                                                     \
+    do {                                                                                  \
+        hipError_t _e = (expr);                                                           \
+        if (_e != hipSuccess)                                                             \
+            return fail((c), MASTG_ERR_DEVICE,                                            \
+                        std::string(#expr " failed: ") + hipGetErrorString(_e));          \
+    } while (0)
+
+int dalloc(mastg_ctx* c, DevBuf& b, size_t bytes) {
+    if (b.p && b.bytes >= bytes) return MASTG_OK;
+    c->device_bytes -= int64_t(b.bytes);
+    b.release();
+    if (bytes == 0) bytes = 8;
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        return fail(c, MASTG_ERR_OOM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+    }
+    b.bytes = bytes;
+    c->device_bytes += int64_t(bytes);
+    return MASTG_OK;
+}
+
+template <class T>
+int upload(mastg_ctx* c, DevBuf& b, const std::vector<T>& v) {
+    int rc = dalloc(c, b, v.size() * sizeof(T));
+    if (rc) return rc;
+    if (!v.empty()) HIPC(c, hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return MASTG_OK;
+}
+
+#define TRY(x)                  \
+    do {                        \
+        int _rc = (x);          \
+        if (_rc) return _rc;    \
+    } while (0)
+
+void material_D3_unitE(double nu, double D[36]) {
+    // IsotropicMaterialProperty::StiffnessMatrix3D at E = 1
+    // (src/property_cards/isotropic_material_property_card.cpp:512-528)
+    std::memset(D, 0, 36 * sizeof(double));
+    for (int i = 0; i < 3; i++) {
+        for (int j = 0; j < 3; j++)
+            D[i * 6 + j] = (i == j) ? (1. - nu) / (1. - nu - 2. * nu * nu) : nu / (1. - nu - 2. * nu * nu);
+        D[(i + 3) * 6 + i + 3] = 1. / 2. / (1. + nu);
+    }
+}
+
+Hex8Args hex8_args(mastg_ctx* c, const double* X, bool want_K) {
+    Hex8Args a{};
+    a.n_elems = c->S.n_elems;
+    a.xyz = c->xyz.as<double>();
+    a.econn = c->econn.as<int32_t>();
+    a.param = c->param.as<double>();
+    a.E0 = c->d.E;
+    a.penalty = c->d.penalty;
+    std::memcpy(a.Dhat, c->Dhat, sizeof(a.Dhat));
+    a.con = c->con.as<uint8_t>();
+    a.X = X;
+    a.Rel = c->Rel.as<double>();
+    a.Kel = want_K ? c->Kel.as<double>() : nullptr;
+    return a;
+}
+
+CGArgs cg_args(mastg_ctx* c) {
+    CGArgs a{};
+    a.n = c->S.n_active;
+    a.row_ptr = c->row_ptr.as<int64_t>();
+    a.col = c->col.as<int32_t>();
+    a.val = c->val.as<double>();
+    a.dinv = c->dinv.as<double>();
+    a.x = c->vx.as<double>();
+    a.r = c->vr.as<double>();
+    a.z = c->vz.as<double>();
+    a.p = c->vp.as<double>();
+    a.q = c->vq.as<double>();
+    a.part0 = c->part0.as<double>();
+    a.part1 = c->part1.as<double>();
+    a.nparts = c->nparts;
+    a.S = c->scal.as<CGScalars>();
+    return a;
+}
+
+// timing helpers: bracket a launch with events when timing is on
+struct TimedRegion {
+    mastg_ctx* c;
+    int kind;
+    TimedRegion(mastg_ctx* c_, int k) : c(c_), kind(k) {
+        if (c->timing) (void)hipEventRecord(c->ev0, c->stream);
+    }
+    ~TimedRegion() {
+        if (!c->timing) return;
+        (void)hipEventRecord(c->ev1, c->stream);
+        (void)hipEventSynchronize(c->ev1);
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+        c->kt[kind].ms += ms;
+        c->kt[kind].launches += 1;
+    }
+};
+
+// caller vector -> active vector (device), or zero
+int to_active(mastg_ctx* c, const double* src, int where, double* dst_active) {
+    const int64_t n = c->S.n_active;
+    if (!src) {
+        launch_fill(n, 0.0, dst_active, c->stream);
+        return MASTG_OK;
+    }
+    const double* s = src;
+    if (where == MASTG_HOST) {
+        TRY(dalloc(c, c->vcaller, c->S.n_dofs * sizeof(double)));
+        HIPC(c, hipMemcpyAsync(c->vcaller.p, src, c->S.n_dofs * sizeof(double), hipMemcpyHostToDevice,
+                               c->stream));
+        s = c->vcaller.as<double>();
+    }
+    launch_gather(n, c->dof_of_active.as<int64_t>(), s, dst_active, c->stream);
+    return MASTG_OK;
+}
+
+// active vector (device) -> caller vector (inactive entries 0)
+int from_active(mastg_ctx* c, const double* src_active, double* dst, int where) {
+    TRY(dalloc(c, c->vcaller, c->S.n_dofs * sizeof(double)));
+    double* t = where == MASTG_HOST ? c->vcaller.as<double>() : dst;
+    launch_fill(c->S.n_dofs, 0.0, t, c->stream);
+    launch_scatter_vec(c->S.n_active, c->dof_of_active.as<int64_t>(), src_active, t, c->stream);
+    if (where == MASTG_HOST) {
+        HIPC(c, hipMemcpyAsync(dst, t, c->S.n_dofs * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPC(c, hipStreamSynchronize(c->stream));
+    }
+    return MASTG_OK;
+}
+
+int ensure_vectors(mastg_ctx* c) {
+    const size_t nb = size_t(c->S.n_active) * sizeof(double);
+    for (DevBuf* b : {&c->vx, &c->vr, &c->vz, &c->vp, &c->vq, &c->vb, &c->vtmp}) TRY(dalloc(c, *b, nb));
+    TRY(dalloc(c, c->part0, c->nparts * sizeof(double)));
+    TRY(dalloc(c, c->part1, c->nparts * sizeof(double)));
+    TRY(dalloc(c, c->scal, sizeof(CGScalars)));
+    TRY(dalloc(c, c->dscalar, 4 * sizeof(double)));
+    TRY(dalloc(c, c->vg, size_t(c->S.n_elems) * sizeof(double)));
+    return MASTG_OK;
+}
+
+// device dot of two active vectors -> host double
+int dot_active(mastg_ctx* c, const double* a, const double* b, double* out) {
+    launch_dot(c->S.n_active, a, b, c->part0.as<double>(), c->nparts, c->dscalar.as<double>(), c->stream);
+    HIPC(c, hipMemcpyAsync(c->h_pin, c->dscalar.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    *out = c->h_pin[0];
+    return MASTG_OK;
+}
+
+// ---- assembly: element kernel -> segmented scatter -> Jacobi diagonal
+int assemble_active(mastg_ctx* c, const double* X_active, bool want_R, bool want_K) {
+    if (!c->params_set) return fail(c, MASTG_ERR_INVALID, "design parameters not set");
+    const Structure& S = c->S;
+    if (c->d.physics != MASTG_HEX8_ELASTIC)
+        return fail(c, MASTG_ERR_INVALID, "physics not available in this build");
+    if (want_K) {
+        TRY(dalloc(c, c->Kel, size_t(S.n_elems) * S.npe * S.npe * S.nav * S.nav * sizeof(double)));
+        TRY(dalloc(c, c->val, size_t(S.nnz) * sizeof(double)));
+        TRY(dalloc(c, c->dinv, size_t(S.n_active) * sizeof(double)));
+    }
+    if (want_R) TRY(dalloc(c, c->Rel, size_t(S.n_elems) * S.npe * S.nav * sizeof(double)));
+    Hex8Args a = hex8_args(c, want_R ? X_active : nullptr, want_K);
+    if (want_R && !X_active) a.X = nullptr;
+    {
+        TimedRegion t(c, 0);
+        if (want_K || (want_R && X_active)) launch_hex8_K(a, c->stream);
+    }
+    if (want_K) {
+        ScatterArgs sa{S.n_nodes, S.nav, c->blk_ptr.as<int64_t>(), c->cptr.as<int32_t>(),
+                       c->cidx.as<int32_t>(), c->Kel.as<double>(), c->val.as<double>()};
+        {
+            TimedRegion t(c, 1);
+            launch_scatter(sa, c->stream);
+        }
+        launch_jacobi_inv(S.n_active, c->diag_pos.as<int64_t>(), c->val.as<double>(), c->dinv.as<double>(),
+                          c->stream);
+        c->assembled = true;
+    }
+    if (want_R)
+        launch_node_residual(S.n_nodes, S.nav, S.npe, c->ninc_ptr.as<int32_t>(), c->ninc.as<int32_t>(),
+                             X_active ? c->Rel.as<double>() : nullptr, c->F.as<double>(),
+                             c->con.as<uint8_t>(), c->vtmp.as<double>(), c->stream);
+    HIPC(c, hipGetLastError());
+    return MASTG_OK;
+}
+
+// ---- FP64 CSR CG + Jacobi on the resident matrix: b (active, device) -> c->vx
+int cg_solve(mastg_ctx* c, const double* b, const mastg_solve_opts* o, mastg_solve_stats* st) {
+    const Structure& S = c->S;
+    const double rtol = (o && o->rtol > 0) ? o->rtol : 1e-10;
+    const int max_it = (o && o->max_iters > 0) ? o->max_iters : 100000;
+    const int chunk = (o && o->check_every > 0) ? o->check_every : 32;
+    CGArgs a = cg_args(c);
+    hipEvent_t e0, e1;
+    HIPC(c, hipEventCreate(&e0));
+    HIPC(c, hipEventCreate(&e1));
+    HIPC(c, hipEventRecord(e0, c->stream));
+    launch_fill(S.n_active, 0.0, a.x, c->stream);
+    HIPC(c, hipMemcpyAsync(a.r, b, S.n_active * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    launch_cg_start(a, b, rtol * rtol, max_it, c->stream);
+    if (!c->cg_exec || c->cg_exec_len != chunk) {
+        if (c->cg_exec) (void)hipGraphExecDestroy(c->cg_exec);
+        c->cg_exec = nullptr;
+        hipGraph_t g;
+        HIPC(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        for (int i = 0; i < chunk; i++) launch_cg_iteration(a, c->stream);
+        HIPC(c, hipStreamEndCapture(c->stream, &g));
+        HIPC(c, hipGraphInstantiate(&c->cg_exec, g, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(g);
+        c->cg_exec_len = chunk;
+    }
+    int done = 0;
+    for (int launched = 0; launched < max_it + chunk; launched += chunk) {
+        HIPC(c, hipGraphLaunch(c->cg_exec, c->stream));
+        HIPC(c, hipMemcpyAsync(c->h_scal, a.S, sizeof(CGScalars), hipMemcpyDeviceToHost, c->stream));
+        HIPC(c, hipStreamSynchronize(c->stream));
+        done = c->h_scal->done;
+        if (done) break;
+    }
+    HIPC(c, hipEventRecord(e1, c->stream));
+    // true residual ||b - A x|| / ||b||
+    launch_spmv_csr(a, a.x, c->vtmp.as<double>(), 0, c->stream);
+    launch_axpby(S.n_active, 1.0, b, -1.0, c->vtmp.as<double>(), c->stream);
+    double rr = 0, bb = 0;
+    TRY(dot_active(c, c->vtmp.as<double>(), c->vtmp.as<double>(), &rr));
+    TRY(dot_active(c, b, b, &bb));
+    float ms = 0;
+    HIPC(c, hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (st) {
+        st->iters = c->h_scal->iter;
+        st->replacements = 0;
+        st->rel_res = bb > 0 ? std::sqrt(rr / bb) : 0.0;
+        st->seconds = ms * 1e-3;
+    }
+    if (done == 3 || done == 4) return fail(c, MASTG_ERR_NAN, "CG breakdown or non-finite iterate");
+    if (done != 1) return fail(c, MASTG_ERR_NOT_CONVERGED, "CG did not reach the tolerance");
+    return MASTG_OK;
+}
+
+}  // namespace
+
+// ========================================================================
+extern "C" {
+
+const char* mastg_last_error(const mastg_ctx* ctx) {
+    return ctx ? ctx->err.c_str() : g_create_error.c_str();
+}
+
+int mastg_ctx_create(const mastg_problem_desc* desc, mastg_ctx** out) {
+    if (!desc || !out) return fail(nullptr, MASTG_ERR_INVALID, "null argument");
+    *out = nullptr;
+    mastg_ctx* c = nullptr;
+    try {
+        c = new mastg_ctx;
+        c->d = *desc;
+        c->d.xyz = nullptr; c->d.conn = nullptr; c->d.node_dof_base = nullptr;
+        c->d.constrained_dofs = nullptr; c->d.side_elem = nullptr; c->d.side_index = nullptr;
+        c->d.side_id = nullptr; c->d.point_loads = nullptr;
+        if (c->d.penalty == 0) c->d.penalty = 3.0;
+        std::string m = build_structure(desc->physics, desc->n_nodes, desc->n_elems, desc->xyz, desc->conn,
+                                        desc->n_vars, desc->node_dof_base, desc->n_constrained,
+                                        desc->constrained_dofs, c->S);
+        if (!m.empty()) {
+            delete c;
+            return fail(nullptr, MASTG_ERR_INVALID, m);
+        }
+    } catch (const std::bad_alloc&) {
+        delete c;
+        return fail(nullptr, MASTG_ERR_OOM, "host allocation failed while building the structure");
+    }
+    const Structure& S = c->S;
+    int rc = MASTG_OK;
+    auto bail = [&](int r) {
+        g_create_error = c->err;
+        mastg_ctx_destroy(c);
+        return r;
+    };
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        g_create_error = "hipStreamCreate failed";
+        delete c;
+        return MASTG_ERR_DEVICE;
+    }
+    if (hipHostMalloc((void**)&c->h_scal, sizeof(CGScalars)) != hipSuccess ||
+        hipHostMalloc((void**)&c->h_pin, 64 * sizeof(double)) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        c->err = "pinned host allocation / event creation failed";
+        return bail(MASTG_ERR_DEVICE);
+    }
+    material_D3_unitE(c->d.nu, c->Dhat);
+    // loads: nodal point loads mapped onto the active layout
+    std::vector<double> F(S.n_active, 0.0);
+    if (desc->point_loads)
+        for (int64_t i = 0; i < S.n_active; i++) F[i] = desc->point_loads[S.caller_dof_of_active[i]];
+    std::vector<int32_t> conv(S.constrained.begin(), S.constrained.end());
+    if ((rc = upload(c, c->xyz, S.xyz_r)) || (rc = upload(c, c->econn, S.econn)) ||
+        (rc = upload(c, c->blk_ptr, S.blk_ptr)) || (rc = upload(c, c->blk_col, S.blk_col)) ||
+        (rc = upload(c, c->row_ptr, S.row_ptr)) || (rc = upload(c, c->col, S.col)) ||
+        (rc = upload(c, c->cptr, S.cptr)) || (rc = upload(c, c->cidx, S.cidx)) ||
+        (rc = upload(c, c->diag_pos, S.diag_pos)) || (rc = upload(c, c->con, S.constrained)) ||
+        (rc = upload(c, c->dof_of_active, S.caller_dof_of_active)) ||
+        (rc = upload(c, c->ninc_ptr, S.ninc_ptr)) || (rc = upload(c, c->ninc, S.ninc)) ||
+        (rc = upload(c, c->F, F)) || (rc = dalloc(c, c->param, S.n_elems * sizeof(double))) ||
+        (rc = ensure_vectors(c)))
+        return bail(rc);
+    *out = c;
+    return MASTG_OK;
+}
+
+int mastg_ctx_destroy(mastg_ctx* c) {
+    if (!c) return MASTG_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->cg_exec) (void)hipGraphExecDestroy(c->cg_exec);
+    if (c->h_scal) (void)hipHostFree(c->h_scal);
+    if (c->h_pin) (void)hipHostFree(c->h_pin);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    hipStream_t s = c->stream;
+    delete c;   // DevBuf destructors free device memory
+    if (s) (void)hipStreamDestroy(s);
+    return MASTG_OK;
+}
+
+int mastg_get_sizes(const mastg_ctx* c, mastg_sizes* o) {
+    if (!c || !o) return MASTG_ERR_INVALID;
+    o->n_dofs = c->S.n_dofs;
+    o->n_active = c->S.n_active;
+    o->nnz = c->S.nnz;
+    o->n_blocks = c->S.n_blocks;
+    o->n_active_vars = c->S.nav;
+    o->nodes_per_elem = c->S.npe;
+    o->device_bytes = c->device_bytes;
+    return MASTG_OK;
+}
+
+int mastg_get_csr_pattern(const mastg_ctx* c, int64_t* row_ptr, int32_t* col_idx) {
+    if (!c) return MASTG_ERR_INVALID;
+    // rows / columns in caller order restricted to the active variables: the
+    // active index is monotone in the caller index, so the pattern is the
+    // same sequence of positions
+    if (row_ptr) std::memcpy(row_ptr, c->S.row_ptr.data(), c->S.row_ptr.size() * sizeof(int64_t));
+    if (col_idx) std::memcpy(col_idx, c->S.col.data(), c->S.col.size() * sizeof(int32_t));
+    return MASTG_OK;
+}
+
+int mastg_get_active_map(const mastg_ctx* c, int64_t* a) {
+    if (!c || !a) return MASTG_ERR_INVALID;
+    for (int64_t i = 0; i < c->S.n_dofs; i++) a[i] = -1;
+    for (int64_t i = 0; i < c->S.n_active; i++) a[c->S.caller_dof_of_active[i]] = i;
+    return MASTG_OK;
+}
+
+int mastg_set_params(mastg_ctx* c, const double* p, int where) {
+    if (!c || !p) return MASTG_ERR_INVALID;
+    const size_t nb = c->S.n_elems * sizeof(double);
+    if (where == MASTG_HOST) {
+        for (int64_t e = 0; e < c->S.n_elems; e++)
+            if (!std::isfinite(p[e]) || p[e] <= 0.0)
+                return fail(c, MASTG_ERR_NAN, "design parameter " + std::to_string(e) + " is not a positive finite number");
+        HIPC(c, hipMemcpyAsync(c->param.p, p, nb, hipMemcpyHostToDevice, c->stream));
+    } else {
+        HIPC(c, hipMemcpyAsync(c->param.p, p, nb, hipMemcpyDeviceToDevice, c->stream));
+    }
+    c->params_set = true;
+    c->assembled = false;
+    return MASTG_OK;
+}
+
+int mastg_assemble(mastg_ctx* c, const double* X, double* R, double* J, int where, int prec) {
+    if (!c) return MASTG_ERR_INVALID;
+    if (prec != MASTG_ELEM_FP64) return fail(c, MASTG_ERR_INVALID, "only FP64 element precision in this build");
+    const Structure& S = c->S;
+    double* Xa = c->vb.as<double>();
+    TRY(to_active(c, X, where, Xa));
+    TRY(assemble_active(c, X ? Xa : nullptr, R != nullptr, J != nullptr || true));
+    if (R) TRY(from_active(c, c->vtmp.as<double>(), R, where));
+    if (J) {
+        HIPC(c, hipMemcpyAsync(J, c->val.p, S.nnz * sizeof(double),
+                               where == MASTG_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice,
+                               c->stream));
+    }
+    HIPC(c, hipStreamSynchronize(c->stream));
+    return MASTG_OK;
+}
+
+int mastg_solve(mastg_ctx* c, const double* rhs, double* x, int where, const mastg_solve_opts* o,
+                mastg_solve_stats* st) {
+    if (!c || !rhs || !x) return MASTG_ERR_INVALID;
+    if (!c->assembled) return fail(c, MASTG_ERR_INVALID, "no assembled matrix");
+    if (o && o->transpose) { /* the elastic Jacobian is symmetric */ }
+    if (o && o->solver != MASTG_SOLVER_CG_JACOBI)
+        return fail(c, MASTG_ERR_INVALID, "solver not available in this build");
+    TRY(to_active(c, rhs, where, c->vb.as<double>()));
+    int rc = cg_solve(c, c->vb.as<double>(), o, st);
+    int rc2 = from_active(c, c->vx.as<double>(), x, where);
+    return rc ? rc : rc2;
+}
+
+int mastg_sensitivity(mastg_ctx* c, const double* X, const double* lam, double* g, int where) {
+    if (!c || !X || !lam || !g) return MASTG_ERR_INVALID;
+    if (c->d.physics != MASTG_HEX8_ELASTIC) return fail(c, MASTG_ERR_INVALID, "physics not available");
+    TRY(to_active(c, X, where, c->vb.as<double>()));
+    TRY(to_active(c, lam, where, c->vtmp.as<double>()));
+    Hex8Args a = hex8_args(c, c->vb.as<double>(), false);
+    double* gd = where == MASTG_HOST ? c->vg.as<double>() : g;
+    {
+        TimedRegion t(c, 4);
+        launch_hex8_sens(a, c->vtmp.as<double>(), gd, c->stream);
+    }
+    if (where == MASTG_HOST)
+        HIPC(c, hipMemcpyAsync(g, gd, c->S.n_elems * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    return MASTG_OK;
+}
+
+int mastg_output(mastg_ctx* c, int kind, const double* X, int where, double* value, double* dv) {
+    if (!c || !X || !value) return MASTG_ERR_INVALID;
+    if (kind != MASTG_OUTPUT_COMPLIANCE) return fail(c, MASTG_ERR_INVALID, "output not available");
+    TRY(to_active(c, X, where, c->vb.as<double>()));
+    // external load vector with constrained entries removed
+    launch_node_residual(c->S.n_nodes, c->S.nav, c->S.npe, c->ninc_ptr.as<int32_t>(), c->ninc.as<int32_t>(),
+                         nullptr, c->F.as<double>(), c->con.as<uint8_t>(), c->vtmp.as<double>(), c->stream);
+    launch_axpby(c->S.n_active, 0.0, c->vtmp.as<double>(), -1.0, c->vtmp.as<double>(), c->stream);
+    TRY(dot_active(c, c->vtmp.as<double>(), c->vb.as<double>(), value));
+    if (dv) TRY(from_active(c, c->vtmp.as<double>(), dv, where));
+    return MASTG_OK;
+}
+
+int mastg_design_eval(mastg_ctx* c, const double* params, double* dobj, int where,
+                      const mastg_solve_opts* o, mastg_design_result* out) {
+    if (!c || !params) return MASTG_ERR_INVALID;
+    if (c->d.physics != MASTG_HEX8_ELASTIC) return fail(c, MASTG_ERR_INVALID, "physics not available");
+    const Structure& S = c->S;
+    TRY(mastg_set_params(c, params, where));
+    hipEvent_t e[4];
+    for (auto& x : e) HIPC(c, hipEventCreate(&x));
+    HIPC(c, hipEventRecord(e[0], c->stream));
+    TRY(assemble_active(c, nullptr, true, true));   // R(0) = -F -> b = F (constrained)
+    launch_axpby(S.n_active, 0.0, c->vtmp.as<double>(), -1.0, c->vtmp.as<double>(), c->stream);
+    HIPC(c, hipMemcpyAsync(c->vb.p, c->vtmp.p, S.n_active * sizeof(double), hipMemcpyDeviceToDevice,
+                           c->stream));
+    HIPC(c, hipEventRecord(e[1], c->stream));
+    mastg_solve_stats st{};
+    int rc = cg_solve(c, c->vb.as<double>(), o, &st);
+    HIPC(c, hipEventRecord(e[2], c->stream));
+    double C = 0;
+    TRY(dot_active(c, c->vb.as<double>(), c->vx.as<double>(), &C));
+    // self-adjoint compliance: lambda = u, dC/drho = -u . dR/drho
+    Hex8Args a = hex8_args(c, c->vx.as<double>(), false);
+    double* gd = (where == MASTG_HOST || !dobj) ? c->vg.as<double>() : dobj;
+    {
+        TimedRegion t(c, 4);
+        launch_hex8_sens(a, c->vx.as<double>(), gd, c->stream);
+    }
+    launch_axpby(S.n_elems, 0.0, gd, -1.0, gd, c->stream);
+    HIPC(c, hipEventRecord(e[3], c->stream));
+    if (dobj && where == MASTG_HOST)
+        HIPC(c, hipMemcpyAsync(dobj, gd, S.n_elems * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    float t01 = 0, t12 = 0, t23 = 0;
+    (void)hipEventElapsedTime(&t01, e[0], e[1]);
+    (void)hipEventElapsedTime(&t12, e[1], e[2]);
+    (void)hipEventElapsedTime(&t23, e[2], e[3]);
+    for (auto& x : e) (void)hipEventDestroy(x);
+    if (out) {
+        out->objective = C;
+        out->iters = st.iters;
+        out->newton_iters = 1;
+        out->rel_res = st.rel_res;
+        out->t_assemble = t01 * 1e-3;
+        out->t_solve = t12 * 1e-3;
+        out->t_sens = t23 * 1e-3;
+    }
+    return rc;
+}
+
+int mastg_device_alloc(mastg_ctx* c, size_t bytes, void** p) {
+    if (!c || !p) return MASTG_ERR_INVALID;
+    HIPC(c, hipMalloc(p, bytes));
+    return MASTG_OK;
+}
+int mastg_device_free(mastg_ctx* c, void* p) {
+    if (!c) return MASTG_ERR_INVALID;
+    HIPC(c, hipFree(p));
+    return MASTG_OK;
+}
+int mastg_copy(mastg_ctx* c, void* dst, const void* src, size_t bytes, int kind) {
+    if (!c) return MASTG_ERR_INVALID;
+    const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : kind == 1 ? hipMemcpyDeviceToHost
+                                                                          : hipMemcpyDeviceToDevice;
+    HIPC(c, hipMemcpyAsync(dst, src, bytes, k, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    return MASTG_OK;
+}
+int mastg_synchronize(mastg_ctx* c) {
+    if (!c) return MASTG_ERR_INVALID;
+    HIPC(c, hipStreamSynchronize(c->stream));
+    return MASTG_OK;
+}
+int mastg_enable_kernel_timing(mastg_ctx* c, int on) {
+    if (!c) return MASTG_ERR_INVALID;
+    c->timing = on != 0;
+    return MASTG_OK;
+}
+int mastg_bench_kernel(mastg_ctx* c, int kind, int n_launch, double* avg_ms) {
+    if (!c || !avg_ms || n_launch <= 0) return MASTG_ERR_INVALID;
+    if (!c->assembled) return fail(c, MASTG_ERR_INVALID, "assemble first");
+    CGArgs a = cg_args(c);
+    Hex8Args h = hex8_args(c, nullptr, true);
+    ScatterArgs sa{c->S.n_nodes, c->S.nav, c->blk_ptr.as<int64_t>(), c->cptr.as<int32_t>(),
+                   c->cidx.as<int32_t>(), c->Kel.as<double>(), c->val.as<double>()};
+    // the CG scalars must not read "done" for the replayed SpMV
+    HIPC(c, hipMemsetAsync(c->scal.p, 0, sizeof(CGScalars), c->stream));
+    auto launch = [&]() {
+        switch (kind) {
+            case 0: launch_hex8_K(h, c->stream); break;
+            case 1: launch_scatter(sa, c->stream); break;
+            case 2: launch_spmv_csr(a, a.p, a.q, 1, c->stream); break;
+            case 4: launch_hex8_sens(hex8_args(c, a.x, false), a.x, c->vg.as<double>(), c->stream); break;
+            default: break;
+        }
+    };
+    if (kind != 0 && kind != 1 && kind != 2 && kind != 4) return fail(c, MASTG_ERR_INVALID, "unknown kernel kind");
+    launch();   // warm
+    HIPC(c, hipEventRecord(c->ev0, c->stream));
+    for (int i = 0; i < n_launch; i++) launch();
+    HIPC(c, hipEventRecord(c->ev1, c->stream));
+    HIPC(c, hipEventSynchronize(c->ev1));
+    float ms = 0;
+    HIPC(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    *avg_ms = double(ms) / n_launch;
+    HIPC(c, hipGetLastError());
+    return MASTG_OK;
+}
+
+int mastg_kernel_time(const mastg_ctx* c, int kind, double* ms, int64_t* n) {
+    if (!c || kind < 0 || kind >= 8) return MASTG_ERR_INVALID;
+    if (ms) *ms = c->kt[kind].ms;
+    if (n) *n = c->kt[kind].launches;
+    return MASTG_OK;
+}
+
+}  // extern "C"

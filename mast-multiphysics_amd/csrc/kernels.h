@@ -1,0 +1,66 @@
+// Kernel argument blocks and launchers shared by kernels.hip and context.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mastg {
+
+struct Hex8Args {
+    int64_t n_elems;
+    const double* xyz;      // [n_nodes*3] by node rank
+    const int32_t* econn;   // [n_elems*8] node ranks
+    const double* param;    // [n_elems] rho_e
+    double E0, penalty;
+    double Dhat[36];        // material matrix at E = 1
+    const uint8_t* con;     // [n_active] Dirichlet mask
+    const double* X;        // [n_active] state or null
+    double* Rel;            // [n_elems*8*3] element residual rows (when X)
+    double* Kel;            // [n_elems*64*9] element blocks or null
+};
+
+struct ScatterArgs {
+    int64_t n_nodes;
+    int nav;
+    const int64_t* blk_ptr;
+    const int32_t* cptr;
+    const int32_t* cidx;
+    const double* Kel;
+    double* val;
+};
+
+struct CGScalars {
+    double rz, pq, rr, bb, alpha, beta, tol2;
+    int iter, max_it, done, pad;
+};
+
+struct CGArgs {
+    int64_t n;
+    const int64_t* row_ptr;
+    const int32_t* col;
+    const double* val;
+    const double* dinv;
+    double *x, *r, *z, *p, *q;
+    double *part0, *part1;
+    int nparts;
+    CGScalars* S;
+};
+
+void launch_hex8_K(const Hex8Args& a, hipStream_t s);
+void launch_hex8_sens(const Hex8Args& a, const double* lam, double* g, hipStream_t s);
+void launch_scatter(const ScatterArgs& a, hipStream_t s);
+void launch_node_residual(int64_t n_nodes, int nav, int npe, const int32_t* ninc_ptr,
+                          const int32_t* ninc, const double* Rel, const double* F,
+                          const uint8_t* con, double* R, hipStream_t s);
+void launch_gather(int64_t n, const int64_t* idx, const double* src, double* dst, hipStream_t s);
+void launch_scatter_vec(int64_t n, const int64_t* idx, const double* src, double* dst, hipStream_t s);
+void launch_fill(int64_t n, double v, double* x, hipStream_t s);
+void launch_axpby(int64_t n, double a, const double* x, double b, double* y, hipStream_t s);
+void launch_jacobi_inv(int64_t n, const int64_t* diag_pos, const double* val, double* dinv,
+                       hipStream_t s);
+void launch_dot(int64_t n, const double* a, const double* b, double* partials, int nparts,
+                double* out, hipStream_t s);
+void launch_spmv_csr(const CGArgs& c, const double* x, double* y, int want_dot, hipStream_t s);
+void launch_cg_start(const CGArgs& c, const double* b, double tol2, int max_it, hipStream_t s);
+void launch_cg_iteration(const CGArgs& c, hipStream_t s);
+
+}  // namespace mastg
