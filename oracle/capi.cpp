@@ -4,6 +4,7 @@
 #include <cstring>
 #include <memory>
 #include <omp.h>
+#include <string>
 
 using namespace oracle;
 
@@ -35,6 +36,26 @@ void* oracle_problem_new(int kind, int nx, int ny, int nz, uint64_t seed, double
 void oracle_problem_free(void* h) { delete H(h); }
 void oracle_set_threads(void* h, int t) { H(h)->threads = t; }
 int oracle_max_threads() { return omp_get_max_threads(); }
+
+// set a scalar of the problem: E nu penalty kappa off k0 alpha conv_h conv_Tinf
+// conv_mask pressure; returns 0 on success, -1 for an unknown name
+int oracle_set_scalar(void* h, const char* name, double v) {
+    Problem& P = H(h)->P;
+    const std::string n(name);
+    if (n == "E") P.E = v;
+    else if (n == "nu") P.nu = v;
+    else if (n == "penalty") P.penalty = v;
+    else if (n == "kappa") P.kappa = v;
+    else if (n == "off") P.off = v;
+    else if (n == "k0") P.k0 = v;
+    else if (n == "alpha") P.alpha = v;
+    else if (n == "conv_h") P.conv_h = v;
+    else if (n == "conv_Tinf") P.conv_Tinf = v;
+    else if (n == "conv_mask") P.conv_bid_mask = uint32_t(v);
+    else if (n == "pressure") P.pressure = v;
+    else return -1;
+    return 0;
+}
 
 int64_t oracle_n_nodes(void* h) { return H(h)->P.mesh.n_nodes; }
 int64_t oracle_n_elems(void* h) { return H(h)->P.mesh.n_elems; }

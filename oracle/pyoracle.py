@@ -31,6 +31,7 @@ def lib():
             "oracle_problem_free": (None, [_P]),
             "oracle_set_threads": (None, [_P, C.c_int]),
             "oracle_max_threads": (C.c_int, []),
+            "oracle_set_scalar": (C.c_int, [_P, C.c_char_p, C.c_double]),
             "oracle_n_nodes": (C.c_int64, [_P]),
             "oracle_n_elems": (C.c_int64, [_P]),
             "oracle_n_dofs": (C.c_int64, [_P]),
@@ -101,6 +102,12 @@ class Problem:
         if getattr(self, "h", None) and _lib is not None:
             _lib.oracle_problem_free(self.h)
             self.h = None
+
+    def set(self, **kw):
+        for k, v in kw.items():
+            if lib().oracle_set_scalar(self.h, k.encode(), float(v)):
+                raise KeyError(k)
+        return self
 
     def mesh(self):
         xyz = np.zeros((self.n_nodes, 3))
