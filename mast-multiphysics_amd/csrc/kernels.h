@@ -12,7 +12,7 @@ struct Hex8Args {
     const double* param;    // [n_elems] rho_e
     double E0, penalty;
     double Dhat[36];        // material matrix at E = 1
-    const uint8_t* con;     // [n_active] Dirichlet mask
+    const uint8_t* con;     // [n_active] Dirichlet mask, null = no constraints
     const double* X;        // [n_active] state or null
     double* Rel;            // [n_elems*8*3] element residual rows (when X)
     double* Kel;            // [n_elems*64*9] element blocks or null
@@ -33,6 +33,19 @@ struct CGScalars {
     int iter, max_it, done, pad;
 };
 
+// node-centric matrix-free operator over congruence classes (hex8, nav = 3)
+struct MFArgs {
+    int64_t n_nodes;
+    int n_classes;
+    const double* Khat;       // [n_classes*64*9] class matrices at E = 1, block layout
+    const double* sE;         // [n_elems] element scale E0 rho^p
+    const uint8_t* eclass;    // [n_elems]
+    const int32_t* econn;     // [n_elems*8]
+    const int32_t* ninc_ptr;  // [n_nodes+1]
+    const int32_t* ninc;      // e*8 + la, la ascending per node
+    const uint8_t* nodecon;   // [n_nodes] bit v = variable v constrained
+};
+
 struct CGArgs {
     int64_t n;
     const int64_t* row_ptr;
@@ -43,6 +56,8 @@ struct CGArgs {
     double *part0, *part1;
     int nparts;
     CGScalars* S;
+    int use_mf;               // operator: 0 CSR SpMV, 1 matrix-free
+    MFArgs mf;
 };
 
 void launch_hex8_K(const Hex8Args& a, hipStream_t s);
@@ -62,5 +77,8 @@ void launch_dot(int64_t n, const double* a, const double* b, double* partials, i
 void launch_spmv_csr(const CGArgs& c, const double* x, double* y, int want_dot, hipStream_t s);
 void launch_cg_start(const CGArgs& c, const double* b, double tol2, int max_it, hipStream_t s);
 void launch_cg_iteration(const CGArgs& c, hipStream_t s);
+void launch_elem_scale(int64_t n, const double* rho, double E0, double penalty, double* sE,
+                       hipStream_t s);
+void launch_mf_apply(const CGArgs& c, const double* x, double* y, int want_dot, hipStream_t s);
 
 }  // namespace mastg

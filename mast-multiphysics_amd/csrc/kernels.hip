@@ -12,6 +12,8 @@
 
 #include "kernels.h"
 
+#define Structure_kMaxClasses 8
+
 namespace mastg {
 
 // 2-point Gauss abscissa; the 2x2x2 rule has unit weights (libMesh QGauss,
@@ -140,12 +142,13 @@ __global__ __launch_bounds__(256) void k_hex8_elastic_K(Hex8Args A) {
         if (b == 0)
 #pragma unroll
             for (int v = 0; v < 3; v++)
-                A.Rel[(e * 8 + a) * 3 + v] = A.con[int64_t(ra) * 3 + v] ? 0.0 : f[v];
+                A.Rel[(e * 8 + a) * 3 + v] = (A.con && A.con[int64_t(ra) * 3 + v]) ? 0.0 : f[v];
     }
     if (A.Kel) {
-        uint8_t ca[3], cb[3];
+        uint8_t ca[3] = {0, 0, 0}, cb[3] = {0, 0, 0};
+        if (A.con)
 #pragma unroll
-        for (int v = 0; v < 3; v++) { ca[v] = A.con[int64_t(ra) * 3 + v]; cb[v] = A.con[int64_t(rb) * 3 + v]; }
+            for (int v = 0; v < 3; v++) { ca[v] = A.con[int64_t(ra) * 3 + v]; cb[v] = A.con[int64_t(rb) * 3 + v]; }
         double* out = A.Kel + (e * 64 + lane) * 9;
 #pragma unroll
         for (int v = 0; v < 3; v++)
@@ -339,6 +342,72 @@ __global__ __launch_bounds__(256) void k_reduce_to(const double* __restrict__ p,
     if (threadIdx.x == 0) *out = s;
 }
 
+
+// ---------------------------------------------------------------------------
+// Matrix-free hex8 operator y = K x over congruence classes:
+//   K = sum_e s_e P_e^T Khat_{class(e)} P_e   (s_e = E0 rho_e^p)
+// with libMesh's constraint convention (constrained rows / columns removed,
+// diagonal = number of elements at the node), i.e. the same matrix the
+// segmented scatter assembles, applied node by node: the thread of node r walks
+// its element incidences in ascending local index, so on structured meshes a
+// wave reads the same Khat block rows from LDS (broadcast) in lock step.
+// Optionally accumulates x.y partials per block (fused CG dot).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_hex8_mf_apply(CGArgs C, const double* __restrict__ x,
+                                                       double* __restrict__ y, int want_dot,
+                                                       int in_cg_loop) {
+    if (in_cg_loop && C.S->done) return;
+    __shared__ double Kh[Structure_kMaxClasses * 64 * 9];
+    const MFArgs& A = C.mf;
+    for (int i = threadIdx.x; i < A.n_classes * 64 * 9; i += 256) Kh[i] = A.Khat[i];
+    __syncthreads();
+    double dsum = 0.0;
+    for (int64_t r = int64_t(blockIdx.x) * 256 + threadIdx.x; r < A.n_nodes;
+         r += int64_t(gridDim.x) * 256) {
+        double y0 = 0.0, y1 = 0.0, y2 = 0.0;
+        const int32_t k0 = A.ninc_ptr[r], k1 = A.ninc_ptr[r + 1];
+        for (int32_t k = k0; k < k1; k++) {
+            const int32_t c = A.ninc[k];
+            const int64_t e = c >> 3;
+            const int la = c & 7;
+            const double s = A.sE[e];
+            const double* K = Kh + (int(A.eclass[e]) * 64 + la * 8) * 9;
+            double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+#pragma unroll
+            for (int lb = 0; lb < 8; lb++) {
+                const int64_t rb = A.econn[e * 8 + lb];
+                const int m = A.nodecon[rb];
+                const double u0 = (m & 1) ? 0.0 : x[rb * 3 + 0];
+                const double u1 = (m & 2) ? 0.0 : x[rb * 3 + 1];
+                const double u2 = (m & 4) ? 0.0 : x[rb * 3 + 2];
+                t0 += K[lb * 9 + 0] * u0 + K[lb * 9 + 1] * u1 + K[lb * 9 + 2] * u2;
+                t1 += K[lb * 9 + 3] * u0 + K[lb * 9 + 4] * u1 + K[lb * 9 + 5] * u2;
+                t2 += K[lb * 9 + 6] * u0 + K[lb * 9 + 7] * u1 + K[lb * 9 + 8] * u2;
+            }
+            y0 += s * t0;
+            y1 += s * t1;
+            y2 += s * t2;
+        }
+        const int m = A.nodecon[r];
+        const double cnt = double(k1 - k0);
+        const double x0 = x[r * 3 + 0], x1 = x[r * 3 + 1], x2 = x[r * 3 + 2];
+        if (m & 1) y0 = cnt * x0;
+        if (m & 2) y1 = cnt * x1;
+        if (m & 4) y2 = cnt * x2;
+        y[r * 3 + 0] = y0;
+        y[r * 3 + 1] = y1;
+        y[r * 3 + 2] = y2;
+        dsum += y0 * x0 + y1 * x1 + y2 * x2;
+    }
+    if (want_dot) block_sum_store(dsum, C.part0, blockIdx.x);
+}
+
+__global__ void k_elem_scale(int64_t n, const double* __restrict__ rho, double E0, double p,
+                             double* __restrict__ sE) {
+    const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e < n) sE[e] = E0 * pow(rho[e], p);
+}
+
 // ------------------------------------------------------- CG (FP64 CSR) ----
 // y = A x over the scalar CSR, 16 lanes per row, and partial x.y per block.
 __global__ __launch_bounds__(256) void k_spmv_csr_dot(CGArgs C, const double* __restrict__ x,
@@ -494,8 +563,18 @@ void launch_cg_start(const CGArgs& c, const double* b, double tol2, int max_it, 
     hipLaunchKernelGGL(k_cg_start, dim3(c.nparts), dim3(256), 0, s, c, b);
     hipLaunchKernelGGL(k_cg_start_scalars, dim3(1), dim3(256), 0, s, c, tol2, max_it);
 }
+void launch_elem_scale(int64_t n, const double* rho, double E0, double penalty, double* sE,
+                       hipStream_t s) {
+    hipLaunchKernelGGL(k_elem_scale, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, rho, E0, penalty, sE);
+}
+void launch_mf_apply(const CGArgs& c, const double* x, double* y, int want_dot, hipStream_t s) {
+    hipLaunchKernelGGL(k_hex8_mf_apply, dim3(c.nparts), dim3(256), 0, s, c, x, y, want_dot, 0);
+}
 void launch_cg_iteration(const CGArgs& c, hipStream_t s) {
-    hipLaunchKernelGGL(k_spmv_csr_dot, dim3(c.nparts), dim3(256), 0, s, c, c.p, c.q, 1, 1);
+    if (c.use_mf)
+        hipLaunchKernelGGL(k_hex8_mf_apply, dim3(c.nparts), dim3(256), 0, s, c, c.p, c.q, 1, 1);
+    else
+        hipLaunchKernelGGL(k_spmv_csr_dot, dim3(c.nparts), dim3(256), 0, s, c, c.p, c.q, 1, 1);
     hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(256), 0, s, c);
     hipLaunchKernelGGL(k_cg_update, dim3(c.nparts), dim3(256), 0, s, c);
     hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(256), 0, s, c);

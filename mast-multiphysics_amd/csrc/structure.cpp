@@ -9,6 +9,7 @@
 #include "structure.h"
 
 #include <algorithm>
+#include <cmath>
 #include <numeric>
 
 namespace mastg {
@@ -90,9 +91,44 @@ std::string build_structure(int physics, int64_t n_nodes, int64_t n_elems, const
     for (int64_t r = 0; r < n_nodes; r++) S.ninc_ptr[r + 1] += S.ninc_ptr[r];
     S.ninc.resize(n_elems * npe);
     {
+        // filled local-index-major so each node's incidences are sorted by la:
+        // on structured meshes the lanes of a wave then walk la in lock step
         std::vector<int32_t> next(S.ninc_ptr.begin(), S.ninc_ptr.end() - 1);
-        for (int64_t e = 0; e < n_elems; e++)
-            for (int a = 0; a < npe; a++) S.ninc[next[S.econn[e * npe + a]]++] = int32_t(e * npe + a);
+        for (int a = 0; a < npe; a++)
+            for (int64_t e = 0; e < n_elems; e++) S.ninc[next[S.econn[e * npe + a]]++] = int32_t(e * npe + a);
+    }
+    // ---- congruence classes
+    {
+        S.elem_class.assign(n_elems, 0);
+        S.n_classes = 0;
+        std::vector<std::vector<double>> rep;
+        bool ok = true;
+        std::vector<double> rel(npe * 3);
+        for (int64_t e = 0; e < n_elems && ok; e++) {
+            double size = 0;
+            for (int a = 0; a < npe; a++)
+                for (int i = 0; i < 3; i++) {
+                    rel[a * 3 + i] = S.xyz_r[int64_t(S.econn[e * npe + a]) * 3 + i] -
+                                     S.xyz_r[int64_t(S.econn[e * npe]) * 3 + i];
+                    size = std::max(size, std::abs(rel[a * 3 + i]));
+                }
+            int found = -1;
+            for (int c = 0; c < int(rep.size()) && found < 0; c++) {
+                bool same = true;
+                for (int k = 0; k < npe * 3 && same; k++)
+                    same = std::abs(rel[k] - rep[c][k]) <= 1e-12 * size;
+                if (same) found = c;
+            }
+            if (found < 0) {
+                if (int(rep.size()) == Structure::kMaxClasses) { ok = false; break; }
+                found = int(rep.size());
+                rep.push_back(rel);
+                S.class_rep.push_back(int32_t(e));
+            }
+            S.elem_class[e] = uint8_t(found);
+        }
+        S.n_classes = ok ? int(rep.size()) : 0;
+        if (!ok) { S.class_rep.clear(); S.elem_class.clear(); }
     }
     // ---- node-block pattern: sorted union of element node ranks around each node
     S.blk_ptr.assign(n_nodes + 1, 0);

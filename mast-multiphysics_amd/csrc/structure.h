@@ -44,7 +44,15 @@ struct Structure {
     std::vector<uint8_t> constrained;      // [n_active]
     // node -> (element, local node) incidence, for residual / load assembly
     std::vector<int32_t> ninc_ptr;         // [n_nodes+1]
-    std::vector<int32_t> ninc;             // [n_elems*npe] = e*npe + la
+    std::vector<int32_t> ninc;             // [n_elems*npe] = e*npe + la (la ascending per node)
+
+    // congruence classes: elements whose node coordinates relative to their
+    // first node agree (to 1e-12 of the element size) share one reference
+    // element matrix; n_classes = 0 when the mesh has more than kMaxClasses
+    static constexpr int kMaxClasses = 8;
+    int n_classes = 0;
+    std::vector<uint8_t> elem_class;       // [n_elems]
+    std::vector<int32_t> class_rep;        // [n_classes] representative element
 };
 
 // Builds the structure; returns an empty string on success, else a message.
