@@ -33,6 +33,13 @@ def spmv_bytes(n, nnz):
     return nnz * 12 + (n + 1) * 8 + 2 * n * 8
 
 
+def mf_bytes(n_active, n_elems):
+    # x read + y write (FP64), element scale (FP64), class id (1 B), connectivity
+    # (8 x int32) and node incidences (8 x int32 per element + pointers)
+    n_nodes = n_active // 3
+    return 2 * n_active * 8 + n_elems * (8 + 1 + 32 + 32) + (n_nodes + 1) * 4 + n_nodes
+
+
 def cpu_baseline(wl, gpu_iters, budget_s=20.0):
     """Oracle (FP64 C++ restatement of MAST's algorithm, OpenMP over all host
     cores) timed on a bounded sample and scaled to one design of the workload."""
@@ -70,7 +77,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
-    ap.add_argument("--solver", default="plain", choices=["plain"])
+    ap.add_argument("--solver", default="both", choices=["plain", "matfree", "both"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--seed", type=int, default=1)
     a = ap.parse_args()
@@ -84,34 +91,51 @@ def main():
     ctx = mastg.Context(w.desc)
     t_setup = time.perf_counter() - t0
     params = w.params
-    solver = mastg.SOLVER_CG_JACOBI
     print(f"# workload {a.workload}: {w.n_elems} elems, {ctx.n_active} active DOF, nnz {ctx.nnz}, "
           f"setup {t_setup:.1f}s", file=sys.stderr, flush=True)
-    for i in range(a.warmup):
-        r = ctx.design_eval(params, solver=solver)
-        print(f"# warmup {i}: C={r['objective']:.10g} iters={r['iters']} "
-              f"t_asm={r['t_assemble']*1e3:.2f}ms t_solve={r['t_solve']*1e3:.1f}ms", file=sys.stderr,
-              flush=True)
-    lat = []
-    res = []
-    ctx._chk(mastg.lib().mastg_synchronize(ctx.h))
-    t0 = time.perf_counter()
-    for i in range(a.steps):
-        t1 = time.perf_counter()
-        r = ctx.design_eval(params, solver=solver)
-        lat.append(time.perf_counter() - t1)
-        res.append(r)
-        print(f"# step {i}: {lat[-1]*1e3:.1f} ms iters={r['iters']}", file=sys.stderr, flush=True)
-    ctx._chk(mastg.lib().mastg_synchronize(ctx.h))
-    elapsed = time.perf_counter() - t0
-    designs_per_s = a.steps / elapsed
+    solvers = {"plain": mastg.SOLVER_CG_JACOBI, "matfree": mastg.SOLVER_MATFREE}
+    order = ["plain", "matfree"] if a.solver == "both" else [a.solver]
+    runs = {}
+    for name in order:
+        sv = solvers[name]
+        for i in range(a.warmup):
+            r = ctx.design_eval(params, solver=sv)
+            print(f"# {name} warmup {i}: C={r['objective']:.10g} iters={r['iters']} "
+                  f"t_asm={r['t_assemble']*1e3:.2f}ms t_solve={r['t_solve']*1e3:.1f}ms "
+                  f"t_sens={r['t_sens']*1e3:.2f}ms", file=sys.stderr, flush=True)
+        lat, res = [], []
+        ctx._chk(mastg.lib().mastg_synchronize(ctx.h))
+        t0 = time.perf_counter()
+        for i in range(a.steps):
+            t1 = time.perf_counter()
+            r = ctx.design_eval(params, solver=sv)
+            lat.append(time.perf_counter() - t1)
+            res.append(r)
+            print(f"# {name} step {i}: {lat[-1]*1e3:.1f} ms iters={r['iters']}", file=sys.stderr,
+                  flush=True)
+        ctx._chk(mastg.lib().mastg_synchronize(ctx.h))
+        elapsed = time.perf_counter() - t0
+        runs[name] = dict(elapsed=elapsed, lat=lat, res=res, dps=a.steps / elapsed)
+    head = "matfree" if "matfree" in runs else "plain"
+    H = runs[head]
+    res = H["res"]
+    lat = H["lat"]
+    elapsed = H["elapsed"]
+    designs_per_s = H["dps"]
     t_elem_path = np.mean([x["t_assemble"] + x["t_sens"] for x in res])
     elem_rate = w.n_elems / t_elem_path
+    lanes, grid = ctx.tuning()
 
-    # roofline of the dominant kernel: the FP64 CSR SpMV of the CG loop
-    spmv_ms = ctx.bench_kernel(2, 20)
-    byt = spmv_bytes(ctx.n_active, ctx.nnz)
-    achieved = byt / (spmv_ms * 1e-3) / 1e9
+    # roofline of the dominant kernel of the headline path
+    if head == "plain":
+        k_ms = ctx.bench_kernel(2, 20)
+        byt = spmv_bytes(ctx.n_active, ctx.nnz)
+        kname = f"k_spmv_csr_dot<{lanes}> (FP64 CSR SpMV)"
+    else:
+        k_ms = ctx.bench_kernel(3, 20)
+        byt = mf_bytes(ctx.n_active, w.n_elems)
+        kname = "k_hex8_mf_apply (FP64 matrix-free operator)"
+    achieved = byt / (k_ms * 1e-3) / 1e9
     out = {
         "metric": "elem K+R+dR/dp assemblies/s; designs/s at rel err<=1e-6; p50/p99 latency",
         "value": designs_per_s,
@@ -126,19 +150,29 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded SplitMix64 densities)",
         "config": {"workload": f"hex8 SIMP cantilever {nx}x{ny}x{nz}", "elements": w.n_elems,
-                   "active_dofs": ctx.n_active, "nnz": ctx.nnz, "solver": a.solver,
-                   "rtol": 1e-10, "rho": [lo, hi], "penalty": 3},
+                   "active_dofs": ctx.n_active, "nnz": ctx.nnz, "solver": head,
+                   "rtol": 1e-10, "rho": [lo, hi], "penalty": 3,
+                   "spmv_lanes_per_row": lanes, "grid": grid},
         "elem_assemblies_per_s": elem_rate,
         "p50_ms": float(np.percentile(lat, 50) * 1e3),
         "p99_ms": float(np.percentile(lat, 99) * 1e3),
         "cg_iters": res[-1]["iters"],
         "rel_res": res[-1]["rel_res"],
         "compliance": res[-1]["objective"],
-        "roofline": {"bound": "hbm", "kernel": "k_spmv_csr_dot (FP64 CSR SpMV)",
+        "roofline": {"bound": "hbm", "kernel": kname,
                      "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": None,
-                     "avg_launch_ms": spmv_ms, "algorithmic_bytes": byt},
+                     "avg_launch_ms": k_ms, "algorithmic_bytes": byt},
     }
+    for name, R in runs.items():
+        out[f"{name}_designs_per_s"] = R["dps"]
+        out[f"{name}_cg_iters"] = R["res"][-1]["iters"]
+        out[f"{name}_compliance"] = R["res"][-1]["objective"]
+    if "plain" in runs and "matfree" in runs:
+        out["speedup_vs_plain_fp64"] = runs["matfree"]["dps"] / runs["plain"]["dps"]
+        out["compliance_rel_diff_paths"] = abs(runs["matfree"]["res"][-1]["objective"] -
+                                               runs["plain"]["res"][-1]["objective"]) / \
+            abs(runs["plain"]["res"][-1]["objective"])
     if not a.no_cpu_baseline:
         t_elem, t_it, nnz2, n_sample, cores = cpu_baseline(a.workload, res[-1]["iters"])
         t_design = w.n_elems * t_elem + res[-1]["iters"] * t_it * (ctx.nnz / nnz2)

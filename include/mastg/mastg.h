@@ -138,6 +138,9 @@ int mastg_assemble(mastg_ctx* ctx, const double* X, double* R, double* J_vals,
 #define MASTG_SOLVER_CG_JACOBI 0        /* FP64 CSR CG + Jacobi (plain path)           */
 #define MASTG_SOLVER_MIXED 1            /* FP32-SpMV PCG + FP64 reliable updates        */
 #define MASTG_SOLVER_BICGSTAB_JACOBI 2  /* FP64 BiCGStab + Jacobi (non-symmetric J)     */
+#define MASTG_SOLVER_MATFREE 3          /* FP64 CG + Jacobi on the matrix-free operator
+                                           over congruent-element classes (the same
+                                           matrix as the CSR; hex8, <= 8 element shapes) */
 
 typedef struct {
     int32_t solver;       /* MASTG_SOLVER_*                                   */
@@ -159,6 +162,12 @@ typedef struct {
  * caller numbering (inactive variables of x are set to 0). */
 int mastg_solve(mastg_ctx* ctx, const double* rhs, double* x, int where,
                 const mastg_solve_opts* opts, mastg_solve_stats* stats);
+
+/* J dX with the Jacobian of the last mastg_assemble, op 0 = assembled CSR,
+ * op 1 = matrix-free operator (same matrix).  Replaces
+ * NonlinearImplicitAssembly::linearized_jacobian_solution_product
+ * (src/base/nonlinear_implicit_assembly.cpp:271-383) for the linear elements. */
+int mastg_jacobian_product(mastg_ctx* ctx, const double* dX, double* JdX, int where, int op);
 
 /* g_e = lambda_e . dR_e/dp_e for every element (constrained), the batched form of
  * sensitivity_assemble + the per-parameter dot of
@@ -196,7 +205,7 @@ int mastg_synchronize(mastg_ctx* ctx);
 
 /* Device time (milliseconds) of named kernels over the context's lifetime,
  * measured with HIP events on the stream the kernels run on. kind:
- * 0 element kernel, 1 scatter, 2 SpMV (FP64 CSR), 3 SpMV (mixed inner),
+ * 0 element kernel, 1 scatter, 2 SpMV (FP64 CSR), 3 matrix-free operator,
  * 4 sensitivity; returns total ms and launch count. */
 int mastg_kernel_time(const mastg_ctx* ctx, int kind, double* total_ms, int64_t* launches);
 int mastg_enable_kernel_timing(mastg_ctx* ctx, int on);
@@ -204,6 +213,10 @@ int mastg_enable_kernel_timing(mastg_ctx* ctx, int on);
  * context's stream and returns the average duration from HIP events (same
  * kinds as mastg_kernel_time).  For roofline measurement. */
 int mastg_bench_kernel(mastg_ctx* ctx, int kind, int n_launch, double* avg_ms);
+/* Launch shapes: CSR SpMV lanes per row and grid size (work-groups of the
+ * SpMV / CG vector kernels).  Autotuned on the first solve unless set. */
+int mastg_get_tuning(const mastg_ctx* ctx, int* spmv_lanes, int* grid);
+int mastg_set_tuning(mastg_ctx* ctx, int spmv_lanes, int grid);
 
 /* ------------------------------------------------ benchmark workloads -- */
 /* Structured problems of the benchmark configurations, built the way libMesh

@@ -50,6 +50,8 @@ struct mastg_ctx {
     // device structure
     DevBuf xyz, econn, blk_ptr, blk_col, row_ptr, col, cptr, cidx, diag_pos, con, dof_of_active;
     DevBuf ninc_ptr, ninc;
+    // matrix-free operator data (hex8 congruence classes)
+    DevBuf Khat, sE, eclass, nodecon, Fel;
     // device data
     DevBuf param, Kel, Rel, val, dinv, F;
     DevBuf vx, vr, vz, vp, vq, vb, vtmp, vcaller, vg, part0, part1, scal, dscalar;
@@ -58,9 +60,13 @@ struct mastg_ctx {
     double Dhat[36];
     bool assembled = false;
     bool params_set = false;
-    // captured CG iterations
-    hipGraphExec_t cg_exec = nullptr;
-    int cg_exec_len = 0;
+    // captured CG iterations, one graph per operator (CSR, matrix-free)
+    hipGraphExec_t cg_exec[2] = {nullptr, nullptr};
+    int cg_exec_len[2] = {0, 0};
+    // launch shapes (autotuned on the first solve)
+    int spmv_sw = 16;
+    bool tuned = false;
+    bool sE_valid = false;
     // timing
     bool timing = false;
     KTimer kt[8];
@@ -155,6 +161,20 @@ CGArgs cg_args(mastg_ctx* c) {
     a.part1 = c->part1.as<double>();
     a.nparts = c->nparts;
     a.S = c->scal.as<CGScalars>();
+    a.use_mf = 0;
+    a.spmv_sw = c->spmv_sw;
+    a.mf.n_nodes = c->S.n_nodes;
+    a.mf.n_classes = c->S.n_classes;
+    a.mf.Khat = c->Khat.as<double>();
+    a.mf.sE = c->sE.as<double>();
+    a.mf.eclass = c->eclass.as<uint8_t>();
+    a.mf.econn = c->econn.as<int32_t>();
+    a.mf.ninc_ptr = c->ninc_ptr.as<int32_t>();
+    a.mf.ninc = c->ninc.as<int32_t>();
+    a.mf.nodecon = c->nodecon.as<uint8_t>();
+    a.mf.n_elems = c->S.n_elems;
+    a.mf.Fel = c->Fel.as<double>();
+    a.nparts_op = c->nparts;
     return a;
 }
 
@@ -265,12 +285,24 @@ int assemble_active(mastg_ctx* c, const double* X_active, bool want_R, bool want
 }
 
 // ---- FP64 CSR CG + Jacobi on the resident matrix: b (active, device) -> c->vx
-int cg_solve(mastg_ctx* c, const double* b, const mastg_solve_opts* o, mastg_solve_stats* st) {
+int autotune(mastg_ctx* c);
+
+int cg_solve(mastg_ctx* c, const double* b, const mastg_solve_opts* o, mastg_solve_stats* st,
+             bool matfree) {
     const Structure& S = c->S;
     const double rtol = (o && o->rtol > 0) ? o->rtol : 1e-10;
     const int max_it = (o && o->max_iters > 0) ? o->max_iters : 100000;
     const int chunk = (o && o->check_every > 0) ? o->check_every : 32;
+    if (!c->tuned) TRY(autotune(c));
+    if (matfree && !c->sE_valid) {
+        launch_elem_scale(S.n_elems, c->param.as<double>(), c->d.E, c->d.penalty, c->sE.as<double>(),
+                          c->stream);
+        c->sE_valid = true;
+    }
     CGArgs a = cg_args(c);
+    a.use_mf = matfree ? 1 : 0;
+    if (matfree) a.nparts_op = mf_node_blocks(S.n_nodes);
+    const int gi = matfree ? 1 : 0;
     hipEvent_t e0, e1;
     HIPC(c, hipEventCreate(&e0));
     HIPC(c, hipEventCreate(&e1));
@@ -278,27 +310,28 @@ int cg_solve(mastg_ctx* c, const double* b, const mastg_solve_opts* o, mastg_sol
     launch_fill(S.n_active, 0.0, a.x, c->stream);
     HIPC(c, hipMemcpyAsync(a.r, b, S.n_active * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     launch_cg_start(a, b, rtol * rtol, max_it, c->stream);
-    if (!c->cg_exec || c->cg_exec_len != chunk) {
-        if (c->cg_exec) (void)hipGraphExecDestroy(c->cg_exec);
-        c->cg_exec = nullptr;
+    if (!c->cg_exec[gi] || c->cg_exec_len[gi] != chunk) {
+        if (c->cg_exec[gi]) (void)hipGraphExecDestroy(c->cg_exec[gi]);
+        c->cg_exec[gi] = nullptr;
         hipGraph_t g;
         HIPC(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         for (int i = 0; i < chunk; i++) launch_cg_iteration(a, c->stream);
         HIPC(c, hipStreamEndCapture(c->stream, &g));
-        HIPC(c, hipGraphInstantiate(&c->cg_exec, g, nullptr, nullptr, 0));
+        HIPC(c, hipGraphInstantiate(&c->cg_exec[gi], g, nullptr, nullptr, 0));
         (void)hipGraphDestroy(g);
-        c->cg_exec_len = chunk;
+        c->cg_exec_len[gi] = chunk;
     }
     int done = 0;
     for (int launched = 0; launched < max_it + chunk; launched += chunk) {
-        HIPC(c, hipGraphLaunch(c->cg_exec, c->stream));
+        HIPC(c, hipGraphLaunch(c->cg_exec[gi], c->stream));
         HIPC(c, hipMemcpyAsync(c->h_scal, a.S, sizeof(CGScalars), hipMemcpyDeviceToHost, c->stream));
         HIPC(c, hipStreamSynchronize(c->stream));
         done = c->h_scal->done;
         if (done) break;
     }
     HIPC(c, hipEventRecord(e1, c->stream));
-    // true residual ||b - A x|| / ||b||
+    // true residual ||b - A x|| / ||b|| with the assembled CSR matrix (for the
+    // matrix-free operator this also checks its result against the CSR)
     launch_spmv_csr(a, a.x, c->vtmp.as<double>(), 0, c->stream);
     launch_axpby(S.n_active, 1.0, b, -1.0, c->vtmp.as<double>(), c->stream);
     double rr = 0, bb = 0;
@@ -316,6 +349,40 @@ int cg_solve(mastg_ctx* c, const double* b, const mastg_solve_opts* o, mastg_sol
     }
     if (done == 3 || done == 4) return fail(c, MASTG_ERR_NAN, "CG breakdown or non-finite iterate");
     if (done != 1) return fail(c, MASTG_ERR_NOT_CONVERGED, "CG did not reach the tolerance");
+    return MASTG_OK;
+}
+
+
+// Launch-shape autotuner: times the CSR SpMV over lanes-per-row x grid size on
+// the resident matrix (HIP events, 3 launches each) and keeps the fastest.
+// The grid size also sets the CG vector kernels and the partial-sum count.
+int autotune(mastg_ctx* c) {
+    c->tuned = true;
+    if (!c->assembled) return MASTG_OK;
+    const int sws[] = {4, 8, 16, 32, 64};
+    const int grids[] = {1024, 2048, 4096, 8192};
+    double best = 1e30;
+    int bsw = c->spmv_sw, bgrid = c->nparts;
+    TRY(dalloc(c, c->part0, 8192 * sizeof(double)));
+    TRY(dalloc(c, c->part1, 8192 * sizeof(double)));
+    for (int g : grids)
+        for (int sw : sws) {
+            c->nparts = g;
+            c->spmv_sw = sw;
+            CGArgs a = cg_args(c);
+            launch_spmv_csr(a, a.p, a.q, 0, c->stream);
+            HIPC(c, hipEventRecord(c->ev0, c->stream));
+            for (int i = 0; i < 3; i++) launch_spmv_csr(a, a.p, a.q, 0, c->stream);
+            HIPC(c, hipEventRecord(c->ev1, c->stream));
+            HIPC(c, hipEventSynchronize(c->ev1));
+            float ms = 0;
+            HIPC(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+            if (ms < best) { best = ms; bsw = sw; bgrid = g; }
+        }
+    c->nparts = bgrid;
+    c->spmv_sw = bsw;
+    for (int i = 0; i < 2; i++)
+        if (c->cg_exec[i]) { (void)hipGraphExecDestroy(c->cg_exec[i]); c->cg_exec[i] = nullptr; }
     return MASTG_OK;
 }
 
@@ -384,6 +451,42 @@ int mastg_ctx_create(const mastg_problem_desc* desc, mastg_ctx** out) {
         (rc = upload(c, c->F, F)) || (rc = dalloc(c, c->param, S.n_elems * sizeof(double))) ||
         (rc = ensure_vectors(c)))
         return bail(rc);
+    if (desc->physics == MASTG_HEX8_ELASTIC && S.n_classes > 0) {
+        // class reference matrices at E = 1 from the element kernel itself
+        std::vector<int32_t> rep_conn(S.n_classes * 8);
+        for (int k = 0; k < S.n_classes; k++)
+            for (int a = 0; a < 8; a++) rep_conn[k * 8 + a] = S.econn[int64_t(S.class_rep[k]) * 8 + a];
+        std::vector<double> ones(S.n_classes, 1.0);
+        std::vector<uint8_t> nodecon(S.n_nodes, 0);
+        for (int64_t r = 0; r < S.n_nodes; r++)
+            for (int v = 0; v < 3; v++)
+                if (S.constrained[r * 3 + v]) nodecon[r] |= uint8_t(1 << v);
+        DevBuf rc_d, ones_d;
+        if ((rc = upload(c, rc_d, rep_conn)) || (rc = upload(c, ones_d, ones)) ||
+            (rc = dalloc(c, c->Khat, size_t(S.n_classes) * 64 * 9 * sizeof(double))) ||
+            (rc = upload(c, c->eclass, S.elem_class)) || (rc = upload(c, c->nodecon, nodecon)) ||
+            (rc = dalloc(c, c->sE, S.n_elems * sizeof(double))) ||
+            (rc = dalloc(c, c->Fel, size_t(S.n_elems) * 24 * sizeof(double))))
+            return bail(rc);
+        Hex8Args a{};
+        a.n_elems = S.n_classes;
+        a.xyz = c->xyz.as<double>();
+        a.econn = rc_d.as<int32_t>();
+        a.param = ones_d.as<double>();
+        a.E0 = 1.0;
+        a.penalty = 1.0;
+        std::memcpy(a.Dhat, c->Dhat, sizeof(a.Dhat));
+        a.con = nullptr;
+        a.X = nullptr;
+        a.Rel = nullptr;
+        a.Kel = c->Khat.as<double>();
+        launch_hex8_K(a, c->stream);
+        if (hipStreamSynchronize(c->stream) != hipSuccess) {
+            c->err = "class matrix kernel failed";
+            return bail(MASTG_ERR_DEVICE);
+        }
+        c->device_bytes -= int64_t(rc_d.bytes + ones_d.bytes);
+    }
     *out = c;
     return MASTG_OK;
 }
@@ -391,7 +494,8 @@ int mastg_ctx_create(const mastg_problem_desc* desc, mastg_ctx** out) {
 int mastg_ctx_destroy(mastg_ctx* c) {
     if (!c) return MASTG_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    if (c->cg_exec) (void)hipGraphExecDestroy(c->cg_exec);
+    for (int i = 0; i < 2; i++)
+        if (c->cg_exec[i]) (void)hipGraphExecDestroy(c->cg_exec[i]);
     if (c->h_scal) (void)hipHostFree(c->h_scal);
     if (c->h_pin) (void)hipHostFree(c->h_pin);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -444,6 +548,7 @@ int mastg_set_params(mastg_ctx* c, const double* p, int where) {
     }
     c->params_set = true;
     c->assembled = false;
+    c->sE_valid = false;
     return MASTG_OK;
 }
 
@@ -469,12 +574,40 @@ int mastg_solve(mastg_ctx* c, const double* rhs, double* x, int where, const mas
     if (!c || !rhs || !x) return MASTG_ERR_INVALID;
     if (!c->assembled) return fail(c, MASTG_ERR_INVALID, "no assembled matrix");
     if (o && o->transpose) { /* the elastic Jacobian is symmetric */ }
-    if (o && o->solver != MASTG_SOLVER_CG_JACOBI)
+    const int solver = o ? o->solver : MASTG_SOLVER_CG_JACOBI;
+    if (solver != MASTG_SOLVER_CG_JACOBI && solver != MASTG_SOLVER_MATFREE)
         return fail(c, MASTG_ERR_INVALID, "solver not available in this build");
+    if (solver == MASTG_SOLVER_MATFREE && c->S.n_classes == 0)
+        return fail(c, MASTG_ERR_INVALID, "matrix-free solver needs a mesh of at most 8 element shapes");
     TRY(to_active(c, rhs, where, c->vb.as<double>()));
-    int rc = cg_solve(c, c->vb.as<double>(), o, st);
+    int rc = cg_solve(c, c->vb.as<double>(), o, st, solver == MASTG_SOLVER_MATFREE);
     int rc2 = from_active(c, c->vx.as<double>(), x, where);
     return rc ? rc : rc2;
+}
+
+int mastg_jacobian_product(mastg_ctx* c, const double* dX, double* JdX, int where, int op) {
+    if (!c || !dX || !JdX) return MASTG_ERR_INVALID;
+    if (!c->assembled) return fail(c, MASTG_ERR_INVALID, "no assembled matrix");
+    if (op == 1 && c->S.n_classes == 0) return fail(c, MASTG_ERR_INVALID, "no matrix-free operator");
+    TRY(to_active(c, dX, where, c->vb.as<double>()));
+    CGArgs a = cg_args(c);
+    if (op == 1) {
+        if (!c->sE_valid) {
+            launch_elem_scale(c->S.n_elems, c->param.as<double>(), c->d.E, c->d.penalty, c->sE.as<double>(),
+                              c->stream);
+            c->sE_valid = true;
+        }
+        // element phase reads a copy zeroed on constrained DOFs; the node phase
+        // applies the constrained rows (diagonal = elements at the node) to the input
+        HIPC(c, hipMemcpyAsync(c->vz.p, c->vb.p, c->S.n_active * sizeof(double), hipMemcpyDeviceToDevice,
+                               c->stream));
+        launch_mask_constrained(c->S.n_active, c->con.as<uint8_t>(), c->vz.as<double>(), c->stream);
+        launch_mf_apply(a, c->vz.as<double>(), c->vb.as<double>(), c->vtmp.as<double>(), 0, c->stream);
+    } else {
+        launch_spmv_csr(a, c->vb.as<double>(), c->vtmp.as<double>(), 0, c->stream);
+    }
+    HIPC(c, hipGetLastError());
+    return from_active(c, c->vtmp.as<double>(), JdX, where);
 }
 
 int mastg_sensitivity(mastg_ctx* c, const double* X, const double* lam, double* g, int where) {
@@ -522,7 +655,12 @@ int mastg_design_eval(mastg_ctx* c, const double* params, double* dobj, int wher
                            c->stream));
     HIPC(c, hipEventRecord(e[1], c->stream));
     mastg_solve_stats st{};
-    int rc = cg_solve(c, c->vb.as<double>(), o, &st);
+    const int solver = o ? o->solver : MASTG_SOLVER_CG_JACOBI;
+    if (solver == MASTG_SOLVER_MATFREE && c->S.n_classes == 0)
+        return fail(c, MASTG_ERR_INVALID, "matrix-free solver needs a mesh of at most 8 element shapes");
+    if (solver != MASTG_SOLVER_CG_JACOBI && solver != MASTG_SOLVER_MATFREE)
+        return fail(c, MASTG_ERR_INVALID, "solver not available in this build");
+    int rc = cg_solve(c, c->vb.as<double>(), o, &st, solver == MASTG_SOLVER_MATFREE);
     HIPC(c, hipEventRecord(e[2], c->stream));
     double C = 0;
     TRY(dot_active(c, c->vb.as<double>(), c->vx.as<double>(), &C));
@@ -583,6 +721,27 @@ int mastg_enable_kernel_timing(mastg_ctx* c, int on) {
     c->timing = on != 0;
     return MASTG_OK;
 }
+int mastg_get_tuning(const mastg_ctx* c, int* spmv_lanes, int* grid) {
+    if (!c) return MASTG_ERR_INVALID;
+    if (spmv_lanes) *spmv_lanes = c->spmv_sw;
+    if (grid) *grid = c->nparts;
+    return MASTG_OK;
+}
+
+int mastg_set_tuning(mastg_ctx* c, int spmv_lanes, int grid) {
+    if (!c || (spmv_lanes != 4 && spmv_lanes != 8 && spmv_lanes != 16 && spmv_lanes != 32 &&
+               spmv_lanes != 64) || grid < 64 || grid > 8192)
+        return MASTG_ERR_INVALID;
+    TRY(dalloc(c, c->part0, 8192 * sizeof(double)));
+    TRY(dalloc(c, c->part1, 8192 * sizeof(double)));
+    c->spmv_sw = spmv_lanes;
+    c->nparts = grid;
+    c->tuned = true;
+    for (int i = 0; i < 2; i++)
+        if (c->cg_exec[i]) { (void)hipGraphExecDestroy(c->cg_exec[i]); c->cg_exec[i] = nullptr; }
+    return MASTG_OK;
+}
+
 int mastg_bench_kernel(mastg_ctx* c, int kind, int n_launch, double* avg_ms) {
     if (!c || !avg_ms || n_launch <= 0) return MASTG_ERR_INVALID;
     if (!c->assembled) return fail(c, MASTG_ERR_INVALID, "assemble first");
@@ -597,11 +756,20 @@ int mastg_bench_kernel(mastg_ctx* c, int kind, int n_launch, double* avg_ms) {
             case 0: launch_hex8_K(h, c->stream); break;
             case 1: launch_scatter(sa, c->stream); break;
             case 2: launch_spmv_csr(a, a.p, a.q, 1, c->stream); break;
+            case 3:
+                if (!c->sE_valid) {
+                    launch_elem_scale(c->S.n_elems, c->param.as<double>(), c->d.E, c->d.penalty,
+                                      c->sE.as<double>(), c->stream);
+                    c->sE_valid = true;
+                }
+                launch_mf_apply(a, a.p, a.p, a.q, 1, c->stream);
+                break;
             case 4: launch_hex8_sens(hex8_args(c, a.x, false), a.x, c->vg.as<double>(), c->stream); break;
             default: break;
         }
     };
-    if (kind != 0 && kind != 1 && kind != 2 && kind != 4) return fail(c, MASTG_ERR_INVALID, "unknown kernel kind");
+    if (kind < 0 || kind > 4) return fail(c, MASTG_ERR_INVALID, "unknown kernel kind");
+    if (kind == 3 && c->S.n_classes == 0) return fail(c, MASTG_ERR_INVALID, "no matrix-free operator");
     launch();   // warm
     HIPC(c, hipEventRecord(c->ev0, c->stream));
     for (int i = 0; i < n_launch; i++) launch();

@@ -44,6 +44,8 @@ struct MFArgs {
     const int32_t* ninc_ptr;  // [n_nodes+1]
     const int32_t* ninc;      // e*8 + la, la ascending per node
     const uint8_t* nodecon;   // [n_nodes] bit v = variable v constrained
+    int64_t n_elems;
+    double* Fel;              // [24][n_elems] element forces (SoA), phase 1 -> phase 2
 };
 
 struct CGArgs {
@@ -55,8 +57,10 @@ struct CGArgs {
     double *x, *r, *z, *p, *q;
     double *part0, *part1;
     int nparts;
+    int nparts_op;            // partial sums written by the operator kernel
     CGScalars* S;
     int use_mf;               // operator: 0 CSR SpMV, 1 matrix-free
+    int spmv_sw;              // CSR SpMV lanes per row (4, 8, 16, 32, 64)
     MFArgs mf;
 };
 
@@ -79,6 +83,10 @@ void launch_cg_start(const CGArgs& c, const double* b, double tol2, int max_it, 
 void launch_cg_iteration(const CGArgs& c, hipStream_t s);
 void launch_elem_scale(int64_t n, const double* rho, double E0, double penalty, double* sE,
                        hipStream_t s);
-void launch_mf_apply(const CGArgs& c, const double* x, double* y, int want_dot, hipStream_t s);
+// x_elem must be zero on constrained DOFs; x_node feeds the constrained rows
+void launch_mf_apply(const CGArgs& c, const double* x_elem, const double* x_node, double* y,
+                     int want_dot, hipStream_t s);
+void launch_mask_constrained(int64_t n, const uint8_t* con, double* x, hipStream_t s);
+int mf_node_blocks(int64_t n_nodes);
 
 }  // namespace mastg

@@ -348,45 +348,72 @@ __global__ __launch_bounds__(256) void k_reduce_to(const double* __restrict__ p,
 //   K = sum_e s_e P_e^T Khat_{class(e)} P_e   (s_e = E0 rho_e^p)
 // with libMesh's constraint convention (constrained rows / columns removed,
 // diagonal = number of elements at the node), i.e. the same matrix the
-// segmented scatter assembles, applied node by node: the thread of node r walks
-// its element incidences in ascending local index, so on structured meshes a
-// wave reads the same Khat block rows from LDS (broadcast) in lock step.
-// Optionally accumulates x.y partials per block (fused CG dot).
+// segmented scatter assembles.  Two deterministic phases:
+//   1. one thread per element gathers its 8 nodes once and writes the element
+//      forces f_e = s_e Khat u_e to Fel (structure-of-arrays, coalesced); Khat
+//      sits in LDS and, with one class, every lane reads the same entry (LDS
+//      broadcast);
+//   2. one thread per node sums the forces of its incident elements in a fixed
+//      order, applies the constrained rows and the fused x.y partial sum.
+// The input must be zero on constrained DOFs (CG directions are; the
+// Jacobian-product entry point masks its input first).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_hex8_mf_apply(CGArgs C, const double* __restrict__ x,
-                                                       double* __restrict__ y, int want_dot,
-                                                       int in_cg_loop) {
-    if (in_cg_loop && C.S->done) return;
-    __shared__ double Kh[Structure_kMaxClasses * 64 * 9];
+template <bool ONE_CLASS>
+__global__ __launch_bounds__(256) void k_hex8_mf_elem(CGArgs C, const double* __restrict__ x) {
+    if (C.S && C.S->done) return;
+    __shared__ double Kh[Structure_kMaxClasses * 576];
     const MFArgs& A = C.mf;
-    for (int i = threadIdx.x; i < A.n_classes * 64 * 9; i += 256) Kh[i] = A.Khat[i];
+    const int ncl = ONE_CLASS ? 1 : A.n_classes;
+    for (int i = threadIdx.x; i < ncl * 576; i += 256) Kh[i] = A.Khat[i];
     __syncthreads();
-    double dsum = 0.0;
-    for (int64_t r = int64_t(blockIdx.x) * 256 + threadIdx.x; r < A.n_nodes;
-         r += int64_t(gridDim.x) * 256) {
-        double y0 = 0.0, y1 = 0.0, y2 = 0.0;
+    const int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (e >= A.n_elems) return;
+    const int4* ec = reinterpret_cast<const int4*>(A.econn + e * 8);
+    const int4 c0 = ec[0], c1 = ec[1];
+    const int32_t rb[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+    double u[24];
+#pragma unroll
+    for (int lb = 0; lb < 8; lb++) {
+        u[lb * 3 + 0] = x[int64_t(rb[lb]) * 3 + 0];
+        u[lb * 3 + 1] = x[int64_t(rb[lb]) * 3 + 1];
+        u[lb * 3 + 2] = x[int64_t(rb[lb]) * 3 + 2];
+    }
+    const double s = A.sE[e];
+    const double* K = Kh + (ONE_CLASS ? 0 : int(A.eclass[e]) * 576);
+    const int64_t ne = A.n_elems;
+#pragma unroll
+    for (int la = 0; la < 8; la++) {
+        double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+#pragma unroll
+        for (int lb = 0; lb < 8; lb++) {
+            const double* k = K + (la * 8 + lb) * 9;
+            f0 += k[0] * u[lb * 3 + 0] + k[1] * u[lb * 3 + 1] + k[2] * u[lb * 3 + 2];
+            f1 += k[3] * u[lb * 3 + 0] + k[4] * u[lb * 3 + 1] + k[5] * u[lb * 3 + 2];
+            f2 += k[6] * u[lb * 3 + 0] + k[7] * u[lb * 3 + 1] + k[8] * u[lb * 3 + 2];
+        }
+        A.Fel[(la * 3 + 0) * ne + e] = s * f0;
+        A.Fel[(la * 3 + 1) * ne + e] = s * f1;
+        A.Fel[(la * 3 + 2) * ne + e] = s * f2;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_hex8_mf_node(CGArgs C, const double* __restrict__ x,
+                                                      double* __restrict__ y, int want_dot) {
+    if (C.S && C.S->done) return;
+    const MFArgs& A = C.mf;
+    const int64_t r = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    double d = 0.0;
+    if (r < A.n_nodes) {
         const int32_t k0 = A.ninc_ptr[r], k1 = A.ninc_ptr[r + 1];
+        const int64_t ne = A.n_elems;
+        double y0 = 0.0, y1 = 0.0, y2 = 0.0;
         for (int32_t k = k0; k < k1; k++) {
             const int32_t c = A.ninc[k];
             const int64_t e = c >> 3;
             const int la = c & 7;
-            const double s = A.sE[e];
-            const double* K = Kh + (int(A.eclass[e]) * 64 + la * 8) * 9;
-            double t0 = 0.0, t1 = 0.0, t2 = 0.0;
-#pragma unroll
-            for (int lb = 0; lb < 8; lb++) {
-                const int64_t rb = A.econn[e * 8 + lb];
-                const int m = A.nodecon[rb];
-                const double u0 = (m & 1) ? 0.0 : x[rb * 3 + 0];
-                const double u1 = (m & 2) ? 0.0 : x[rb * 3 + 1];
-                const double u2 = (m & 4) ? 0.0 : x[rb * 3 + 2];
-                t0 += K[lb * 9 + 0] * u0 + K[lb * 9 + 1] * u1 + K[lb * 9 + 2] * u2;
-                t1 += K[lb * 9 + 3] * u0 + K[lb * 9 + 4] * u1 + K[lb * 9 + 5] * u2;
-                t2 += K[lb * 9 + 6] * u0 + K[lb * 9 + 7] * u1 + K[lb * 9 + 8] * u2;
-            }
-            y0 += s * t0;
-            y1 += s * t1;
-            y2 += s * t2;
+            y0 += A.Fel[(la * 3 + 0) * ne + e];
+            y1 += A.Fel[(la * 3 + 1) * ne + e];
+            y2 += A.Fel[(la * 3 + 2) * ne + e];
         }
         const int m = A.nodecon[r];
         const double cnt = double(k1 - k0);
@@ -397,9 +424,14 @@ __global__ __launch_bounds__(256) void k_hex8_mf_apply(CGArgs C, const double* _
         y[r * 3 + 0] = y0;
         y[r * 3 + 1] = y1;
         y[r * 3 + 2] = y2;
-        dsum += y0 * x0 + y1 * x1 + y2 * x2;
+        d = y0 * x0 + y1 * x1 + y2 * x2;
     }
-    if (want_dot) block_sum_store(dsum, C.part0, blockIdx.x);
+    if (want_dot) block_sum_store(d, C.part0, blockIdx.x);
+}
+
+__global__ void k_mask_constrained(int64_t n, const uint8_t* __restrict__ con, double* __restrict__ x) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n && con[i]) x[i] = 0.0;
 }
 
 __global__ void k_elem_scale(int64_t n, const double* __restrict__ rho, double E0, double p,
@@ -409,20 +441,23 @@ __global__ void k_elem_scale(int64_t n, const double* __restrict__ rho, double E
 }
 
 // ------------------------------------------------------- CG (FP64 CSR) ----
-// y = A x over the scalar CSR, 16 lanes per row, and partial x.y per block.
+// y = A x over the scalar CSR with SW lanes per row (SW in 4..64, chosen by the
+// autotuner), grid-stride over rows, and partial x.y per block.
+template <int SW>
 __global__ __launch_bounds__(256) void k_spmv_csr_dot(CGArgs C, const double* __restrict__ x,
                                                       double* __restrict__ y, int want_dot,
                                                       int in_cg_loop) {
     if (in_cg_loop && C.S->done) return;
-    const int sub = threadIdx.x & 15;
+    const int sub = threadIdx.x & (SW - 1);
     double dsum = 0.0;
-    for (int64_t row = (int64_t(blockIdx.x) * 256 + threadIdx.x) >> 4; row < C.n;
-         row += (int64_t(gridDim.x) * 256) >> 4) {
+    const int64_t rows_per_grid = (int64_t(gridDim.x) * 256) / SW;
+    for (int64_t row = (int64_t(blockIdx.x) * 256 + threadIdx.x) / SW; row < C.n; row += rows_per_grid) {
         const int64_t k0 = C.row_ptr[row], k1 = C.row_ptr[row + 1];
         double s = 0.0;
-        for (int64_t k = k0 + sub; k < k1; k += 16) s += C.val[k] * x[C.col[k]];
+#pragma unroll 4
+        for (int64_t k = k0 + sub; k < k1; k += SW) s += C.val[k] * x[C.col[k]];
 #pragma unroll
-        for (int m = 8; m >= 1; m >>= 1) s += __shfl_xor(s, m, 16);
+        for (int m = SW / 2; m >= 1; m >>= 1) s += __shfl_xor(s, m, SW);
         if (sub == 0) {
             y[row] = s;
             dsum += s * x[row];
@@ -434,7 +469,7 @@ __global__ __launch_bounds__(256) void k_spmv_csr_dot(CGArgs C, const double* __
 // alpha = rz / pq
 __global__ __launch_bounds__(256) void k_cg_alpha(CGArgs C) {
     if (C.S->done) return;
-    const double pq = reduce_partials(C.part0, C.nparts);
+    const double pq = reduce_partials(C.part0, C.nparts_op);
     if (threadIdx.x == 0) {
         C.S->pq = pq;
         C.S->alpha = C.S->rz / pq;
@@ -556,8 +591,18 @@ void launch_dot(int64_t n, const double* a, const double* b, double* partials, i
     hipLaunchKernelGGL(k_dot, dim3(nparts), dim3(256), 0, s, n, a, b, partials);
     hipLaunchKernelGGL(k_reduce_to, dim3(1), dim3(256), 0, s, partials, nparts, out);
 }
+static void spmv_dispatch(const CGArgs& c, const double* x, double* y, int want_dot, int in_loop,
+                          hipStream_t s) {
+    switch (c.spmv_sw) {
+        case 4: hipLaunchKernelGGL(k_spmv_csr_dot<4>, dim3(c.nparts), dim3(256), 0, s, c, x, y, want_dot, in_loop); break;
+        case 8: hipLaunchKernelGGL(k_spmv_csr_dot<8>, dim3(c.nparts), dim3(256), 0, s, c, x, y, want_dot, in_loop); break;
+        case 32: hipLaunchKernelGGL(k_spmv_csr_dot<32>, dim3(c.nparts), dim3(256), 0, s, c, x, y, want_dot, in_loop); break;
+        case 64: hipLaunchKernelGGL(k_spmv_csr_dot<64>, dim3(c.nparts), dim3(256), 0, s, c, x, y, want_dot, in_loop); break;
+        default: hipLaunchKernelGGL(k_spmv_csr_dot<16>, dim3(c.nparts), dim3(256), 0, s, c, x, y, want_dot, in_loop); break;
+    }
+}
 void launch_spmv_csr(const CGArgs& c, const double* x, double* y, int want_dot, hipStream_t s) {
-    hipLaunchKernelGGL(k_spmv_csr_dot, dim3(c.nparts), dim3(256), 0, s, c, x, y, want_dot, 0);
+    spmv_dispatch(c, x, y, want_dot, 0, s);
 }
 void launch_cg_start(const CGArgs& c, const double* b, double tol2, int max_it, hipStream_t s) {
     hipLaunchKernelGGL(k_cg_start, dim3(c.nparts), dim3(256), 0, s, c, b);
@@ -567,14 +612,31 @@ void launch_elem_scale(int64_t n, const double* rho, double E0, double penalty, 
                        hipStream_t s) {
     hipLaunchKernelGGL(k_elem_scale, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, rho, E0, penalty, sE);
 }
-void launch_mf_apply(const CGArgs& c, const double* x, double* y, int want_dot, hipStream_t s) {
-    hipLaunchKernelGGL(k_hex8_mf_apply, dim3(c.nparts), dim3(256), 0, s, c, x, y, want_dot, 0);
+int mf_node_blocks(int64_t n_nodes) { return int((n_nodes + 255) / 256); }
+static void mf_dispatch(const CGArgs& c, const double* x_elem, const double* x_node, double* y,
+                        int want_dot, hipStream_t s) {
+    const unsigned ge = blocks_for(c.mf.n_elems, 256);
+    if (c.mf.n_classes == 1)
+        hipLaunchKernelGGL(k_hex8_mf_elem<true>, dim3(ge), dim3(256), 0, s, c, x_elem);
+    else
+        hipLaunchKernelGGL(k_hex8_mf_elem<false>, dim3(ge), dim3(256), 0, s, c, x_elem);
+    hipLaunchKernelGGL(k_hex8_mf_node, dim3(mf_node_blocks(c.mf.n_nodes)), dim3(256), 0, s, c, x_node, y,
+                       want_dot);
+}
+void launch_mf_apply(const CGArgs& c, const double* x_elem, const double* x_node, double* y,
+                     int want_dot, hipStream_t s) {
+    CGArgs d = c;
+    d.S = nullptr;   // standalone application: ignore the CG termination flag
+    mf_dispatch(d, x_elem, x_node, y, want_dot, s);
+}
+void launch_mask_constrained(int64_t n, const uint8_t* con, double* x, hipStream_t s) {
+    hipLaunchKernelGGL(k_mask_constrained, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, con, x);
 }
 void launch_cg_iteration(const CGArgs& c, hipStream_t s) {
     if (c.use_mf)
-        hipLaunchKernelGGL(k_hex8_mf_apply, dim3(c.nparts), dim3(256), 0, s, c, c.p, c.q, 1, 1);
+        mf_dispatch(c, c.p, c.p, c.q, 1, s);
     else
-        hipLaunchKernelGGL(k_spmv_csr_dot, dim3(c.nparts), dim3(256), 0, s, c, c.p, c.q, 1, 1);
+        spmv_dispatch(c, c.p, c.q, 1, 1, s);
     hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(256), 0, s, c);
     hipLaunchKernelGGL(k_cg_update, dim3(c.nparts), dim3(256), 0, s, c);
     hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(256), 0, s, c);

@@ -14,7 +14,7 @@ LIB_PATH = os.path.normpath(os.path.join(_HERE, "..", "libmastg.so"))
 HEX8_ELASTIC, QUAD4_MINDLIN, HEX8_HEAT = 1, 2, 3
 HOST, DEVICE = 0, 1
 ELEM_FP64, ELEM_FP32 = 0, 1
-SOLVER_CG_JACOBI, SOLVER_MIXED, SOLVER_BICGSTAB = 0, 1, 2
+SOLVER_CG_JACOBI, SOLVER_MIXED, SOLVER_BICGSTAB, SOLVER_MATFREE = 0, 1, 2, 3
 OUTPUT_COMPLIANCE, OUTPUT_MEAN_T = 1, 2
 ERRORS = {0: "ok", -1: "invalid", -2: "out of memory", -3: "not converged", -4: "nan",
           -5: "device error", -6: "admission", -7: "busy"}
@@ -83,6 +83,7 @@ def lib():
             "mastg_solve": (C.c_int, [_P, _D, _D, C.c_int, C.POINTER(SolveOpts),
                                       C.POINTER(SolveStats)]),
             "mastg_sensitivity": (C.c_int, [_P, _D, _D, _D, C.c_int]),
+            "mastg_jacobian_product": (C.c_int, [_P, _D, _D, C.c_int, C.c_int]),
             "mastg_output": (C.c_int, [_P, C.c_int, _D, C.c_int, _D, _D]),
             "mastg_design_eval": (C.c_int, [_P, _D, _D, C.c_int, C.POINTER(SolveOpts),
                                             C.POINTER(DesignResult)]),
@@ -90,6 +91,8 @@ def lib():
             "mastg_kernel_time": (C.c_int, [_P, C.c_int, _D, C.POINTER(C.c_int64)]),
             "mastg_synchronize": (C.c_int, [_P]),
             "mastg_bench_kernel": (C.c_int, [_P, C.c_int, C.c_int, _D]),
+            "mastg_get_tuning": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+            "mastg_set_tuning": (C.c_int, [_P, C.c_int, C.c_int]),
             "mastg_workload_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64,
                                                 C.c_double, C.c_double, C.c_int, C.POINTER(_P)]),
             "mastg_workload_desc": (C.POINTER(ProblemDesc), [_P]),
@@ -233,6 +236,12 @@ class Context:
         return x, dict(iters=st.iters, replacements=st.replacements, rel_res=st.rel_res,
                        seconds=st.seconds, rc=rc)
 
+    def jacobian_product(self, dX, op=0):
+        y = np.zeros(self.n_dofs)
+        dX = np.ascontiguousarray(dX, np.float64)
+        self._chk(lib().mastg_jacobian_product(self.h, _p(dX), _p(y), HOST, op))
+        return y
+
     def sensitivity(self, X, lam):
         g = np.zeros(self.n_elems)
         X = np.ascontiguousarray(X, np.float64)
@@ -267,6 +276,14 @@ class Context:
         ms = C.c_double()
         self._chk(lib().mastg_bench_kernel(self.h, kind, n_launch, C.byref(ms)))
         return ms.value
+
+    def tuning(self):
+        a, b = C.c_int(), C.c_int()
+        self._chk(lib().mastg_get_tuning(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def set_tuning(self, lanes, grid):
+        self._chk(lib().mastg_set_tuning(self.h, lanes, grid))
 
     def kernel_time(self, kind):
         ms = C.c_double()

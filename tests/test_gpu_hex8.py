@@ -136,3 +136,51 @@ def test_error_paths(oracle, mastg):
     r = ctx.design_eval(P.params(), max_iters=3, raise_on_error=False)
     assert r["rc"] == -3
     ctx.close()
+
+
+@pytest.mark.parametrize("mesh", [(2, 1, 1), (20, 10, 10), (7, 5, 3)])
+def test_matfree_operator_equals_csr(oracle, mastg, mesh):
+    print(f"[hex8 matrix-free J dX] mesh={mesh}", flush=True)
+    P = oracle.Problem(oracle.CANTILEVER, *mesh, seed=4, lo=0.01, hi=1.0, mast_exact=True)
+    ctx = ctx_from_oracle(mastg, P)
+    R, J = ctx.assemble()
+    dX = np.random.default_rng(7).standard_normal(P.n_dofs)
+    y0 = ctx.jacobian_product(dX, op=0)
+    y1 = ctx.jacobian_product(dX, op=1)
+    # the oracle's own matrix-vector product (it keeps the rotational DOFs)
+    P.assemble()
+    yo = P.spmv(np.where(np.isin(np.arange(P.n_dofs), np.nonzero(ctx.active_map() >= 0)[0]), dX, 0.0))
+    act = ctx.active_map() >= 0
+    scale = np.abs(y0).max()
+    assert np.max(np.abs(y0 - yo)[act]) <= 1e-12 * scale
+    assert np.max(np.abs(y1 - y0)) <= 1e-12 * scale
+    ctx.close()
+
+
+@pytest.mark.parametrize("mesh,lo", [((20, 10, 10), 0.3), ((16, 8, 8), 0.01)])
+def test_design_eval_matfree(oracle, mastg, mesh, lo):
+    print(f"[hex8 design eval matrix-free] mesh={mesh} lo={lo}", flush=True)
+    P = oracle.Problem(oracle.CANTILEVER, *mesh, seed=1, lo=lo, hi=1.0)
+    ref = P.solve_elastic(1e-10)
+    ctx = ctx_from_oracle(mastg, P)
+    r = ctx.design_eval(P.params(), solver=mastg.SOLVER_MATFREE)
+    assert r["rel_res"] <= 1.5e-10
+    assert abs(r["objective"] - ref["compliance"]) <= 1e-9 * abs(ref["compliance"])
+    assert rel_err_floor(r["grad"], ref["dc"], floor=1e-6) <= 1e-7
+    assert abs(r["iters"] - ref["iters"]) <= max(3, 0.02 * ref["iters"])
+    ctx.close()
+
+
+def test_autotuner_choice_is_valid(oracle, mastg):
+    print("[autotune] 12x6x6", flush=True)
+    P = oracle.Problem(oracle.CANTILEVER, 12, 6, 6, seed=2, lo=0.3, hi=1.0)
+    ctx = ctx_from_oracle(mastg, P)
+    r0 = ctx.design_eval(P.params())
+    lanes, grid = ctx.tuning()
+    assert lanes in (4, 8, 16, 32, 64) and 1024 <= grid <= 8192
+    # every launch shape gives the same answer to rounding
+    for lanes, grid in [(4, 1024), (64, 8192)]:
+        ctx.set_tuning(lanes, grid)
+        r = ctx.design_eval(P.params())
+        assert abs(r["objective"] - r0["objective"]) <= 1e-11 * r0["objective"]
+    ctx.close()
