@@ -80,6 +80,8 @@ def main():
     ap.add_argument("--solver", default="both", choices=["plain", "matfree", "both"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch the solver kernels directly instead of replaying hipGraphs")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the multi-GPU path is not built", file=sys.stderr)
@@ -90,6 +92,8 @@ def main():
     t0 = time.perf_counter()
     ctx = mastg.Context(w.desc)
     t_setup = time.perf_counter() - t0
+    if a.no_graphs:
+        ctx.set_option(mastg.OPT_GRAPHS, 0)
     params = w.params
     print(f"# workload {a.workload}: {w.n_elems} elems, {ctx.n_active} active DOF, nnz {ctx.nnz}, "
           f"setup {t_setup:.1f}s", file=sys.stderr, flush=True)

@@ -216,6 +216,10 @@ int mastg_bench_kernel(mastg_ctx* ctx, int kind, int n_launch, double* avg_ms);
 /* Launch shapes: CSR SpMV lanes per row and grid size (work-groups of the
  * SpMV / CG vector kernels).  Autotuned on the first solve unless set. */
 int mastg_get_tuning(const mastg_ctx* ctx, int* spmv_lanes, int* grid);
+/* options: MASTG_OPT_GRAPHS (1 = capture solver iterations in hipGraphs, the
+ * default; 0 = launch kernels directly) */
+#define MASTG_OPT_GRAPHS 1
+int mastg_set_option(mastg_ctx* ctx, int option, int value);
 int mastg_set_tuning(mastg_ctx* ctx, int spmv_lanes, int grid);
 
 /* ------------------------------------------------ benchmark workloads -- */

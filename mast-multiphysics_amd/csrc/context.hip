@@ -52,12 +52,19 @@ struct mastg_ctx {
     DevBuf ninc_ptr, ninc;
     // matrix-free operator data (hex8 congruence classes)
     DevBuf Khat, sE, eclass, nodecon, Fel;
+    // heat: convective side masks, mean-T weights, BiCGStab work space, A^T values
+    DevBuf sidemask, wmean, tpos, valT, vr0, vv, vs, vt, vph, vsh, bscal, vT;
+    double volume = 0;
+    hipGraphExec_t bicg_exec = nullptr;
+    int bicg_exec_len = 0;
+    bool bicg_exec_T = false;
     // device data
     DevBuf param, Kel, Rel, val, dinv, F;
     DevBuf vx, vr, vz, vp, vq, vb, vtmp, vcaller, vg, part0, part1, scal, dscalar;
     CGScalars* h_scal = nullptr;      // pinned mirror of the CG scalars
     double* h_pin = nullptr;          // pinned scratch for scalar read-back
     double Dhat[36];
+    double Dps[9], Gmod = 0;          // plate: plane-stress D and shear modulus
     bool assembled = false;
     bool params_set = false;
     // captured CG iterations, one graph per operator (CSR, matrix-free)
@@ -65,6 +72,7 @@ struct mastg_ctx {
     int cg_exec_len[2] = {0, 0};
     // launch shapes (autotuned on the first solve)
     int spmv_sw = 16;
+    bool use_graphs = true;
     bool tuned = false;
     bool sE_valid = false;
     // timing
@@ -145,6 +153,55 @@ Hex8Args hex8_args(mastg_ctx* c, const double* X, bool want_K) {
     return a;
 }
 
+Quad4Args quad4_args(mastg_ctx* c, const double* X, bool want_K) {
+    Quad4Args a{};
+    a.n_elems = c->S.n_elems;
+    a.xyz = c->xyz.as<double>();
+    a.econn = c->econn.as<int32_t>();
+    a.param = c->param.as<double>();
+    std::memcpy(a.Dps, c->Dps, sizeof(a.Dps));
+    a.G = c->Gmod;
+    a.kappa = c->d.kappa;
+    a.off = c->d.offset;
+    a.con = c->con.as<uint8_t>();
+    a.X = X;
+    a.Rel = c->Rel.as<double>();
+    a.Kel = want_K ? c->Kel.as<double>() : nullptr;
+    return a;
+}
+
+HeatArgs heat_args(mastg_ctx* c, const double* X, bool want_R, bool want_K) {
+    HeatArgs a{};
+    a.n_elems = c->S.n_elems;
+    a.xyz = c->xyz.as<double>();
+    a.econn = c->econn.as<int32_t>();
+    a.param = c->param.as<double>();
+    a.k0 = c->d.k0;
+    a.alpha = c->d.alpha;
+    a.h = c->d.conv_h;
+    a.Tinf = c->d.conv_tinf;
+    a.sides = c->sidemask.as<uint8_t>();
+    a.con = c->con.as<uint8_t>();
+    a.X = X;
+    a.Rel = want_R ? c->Rel.as<double>() : nullptr;
+    a.Kel = want_K ? c->Kel.as<double>() : nullptr;
+    return a;
+}
+
+// element kernel of the context's physics
+void launch_elements(mastg_ctx* c, const double* X, bool want_K) {
+    if (c->d.physics == MASTG_HEX8_ELASTIC) launch_hex8_K(hex8_args(c, X, want_K), c->stream);
+    else if (c->d.physics == MASTG_QUAD4_MINDLIN) launch_quad4_K(quad4_args(c, X, want_K), c->stream);
+    else launch_heat_K(heat_args(c, X ? X : c->vT.as<double>(), X != nullptr, want_K), c->stream);
+}
+
+// g_e = lambda . dR_e/dp_e for the context's physics
+void launch_sens(mastg_ctx* c, const double* X, const double* lam, double* g) {
+    if (c->d.physics == MASTG_HEX8_ELASTIC) launch_hex8_sens(hex8_args(c, X, false), lam, g, c->stream);
+    else if (c->d.physics == MASTG_QUAD4_MINDLIN) launch_quad4_sens(quad4_args(c, X, false), lam, g, c->stream);
+    else launch_heat_source_dot(heat_args(c, X, false, false), lam, -1.0, g, c->stream);
+}
+
 CGArgs cg_args(mastg_ctx* c) {
     CGArgs a{};
     a.n = c->S.n_active;
@@ -175,6 +232,7 @@ CGArgs cg_args(mastg_ctx* c) {
     a.mf.n_elems = c->S.n_elems;
     a.mf.Fel = c->Fel.as<double>();
     a.nparts_op = c->nparts;
+    a.nvec = std::min<int64_t>(2048, std::max<int64_t>(1, (c->S.n_active + 255) / 256));
     return a;
 }
 
@@ -230,8 +288,11 @@ int from_active(mastg_ctx* c, const double* src_active, double* dst, int where) 
 int ensure_vectors(mastg_ctx* c) {
     const size_t nb = size_t(c->S.n_active) * sizeof(double);
     for (DevBuf* b : {&c->vx, &c->vr, &c->vz, &c->vp, &c->vq, &c->vb, &c->vtmp}) TRY(dalloc(c, *b, nb));
-    TRY(dalloc(c, c->part0, c->nparts * sizeof(double)));
-    TRY(dalloc(c, c->part1, c->nparts * sizeof(double)));
+    // partial sums: enough for the largest grid any kernel writes them from
+    // (autotuned SpMV grid up to 8192, matrix-free node phase n_nodes/256)
+    const size_t np = size_t(std::max<int64_t>(8192, mf_node_blocks(c->S.n_nodes)));
+    TRY(dalloc(c, c->part0, np * sizeof(double)));
+    TRY(dalloc(c, c->part1, np * sizeof(double)));
     TRY(dalloc(c, c->scal, sizeof(CGScalars)));
     TRY(dalloc(c, c->dscalar, 4 * sizeof(double)));
     TRY(dalloc(c, c->vg, size_t(c->S.n_elems) * sizeof(double)));
@@ -251,19 +312,21 @@ int dot_active(mastg_ctx* c, const double* a, const double* b, double* out) {
 int assemble_active(mastg_ctx* c, const double* X_active, bool want_R, bool want_K) {
     if (!c->params_set) return fail(c, MASTG_ERR_INVALID, "design parameters not set");
     const Structure& S = c->S;
-    if (c->d.physics != MASTG_HEX8_ELASTIC)
-        return fail(c, MASTG_ERR_INVALID, "physics not available in this build");
     if (want_K) {
         TRY(dalloc(c, c->Kel, size_t(S.n_elems) * S.npe * S.npe * S.nav * S.nav * sizeof(double)));
         TRY(dalloc(c, c->val, size_t(S.nnz) * sizeof(double)));
         TRY(dalloc(c, c->dinv, size_t(S.n_active) * sizeof(double)));
     }
     if (want_R) TRY(dalloc(c, c->Rel, size_t(S.n_elems) * S.npe * S.nav * sizeof(double)));
-    Hex8Args a = hex8_args(c, want_R ? X_active : nullptr, want_K);
-    if (want_R && !X_active) a.X = nullptr;
-    {
+    if (c->d.physics == MASTG_HEX8_HEAT) {
+        // the conduction residual depends on the state (k(T), convection,
+        // source): the element kernel always runs, at T = 0 when no state is given
+        if (!X_active) launch_fill(S.n_active, 0.0, c->vT.as<double>(), c->stream);
         TimedRegion t(c, 0);
-        if (want_K || (want_R && X_active)) launch_hex8_K(a, c->stream);
+        launch_heat_K(heat_args(c, X_active ? X_active : c->vT.as<double>(), want_R, want_K), c->stream);
+    } else {
+        TimedRegion t(c, 0);
+        if (want_K || (want_R && X_active)) launch_elements(c, want_R ? X_active : nullptr, want_K);
     }
     if (want_K) {
         ScatterArgs sa{S.n_nodes, S.nav, c->blk_ptr.as<int64_t>(), c->cptr.as<int32_t>(),
@@ -276,16 +339,20 @@ int assemble_active(mastg_ctx* c, const double* X_active, bool want_R, bool want
                           c->stream);
         c->assembled = true;
     }
-    if (want_R)
+    if (want_R) {
+        const bool have_rel = X_active || c->d.physics == MASTG_HEX8_HEAT;
         launch_node_residual(S.n_nodes, S.nav, S.npe, c->ninc_ptr.as<int32_t>(), c->ninc.as<int32_t>(),
-                             X_active ? c->Rel.as<double>() : nullptr, c->F.as<double>(),
+                             have_rel ? c->Rel.as<double>() : nullptr, c->F.as<double>(),
                              c->con.as<uint8_t>(), c->vtmp.as<double>(), c->stream);
+    }
     HIPC(c, hipGetLastError());
     return MASTG_OK;
 }
 
 // ---- FP64 CSR CG + Jacobi on the resident matrix: b (active, device) -> c->vx
 int autotune(mastg_ctx* c);
+int cg_finish(mastg_ctx* c, const CGArgs& a, const double* b, hipEvent_t e0, hipEvent_t e1,
+              mastg_solve_stats* st, int done);
 
 int cg_solve(mastg_ctx* c, const double* b, const mastg_solve_opts* o, mastg_solve_stats* st,
              bool matfree) {
@@ -310,6 +377,18 @@ int cg_solve(mastg_ctx* c, const double* b, const mastg_solve_opts* o, mastg_sol
     launch_fill(S.n_active, 0.0, a.x, c->stream);
     HIPC(c, hipMemcpyAsync(a.r, b, S.n_active * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     launch_cg_start(a, b, rtol * rtol, max_it, c->stream);
+    if (!c->use_graphs) {
+        // direct launches, polled every `chunk` iterations
+        int done = 0;
+        for (int launched = 0; launched < max_it + chunk; launched += chunk) {
+            for (int i = 0; i < chunk; i++) launch_cg_iteration(a, c->stream);
+            HIPC(c, hipMemcpyAsync(c->h_scal, a.S, sizeof(CGScalars), hipMemcpyDeviceToHost, c->stream));
+            HIPC(c, hipStreamSynchronize(c->stream));
+            done = c->h_scal->done;
+            if (done) break;
+        }
+        return cg_finish(c, a, b, e0, e1, st, done);
+    }
     if (!c->cg_exec[gi] || c->cg_exec_len[gi] != chunk) {
         if (c->cg_exec[gi]) (void)hipGraphExecDestroy(c->cg_exec[gi]);
         c->cg_exec[gi] = nullptr;
@@ -329,6 +408,12 @@ int cg_solve(mastg_ctx* c, const double* b, const mastg_solve_opts* o, mastg_sol
         done = c->h_scal->done;
         if (done) break;
     }
+    return cg_finish(c, a, b, e0, e1, st, done);
+}
+
+int cg_finish(mastg_ctx* c, const CGArgs& a, const double* b, hipEvent_t e0, hipEvent_t e1,
+              mastg_solve_stats* st, int done) {
+    const Structure& S = c->S;
     HIPC(c, hipEventRecord(e1, c->stream));
     // true residual ||b - A x|| / ||b|| with the assembled CSR matrix (for the
     // matrix-free operator this also checks its result against the CSR)
@@ -353,6 +438,105 @@ int cg_solve(mastg_ctx* c, const double* b, const mastg_solve_opts* o, mastg_sol
 }
 
 
+
+BiCGArgs bicg_args(mastg_ctx* c, bool transpose) {
+    BiCGArgs b{};
+    b.n = c->S.n_active;
+    b.row_ptr = c->row_ptr.as<int64_t>();
+    b.col = c->col.as<int32_t>();
+    b.val = transpose ? c->valT.as<double>() : c->val.as<double>();
+    b.dinv = c->dinv.as<double>();
+    b.x = c->vx.as<double>(); b.r = c->vr.as<double>(); b.r0 = c->vr0.as<double>();
+    b.p = c->vp.as<double>(); b.v = c->vv.as<double>(); b.s = c->vs.as<double>();
+    b.t = c->vt.as<double>(); b.ph = c->vph.as<double>(); b.sh = c->vsh.as<double>();
+    b.part0 = c->part0.as<double>();
+    b.part1 = c->part1.as<double>();
+    b.nparts = c->nparts;
+    b.nvec = std::min<int64_t>(2048, std::max<int64_t>(1, (c->S.n_active + 255) / 256));
+    b.spmv_sw = c->spmv_sw;
+    b.S = c->bscal.as<BiCGScalars>();
+    return b;
+}
+
+int ensure_bicg(mastg_ctx* c) {
+    const size_t nb = size_t(c->S.n_active) * sizeof(double);
+    for (DevBuf* b : {&c->vr0, &c->vv, &c->vs, &c->vt, &c->vph, &c->vsh}) TRY(dalloc(c, *b, nb));
+    TRY(dalloc(c, c->bscal, sizeof(BiCGScalars)));
+    return MASTG_OK;
+}
+
+// BiCGStab + Jacobi on the resident matrix (or its transpose): b -> c->vx.
+// The true residual ||b - A x|| / ||b|| is recomputed at the end.
+int bicg_solve(mastg_ctx* c, const double* b, const mastg_solve_opts* o, mastg_solve_stats* st,
+               bool transpose) {
+    const Structure& S = c->S;
+    const double rtol = (o && o->rtol > 0) ? o->rtol : 1e-10;
+    const int max_it = (o && o->max_iters > 0) ? o->max_iters : 100000;
+    const int chunk = (o && o->check_every > 0) ? o->check_every : 16;
+    if (!c->tuned) TRY(autotune(c));
+    TRY(ensure_bicg(c));
+    if (transpose) {
+        if (c->S.nav != 1) return fail(c, MASTG_ERR_INVALID, "transpose solve needs a scalar field");
+        TRY(dalloc(c, c->valT, size_t(S.nnz) * sizeof(double)));
+        launch_gather_values(S.nnz, c->tpos.as<int32_t>(), c->val.as<double>(), c->valT.as<double>(),
+                             c->stream);
+    }
+    BiCGArgs a = bicg_args(c, transpose);
+    hipEvent_t e0, e1;
+    HIPC(c, hipEventCreate(&e0));
+    HIPC(c, hipEventCreate(&e1));
+    HIPC(c, hipEventRecord(e0, c->stream));
+    launch_bicg_start(a, b, rtol * rtol, max_it, c->stream);
+    int done = 0;
+    if (c->use_graphs) {
+        if (!c->bicg_exec || c->bicg_exec_len != chunk || c->bicg_exec_T != transpose) {
+            if (c->bicg_exec) (void)hipGraphExecDestroy(c->bicg_exec);
+            c->bicg_exec = nullptr;
+            hipGraph_t g;
+            HIPC(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            for (int i = 0; i < chunk; i++) launch_bicg_iteration(a, c->stream);
+            HIPC(c, hipStreamEndCapture(c->stream, &g));
+            HIPC(c, hipGraphInstantiate(&c->bicg_exec, g, nullptr, nullptr, 0));
+            (void)hipGraphDestroy(g);
+            c->bicg_exec_len = chunk;
+            c->bicg_exec_T = transpose;
+        }
+    }
+    BiCGScalars hs{};
+    for (int launched = 0; launched < max_it + chunk; launched += chunk) {
+        if (c->use_graphs) HIPC(c, hipGraphLaunch(c->bicg_exec, c->stream));
+        else
+            for (int i = 0; i < chunk; i++) launch_bicg_iteration(a, c->stream);
+        HIPC(c, hipMemcpyAsync(c->h_pin, a.S, sizeof(BiCGScalars), hipMemcpyDeviceToHost, c->stream));
+        HIPC(c, hipStreamSynchronize(c->stream));
+        std::memcpy(&hs, c->h_pin, sizeof(BiCGScalars));
+        done = hs.done;
+        if (done) break;
+    }
+    HIPC(c, hipEventRecord(e1, c->stream));
+    // true residual with the operator that was solved
+    CGArgs ca = cg_args(c);
+    ca.val = a.val;
+    launch_spmv_csr(ca, c->vx.as<double>(), c->vtmp.as<double>(), 0, c->stream);
+    launch_axpby(S.n_active, 1.0, b, -1.0, c->vtmp.as<double>(), c->stream);
+    double rr = 0, bb = 0;
+    TRY(dot_active(c, c->vtmp.as<double>(), c->vtmp.as<double>(), &rr));
+    TRY(dot_active(c, b, b, &bb));
+    float ms = 0;
+    HIPC(c, hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (st) {
+        st->iters = hs.iter;
+        st->replacements = 0;
+        st->rel_res = bb > 0 ? std::sqrt(rr / bb) : 0.0;
+        st->seconds = ms * 1e-3;
+    }
+    if (done == 3 || done == 4) return fail(c, MASTG_ERR_NAN, "BiCGStab breakdown or non-finite iterate");
+    if (done != 1) return fail(c, MASTG_ERR_NOT_CONVERGED, "BiCGStab did not reach the tolerance");
+    return MASTG_OK;
+}
+
 // Launch-shape autotuner: times the CSR SpMV over lanes-per-row x grid size on
 // the resident matrix (HIP events, 3 launches each) and keeps the fastest.
 // The grid size also sets the CG vector kernels and the partial-sum count.
@@ -363,8 +547,6 @@ int autotune(mastg_ctx* c) {
     const int grids[] = {1024, 2048, 4096, 8192};
     double best = 1e30;
     int bsw = c->spmv_sw, bgrid = c->nparts;
-    TRY(dalloc(c, c->part0, 8192 * sizeof(double)));
-    TRY(dalloc(c, c->part1, 8192 * sizeof(double)));
     for (int g : grids)
         for (int sw : sws) {
             c->nparts = g;
@@ -383,6 +565,95 @@ int autotune(mastg_ctx* c) {
     c->spmv_sw = bsw;
     for (int i = 0; i < 2; i++)
         if (c->cg_exec[i]) { (void)hipGraphExecDestroy(c->cg_exec[i]); c->cg_exec[i] = nullptr; }
+    return MASTG_OK;
+}
+
+
+// Steady nonlinear conduction: Newton to ||R|| <= rtol_newton ||R_0|| (one fresh
+// residual + Jacobian assembly per step, BiCGStab + Jacobi inner solves with
+// an Eisenstat-Walker style forcing term), then the mean temperature and its
+// adjoint sensitivity: J^T lambda = dJ/dT, dJ/dq_e = lambda_e . int N_e.
+int heat_design_eval(mastg_ctx* c, double* dobj, int where, const mastg_solve_opts* o,
+                     mastg_design_result* out) {
+    const Structure& S = c->S;
+    const double newton_rtol = (o && o->rtol > 0) ? o->rtol : 1e-10;
+    hipEvent_t e[4];
+    for (auto& x : e) HIPC(c, hipEventCreate(&x));
+    double t_asm = 0, t_solve = 0;
+    float ms = 0;
+    launch_fill(S.n_active, 0.0, c->vT.as<double>(), c->stream);
+    double r0 = -1, rn = 0;
+    int newton = 0, lin_iters = 0, rc = MASTG_OK;
+    double last_rel = 0;
+    for (int it = 0; it < 50; it++) {
+        HIPC(c, hipEventRecord(e[0], c->stream));
+        TRY(assemble_active(c, c->vT.as<double>(), true, true));
+        HIPC(c, hipEventRecord(e[1], c->stream));
+        double rr = 0;
+        TRY(dot_active(c, c->vtmp.as<double>(), c->vtmp.as<double>(), &rr));
+        HIPC(c, hipEventElapsedTime(&ms, e[0], e[1]));
+        t_asm += ms * 1e-3;
+        rn = std::sqrt(rr);
+        if (r0 < 0) r0 = rn;
+        if (!std::isfinite(rn)) return fail(c, MASTG_ERR_NAN, "non-finite residual in Newton");
+        if (rn <= newton_rtol * r0 || rn == 0.0) break;
+        // b = -R
+        HIPC(c, hipMemcpyAsync(c->vb.p, c->vtmp.p, S.n_active * sizeof(double), hipMemcpyDeviceToDevice,
+                               c->stream));
+        launch_axpby(S.n_active, 0.0, c->vb.as<double>(), -1.0, c->vb.as<double>(), c->stream);
+        mastg_solve_opts lo{};
+        lo.solver = MASTG_SOLVER_BICGSTAB_JACOBI;
+        lo.rtol = std::max(1e-13, std::min(1e-4, 1e-2 * rn / r0));
+        lo.max_iters = (o && o->max_iters > 0) ? o->max_iters : 100000;
+        mastg_solve_stats st{};
+        rc = bicg_solve(c, c->vb.as<double>(), &lo, &st, false);
+        t_solve += st.seconds;
+        lin_iters += st.iters;
+        last_rel = st.rel_res;
+        if (rc) return rc;
+        launch_axpby(S.n_active, 1.0, c->vx.as<double>(), 1.0, c->vT.as<double>(), c->stream);
+        newton = it + 1;
+    }
+    if (!(rn <= newton_rtol * r0 || rn == 0.0))
+        return fail(c, MASTG_ERR_NOT_CONVERGED, "Newton did not reach the tolerance");
+    // output and adjoint at the converged state (the resident matrix is J(T*))
+    double J = 0;
+    TRY(dot_active(c, c->wmean.as<double>(), c->vT.as<double>(), &J));
+    HIPC(c, hipEventRecord(e[2], c->stream));
+    HIPC(c, hipMemcpyAsync(c->vb.p, c->wmean.p, S.n_active * sizeof(double), hipMemcpyDeviceToDevice,
+                           c->stream));
+    launch_mask_constrained(S.n_active, c->con.as<uint8_t>(), c->vb.as<double>(), c->stream);
+    mastg_solve_opts ao{};
+    ao.solver = MASTG_SOLVER_BICGSTAB_JACOBI;
+    ao.rtol = 1e-12;
+    ao.max_iters = (o && o->max_iters > 0) ? o->max_iters : 100000;
+    mastg_solve_stats ast{};
+    rc = bicg_solve(c, c->vb.as<double>(), &ao, &ast, true);
+    if (rc) return rc;
+    lin_iters += ast.iters;
+    // dJ/dq_e = -lambda . dR/dq_e = lambda_e . int N_e
+    double* gd = (where == MASTG_HOST || !dobj) ? c->vg.as<double>() : dobj;
+    {
+        TimedRegion t(c, 4);
+        launch_heat_source_dot(heat_args(c, c->vT.as<double>(), false, false), c->vx.as<double>(), 1.0, gd,
+                               c->stream);
+    }
+    HIPC(c, hipEventRecord(e[3], c->stream));
+    if (dobj && where == MASTG_HOST)
+        HIPC(c, hipMemcpyAsync(dobj, gd, S.n_elems * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    float t23 = 0;
+    (void)hipEventElapsedTime(&t23, e[2], e[3]);
+    for (auto& x : e) (void)hipEventDestroy(x);
+    if (out) {
+        out->objective = J;
+        out->iters = lin_iters;
+        out->newton_iters = newton;
+        out->rel_res = last_rel;
+        out->t_assemble = t_asm;
+        out->t_solve = t_solve;
+        out->t_sens = t23 * 1e-3;
+    }
     return MASTG_OK;
 }
 
@@ -440,6 +711,51 @@ int mastg_ctx_create(const mastg_problem_desc* desc, mastg_ctx** out) {
     std::vector<double> F(S.n_active, 0.0);
     if (desc->point_loads)
         for (int64_t i = 0; i < S.n_active; i++) F[i] = desc->point_loads[S.caller_dof_of_active[i]];
+    if (desc->physics == MASTG_QUAD4_MINDLIN) {
+        // plane-stress D and G (IsotropicMaterialProperty StiffnessMatrix2D / TransverseShear,
+        // src/property_cards/isotropic_material_property_card.cpp:334-426)
+        const double E = c->d.E, nu = c->d.nu;
+        std::memset(c->Dps, 0, sizeof(c->Dps));
+        c->Dps[0] = c->Dps[4] = E / (1. - nu * nu);
+        c->Dps[1] = c->Dps[3] = E * nu / (1. - nu * nu);
+        c->Dps[8] = E / 2. / (1. + nu);
+        c->Gmod = E / 2. / (1. + nu);
+        // elements must lie in one z = const plane (no local frame)
+        const double z0 = S.xyz_r[2];
+        for (int64_t r = 0; r < S.n_nodes; r++)
+            if (S.xyz_r[r * 3 + 2] != z0) {
+                c->err = "Mindlin plate elements must lie in one z = const plane";
+                return bail(MASTG_ERR_INVALID);
+            }
+        // uniform pressure along local -z: R_w += p int N (surface_pressure_residual,
+        // src/elasticity/structural_element_base.cpp:1167-1235), i.e. F_w = -p int N
+        if (c->d.pressure != 0.0) {
+            const double gq = 0.57735026918962576451;
+            for (int64_t e = 0; e < S.n_elems; e++) {
+                double X[4][2];
+                for (int n = 0; n < 4; n++)
+                    for (int i = 0; i < 2; i++) X[n][i] = S.xyz_r[int64_t(S.econn[e * 4 + n]) * 3 + i];
+                for (int qj = 0; qj < 2; qj++)
+                    for (int qi = 0; qi < 2; qi++) {
+                        const double xi = qi ? gq : -gq, eta = qj ? gq : -gq;
+                        double N[4], dxi[4], deta[4];
+                        for (int n = 0; n < 4; n++) {
+                            const double sx = (n == 1 || n == 2) ? 1.0 : -1.0, sy = (n >= 2) ? 1.0 : -1.0;
+                            const double fx = 0.5 * (1.0 + sx * xi), fy = 0.5 * (1.0 + sy * eta);
+                            N[n] = fx * fy; dxi[n] = 0.5 * sx * fy; deta[n] = fx * 0.5 * sy;
+                        }
+                        double j00 = 0, j01 = 0, j10 = 0, j11 = 0;
+                        for (int n = 0; n < 4; n++) {
+                            j00 += X[n][0] * dxi[n]; j01 += X[n][0] * deta[n];
+                            j10 += X[n][1] * dxi[n]; j11 += X[n][1] * deta[n];
+                        }
+                        const double jxw = std::fabs(j00 * j11 - j01 * j10);
+                        for (int n = 0; n < 4; n++)
+                            F[int64_t(S.econn[e * 4 + n]) * 5 + 2] -= jxw * N[n] * c->d.pressure;
+                    }
+            }
+        }
+    }
     std::vector<int32_t> conv(S.constrained.begin(), S.constrained.end());
     if ((rc = upload(c, c->xyz, S.xyz_r)) || (rc = upload(c, c->econn, S.econn)) ||
         (rc = upload(c, c->blk_ptr, S.blk_ptr)) || (rc = upload(c, c->blk_col, S.blk_col)) ||
@@ -451,6 +767,50 @@ int mastg_ctx_create(const mastg_problem_desc* desc, mastg_ctx** out) {
         (rc = upload(c, c->F, F)) || (rc = dalloc(c, c->param, S.n_elems * sizeof(double))) ||
         (rc = ensure_vectors(c)))
         return bail(rc);
+    if (desc->physics == MASTG_HEX8_HEAT) {
+        // convective sides per element and the mean-temperature weights
+        std::vector<uint8_t> sm(S.n_elems, 0);
+        for (int64_t k = 0; k < desc->n_sides; k++) {
+            const int32_t e = desc->side_elem[k], sd = desc->side_index[k], id = desc->side_id[k];
+            if (e < 0 || e >= S.n_elems || sd < 0 || sd > 5 || id < 0 || id > 31) {
+                c->err = "boundary side out of range";
+                return bail(MASTG_ERR_INVALID);
+            }
+            if ((c->d.conv_bid_mask >> id) & 1u) sm[e] |= uint8_t(1u << sd);
+        }
+        std::vector<double> w(S.n_active, 0.0);
+        const double gq = 0.57735026918962576451;
+        double V = 0;
+        for (int64_t e = 0; e < S.n_elems; e++) {
+            double X[8][3];
+            for (int n = 0; n < 8; n++)
+                for (int i = 0; i < 3; i++) X[n][i] = S.xyz_r[int64_t(S.econn[e * 8 + n]) * 3 + i];
+            for (int q = 0; q < 8; q++) {
+                const double xi = (q & 1) ? gq : -gq, eta = (q & 2) ? gq : -gq, zeta = (q & 4) ? gq : -gq;
+                double N[8], d[8][3], J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+                for (int n = 0; n < 8; n++) {
+                    const double sx = (n == 1 || n == 2 || n == 5 || n == 6) ? 1.0 : -1.0;
+                    const double sy = (n == 2 || n == 3 || n == 6 || n == 7) ? 1.0 : -1.0;
+                    const double sz = (n >= 4) ? 1.0 : -1.0;
+                    const double fx = 0.5 * (1 + sx * xi), fy = 0.5 * (1 + sy * eta), fz = 0.5 * (1 + sz * zeta);
+                    N[n] = fx * fy * fz;
+                    d[n][0] = 0.5 * sx * fy * fz; d[n][1] = fx * 0.5 * sy * fz; d[n][2] = fx * fy * 0.5 * sz;
+                    for (int i = 0; i < 3; i++)
+                        for (int r = 0; r < 3; r++) J[i * 3 + r] += X[n][i] * d[n][r];
+                }
+                const double det = J[0] * (J[4] * J[8] - J[7] * J[5]) + J[3] * (J[7] * J[2] - J[1] * J[8]) +
+                                   J[6] * (J[1] * J[5] - J[4] * J[2]);
+                for (int n = 0; n < 8; n++) w[S.econn[e * 8 + n]] += det * N[n];
+                V += det;
+            }
+        }
+        for (auto& x : w) x /= V;
+        c->volume = V;
+        std::vector<int32_t> tp(S.blk_transpose.begin(), S.blk_transpose.end());
+        if ((rc = upload(c, c->sidemask, sm)) || (rc = upload(c, c->wmean, w)) || (rc = upload(c, c->tpos, tp)) ||
+            (rc = dalloc(c, c->vT, S.n_active * sizeof(double))) || (rc = ensure_bicg(c)))
+            return bail(rc);
+    }
     if (desc->physics == MASTG_HEX8_ELASTIC && S.n_classes > 0) {
         // class reference matrices at E = 1 from the element kernel itself
         std::vector<int32_t> rep_conn(S.n_classes * 8);
@@ -496,6 +856,7 @@ int mastg_ctx_destroy(mastg_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (int i = 0; i < 2; i++)
         if (c->cg_exec[i]) (void)hipGraphExecDestroy(c->cg_exec[i]);
+    if (c->bicg_exec) (void)hipGraphExecDestroy(c->bicg_exec);
     if (c->h_scal) (void)hipHostFree(c->h_scal);
     if (c->h_pin) (void)hipHostFree(c->h_pin);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -574,11 +935,19 @@ int mastg_solve(mastg_ctx* c, const double* rhs, double* x, int where, const mas
     if (!c || !rhs || !x) return MASTG_ERR_INVALID;
     if (!c->assembled) return fail(c, MASTG_ERR_INVALID, "no assembled matrix");
     if (o && o->transpose) { /* the elastic Jacobian is symmetric */ }
-    const int solver = o ? o->solver : MASTG_SOLVER_CG_JACOBI;
+    int solver = o ? o->solver : MASTG_SOLVER_CG_JACOBI;
+    if (c->d.physics == MASTG_HEX8_HEAT && solver == MASTG_SOLVER_CG_JACOBI)
+        solver = MASTG_SOLVER_BICGSTAB_JACOBI;   // the k(T) Jacobian is not symmetric
+    if (solver == MASTG_SOLVER_BICGSTAB_JACOBI) {
+        TRY(to_active(c, rhs, where, c->vb.as<double>()));
+        int rc = bicg_solve(c, c->vb.as<double>(), o, st, o && o->transpose);
+        int rc2 = from_active(c, c->vx.as<double>(), x, where);
+        return rc ? rc : rc2;
+    }
     if (solver != MASTG_SOLVER_CG_JACOBI && solver != MASTG_SOLVER_MATFREE)
         return fail(c, MASTG_ERR_INVALID, "solver not available in this build");
-    if (solver == MASTG_SOLVER_MATFREE && c->S.n_classes == 0)
-        return fail(c, MASTG_ERR_INVALID, "matrix-free solver needs a mesh of at most 8 element shapes");
+    if (solver == MASTG_SOLVER_MATFREE && (c->S.n_classes == 0 || c->d.physics != MASTG_HEX8_ELASTIC))
+        return fail(c, MASTG_ERR_INVALID, "matrix-free solver needs hex8 elasticity on a mesh of at most 8 element shapes");
     TRY(to_active(c, rhs, where, c->vb.as<double>()));
     int rc = cg_solve(c, c->vb.as<double>(), o, st, solver == MASTG_SOLVER_MATFREE);
     int rc2 = from_active(c, c->vx.as<double>(), x, where);
@@ -588,7 +957,8 @@ int mastg_solve(mastg_ctx* c, const double* rhs, double* x, int where, const mas
 int mastg_jacobian_product(mastg_ctx* c, const double* dX, double* JdX, int where, int op) {
     if (!c || !dX || !JdX) return MASTG_ERR_INVALID;
     if (!c->assembled) return fail(c, MASTG_ERR_INVALID, "no assembled matrix");
-    if (op == 1 && c->S.n_classes == 0) return fail(c, MASTG_ERR_INVALID, "no matrix-free operator");
+    if (op == 1 && (c->S.n_classes == 0 || c->d.physics != MASTG_HEX8_ELASTIC))
+        return fail(c, MASTG_ERR_INVALID, "no matrix-free operator");
     TRY(to_active(c, dX, where, c->vb.as<double>()));
     CGArgs a = cg_args(c);
     if (op == 1) {
@@ -612,14 +982,14 @@ int mastg_jacobian_product(mastg_ctx* c, const double* dX, double* JdX, int wher
 
 int mastg_sensitivity(mastg_ctx* c, const double* X, const double* lam, double* g, int where) {
     if (!c || !X || !lam || !g) return MASTG_ERR_INVALID;
-    if (c->d.physics != MASTG_HEX8_ELASTIC) return fail(c, MASTG_ERR_INVALID, "physics not available");
+    if (c->d.physics == MASTG_HEX8_HEAT) return fail(c, MASTG_ERR_INVALID, "physics not available");
+    if (!c->params_set) return fail(c, MASTG_ERR_INVALID, "design parameters not set");
     TRY(to_active(c, X, where, c->vb.as<double>()));
     TRY(to_active(c, lam, where, c->vtmp.as<double>()));
-    Hex8Args a = hex8_args(c, c->vb.as<double>(), false);
     double* gd = where == MASTG_HOST ? c->vg.as<double>() : g;
     {
         TimedRegion t(c, 4);
-        launch_hex8_sens(a, c->vtmp.as<double>(), gd, c->stream);
+        launch_sens(c, c->vb.as<double>(), c->vtmp.as<double>(), gd);
     }
     if (where == MASTG_HOST)
         HIPC(c, hipMemcpyAsync(g, gd, c->S.n_elems * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -629,6 +999,20 @@ int mastg_sensitivity(mastg_ctx* c, const double* X, const double* lam, double* 
 
 int mastg_output(mastg_ctx* c, int kind, const double* X, int where, double* value, double* dv) {
     if (!c || !X || !value) return MASTG_ERR_INVALID;
+    if (kind == MASTG_OUTPUT_MEAN_T) {
+        if (c->d.physics != MASTG_HEX8_HEAT) return fail(c, MASTG_ERR_INVALID, "mean temperature needs heat physics");
+        TRY(to_active(c, X, where, c->vb.as<double>()));
+        // J = sum_i w_i T_i with w = (1/V) int N (over all nodes; the constrained
+        // ones carry T = 0), dJ/dT = w with constrained entries removed
+        TRY(dot_active(c, c->wmean.as<double>(), c->vb.as<double>(), value));
+        if (dv) {
+            HIPC(c, hipMemcpyAsync(c->vtmp.p, c->wmean.p, c->S.n_active * sizeof(double),
+                                   hipMemcpyDeviceToDevice, c->stream));
+            launch_mask_constrained(c->S.n_active, c->con.as<uint8_t>(), c->vtmp.as<double>(), c->stream);
+            TRY(from_active(c, c->vtmp.as<double>(), dv, where));
+        }
+        return MASTG_OK;
+    }
     if (kind != MASTG_OUTPUT_COMPLIANCE) return fail(c, MASTG_ERR_INVALID, "output not available");
     TRY(to_active(c, X, where, c->vb.as<double>()));
     // external load vector with constrained entries removed
@@ -643,9 +1027,9 @@ int mastg_output(mastg_ctx* c, int kind, const double* X, int where, double* val
 int mastg_design_eval(mastg_ctx* c, const double* params, double* dobj, int where,
                       const mastg_solve_opts* o, mastg_design_result* out) {
     if (!c || !params) return MASTG_ERR_INVALID;
-    if (c->d.physics != MASTG_HEX8_ELASTIC) return fail(c, MASTG_ERR_INVALID, "physics not available");
     const Structure& S = c->S;
     TRY(mastg_set_params(c, params, where));
+    if (c->d.physics == MASTG_HEX8_HEAT) return heat_design_eval(c, dobj, where, o, out);
     hipEvent_t e[4];
     for (auto& x : e) HIPC(c, hipEventCreate(&x));
     HIPC(c, hipEventRecord(e[0], c->stream));
@@ -656,20 +1040,19 @@ int mastg_design_eval(mastg_ctx* c, const double* params, double* dobj, int wher
     HIPC(c, hipEventRecord(e[1], c->stream));
     mastg_solve_stats st{};
     const int solver = o ? o->solver : MASTG_SOLVER_CG_JACOBI;
-    if (solver == MASTG_SOLVER_MATFREE && c->S.n_classes == 0)
-        return fail(c, MASTG_ERR_INVALID, "matrix-free solver needs a mesh of at most 8 element shapes");
+    if (solver == MASTG_SOLVER_MATFREE && (c->S.n_classes == 0 || c->d.physics != MASTG_HEX8_ELASTIC))
+        return fail(c, MASTG_ERR_INVALID, "matrix-free solver needs hex8 elasticity on a mesh of at most 8 element shapes");
     if (solver != MASTG_SOLVER_CG_JACOBI && solver != MASTG_SOLVER_MATFREE)
         return fail(c, MASTG_ERR_INVALID, "solver not available in this build");
     int rc = cg_solve(c, c->vb.as<double>(), o, &st, solver == MASTG_SOLVER_MATFREE);
     HIPC(c, hipEventRecord(e[2], c->stream));
     double C = 0;
     TRY(dot_active(c, c->vb.as<double>(), c->vx.as<double>(), &C));
-    // self-adjoint compliance: lambda = u, dC/drho = -u . dR/drho
-    Hex8Args a = hex8_args(c, c->vx.as<double>(), false);
+    // self-adjoint compliance: lambda = u, dC/dp = -u . dR/dp
     double* gd = (where == MASTG_HOST || !dobj) ? c->vg.as<double>() : dobj;
     {
         TimedRegion t(c, 4);
-        launch_hex8_sens(a, c->vx.as<double>(), gd, c->stream);
+        launch_sens(c, c->vx.as<double>(), c->vx.as<double>(), gd);
     }
     launch_axpby(S.n_elems, 0.0, gd, -1.0, gd, c->stream);
     HIPC(c, hipEventRecord(e[3], c->stream));
@@ -721,6 +1104,14 @@ int mastg_enable_kernel_timing(mastg_ctx* c, int on) {
     c->timing = on != 0;
     return MASTG_OK;
 }
+int mastg_set_option(mastg_ctx* c, int option, int value) {
+    if (!c) return MASTG_ERR_INVALID;
+    switch (option) {
+        case MASTG_OPT_GRAPHS: c->use_graphs = value != 0; return MASTG_OK;
+        default: return fail(c, MASTG_ERR_INVALID, "unknown option");
+    }
+}
+
 int mastg_get_tuning(const mastg_ctx* c, int* spmv_lanes, int* grid) {
     if (!c) return MASTG_ERR_INVALID;
     if (spmv_lanes) *spmv_lanes = c->spmv_sw;
@@ -732,8 +1123,6 @@ int mastg_set_tuning(mastg_ctx* c, int spmv_lanes, int grid) {
     if (!c || (spmv_lanes != 4 && spmv_lanes != 8 && spmv_lanes != 16 && spmv_lanes != 32 &&
                spmv_lanes != 64) || grid < 64 || grid > 8192)
         return MASTG_ERR_INVALID;
-    TRY(dalloc(c, c->part0, 8192 * sizeof(double)));
-    TRY(dalloc(c, c->part1, 8192 * sizeof(double)));
     c->spmv_sw = spmv_lanes;
     c->nparts = grid;
     c->tuned = true;
@@ -753,7 +1142,7 @@ int mastg_bench_kernel(mastg_ctx* c, int kind, int n_launch, double* avg_ms) {
     HIPC(c, hipMemsetAsync(c->scal.p, 0, sizeof(CGScalars), c->stream));
     auto launch = [&]() {
         switch (kind) {
-            case 0: launch_hex8_K(h, c->stream); break;
+            case 0: launch_elements(c, nullptr, true); break;
             case 1: launch_scatter(sa, c->stream); break;
             case 2: launch_spmv_csr(a, a.p, a.q, 1, c->stream); break;
             case 3:
@@ -764,12 +1153,13 @@ int mastg_bench_kernel(mastg_ctx* c, int kind, int n_launch, double* avg_ms) {
                 }
                 launch_mf_apply(a, a.p, a.p, a.q, 1, c->stream);
                 break;
-            case 4: launch_hex8_sens(hex8_args(c, a.x, false), a.x, c->vg.as<double>(), c->stream); break;
+            case 4: launch_sens(c, a.x, a.x, c->vg.as<double>()); break;
             default: break;
         }
     };
     if (kind < 0 || kind > 4) return fail(c, MASTG_ERR_INVALID, "unknown kernel kind");
-    if (kind == 3 && c->S.n_classes == 0) return fail(c, MASTG_ERR_INVALID, "no matrix-free operator");
+    if (kind == 3 && (c->S.n_classes == 0 || c->d.physics != MASTG_HEX8_ELASTIC))
+        return fail(c, MASTG_ERR_INVALID, "no matrix-free operator");
     launch();   // warm
     HIPC(c, hipEventRecord(c->ev0, c->stream));
     for (int i = 0; i < n_launch; i++) launch();

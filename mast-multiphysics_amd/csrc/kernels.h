@@ -18,6 +18,36 @@ struct Hex8Args {
     double* Kel;            // [n_elems*64*9] element blocks or null
 };
 
+// quad4 Mindlin plate (elements in a z = const plane), active vars u v w tx ty
+struct Quad4Args {
+    int64_t n_elems;
+    const double* xyz;      // [n_nodes*3] by node rank
+    const int32_t* econn;   // [n_elems*4]
+    const double* param;    // [n_elems] thickness h_e
+    double Dps[9];          // plane-stress material matrix
+    double G;               // shear modulus
+    double kappa, off;      // shear factor, section offset
+    const uint8_t* con;     // [n_active] or null
+    const double* X;        // [n_active] state or null
+    double* Rel;            // [n_elems*4*5] element residual rows (when X)
+    double* Kel;            // [n_elems*16*25] element blocks or null
+};
+
+// hex8 steady conduction, k(T) = k0 (1 + alpha T)
+struct HeatArgs {
+    int64_t n_elems;
+    const double* xyz;      // [n_nodes*3] by node rank
+    const int32_t* econn;   // [n_elems*8]
+    const double* param;    // [n_elems] volumetric source q_e
+    double k0, alpha;
+    double h, Tinf;         // convection coefficient / ambient temperature
+    const uint8_t* sides;   // [n_elems] bit s = side s convects
+    const uint8_t* con;     // [n_active] or null
+    const double* X;        // [n_active] temperature (state), required
+    double* Rel;            // [n_elems*8] element residual or null
+    double* Kel;            // [n_elems*64] element Jacobian or null
+};
+
 struct ScatterArgs {
     int64_t n_nodes;
     int nav;
@@ -58,15 +88,40 @@ struct CGArgs {
     double *part0, *part1;
     int nparts;
     int nparts_op;            // partial sums written by the operator kernel
+    int nvec;                 // grid of the CG vector kernels (and their partials)
     CGScalars* S;
     int use_mf;               // operator: 0 CSR SpMV, 1 matrix-free
     int spmv_sw;              // CSR SpMV lanes per row (4, 8, 16, 32, 64)
     MFArgs mf;
 };
 
+// right-preconditioned BiCGStab + Jacobi on a scalar CSR (PETSc KSPBCGS role)
+struct BiCGScalars {
+    double rho, rho_old, alpha, omega, beta, rr, bb, tol2;
+    int iter, max_it, done, pad;
+};
+struct BiCGArgs {
+    int64_t n;
+    const int64_t* row_ptr;
+    const int32_t* col;
+    const double* val;        // A or A^T values in the pattern of A (pattern symmetric)
+    const double* dinv;       // inverse diagonal (same for A and A^T)
+    double *x, *r, *r0, *p, *v, *s, *t, *ph, *sh;
+    double *part0, *part1;
+    int nparts, nvec, spmv_sw;
+    BiCGScalars* S;
+};
+void launch_bicg_start(const BiCGArgs& a, const double* b, double tol2, int max_it, hipStream_t s);
+void launch_bicg_iteration(const BiCGArgs& a, hipStream_t s);
+void launch_gather_values(int64_t n, const int32_t* idx, const double* src, double* dst, hipStream_t s);
+
 void launch_hex8_K(const Hex8Args& a, hipStream_t s);
 void launch_hex8_sens(const Hex8Args& a, const double* lam, double* g, hipStream_t s);
 void launch_scatter(const ScatterArgs& a, hipStream_t s);
+void launch_quad4_K(const Quad4Args& a, hipStream_t s);
+void launch_heat_K(const HeatArgs& a, hipStream_t s);
+void launch_heat_source_dot(const HeatArgs& a, const double* lam, double scale, double* g, hipStream_t s);
+void launch_quad4_sens(const Quad4Args& a, const double* lam, double* g, hipStream_t s);
 void launch_node_residual(int64_t n_nodes, int nav, int npe, const int32_t* ninc_ptr,
                           const int32_t* ninc, const double* Rel, const double* F,
                           const uint8_t* con, double* R, hipStream_t s);

@@ -230,6 +230,393 @@ __global__ __launch_bounds__(128) void k_hex8_elastic_sens(Hex8Args A, const dou
     g[e] = dE * acc;
 }
 
+
+// ---------------------------------------------------------------------------
+// quad4 Mindlin plate: StructuralElement2D::internal_residual with the
+// MindlinBendingOperator (src/elasticity/structural_element_2d.cpp:1075-1227,
+// 1765-2019; src/elasticity/mindlin_bending_operator.cpp:57-361), linear strain,
+// element in a z = const plane (no local-frame transform, geom_elem.cpp:358-373).
+//   membrane  eps = Bm u  (exx = du/dx, eyy = dv/dy, gxy = du/dy + dv/dx)
+//   bending   kap = Bb u  (kx = d ty/dx, ky = -d tx/dy, kxy = d ty/dy - d tx/dx)
+//   shear     gam = Bs u  (gxz = dw/dx + ty, gyz = dw/dy - tx) on the 1-point rule
+//   K = sum_q c [Bm^T A Bm + Bm^T B Bb + Bb^T B^T Bm + Bb^T Db Bb] + c_s Bs^T S Bs
+// with A = Dps h, B = Dps h off, Db = Dps (h^3/12 + h off^2), S = G h kappa I
+// (Solid2DSectionProperty, solid_2d_section_element_property_card.cpp:105-157, 393-531).
+// One wave = 4 elements x 16 node pairs; lane (a, b) builds the 5x5 block.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void quad4_shape(int n, double xi, double eta, double& N, double& dxi,
+                                            double& deta) {
+    const double sx = (n == 1 || n == 2) ? 1.0 : -1.0;
+    const double sy = (n >= 2) ? 1.0 : -1.0;
+    const double fx = 0.5 * (1.0 + sx * xi), fy = 0.5 * (1.0 + sy * eta);
+    N = fx * fy;
+    dxi = 0.5 * sx * fy;
+    deta = fx * 0.5 * sy;
+}
+
+// physical gradients of the 4 shape functions at (xi, eta) from 4 node
+// coordinates; returns the area Jacobian
+__device__ __forceinline__ double quad4_grad(const double X[4][2], double xi, double eta,
+                                            double g[4][2], double N[4]) {
+    double dN[4][2];
+    double j00 = 0, j01 = 0, j10 = 0, j11 = 0;   // d x_i / d xi_r
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+        quad4_shape(n, xi, eta, N[n], dN[n][0], dN[n][1]);
+        j00 += X[n][0] * dN[n][0]; j01 += X[n][0] * dN[n][1];
+        j10 += X[n][1] * dN[n][0]; j11 += X[n][1] * dN[n][1];
+    }
+    const double det = j00 * j11 - j01 * j10;
+    const double id = 1.0 / det;
+    // inverse map: dxi/dx = j11/det, dxi/dy = -j01/det, deta/dx = -j10/det, deta/dy = j00/det
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+        g[n][0] = (dN[n][0] * j11 - dN[n][1] * j10) * id;
+        g[n][1] = (-dN[n][0] * j01 + dN[n][1] * j00) * id;
+    }
+    return det;
+}
+
+__global__ __launch_bounds__(256) void k_quad4_mindlin_K(Quad4Args A) {
+    const int sub = threadIdx.x >> 4, lane = threadIdx.x & 15;   // 16 elements per block
+    const int64_t e = int64_t(blockIdx.x) * 16 + sub;
+    if (e >= A.n_elems) return;
+    const int a = lane >> 2, b = lane & 3;
+    double X[4][2];
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+        const int64_t r = A.econn[e * 4 + n];
+        X[n][0] = A.xyz[r * 3 + 0];
+        X[n][1] = A.xyz[r * 3 + 1];
+    }
+    // unit-section blocks: membrane (u,v)x(u,v), coupling (u,v)x(tx,ty),
+    // bending (tx,ty)x(tx,ty), shear (w,tx,ty)x(w,tx,ty)
+    double Km[4] = {0, 0, 0, 0}, Kc[4] = {0, 0, 0, 0}, Kd[4] = {0, 0, 0, 0}, Kt[4] = {0, 0, 0, 0};
+    double Ks[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#define DP(i) A.Dps[i]
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        double g[4][2], N[4];
+        const double c = quad4_grad(X, gp_coord(q & 1), gp_coord(q >> 1), g, N) * 1.0;
+        const double ax = g[a][0], ay = g[a][1], bx = g[b][0], by = g[b][1];
+        // Bm_b columns (u, v): [bx, 0, by], [0, by, bx]; Bb_b columns (tx, ty): [0, -by, -bx], [bx, 0, by]
+        const double Bm_b[2][3] = {{bx, 0.0, by}, {0.0, by, bx}};
+        const double Bb_b[2][3] = {{0.0, -by, -bx}, {bx, 0.0, by}};
+        const double Bm_a[2][3] = {{ax, 0.0, ay}, {0.0, ay, ax}};
+        const double Bb_a[2][3] = {{0.0, -ay, -ax}, {ax, 0.0, ay}};
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++) {
+                double mm = 0, mb = 0, bm = 0, bb = 0;
+#pragma unroll
+                for (int r = 0; r < 3; r++) {
+                    double dmb = 0, dbb = 0, dmm = 0, dbm = 0;
+#pragma unroll
+                    for (int s2 = 0; s2 < 3; s2++) {
+                        dmm += DP(r * 3 + s2) * Bm_b[j][s2];
+                        dbb += DP(r * 3 + s2) * Bb_b[j][s2];
+                    }
+                    dmb = dbb;   // B = Dps * (h off): same unit matrix, other factor
+                    dbm = dmm;
+                    mm += Bm_a[i][r] * dmm;
+                    mb += Bm_a[i][r] * dmb;
+                    bm += Bb_a[i][r] * dbm;
+                    bb += Bb_a[i][r] * dbb;
+                }
+                Km[i * 2 + j] += c * mm;
+                Kc[i * 2 + j] += c * mb;
+                Kt[i * 2 + j] += c * bm;
+                Kd[i * 2 + j] += c * bb;
+            }
+    }
+#undef DP
+    {
+        // transverse shear on the 1-point rule (weight 4)
+        double g[4][2], N[4];
+        const double c = quad4_grad(X, 0.0, 0.0, g, N) * 4.0;
+        // Bs columns (w, tx, ty): gxz = [gx, 0, N], gyz = [gy, -N, 0]
+        const double Ba[2][3] = {{g[a][0], 0.0, N[a]}, {g[a][1], -N[a], 0.0}};
+        const double Bb2[2][3] = {{g[b][0], 0.0, N[b]}, {g[b][1], -N[b], 0.0}};
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) Ks[i * 3 + j] += c * (Ba[0][i] * Bb2[0][j] + Ba[1][i] * Bb2[1][j]);
+    }
+    const double h = A.param[e];
+    const double fA = h, fB = h * A.off, fD = h * h * h / 12.0 + h * A.off * A.off, fS = A.G * h * A.kappa;
+    double K[25];
+#pragma unroll
+    for (int i = 0; i < 25; i++) K[i] = 0.0;
+    // vars: 0 u, 1 v, 2 w, 3 tx, 4 ty
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            K[i * 5 + j] = fA * Km[i * 2 + j];
+            K[i * 5 + 3 + j] = fB * Kc[i * 2 + j];
+            K[(3 + i) * 5 + j] = fB * Kt[i * 2 + j];
+            K[(3 + i) * 5 + 3 + j] = fD * Kd[i * 2 + j];
+        }
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) K[(2 + i) * 5 + 2 + j] += fS * Ks[i * 3 + j];
+    const int32_t ra = A.econn[e * 4 + a], rb = A.econn[e * 4 + b];
+    if (A.X) {
+        double f[5];
+#pragma unroll
+        for (int v = 0; v < 5; v++) {
+            double t = 0.0;
+#pragma unroll
+            for (int w = 0; w < 5; w++) t += K[v * 5 + w] * A.X[int64_t(rb) * 5 + w];
+            f[v] = t;
+        }
+#pragma unroll
+        for (int m = 1; m < 4; m <<= 1)
+#pragma unroll
+            for (int v = 0; v < 5; v++) f[v] += __shfl_xor(f[v], m, 64);
+        if (b == 0)
+#pragma unroll
+            for (int v = 0; v < 5; v++)
+                A.Rel[(e * 4 + a) * 5 + v] = (A.con && A.con[int64_t(ra) * 5 + v]) ? 0.0 : f[v];
+    }
+    if (A.Kel) {
+        double* out = A.Kel + (e * 16 + lane) * 25;
+#pragma unroll
+        for (int v = 0; v < 5; v++)
+#pragma unroll
+            for (int w = 0; w < 5; w++) {
+                double kv = K[v * 5 + w];
+                if (A.con && (A.con[int64_t(ra) * 5 + v] || A.con[int64_t(rb) * 5 + w]))
+                    kv = (a == b && v == w) ? 1.0 : 0.0;
+                out[v * 5 + w] = kv;
+            }
+    }
+}
+
+// g_e = lambda_e . dR_e/dh_e with the section derivatives
+// dA = Dps, dB = Dps off, dDb = Dps (h^2/4 + off^2), dS = G kappa
+// (..._StiffnessMatrix::derivative, solid_2d_section_element_property_card.cpp:105-157, 393-531)
+__global__ __launch_bounds__(128) void k_quad4_mindlin_sens(Quad4Args A, const double* __restrict__ lam,
+                                                            double* __restrict__ gout) {
+    const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= A.n_elems) return;
+    double X[4][2], u[4][5], l[4][5];
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+        const int64_t r = A.econn[e * 4 + n];
+        X[n][0] = A.xyz[r * 3 + 0];
+        X[n][1] = A.xyz[r * 3 + 1];
+#pragma unroll
+        for (int v = 0; v < 5; v++) {
+            u[n][v] = A.X[r * 5 + v];
+            l[n][v] = A.con[r * 5 + v] ? 0.0 : lam[r * 5 + v];
+        }
+    }
+    const double h = A.param[e];
+    const double dA = 1.0, dB = A.off, dD = h * h / 4.0 + A.off * A.off, dS = A.G * A.kappa;
+    double acc = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        double g[4][2], N[4];
+        const double c = quad4_grad(X, gp_coord(q & 1), gp_coord(q >> 1), g, N) * 1.0;
+        double em[3] = {0, 0, 0}, kb[3] = {0, 0, 0}, lm[3] = {0, 0, 0}, lk[3] = {0, 0, 0};
+#pragma unroll
+        for (int n = 0; n < 4; n++) {
+            const double gx = g[n][0], gy = g[n][1];
+            em[0] += gx * u[n][0]; em[1] += gy * u[n][1]; em[2] += gy * u[n][0] + gx * u[n][1];
+            kb[0] += gx * u[n][4]; kb[1] -= gy * u[n][3]; kb[2] += gy * u[n][4] - gx * u[n][3];
+            lm[0] += gx * l[n][0]; lm[1] += gy * l[n][1]; lm[2] += gy * l[n][0] + gx * l[n][1];
+            lk[0] += gx * l[n][4]; lk[1] -= gy * l[n][3]; lk[2] += gy * l[n][4] - gx * l[n][3];
+        }
+        double t = 0.0;
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            double Dem = 0, Dkb = 0;
+#pragma unroll
+            for (int s2 = 0; s2 < 3; s2++) {
+                Dem += A.Dps[r * 3 + s2] * em[s2];
+                Dkb += A.Dps[r * 3 + s2] * kb[s2];
+            }
+            t += lm[r] * (dA * Dem + dB * Dkb) + lk[r] * (dB * Dem + dD * Dkb);
+        }
+        acc += c * t;
+    }
+    {
+        double g[4][2], N[4];
+        const double c = quad4_grad(X, 0.0, 0.0, g, N) * 4.0;
+        double gu[2] = {0, 0}, gl[2] = {0, 0};
+#pragma unroll
+        for (int n = 0; n < 4; n++) {
+            gu[0] += g[n][0] * u[n][2] + N[n] * u[n][4];
+            gu[1] += g[n][1] * u[n][2] - N[n] * u[n][3];
+            gl[0] += g[n][0] * l[n][2] + N[n] * l[n][4];
+            gl[1] += g[n][1] * l[n][2] - N[n] * l[n][3];
+        }
+        acc += c * dS * (gl[0] * gu[0] + gl[1] * gu[1]);
+    }
+    gout[e] = acc;
+}
+
+
+// ---------------------------------------------------------------------------
+// hex8 steady conduction: HeatConductionElementBase::internal_residual with the
+// temperature-dependent conductance k(T) = k0 (1 + alpha T) evaluated at the
+// quadrature-point temperature, its exact Jacobian including the dk/dT term
+// (src/heat_conduction/heat_conduction_elem_base.cpp:53-149), convection on
+// the element's marked sides (:1099-1154) and the volumetric source (:1709-1745):
+//   R_a = sum_q c k(T_q) g_a.gradT_q + sum_sides int h N_a (T - Tinf) - q_e int N_a
+//   J_ab = sum_q c [k(T_q) g_a.g_b + k' (gradT_q.g_a) N_b] + sum_sides int h N_a N_b
+// One wave per element, lane = (a, b).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void hex8_side_xi(int side, double s1, double s2, double& xi, double& eta,
+                                             double& zeta, int& r0, int& r1) {
+    switch (side) {
+        case 0: xi = s1; eta = s2; zeta = -1; r0 = 0; r1 = 1; break;
+        case 1: xi = s1; eta = -1; zeta = s2; r0 = 0; r1 = 2; break;
+        case 2: xi = 1; eta = s1; zeta = s2; r0 = 1; r1 = 2; break;
+        case 3: xi = s1; eta = 1; zeta = s2; r0 = 0; r1 = 2; break;
+        case 4: xi = -1; eta = s1; zeta = s2; r0 = 1; r1 = 2; break;
+        default: xi = s1; eta = s2; zeta = 1; r0 = 0; r1 = 1; break;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_hex8_heat_K(HeatArgs A) {
+    __shared__ double sg[4][8][8][3];
+    __shared__ double sN[4][8][8];
+    __shared__ double sc[4][8];
+    __shared__ double sT[4][8];
+    __shared__ double sX[4][8][3];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t e = int64_t(blockIdx.x) * 4 + wave;
+    const bool valid = e < A.n_elems;
+    const int64_t ee = valid ? e : 0;
+    {
+        const int q = lane >> 3, n = lane & 7;
+        const int32_t rn = A.econn[ee * 8 + n];
+        const double x = A.xyz[int64_t(rn) * 3 + 0], y = A.xyz[int64_t(rn) * 3 + 1],
+                     z = A.xyz[int64_t(rn) * 3 + 2];
+        if (q == 0) {
+            sT[wave][n] = A.X[rn];
+            sX[wave][n][0] = x; sX[wave][n][1] = y; sX[wave][n][2] = z;
+        }
+        double dxi, deta, dzeta, N;
+        hex8_dshape(n, gp_coord(q & 1), gp_coord((q >> 1) & 1), gp_coord(q >> 2), dxi, deta, dzeta, N);
+        double J[9] = {x * dxi, x * deta, x * dzeta, y * dxi, y * deta, y * dzeta,
+                       z * dxi, z * deta, z * dzeta};
+#pragma unroll
+        for (int m = 1; m < 8; m <<= 1)
+#pragma unroll
+            for (int i = 0; i < 9; i++) J[i] += __shfl_xor(J[i], m, 64);
+        double ixi[3], ieta[3], izeta[3];
+        const double jac = inv3(J, ixi, ieta, izeta);
+        sg[wave][q][n][0] = dxi * ixi[0] + deta * ieta[0] + dzeta * izeta[0];
+        sg[wave][q][n][1] = dxi * ixi[1] + deta * ieta[1] + dzeta * izeta[1];
+        sg[wave][q][n][2] = dxi * ixi[2] + deta * ieta[2] + dzeta * izeta[2];
+        sN[wave][q][n] = N;
+        if (n == 0) sc[wave][q] = jac * 1.0;
+    }
+    __syncthreads();
+    if (!valid) return;
+    const int a = lane >> 3, b = lane & 7;
+    double K = 0.0, f = 0.0;
+    const double dk = A.k0 * A.alpha;
+    for (int q = 0; q < 8; q++) {
+        double Tq = 0.0, gT0 = 0.0, gT1 = 0.0, gT2 = 0.0;
+#pragma unroll
+        for (int n = 0; n < 8; n++) {
+            const double t = sT[wave][n];
+            Tq += sN[wave][q][n] * t;
+            gT0 += sg[wave][q][n][0] * t;
+            gT1 += sg[wave][q][n][1] * t;
+            gT2 += sg[wave][q][n][2] * t;
+        }
+        const double c = sc[wave][q];
+        const double kk = A.k0 * (1.0 + A.alpha * Tq);
+        const double ga0 = sg[wave][q][a][0], ga1 = sg[wave][q][a][1], ga2 = sg[wave][q][a][2];
+        const double gb0 = sg[wave][q][b][0], gb1 = sg[wave][q][b][1], gb2 = sg[wave][q][b][2];
+        K += c * kk * (ga0 * gb0 + ga1 * gb1 + ga2 * gb2);
+        K += c * (gT0 * ga0 + gT1 * ga1 + gT2 * ga2) * dk * sN[wave][q][b];
+        if (b == 0) {
+            f += c * kk * (ga0 * gT0 + ga1 * gT1 + ga2 * gT2);
+            f -= c * sN[wave][q][a] * A.param[e];
+        }
+    }
+    // convection on marked sides (2x2 Gauss on the face)
+    const int sm = A.sides ? A.sides[e] : 0;
+    if (sm)
+        for (int sd = 0; sd < 6; sd++) {
+            if (!((sm >> sd) & 1)) continue;
+            for (int qq = 0; qq < 4; qq++) {
+                double xi, eta, zeta;
+                int r0, r1;
+                hex8_side_xi(sd, gp_coord(qq & 1), gp_coord(qq >> 1), xi, eta, zeta, r0, r1);
+                double Na = 0, Nb = 0, T = 0, t0[3] = {0, 0, 0}, t1[3] = {0, 0, 0};
+#pragma unroll
+                for (int n = 0; n < 8; n++) {
+                    double d[3], N;
+                    hex8_dshape(n, xi, eta, zeta, d[0], d[1], d[2], N);
+                    if (n == a) Na = N;
+                    if (n == b) Nb = N;
+                    T += N * sT[wave][n];
+#pragma unroll
+                    for (int i = 0; i < 3; i++) {
+                        t0[i] += sX[wave][n][i] * d[r0];
+                        t1[i] += sX[wave][n][i] * d[r1];
+                    }
+                }
+                const double g11 = t0[0] * t0[0] + t0[1] * t0[1] + t0[2] * t0[2];
+                const double g12 = t0[0] * t1[0] + t0[1] * t1[1] + t0[2] * t1[2];
+                const double g22 = t1[0] * t1[0] + t1[1] * t1[1] + t1[2] * t1[2];
+                const double jxw = sqrt(g11 * g22 - g12 * g12) * 1.0;
+                K += jxw * (Na * Nb) * A.h;
+                if (b == 0) f += jxw * Na * A.h * (T - A.Tinf);
+            }
+        }
+    const int32_t ra = A.econn[e * 8 + a], rb = A.econn[e * 8 + b];
+    const bool ca = A.con && A.con[ra], cb = A.con && A.con[rb];
+    if (A.Rel && b == 0) A.Rel[e * 8 + a] = ca ? 0.0 : f;
+    if (A.Kel) A.Kel[e * 64 + lane] = (ca || cb) ? ((a == b) ? 1.0 : 0.0) : K;
+}
+
+// g_e = scale * lambda_e . int N (d R/d q_e = -int N: volume_heat_source_residual_sensitivity,
+// heat_conduction_elem_base.cpp:1749-1791); constrained entries of lambda ignored
+__global__ __launch_bounds__(128) void k_hex8_source_dot(HeatArgs A, const double* __restrict__ lam,
+                                                         double scale, double* __restrict__ g) {
+    const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= A.n_elems) return;
+    double x[8][3], l[8];
+#pragma unroll
+    for (int n = 0; n < 8; n++) {
+        const int64_t r = A.econn[e * 8 + n];
+#pragma unroll
+        for (int i = 0; i < 3; i++) x[n][i] = A.xyz[r * 3 + i];
+        l[n] = (A.con && A.con[r]) ? 0.0 : lam[r];
+    }
+    double acc = 0.0;
+    for (int q = 0; q < 8; q++) {
+        const double xi = gp_coord(q & 1), eta = gp_coord((q >> 1) & 1), zeta = gp_coord(q >> 2);
+        double J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, N[8];
+#pragma unroll
+        for (int n = 0; n < 8; n++) {
+            double d[3];
+            hex8_dshape(n, xi, eta, zeta, d[0], d[1], d[2], N[n]);
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int r = 0; r < 3; r++) J[i * 3 + r] += x[n][i] * d[r];
+        }
+        double ixi[3], ieta[3], izeta[3];
+        const double jac = inv3(J, ixi, ieta, izeta);
+        double t = 0.0;
+#pragma unroll
+        for (int n = 0; n < 8; n++) t += N[n] * l[n];
+        acc += jac * t;
+    }
+    g[e] = scale * acc;
+}
+
 // ---------------------------------------------------------------------------
 // Segmented-reduction scatter of materialised element blocks into the scalar
 // CSR values (NonlinearImplicitAssembly add_matrix, nonlinear_implicit_assembly.cpp:182-183):
@@ -325,8 +712,17 @@ __global__ __launch_bounds__(256) void k_dot(int64_t n, const double* __restrict
 // deterministic final reduction of `n` partials (one block of 256)
 __device__ __forceinline__ double reduce_partials(const double* __restrict__ p, int n) {
     __shared__ double red[4];
-    double s = 0.0;
-    for (int i = threadIdx.x; i < n; i += 256) s += p[i];
+    // four independent accumulators keep four loads in flight per thread
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int i = threadIdx.x;
+    for (; i + 768 < n; i += 1024) {
+        s0 += p[i];
+        s1 += p[i + 256];
+        s2 += p[i + 512];
+        s3 += p[i + 768];
+    }
+    for (; i < n; i += 256) s0 += p[i];
+    double s = (s0 + s1) + (s2 + s3);
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) s += __shfl_xor(s, m, 64);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
@@ -499,8 +895,8 @@ __global__ __launch_bounds__(256) void k_cg_update(CGArgs C) {
 // beta = rz_new / rz ; convergence on ||r||^2 <= tol2 * ||b||^2
 __global__ __launch_bounds__(256) void k_cg_beta(CGArgs C) {
     if (C.S->done) return;
-    const double rz = reduce_partials(C.part0, C.nparts);
-    const double rr = reduce_partials(C.part1, C.nparts);
+    const double rz = reduce_partials(C.part0, C.nvec);
+    const double rr = reduce_partials(C.part1, C.nvec);
     if (threadIdx.x == 0) {
         CGScalars* S = C.S;
         S->beta = rz / S->rz;
@@ -538,8 +934,8 @@ __global__ __launch_bounds__(256) void k_cg_start(CGArgs C, const double* __rest
 }
 
 __global__ __launch_bounds__(256) void k_cg_start_scalars(CGArgs C, double tol2, int max_it) {
-    const double rz = reduce_partials(C.part0, C.nparts);
-    const double bb = reduce_partials(C.part1, C.nparts);
+    const double rz = reduce_partials(C.part0, C.nvec);
+    const double bb = reduce_partials(C.part1, C.nvec);
     if (threadIdx.x == 0) {
         CGScalars* S = C.S;
         S->rz = rz;
@@ -552,6 +948,146 @@ __global__ __launch_bounds__(256) void k_cg_start_scalars(CGArgs C, double tol2,
     }
 }
 
+
+// ------------------------------------------------- BiCGStab (FP64 CSR) ----
+// y = A x with SW lanes per row; optional partials sum(y.w1) -> part0 and
+// sum(y.y) -> part1 (per block).  Stops immediately once the solver is done.
+template <int SW>
+__global__ __launch_bounds__(256) void k_spmv_csr_gen(BiCGArgs B, const double* __restrict__ x,
+                                                      double* __restrict__ y, const double* __restrict__ w1,
+                                                      int want_yy) {
+    if (B.S->done) return;
+    const int sub = threadIdx.x & (SW - 1);
+    double d1 = 0.0, d2 = 0.0;
+    const int64_t rows_per_grid = (int64_t(gridDim.x) * 256) / SW;
+    for (int64_t row = (int64_t(blockIdx.x) * 256 + threadIdx.x) / SW; row < B.n; row += rows_per_grid) {
+        const int64_t k0 = B.row_ptr[row], k1 = B.row_ptr[row + 1];
+        double s = 0.0;
+#pragma unroll 4
+        for (int64_t k = k0 + sub; k < k1; k += SW) s += B.val[k] * x[B.col[k]];
+#pragma unroll
+        for (int m = SW / 2; m >= 1; m >>= 1) s += __shfl_xor(s, m, SW);
+        if (sub == 0) {
+            y[row] = s;
+            if (w1) d1 += s * w1[row];
+            d2 += s * s;
+        }
+    }
+    if (w1) block_sum_store(d1, B.part0, blockIdx.x);
+    if (want_yy) {
+        __syncthreads();
+        block_sum_store(d2, B.part1, blockIdx.x);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_bicg_start(BiCGArgs B, const double* __restrict__ b) {
+    double bb = 0.0;
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < B.n; i += int64_t(gridDim.x) * 256) {
+        const double v = b[i];
+        B.x[i] = 0.0; B.r[i] = v; B.r0[i] = v; B.p[i] = 0.0; B.v[i] = 0.0;
+        bb += v * v;
+    }
+    block_sum_store(bb, B.part0, blockIdx.x);
+}
+
+__global__ __launch_bounds__(256) void k_bicg_start_s(BiCGArgs B, double tol2, int max_it) {
+    const double bb = reduce_partials(B.part0, B.nvec);
+    if (threadIdx.x == 0) {
+        BiCGScalars* S = B.S;
+        S->bb = bb; S->rr = bb; S->rho_old = 1.0; S->alpha = 1.0; S->omega = 1.0;
+        S->iter = 0; S->max_it = max_it; S->tol2 = tol2;
+        S->done = (bb == 0.0) ? 1 : (isfinite(bb) ? 0 : 4);
+    }
+}
+
+// partial r0.r
+__global__ __launch_bounds__(256) void k_bicg_rho(BiCGArgs B) {
+    if (B.S->done) return;
+    double d = 0.0;
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < B.n; i += int64_t(gridDim.x) * 256)
+        d += B.r0[i] * B.r[i];
+    block_sum_store(d, B.part0, blockIdx.x);
+}
+
+// rho, beta; p = r + beta (p - omega v); ph = dinv p  (every block reduces the
+// partials itself: deterministic, one launch less)
+__global__ __launch_bounds__(256) void k_bicg_p(BiCGArgs B) {
+    if (B.S->done) return;
+    const double rho = reduce_partials(B.part0, B.nvec);
+    const double beta = (rho / B.S->rho_old) * (B.S->alpha / B.S->omega);
+    const double om = B.S->omega;
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < B.n; i += int64_t(gridDim.x) * 256) {
+        const double p = B.r[i] + beta * (B.p[i] - om * B.v[i]);
+        B.p[i] = p;
+        B.ph[i] = B.dinv[i] * p;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        B.S->beta = beta;
+        B.S->rho = rho;
+    }
+}
+
+// alpha = rho / r0.v
+__global__ __launch_bounds__(256) void k_bicg_alpha(BiCGArgs B) {
+    if (B.S->done) return;
+    const double r0v = reduce_partials(B.part0, B.nparts);
+    if (threadIdx.x == 0) {
+        B.S->alpha = B.S->rho / r0v;
+        if (!(r0v != 0.0) || !isfinite(r0v)) B.S->done = 3;
+    }
+}
+
+// s = r - alpha v ; sh = dinv s
+__global__ __launch_bounds__(256) void k_bicg_s(BiCGArgs B) {
+    if (B.S->done) return;
+    const double al = B.S->alpha;
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < B.n; i += int64_t(gridDim.x) * 256) {
+        const double sv = B.r[i] - al * B.v[i];
+        B.s[i] = sv;
+        B.sh[i] = B.dinv[i] * sv;
+    }
+}
+
+// omega = (t.s)/(t.t); x += alpha ph + omega sh; r = s - omega t; partial r.r
+__global__ __launch_bounds__(256) void k_bicg_xr(BiCGArgs B) {
+    if (B.S->done) return;
+    const double ts = reduce_partials(B.part0, B.nparts);
+    const double tt = reduce_partials(B.part1, B.nparts);
+    const double om = tt > 0.0 ? ts / tt : 0.0;
+    const double al = B.S->alpha;
+    double rr = 0.0;
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < B.n; i += int64_t(gridDim.x) * 256) {
+        B.x[i] += al * B.ph[i] + om * B.sh[i];
+        const double rv = B.s[i] - om * B.t[i];
+        B.r[i] = rv;
+        rr += rv * rv;
+    }
+    __syncthreads();
+    block_sum_store(rr, B.part0 + B.nparts, blockIdx.x);
+    if (blockIdx.x == 0 && threadIdx.x == 0) B.S->omega = om;
+}
+
+__global__ __launch_bounds__(256) void k_bicg_check(BiCGArgs B) {
+    if (B.S->done) return;
+    const double rr = reduce_partials(B.part0 + B.nparts, B.nvec);
+    if (threadIdx.x == 0) {
+        BiCGScalars* S = B.S;
+        S->rr = rr;
+        S->iter += 1;
+        S->rho_old = S->rho;
+        if (!isfinite(rr)) S->done = 4;
+        else if (rr <= S->tol2 * S->bb) S->done = 1;
+        else if (S->omega == 0.0) S->done = 3;
+        else if (S->iter >= S->max_it) S->done = 2;
+    }
+}
+
+__global__ void k_gather_values(int64_t n, const int32_t* __restrict__ idx, const double* __restrict__ src,
+                                double* __restrict__ dst) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[idx[i]];
+}
+
 // ---------------------------------------------------------- launchers ----
 static inline unsigned blocks_for(int64_t n, int per) { return unsigned((n + per - 1) / per); }
 
@@ -560,6 +1096,18 @@ void launch_hex8_K(const Hex8Args& a, hipStream_t s) {
 }
 void launch_hex8_sens(const Hex8Args& a, const double* lam, double* g, hipStream_t s) {
     hipLaunchKernelGGL(k_hex8_elastic_sens, dim3(blocks_for(a.n_elems, 128)), dim3(128), 0, s, a, lam, g);
+}
+void launch_quad4_K(const Quad4Args& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_quad4_mindlin_K, dim3(blocks_for(a.n_elems, 16)), dim3(256), 0, s, a);
+}
+void launch_quad4_sens(const Quad4Args& a, const double* lam, double* g, hipStream_t s) {
+    hipLaunchKernelGGL(k_quad4_mindlin_sens, dim3(blocks_for(a.n_elems, 128)), dim3(128), 0, s, a, lam, g);
+}
+void launch_heat_K(const HeatArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_hex8_heat_K, dim3(blocks_for(a.n_elems, 4)), dim3(256), 0, s, a);
+}
+void launch_heat_source_dot(const HeatArgs& a, const double* lam, double scale, double* g, hipStream_t s) {
+    hipLaunchKernelGGL(k_hex8_source_dot, dim3(blocks_for(a.n_elems, 128)), dim3(128), 0, s, a, lam, scale, g);
 }
 void launch_scatter(const ScatterArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_scatter_blocks, dim3(blocks_for(a.n_nodes, 4)), dim3(256), 0, s, a);
@@ -605,8 +1153,35 @@ void launch_spmv_csr(const CGArgs& c, const double* x, double* y, int want_dot, 
     spmv_dispatch(c, x, y, want_dot, 0, s);
 }
 void launch_cg_start(const CGArgs& c, const double* b, double tol2, int max_it, hipStream_t s) {
-    hipLaunchKernelGGL(k_cg_start, dim3(c.nparts), dim3(256), 0, s, c, b);
+    hipLaunchKernelGGL(k_cg_start, dim3(c.nvec), dim3(256), 0, s, c, b);
     hipLaunchKernelGGL(k_cg_start_scalars, dim3(1), dim3(256), 0, s, c, tol2, max_it);
+}
+static void spmv_gen(const BiCGArgs& B, const double* x, double* y, const double* w1, int want_yy,
+                     hipStream_t s) {
+    switch (B.spmv_sw) {
+        case 4: hipLaunchKernelGGL(k_spmv_csr_gen<4>, dim3(B.nparts), dim3(256), 0, s, B, x, y, w1, want_yy); break;
+        case 8: hipLaunchKernelGGL(k_spmv_csr_gen<8>, dim3(B.nparts), dim3(256), 0, s, B, x, y, w1, want_yy); break;
+        case 32: hipLaunchKernelGGL(k_spmv_csr_gen<32>, dim3(B.nparts), dim3(256), 0, s, B, x, y, w1, want_yy); break;
+        case 64: hipLaunchKernelGGL(k_spmv_csr_gen<64>, dim3(B.nparts), dim3(256), 0, s, B, x, y, w1, want_yy); break;
+        default: hipLaunchKernelGGL(k_spmv_csr_gen<16>, dim3(B.nparts), dim3(256), 0, s, B, x, y, w1, want_yy); break;
+    }
+}
+void launch_bicg_start(const BiCGArgs& a, const double* b, double tol2, int max_it, hipStream_t s) {
+    hipLaunchKernelGGL(k_bicg_start, dim3(a.nvec), dim3(256), 0, s, a, b);
+    hipLaunchKernelGGL(k_bicg_start_s, dim3(1), dim3(256), 0, s, a, tol2, max_it);
+}
+void launch_bicg_iteration(const BiCGArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_bicg_rho, dim3(a.nvec), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_bicg_p, dim3(a.nvec), dim3(256), 0, s, a);
+    spmv_gen(a, a.ph, a.v, a.r0, 0, s);
+    hipLaunchKernelGGL(k_bicg_alpha, dim3(1), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_bicg_s, dim3(a.nvec), dim3(256), 0, s, a);
+    spmv_gen(a, a.sh, a.t, a.s, 1, s);
+    hipLaunchKernelGGL(k_bicg_xr, dim3(a.nvec), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_bicg_check, dim3(1), dim3(256), 0, s, a);
+}
+void launch_gather_values(int64_t n, const int32_t* idx, const double* src, double* dst, hipStream_t s) {
+    hipLaunchKernelGGL(k_gather_values, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, idx, src, dst);
 }
 void launch_elem_scale(int64_t n, const double* rho, double E0, double penalty, double* sE,
                        hipStream_t s) {
@@ -638,9 +1213,9 @@ void launch_cg_iteration(const CGArgs& c, hipStream_t s) {
     else
         spmv_dispatch(c, c.p, c.q, 1, 1, s);
     hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(256), 0, s, c);
-    hipLaunchKernelGGL(k_cg_update, dim3(c.nparts), dim3(256), 0, s, c);
+    hipLaunchKernelGGL(k_cg_update, dim3(c.nvec), dim3(256), 0, s, c);
     hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(256), 0, s, c);
-    hipLaunchKernelGGL(k_cg_p, dim3(c.nparts), dim3(256), 0, s, c);
+    hipLaunchKernelGGL(k_cg_p, dim3(c.nvec), dim3(256), 0, s, c);
 }
 
 }  // namespace mastg

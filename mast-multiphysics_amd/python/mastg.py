@@ -16,6 +16,7 @@ HOST, DEVICE = 0, 1
 ELEM_FP64, ELEM_FP32 = 0, 1
 SOLVER_CG_JACOBI, SOLVER_MIXED, SOLVER_BICGSTAB, SOLVER_MATFREE = 0, 1, 2, 3
 OUTPUT_COMPLIANCE, OUTPUT_MEAN_T = 1, 2
+OPT_GRAPHS = 1
 ERRORS = {0: "ok", -1: "invalid", -2: "out of memory", -3: "not converged", -4: "nan",
           -5: "device error", -6: "admission", -7: "busy"}
 
@@ -92,6 +93,7 @@ def lib():
             "mastg_synchronize": (C.c_int, [_P]),
             "mastg_bench_kernel": (C.c_int, [_P, C.c_int, C.c_int, _D]),
             "mastg_get_tuning": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+            "mastg_set_option": (C.c_int, [_P, C.c_int, C.c_int]),
             "mastg_set_tuning": (C.c_int, [_P, C.c_int, C.c_int]),
             "mastg_workload_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64,
                                                 C.c_double, C.c_double, C.c_int, C.POINTER(_P)]),
@@ -276,6 +278,9 @@ class Context:
         ms = C.c_double()
         self._chk(lib().mastg_bench_kernel(self.h, kind, n_launch, C.byref(ms)))
         return ms.value
+
+    def set_option(self, option, value):
+        self._chk(lib().mastg_set_option(self.h, option, value))
 
     def tuning(self):
         a, b = C.c_int(), C.c_int()

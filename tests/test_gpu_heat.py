@@ -1,0 +1,70 @@
+"""GPU parity of the hex8 steady-conduction path against the FP64 oracle (C ABI).
+
+The element Jacobian includes the dk/dT term of k(T) = k0 (1 + alpha T)
+(heat_conduction_elem_base.cpp:124-142), so J is non-symmetric: the linear and
+adjoint solves use BiCGStab + Jacobi, the adjoint through the transposed values.
+Bars: CSR pattern bit-exact; R and J at a temperature state within 1e-12
+(J relative to max(|entry|, 1e-2 x row max)); Newton-converged mean temperature
+within 1e-9 and d(mean T)/dq_e within 1e-7 of the oracle's.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from helpers import ctx_from_oracle, rel_err_floor
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 6])
+def test_heat_csr_R_J_at_state(oracle, mastg, n):
+    print(f"[heat K] {n}^3", flush=True)
+    P = oracle.Problem(oracle.HEAT, n, seed=2)
+    ctx = ctx_from_oracle(mastg, P)
+    rp, col = ctx.csr_pattern()
+    orp, ocol = P.csr()
+    assert np.array_equal(rp, orp) and np.array_equal(col, ocol)
+    _, con = P.dofmap()
+    for T in (np.zeros(P.n_dofs), 300 + 50 * np.random.default_rng(n).standard_normal(P.n_dofs)):
+        T[con == 1] = 0.0
+        R0, K0 = P.assemble(T)
+        R, J = ctx.assemble(T)
+        assert rel_err_floor(J, K0, orp, floor=1e-2) <= 1e-12
+        np.testing.assert_allclose(R, R0, rtol=0, atol=1e-12 * np.abs(R0).max())
+    ctx.close()
+
+
+def test_heat_transpose_solve(oracle, mastg):
+    print("[heat A^T x = b] 5^3", flush=True)
+    P = oracle.Problem(oracle.HEAT, 5, seed=3)
+    ctx = ctx_from_oracle(mastg, P)
+    _, con = P.dofmap()
+    T = 200 + 100 * np.random.default_rng(1).random(P.n_dofs)
+    T[con == 1] = 0
+    R, J = ctx.assemble(T)
+    rp, col = ctx.csr_pattern()
+    A = sp.csr_matrix((J, col, rp), shape=(P.n_dofs, P.n_dofs))
+    b = np.random.default_rng(2).standard_normal(P.n_dofs)
+    b[con == 1] = 0
+    for transpose in (False, True):
+        x, st = ctx.solve(b, solver=mastg.SOLVER_BICGSTAB, rtol=1e-12, transpose=transpose)
+        M = A.T if transpose else A
+        res = np.linalg.norm(b - M @ x) / np.linalg.norm(b)
+        assert res <= 2e-12 and abs(res - st["rel_res"]) <= 1e-3 * res + 1e-15
+    ctx.close()
+
+
+@pytest.mark.parametrize("n", [4, 8])
+def test_heat_design_eval(oracle, mastg, n):
+    print(f"[heat Newton + adjoint] {n}^3", flush=True)
+    P = oracle.Problem(oracle.HEAT, n, seed=5)
+    ref = P.solve_heat(1e-12, 1e-14)
+    ctx = ctx_from_oracle(mastg, P)
+    r = ctx.design_eval(P.params(), rtol=1e-12)
+    assert 2 <= r["newton_iters"] <= 10
+    assert abs(r["objective"] - ref["mean_T"]) <= 1e-9 * abs(ref["mean_T"])
+    assert rel_err_floor(r["grad"], ref["dJ"], floor=1e-6) <= 1e-7
+    # mean temperature output entry point at the oracle's state
+    mt = ctx.output(mastg.OUTPUT_MEAN_T, ref["T"])
+    assert abs(mt - ref["mean_T"]) <= 1e-12 * abs(ref["mean_T"])
+    ctx.close()

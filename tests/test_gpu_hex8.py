@@ -184,3 +184,18 @@ def test_autotuner_choice_is_valid(oracle, mastg):
         r = ctx.design_eval(P.params())
         assert abs(r["objective"] - r0["objective"]) <= 1e-11 * r0["objective"]
     ctx.close()
+
+
+def test_direct_launch_path_matches_graphs(oracle, mastg):
+    print("[no graphs] 10x5x5", flush=True)
+    P = oracle.Problem(oracle.CANTILEVER, 10, 5, 5, seed=6, lo=0.01, hi=1.0)
+    ctx = ctx_from_oracle(mastg, P)
+    for sv in (mastg.SOLVER_CG_JACOBI, mastg.SOLVER_MATFREE):
+        ctx.set_option(mastg.OPT_GRAPHS, 1)
+        r1 = ctx.design_eval(P.params(), solver=sv)
+        ctx.set_option(mastg.OPT_GRAPHS, 0)
+        r2 = ctx.design_eval(P.params(), solver=sv)
+        assert r1["iters"] == r2["iters"]
+        assert r1["objective"] == r2["objective"]
+        assert np.array_equal(r1["grad"], r2["grad"])
+    ctx.close()
