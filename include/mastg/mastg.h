@@ -222,6 +222,43 @@ int mastg_get_tuning(const mastg_ctx* ctx, int* spmv_lanes, int* grid);
 int mastg_set_option(mastg_ctx* ctx, int option, int value);
 int mastg_set_tuning(mastg_ctx* ctx, int spmv_lanes, int grid);
 
+/* ---------------------------------------------------- request server --- */
+/* Continuous batching of design evaluations on one GPU (no counterpart in the
+ * reference, whose optimizer drives one evaluation at a time through
+ * FunctionEvaluation::evaluate, e.g. examples/structural/example_6/example_6.cpp:450-655).
+ * Requests name a structured workload (kind 1 hex8 cantilever nx x ny x nz,
+ * kind 2 Mindlin plate nx x ny) and carry their design parameters; n_slots
+ * requests run concurrently on separate HIP streams; per-mesh structure is
+ * cached and reused; admission is by estimated device footprint against
+ * hbm_budget (idle contexts of other meshes are evicted LRU first). */
+typedef struct mastg_server mastg_server;
+typedef struct {
+    int32_t n_slots;          /* concurrent requests on the device                 */
+    int32_t solver;           /* MASTG_SOLVER_* for hex8 requests (plates use CG)  */
+    int64_t hbm_budget;       /* device bytes the server may hold; 0 = 64 GiB      */
+    int32_t queue_capacity;   /* pending requests before submit returns BUSY       */
+    double rtol;              /* solve tolerance, 0 -> 1e-10                       */
+} mastg_server_opts;
+typedef struct {
+    int32_t kind, nx, ny, nz;
+    const double* params;     /* [n_elems], copied at submit                       */
+    double* grad_out;         /* [n_elems] written when done; caller keeps it alive */
+} mastg_request;
+typedef struct {
+    int32_t status;           /* 0 queued, 1 running, 2 done, < 0 MASTG_ERR_*      */
+    int32_t iters;
+    double objective;
+    double t_queue, t_run;    /* seconds waiting / running                          */
+} mastg_request_result;
+int mastg_server_create(const mastg_server_opts* opts, mastg_server** out);
+int mastg_server_submit(mastg_server* s, const mastg_request* req, int64_t* id);
+/* non-blocking; a done / failed result is returned once and then retired */
+int mastg_server_poll(mastg_server* s, int64_t id, mastg_request_result* res);
+int mastg_server_wait(mastg_server* s, int64_t id, mastg_request_result* res);
+int mastg_server_stats(mastg_server* s, int64_t* bytes_in_use, int32_t* n_contexts, int32_t* queued);
+int mastg_server_destroy(mastg_server* s);   /* drains the queue first */
+int mastg_request_footprint(const mastg_request* req, int64_t* bytes);
+
 /* ------------------------------------------------ benchmark workloads -- */
 /* Structured problems of the benchmark configurations, built the way libMesh
  * MeshTools::Generation::build_cube / build_square number meshes (what MAST's

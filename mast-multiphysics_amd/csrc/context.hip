@@ -1135,14 +1135,16 @@ int mastg_bench_kernel(mastg_ctx* c, int kind, int n_launch, double* avg_ms) {
     if (!c || !avg_ms || n_launch <= 0) return MASTG_ERR_INVALID;
     if (!c->assembled) return fail(c, MASTG_ERR_INVALID, "assemble first");
     CGArgs a = cg_args(c);
-    Hex8Args h = hex8_args(c, nullptr, true);
     ScatterArgs sa{c->S.n_nodes, c->S.nav, c->blk_ptr.as<int64_t>(), c->cptr.as<int32_t>(),
                    c->cidx.as<int32_t>(), c->Kel.as<double>(), c->val.as<double>()};
     // the CG scalars must not read "done" for the replayed SpMV
     HIPC(c, hipMemsetAsync(c->scal.p, 0, sizeof(CGScalars), c->stream));
     auto launch = [&]() {
         switch (kind) {
-            case 0: launch_elements(c, nullptr, true); break;
+            case 0:
+                if (c->d.physics == MASTG_HEX8_HEAT) launch_elements(c, c->vT.as<double>(), true);
+                else launch_elements(c, nullptr, true);
+                break;
             case 1: launch_scatter(sa, c->stream); break;
             case 2: launch_spmv_csr(a, a.p, a.q, 1, c->stream); break;
             case 3:

@@ -54,6 +54,21 @@ class SolveStats(C.Structure):
                 ("seconds", C.c_double)]
 
 
+class ServerOpts(C.Structure):
+    _fields_ = [("n_slots", C.c_int32), ("solver", C.c_int32), ("hbm_budget", C.c_int64),
+                ("queue_capacity", C.c_int32), ("rtol", C.c_double)]
+
+
+class Request(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+                ("params", C.POINTER(C.c_double)), ("grad_out", C.POINTER(C.c_double))]
+
+
+class RequestResult(C.Structure):
+    _fields_ = [("status", C.c_int32), ("iters", C.c_int32), ("objective", C.c_double),
+                ("t_queue", C.c_double), ("t_run", C.c_double)]
+
+
 class DesignResult(C.Structure):
     _fields_ = [("objective", C.c_double), ("iters", C.c_int32), ("newton_iters", C.c_int32),
                 ("rel_res", C.c_double), ("t_assemble", C.c_double), ("t_solve", C.c_double),
@@ -95,6 +110,14 @@ def lib():
             "mastg_get_tuning": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
             "mastg_set_option": (C.c_int, [_P, C.c_int, C.c_int]),
             "mastg_set_tuning": (C.c_int, [_P, C.c_int, C.c_int]),
+            "mastg_server_create": (C.c_int, [C.POINTER(ServerOpts), C.POINTER(_P)]),
+            "mastg_server_submit": (C.c_int, [_P, C.POINTER(Request), C.POINTER(C.c_int64)]),
+            "mastg_server_poll": (C.c_int, [_P, C.c_int64, C.POINTER(RequestResult)]),
+            "mastg_server_wait": (C.c_int, [_P, C.c_int64, C.POINTER(RequestResult)]),
+            "mastg_server_stats": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
+                                              C.POINTER(C.c_int32)]),
+            "mastg_server_destroy": (C.c_int, [_P]),
+            "mastg_request_footprint": (C.c_int, [C.POINTER(Request), C.POINTER(C.c_int64)]),
             "mastg_workload_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64,
                                                 C.c_double, C.c_double, C.c_int, C.POINTER(_P)]),
             "mastg_workload_desc": (C.POINTER(ProblemDesc), [_P]),
@@ -295,3 +318,69 @@ class Context:
         n = C.c_int64()
         self._chk(lib().mastg_kernel_time(self.h, kind, C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+
+class Server:
+    """Request server (continuous batching of design evaluations)."""
+
+    def __init__(self, n_slots=4, solver=SOLVER_MATFREE, hbm_budget=0, queue_capacity=0, rtol=1e-10):
+        h = _P()
+        o = ServerOpts(n_slots, solver, hbm_budget, queue_capacity, rtol)
+        rc = lib().mastg_server_create(C.byref(o), C.byref(h))
+        if rc:
+            raise MastgError(rc, "server_create")
+        self.h = h
+        self._keep = {}
+
+    def submit(self, kind, nx, ny, nz, params):
+        params = np.ascontiguousarray(params, np.float64)
+        grad = np.zeros(params.size)
+        r = Request(kind, nx, ny, nz, _p(params), _p(grad))
+        rid = C.c_int64()
+        rc = lib().mastg_server_submit(self.h, C.byref(r), C.byref(rid))
+        if rc:
+            raise MastgError(rc, "server_submit")
+        self._keep[rid.value] = (params, grad)
+        return rid.value
+
+    def _result(self, rid, res):
+        params, grad = self._keep.pop(rid)
+        return dict(status=res.status, objective=res.objective, iters=res.iters,
+                    t_queue=res.t_queue, t_run=res.t_run, grad=grad)
+
+    def poll(self, rid):
+        res = RequestResult()
+        rc = lib().mastg_server_poll(self.h, rid, C.byref(res))
+        if rc:
+            raise MastgError(rc, "server_poll")
+        if res.status == 2 or res.status < 0:
+            return self._result(rid, res)
+        return None
+
+    def wait(self, rid):
+        res = RequestResult()
+        rc = lib().mastg_server_wait(self.h, rid, C.byref(res))
+        if rc:
+            raise MastgError(rc, "server_wait")
+        return self._result(rid, res)
+
+    def stats(self):
+        b, n, q = C.c_int64(), C.c_int32(), C.c_int32()
+        lib().mastg_server_stats(self.h, C.byref(b), C.byref(n), C.byref(q))
+        return dict(bytes_in_use=b.value, contexts=n.value, queued=q.value)
+
+    def close(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.mastg_server_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+
+def request_footprint(kind, nx, ny, nz):
+    r = Request(kind, nx, ny, nz, None, None)
+    b = C.c_int64()
+    rc = lib().mastg_request_footprint(C.byref(r), C.byref(b))
+    if rc:
+        raise MastgError(rc, "request_footprint")
+    return b.value
