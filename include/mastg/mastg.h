@@ -121,7 +121,9 @@ int mastg_get_active_map(const mastg_ctx* ctx, int64_t* active_of_dof);
  * (examples/structural/example_6/example_6.cpp:100-137). */
 int mastg_set_params(mastg_ctx* ctx, const double* params, int where);
 
-/* element precision for the Jacobian build */
+/* element precision for the Jacobian build: FP64 everywhere, or (hex8
+ * elasticity) FP32 element matrices on the matrix cores, summed into the FP64
+ * global matrix; entries then carry ~1e-7 relative rounding */
 #define MASTG_ELEM_FP64 0
 #define MASTG_ELEM_FP32 1
 
@@ -211,7 +213,8 @@ int mastg_kernel_time(const mastg_ctx* ctx, int kind, double* total_ms, int64_t*
 int mastg_enable_kernel_timing(mastg_ctx* ctx, int on);
 /* Replays one kernel of the resident state n_launch times back to back on the
  * context's stream and returns the average duration from HIP events (same
- * kinds as mastg_kernel_time).  For roofline measurement. */
+ * kinds as mastg_kernel_time, plus 5 = the FP32 matrix-core element kernel,
+ * hex8 only, into a buffer of its own).  For roofline measurement. */
 int mastg_bench_kernel(mastg_ctx* ctx, int kind, int n_launch, double* avg_ms);
 /* Launch shapes: CSR SpMV lanes per row and grid size (work-groups of the
  * SpMV / CG vector kernels).  Autotuned on the first solve unless set. */

@@ -59,7 +59,7 @@ struct mastg_ctx {
     int bicg_exec_len = 0;
     bool bicg_exec_T = false;
     // device data
-    DevBuf param, Kel, Rel, val, dinv, F;
+    DevBuf param, Kel, Kel32, Rel, val, dinv, F;
     DevBuf vx, vr, vz, vp, vq, vb, vtmp, vcaller, vg, part0, part1, scal, dscalar;
     CGScalars* h_scal = nullptr;      // pinned mirror of the CG scalars
     double* h_pin = nullptr;          // pinned scratch for scalar read-back
@@ -289,8 +289,9 @@ int ensure_vectors(mastg_ctx* c) {
     const size_t nb = size_t(c->S.n_active) * sizeof(double);
     for (DevBuf* b : {&c->vx, &c->vr, &c->vz, &c->vp, &c->vq, &c->vb, &c->vtmp}) TRY(dalloc(c, *b, nb));
     // partial sums: enough for the largest grid any kernel writes them from
-    // (autotuned SpMV grid up to 8192, matrix-free node phase n_nodes/256)
-    const size_t np = size_t(std::max<int64_t>(8192, mf_node_blocks(c->S.n_nodes)));
+    // (autotuned SpMV grid up to 8192, matrix-free node phase n_nodes/256;
+    // BiCGStab keeps the SpMV partials and a vector grid of up to 2048 after them)
+    const size_t np = size_t(std::max<int64_t>(8192 + 2048, mf_node_blocks(c->S.n_nodes)));
     TRY(dalloc(c, c->part0, np * sizeof(double)));
     TRY(dalloc(c, c->part1, np * sizeof(double)));
     TRY(dalloc(c, c->scal, sizeof(CGScalars)));
@@ -309,11 +310,13 @@ int dot_active(mastg_ctx* c, const double* a, const double* b, double* out) {
 }
 
 // ---- assembly: element kernel -> segmented scatter -> Jacobi diagonal
-int assemble_active(mastg_ctx* c, const double* X_active, bool want_R, bool want_K) {
+int assemble_active(mastg_ctx* c, const double* X_active, bool want_R, bool want_K, bool fp32 = false) {
     if (!c->params_set) return fail(c, MASTG_ERR_INVALID, "design parameters not set");
     const Structure& S = c->S;
     if (want_K) {
-        TRY(dalloc(c, c->Kel, size_t(S.n_elems) * S.npe * S.npe * S.nav * S.nav * sizeof(double)));
+        const size_t nkel = size_t(S.n_elems) * S.npe * S.npe * S.nav * S.nav;
+        if (fp32) TRY(dalloc(c, c->Kel32, nkel * sizeof(float)));
+        else TRY(dalloc(c, c->Kel, nkel * sizeof(double)));
         TRY(dalloc(c, c->val, size_t(S.nnz) * sizeof(double)));
         TRY(dalloc(c, c->dinv, size_t(S.n_active) * sizeof(double)));
     }
@@ -324,6 +327,11 @@ int assemble_active(mastg_ctx* c, const double* X_active, bool want_R, bool want
         if (!X_active) launch_fill(S.n_active, 0.0, c->vT.as<double>(), c->stream);
         TimedRegion t(c, 0);
         launch_heat_K(heat_args(c, X_active ? X_active : c->vT.as<double>(), want_R, want_K), c->stream);
+    } else if (fp32) {
+        // FP32 element blocks on MFMA (hex8 only), FP64 residual when a state is given
+        TimedRegion t(c, 0);
+        launch_hex8_K_f32(hex8_args(c, nullptr, false), c->Kel32.as<float>(), c->stream);
+        if (want_R && X_active) launch_elements(c, X_active, false);
     } else {
         TimedRegion t(c, 0);
         if (want_K || (want_R && X_active)) launch_elements(c, want_R ? X_active : nullptr, want_K);
@@ -333,7 +341,8 @@ int assemble_active(mastg_ctx* c, const double* X_active, bool want_R, bool want
                        c->cidx.as<int32_t>(), c->Kel.as<double>(), c->val.as<double>()};
         {
             TimedRegion t(c, 1);
-            launch_scatter(sa, c->stream);
+            if (fp32) launch_scatter_f32(sa, c->Kel32.as<float>(), c->stream);
+            else launch_scatter(sa, c->stream);
         }
         launch_jacobi_inv(S.n_active, c->diag_pos.as<int64_t>(), c->val.as<double>(), c->dinv.as<double>(),
                           c->stream);
@@ -915,11 +924,14 @@ int mastg_set_params(mastg_ctx* c, const double* p, int where) {
 
 int mastg_assemble(mastg_ctx* c, const double* X, double* R, double* J, int where, int prec) {
     if (!c) return MASTG_ERR_INVALID;
-    if (prec != MASTG_ELEM_FP64) return fail(c, MASTG_ERR_INVALID, "only FP64 element precision in this build");
+    if (prec != MASTG_ELEM_FP64 && prec != MASTG_ELEM_FP32)
+        return fail(c, MASTG_ERR_INVALID, "unknown element precision");
+    if (prec == MASTG_ELEM_FP32 && c->d.physics != MASTG_HEX8_ELASTIC)
+        return fail(c, MASTG_ERR_INVALID, "FP32 element matrices are built for hex8 elasticity only");
     const Structure& S = c->S;
     double* Xa = c->vb.as<double>();
     TRY(to_active(c, X, where, Xa));
-    TRY(assemble_active(c, X ? Xa : nullptr, R != nullptr, J != nullptr || true));
+    TRY(assemble_active(c, X ? Xa : nullptr, R != nullptr, true, prec == MASTG_ELEM_FP32));
     if (R) TRY(from_active(c, c->vtmp.as<double>(), R, where));
     if (J) {
         HIPC(c, hipMemcpyAsync(J, c->val.p, S.nnz * sizeof(double),
@@ -1156,10 +1168,16 @@ int mastg_bench_kernel(mastg_ctx* c, int kind, int n_launch, double* avg_ms) {
                 launch_mf_apply(a, a.p, a.p, a.q, 1, c->stream);
                 break;
             case 4: launch_sens(c, a.x, a.x, c->vg.as<double>()); break;
+            case 5: launch_hex8_K_f32(hex8_args(c, nullptr, false), c->Kel32.as<float>(), c->stream); break;
             default: break;
         }
     };
-    if (kind < 0 || kind > 4) return fail(c, MASTG_ERR_INVALID, "unknown kernel kind");
+    if (kind < 0 || kind > 5) return fail(c, MASTG_ERR_INVALID, "unknown kernel kind");
+    if (kind == 5) {
+        if (c->d.physics != MASTG_HEX8_ELASTIC) return fail(c, MASTG_ERR_INVALID, "FP32 element kernel is hex8 only");
+        TRY(dalloc(c, c->Kel32, size_t(c->S.n_elems) * 64 * 9 * sizeof(float)));
+    }
+    if (kind == 1 && !c->Kel.p) return fail(c, MASTG_ERR_INVALID, "no FP64 element blocks resident");
     if (kind == 3 && (c->S.n_classes == 0 || c->d.physics != MASTG_HEX8_ELASTIC))
         return fail(c, MASTG_ERR_INVALID, "no matrix-free operator");
     launch();   // warm

@@ -117,6 +117,8 @@ void launch_gather_values(int64_t n, const int32_t* idx, const double* src, doub
 
 void launch_hex8_K(const Hex8Args& a, hipStream_t s);
 void launch_hex8_sens(const Hex8Args& a, const double* lam, double* g, hipStream_t s);
+void launch_hex8_K_f32(const Hex8Args& a, float* Kel32, hipStream_t s);
+void launch_scatter_f32(const ScatterArgs& a, const float* Kel32, hipStream_t s);
 void launch_scatter(const ScatterArgs& a, hipStream_t s);
 void launch_quad4_K(const Quad4Args& a, hipStream_t s);
 void launch_heat_K(const HeatArgs& a, hipStream_t s);

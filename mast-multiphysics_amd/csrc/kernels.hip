@@ -161,6 +161,122 @@ __global__ __launch_bounds__(256) void k_hex8_elastic_K(Hex8Args A) {
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// hex8 K_e in FP32 on MFMA (v_mfma_f32_16x16x4f32, FP32 accumulation):
+//   K_e = G^T H,  G[s][c] = B_q[row][c],  H[s][c] = JxW_q (D B_q)[row][c],
+//   s = 6 q + row (K = 48), c = v*8 + a (24 columns, padded to 32: 4 tiles x 12 MFMAs)
+// Lane layout of the 16x16x4 instruction: A[i][k] from lane i + 16k, B[k][j]
+// from lane j + 16k, D[4*(l/16)+r][l%16] in accumulator r.  Geometry in FP64,
+// operands rounded to FP32.  Output: FP32 node-pair blocks, constrained,
+// summed into the FP64 CSR by the scatter.  Entries carry ~1e-7 relative
+// rounding; the rounded matrix no longer maps rigid-body motions exactly to
+// zero, which moves compliance by 1e-4..1e-3 on ill-conditioned SIMP designs,
+// so this path serves Jacobian exports and preconditioners, not the FP64 solve.
+// ---------------------------------------------------------------------------
+typedef float mastg_floatx4 __attribute__((ext_vector_type(4)));
+
+// strain row -> component of grad N feeding column variable v (or -1)
+__constant__ int8_t kBcomp[6][3] = {{0, -1, -1}, {-1, 1, -1}, {-1, -1, 2},
+                                    {1, 0, -1}, {-1, 2, 1}, {2, -1, 0}};
+// column variable v -> its three strain rows and the gradient components
+__constant__ int8_t kBrows[3][3] = {{0, 3, 5}, {1, 3, 4}, {2, 4, 5}};
+__constant__ int8_t kBdirs[3][3] = {{0, 1, 2}, {1, 0, 2}, {2, 1, 0}};
+
+__global__ __launch_bounds__(256) void k_hex8_elastic_K_f32(Hex8Args A, float* __restrict__ Kel32) {
+    __shared__ float sg[4][8][8][3];   // [wave][q][node][d]
+    __shared__ float sc[4][8];
+    __shared__ float sD[36];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t e = int64_t(blockIdx.x) * 4 + wave;
+    const bool valid = e < A.n_elems;
+    const int64_t ee = valid ? e : 0;
+    if (threadIdx.x < 36) sD[threadIdx.x] = float(A.Dhat[threadIdx.x]);
+    {
+        const int q = lane >> 3, n = lane & 7;
+        const int32_t rn = A.econn[ee * 8 + n];
+        const double x = A.xyz[int64_t(rn) * 3 + 0], y = A.xyz[int64_t(rn) * 3 + 1],
+                     z = A.xyz[int64_t(rn) * 3 + 2];
+        double dxi, deta, dzeta, N;
+        hex8_dshape(n, gp_coord(q & 1), gp_coord((q >> 1) & 1), gp_coord(q >> 2), dxi, deta, dzeta, N);
+        double J[9] = {x * dxi, x * deta, x * dzeta, y * dxi, y * deta, y * dzeta,
+                       z * dxi, z * deta, z * dzeta};
+#pragma unroll
+        for (int m = 1; m < 8; m <<= 1)
+#pragma unroll
+            for (int i = 0; i < 9; i++) J[i] += __shfl_xor(J[i], m, 64);
+        double ixi[3], ieta[3], izeta[3];
+        const double jac = inv3(J, ixi, ieta, izeta);
+        sg[wave][q][n][0] = float(dxi * ixi[0] + deta * ieta[0] + dzeta * izeta[0]);
+        sg[wave][q][n][1] = float(dxi * ixi[1] + deta * ieta[1] + dzeta * izeta[1]);
+        sg[wave][q][n][2] = float(dxi * ixi[2] + deta * ieta[2] + dzeta * izeta[2]);
+        if (n == 0) sc[wave][q] = float(jac);
+    }
+    __syncthreads();
+    if (!valid) return;
+    const int li = lane & 15, lk = lane >> 4;
+    const double Ee = A.E0 * pow(A.param[e], A.penalty);
+#pragma unroll
+    for (int tile = 0; tile < 4; tile++) {
+        const int I = tile >> 1, Jt = tile & 1;
+        const int ca = 16 * I + li, cb = 16 * Jt + li;     // this lane's A column / B column
+        mastg_floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll 4
+        for (int ks = 0; ks < 12; ks++) {
+            const int sidx = 4 * ks + lk;
+            const int q = sidx / 6, row = sidx - 6 * q;
+            float av = 0.f, bv = 0.f;
+            if (ca < 24) {
+                const int d = kBcomp[row][ca >> 3];
+                if (d >= 0) av = sg[wave][q][ca & 7][d];
+            }
+            if (cb < 24) {
+                const int v = cb >> 3, b = cb & 7;
+                float h = 0.f;
+#pragma unroll
+                for (int t = 0; t < 3; t++) h += sD[row * 6 + kBrows[v][t]] * sg[wave][q][b][kBdirs[v][t]];
+                bv = sc[wave][q] * h;
+            }
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int i = 16 * I + 4 * lk + r, j = 16 * Jt + li;
+            if (i < 24 && j < 24) {
+                const int a = i & 7, v = i >> 3, b = j & 7, w = j >> 3;
+                const int32_t ra = A.econn[e * 8 + a], rb = A.econn[e * 8 + b];
+                float kv = float(Ee * double(acc[r]));
+                if (A.con && (A.con[int64_t(ra) * 3 + v] || A.con[int64_t(rb) * 3 + w]))
+                    kv = (a == b && v == w) ? 1.0f : 0.0f;
+                Kel32[(e * 64 + a * 8 + b) * 9 + v * 3 + w] = kv;
+            }
+        }
+    }
+}
+
+// segmented reduction of FP32 element blocks into the FP64 CSR values
+__global__ __launch_bounds__(256) void k_scatter_blocks_f32(ScatterArgs S, const float* __restrict__ Kel32) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t r = int64_t(blockIdx.x) * 4 + wave;
+    if (r >= S.n_nodes) return;
+    const int nav = S.nav, nav2 = nav * nav;
+    const int64_t k0 = S.blk_ptr[r];
+    const int64_t deg = S.blk_ptr[r + 1] - k0;
+    const int64_t cnt = deg * nav2;
+    const int64_t base = k0 * nav2;
+    for (int64_t t = lane; t < cnt; t += 64) {
+        const int64_t v = t / (deg * nav);
+        const int64_t rem = t - v * deg * nav;
+        const int64_t k = rem / nav;
+        const int64_t w = rem - k * nav;
+        const int64_t blk = k0 + k;
+        double s = 0.0;
+        for (int32_t c = S.cptr[blk]; c < S.cptr[blk + 1]; c++)
+            s += double(Kel32[int64_t(S.cidx[c]) * nav2 + v * nav + w]);
+        S.val[base + t] = s;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // hex8 sensitivity dot: g_e = lambda_e . dR_e/drho_e with dR_e/drho_e =
 // sum_qp JxW B^T D' B u_e, D' = dE/drho * D(E=1)
@@ -1093,6 +1209,12 @@ static inline unsigned blocks_for(int64_t n, int per) { return unsigned((n + per
 
 void launch_hex8_K(const Hex8Args& a, hipStream_t s) {
     hipLaunchKernelGGL(k_hex8_elastic_K, dim3(blocks_for(a.n_elems, 4)), dim3(256), 0, s, a);
+}
+void launch_hex8_K_f32(const Hex8Args& a, float* Kel32, hipStream_t s) {
+    hipLaunchKernelGGL(k_hex8_elastic_K_f32, dim3(blocks_for(a.n_elems, 4)), dim3(256), 0, s, a, Kel32);
+}
+void launch_scatter_f32(const ScatterArgs& a, const float* Kel32, hipStream_t s) {
+    hipLaunchKernelGGL(k_scatter_blocks_f32, dim3(blocks_for(a.n_nodes, 4)), dim3(256), 0, s, a, Kel32);
 }
 void launch_hex8_sens(const Hex8Args& a, const double* lam, double* g, hipStream_t s) {
     hipLaunchKernelGGL(k_hex8_elastic_sens, dim3(blocks_for(a.n_elems, 128)), dim3(128), 0, s, a, lam, g);

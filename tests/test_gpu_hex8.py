@@ -199,3 +199,73 @@ def test_direct_launch_path_matches_graphs(oracle, mastg):
         assert r1["objective"] == r2["objective"]
         assert np.array_equal(r1["grad"], r2["grad"])
     ctx.close()
+
+
+def distorted_cantilever(oracle, mesh, seed):
+    """Oracle cantilever mesh with interior nodes moved by up to 15% of the
+    element size (general trilinear geometry), 3 variables per node numbered
+    node-major, no constraints; and the dense FP64 oracle matrix over it."""
+    P = oracle.Problem(oracle.CANTILEVER, *mesh, seed=seed, lo=0.01, hi=1.0)
+    xyz, conn = P.mesh()
+    xyz = xyz.copy()
+    h = np.array([1.0 / mesh[0], 0.5 / mesh[1], 0.5 / mesh[2]])
+    lo, hi = xyz.min(0), xyz.max(0)
+    inner = np.all((xyz > lo + 1e-9) & (xyz < hi - 1e-9), axis=1)
+    rng = np.random.default_rng(seed)
+    xyz[inner] += rng.uniform(-0.15, 0.15, (inner.sum(), 3)) * h
+    rho = P.params()
+    n = xyz.shape[0]
+    K = np.zeros((3 * n, 3 * n))
+    for e in range(conn.shape[0]):
+        _, Ke = oracle.hex8_elastic(xyz[conn[e]], 72e9 * rho[e] ** 3, 0.33)
+        dofs = np.array([3 * conn[e][a] + v for v in range(3) for a in range(8)])
+        K[np.ix_(dofs, dofs)] += Ke[:24, :24]
+    return xyz, conn, rho, K
+
+
+@pytest.mark.parametrize("mesh", [(1, 1, 1), (3, 2, 2), (8, 4, 4)])
+def test_fp32_matrix_core_elements(oracle, mastg, mesh):
+    """FP32 element matrices on the matrix cores (v_mfma_f32_16x16x4f32),
+    summed in FP64, against the FP64 oracle on distorted elements.  Bars: FP64
+    path 1e-12 as above; FP32 path every entry within 1e-5 of its row's largest
+    entry (operands rounded to FP32, 48-term FP32 dot products) and a median
+    error below 1e-6 of it."""
+    print(f"[hex8 FP32 MFMA K] mesh={mesh}", flush=True)
+    xyz, conn, rho, K0 = distorted_cantilever(oracle, mesh, seed=7)
+    n = xyz.shape[0]
+    desc = mastg.desc_from_arrays(mastg.HEX8_ELASTIC, xyz, conn, 3, node_dof_base=3 * np.arange(n),
+                                  E=72e9, nu=0.33, penalty=3.0)
+    ctx = mastg.Context(desc)
+    ctx.set_params(rho)
+    rp, col = ctx.csr_pattern()
+    rows = np.repeat(np.arange(3 * n), np.diff(rp))
+    ref = K0[rows, col]
+    _, J64 = ctx.assemble(want_R=False)
+    assert rel_err_floor(J64, ref, rp, floor=1e-2) <= 1e-12
+    _, J32 = ctx.assemble(want_R=False, elem_precision=mastg.ELEM_FP32)
+    rowmax = np.maximum.reduceat(np.abs(ref), rp[:-1])[rows]
+    err = np.abs(J32 - ref) / rowmax
+    assert err.max() <= 1e-5, err.max()
+    assert np.median(err) <= 1e-6
+    ctx.close()
+
+
+def test_fp32_jacobian_solve_and_limits(oracle, mastg):
+    """A solve on the FP32-element matrix: compliance moves by the rounding of
+    the element matrices (documented 1e-4..1e-3 on SIMP designs, bar 1e-2); the
+    FP32 path is refused for physics without it."""
+    print("[hex8 FP32 solve] 12x6x6", flush=True)
+    P = oracle.Problem(oracle.CANTILEVER, 12, 6, 6, seed=2, lo=0.01, hi=1.0)
+    ref = P.solve_elastic(1e-10)
+    ctx = ctx_from_oracle(mastg, P)
+    R, _ = ctx.assemble(elem_precision=mastg.ELEM_FP32)
+    x, st = ctx.solve(-R, rtol=1e-10)
+    C32 = ctx.output(mastg.OUTPUT_COMPLIANCE, x)
+    assert abs(C32 - ref["compliance"]) <= 1e-2 * ref["compliance"]
+    ctx.close()
+    Pp = oracle.Problem(oracle.PLATE, 8, 8, seed=1, lo=0.002, hi=0.01)
+    cp = ctx_from_oracle(mastg, Pp)
+    with pytest.raises(mastg.MastgError) as e:
+        cp.assemble(elem_precision=mastg.ELEM_FP32)
+    assert e.value.code == -1
+    cp.close()
