@@ -158,7 +158,8 @@ def main():
     ap.add_argument("--c5-requests", type=int, default=256)
     ap.add_argument("--c5-rate", type=float, default=20.0)
     ap.add_argument("--c5-slots", type=int, default=6)
-    ap.add_argument("--solver", default="both", choices=["plain", "matfree", "both"])
+    ap.add_argument("--solver", default="all", choices=["plain", "mixed", "matfree", "both", "all"],
+                    help="both = plain + matfree; all = plain + mixed + matfree")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-graphs", action="store_true",
@@ -181,8 +182,9 @@ def main():
     params = w.params
     print(f"# workload {a.workload}: {w.n_elems} elems, {ctx.n_active} active DOF, nnz {ctx.nnz}, "
           f"setup {t_setup:.1f}s", file=sys.stderr, flush=True)
-    solvers = {"plain": mastg.SOLVER_CG_JACOBI, "matfree": mastg.SOLVER_MATFREE}
-    order = ["plain", "matfree"] if a.solver == "both" else [a.solver]
+    solvers = {"plain": mastg.SOLVER_CG_JACOBI, "mixed": mastg.SOLVER_MIXED,
+               "matfree": mastg.SOLVER_MATFREE}
+    order = {"both": ["plain", "matfree"], "all": ["plain", "mixed", "matfree"]}.get(a.solver, [a.solver])
     runs = {}
     for name in order:
         sv = solvers[name]
@@ -204,7 +206,7 @@ def main():
         ctx._chk(mastg.lib().mastg_synchronize(ctx.h))
         elapsed = time.perf_counter() - t0
         runs[name] = dict(elapsed=elapsed, lat=lat, res=res, dps=a.steps / elapsed)
-    head = "matfree" if "matfree" in runs else "plain"
+    head = "matfree" if "matfree" in runs else ("mixed" if "mixed" in runs else "plain")
     H = runs[head]
     res = H["res"]
     lat = H["lat"]
@@ -215,7 +217,7 @@ def main():
     lanes, grid = ctx.tuning()
 
     # roofline of the dominant kernel of the headline path
-    if head == "plain":
+    if head != "matfree":
         k_ms = ctx.bench_kernel(2, 20)
         byt = spmv_bytes(ctx.n_active, ctx.nnz)
         kname = f"k_spmv_csr_dot<{lanes}> (FP64 CSR SpMV)"
@@ -256,6 +258,16 @@ def main():
         out[f"{name}_designs_per_s"] = R["dps"]
         out[f"{name}_cg_iters"] = R["res"][-1]["iters"]
         out[f"{name}_compliance"] = R["res"][-1]["objective"]
+        if name != "plain" and "plain" in runs:
+            out[f"{name}_speedup_vs_plain_fp64"] = R["dps"] / runs["plain"]["dps"]
+    # element kernels against their arithmetic peaks (dense B^T D B count per element)
+    ms64 = ctx.bench_kernel(0, 10)
+    ms32 = ctx.bench_kernel(5, 10)
+    fl = 69120.0 * w.n_elems
+    out["element_kernels"] = {
+        "fp64_valu": {"avg_launch_ms": ms64, "tflops": fl / (ms64 * 1e-3) / 1e12, "peak_tflops": 78.6},
+        "fp32_mfma": {"avg_launch_ms": ms32, "tflops": fl / (ms32 * 1e-3) / 1e12, "peak_tflops": 157.3},
+        "flop_per_element": 69120}
     if "plain" in runs and "matfree" in runs:
         out["speedup_vs_plain_fp64"] = runs["matfree"]["dps"] / runs["plain"]["dps"]
         out["compliance_rel_diff_paths"] = abs(runs["matfree"]["res"][-1]["objective"] -

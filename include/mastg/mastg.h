@@ -166,7 +166,8 @@ int mastg_solve(mastg_ctx* ctx, const double* rhs, double* x, int where,
                 const mastg_solve_opts* opts, mastg_solve_stats* stats);
 
 /* J dX with the Jacobian of the last mastg_assemble, op 0 = assembled CSR,
- * op 1 = matrix-free operator (same matrix).  Replaces
+ * op 1 = matrix-free operator (same matrix), op 2 = the FP32-rounded values
+ * read by node blocks (the mixed-precision solver's operator).  Replaces
  * NonlinearImplicitAssembly::linearized_jacobian_solution_product
  * (src/base/nonlinear_implicit_assembly.cpp:271-383) for the linear elements. */
 int mastg_jacobian_product(mastg_ctx* ctx, const double* dX, double* JdX, int where, int op);

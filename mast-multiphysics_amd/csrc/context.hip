@@ -68,8 +68,11 @@ struct mastg_ctx {
     bool assembled = false;
     bool params_set = false;
     // captured CG iterations, one graph per operator (CSR, matrix-free)
-    hipGraphExec_t cg_exec[2] = {nullptr, nullptr};
-    int cg_exec_len[2] = {0, 0};
+    hipGraphExec_t cg_exec[3] = {nullptr, nullptr, nullptr};   // CSR, matrix-free, FP32 node-block
+    int cg_exec_len[3] = {0, 0, 0};
+    DevBuf val32;              // FP32 copy of val for the mixed-precision solver
+    bool val32_valid = false;
+    int cg_replacements = 0;
     // launch shapes (autotuned on the first solve)
     int spmv_sw = 16;
     bool use_graphs = true;
@@ -233,6 +236,10 @@ CGArgs cg_args(mastg_ctx* c) {
     a.mf.Fel = c->Fel.as<double>();
     a.nparts_op = c->nparts;
     a.nvec = std::min<int64_t>(2048, std::max<int64_t>(1, (c->S.n_active + 255) / 256));
+    a.blk_ptr = c->blk_ptr.as<int64_t>();
+    a.blk_col = c->blk_col.as<int32_t>();
+    a.val32 = c->val32.as<float>();
+    a.nav = c->S.nav;
     return a;
 }
 
@@ -347,6 +354,7 @@ int assemble_active(mastg_ctx* c, const double* X_active, bool want_R, bool want
         launch_jacobi_inv(S.n_active, c->diag_pos.as<int64_t>(), c->val.as<double>(), c->dinv.as<double>(),
                           c->stream);
         c->assembled = true;
+        c->val32_valid = false;
     }
     if (want_R) {
         const bool have_rel = X_active || c->d.physics == MASTG_HEX8_HEAT;
@@ -363,8 +371,14 @@ int autotune(mastg_ctx* c);
 int cg_finish(mastg_ctx* c, const CGArgs& a, const double* b, hipEvent_t e0, hipEvent_t e1,
               mastg_solve_stats* st, int done);
 
-int cg_solve(mastg_ctx* c, const double* b, const mastg_solve_opts* o, mastg_solve_stats* st,
-             bool matfree) {
+int cg_mixed_loop(mastg_ctx* c, const CGArgs& a, const double* b, const mastg_solve_opts* o, int chunk,
+                  int max_it, hipEvent_t e0, hipEvent_t e1, mastg_solve_stats* st);
+
+// op: 0 FP64 CSR SpMV, 1 matrix-free FP64, 2 mixed precision (FP32 node-block
+// SpMV inside CG whose residual is replaced by the FP64 true residual b - A x
+// each time it has dropped by reliable_delta, and at convergence)
+int cg_solve(mastg_ctx* c, const double* b, const mastg_solve_opts* o, mastg_solve_stats* st, int op) {
+    const bool matfree = op == 1;
     const Structure& S = c->S;
     const double rtol = (o && o->rtol > 0) ? o->rtol : 1e-10;
     const int max_it = (o && o->max_iters > 0) ? o->max_iters : 100000;
@@ -375,10 +389,16 @@ int cg_solve(mastg_ctx* c, const double* b, const mastg_solve_opts* o, mastg_sol
                           c->stream);
         c->sE_valid = true;
     }
+    if (op == 2 && !c->val32_valid) {
+        TRY(dalloc(c, c->val32, size_t(S.nnz) * sizeof(float)));
+        launch_to_f32(S.nnz, c->val.as<double>(), c->val32.as<float>(), c->stream);
+        c->val32_valid = true;
+    }
     CGArgs a = cg_args(c);
-    a.use_mf = matfree ? 1 : 0;
+    a.use_mf = op;
     if (matfree) a.nparts_op = mf_node_blocks(S.n_nodes);
-    const int gi = matfree ? 1 : 0;
+    const int gi = op;
+    c->cg_replacements = 0;
     hipEvent_t e0, e1;
     HIPC(c, hipEventCreate(&e0));
     HIPC(c, hipEventCreate(&e1));
@@ -386,6 +406,7 @@ int cg_solve(mastg_ctx* c, const double* b, const mastg_solve_opts* o, mastg_sol
     launch_fill(S.n_active, 0.0, a.x, c->stream);
     HIPC(c, hipMemcpyAsync(a.r, b, S.n_active * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     launch_cg_start(a, b, rtol * rtol, max_it, c->stream);
+    if (op == 2) return cg_mixed_loop(c, a, b, o, chunk, max_it, e0, e1, st);
     if (!c->use_graphs) {
         // direct launches, polled every `chunk` iterations
         int done = 0;
@@ -420,6 +441,49 @@ int cg_solve(mastg_ctx* c, const double* b, const mastg_solve_opts* o, mastg_sol
     return cg_finish(c, a, b, e0, e1, st, done);
 }
 
+int cg_mixed_loop(mastg_ctx* c, const CGArgs& a, const double* b, const mastg_solve_opts* o, int chunk,
+                  int max_it, hipEvent_t e0, hipEvent_t e1, mastg_solve_stats* st) {
+    const Structure& S = c->S;
+    const double delta = (o && o->reliable_delta > 0) ? o->reliable_delta : 0.1;
+    if (c->use_graphs && (!c->cg_exec[2] || c->cg_exec_len[2] != chunk)) {
+        if (c->cg_exec[2]) (void)hipGraphExecDestroy(c->cg_exec[2]);
+        c->cg_exec[2] = nullptr;
+        hipGraph_t g;
+        HIPC(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        for (int i = 0; i < chunk; i++) launch_cg_iteration(a, c->stream);
+        HIPC(c, hipStreamEndCapture(c->stream, &g));
+        HIPC(c, hipGraphInstantiate(&c->cg_exec[2], g, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(g);
+        c->cg_exec_len[2] = chunk;
+    }
+    HIPC(c, hipMemcpyAsync(c->h_scal, a.S, sizeof(CGScalars), hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    int done = c->h_scal->done;
+    double rref2 = c->h_scal->rr;   // ||r||^2 at the last replacement (start: ||b||^2)
+    for (int launched = 0; !done && launched < max_it + chunk; launched += chunk) {
+        if (c->use_graphs) HIPC(c, hipGraphLaunch(c->cg_exec[2], c->stream));
+        else
+            for (int i = 0; i < chunk; i++) launch_cg_iteration(a, c->stream);
+        HIPC(c, hipMemcpyAsync(c->h_scal, a.S, sizeof(CGScalars), hipMemcpyDeviceToHost, c->stream));
+        HIPC(c, hipStreamSynchronize(c->stream));
+        done = c->h_scal->done;
+        if (done == 3 || done == 4) break;
+        if (done == 1 || done == 2 || c->h_scal->rr <= delta * delta * rref2) {
+            // the recursive residual of the FP32 operator drifts from b - A x:
+            // replace it by the FP64 true residual (search direction kept)
+            launch_spmv_csr(a, a.x, c->vtmp.as<double>(), 0, c->stream);
+            launch_axpby(S.n_active, 1.0, b, -1.0, c->vtmp.as<double>(), c->stream);
+            launch_cg_replace(a, c->vtmp.as<double>(), c->stream);
+            HIPC(c, hipMemcpyAsync(c->h_scal, a.S, sizeof(CGScalars), hipMemcpyDeviceToHost, c->stream));
+            HIPC(c, hipStreamSynchronize(c->stream));
+            c->cg_replacements++;
+            done = c->h_scal->done;
+            rref2 = c->h_scal->rr;
+        }
+    }
+    return cg_finish(c, a, b, e0, e1, st, done);
+}
+
 int cg_finish(mastg_ctx* c, const CGArgs& a, const double* b, hipEvent_t e0, hipEvent_t e1,
               mastg_solve_stats* st, int done) {
     const Structure& S = c->S;
@@ -437,7 +501,7 @@ int cg_finish(mastg_ctx* c, const CGArgs& a, const double* b, hipEvent_t e0, hip
     (void)hipEventDestroy(e1);
     if (st) {
         st->iters = c->h_scal->iter;
-        st->replacements = 0;
+        st->replacements = c->cg_replacements;
         st->rel_res = bb > 0 ? std::sqrt(rr / bb) : 0.0;
         st->seconds = ms * 1e-3;
     }
@@ -572,7 +636,7 @@ int autotune(mastg_ctx* c) {
         }
     c->nparts = bgrid;
     c->spmv_sw = bsw;
-    for (int i = 0; i < 2; i++)
+    for (int i = 0; i < 3; i++)
         if (c->cg_exec[i]) { (void)hipGraphExecDestroy(c->cg_exec[i]); c->cg_exec[i] = nullptr; }
     return MASTG_OK;
 }
@@ -863,7 +927,7 @@ int mastg_ctx_create(const mastg_problem_desc* desc, mastg_ctx** out) {
 int mastg_ctx_destroy(mastg_ctx* c) {
     if (!c) return MASTG_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    for (int i = 0; i < 2; i++)
+    for (int i = 0; i < 3; i++)
         if (c->cg_exec[i]) (void)hipGraphExecDestroy(c->cg_exec[i]);
     if (c->bicg_exec) (void)hipGraphExecDestroy(c->bicg_exec);
     if (c->h_scal) (void)hipHostFree(c->h_scal);
@@ -942,6 +1006,19 @@ int mastg_assemble(mastg_ctx* c, const double* X, double* R, double* J, int wher
     return MASTG_OK;
 }
 
+static int cg_op(int solver) {
+    return solver == MASTG_SOLVER_MATFREE ? 1 : solver == MASTG_SOLVER_MIXED ? 2 : 0;
+}
+static int check_cg_solver(mastg_ctx* c, int solver) {
+    if (solver != MASTG_SOLVER_CG_JACOBI && solver != MASTG_SOLVER_MATFREE && solver != MASTG_SOLVER_MIXED)
+        return fail(c, MASTG_ERR_INVALID, "unknown solver");
+    if (solver == MASTG_SOLVER_MATFREE && (c->S.n_classes == 0 || c->d.physics != MASTG_HEX8_ELASTIC))
+        return fail(c, MASTG_ERR_INVALID, "matrix-free solver needs hex8 elasticity on a mesh of at most 8 element shapes");
+    if (solver == MASTG_SOLVER_MIXED && c->d.physics == MASTG_HEX8_HEAT)
+        return fail(c, MASTG_ERR_INVALID, "mixed-precision CG is for the symmetric structural Jacobians");
+    return MASTG_OK;
+}
+
 int mastg_solve(mastg_ctx* c, const double* rhs, double* x, int where, const mastg_solve_opts* o,
                 mastg_solve_stats* st) {
     if (!c || !rhs || !x) return MASTG_ERR_INVALID;
@@ -956,12 +1033,9 @@ int mastg_solve(mastg_ctx* c, const double* rhs, double* x, int where, const mas
         int rc2 = from_active(c, c->vx.as<double>(), x, where);
         return rc ? rc : rc2;
     }
-    if (solver != MASTG_SOLVER_CG_JACOBI && solver != MASTG_SOLVER_MATFREE)
-        return fail(c, MASTG_ERR_INVALID, "solver not available in this build");
-    if (solver == MASTG_SOLVER_MATFREE && (c->S.n_classes == 0 || c->d.physics != MASTG_HEX8_ELASTIC))
-        return fail(c, MASTG_ERR_INVALID, "matrix-free solver needs hex8 elasticity on a mesh of at most 8 element shapes");
+    TRY(check_cg_solver(c, solver));
     TRY(to_active(c, rhs, where, c->vb.as<double>()));
-    int rc = cg_solve(c, c->vb.as<double>(), o, st, solver == MASTG_SOLVER_MATFREE);
+    int rc = cg_solve(c, c->vb.as<double>(), o, st, cg_op(solver));
     int rc2 = from_active(c, c->vx.as<double>(), x, where);
     return rc ? rc : rc2;
 }
@@ -971,9 +1045,19 @@ int mastg_jacobian_product(mastg_ctx* c, const double* dX, double* JdX, int wher
     if (!c->assembled) return fail(c, MASTG_ERR_INVALID, "no assembled matrix");
     if (op == 1 && (c->S.n_classes == 0 || c->d.physics != MASTG_HEX8_ELASTIC))
         return fail(c, MASTG_ERR_INVALID, "no matrix-free operator");
+    if (op < 0 || op > 2) return fail(c, MASTG_ERR_INVALID, "unknown operator");
+    if (op == 2 && c->d.physics == MASTG_HEX8_HEAT)
+        return fail(c, MASTG_ERR_INVALID, "FP32 node-block operator is for the structural Jacobians");
     TRY(to_active(c, dX, where, c->vb.as<double>()));
+    if (op == 2 && !c->val32_valid) {
+        TRY(dalloc(c, c->val32, size_t(c->S.nnz) * sizeof(float)));
+        launch_to_f32(c->S.nnz, c->val.as<double>(), c->val32.as<float>(), c->stream);
+        c->val32_valid = true;
+    }
     CGArgs a = cg_args(c);
-    if (op == 1) {
+    if (op == 2) {
+        launch_spmv_nb32(a, c->vb.as<double>(), c->vtmp.as<double>(), 0, c->stream);
+    } else if (op == 1) {
         if (!c->sE_valid) {
             launch_elem_scale(c->S.n_elems, c->param.as<double>(), c->d.E, c->d.penalty, c->sE.as<double>(),
                               c->stream);
@@ -1052,11 +1136,8 @@ int mastg_design_eval(mastg_ctx* c, const double* params, double* dobj, int wher
     HIPC(c, hipEventRecord(e[1], c->stream));
     mastg_solve_stats st{};
     const int solver = o ? o->solver : MASTG_SOLVER_CG_JACOBI;
-    if (solver == MASTG_SOLVER_MATFREE && (c->S.n_classes == 0 || c->d.physics != MASTG_HEX8_ELASTIC))
-        return fail(c, MASTG_ERR_INVALID, "matrix-free solver needs hex8 elasticity on a mesh of at most 8 element shapes");
-    if (solver != MASTG_SOLVER_CG_JACOBI && solver != MASTG_SOLVER_MATFREE)
-        return fail(c, MASTG_ERR_INVALID, "solver not available in this build");
-    int rc = cg_solve(c, c->vb.as<double>(), o, &st, solver == MASTG_SOLVER_MATFREE);
+    TRY(check_cg_solver(c, solver));
+    int rc = cg_solve(c, c->vb.as<double>(), o, &st, cg_op(solver));
     HIPC(c, hipEventRecord(e[2], c->stream));
     double C = 0;
     TRY(dot_active(c, c->vb.as<double>(), c->vx.as<double>(), &C));
@@ -1138,7 +1219,7 @@ int mastg_set_tuning(mastg_ctx* c, int spmv_lanes, int grid) {
     c->spmv_sw = spmv_lanes;
     c->nparts = grid;
     c->tuned = true;
-    for (int i = 0; i < 2; i++)
+    for (int i = 0; i < 3; i++)
         if (c->cg_exec[i]) { (void)hipGraphExecDestroy(c->cg_exec[i]); c->cg_exec[i] = nullptr; }
     return MASTG_OK;
 }

@@ -90,9 +90,15 @@ struct CGArgs {
     int nparts_op;            // partial sums written by the operator kernel
     int nvec;                 // grid of the CG vector kernels (and their partials)
     CGScalars* S;
-    int use_mf;               // operator: 0 CSR SpMV, 1 matrix-free
+    int use_mf;               // operator: 0 CSR SpMV, 1 matrix-free, 2 FP32 node-block SpMV
     int spmv_sw;              // CSR SpMV lanes per row (4, 8, 16, 32, 64)
     MFArgs mf;
+    // FP32 copy of the CSR values read by node blocks (operator 2): node rank
+    // r owns rows r*nav..r*nav+nav-1 and the blocks blk_ptr[r]..blk_ptr[r+1]
+    const int64_t* blk_ptr;
+    const int32_t* blk_col;
+    const float* val32;
+    int nav;
 };
 
 // right-preconditioned BiCGStab + Jacobi on a scalar CSR (PETSc KSPBCGS role)
@@ -115,6 +121,11 @@ void launch_bicg_start(const BiCGArgs& a, const double* b, double tol2, int max_
 void launch_bicg_iteration(const BiCGArgs& a, hipStream_t s);
 void launch_gather_values(int64_t n, const int32_t* idx, const double* src, double* dst, hipStream_t s);
 
+void launch_to_f32(int64_t n, const double* src, float* dst, hipStream_t s);
+void launch_spmv_nb32(const CGArgs& c, const double* x, double* y, int want_dot, hipStream_t s);
+// reliable update: r = rt (a true FP64 residual), z = dinv r, fresh r.z / r.r;
+// done = 1 if ||r|| meets the tolerance, else 0 (2 past max_it)
+void launch_cg_replace(const CGArgs& c, const double* rt, hipStream_t s);
 void launch_hex8_K(const Hex8Args& a, hipStream_t s);
 void launch_hex8_sens(const Hex8Args& a, const double* lam, double* g, hipStream_t s);
 void launch_hex8_K_f32(const Hex8Args& a, float* Kel32, hipStream_t s);

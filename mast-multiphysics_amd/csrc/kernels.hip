@@ -1065,6 +1065,84 @@ __global__ __launch_bounds__(256) void k_cg_start_scalars(CGArgs C, double tol2,
 }
 
 
+// reliable residual replacement (mixed-precision CG): r = rt, z = dinv r
+__global__ __launch_bounds__(256) void k_cg_replace(CGArgs C, const double* __restrict__ rt) {
+    double rz = 0.0, rr = 0.0;
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < C.n; i += int64_t(gridDim.x) * 256) {
+        const double r = rt[i];
+        C.r[i] = r;
+        const double z = C.dinv[i] * r;
+        C.z[i] = z;
+        rz += r * z;
+        rr += r * r;
+    }
+    block_sum_store(rz, C.part0, blockIdx.x);
+    __syncthreads();
+    block_sum_store(rr, C.part1, blockIdx.x);
+}
+
+__global__ __launch_bounds__(256) void k_cg_replace_scalars(CGArgs C) {
+    const double rz = reduce_partials(C.part0, C.nvec);
+    const double rr = reduce_partials(C.part1, C.nvec);
+    if (threadIdx.x == 0) {
+        CGScalars* S = C.S;
+        S->rz = rz;
+        S->rr = rr;
+        if (!isfinite(rr)) S->done = 4;
+        else if (rr <= S->tol2 * S->bb) S->done = 1;
+        else if (S->iter >= S->max_it) S->done = 2;
+        else S->done = 0;
+    }
+}
+
+// y = A32 x over node blocks: SW lanes per node row, NAV rows per node; values
+// FP32 in the scalar CSR order (row v of node r: deg*NAV entries, block-major),
+// products and sums in FP64; optional partial sums of y.x
+template <int NAV, int SW>
+__global__ __launch_bounds__(256) void k_spmv_nb32_dot(CGArgs C, const double* __restrict__ x,
+                                                       double* __restrict__ y, int want_dot,
+                                                       int in_cg_loop) {
+    if (in_cg_loop && C.S->done) return;
+    const int sub = threadIdx.x & (SW - 1);
+    double dsum = 0.0;
+    const int64_t nodes_per_grid = (int64_t(gridDim.x) * 256) / SW;
+    const int64_t n_nodes = C.n / NAV;
+    for (int64_t r = (int64_t(blockIdx.x) * 256 + threadIdx.x) / SW; r < n_nodes; r += nodes_per_grid) {
+        const int64_t k0 = C.blk_ptr[r];
+        const int deg = int(C.blk_ptr[r + 1] - k0);
+        const float* __restrict__ V = C.val32 + k0 * (NAV * NAV);
+        double s[NAV];
+#pragma unroll
+        for (int v = 0; v < NAV; v++) s[v] = 0.0;
+        for (int k = sub; k < deg; k += SW) {
+            const int64_t cb = int64_t(C.blk_col[k0 + k]) * NAV;
+            double xw[NAV];
+#pragma unroll
+            for (int w = 0; w < NAV; w++) xw[w] = x[cb + w];
+#pragma unroll
+            for (int v = 0; v < NAV; v++)
+#pragma unroll
+                for (int w = 0; w < NAV; w++) s[v] += double(V[(int64_t(v) * deg + k) * NAV + w]) * xw[w];
+        }
+#pragma unroll
+        for (int v = 0; v < NAV; v++)
+#pragma unroll
+            for (int m = SW / 2; m >= 1; m >>= 1) s[v] += __shfl_xor(s[v], m, SW);
+        if (sub == 0)
+#pragma unroll
+            for (int v = 0; v < NAV; v++) {
+                y[r * NAV + v] = s[v];
+                dsum += s[v] * x[r * NAV + v];
+            }
+    }
+    if (want_dot) block_sum_store(dsum, C.part0, blockIdx.x);
+}
+
+__global__ void k_to_f32(int64_t n, const double* __restrict__ src, float* __restrict__ dst) {
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256)
+        dst[i] = float(src[i]);
+}
+
 // ------------------------------------------------- BiCGStab (FP64 CSR) ----
 // y = A x with SW lanes per row; optional partials sum(y.w1) -> part0 and
 // sum(y.y) -> part1 (per block).  Stops immediately once the solver is done.
@@ -1274,6 +1352,26 @@ static void spmv_dispatch(const CGArgs& c, const double* x, double* y, int want_
 void launch_spmv_csr(const CGArgs& c, const double* x, double* y, int want_dot, hipStream_t s) {
     spmv_dispatch(c, x, y, want_dot, 0, s);
 }
+// node-block FP32 SpMV: 8 lanes per node row (a hex8 interior node has 27
+// blocks, a quad4 one 9), grid of the CSR SpMV
+static void nb32_dispatch(const CGArgs& c, const double* x, double* y, int want_dot, int in_loop,
+                          hipStream_t s) {
+    if (c.nav == 5)
+        hipLaunchKernelGGL((k_spmv_nb32_dot<5, 8>), dim3(c.nparts), dim3(256), 0, s, c, x, y, want_dot, in_loop);
+    else
+        hipLaunchKernelGGL((k_spmv_nb32_dot<3, 8>), dim3(c.nparts), dim3(256), 0, s, c, x, y, want_dot, in_loop);
+}
+void launch_spmv_nb32(const CGArgs& c, const double* x, double* y, int want_dot, hipStream_t s) {
+    nb32_dispatch(c, x, y, want_dot, 0, s);
+}
+void launch_to_f32(int64_t n, const double* src, float* dst, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_to_f32, dim3(std::min<int64_t>(blocks_for(n, 256), 8192)), dim3(256), 0, s, n, src, dst);
+}
+void launch_cg_replace(const CGArgs& c, const double* rt, hipStream_t s) {
+    hipLaunchKernelGGL(k_cg_replace, dim3(c.nvec), dim3(256), 0, s, c, rt);
+    hipLaunchKernelGGL(k_cg_replace_scalars, dim3(1), dim3(256), 0, s, c);
+}
 void launch_cg_start(const CGArgs& c, const double* b, double tol2, int max_it, hipStream_t s) {
     hipLaunchKernelGGL(k_cg_start, dim3(c.nvec), dim3(256), 0, s, c, b);
     hipLaunchKernelGGL(k_cg_start_scalars, dim3(1), dim3(256), 0, s, c, tol2, max_it);
@@ -1330,8 +1428,10 @@ void launch_mask_constrained(int64_t n, const uint8_t* con, double* x, hipStream
     hipLaunchKernelGGL(k_mask_constrained, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, con, x);
 }
 void launch_cg_iteration(const CGArgs& c, hipStream_t s) {
-    if (c.use_mf)
+    if (c.use_mf == 1)
         mf_dispatch(c, c.p, c.p, c.q, 1, s);
+    else if (c.use_mf == 2)
+        nb32_dispatch(c, c.p, c.q, 1, 1, s);
     else
         spmv_dispatch(c, c.p, c.q, 1, 1, s);
     hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(256), 0, s, c);

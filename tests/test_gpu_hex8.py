@@ -269,3 +269,48 @@ def test_fp32_jacobian_solve_and_limits(oracle, mastg):
         cp.assemble(elem_precision=mastg.ELEM_FP32)
     assert e.value.code == -1
     cp.close()
+
+
+@pytest.mark.parametrize("mesh", [(2, 1, 1), (7, 5, 3)])
+def test_fp32_node_block_operator(oracle, mastg, mesh):
+    """The mixed solver's operator (FP32-rounded CSR values, read per node
+    block, FP64 products): within FP32 rounding of the FP64 CSR product,
+    |y32 - y| <= 1e-6 * (|A| |x|) entry-wise."""
+    print(f"[hex8 FP32 node-block SpMV] mesh={mesh}", flush=True)
+    P = oracle.Problem(oracle.CANTILEVER, *mesh, seed=8, lo=0.01, hi=1.0)
+    ctx = ctx_from_oracle(mastg, P)
+    _, J = ctx.assemble()
+    rp, col = ctx.csr_pattern()
+    x = np.random.default_rng(2).standard_normal(P.n_dofs)
+    amap = ctx.active_map()
+    act = np.nonzero(amap >= 0)[0]
+    y64 = ctx.jacobian_product(x, op=0)
+    y32 = ctx.jacobian_product(x, op=2)
+    xa = x[act]
+    rows = np.repeat(np.arange(rp.size - 1), np.diff(rp))
+    absAx = np.zeros(rp.size - 1)
+    np.add.at(absAx, rows, np.abs(J) * np.abs(xa[col]))
+    assert np.all(np.abs(y32[act] - y64[act]) <= 1e-6 * absAx + 1e-300)
+    assert np.max(np.abs(y32[act] - y64[act])) > 0.0     # it is the FP32 copy that is read
+    ctx.close()
+
+
+@pytest.mark.parametrize("mesh,lo", [((20, 10, 10), 0.3), ((16, 8, 8), 0.01)])
+def test_design_eval_mixed(oracle, mastg, mesh, lo):
+    """FP32 node-block SpMV inside CG, residual replaced by the FP64 true
+    residual (reliable updates): the answer is fixed by the FP64 matrix, so the
+    bars are the FP64 paths' (true ||b - A x||/||b|| <= 1e-10, compliance 1e-9,
+    dC/drho 1e-7); the FP32 operator may cost extra iterations."""
+    print(f"[hex8 design eval mixed] mesh={mesh} lo={lo}", flush=True)
+    P = oracle.Problem(oracle.CANTILEVER, *mesh, seed=1, lo=lo, hi=1.0)
+    ref = P.solve_elastic(1e-10)
+    ctx = ctx_from_oracle(mastg, P)
+    r = ctx.design_eval(P.params(), solver=mastg.SOLVER_MIXED)
+    assert r["rel_res"] <= 1.5e-10
+    assert abs(r["objective"] - ref["compliance"]) <= 1e-9 * abs(ref["compliance"])
+    assert rel_err_floor(r["grad"], ref["dc"], floor=1e-6) <= 1e-7
+    assert r["iters"] <= 3 * ref["iters"]
+    R, _ = ctx.assemble()
+    x, st = ctx.solve(-R, solver=mastg.SOLVER_MIXED)
+    assert st["replacements"] >= 1 and st["rel_res"] <= 1.5e-10
+    ctx.close()

@@ -67,6 +67,24 @@ def test_plate_design_eval(oracle, mastg, n):
     ctx.close()
 
 
+def test_plate_design_eval_mixed(oracle, mastg):
+    """Mixed-precision CG (FP32 node-block SpMV over the 5x5 plate blocks,
+    FP64 residual replacement) reaches the FP64 paths' bars."""
+    print("[plate design eval mixed] 24x24", flush=True)
+    P = oracle.Problem(oracle.PLATE, 24, 24, seed=1)
+    ref = P.solve_elastic(1e-10)
+    ctx = ctx_from_oracle(mastg, P)
+    r = ctx.design_eval(P.params(), solver=mastg.SOLVER_MIXED)
+    assert r["rel_res"] <= 1.5e-10
+    assert abs(r["objective"] - ref["compliance"]) <= 1e-9 * abs(ref["compliance"])
+    assert rel_err_floor(r["grad"], ref["dc"], floor=1e-6) <= 1e-7
+    x = np.random.default_rng(1).standard_normal(P.n_dofs)
+    y64 = ctx.jacobian_product(x, op=0)
+    y32 = ctx.jacobian_product(x, op=2)
+    assert np.max(np.abs(y32 - y64)) <= 1e-6 * np.max(np.abs(y64))
+    ctx.close()
+
+
 def test_plate_adjoint_vs_fd_on_gpu(mastg):
     print("[plate FD on GPU] 12x12", flush=True)
     w = mastg.Workload(2, 12, 12, seed=5)
