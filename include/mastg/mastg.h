@@ -143,6 +143,13 @@ int mastg_assemble(mastg_ctx* ctx, const double* X, double* R, double* J_vals,
 #define MASTG_SOLVER_MATFREE 3          /* FP64 CG + Jacobi on the matrix-free operator
                                            over congruent-element classes (the same
                                            matrix as the CSR; hex8, <= 8 element shapes) */
+#define MASTG_SOLVER_MG 4               /* FP64 CG preconditioned by a geometric multigrid
+                                           V-cycle: trilinear transfers, Galerkin coarse
+                                           operators re-formed per assembly, Chebyshev/
+                                           Jacobi smoothing, dense coarsest solve; the
+                                           matrix-free operator on the problem level.
+                                           hex8 meshes that fill a lattice whose cell
+                                           counts halve to <= 1536 coarse DOFs */
 
 typedef struct {
     int32_t solver;       /* MASTG_SOLVER_*                                   */

@@ -7,12 +7,14 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <new>
 #include <string>
 #include <vector>
 
 #include "kernels.h"
 #include "mastg/mastg.h"
+#include "multigrid.h"
 #include "structure.h"
 
 using namespace mastg;
@@ -36,6 +38,15 @@ struct DevBuf {
 struct KTimer {   // accumulated device time of one kernel kind
     double ms = 0;
     int64_t launches = 0;
+};
+
+// one multigrid level on the device (level 0 borrows the context's operator)
+struct MGLevelDev {
+    MGLevelArgs a{};
+    int64_t n_elems = 0;
+    DevBuf rank_of_lat, elem_of_cell;                  // level 0
+    DevBuf blk_ptr, blk_col, cptr, cidx, econn, con;   // levels >= 1
+    DevBuf Kel, val, dinv, r, x, res, da, db;
 };
 
 }  // namespace
@@ -73,6 +84,17 @@ struct mastg_ctx {
     DevBuf val32;              // FP32 copy of val for the mixed-precision solver
     bool val32_valid = false;
     int cg_replacements = 0;
+    bool kel64_valid = false;  // FP64 element blocks of the last assembly are resident
+    // geometric multigrid (hex8 lattice meshes), built on first use
+    std::vector<std::unique_ptr<MGLevelDev>> mg;
+    bool mg_tried = false;
+    std::string mg_why;
+    bool mg_valid = false;     // coarse operators match the last assembly
+    int mg_k = 3;              // Chebyshev steps per smoothing
+    int64_t mg_n_coarse = 0;   // DOFs of the dense coarsest level
+    DevBuf mg_cheb, mg_lam, mg_part, mg_dense;
+    hipGraphExec_t mg_exec = nullptr;
+    int mg_exec_len = 0;
     // launch shapes (autotuned on the first solve)
     int spmv_sw = 16;
     bool use_graphs = true;
@@ -355,6 +377,8 @@ int assemble_active(mastg_ctx* c, const double* X_active, bool want_R, bool want
                           c->stream);
         c->assembled = true;
         c->val32_valid = false;
+        c->kel64_valid = !fp32;
+        c->mg_valid = false;
     }
     if (want_R) {
         const bool have_rel = X_active || c->d.physics == MASTG_HEX8_HEAT;
@@ -481,6 +505,211 @@ int cg_mixed_loop(mastg_ctx* c, const CGArgs& a, const double* b, const mastg_so
             rref2 = c->h_scal->rr;
         }
     }
+    return cg_finish(c, a, b, e0, e1, st, done);
+}
+
+// ---- geometric multigrid preconditioned CG (hex8 lattice meshes) ----------
+// The hierarchy (lattice detection, coarse meshes, their node-block structure)
+// is built on the first multigrid solve; coarse operators are re-formed after
+// every assembly (Galerkin products of the element matrices, so they follow
+// the densities), all on the device.
+int mg_build(mastg_ctx* c) {
+    if (c->mg_tried) return c->mg.empty() ? fail(c, MASTG_ERR_INVALID, "multigrid unavailable: " + c->mg_why)
+                                          : MASTG_OK;
+    c->mg_tried = true;
+    if (c->d.physics != MASTG_HEX8_ELASTIC || c->S.n_classes == 0) {
+        c->mg_why = "multigrid is built for hex8 elasticity on lattice meshes";
+        return fail(c, MASTG_ERR_INVALID, "multigrid unavailable: " + c->mg_why);
+    }
+    MGHierarchy H = build_mg_hierarchy(c->S);
+    if (!H.ok) {
+        c->mg_why = H.why;
+        return fail(c, MASTG_ERR_INVALID, "multigrid unavailable: " + H.why);
+    }
+    const int nl = int(H.levels.size());
+    std::vector<std::unique_ptr<MGLevelDev>> lv;
+    for (int l = 0; l < nl; l++) {
+        const MGLevelHost& h = H.levels[l];
+        auto L = std::make_unique<MGLevelDev>();
+        MGLevelArgs& a = L->a;
+        a.n_nodes = h.n_nodes;
+        a.nx = h.nx; a.ny = h.ny; a.nz = h.nz;
+        L->n_elems = h.n_elems;
+        const size_t nv = size_t(h.n_nodes) * 3 * sizeof(double);
+        if (l == 0) {
+            TRY(upload(c, L->rank_of_lat, h.rank_of_lat));
+            TRY(upload(c, L->elem_of_cell, h.elem_of_cell));
+            a.rank_of_lat = L->rank_of_lat.as<int32_t>();
+            a.con = c->con.as<uint8_t>();
+            a.dinv = c->dinv.as<double>();
+            a.r = c->vr.as<double>();
+        } else {
+            const Structure& S = h.S;
+            TRY(upload(c, L->blk_ptr, S.blk_ptr));
+            TRY(upload(c, L->blk_col, S.blk_col));
+            TRY(upload(c, L->cptr, S.cptr));
+            TRY(upload(c, L->cidx, S.cidx));
+            TRY(upload(c, L->econn, S.econn));
+            TRY(upload(c, L->con, S.constrained));
+            TRY(dalloc(c, L->Kel, size_t(S.n_elems) * 64 * 9 * sizeof(double)));
+            TRY(dalloc(c, L->val, size_t(S.n_blocks) * 9 * sizeof(double)));
+            TRY(dalloc(c, L->dinv, nv));
+            TRY(dalloc(c, L->r, nv));
+            a.rank_of_lat = nullptr;    // coarse levels are numbered in lattice order
+            a.con = L->con.as<uint8_t>();
+            a.blk_ptr = L->blk_ptr.as<int64_t>();
+            a.blk_col = L->blk_col.as<int32_t>();
+            a.val = L->val.as<double>();
+            a.dinv = L->dinv.as<double>();
+            a.r = L->r.as<double>();
+        }
+        TRY(dalloc(c, L->x, nv));
+        TRY(dalloc(c, L->res, nv));
+        TRY(dalloc(c, L->da, nv));
+        TRY(dalloc(c, L->db, nv));
+        a.x = L->x.as<double>();
+        a.res = L->res.as<double>();
+        a.da = L->da.as<double>();
+        a.db = L->db.as<double>();
+        lv.push_back(std::move(L));
+    }
+    TRY(dalloc(c, c->mg_cheb, size_t(nl) * kMGChebStride * sizeof(double)));
+    TRY(dalloc(c, c->mg_lam, size_t(nl) * sizeof(double)));
+    TRY(dalloc(c, c->mg_part, size_t((c->S.n_active + 255) / 256 + 1) * sizeof(double)));
+    c->mg_n_coarse = H.coarsest_dofs;
+    TRY(dalloc(c, c->mg_dense, size_t(H.coarsest_dofs * H.coarsest_dofs) * sizeof(double)));
+    for (int l = 0; l < nl; l++) lv[l]->a.cheb = c->mg_cheb.as<double>() + l * kMGChebStride;
+    c->mg = std::move(lv);
+    return MASTG_OK;
+}
+
+// coarse operators, smoothing bounds and the coarsest inverse for the current
+// assembly (needs the FP64 element blocks and CSR values of that assembly)
+int mg_setup(mastg_ctx* c) {
+    if (c->mg_valid) return MASTG_OK;
+    if (!c->kel64_valid) return fail(c, MASTG_ERR_INVALID, "multigrid needs an FP64 assembly");
+    const int nl = int(c->mg.size());
+    hipStream_t s = c->stream;
+    launch_mg_rowinfo_csr(c->S.n_active, c->row_ptr.as<int64_t>(), c->val.as<double>(), c->diag_pos.as<int64_t>(),
+                          c->mg_part.as<double>(), c->mg_lam.as<double>(), s);
+    for (int l = 0; l + 1 < nl; l++) {
+        MGLevelDev& F = *c->mg[l];
+        MGLevelDev& C = *c->mg[l + 1];
+        MGGalerkinArgs g{};
+        g.n_coarse = C.n_elems;
+        g.cnx = C.a.nx; g.cny = C.a.ny;
+        g.fnx = F.a.nx; g.fny = F.a.ny;
+        g.f_elem_of_cell = l == 0 ? F.elem_of_cell.as<int32_t>() : nullptr;
+        g.f_econn = l == 0 ? c->econn.as<int32_t>() : F.econn.as<int32_t>();
+        g.f_con = F.a.con;
+        g.c_econn = C.econn.as<int32_t>();
+        g.c_con = C.a.con;
+        g.Kf = l == 0 ? c->Kel.as<double>() : F.Kel.as<double>();
+        g.Kc = C.Kel.as<double>();
+        launch_mg_galerkin(g, s);
+        ScatterArgs sa{C.a.n_nodes, 3, C.a.blk_ptr, C.cptr.as<int32_t>(), C.cidx.as<int32_t>(),
+                       C.Kel.as<double>(), C.val.as<double>()};
+        launch_scatter(sa, s);
+        launch_mg_rowinfo_bsr(C.a, C.dinv.as<double>(), c->mg_part.as<double>(), c->mg_lam.as<double>() + l + 1, s);
+    }
+    launch_mg_cheb(c->mg_lam.as<double>(), c->mg_cheb.as<double>(), nl, c->mg_k, 30.0, s);
+    const MGLevelDev& Lc = *c->mg[nl - 1];
+    const int64_t n = c->mg_n_coarse;
+    launch_fill(n * n, 0.0, c->mg_dense.as<double>(), s);
+    launch_mg_dense_gather(Lc.a, c->mg_dense.as<double>(), s);
+    launch_mg_invert(int(n), c->mg_dense.as<double>(), s);
+    HIPC(c, hipGetLastError());
+    c->mg_valid = true;
+    return MASTG_OK;
+}
+
+// one V-cycle on a.r; the level-0 output is x0 + dlast0 (formed by the caller)
+const double* mg_vcycle(mastg_ctx* c, const CGArgs& a, const int* done, hipStream_t s) {
+    const int nl = int(c->mg.size());
+    const int k = c->mg_k;
+    auto dbuf = [&](int l, int i) { return (i % 2 == 0) ? c->mg[l]->a.da : c->mg[l]->a.db; };
+    auto apply = [&](int l, int mode, int step, const double* din, double* dout) {
+        if (l == 0) launch_mg_fine(a, c->mg[0]->a, mode, step, din, dout, done, s);
+        else launch_mg_bsr(c->mg[l]->a, mode, step, din, dout, done, s);
+    };
+    for (int l = 0; l + 1 < nl; l++) {
+        launch_mg_pre_init(c->mg[l]->a, done, s);
+        for (int i = 1; i < k; i++) apply(l, MG_STEP, i, dbuf(l, i - 1), dbuf(l, i));
+        apply(l, MG_FINAL, 0, dbuf(l, k - 1), nullptr);
+        launch_mg_restrict(c->mg[l]->a, c->mg[l + 1]->a, done, s);
+    }
+    const MGLevelArgs& Lc = c->mg[nl - 1]->a;
+    launch_mg_coarse_solve(int(c->mg_n_coarse), c->mg_dense.as<double>(), Lc.r, Lc.x, done, s);
+    const double* dlast = nullptr;
+    for (int l = nl - 2; l >= 0; l--) {
+        launch_mg_prolong(c->mg[l]->a, c->mg[l + 1]->a, dlast, done, s);
+        apply(l, MG_RESID_INIT, 0, nullptr, dbuf(l, 0));
+        for (int i = 1; i < k; i++) apply(l, MG_STEP, i, dbuf(l, i - 1), dbuf(l, i));
+        dlast = dbuf(l, k - 1);
+    }
+    return dlast;
+}
+
+void mg_pcg_iteration(mastg_ctx* c, const CGArgs& a, hipStream_t s) {
+    const int* done = &a.S->done;
+    launch_cg_op_alpha(a, s);
+    launch_pcg_update(a, s);
+    const double* dlast = mg_vcycle(c, a, done, s);
+    launch_pcg_rz(a, c->mg[0]->a.x, dlast, s);
+    launch_pcg_beta(a, s);
+    launch_cg_p(a, s);
+}
+
+int mg_solve(mastg_ctx* c, const double* b, const mastg_solve_opts* o, mastg_solve_stats* st) {
+    const Structure& S = c->S;
+    const double rtol = (o && o->rtol > 0) ? o->rtol : 1e-10;
+    const int max_it = (o && o->max_iters > 0) ? o->max_iters : 100000;
+    const int chunk = (o && o->check_every > 0) ? o->check_every : 4;
+    TRY(mg_build(c));
+    if (!c->tuned) TRY(autotune(c));   // launch shape of the final CSR residual check
+    if (!c->sE_valid) {
+        launch_elem_scale(S.n_elems, c->param.as<double>(), c->d.E, c->d.penalty, c->sE.as<double>(), c->stream);
+        c->sE_valid = true;
+    }
+    hipEvent_t e0, e1;
+    HIPC(c, hipEventCreate(&e0));
+    HIPC(c, hipEventCreate(&e1));
+    HIPC(c, hipEventRecord(e0, c->stream));
+    TRY(mg_setup(c));
+    CGArgs a = cg_args(c);
+    a.use_mf = 1;
+    a.nparts_op = mf_node_blocks(S.n_nodes);
+    c->cg_replacements = 0;
+    launch_fill(S.n_active, 0.0, a.x, c->stream);
+    HIPC(c, hipMemcpyAsync(a.r, b, S.n_active * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    launch_pcg_start(a, b, rtol * rtol, max_it, c->stream);
+    const double* dlast = mg_vcycle(c, a, &a.S->done, c->stream);
+    launch_pcg_rz(a, c->mg[0]->a.x, dlast, c->stream);
+    launch_pcg_start2(a, c->stream);
+    HIPC(c, hipMemcpyAsync(a.p, a.z, S.n_active * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    HIPC(c, hipMemcpyAsync(c->h_scal, a.S, sizeof(CGScalars), hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    int done = c->h_scal->done;
+    if (c->use_graphs && !done && (!c->mg_exec || c->mg_exec_len != chunk)) {
+        if (c->mg_exec) (void)hipGraphExecDestroy(c->mg_exec);
+        c->mg_exec = nullptr;
+        hipGraph_t g;
+        HIPC(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        for (int i = 0; i < chunk; i++) mg_pcg_iteration(c, a, c->stream);
+        HIPC(c, hipStreamEndCapture(c->stream, &g));
+        HIPC(c, hipGraphInstantiate(&c->mg_exec, g, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(g);
+        c->mg_exec_len = chunk;
+    }
+    for (int launched = 0; !done && launched < max_it + chunk; launched += chunk) {
+        if (c->use_graphs) HIPC(c, hipGraphLaunch(c->mg_exec, c->stream));
+        else
+            for (int i = 0; i < chunk; i++) mg_pcg_iteration(c, a, c->stream);
+        HIPC(c, hipMemcpyAsync(c->h_scal, a.S, sizeof(CGScalars), hipMemcpyDeviceToHost, c->stream));
+        HIPC(c, hipStreamSynchronize(c->stream));
+        done = c->h_scal->done;
+    }
+    HIPC(c, hipGetLastError());
     return cg_finish(c, a, b, e0, e1, st, done);
 }
 
@@ -930,6 +1159,7 @@ int mastg_ctx_destroy(mastg_ctx* c) {
     for (int i = 0; i < 3; i++)
         if (c->cg_exec[i]) (void)hipGraphExecDestroy(c->cg_exec[i]);
     if (c->bicg_exec) (void)hipGraphExecDestroy(c->bicg_exec);
+    if (c->mg_exec) (void)hipGraphExecDestroy(c->mg_exec);
     if (c->h_scal) (void)hipHostFree(c->h_scal);
     if (c->h_pin) (void)hipHostFree(c->h_pin);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -1010,8 +1240,10 @@ static int cg_op(int solver) {
     return solver == MASTG_SOLVER_MATFREE ? 1 : solver == MASTG_SOLVER_MIXED ? 2 : 0;
 }
 static int check_cg_solver(mastg_ctx* c, int solver) {
-    if (solver != MASTG_SOLVER_CG_JACOBI && solver != MASTG_SOLVER_MATFREE && solver != MASTG_SOLVER_MIXED)
+    if (solver != MASTG_SOLVER_CG_JACOBI && solver != MASTG_SOLVER_MATFREE && solver != MASTG_SOLVER_MIXED &&
+        solver != MASTG_SOLVER_MG)
         return fail(c, MASTG_ERR_INVALID, "unknown solver");
+    if (solver == MASTG_SOLVER_MG) return mg_build(c);
     if (solver == MASTG_SOLVER_MATFREE && (c->S.n_classes == 0 || c->d.physics != MASTG_HEX8_ELASTIC))
         return fail(c, MASTG_ERR_INVALID, "matrix-free solver needs hex8 elasticity on a mesh of at most 8 element shapes");
     if (solver == MASTG_SOLVER_MIXED && c->d.physics == MASTG_HEX8_HEAT)
@@ -1035,7 +1267,8 @@ int mastg_solve(mastg_ctx* c, const double* rhs, double* x, int where, const mas
     }
     TRY(check_cg_solver(c, solver));
     TRY(to_active(c, rhs, where, c->vb.as<double>()));
-    int rc = cg_solve(c, c->vb.as<double>(), o, st, cg_op(solver));
+    int rc = solver == MASTG_SOLVER_MG ? mg_solve(c, c->vb.as<double>(), o, st)
+                                       : cg_solve(c, c->vb.as<double>(), o, st, cg_op(solver));
     int rc2 = from_active(c, c->vx.as<double>(), x, where);
     return rc ? rc : rc2;
 }
@@ -1137,7 +1370,8 @@ int mastg_design_eval(mastg_ctx* c, const double* params, double* dobj, int wher
     mastg_solve_stats st{};
     const int solver = o ? o->solver : MASTG_SOLVER_CG_JACOBI;
     TRY(check_cg_solver(c, solver));
-    int rc = cg_solve(c, c->vb.as<double>(), o, &st, cg_op(solver));
+    int rc = solver == MASTG_SOLVER_MG ? mg_solve(c, c->vb.as<double>(), o, &st)
+                                       : cg_solve(c, c->vb.as<double>(), o, &st, cg_op(solver));
     HIPC(c, hipEventRecord(e[2], c->stream));
     double C = 0;
     TRY(dot_active(c, c->vb.as<double>(), c->vx.as<double>(), &C));

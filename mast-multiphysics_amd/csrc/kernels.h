@@ -157,4 +157,71 @@ void launch_mf_apply(const CGArgs& c, const double* x_elem, const double* x_node
 void launch_mask_constrained(int64_t n, const uint8_t* con, double* x, hipStream_t s);
 int mf_node_blocks(int64_t n_nodes);
 
+// ---- geometric multigrid V-cycle (hex8 elasticity on lattice meshes) ------
+constexpr int kMGMaxCheb = 8;          // Chebyshev smoothing steps at most
+constexpr int kMGChebStride = 16;      // coefficients per level: 1/theta, (c1_i, c2_i)
+
+struct MGLevelArgs {
+    int64_t n_nodes;
+    int nx, ny, nz;                    // lattice cells per dimension
+    const int32_t* rank_of_lat;        // node rank of lattice point; null = identity
+    const uint8_t* con;                // [3 n_nodes] constrained DOFs
+    const int64_t* blk_ptr;            // operator of levels >= 1: node-block CSR, FP64
+    const int32_t* blk_col;
+    const double* val;
+    const double* dinv;                // [3 n_nodes] inverse diagonal
+    double *r, *x, *res, *da, *db;     // right side, iterate, residual, smoothing directions
+    const double* cheb;                // this level's Chebyshev coefficients
+};
+
+struct MGGalerkinArgs {                // coarse element matrices = sum_c Q_c^T K_c Q_c
+    int64_t n_coarse;                  // coarse elements (lattice cell order)
+    int cnx, cny;                      // coarse cells in x, y
+    int fnx, fny;                      // fine cells in x, y
+    const int32_t* f_elem_of_cell;     // null = identity
+    const int32_t* f_econn;            // fine element node ranks
+    const uint8_t* f_con;              // fine constraints by rank*3+v
+    const int32_t* c_econn;
+    const uint8_t* c_con;
+    const double* Kf;                  // [n_fine][64][9] fine element blocks
+    double* Kc;                        // [n_coarse][64][9]
+};
+
+// smoothing modes (row-local epilogue after y = A in):
+//   MG_STEP:       x += din; res -= y; dout = c1_i din + c2_i dinv res   (in = din)
+//   MG_FINAL:      x += din; res -= y                                     (in = din)
+//   MG_RESID_INIT: res = r - y; dout = dinv res / theta                   (in = x)
+enum { MG_STEP = 1, MG_FINAL = 2, MG_RESID_INIT = 3 };
+
+void launch_mg_galerkin(const MGGalerkinArgs& g, hipStream_t s);
+// dinv (levels >= 1, from the node-block values) and the Gershgorin bound
+// max_i sum_j |a_ij| / a_ii into *lam; level 0 reads the scalar CSR
+void launch_mg_rowinfo_bsr(const MGLevelArgs& L, double* dinv, double* part, double* lam, hipStream_t s);
+void launch_mg_rowinfo_csr(int64_t n, const int64_t* row_ptr, const double* val, const int64_t* diag_pos,
+                           double* part, double* lam, hipStream_t s);
+void launch_mg_cheb(const double* lam, double* cheb, int n_levels, int k, double ratio, hipStream_t s);
+void launch_mg_dense_gather(const MGLevelArgs& L, double* A, hipStream_t s);
+void launch_mg_invert(int n, double* A, hipStream_t s);
+// V-cycle pieces; every kernel returns at once when *done != 0 (done may be null)
+void launch_mg_pre_init(const MGLevelArgs& L, const int* done, hipStream_t s);
+void launch_mg_bsr(const MGLevelArgs& L, int mode, int step, const double* din, double* dout, const int* done,
+                   hipStream_t s);
+void launch_mg_fine(const CGArgs& c, const MGLevelArgs& L, int mode, int step, const double* din,
+                    double* dout, const int* done, hipStream_t s);
+void launch_mg_restrict(const MGLevelArgs& F, const MGLevelArgs& C, const int* done, hipStream_t s);
+void launch_mg_prolong(const MGLevelArgs& F, const MGLevelArgs& C, const double* dlast, const int* done,
+                       hipStream_t s);
+void launch_mg_coarse_solve(int n, const double* Ainv, const double* r, double* x, const int* done,
+                            hipStream_t s);
+// preconditioned CG around the V-cycle
+void launch_pcg_update(const CGArgs& c, hipStream_t s);
+void launch_pcg_rz(const CGArgs& c, const double* x0, const double* dlast, hipStream_t s);
+void launch_pcg_start(const CGArgs& c, const double* b, double tol2, int max_it, hipStream_t s);
+void launch_pcg_start2(const CGArgs& c, hipStream_t s);
+void launch_pcg_beta(const CGArgs& c, hipStream_t s);
+// pieces of the CG iteration shared with the preconditioned variant
+void launch_mf_elem(const CGArgs& c, const double* x, hipStream_t s);   // element phase only
+void launch_cg_op_alpha(const CGArgs& c, hipStream_t s);                // q = A p, p.q, alpha
+void launch_cg_p(const CGArgs& c, hipStream_t s);                       // p = z + beta p
+
 }  // namespace mastg

@@ -11,6 +11,7 @@
 #include <stdint.h>
 
 #include "kernels.h"
+#include "device_util.h"
 
 #define Structure_kMaxClasses 8
 
@@ -807,45 +808,12 @@ __global__ void k_jacobi_inv(int64_t n, const int64_t* __restrict__ diag_pos, co
     }
 }
 
-// block-level FP64 sum into partials[blockIdx.x] (blockDim 256)
-__device__ __forceinline__ void block_sum_store(double v, double* partials, int slot) {
-    __shared__ double red[4];
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) partials[slot] = (red[0] + red[1]) + (red[2] + red[3]);
-}
-
 __global__ __launch_bounds__(256) void k_dot(int64_t n, const double* __restrict__ a,
                                              const double* __restrict__ b, double* __restrict__ partials) {
     double s = 0.0;
     for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256)
         s += a[i] * b[i];
     block_sum_store(s, partials, blockIdx.x);
-}
-
-// deterministic final reduction of `n` partials (one block of 256)
-__device__ __forceinline__ double reduce_partials(const double* __restrict__ p, int n) {
-    __shared__ double red[4];
-    // four independent accumulators keep four loads in flight per thread
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-    int i = threadIdx.x;
-    for (; i + 768 < n; i += 1024) {
-        s0 += p[i];
-        s1 += p[i + 256];
-        s2 += p[i + 512];
-        s3 += p[i + 768];
-    }
-    for (; i < n; i += 256) s0 += p[i];
-    double s = (s0 + s1) + (s2 + s3);
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) s += __shfl_xor(s, m, 64);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
-    __syncthreads();
-    const double t = (red[0] + red[1]) + (red[2] + red[3]);
-    __syncthreads();
-    return t;
 }
 
 __global__ __launch_bounds__(256) void k_reduce_to(const double* __restrict__ p, int n,
@@ -1426,6 +1394,25 @@ void launch_mf_apply(const CGArgs& c, const double* x_elem, const double* x_node
 }
 void launch_mask_constrained(int64_t n, const uint8_t* con, double* x, hipStream_t s) {
     hipLaunchKernelGGL(k_mask_constrained, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, con, x);
+}
+void launch_mf_elem(const CGArgs& c, const double* x, hipStream_t s) {
+    const unsigned ge = blocks_for(c.mf.n_elems, 256);
+    if (c.mf.n_classes == 1)
+        hipLaunchKernelGGL(k_hex8_mf_elem<true>, dim3(ge), dim3(256), 0, s, c, x);
+    else
+        hipLaunchKernelGGL(k_hex8_mf_elem<false>, dim3(ge), dim3(256), 0, s, c, x);
+}
+void launch_cg_op_alpha(const CGArgs& c, hipStream_t s) {
+    if (c.use_mf == 1)
+        mf_dispatch(c, c.p, c.p, c.q, 1, s);
+    else if (c.use_mf == 2)
+        nb32_dispatch(c, c.p, c.q, 1, 1, s);
+    else
+        spmv_dispatch(c, c.p, c.q, 1, 1, s);
+    hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(256), 0, s, c);
+}
+void launch_cg_p(const CGArgs& c, hipStream_t s) {
+    hipLaunchKernelGGL(k_cg_p, dim3(c.nvec), dim3(256), 0, s, c);
 }
 void launch_cg_iteration(const CGArgs& c, hipStream_t s) {
     if (c.use_mf == 1)

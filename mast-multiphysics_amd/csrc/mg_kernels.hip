@@ -1,0 +1,571 @@
+// Geometric multigrid for hex8 elasticity on lattice meshes: Galerkin coarse
+// element matrices, Chebyshev/Jacobi smoothing fused into the level operators,
+// trilinear restriction / prolongation, dense coarsest solve, and the
+// preconditioned CG kernels around the V-cycle.
+//
+// Every reduction is a fixed-order sum (no atomics), so a solve is bitwise
+// reproducible for a given launch configuration.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "kernels.h"
+#include "device_util.h"
+
+namespace mastg {
+
+// libMesh hex8 local node -> cell corner offset
+__constant__ int8_t kMGCorner[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+                                       {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+
+// trilinear weight of coarse corner A at fine local node a of child cell c
+// (c bit d = the child's offset along dimension d inside the coarse cell)
+__device__ __forceinline__ double mg_q(int c, int a, int A) {
+    double w = 1.0;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const int t = ((c >> d) & 1) + kMGCorner[a][d];   // 0, 1, 2 fine cells
+        w *= kMGCorner[A][d] ? 0.5 * t : 1.0 - 0.5 * t;
+    }
+    return w;
+}
+
+// ---- Galerkin coarse element matrices: K_E = sum_c Q_c^T K_c Q_c ---------
+// One wave per coarse element, 4 per block.  Fine blocks are masked (the rows
+// and columns of constrained fine DOFs removed, as P has no row for them), the
+// result is made exactly symmetric and gets the libMesh constraint convention
+// for constrained coarse DOFs (rows / columns zero, unit diagonal).
+__global__ __launch_bounds__(256) void k_mg_galerkin(MGGalerkinArgs G) {
+    __shared__ double sK[4][64 * 9];
+    __shared__ double sT[4][64 * 9];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int hi = lane >> 3, lo = lane & 7;
+    const int64_t E = int64_t(blockIdx.x) * 4 + wave;
+    const bool valid = E < G.n_coarse;
+    const int64_t Ev = valid ? E : 0;
+    const int64_t I = Ev % G.cnx, J = (Ev / G.cnx) % G.cny, K = Ev / (int64_t(G.cnx) * G.cny);
+    double acc[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) acc[i] = 0.0;
+    for (int c = 0; c < 8; c++) {
+        const int64_t fi = 2 * I + (c & 1), fj = 2 * J + ((c >> 1) & 1), fk = 2 * K + ((c >> 2) & 1);
+        const int64_t cell = fi + int64_t(G.fnx) * (fj + int64_t(G.fny) * fk);
+        const int64_t fe = G.f_elem_of_cell ? int64_t(G.f_elem_of_cell[cell]) : cell;
+        {   // masked fine block (a, b) = (hi, lo)
+            const int32_t ra = G.f_econn[fe * 8 + hi], rb = G.f_econn[fe * 8 + lo];
+            const double* kb = G.Kf + (fe * 64 + lane) * 9;
+#pragma unroll
+            for (int v = 0; v < 3; v++)
+#pragma unroll
+                for (int w = 0; w < 3; w++) {
+                    double x = kb[v * 3 + w];
+                    if (G.f_con[int64_t(ra) * 3 + v] || G.f_con[int64_t(rb) * 3 + w]) x = 0.0;
+                    sK[wave][lane * 9 + v * 3 + w] = x;
+                }
+        }
+        __syncthreads();
+        {   // T[a][B] = sum_b K[a][b] Q_c[b][B], lane = (a, B)
+            double t[9];
+#pragma unroll
+            for (int i = 0; i < 9; i++) t[i] = 0.0;
+            for (int b = 0; b < 8; b++) {
+                const double q = mg_q(c, b, lo);
+                if (q != 0.0) {
+                    const double* kb = &sK[wave][(hi * 8 + b) * 9];
+#pragma unroll
+                    for (int i = 0; i < 9; i++) t[i] += kb[i] * q;
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < 9; i++) sT[wave][lane * 9 + i] = t[i];
+        }
+        __syncthreads();
+        // K_E[A][B] += sum_a Q_c[a][A] T[a][B], lane = (A, B)
+        for (int a = 0; a < 8; a++) {
+            const double q = mg_q(c, a, hi);
+            if (q != 0.0) {
+                const double* tb = &sT[wave][(a * 8 + lo) * 9];
+#pragma unroll
+                for (int i = 0; i < 9; i++) acc[i] += q * tb[i];
+            }
+        }
+        __syncthreads();
+    }
+    // exact symmetry: lanes below the diagonal take the transpose of their partner
+#pragma unroll
+    for (int i = 0; i < 9; i++) sK[wave][lane * 9 + i] = acc[i];
+    __syncthreads();
+    double out[9];
+    if (hi > lo) {
+        const double* tb = &sK[wave][(lo * 8 + hi) * 9];
+#pragma unroll
+        for (int v = 0; v < 3; v++)
+#pragma unroll
+            for (int w = 0; w < 3; w++) out[v * 3 + w] = tb[w * 3 + v];
+    } else if (hi == lo) {
+#pragma unroll
+        for (int v = 0; v < 3; v++)
+#pragma unroll
+            for (int w = 0; w < 3; w++) out[v * 3 + w] = 0.5 * (acc[v * 3 + w] + acc[w * 3 + v]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < 9; i++) out[i] = acc[i];
+    }
+    if (!valid) return;
+    const int32_t cA = G.c_econn[E * 8 + hi], cB = G.c_econn[E * 8 + lo];
+    double* o = G.Kc + (E * 64 + lane) * 9;
+#pragma unroll
+    for (int v = 0; v < 3; v++)
+#pragma unroll
+        for (int w = 0; w < 3; w++) {
+            double x = out[v * 3 + w];
+            if (G.c_con[int64_t(cA) * 3 + v] || G.c_con[int64_t(cB) * 3 + w]) x = (hi == lo && v == w) ? 1.0 : 0.0;
+            o[v * 3 + w] = x;
+        }
+}
+
+// ---- row information: dinv and the Gershgorin bound of D^-1 A -------------
+__global__ __launch_bounds__(256) void k_mg_rowinfo_bsr(MGLevelArgs L, double* __restrict__ dinv,
+                                                        double* __restrict__ part) {
+    const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    double g = 0.0;
+    if (i < L.n_nodes * 3) {
+        const int64_t r = i / 3;
+        const int v = int(i - r * 3);
+        const int64_t k0 = L.blk_ptr[r];
+        const int64_t deg = L.blk_ptr[r + 1] - k0;
+        const double* row = L.val + k0 * 9 + int64_t(v) * deg * 3;
+        double sum = 0.0, diag = 0.0;
+        for (int64_t k = 0; k < deg; k++) {
+            const int32_t c = L.blk_col[k0 + k];
+#pragma unroll
+            for (int w = 0; w < 3; w++) {
+                const double a = row[k * 3 + w];
+                sum += fabs(a);
+                if (c == r && w == v) diag = a;
+            }
+        }
+        dinv[i] = diag > 0.0 ? 1.0 / diag : 0.0;
+        g = diag > 0.0 ? sum / diag : 0.0;
+    }
+    block_max_store(g, part, blockIdx.x);
+}
+
+__global__ __launch_bounds__(256) void k_mg_rowinfo_csr(int64_t n, const int64_t* __restrict__ row_ptr,
+                                                        const double* __restrict__ val,
+                                                        const int64_t* __restrict__ diag_pos,
+                                                        double* __restrict__ part) {
+    const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    double g = 0.0;
+    if (i < n) {
+        double sum = 0.0;
+        for (int64_t k = row_ptr[i]; k < row_ptr[i + 1]; k++) sum += fabs(val[k]);
+        const double diag = val[diag_pos[i]];
+        g = diag > 0.0 ? sum / diag : 0.0;
+    }
+    block_max_store(g, part, blockIdx.x);
+}
+
+__global__ __launch_bounds__(256) void k_mg_max(const double* __restrict__ part, int n, double* __restrict__ out) {
+    double m = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) m = fmax(m, part[i]);
+    block_max_store(m, out, 0);
+}
+
+// Chebyshev coefficients on [lam/ratio, lam] for every level
+__global__ void k_mg_cheb(const double* __restrict__ lam, double* __restrict__ cheb, int n_levels, int k,
+                          double ratio) {
+    const int l = threadIdx.x;
+    if (l >= n_levels) return;
+    const double b = lam[l] > 0.0 ? lam[l] : 1.0, a = b / ratio;
+    const double theta = 0.5 * (b + a), delta = 0.5 * (b - a), sigma = theta / delta;
+    double* c = cheb + l * kMGChebStride;
+    c[0] = 1.0 / theta;
+    c[1] = 0.0;
+    double rho = 1.0 / sigma;
+    for (int i = 1; i < k; i++) {
+        const double rn = 1.0 / (2.0 * sigma - rho);
+        c[2 * i] = rn * rho;
+        c[2 * i + 1] = 2.0 * rn / delta;
+        rho = rn;
+    }
+}
+
+// ---- smoothing ---------------------------------------------------------------
+__device__ __forceinline__ void mg_epilogue(const MGLevelArgs& L, int mode, int64_t i, double y,
+                                            const double* __restrict__ din, double* __restrict__ dout,
+                                            double c1, double c2, double ith) {
+    if (mode == MG_RESID_INIT) {
+        const double rs = L.r[i] - y;
+        L.res[i] = rs;
+        dout[i] = L.dinv[i] * rs * ith;
+    } else {
+        const double dv = din[i];
+        L.x[i] += dv;
+        const double rs = L.res[i] - y;
+        L.res[i] = rs;
+        if (mode == MG_STEP) dout[i] = c1 * dv + c2 * L.dinv[i] * rs;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_mg_pre_init(MGLevelArgs L, const int* __restrict__ done) {
+    if (done && *done) return;
+    const double ith = L.cheb[0];
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < L.n_nodes * 3; i += int64_t(gridDim.x) * 256) {
+        const double r = L.r[i];
+        L.x[i] = 0.0;
+        L.res[i] = r;
+        L.da[i] = L.dinv[i] * r * ith;
+    }
+}
+
+// node-block FP64 operator of a coarse level, 8 lanes per node, fused epilogue
+__global__ __launch_bounds__(256) void k_mg_bsr(MGLevelArgs L, int mode, int step, const double* __restrict__ din,
+                                                double* __restrict__ dout, const int* __restrict__ done) {
+    if (done && *done) return;
+    const int sub = threadIdx.x & 7;
+    const double* __restrict__ in = mode == MG_RESID_INIT ? L.x : din;
+    const double ith = L.cheb[0];
+    const double c1 = step > 0 ? L.cheb[2 * step] : 0.0, c2 = step > 0 ? L.cheb[2 * step + 1] : 0.0;
+    const int64_t nodes_per_grid = (int64_t(gridDim.x) * 256) / 8;
+    for (int64_t r = (int64_t(blockIdx.x) * 256 + threadIdx.x) / 8; r < L.n_nodes; r += nodes_per_grid) {
+        const int64_t k0 = L.blk_ptr[r];
+        const int deg = int(L.blk_ptr[r + 1] - k0);
+        const double* __restrict__ V = L.val + k0 * 9;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+        for (int k = sub; k < deg; k += 8) {
+            const int64_t cb = int64_t(L.blk_col[k0 + k]) * 3;
+            const double x0 = in[cb], x1 = in[cb + 1], x2 = in[cb + 2];
+            const double* v0 = V + int64_t(k) * 3;
+            const double* v1 = V + (int64_t(deg) + k) * 3;
+            const double* v2 = V + (int64_t(2 * deg) + k) * 3;
+            s0 += v0[0] * x0 + v0[1] * x1 + v0[2] * x2;
+            s1 += v1[0] * x0 + v1[1] * x1 + v1[2] * x2;
+            s2 += v2[0] * x0 + v2[1] * x1 + v2[2] * x2;
+        }
+#pragma unroll
+        for (int m = 4; m >= 1; m >>= 1) {
+            s0 += __shfl_xor(s0, m, 8);
+            s1 += __shfl_xor(s1, m, 8);
+            s2 += __shfl_xor(s2, m, 8);
+        }
+        if (sub < 3) {
+            const double y = sub == 0 ? s0 : (sub == 1 ? s1 : s2);
+            mg_epilogue(L, mode, r * 3 + sub, y, din, dout, c1, c2, ith);
+        }
+    }
+}
+
+// level 0: node phase of the matrix-free operator (element phase ran on `in`)
+// with the fused epilogue; constrained rows act as diag = incident elements
+__global__ __launch_bounds__(256) void k_mg_fine_node(CGArgs C, MGLevelArgs L, int mode, int step,
+                                                      const double* __restrict__ din, double* __restrict__ dout,
+                                                      const int* __restrict__ done) {
+    if (done && *done) return;
+    const MFArgs& A = C.mf;
+    const int64_t r = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (r >= A.n_nodes) return;
+    const double* __restrict__ in = mode == MG_RESID_INIT ? L.x : din;
+    const int32_t k0 = A.ninc_ptr[r], k1 = A.ninc_ptr[r + 1];
+    const int64_t ne = A.n_elems;
+    double y[3] = {0.0, 0.0, 0.0};
+    for (int32_t k = k0; k < k1; k++) {
+        const int32_t c = A.ninc[k];
+        const int64_t e = c >> 3;
+        const int la = c & 7;
+        y[0] += A.Fel[(la * 3 + 0) * ne + e];
+        y[1] += A.Fel[(la * 3 + 1) * ne + e];
+        y[2] += A.Fel[(la * 3 + 2) * ne + e];
+    }
+    const int m = A.nodecon[r];
+    const double cnt = double(k1 - k0);
+    const double ith = L.cheb[0];
+    const double c1 = step > 0 ? L.cheb[2 * step] : 0.0, c2 = step > 0 ? L.cheb[2 * step + 1] : 0.0;
+#pragma unroll
+    for (int v = 0; v < 3; v++) {
+        const double yv = (m >> v) & 1 ? cnt * in[r * 3 + v] : y[v];
+        mg_epilogue(L, mode, r * 3 + v, yv, din, dout, c1, c2, ith);
+    }
+}
+
+// ---- transfers ---------------------------------------------------------------
+// r_C = sum over the 27 fine points around 2C of w * res_f (w = prod (1 or 1/2)),
+// constrained fine DOFs skipped, constrained coarse DOFs zero (P^T)
+__global__ __launch_bounds__(256) void k_mg_restrict(MGLevelArgs F, MGLevelArgs Cl, const int* __restrict__ done) {
+    if (done && *done) return;
+    const int64_t p = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (p >= Cl.n_nodes) return;
+    const int64_t cx = Cl.nx + 1, cy = Cl.ny + 1;
+    const int64_t I = p % cx, J = (p / cx) % cy, K = p / (cx * cy);
+    const int64_t fx = F.nx + 1, fy = F.ny + 1;
+    double s[3] = {0.0, 0.0, 0.0};
+    for (int ok = -1; ok <= 1; ok++) {
+        const int64_t fk = 2 * K + ok;
+        if (fk < 0 || fk > F.nz) continue;
+        for (int oj = -1; oj <= 1; oj++) {
+            const int64_t fj = 2 * J + oj;
+            if (fj < 0 || fj > F.ny) continue;
+            for (int oi = -1; oi <= 1; oi++) {
+                const int64_t fi = 2 * I + oi;
+                if (fi < 0 || fi > F.nx) continue;
+                const double w = (ok ? 0.5 : 1.0) * (oj ? 0.5 : 1.0) * (oi ? 0.5 : 1.0);
+                const int64_t fp = fi + fx * (fj + fy * fk);
+                const int64_t rf = F.rank_of_lat ? int64_t(F.rank_of_lat[fp]) : fp;
+#pragma unroll
+                for (int v = 0; v < 3; v++)
+                    if (!F.con[rf * 3 + v]) s[v] += w * F.res[rf * 3 + v];
+            }
+        }
+    }
+    const int64_t rc = Cl.rank_of_lat ? int64_t(Cl.rank_of_lat[p]) : p;
+#pragma unroll
+    for (int v = 0; v < 3; v++) Cl.r[rc * 3 + v] = Cl.con[rc * 3 + v] ? 0.0 : s[v];
+}
+
+// x_f += P (x_C + dlast_C) at unconstrained fine DOFs (trilinear interpolation)
+__global__ __launch_bounds__(256) void k_mg_prolong(MGLevelArgs F, MGLevelArgs Cl, const double* __restrict__ dlast,
+                                                    const int* __restrict__ done) {
+    if (done && *done) return;
+    const int64_t p = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (p >= F.n_nodes) return;
+    const int64_t fx = F.nx + 1, fy = F.ny + 1;
+    const int64_t fi = p % fx, fj = (p / fx) % fy, fk = p / (fx * fy);
+    const int64_t cx = Cl.nx + 1, cy = Cl.ny + 1;
+    const int ni = (fi & 1) ? 2 : 1, nj = (fj & 1) ? 2 : 1, nk = (fk & 1) ? 2 : 1;
+    double s[3] = {0.0, 0.0, 0.0};
+    for (int c = 0; c < nk; c++) {
+        const int64_t K = (fk >> 1) + c;
+        for (int b = 0; b < nj; b++) {
+            const int64_t J = (fj >> 1) + b;
+            for (int a = 0; a < ni; a++) {
+                const int64_t I = (fi >> 1) + a;
+                const double w = (ni == 2 ? 0.5 : 1.0) * (nj == 2 ? 0.5 : 1.0) * (nk == 2 ? 0.5 : 1.0);
+                const int64_t cp = I + cx * (J + cy * K);
+                const int64_t rc = Cl.rank_of_lat ? int64_t(Cl.rank_of_lat[cp]) : cp;
+#pragma unroll
+                for (int v = 0; v < 3; v++) {
+                    double xv = Cl.x[rc * 3 + v];
+                    if (dlast) xv += dlast[rc * 3 + v];
+                    s[v] += w * xv;
+                }
+            }
+        }
+    }
+    const int64_t rf = F.rank_of_lat ? int64_t(F.rank_of_lat[p]) : p;
+#pragma unroll
+    for (int v = 0; v < 3; v++)
+        if (!F.con[rf * 3 + v]) F.x[rf * 3 + v] += s[v];
+}
+
+// ---- coarsest level: dense inverse -------------------------------------------
+__global__ __launch_bounds__(256) void k_mg_dense_gather(MGLevelArgs L, double* __restrict__ A) {
+    const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    const int64_t n = L.n_nodes * 3;
+    if (i >= n) return;
+    const int64_t r = i / 3;
+    const int v = int(i - r * 3);
+    const int64_t k0 = L.blk_ptr[r];
+    const int64_t deg = L.blk_ptr[r + 1] - k0;
+    const double* row = L.val + k0 * 9 + int64_t(v) * deg * 3;
+    for (int64_t k = 0; k < deg; k++) {
+        const int64_t c = L.blk_col[k0 + k];
+#pragma unroll
+        for (int w = 0; w < 3; w++) A[i * n + c * 3 + w] = row[k * 3 + w];
+    }
+}
+
+// in-place Gauss-Jordan inverse of an SPD matrix (no pivoting), one block
+__global__ __launch_bounds__(1024) void k_mg_invert(int n, double* __restrict__ A) {
+    __shared__ double colk[1536];
+    for (int k = 0; k < n; k++) {
+        for (int i = threadIdx.x; i < n; i += 1024) colk[i] = A[int64_t(i) * n + k];
+        __syncthreads();
+        const double p = 1.0 / colk[k];
+        for (int j = threadIdx.x; j < n; j += 1024)
+            A[int64_t(k) * n + j] = (j == k) ? p : A[int64_t(k) * n + j] * p;
+        __syncthreads();
+        const int64_t nn = int64_t(n) * n;
+        for (int64_t idx = threadIdx.x; idx < nn; idx += 1024) {
+            const int64_t i = idx / n, j = idx - i * n;
+            if (i == k) continue;
+            A[idx] = (j == k) ? -colk[i] * p : A[idx] - colk[i] * A[int64_t(k) * n + j];
+        }
+        __syncthreads();
+    }
+}
+
+// x = Ainv r, one wave per row
+__global__ __launch_bounds__(256) void k_mg_coarse_solve(int n, const double* __restrict__ Ainv,
+                                                         const double* __restrict__ r, double* __restrict__ x,
+                                                         const int* __restrict__ done) {
+    if (done && *done) return;
+    const int64_t i = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int j = lane; j < n; j += 64) s += Ainv[i * n + j] * r[j];
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) s += __shfl_xor(s, m, 64);
+    if (lane == 0) x[i] = s;
+}
+
+// ---- preconditioned CG ---------------------------------------------------------
+// x += alpha p ; r -= alpha q ; partials r.r
+__global__ __launch_bounds__(256) void k_pcg_update(CGArgs C) {
+    if (C.S->done) return;
+    const double a = C.S->alpha;
+    double rr = 0.0;
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < C.n; i += int64_t(gridDim.x) * 256) {
+        C.x[i] += a * C.p[i];
+        const double r = C.r[i] - a * C.q[i];
+        C.r[i] = r;
+        rr += r * r;
+    }
+    block_sum_store(rr, C.part1, blockIdx.x);
+}
+
+// convergence on the updated residual, before the V-cycle runs
+__global__ __launch_bounds__(256) void k_pcg_check(CGArgs C) {
+    if (C.S->done) return;
+    const double rr = reduce_partials(C.part1, C.nvec);
+    if (threadIdx.x == 0) {
+        CGScalars* S = C.S;
+        S->rr = rr;
+        S->iter += 1;
+        if (!isfinite(rr)) S->done = 4;
+        else if (rr <= S->tol2 * S->bb) S->done = 1;
+        else if (S->iter >= S->max_it) S->done = 2;
+    }
+}
+
+// z = x0 + dlast (the level-0 V-cycle output) ; partials r.z
+__global__ __launch_bounds__(256) void k_pcg_rz(CGArgs C, const double* __restrict__ x0,
+                                                const double* __restrict__ dlast) {
+    if (C.S->done) return;
+    double rz = 0.0;
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < C.n; i += int64_t(gridDim.x) * 256) {
+        const double z = dlast ? x0[i] + dlast[i] : x0[i];
+        C.z[i] = z;
+        rz += C.r[i] * z;
+    }
+    block_sum_store(rz, C.part0, blockIdx.x);
+}
+
+// beta = rz_new / rz ; a non-positive r.z means the preconditioner is not SPD
+__global__ __launch_bounds__(256) void k_pcg_beta(CGArgs C) {
+    if (C.S->done) return;
+    const double rz = reduce_partials(C.part0, C.nvec);
+    if (threadIdx.x == 0) {
+        CGScalars* S = C.S;
+        if (!(rz > 0.0) || !isfinite(rz)) { S->done = 3; return; }
+        S->beta = rz / S->rz;
+        S->rz = rz;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_pcg_start(CGArgs C, const double* __restrict__ b) {
+    double bb = 0.0;
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < C.n; i += int64_t(gridDim.x) * 256)
+        bb += b[i] * b[i];
+    block_sum_store(bb, C.part1, blockIdx.x);
+}
+
+__global__ __launch_bounds__(256) void k_pcg_start_scalars(CGArgs C, double tol2, int max_it) {
+    const double bb = reduce_partials(C.part1, C.nvec);
+    if (threadIdx.x == 0) {
+        CGScalars* S = C.S;
+        S->bb = bb;
+        S->rr = bb;
+        S->rz = 0.0;
+        S->iter = 0;
+        S->tol2 = tol2;
+        S->max_it = max_it;
+        S->done = (bb == 0.0) ? 1 : (isfinite(bb) ? 0 : 4);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_pcg_start2(CGArgs C) {
+    if (C.S->done) return;
+    const double rz = reduce_partials(C.part0, C.nvec);
+    if (threadIdx.x == 0) {
+        if (!(rz > 0.0) || !isfinite(rz)) C.S->done = 3;
+        C.S->rz = rz;
+    }
+}
+
+// ---- launchers -------------------------------------------------------------------
+static inline unsigned mg_blocks(int64_t n, int per) { return unsigned((n + per - 1) / per); }
+static inline unsigned mg_grid(int64_t n) {
+    const int64_t b = (n + 255) / 256;
+    return unsigned(b < 1 ? 1 : (b > 4096 ? 4096 : b));
+}
+
+void launch_mg_galerkin(const MGGalerkinArgs& g, hipStream_t s) {
+    if (g.n_coarse <= 0) return;
+    hipLaunchKernelGGL(k_mg_galerkin, dim3(mg_blocks(g.n_coarse, 4)), dim3(256), 0, s, g);
+}
+void launch_mg_rowinfo_bsr(const MGLevelArgs& L, double* dinv, double* part, double* lam, hipStream_t s) {
+    const unsigned nb = mg_blocks(L.n_nodes * 3, 256);
+    hipLaunchKernelGGL(k_mg_rowinfo_bsr, dim3(nb), dim3(256), 0, s, L, dinv, part);
+    hipLaunchKernelGGL(k_mg_max, dim3(1), dim3(256), 0, s, part, int(nb), lam);
+}
+void launch_mg_rowinfo_csr(int64_t n, const int64_t* row_ptr, const double* val, const int64_t* diag_pos,
+                           double* part, double* lam, hipStream_t s) {
+    const unsigned nb = mg_blocks(n, 256);
+    hipLaunchKernelGGL(k_mg_rowinfo_csr, dim3(nb), dim3(256), 0, s, n, row_ptr, val, diag_pos, part);
+    hipLaunchKernelGGL(k_mg_max, dim3(1), dim3(256), 0, s, part, int(nb), lam);
+}
+void launch_mg_cheb(const double* lam, double* cheb, int n_levels, int k, double ratio, hipStream_t s) {
+    hipLaunchKernelGGL(k_mg_cheb, dim3(1), dim3(64), 0, s, lam, cheb, n_levels, k, ratio);
+}
+void launch_mg_dense_gather(const MGLevelArgs& L, double* A, hipStream_t s) {
+    hipLaunchKernelGGL(k_mg_dense_gather, dim3(mg_blocks(L.n_nodes * 3, 256)), dim3(256), 0, s, L, A);
+}
+void launch_mg_invert(int n, double* A, hipStream_t s) {
+    hipLaunchKernelGGL(k_mg_invert, dim3(1), dim3(1024), 0, s, n, A);
+}
+void launch_mg_pre_init(const MGLevelArgs& L, const int* done, hipStream_t s) {
+    hipLaunchKernelGGL(k_mg_pre_init, dim3(mg_grid(L.n_nodes * 3)), dim3(256), 0, s, L, done);
+}
+void launch_mg_bsr(const MGLevelArgs& L, int mode, int step, const double* din, double* dout, const int* done,
+                   hipStream_t s) {
+    const int64_t blocks = (L.n_nodes * 8 + 255) / 256;
+    const unsigned g = unsigned(blocks < 1 ? 1 : (blocks > 8192 ? 8192 : blocks));
+    hipLaunchKernelGGL(k_mg_bsr, dim3(g), dim3(256), 0, s, L, mode, step, din, dout, done);
+}
+void launch_mg_fine(const CGArgs& c, const MGLevelArgs& L, int mode, int step, const double* din,
+                    double* dout, const int* done, hipStream_t s) {
+    const double* in = mode == MG_RESID_INIT ? L.x : din;
+    launch_mf_elem(c, in, s);
+    hipLaunchKernelGGL(k_mg_fine_node, dim3(mg_blocks(c.mf.n_nodes, 256)), dim3(256), 0, s, c, L, mode, step, din,
+                       dout, done);
+}
+void launch_mg_restrict(const MGLevelArgs& F, const MGLevelArgs& C, const int* done, hipStream_t s) {
+    hipLaunchKernelGGL(k_mg_restrict, dim3(mg_blocks(C.n_nodes, 256)), dim3(256), 0, s, F, C, done);
+}
+void launch_mg_prolong(const MGLevelArgs& F, const MGLevelArgs& C, const double* dlast, const int* done,
+                       hipStream_t s) {
+    hipLaunchKernelGGL(k_mg_prolong, dim3(mg_blocks(F.n_nodes, 256)), dim3(256), 0, s, F, C, dlast, done);
+}
+void launch_mg_coarse_solve(int n, const double* Ainv, const double* r, double* x, const int* done,
+                            hipStream_t s) {
+    hipLaunchKernelGGL(k_mg_coarse_solve, dim3(mg_blocks(n, 4)), dim3(256), 0, s, n, Ainv, r, x, done);
+}
+void launch_pcg_update(const CGArgs& c, hipStream_t s) {
+    hipLaunchKernelGGL(k_pcg_update, dim3(c.nvec), dim3(256), 0, s, c);
+    hipLaunchKernelGGL(k_pcg_check, dim3(1), dim3(256), 0, s, c);
+}
+void launch_pcg_rz(const CGArgs& c, const double* x0, const double* dlast, hipStream_t s) {
+    hipLaunchKernelGGL(k_pcg_rz, dim3(c.nvec), dim3(256), 0, s, c, x0, dlast);
+}
+void launch_pcg_beta(const CGArgs& c, hipStream_t s) {
+    hipLaunchKernelGGL(k_pcg_beta, dim3(1), dim3(256), 0, s, c);
+}
+void launch_pcg_start(const CGArgs& c, const double* b, double tol2, int max_it, hipStream_t s) {
+    hipLaunchKernelGGL(k_pcg_start, dim3(c.nvec), dim3(256), 0, s, c, b);
+    hipLaunchKernelGGL(k_pcg_start_scalars, dim3(1), dim3(256), 0, s, c, tol2, max_it);
+}
+void launch_pcg_start2(const CGArgs& c, hipStream_t s) {
+    hipLaunchKernelGGL(k_pcg_start2, dim3(1), dim3(256), 0, s, c);
+}
+
+}  // namespace mastg

@@ -1,0 +1,43 @@
+// Geometric multigrid hierarchy for hex8 elasticity on lattice meshes (host).
+//
+// A mesh qualifies when its nodes form a complete (nx+1)(ny+1)(nz+1) lattice of
+// axis-aligned boxes and every element is one lattice cell in libMesh hex8
+// node order.  Levels halve every dimension while all three stay even; level
+// l+1's nodes are level l's nodes at even lattice indices.  Prolongation is
+// trilinear interpolation (a fine node interpolates from the corners of the
+// coarse cell containing it), coarse operators are Galerkin products
+// P^T A P formed element by element (exact for nested cells), and a coarse
+// DOF is constrained when the fine DOF at the same point is.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "structure.h"
+
+namespace mastg {
+
+struct MGLevelHost {
+    int nx = 0, ny = 0, nz = 0;          // cells per dimension
+    int64_t n_nodes = 0, n_elems = 0;
+    // node rank of lattice point (i + (nx+1)(j + (ny+1) k)); identity on coarse levels
+    std::vector<int32_t> rank_of_lat;
+    // element of lattice cell (i + nx (j + ny k)); identity on coarse levels
+    std::vector<int32_t> elem_of_cell;
+    Structure S;                         // coarse levels only (level 0 is the context's)
+};
+
+struct MGHierarchy {
+    bool ok = false;
+    std::string why;                     // reason when !ok
+    std::vector<MGLevelHost> levels;     // [0] = the problem mesh
+    int64_t coarsest_dofs = 0;
+};
+
+// Largest coarsest-level system solved by the dense inverse.
+constexpr int64_t kMGMaxCoarseDofs = 1536;
+
+// Detects the lattice and builds the coarse levels (hex8 elasticity, nav = 3).
+MGHierarchy build_mg_hierarchy(const Structure& S);
+
+}  // namespace mastg

@@ -14,7 +14,7 @@ LIB_PATH = os.path.normpath(os.path.join(_HERE, "..", "libmastg.so"))
 HEX8_ELASTIC, QUAD4_MINDLIN, HEX8_HEAT = 1, 2, 3
 HOST, DEVICE = 0, 1
 ELEM_FP64, ELEM_FP32 = 0, 1
-SOLVER_CG_JACOBI, SOLVER_MIXED, SOLVER_BICGSTAB, SOLVER_MATFREE = 0, 1, 2, 3
+SOLVER_CG_JACOBI, SOLVER_MIXED, SOLVER_BICGSTAB, SOLVER_MATFREE, SOLVER_MG = 0, 1, 2, 3, 4
 OUTPUT_COMPLIANCE, OUTPUT_MEAN_T = 1, 2
 OPT_GRAPHS = 1
 ERRORS = {0: "ok", -1: "invalid", -2: "out of memory", -3: "not converged", -4: "nan",
