@@ -158,8 +158,8 @@ def main():
     ap.add_argument("--c5-requests", type=int, default=256)
     ap.add_argument("--c5-rate", type=float, default=20.0)
     ap.add_argument("--c5-slots", type=int, default=6)
-    ap.add_argument("--solver", default="all", choices=["plain", "mixed", "matfree", "both", "all"],
-                    help="both = plain + matfree; all = plain + mixed + matfree")
+    ap.add_argument("--solver", default="all", choices=["plain", "mixed", "matfree", "mg", "both", "all"],
+                    help="both = plain + matfree; all = plain + mixed + matfree + mg")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-graphs", action="store_true",
@@ -183,8 +183,8 @@ def main():
     print(f"# workload {a.workload}: {w.n_elems} elems, {ctx.n_active} active DOF, nnz {ctx.nnz}, "
           f"setup {t_setup:.1f}s", file=sys.stderr, flush=True)
     solvers = {"plain": mastg.SOLVER_CG_JACOBI, "mixed": mastg.SOLVER_MIXED,
-               "matfree": mastg.SOLVER_MATFREE}
-    order = {"both": ["plain", "matfree"], "all": ["plain", "mixed", "matfree"]}.get(a.solver, [a.solver])
+               "matfree": mastg.SOLVER_MATFREE, "mg": mastg.SOLVER_MG}
+    order = {"both": ["plain", "matfree"], "all": ["plain", "mixed", "matfree", "mg"]}.get(a.solver, [a.solver])
     runs = {}
     for name in order:
         sv = solvers[name]
@@ -206,7 +206,7 @@ def main():
         ctx._chk(mastg.lib().mastg_synchronize(ctx.h))
         elapsed = time.perf_counter() - t0
         runs[name] = dict(elapsed=elapsed, lat=lat, res=res, dps=a.steps / elapsed)
-    head = "matfree" if "matfree" in runs else ("mixed" if "mixed" in runs else "plain")
+    head = next((h for h in ("mg", "matfree", "mixed", "plain") if h in runs))
     H = runs[head]
     res = H["res"]
     lat = H["lat"]
@@ -217,7 +217,7 @@ def main():
     lanes, grid = ctx.tuning()
 
     # roofline of the dominant kernel of the headline path
-    if head != "matfree":
+    if head not in ("matfree", "mg"):
         k_ms = ctx.bench_kernel(2, 20)
         byt = spmv_bytes(ctx.n_active, ctx.nnz)
         kname = f"k_spmv_csr_dot<{lanes}> (FP64 CSR SpMV)"

@@ -395,8 +395,8 @@ int autotune(mastg_ctx* c);
 int cg_finish(mastg_ctx* c, const CGArgs& a, const double* b, hipEvent_t e0, hipEvent_t e1,
               mastg_solve_stats* st, int done);
 
-int cg_mixed_loop(mastg_ctx* c, const CGArgs& a, const double* b, const mastg_solve_opts* o, int chunk,
-                  int max_it, hipEvent_t e0, hipEvent_t e1, mastg_solve_stats* st);
+int cg_loop(mastg_ctx* c, const CGArgs& a, const double* b, const mastg_solve_opts* o, int op, int chunk,
+            hipEvent_t e0, hipEvent_t e1, mastg_solve_stats* st);
 
 // op: 0 FP64 CSR SpMV, 1 matrix-free FP64, 2 mixed precision (FP32 node-block
 // SpMV inside CG whose residual is replaced by the FP64 true residual b - A x
@@ -421,7 +421,6 @@ int cg_solve(mastg_ctx* c, const double* b, const mastg_solve_opts* o, mastg_sol
     CGArgs a = cg_args(c);
     a.use_mf = op;
     if (matfree) a.nparts_op = mf_node_blocks(S.n_nodes);
-    const int gi = op;
     c->cg_replacements = 0;
     hipEvent_t e0, e1;
     HIPC(c, hipEventCreate(&e0));
@@ -430,71 +429,47 @@ int cg_solve(mastg_ctx* c, const double* b, const mastg_solve_opts* o, mastg_sol
     launch_fill(S.n_active, 0.0, a.x, c->stream);
     HIPC(c, hipMemcpyAsync(a.r, b, S.n_active * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     launch_cg_start(a, b, rtol * rtol, max_it, c->stream);
-    if (op == 2) return cg_mixed_loop(c, a, b, o, chunk, max_it, e0, e1, st);
-    if (!c->use_graphs) {
-        // direct launches, polled every `chunk` iterations
-        int done = 0;
-        for (int launched = 0; launched < max_it + chunk; launched += chunk) {
-            for (int i = 0; i < chunk; i++) launch_cg_iteration(a, c->stream);
-            HIPC(c, hipMemcpyAsync(c->h_scal, a.S, sizeof(CGScalars), hipMemcpyDeviceToHost, c->stream));
-            HIPC(c, hipStreamSynchronize(c->stream));
-            done = c->h_scal->done;
-            if (done) break;
-        }
-        return cg_finish(c, a, b, e0, e1, st, done);
-    }
-    if (!c->cg_exec[gi] || c->cg_exec_len[gi] != chunk) {
-        if (c->cg_exec[gi]) (void)hipGraphExecDestroy(c->cg_exec[gi]);
-        c->cg_exec[gi] = nullptr;
-        hipGraph_t g;
-        HIPC(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        for (int i = 0; i < chunk; i++) launch_cg_iteration(a, c->stream);
-        HIPC(c, hipStreamEndCapture(c->stream, &g));
-        HIPC(c, hipGraphInstantiate(&c->cg_exec[gi], g, nullptr, nullptr, 0));
-        (void)hipGraphDestroy(g);
-        c->cg_exec_len[gi] = chunk;
-    }
-    int done = 0;
-    for (int launched = 0; launched < max_it + chunk; launched += chunk) {
-        HIPC(c, hipGraphLaunch(c->cg_exec[gi], c->stream));
-        HIPC(c, hipMemcpyAsync(c->h_scal, a.S, sizeof(CGScalars), hipMemcpyDeviceToHost, c->stream));
-        HIPC(c, hipStreamSynchronize(c->stream));
-        done = c->h_scal->done;
-        if (done) break;
-    }
-    return cg_finish(c, a, b, e0, e1, st, done);
+    return cg_loop(c, a, b, o, op, chunk, e0, e1, st);
 }
 
-int cg_mixed_loop(mastg_ctx* c, const CGArgs& a, const double* b, const mastg_solve_opts* o, int chunk,
-                  int max_it, hipEvent_t e0, hipEvent_t e1, mastg_solve_stats* st) {
+// Iterations run in chunks (hipGraph replays or direct launches) between host
+// polls of the device scalars.  The recursive residual drifts from b - A x on
+// ill-conditioned systems (and with the FP32 operator), so convergence is
+// confirmed on the FP64 true residual: when it misses the tolerance it
+// replaces the recursive residual (search direction kept) and CG continues.
+// The mixed-precision operator also replaces whenever ||r|| has dropped by
+// reliable_delta.  Eight replacements in a row that fail to halve the best
+// true residual end the solve as not converged (stagnation).
+int cg_loop(mastg_ctx* c, const CGArgs& a, const double* b, const mastg_solve_opts* o, int op, int chunk,
+            hipEvent_t e0, hipEvent_t e1, mastg_solve_stats* st) {
     const Structure& S = c->S;
     const double delta = (o && o->reliable_delta > 0) ? o->reliable_delta : 0.1;
-    if (c->use_graphs && (!c->cg_exec[2] || c->cg_exec_len[2] != chunk)) {
-        if (c->cg_exec[2]) (void)hipGraphExecDestroy(c->cg_exec[2]);
-        c->cg_exec[2] = nullptr;
+    if (c->use_graphs && (!c->cg_exec[op] || c->cg_exec_len[op] != chunk)) {
+        if (c->cg_exec[op]) (void)hipGraphExecDestroy(c->cg_exec[op]);
+        c->cg_exec[op] = nullptr;
         hipGraph_t g;
         HIPC(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         for (int i = 0; i < chunk; i++) launch_cg_iteration(a, c->stream);
         HIPC(c, hipStreamEndCapture(c->stream, &g));
-        HIPC(c, hipGraphInstantiate(&c->cg_exec[2], g, nullptr, nullptr, 0));
+        HIPC(c, hipGraphInstantiate(&c->cg_exec[op], g, nullptr, nullptr, 0));
         (void)hipGraphDestroy(g);
-        c->cg_exec_len[2] = chunk;
+        c->cg_exec_len[op] = chunk;
     }
     HIPC(c, hipMemcpyAsync(c->h_scal, a.S, sizeof(CGScalars), hipMemcpyDeviceToHost, c->stream));
     HIPC(c, hipStreamSynchronize(c->stream));
     int done = c->h_scal->done;
     double rref2 = c->h_scal->rr;   // ||r||^2 at the last replacement (start: ||b||^2)
-    for (int launched = 0; !done && launched < max_it + chunk; launched += chunk) {
-        if (c->use_graphs) HIPC(c, hipGraphLaunch(c->cg_exec[2], c->stream));
+    double best2 = rref2;
+    int stalls = 0;
+    while (!done) {
+        if (c->use_graphs) HIPC(c, hipGraphLaunch(c->cg_exec[op], c->stream));
         else
             for (int i = 0; i < chunk; i++) launch_cg_iteration(a, c->stream);
         HIPC(c, hipMemcpyAsync(c->h_scal, a.S, sizeof(CGScalars), hipMemcpyDeviceToHost, c->stream));
         HIPC(c, hipStreamSynchronize(c->stream));
         done = c->h_scal->done;
-        if (done == 3 || done == 4) break;
-        if (done == 1 || done == 2 || c->h_scal->rr <= delta * delta * rref2) {
-            // the recursive residual of the FP32 operator drifts from b - A x:
-            // replace it by the FP64 true residual (search direction kept)
+        if (done >= 2) break;
+        if (done == 1 || (op == 2 && c->h_scal->rr <= delta * delta * rref2)) {
             launch_spmv_csr(a, a.x, c->vtmp.as<double>(), 0, c->stream);
             launch_axpby(S.n_active, 1.0, b, -1.0, c->vtmp.as<double>(), c->stream);
             launch_cg_replace(a, c->vtmp.as<double>(), c->stream);
@@ -503,6 +478,11 @@ int cg_mixed_loop(mastg_ctx* c, const CGArgs& a, const double* b, const mastg_so
             c->cg_replacements++;
             done = c->h_scal->done;
             rref2 = c->h_scal->rr;
+            if (!done) {
+                if (rref2 < 0.25 * best2) stalls = 0;
+                else if (++stalls >= 8) { done = 5; break; }
+                best2 = std::min(best2, rref2);
+            }
         }
     }
     return cg_finish(c, a, b, e0, e1, st, done);
@@ -735,6 +715,7 @@ int cg_finish(mastg_ctx* c, const CGArgs& a, const double* b, hipEvent_t e0, hip
         st->seconds = ms * 1e-3;
     }
     if (done == 3 || done == 4) return fail(c, MASTG_ERR_NAN, "CG breakdown or non-finite iterate");
+    if (done == 5) return fail(c, MASTG_ERR_NOT_CONVERGED, "CG stagnated above the tolerance");
     if (done != 1) return fail(c, MASTG_ERR_NOT_CONVERGED, "CG did not reach the tolerance");
     return MASTG_OK;
 }
@@ -1246,8 +1227,10 @@ static int check_cg_solver(mastg_ctx* c, int solver) {
     if (solver == MASTG_SOLVER_MG) return mg_build(c);
     if (solver == MASTG_SOLVER_MATFREE && (c->S.n_classes == 0 || c->d.physics != MASTG_HEX8_ELASTIC))
         return fail(c, MASTG_ERR_INVALID, "matrix-free solver needs hex8 elasticity on a mesh of at most 8 element shapes");
-    if (solver == MASTG_SOLVER_MIXED && c->d.physics == MASTG_HEX8_HEAT)
-        return fail(c, MASTG_ERR_INVALID, "mixed-precision CG is for the symmetric structural Jacobians");
+    if (solver == MASTG_SOLVER_MIXED && c->d.physics != MASTG_HEX8_ELASTIC)
+        return fail(c, MASTG_ERR_INVALID,
+                    "mixed-precision CG is for hex8 elasticity (the plate Jacobian's h^3 bending vs h membrane "
+                    "conditioning is beyond what an FP32 operator resolves)");
     return MASTG_OK;
 }
 

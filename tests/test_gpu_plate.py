@@ -67,17 +67,18 @@ def test_plate_design_eval(oracle, mastg, n):
     ctx.close()
 
 
-def test_plate_design_eval_mixed(oracle, mastg):
-    """Mixed-precision CG (FP32 node-block SpMV over the 5x5 plate blocks,
-    FP64 residual replacement) reaches the FP64 paths' bars."""
-    print("[plate design eval mixed] 24x24", flush=True)
+def test_plate_mixed_refused_fp32_operator_accurate(oracle, mastg):
+    """The mixed-precision solver is refused for plates (thin-plate bending
+    ~h^3 against membrane ~h puts the Jacobian's conditioning beyond an FP32
+    operator); the FP32 node-block operator itself (5x5 blocks) is still
+    within FP32 rounding of the FP64 product."""
+    print("[plate mixed refused, FP32 operator] 24x24", flush=True)
     P = oracle.Problem(oracle.PLATE, 24, 24, seed=1)
-    ref = P.solve_elastic(1e-10)
     ctx = ctx_from_oracle(mastg, P)
-    r = ctx.design_eval(P.params(), solver=mastg.SOLVER_MIXED)
-    assert r["rel_res"] <= 1.5e-10
-    assert abs(r["objective"] - ref["compliance"]) <= 1e-9 * abs(ref["compliance"])
-    assert rel_err_floor(r["grad"], ref["dc"], floor=1e-6) <= 1e-7
+    with pytest.raises(mastg.MastgError) as e:
+        ctx.design_eval(P.params(), solver=mastg.SOLVER_MIXED)
+    assert e.value.code == -1
+    ctx.assemble()
     x = np.random.default_rng(1).standard_normal(P.n_dofs)
     y64 = ctx.jacobian_product(x, op=0)
     y32 = ctx.jacobian_product(x, op=2)

@@ -77,7 +77,7 @@ def test_admission_and_eviction(oracle, mastg):
     outs = [s.wait(i) for i in ids]
     st = s.stats()
     s.close()
-    assert all(o["status"] == 2 for o in outs)
+    assert all(o["status"] == 2 for o in outs), [o["status"] for o in outs]
     assert st["bytes_in_use"] <= big + small // 2
 
 
