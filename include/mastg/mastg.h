@@ -268,6 +268,8 @@ int mastg_server_poll(mastg_server* s, int64_t id, mastg_request_result* res);
 int mastg_server_wait(mastg_server* s, int64_t id, mastg_request_result* res);
 int mastg_server_stats(mastg_server* s, int64_t* bytes_in_use, int32_t* n_contexts, int32_t* queued);
 int mastg_server_destroy(mastg_server* s);   /* drains the queue first */
+/* message of the most recent failed request ("" if none) */
+const char* mastg_server_last_error(mastg_server* s);
 int mastg_request_footprint(const mastg_request* req, int64_t* bytes);
 
 /* ------------------------------------------------ benchmark workloads -- */

@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -23,12 +24,24 @@ namespace {
 
 thread_local std::string g_create_error;
 
+// Stream capture and the implicitly synchronising HIP calls (allocation, free,
+// synchronous copies, stream / event creation) are serialised process-wide:
+// contexts driven from several threads (the request server) must never
+// allocate while another thread captures a solver graph.
+std::recursive_mutex g_hip_mu;
+using HipLock = std::lock_guard<std::recursive_mutex>;
+inline hipError_t ev_create(hipEvent_t* e) { HipLock l(g_hip_mu); return hipEventCreate(e); }
+inline hipError_t ev_destroy(hipEvent_t e) { HipLock l(g_hip_mu); return hipEventDestroy(e); }
+
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
     ~DevBuf() { release(); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            HipLock l(g_hip_mu);
+            (void)hipFree(p);
+        }
         p = nullptr;
         bytes = 0;
     }
@@ -127,7 +140,11 @@ int dalloc(mastg_ctx* c, DevBuf& b, size_t bytes) {
     c->device_bytes -= int64_t(b.bytes);
     b.release();
     if (bytes == 0) bytes = 8;
-    hipError_t e = hipMalloc(&b.p, bytes);
+    hipError_t e;
+    {
+        HipLock l(g_hip_mu);
+        e = hipMalloc(&b.p, bytes);
+    }
     if (e != hipSuccess) {
         b.p = nullptr;
         return fail(c, MASTG_ERR_OOM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
@@ -141,7 +158,10 @@ template <class T>
 int upload(mastg_ctx* c, DevBuf& b, const std::vector<T>& v) {
     int rc = dalloc(c, b, v.size() * sizeof(T));
     if (rc) return rc;
-    if (!v.empty()) HIPC(c, hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    if (!v.empty()) {
+        HipLock l(g_hip_mu);
+        HIPC(c, hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    }
     return MASTG_OK;
 }
 
@@ -423,8 +443,8 @@ int cg_solve(mastg_ctx* c, const double* b, const mastg_solve_opts* o, mastg_sol
     if (matfree) a.nparts_op = mf_node_blocks(S.n_nodes);
     c->cg_replacements = 0;
     hipEvent_t e0, e1;
-    HIPC(c, hipEventCreate(&e0));
-    HIPC(c, hipEventCreate(&e1));
+    HIPC(c, ev_create(&e0));
+    HIPC(c, ev_create(&e1));
     HIPC(c, hipEventRecord(e0, c->stream));
     launch_fill(S.n_active, 0.0, a.x, c->stream);
     HIPC(c, hipMemcpyAsync(a.r, b, S.n_active * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
@@ -436,9 +456,9 @@ int cg_solve(mastg_ctx* c, const double* b, const mastg_solve_opts* o, mastg_sol
 // polls of the device scalars.  The recursive residual drifts from b - A x on
 // ill-conditioned systems (and with the FP32 operator), so convergence is
 // confirmed on the FP64 true residual: when it misses the tolerance it
-// replaces the recursive residual (search direction kept) and CG continues.
-// The mixed-precision operator also replaces whenever ||r|| has dropped by
-// reliable_delta.  Eight replacements in a row that fail to halve the best
+// replaces the recursive residual and CG restarts from there.  The
+// mixed-precision operator also replaces whenever ||r|| has dropped by
+// reliable_delta (search direction kept: reliable updates).  Eight replacements in a row that fail to halve the best
 // true residual end the solve as not converged (stagnation).
 int cg_loop(mastg_ctx* c, const CGArgs& a, const double* b, const mastg_solve_opts* o, int op, int chunk,
             hipEvent_t e0, hipEvent_t e1, mastg_solve_stats* st) {
@@ -447,6 +467,7 @@ int cg_loop(mastg_ctx* c, const CGArgs& a, const double* b, const mastg_solve_op
     if (c->use_graphs && (!c->cg_exec[op] || c->cg_exec_len[op] != chunk)) {
         if (c->cg_exec[op]) (void)hipGraphExecDestroy(c->cg_exec[op]);
         c->cg_exec[op] = nullptr;
+        HipLock capture_lock(g_hip_mu);
         hipGraph_t g;
         HIPC(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         for (int i = 0; i < chunk; i++) launch_cg_iteration(a, c->stream);
@@ -472,7 +493,10 @@ int cg_loop(mastg_ctx* c, const CGArgs& a, const double* b, const mastg_solve_op
         if (done == 1 || (op == 2 && c->h_scal->rr <= delta * delta * rref2)) {
             launch_spmv_csr(a, a.x, c->vtmp.as<double>(), 0, c->stream);
             launch_axpby(S.n_active, 1.0, b, -1.0, c->vtmp.as<double>(), c->stream);
-            launch_cg_replace(a, c->vtmp.as<double>(), c->stream);
+            // a confirmation replacement restarts CG from the true residual
+            // (p = z): keeping a direction that is not conjugate to the new
+            // residual diverges on ill-conditioned plates
+            launch_cg_replace(a, c->vtmp.as<double>(), done == 1 ? 1 : 0, c->stream);
             HIPC(c, hipMemcpyAsync(c->h_scal, a.S, sizeof(CGScalars), hipMemcpyDeviceToHost, c->stream));
             HIPC(c, hipStreamSynchronize(c->stream));
             c->cg_replacements++;
@@ -652,8 +676,8 @@ int mg_solve(mastg_ctx* c, const double* b, const mastg_solve_opts* o, mastg_sol
         c->sE_valid = true;
     }
     hipEvent_t e0, e1;
-    HIPC(c, hipEventCreate(&e0));
-    HIPC(c, hipEventCreate(&e1));
+    HIPC(c, ev_create(&e0));
+    HIPC(c, ev_create(&e1));
     HIPC(c, hipEventRecord(e0, c->stream));
     TRY(mg_setup(c));
     CGArgs a = cg_args(c);
@@ -673,6 +697,7 @@ int mg_solve(mastg_ctx* c, const double* b, const mastg_solve_opts* o, mastg_sol
     if (c->use_graphs && !done && (!c->mg_exec || c->mg_exec_len != chunk)) {
         if (c->mg_exec) (void)hipGraphExecDestroy(c->mg_exec);
         c->mg_exec = nullptr;
+        HipLock capture_lock(g_hip_mu);
         hipGraph_t g;
         HIPC(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         for (int i = 0; i < chunk; i++) mg_pcg_iteration(c, a, c->stream);
@@ -706,8 +731,8 @@ int cg_finish(mastg_ctx* c, const CGArgs& a, const double* b, hipEvent_t e0, hip
     TRY(dot_active(c, b, b, &bb));
     float ms = 0;
     HIPC(c, hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    (void)ev_destroy(e0);
+    (void)ev_destroy(e1);
     if (st) {
         st->iters = c->h_scal->iter;
         st->replacements = c->cg_replacements;
@@ -766,8 +791,8 @@ int bicg_solve(mastg_ctx* c, const double* b, const mastg_solve_opts* o, mastg_s
     }
     BiCGArgs a = bicg_args(c, transpose);
     hipEvent_t e0, e1;
-    HIPC(c, hipEventCreate(&e0));
-    HIPC(c, hipEventCreate(&e1));
+    HIPC(c, ev_create(&e0));
+    HIPC(c, ev_create(&e1));
     HIPC(c, hipEventRecord(e0, c->stream));
     launch_bicg_start(a, b, rtol * rtol, max_it, c->stream);
     int done = 0;
@@ -775,7 +800,8 @@ int bicg_solve(mastg_ctx* c, const double* b, const mastg_solve_opts* o, mastg_s
         if (!c->bicg_exec || c->bicg_exec_len != chunk || c->bicg_exec_T != transpose) {
             if (c->bicg_exec) (void)hipGraphExecDestroy(c->bicg_exec);
             c->bicg_exec = nullptr;
-            hipGraph_t g;
+            HipLock capture_lock(g_hip_mu);
+        hipGraph_t g;
             HIPC(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
             for (int i = 0; i < chunk; i++) launch_bicg_iteration(a, c->stream);
             HIPC(c, hipStreamEndCapture(c->stream, &g));
@@ -807,8 +833,8 @@ int bicg_solve(mastg_ctx* c, const double* b, const mastg_solve_opts* o, mastg_s
     TRY(dot_active(c, b, b, &bb));
     float ms = 0;
     HIPC(c, hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    (void)ev_destroy(e0);
+    (void)ev_destroy(e1);
     if (st) {
         st->iters = hs.iter;
         st->replacements = 0;
@@ -861,7 +887,7 @@ int heat_design_eval(mastg_ctx* c, double* dobj, int where, const mastg_solve_op
     const Structure& S = c->S;
     const double newton_rtol = (o && o->rtol > 0) ? o->rtol : 1e-10;
     hipEvent_t e[4];
-    for (auto& x : e) HIPC(c, hipEventCreate(&x));
+    for (auto& x : e) HIPC(c, ev_create(&x));
     double t_asm = 0, t_solve = 0;
     float ms = 0;
     launch_fill(S.n_active, 0.0, c->vT.as<double>(), c->stream);
@@ -927,7 +953,7 @@ int heat_design_eval(mastg_ctx* c, double* dobj, int where, const mastg_solve_op
     HIPC(c, hipStreamSynchronize(c->stream));
     float t23 = 0;
     (void)hipEventElapsedTime(&t23, e[2], e[3]);
-    for (auto& x : e) (void)hipEventDestroy(x);
+    for (auto& x : e) (void)ev_destroy(x);
     if (out) {
         out->objective = J;
         out->iters = lin_iters;
@@ -978,6 +1004,7 @@ int mastg_ctx_create(const mastg_problem_desc* desc, mastg_ctx** out) {
         mastg_ctx_destroy(c);
         return r;
     };
+    std::unique_lock<std::recursive_mutex> create_lock(g_hip_mu);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         g_create_error = "hipStreamCreate failed";
         delete c;
@@ -989,6 +1016,7 @@ int mastg_ctx_create(const mastg_problem_desc* desc, mastg_ctx** out) {
         c->err = "pinned host allocation / event creation failed";
         return bail(MASTG_ERR_DEVICE);
     }
+    create_lock.unlock();
     material_D3_unitE(c->d.nu, c->Dhat);
     // loads: nodal point loads mapped onto the active layout
     std::vector<double> F(S.n_active, 0.0);
@@ -1136,6 +1164,7 @@ int mastg_ctx_create(const mastg_problem_desc* desc, mastg_ctx** out) {
 
 int mastg_ctx_destroy(mastg_ctx* c) {
     if (!c) return MASTG_OK;
+    HipLock destroy_lock(g_hip_mu);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (int i = 0; i < 3; i++)
         if (c->cg_exec[i]) (void)hipGraphExecDestroy(c->cg_exec[i]);
@@ -1343,7 +1372,7 @@ int mastg_design_eval(mastg_ctx* c, const double* params, double* dobj, int wher
     TRY(mastg_set_params(c, params, where));
     if (c->d.physics == MASTG_HEX8_HEAT) return heat_design_eval(c, dobj, where, o, out);
     hipEvent_t e[4];
-    for (auto& x : e) HIPC(c, hipEventCreate(&x));
+    for (auto& x : e) HIPC(c, ev_create(&x));
     HIPC(c, hipEventRecord(e[0], c->stream));
     TRY(assemble_active(c, nullptr, true, true));   // R(0) = -F -> b = F (constrained)
     launch_axpby(S.n_active, 0.0, c->vtmp.as<double>(), -1.0, c->vtmp.as<double>(), c->stream);
@@ -1373,7 +1402,7 @@ int mastg_design_eval(mastg_ctx* c, const double* params, double* dobj, int wher
     (void)hipEventElapsedTime(&t01, e[0], e[1]);
     (void)hipEventElapsedTime(&t12, e[1], e[2]);
     (void)hipEventElapsedTime(&t23, e[2], e[3]);
-    for (auto& x : e) (void)hipEventDestroy(x);
+    for (auto& x : e) (void)ev_destroy(x);
     if (out) {
         out->objective = C;
         out->iters = st.iters;
@@ -1388,11 +1417,13 @@ int mastg_design_eval(mastg_ctx* c, const double* params, double* dobj, int wher
 
 int mastg_device_alloc(mastg_ctx* c, size_t bytes, void** p) {
     if (!c || !p) return MASTG_ERR_INVALID;
+    HipLock l(g_hip_mu);
     HIPC(c, hipMalloc(p, bytes));
     return MASTG_OK;
 }
 int mastg_device_free(mastg_ctx* c, void* p) {
     if (!c) return MASTG_ERR_INVALID;
+    HipLock l(g_hip_mu);
     HIPC(c, hipFree(p));
     return MASTG_OK;
 }

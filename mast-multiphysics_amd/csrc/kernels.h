@@ -123,9 +123,10 @@ void launch_gather_values(int64_t n, const int32_t* idx, const double* src, doub
 
 void launch_to_f32(int64_t n, const double* src, float* dst, hipStream_t s);
 void launch_spmv_nb32(const CGArgs& c, const double* x, double* y, int want_dot, hipStream_t s);
-// reliable update: r = rt (a true FP64 residual), z = dinv r, fresh r.z / r.r;
-// done = 1 if ||r|| meets the tolerance, else 0 (2 past max_it)
-void launch_cg_replace(const CGArgs& c, const double* rt, hipStream_t s);
+// residual replacement: r = rt (a true FP64 residual), z = dinv r, fresh r.z /
+// r.r, p = z when restart; done = 1 if ||r|| meets the tolerance, else 0 (2
+// past max_it)
+void launch_cg_replace(const CGArgs& c, const double* rt, int restart, hipStream_t s);
 void launch_hex8_K(const Hex8Args& a, hipStream_t s);
 void launch_hex8_sens(const Hex8Args& a, const double* lam, double* g, hipStream_t s);
 void launch_hex8_K_f32(const Hex8Args& a, float* Kel32, hipStream_t s);

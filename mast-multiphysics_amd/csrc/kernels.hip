@@ -1034,13 +1034,14 @@ __global__ __launch_bounds__(256) void k_cg_start_scalars(CGArgs C, double tol2,
 
 
 // reliable residual replacement (mixed-precision CG): r = rt, z = dinv r
-__global__ __launch_bounds__(256) void k_cg_replace(CGArgs C, const double* __restrict__ rt) {
+__global__ __launch_bounds__(256) void k_cg_replace(CGArgs C, const double* __restrict__ rt, int restart) {
     double rz = 0.0, rr = 0.0;
     for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < C.n; i += int64_t(gridDim.x) * 256) {
         const double r = rt[i];
         C.r[i] = r;
         const double z = C.dinv[i] * r;
         C.z[i] = z;
+        if (restart) C.p[i] = z;
         rz += r * z;
         rr += r * r;
     }
@@ -1336,8 +1337,8 @@ void launch_to_f32(int64_t n, const double* src, float* dst, hipStream_t s) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_to_f32, dim3(std::min<int64_t>(blocks_for(n, 256), 8192)), dim3(256), 0, s, n, src, dst);
 }
-void launch_cg_replace(const CGArgs& c, const double* rt, hipStream_t s) {
-    hipLaunchKernelGGL(k_cg_replace, dim3(c.nvec), dim3(256), 0, s, c, rt);
+void launch_cg_replace(const CGArgs& c, const double* rt, int restart, hipStream_t s) {
+    hipLaunchKernelGGL(k_cg_replace, dim3(c.nvec), dim3(256), 0, s, c, rt, restart);
     hipLaunchKernelGGL(k_cg_replace_scalars, dim3(1), dim3(256), 0, s, c);
 }
 void launch_cg_start(const CGArgs& c, const double* b, double tol2, int max_it, hipStream_t s) {

@@ -150,7 +150,10 @@ void run_request(mastg_server* s, Req* q, CachedCtx* cc, bool create) {
         q->res.objective = dr.objective;
         q->res.iters = dr.iters;
     }
+    std::string msg;
+    if (rc) msg = "request " + std::to_string(q->id) + ": " + mastg_last_error(cc->ctx);
     std::lock_guard<std::mutex> lk(s->mu);
+    if (rc) s->err = msg;
     q->t_end = Clock::now();
     q->res.status = rc ? rc : 2;
     q->res.t_queue = std::chrono::duration<double>(q->t_start - q->t_submit).count();
@@ -277,6 +280,12 @@ int mastg_server_stats(mastg_server* s, int64_t* bytes_in_use, int32_t* n_contex
     if (n_contexts) *n_contexts = int32_t(s->ctxs.size());
     if (queued) *queued = int32_t(s->queue.size());
     return MASTG_OK;
+}
+
+const char* mastg_server_last_error(mastg_server* s) {
+    if (!s) return "";
+    std::lock_guard<std::mutex> lk(s->mu);
+    return s->err.c_str();
 }
 
 int mastg_server_destroy(mastg_server* s) {

@@ -117,6 +117,7 @@ def lib():
             "mastg_server_stats": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
                                               C.POINTER(C.c_int32)]),
             "mastg_server_destroy": (C.c_int, [_P]),
+            "mastg_server_last_error": (C.c_char_p, [_P]),
             "mastg_request_footprint": (C.c_int, [C.POINTER(Request), C.POINTER(C.c_int64)]),
             "mastg_workload_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64,
                                                 C.c_double, C.c_double, C.c_int, C.POINTER(_P)]),
@@ -363,6 +364,9 @@ class Server:
         if rc:
             raise MastgError(rc, "server_wait")
         return self._result(rid, res)
+
+    def last_error(self):
+        return lib().mastg_server_last_error(self.h).decode()
 
     def stats(self):
         b, n, q = C.c_int64(), C.c_int32(), C.c_int32()

@@ -40,8 +40,9 @@ def test_alone_equals_in_mix(oracle, mastg):
             tid = s.submit(*target, p)
     outs = [s.wait(i) for i in ids]
     mixed = s.wait(tid)
+    err = s.last_error()
     s.close()
-    assert alone["status"] == 2 and mixed["status"] == 2
+    assert alone["status"] == 2 and mixed["status"] == 2, err
     assert all(o["status"] == 2 for o in outs)
     assert mixed["objective"] == alone["objective"]
     assert np.array_equal(mixed["grad"], alone["grad"])
@@ -76,8 +77,9 @@ def test_admission_and_eviction(oracle, mastg):
            for i, m in enumerate([(1, 48, 24, 24), (1, 16, 8, 8), (1, 48, 24, 24), (1, 16, 8, 8)])]
     outs = [s.wait(i) for i in ids]
     st = s.stats()
+    err = s.last_error()
     s.close()
-    assert all(o["status"] == 2 for o in outs), [o["status"] for o in outs]
+    assert all(o["status"] == 2 for o in outs), ([o["status"] for o in outs], err)
     assert st["bytes_in_use"] <= big + small // 2
 
 
