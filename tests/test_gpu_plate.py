@@ -88,16 +88,18 @@ def test_plate_mixed_refused_fp32_operator_accurate(oracle, mastg):
 
 def test_plate_adjoint_vs_fd_on_gpu(mastg):
     print("[plate FD on GPU] 12x12", flush=True)
+    # the FP64 floor of ||b - K u|| / ||b|| on this thin plate (cond ~2e9) is
+    # ~1e-11 (a dense FP64 solve reaches 9.9e-12), so the solves ask 2e-11
     w = mastg.Workload(2, 12, 12, seed=5)
     ctx = mastg.Context(w.desc)
     h = w.params.copy()
-    r = ctx.design_eval(h, rtol=1e-13)
+    r = ctx.design_eval(h, rtol=2e-11)
     for e in (0, 17, 70, 143):
         st = 1e-4 * h[e]
         hp = h.copy(); hp[e] += st
-        Cp = ctx.design_eval(hp, rtol=1e-13, want_grad=False)["objective"]
+        Cp = ctx.design_eval(hp, rtol=2e-11, want_grad=False)["objective"]
         hp[e] -= 2 * st
-        Cm = ctx.design_eval(hp, rtol=1e-13, want_grad=False)["objective"]
+        Cm = ctx.design_eval(hp, rtol=2e-11, want_grad=False)["objective"]
         fd = (Cp - Cm) / (2 * st)
         assert abs(r["grad"][e] - fd) <= 1e-5 * abs(fd), (e, r["grad"][e], fd)
     ctx.close()
