@@ -174,7 +174,9 @@ int mastg_solve(mastg_ctx* ctx, const double* rhs, double* x, int where,
 
 /* J dX with the Jacobian of the last mastg_assemble, op 0 = assembled CSR,
  * op 1 = matrix-free operator (same matrix), op 2 = the FP32-rounded values
- * read by node blocks (the mixed-precision solver's operator).  Replaces
+ * read by node blocks (the mixed-precision solver's operator), op 3 = the
+ * per-node lattice matrix-free operator of the multigrid solver applied to dX
+ * with its constrained entries zeroed (hex8 lattice meshes).  Replaces
  * NonlinearImplicitAssembly::linearized_jacobian_solution_product
  * (src/base/nonlinear_implicit_assembly.cpp:271-383) for the linear elements. */
 int mastg_jacobian_product(mastg_ctx* ctx, const double* dX, double* JdX, int where, int op);
@@ -222,7 +224,8 @@ int mastg_enable_kernel_timing(mastg_ctx* ctx, int on);
 /* Replays one kernel of the resident state n_launch times back to back on the
  * context's stream and returns the average duration from HIP events (same
  * kinds as mastg_kernel_time, plus 5 = the FP32 matrix-core element kernel,
- * hex8 only, into a buffer of its own).  For roofline measurement. */
+ * hex8 only, into a buffer of its own, and 6 = the per-node lattice operator).
+ * For roofline measurement. */
 int mastg_bench_kernel(mastg_ctx* ctx, int kind, int n_launch, double* avg_ms);
 /* Launch shapes: CSR SpMV lanes per row and grid size (work-groups of the
  * SpMV / CG vector kernels).  Autotuned on the first solve unless set. */

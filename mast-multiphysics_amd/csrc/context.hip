@@ -579,7 +579,7 @@ int mg_build(mastg_ctx* c) {
     }
     TRY(dalloc(c, c->mg_cheb, size_t(nl) * kMGChebStride * sizeof(double)));
     TRY(dalloc(c, c->mg_lam, size_t(nl) * sizeof(double)));
-    TRY(dalloc(c, c->mg_part, size_t((c->S.n_active + 255) / 256 + 1) * sizeof(double)));
+    TRY(dalloc(c, c->mg_part, size_t(std::max<int64_t>(8192, (c->S.n_active + 255) / 256 + 1)) * sizeof(double)));
     c->mg_n_coarse = H.coarsest_dofs;
     TRY(dalloc(c, c->mg_dense, size_t(H.coarsest_dofs * H.coarsest_dofs) * sizeof(double)));
     for (int l = 0; l < nl; l++) lv[l]->a.cheb = c->mg_cheb.as<double>() + l * kMGChebStride;
@@ -594,8 +594,8 @@ int mg_setup(mastg_ctx* c) {
     if (!c->kel64_valid) return fail(c, MASTG_ERR_INVALID, "multigrid needs an FP64 assembly");
     const int nl = int(c->mg.size());
     hipStream_t s = c->stream;
-    launch_mg_rowinfo_csr(c->S.n_active, c->row_ptr.as<int64_t>(), c->val.as<double>(), c->diag_pos.as<int64_t>(),
-                          c->mg_part.as<double>(), c->mg_lam.as<double>(), s);
+    launch_mg_rowinfo_nb(c->S.n_nodes, c->blk_ptr.as<int64_t>(), c->blk_col.as<int32_t>(), c->val.as<double>(),
+                         nullptr, c->mg_part.as<double>(), c->mg_lam.as<double>(), s);
     for (int l = 0; l + 1 < nl; l++) {
         MGLevelDev& F = *c->mg[l];
         MGLevelDev& C = *c->mg[l + 1];
@@ -614,7 +614,8 @@ int mg_setup(mastg_ctx* c) {
         ScatterArgs sa{C.a.n_nodes, 3, C.a.blk_ptr, C.cptr.as<int32_t>(), C.cidx.as<int32_t>(),
                        C.Kel.as<double>(), C.val.as<double>()};
         launch_scatter(sa, s);
-        launch_mg_rowinfo_bsr(C.a, C.dinv.as<double>(), c->mg_part.as<double>(), c->mg_lam.as<double>() + l + 1, s);
+        launch_mg_rowinfo_nb(C.a.n_nodes, C.a.blk_ptr, C.a.blk_col, C.a.val, C.dinv.as<double>(),
+                             c->mg_part.as<double>(), c->mg_lam.as<double>() + l + 1, s);
     }
     launch_mg_cheb(c->mg_lam.as<double>(), c->mg_cheb.as<double>(), nl, c->mg_k, 30.0, s);
     const MGLevelDev& Lc = *c->mg[nl - 1];
@@ -625,6 +626,20 @@ int mg_setup(mastg_ctx* c) {
     HIPC(c, hipGetLastError());
     c->mg_valid = true;
     return MASTG_OK;
+}
+
+// the per-node lattice operator's arguments (built multigrid hierarchy, one
+// element class); false when the mesh does not qualify
+bool lattice_args(mastg_ctx* c, CGArgs& a) {
+    if (c->mg.empty() || c->S.n_classes != 1) return false;
+    const MGLevelDev& L0 = *c->mg[0];
+    a.lat.nx = L0.a.nx; a.lat.ny = L0.a.ny; a.lat.nz = L0.a.nz;
+    a.lat.rank_of_lat = L0.rank_of_lat.as<int32_t>();
+    a.lat.elem_of_cell = L0.elem_of_cell.as<int32_t>();
+    a.lat.sE = c->sE.as<double>();
+    a.lat.Khat = c->Khat.as<double>();
+    a.lat.nodecon = c->nodecon.as<uint8_t>();
+    return true;
 }
 
 // one V-cycle on a.r; the level-0 output is x0 + dlast0 (formed by the caller)
@@ -683,6 +698,8 @@ int mg_solve(mastg_ctx* c, const double* b, const mastg_solve_opts* o, mastg_sol
     CGArgs a = cg_args(c);
     a.use_mf = 1;
     a.nparts_op = mf_node_blocks(S.n_nodes);
+    // per-node lattice operator for the CG product and the level-0 smoothing
+    if (lattice_args(c, a)) a.use_mf = 3;
     c->cg_replacements = 0;
     launch_fill(S.n_active, 0.0, a.x, c->stream);
     HIPC(c, hipMemcpyAsync(a.r, b, S.n_active * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
@@ -1290,7 +1307,8 @@ int mastg_jacobian_product(mastg_ctx* c, const double* dX, double* JdX, int wher
     if (!c->assembled) return fail(c, MASTG_ERR_INVALID, "no assembled matrix");
     if (op == 1 && (c->S.n_classes == 0 || c->d.physics != MASTG_HEX8_ELASTIC))
         return fail(c, MASTG_ERR_INVALID, "no matrix-free operator");
-    if (op < 0 || op > 2) return fail(c, MASTG_ERR_INVALID, "unknown operator");
+    if (op < 0 || op > 3) return fail(c, MASTG_ERR_INVALID, "unknown operator");
+    if (op == 3) TRY(mg_build(c));
     if (op == 2 && c->d.physics == MASTG_HEX8_HEAT)
         return fail(c, MASTG_ERR_INVALID, "FP32 node-block operator is for the structural Jacobians");
     TRY(to_active(c, dX, where, c->vb.as<double>()));
@@ -1300,7 +1318,16 @@ int mastg_jacobian_product(mastg_ctx* c, const double* dX, double* JdX, int wher
         c->val32_valid = true;
     }
     CGArgs a = cg_args(c);
-    if (op == 2) {
+    if (op == 3) {
+        if (!lattice_args(c, a)) return fail(c, MASTG_ERR_INVALID, "no lattice operator for this mesh");
+        if (!c->sE_valid) {
+            launch_elem_scale(c->S.n_elems, c->param.as<double>(), c->d.E, c->d.penalty, c->sE.as<double>(),
+                              c->stream);
+            c->sE_valid = true;
+        }
+        launch_mask_constrained(c->S.n_active, c->con.as<uint8_t>(), c->vb.as<double>(), c->stream);
+        launch_lattice_apply(a, c->vb.as<double>(), c->vtmp.as<double>(), c->stream);
+    } else if (op == 2) {
         launch_spmv_nb32(a, c->vb.as<double>(), c->vtmp.as<double>(), 0, c->stream);
     } else if (op == 1) {
         if (!c->sE_valid) {
@@ -1498,10 +1525,21 @@ int mastg_bench_kernel(mastg_ctx* c, int kind, int n_launch, double* avg_ms) {
                 break;
             case 4: launch_sens(c, a.x, a.x, c->vg.as<double>()); break;
             case 5: launch_hex8_K_f32(hex8_args(c, nullptr, false), c->Kel32.as<float>(), c->stream); break;
+            case 6: launch_lattice_apply(a, a.p, a.q, c->stream); break;
             default: break;
         }
     };
-    if (kind < 0 || kind > 5) return fail(c, MASTG_ERR_INVALID, "unknown kernel kind");
+    if (kind < 0 || kind > 6) return fail(c, MASTG_ERR_INVALID, "unknown kernel kind");
+    if (kind == 6) {
+        TRY(mg_build(c));
+        a.S = nullptr;
+        if (!lattice_args(c, a)) return fail(c, MASTG_ERR_INVALID, "no lattice operator for this mesh");
+        if (!c->sE_valid) {
+            launch_elem_scale(c->S.n_elems, c->param.as<double>(), c->d.E, c->d.penalty, c->sE.as<double>(),
+                              c->stream);
+            c->sE_valid = true;
+        }
+    }
     if (kind == 5) {
         if (c->d.physics != MASTG_HEX8_ELASTIC) return fail(c, MASTG_ERR_INVALID, "FP32 element kernel is hex8 only");
         TRY(dalloc(c, c->Kel32, size_t(c->S.n_elems) * 64 * 9 * sizeof(float)));

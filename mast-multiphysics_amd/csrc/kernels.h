@@ -78,6 +78,18 @@ struct MFArgs {
     double* Fel;              // [24][n_elems] element forces (SoA), phase 1 -> phase 2
 };
 
+// matrix-free hex8 operator on a lattice mesh, evaluated per node: y_n =
+// sum over the (up to 8) cells at n of s_e Khat[a_n][b] x_b, constrained rows
+// as diag = cells at the node (inputs must be zero on constrained DOFs)
+struct LatticeOpArgs {
+    int nx, ny, nz;                    // cells per dimension
+    const int32_t* rank_of_lat;        // node rank of lattice point
+    const int32_t* elem_of_cell;       // element of lattice cell
+    const double* sE;                  // [n_elems] element scales
+    const double* Khat;                // [64*9] the (single) class matrix at E = 1
+    const uint8_t* nodecon;            // [n_nodes] bit v = variable v constrained
+};
+
 struct CGArgs {
     int64_t n;
     const int64_t* row_ptr;
@@ -99,6 +111,7 @@ struct CGArgs {
     const int32_t* blk_col;
     const float* val32;
     int nav;
+    LatticeOpArgs lat;        // operator 3: per-node lattice matrix-free product
 };
 
 // right-preconditioned BiCGStab + Jacobi on a scalar CSR (PETSc KSPBCGS role)
@@ -209,6 +222,10 @@ void launch_mg_bsr(const MGLevelArgs& L, int mode, int step, const double* din, 
                    hipStream_t s);
 void launch_mg_fine(const CGArgs& c, const MGLevelArgs& L, int mode, int step, const double* din,
                     double* dout, const int* done, hipStream_t s);
+// y = A x with the lattice operator and partial sums of x.y into c.part0
+void launch_lattice_apply(const CGArgs& c, const double* x, double* y, hipStream_t s);
+void launch_mg_rowinfo_nb(int64_t n_nodes, const int64_t* blk_ptr, const int32_t* blk_col, const double* val,
+                          double* dinv, double* part, double* lam, hipStream_t s);
 void launch_mg_restrict(const MGLevelArgs& F, const MGLevelArgs& C, const int* done, hipStream_t s);
 void launch_mg_prolong(const MGLevelArgs& F, const MGLevelArgs& C, const double* dlast, const int* done,
                        hipStream_t s);

@@ -1404,7 +1404,9 @@ void launch_mf_elem(const CGArgs& c, const double* x, hipStream_t s) {
         hipLaunchKernelGGL(k_hex8_mf_elem<false>, dim3(ge), dim3(256), 0, s, c, x);
 }
 void launch_cg_op_alpha(const CGArgs& c, hipStream_t s) {
-    if (c.use_mf == 1)
+    if (c.use_mf == 3)
+        launch_lattice_apply(c, c.p, c.q, s);
+    else if (c.use_mf == 1)
         mf_dispatch(c, c.p, c.p, c.q, 1, s);
     else if (c.use_mf == 2)
         nb32_dispatch(c, c.p, c.q, 1, 1, s);

@@ -287,6 +287,155 @@ __global__ __launch_bounds__(256) void k_mg_fine_node(CGArgs C, MGLevelArgs L, i
     }
 }
 
+// ---- level 0 on a lattice: per-node matrix-free operator -------------------
+// Thread per lattice point n; for each existing cell at n (local corner a) the
+// 8 cell nodes' values are gathered (cached; the 27 neighbour ranks are looked
+// up once) and y_n += s_e Khat[a][b] x_b.  Khat indices are compile-time after
+// unrolling, so the class matrix is read with uniform (scalar-cache) loads.
+// Each y_n is a fixed-order sum: results are deterministic.
+__host__ __device__ constexpr int lat_cx(int b) { return (b == 1 || b == 2 || b == 5 || b == 6) ? 1 : 0; }
+__host__ __device__ constexpr int lat_cy(int b) { return (b == 2 || b == 3 || b == 6 || b == 7) ? 1 : 0; }
+__host__ __device__ constexpr int lat_cz(int b) { return b >= 4 ? 1 : 0; }
+__host__ __device__ constexpr int lat_corner(int dx, int dy, int dz) {
+    return dz * 4 + (dy ? (dx ? 2 : 3) : (dx ? 1 : 0));
+}
+
+// KIND 0: y = A x and partial x.y (CG operator); KIND 1: multigrid smoothing
+// step with the row-local epilogue
+template <int KIND>
+__global__ __launch_bounds__(256) void k_lattice_apply(CGArgs C, const double* __restrict__ xcg,
+                                                       double* __restrict__ ycg, MGLevelArgs L, int mode, int step,
+                                                       const double* __restrict__ din, double* __restrict__ dout,
+                                                       const int* __restrict__ done) {
+    if (done && *done) return;
+    const LatticeOpArgs& O = C.lat;
+    const int64_t nxp = O.nx + 1, nyp = O.ny + 1;
+    const int64_t n_lat = nxp * nyp * (O.nz + 1);
+    const int64_t p = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    const bool valid = p < n_lat;
+    const int64_t pv = valid ? p : 0;
+    const int i = int(pv % nxp), j = int((pv / nxp) % nyp), k = int(pv / (nxp * nyp));
+    const double* __restrict__ in = KIND == 0 ? xcg : (mode == MG_RESID_INIT ? L.x : din);
+    // ranks of the 27 neighbours (-1 outside the lattice)
+    int32_t rn[27];
+#pragma unroll
+    for (int o = 0; o < 27; o++) {
+        const int mi = i + (o % 3) - 1, mj = j + ((o / 3) % 3) - 1, mk = k + (o / 9) - 1;
+        const bool in_lat = valid && mi >= 0 && mi <= O.nx && mj >= 0 && mj <= O.ny && mk >= 0 && mk <= O.nz;
+        rn[o] = in_lat ? O.rank_of_lat[mi + nxp * (mj + nyp * int64_t(mk))] : -1;
+    }
+    double y0 = 0.0, y1 = 0.0, y2 = 0.0;
+    int cnt = 0;
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+        const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
+        const int ci = i - 1 + cx, cj = j - 1 + cy, ck = k - 1 + cz;
+        if (!valid || ci < 0 || ci >= O.nx || cj < 0 || cj >= O.ny || ck < 0 || ck >= O.nz) continue;
+        cnt++;
+        const double s = O.sE[O.elem_of_cell[ci + int64_t(O.nx) * (cj + int64_t(O.ny) * ck)]];
+        const int a = lat_corner(1 - cx, 1 - cy, 1 - cz);
+        double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const int o = (cx + lat_cx(b)) + 3 * (cy + lat_cy(b)) + 9 * (cz + lat_cz(b));
+            const int64_t rm = int64_t(rn[o]) * 3;
+            const double xa = in[rm], xb = in[rm + 1], xc = in[rm + 2];
+            const double* K = O.Khat + (a * 8 + b) * 9;
+            t0 += K[0] * xa + K[1] * xb + K[2] * xc;
+            t1 += K[3] * xa + K[4] * xb + K[5] * xc;
+            t2 += K[6] * xa + K[7] * xb + K[8] * xc;
+        }
+        y0 += s * t0;
+        y1 += s * t1;
+        y2 += s * t2;
+    }
+    const int64_t r = rn[13];
+    double dsum = 0.0;
+    if (valid) {
+        const int m = O.nodecon[r];
+        if (m & 1) y0 = cnt * in[r * 3 + 0];
+        if (m & 2) y1 = cnt * in[r * 3 + 1];
+        if (m & 4) y2 = cnt * in[r * 3 + 2];
+        if (KIND == 0) {
+            ycg[r * 3 + 0] = y0;
+            ycg[r * 3 + 1] = y1;
+            ycg[r * 3 + 2] = y2;
+            dsum = y0 * in[r * 3 + 0] + y1 * in[r * 3 + 1] + y2 * in[r * 3 + 2];
+        } else {
+            const double ith = L.cheb[0];
+            const double c1 = step > 0 ? L.cheb[2 * step] : 0.0, c2 = step > 0 ? L.cheb[2 * step + 1] : 0.0;
+            mg_epilogue(L, mode, r * 3 + 0, y0, din, dout, c1, c2, ith);
+            mg_epilogue(L, mode, r * 3 + 1, y1, din, dout, c1, c2, ith);
+            mg_epilogue(L, mode, r * 3 + 2, y2, din, dout, c1, c2, ith);
+        }
+    }
+    if (KIND == 0) block_sum_store(dsum, C.part0, blockIdx.x);
+}
+
+// dinv (when wanted) and the Gershgorin bound over node blocks, 8 lanes per node
+__global__ __launch_bounds__(256) void k_mg_rowinfo_nb(int64_t n_nodes, const int64_t* __restrict__ blk_ptr,
+                                                       const int32_t* __restrict__ blk_col,
+                                                       const double* __restrict__ val, double* __restrict__ dinv,
+                                                       double* __restrict__ part) {
+    const int sub = threadIdx.x & 7;
+    double g = 0.0;
+    const int64_t nodes_per_grid = (int64_t(gridDim.x) * 256) / 8;
+    for (int64_t r = (int64_t(blockIdx.x) * 256 + threadIdx.x) / 8; r < n_nodes; r += nodes_per_grid) {
+        const int64_t k0 = blk_ptr[r];
+        const int deg = int(blk_ptr[r + 1] - k0);
+        const double* __restrict__ V = val + k0 * 9;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, d0 = 0.0, d1 = 0.0, d2 = 0.0;
+        for (int kk = sub; kk < deg; kk += 8) {
+            const double* v0 = V + int64_t(kk) * 3;
+            const double* v1 = V + (int64_t(deg) + kk) * 3;
+            const double* v2 = V + (int64_t(2 * deg) + kk) * 3;
+            s0 += fabs(v0[0]) + fabs(v0[1]) + fabs(v0[2]);
+            s1 += fabs(v1[0]) + fabs(v1[1]) + fabs(v1[2]);
+            s2 += fabs(v2[0]) + fabs(v2[1]) + fabs(v2[2]);
+            if (blk_col[k0 + kk] == r) { d0 = v0[0]; d1 = v1[1]; d2 = v2[2]; }
+        }
+#pragma unroll
+        for (int m = 4; m >= 1; m >>= 1) {
+            s0 += __shfl_xor(s0, m, 8); s1 += __shfl_xor(s1, m, 8); s2 += __shfl_xor(s2, m, 8);
+            d0 += __shfl_xor(d0, m, 8); d1 += __shfl_xor(d1, m, 8); d2 += __shfl_xor(d2, m, 8);
+        }
+        if (sub == 0) {
+            g = fmax(g, d0 > 0.0 ? s0 / d0 : 0.0);
+            g = fmax(g, d1 > 0.0 ? s1 / d1 : 0.0);
+            g = fmax(g, d2 > 0.0 ? s2 / d2 : 0.0);
+            if (dinv) {
+                dinv[r * 3 + 0] = d0 > 0.0 ? 1.0 / d0 : 0.0;
+                dinv[r * 3 + 1] = d1 > 0.0 ? 1.0 / d1 : 0.0;
+                dinv[r * 3 + 2] = d2 > 0.0 ? 1.0 / d2 : 0.0;
+            }
+        }
+    }
+    block_max_store(g, part, blockIdx.x);
+}
+
+// Gauss-Jordan inverse held in LDS (n <= 136), one block
+__global__ __launch_bounds__(1024) void k_mg_invert_lds(int n, double* __restrict__ A) {
+    __shared__ double sA[136 * 136];
+    __shared__ double colk[136];
+    const int nn = n * n;
+    for (int idx = threadIdx.x; idx < nn; idx += 1024) sA[idx] = A[idx];
+    __syncthreads();
+    for (int k = 0; k < n; k++) {
+        for (int i = threadIdx.x; i < n; i += 1024) colk[i] = sA[i * n + k];
+        __syncthreads();
+        const double p = 1.0 / colk[k];
+        for (int j = threadIdx.x; j < n; j += 1024) sA[k * n + j] = (j == k) ? p : sA[k * n + j] * p;
+        __syncthreads();
+        for (int idx = threadIdx.x; idx < nn; idx += 1024) {
+            const int i = idx / n, j = idx - i * n;
+            if (i == k) continue;
+            sA[idx] = (j == k) ? -colk[i] * p : sA[idx] - colk[i] * sA[k * n + j];
+        }
+        __syncthreads();
+    }
+    for (int idx = threadIdx.x; idx < nn; idx += 1024) A[idx] = sA[idx];
+}
+
 // ---- transfers ---------------------------------------------------------------
 // r_C = sum over the 27 fine points around 2C of w * res_f (w = prod (1 or 1/2)),
 // constrained fine DOFs skipped, constrained coarse DOFs zero (P^T)
@@ -521,7 +670,23 @@ void launch_mg_dense_gather(const MGLevelArgs& L, double* A, hipStream_t s) {
     hipLaunchKernelGGL(k_mg_dense_gather, dim3(mg_blocks(L.n_nodes * 3, 256)), dim3(256), 0, s, L, A);
 }
 void launch_mg_invert(int n, double* A, hipStream_t s) {
-    hipLaunchKernelGGL(k_mg_invert, dim3(1), dim3(1024), 0, s, n, A);
+    if (n <= 136)
+        hipLaunchKernelGGL(k_mg_invert_lds, dim3(1), dim3(1024), 0, s, n, A);
+    else
+        hipLaunchKernelGGL(k_mg_invert, dim3(1), dim3(1024), 0, s, n, A);
+}
+void launch_mg_rowinfo_nb(int64_t n_nodes, const int64_t* blk_ptr, const int32_t* blk_col, const double* val,
+                          double* dinv, double* part, double* lam, hipStream_t s) {
+    const int64_t blocks = (n_nodes * 8 + 255) / 256;
+    const int nb = int(blocks < 1 ? 1 : (blocks > 8192 ? 8192 : blocks));
+    hipLaunchKernelGGL(k_mg_rowinfo_nb, dim3(nb), dim3(256), 0, s, n_nodes, blk_ptr, blk_col, val, dinv, part);
+    hipLaunchKernelGGL(k_mg_max, dim3(1), dim3(256), 0, s, part, nb, lam);
+}
+void launch_lattice_apply(const CGArgs& c, const double* x, double* y, hipStream_t s) {
+    const int64_t n_lat = int64_t(c.lat.nx + 1) * (c.lat.ny + 1) * (c.lat.nz + 1);
+    MGLevelArgs none{};
+    hipLaunchKernelGGL(k_lattice_apply<0>, dim3(mg_blocks(n_lat, 256)), dim3(256), 0, s, c, x, y, none, 0, 0,
+                       nullptr, nullptr, c.S ? &c.S->done : nullptr);
 }
 void launch_mg_pre_init(const MGLevelArgs& L, const int* done, hipStream_t s) {
     hipLaunchKernelGGL(k_mg_pre_init, dim3(mg_grid(L.n_nodes * 3)), dim3(256), 0, s, L, done);
@@ -534,6 +699,12 @@ void launch_mg_bsr(const MGLevelArgs& L, int mode, int step, const double* din, 
 }
 void launch_mg_fine(const CGArgs& c, const MGLevelArgs& L, int mode, int step, const double* din,
                     double* dout, const int* done, hipStream_t s) {
+    if (c.lat.rank_of_lat) {
+        const int64_t n_lat = int64_t(c.lat.nx + 1) * (c.lat.ny + 1) * (c.lat.nz + 1);
+        hipLaunchKernelGGL(k_lattice_apply<1>, dim3(mg_blocks(n_lat, 256)), dim3(256), 0, s, c, nullptr, nullptr, L,
+                           mode, step, din, dout, done);
+        return;
+    }
     const double* in = mode == MG_RESID_INIT ? L.x : din;
     launch_mf_elem(c, in, s);
     hipLaunchKernelGGL(k_mg_fine_node, dim3(mg_blocks(c.mf.n_nodes, 256)), dim3(256), 0, s, c, L, mode, step, din,
