@@ -59,7 +59,7 @@ struct MGLevelDev {
     int64_t n_elems = 0;
     DevBuf rank_of_lat, elem_of_cell;                  // level 0
     DevBuf blk_ptr, blk_col, cptr, cidx, econn, con;   // levels >= 1
-    DevBuf Kel, val, dinv, r, x, res, da, db;
+    DevBuf Kel, val, val32, dinv, r, x, res, da, db;
 };
 
 }  // namespace
@@ -557,6 +557,7 @@ int mg_build(mastg_ctx* c) {
             TRY(upload(c, L->con, S.constrained));
             TRY(dalloc(c, L->Kel, size_t(S.n_elems) * 64 * 9 * sizeof(double)));
             TRY(dalloc(c, L->val, size_t(S.n_blocks) * 9 * sizeof(double)));
+            TRY(dalloc(c, L->val32, size_t(S.n_blocks) * 9 * sizeof(float)));
             TRY(dalloc(c, L->dinv, nv));
             TRY(dalloc(c, L->r, nv));
             a.rank_of_lat = nullptr;    // coarse levels are numbered in lattice order
@@ -564,6 +565,10 @@ int mg_build(mastg_ctx* c) {
             a.blk_ptr = L->blk_ptr.as<int64_t>();
             a.blk_col = L->blk_col.as<int32_t>();
             a.val = L->val.as<double>();
+            // the smoother reads FP32 values (an exactly symmetric FP64 matrix
+            // rounds to an exactly symmetric FP32 one); the coarsest level's
+            // dense inverse and the bounds use the FP64 values
+            a.val32 = L->val32.as<float>();
             a.dinv = L->dinv.as<double>();
             a.r = L->r.as<double>();
         }
@@ -616,6 +621,7 @@ int mg_setup(mastg_ctx* c) {
         launch_scatter(sa, s);
         launch_mg_rowinfo_nb(C.a.n_nodes, C.a.blk_ptr, C.a.blk_col, C.a.val, C.dinv.as<double>(),
                              c->mg_part.as<double>(), c->mg_lam.as<double>() + l + 1, s);
+        launch_to_f32(int64_t(C.val32.bytes / sizeof(float)), C.a.val, C.val32.as<float>(), s);
     }
     launch_mg_cheb(c->mg_lam.as<double>(), c->mg_cheb.as<double>(), nl, c->mg_k, 30.0, s);
     const MGLevelDev& Lc = *c->mg[nl - 1];

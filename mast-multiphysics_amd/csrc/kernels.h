@@ -183,6 +183,7 @@ struct MGLevelArgs {
     const int64_t* blk_ptr;            // operator of levels >= 1: node-block CSR, FP64
     const int32_t* blk_col;
     const double* val;
+    const float* val32;                // FP32 copy of val read by the smoother (null = FP64)
     const double* dinv;                // [3 n_nodes] inverse diagonal
     double *r, *x, *res, *da, *db;     // right side, iterate, residual, smoothing directions
     const double* cheb;                // this level's Chebyshev coefficients

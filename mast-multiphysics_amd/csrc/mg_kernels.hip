@@ -218,9 +218,12 @@ __global__ __launch_bounds__(256) void k_mg_pre_init(MGLevelArgs L, const int* _
     }
 }
 
-// node-block FP64 operator of a coarse level, 8 lanes per node, fused epilogue
-__global__ __launch_bounds__(256) void k_mg_bsr(MGLevelArgs L, int mode, int step, const double* __restrict__ din,
-                                                double* __restrict__ dout, const int* __restrict__ done) {
+// node-block operator of a coarse level (values T = double or their FP32
+// copy; products and sums in FP64), 8 lanes per node, fused epilogue
+template <typename T>
+__global__ __launch_bounds__(256) void k_mg_bsr(MGLevelArgs L, const T* __restrict__ vals, int mode, int step,
+                                                const double* __restrict__ din, double* __restrict__ dout,
+                                                const int* __restrict__ done) {
     if (done && *done) return;
     const int sub = threadIdx.x & 7;
     const double* __restrict__ in = mode == MG_RESID_INIT ? L.x : din;
@@ -230,17 +233,17 @@ __global__ __launch_bounds__(256) void k_mg_bsr(MGLevelArgs L, int mode, int ste
     for (int64_t r = (int64_t(blockIdx.x) * 256 + threadIdx.x) / 8; r < L.n_nodes; r += nodes_per_grid) {
         const int64_t k0 = L.blk_ptr[r];
         const int deg = int(L.blk_ptr[r + 1] - k0);
-        const double* __restrict__ V = L.val + k0 * 9;
+        const T* __restrict__ V = vals + k0 * 9;
         double s0 = 0.0, s1 = 0.0, s2 = 0.0;
         for (int k = sub; k < deg; k += 8) {
             const int64_t cb = int64_t(L.blk_col[k0 + k]) * 3;
             const double x0 = in[cb], x1 = in[cb + 1], x2 = in[cb + 2];
-            const double* v0 = V + int64_t(k) * 3;
-            const double* v1 = V + (int64_t(deg) + k) * 3;
-            const double* v2 = V + (int64_t(2 * deg) + k) * 3;
-            s0 += v0[0] * x0 + v0[1] * x1 + v0[2] * x2;
-            s1 += v1[0] * x0 + v1[1] * x1 + v1[2] * x2;
-            s2 += v2[0] * x0 + v2[1] * x1 + v2[2] * x2;
+            const T* v0 = V + int64_t(k) * 3;
+            const T* v1 = V + (int64_t(deg) + k) * 3;
+            const T* v2 = V + (int64_t(2 * deg) + k) * 3;
+            s0 += double(v0[0]) * x0 + double(v0[1]) * x1 + double(v0[2]) * x2;
+            s1 += double(v1[0]) * x0 + double(v1[1]) * x1 + double(v1[2]) * x2;
+            s2 += double(v2[0]) * x0 + double(v2[1]) * x1 + double(v2[2]) * x2;
         }
 #pragma unroll
         for (int m = 4; m >= 1; m >>= 1) {
@@ -324,30 +327,47 @@ __global__ __launch_bounds__(256) void k_lattice_apply(CGArgs C, const double* _
         const bool in_lat = valid && mi >= 0 && mi <= O.nx && mj >= 0 && mj <= O.ny && mk >= 0 && mk <= O.nz;
         rn[o] = in_lat ? O.rank_of_lat[mi + nxp * (mj + nyp * int64_t(mk))] : -1;
     }
-    double y0 = 0.0, y1 = 0.0, y2 = 0.0;
+    // cells at n (c bit d: the cell extends towards +d from n - 1) and their scales
+    bool has[8];
+    double sc[8];
     int cnt = 0;
 #pragma unroll
     for (int c = 0; c < 8; c++) {
         const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
         const int ci = i - 1 + cx, cj = j - 1 + cy, ck = k - 1 + cz;
-        if (!valid || ci < 0 || ci >= O.nx || cj < 0 || cj >= O.ny || ck < 0 || ck >= O.nz) continue;
-        cnt++;
-        const double s = O.sE[O.elem_of_cell[ci + int64_t(O.nx) * (cj + int64_t(O.ny) * ck)]];
-        const int a = lat_corner(1 - cx, 1 - cy, 1 - cz);
-        double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+        has[c] = valid && ci >= 0 && ci < O.nx && cj >= 0 && cj < O.ny && ck >= 0 && ck < O.nz;
+        sc[c] = has[c] ? O.sE[O.elem_of_cell[ci + int64_t(O.nx) * (cj + int64_t(O.ny) * ck)]] : 0.0;
+        cnt += has[c] ? 1 : 0;
+    }
+    // neighbour-outer: each of the 27 neighbour values is read once and feeds
+    // every cell shared with n: t_c += Khat[a_c][b] x_o
+    double t[8][3];
 #pragma unroll
-        for (int b = 0; b < 8; b++) {
-            const int o = (cx + lat_cx(b)) + 3 * (cy + lat_cy(b)) + 9 * (cz + lat_cz(b));
-            const int64_t rm = int64_t(rn[o]) * 3;
-            const double xa = in[rm], xb = in[rm + 1], xc = in[rm + 2];
+    for (int c = 0; c < 8; c++) t[c][0] = t[c][1] = t[c][2] = 0.0;
+#pragma unroll
+    for (int o = 0; o < 27; o++) {
+        const int ox = (o % 3) - 1, oy = ((o / 3) % 3) - 1, oz = (o / 9) - 1;
+        if (rn[o] < 0) continue;
+        const int64_t rm = int64_t(rn[o]) * 3;
+        const double xa = in[rm], xb = in[rm + 1], xc = in[rm + 2];
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+            const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
+            const int bx = ox + 1 - cx, by = oy + 1 - cy, bz = oz + 1 - cz;
+            if (bx < 0 || bx > 1 || by < 0 || by > 1 || bz < 0 || bz > 1) continue;   // compile time
+            const int a = lat_corner(1 - cx, 1 - cy, 1 - cz), b = lat_corner(bx, by, bz);
             const double* K = O.Khat + (a * 8 + b) * 9;
-            t0 += K[0] * xa + K[1] * xb + K[2] * xc;
-            t1 += K[3] * xa + K[4] * xb + K[5] * xc;
-            t2 += K[6] * xa + K[7] * xb + K[8] * xc;
+            t[c][0] += K[0] * xa + K[1] * xb + K[2] * xc;
+            t[c][1] += K[3] * xa + K[4] * xb + K[5] * xc;
+            t[c][2] += K[6] * xa + K[7] * xb + K[8] * xc;
         }
-        y0 += s * t0;
-        y1 += s * t1;
-        y2 += s * t2;
+    }
+    double y0 = 0.0, y1 = 0.0, y2 = 0.0;
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+        y0 += sc[c] * t[c][0];
+        y1 += sc[c] * t[c][1];
+        y2 += sc[c] * t[c][2];
     }
     const int64_t r = rn[13];
     double dsum = 0.0;
@@ -446,28 +466,24 @@ __global__ __launch_bounds__(256) void k_mg_restrict(MGLevelArgs F, MGLevelArgs 
     const int64_t cx = Cl.nx + 1, cy = Cl.ny + 1;
     const int64_t I = p % cx, J = (p / cx) % cy, K = p / (cx * cy);
     const int64_t fx = F.nx + 1, fy = F.ny + 1;
-    double s[3] = {0.0, 0.0, 0.0};
-    for (int ok = -1; ok <= 1; ok++) {
-        const int64_t fk = 2 * K + ok;
-        if (fk < 0 || fk > F.nz) continue;
-        for (int oj = -1; oj <= 1; oj++) {
-            const int64_t fj = 2 * J + oj;
-            if (fj < 0 || fj > F.ny) continue;
-            for (int oi = -1; oi <= 1; oi++) {
-                const int64_t fi = 2 * I + oi;
-                if (fi < 0 || fi > F.nx) continue;
-                const double w = (ok ? 0.5 : 1.0) * (oj ? 0.5 : 1.0) * (oi ? 0.5 : 1.0);
-                const int64_t fp = fi + fx * (fj + fy * fk);
-                const int64_t rf = F.rank_of_lat ? int64_t(F.rank_of_lat[fp]) : fp;
+    // all 27 loads are independent (unrolled, predicated) so they are in flight together
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
 #pragma unroll
-                for (int v = 0; v < 3; v++)
-                    if (!F.con[rf * 3 + v]) s[v] += w * F.res[rf * 3 + v];
-            }
-        }
+    for (int o = 0; o < 27; o++) {
+        const int oi = (o % 3) - 1, oj = ((o / 3) % 3) - 1, ok = (o / 9) - 1;
+        const int64_t fi = 2 * I + oi, fj = 2 * J + oj, fk = 2 * K + ok;
+        if (fi < 0 || fi > F.nx || fj < 0 || fj > F.ny || fk < 0 || fk > F.nz) continue;
+        const double w = (ok ? 0.5 : 1.0) * (oj ? 0.5 : 1.0) * (oi ? 0.5 : 1.0);
+        const int64_t fp = fi + fx * (fj + fy * fk);
+        const int64_t rf = (F.rank_of_lat ? int64_t(F.rank_of_lat[fp]) : fp) * 3;
+        s0 += F.con[rf + 0] ? 0.0 : w * F.res[rf + 0];
+        s1 += F.con[rf + 1] ? 0.0 : w * F.res[rf + 1];
+        s2 += F.con[rf + 2] ? 0.0 : w * F.res[rf + 2];
     }
-    const int64_t rc = Cl.rank_of_lat ? int64_t(Cl.rank_of_lat[p]) : p;
-#pragma unroll
-    for (int v = 0; v < 3; v++) Cl.r[rc * 3 + v] = Cl.con[rc * 3 + v] ? 0.0 : s[v];
+    const int64_t rc = (Cl.rank_of_lat ? int64_t(Cl.rank_of_lat[p]) : p) * 3;
+    Cl.r[rc + 0] = Cl.con[rc + 0] ? 0.0 : s0;
+    Cl.r[rc + 1] = Cl.con[rc + 1] ? 0.0 : s1;
+    Cl.r[rc + 2] = Cl.con[rc + 2] ? 0.0 : s2;
 }
 
 // x_f += P (x_C + dlast_C) at unconstrained fine DOFs (trilinear interpolation)
@@ -479,30 +495,29 @@ __global__ __launch_bounds__(256) void k_mg_prolong(MGLevelArgs F, MGLevelArgs C
     const int64_t fx = F.nx + 1, fy = F.ny + 1;
     const int64_t fi = p % fx, fj = (p / fx) % fy, fk = p / (fx * fy);
     const int64_t cx = Cl.nx + 1, cy = Cl.ny + 1;
-    const int ni = (fi & 1) ? 2 : 1, nj = (fj & 1) ? 2 : 1, nk = (fk & 1) ? 2 : 1;
-    double s[3] = {0.0, 0.0, 0.0};
-    for (int c = 0; c < nk; c++) {
-        const int64_t K = (fk >> 1) + c;
-        for (int b = 0; b < nj; b++) {
-            const int64_t J = (fj >> 1) + b;
-            for (int a = 0; a < ni; a++) {
-                const int64_t I = (fi >> 1) + a;
-                const double w = (ni == 2 ? 0.5 : 1.0) * (nj == 2 ? 0.5 : 1.0) * (nk == 2 ? 0.5 : 1.0);
-                const int64_t cp = I + cx * (J + cy * K);
-                const int64_t rc = Cl.rank_of_lat ? int64_t(Cl.rank_of_lat[cp]) : cp;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
 #pragma unroll
-                for (int v = 0; v < 3; v++) {
-                    double xv = Cl.x[rc * 3 + v];
-                    if (dlast) xv += dlast[rc * 3 + v];
-                    s[v] += w * xv;
-                }
-            }
+    for (int q = 0; q < 8; q++) {
+        const int a = q & 1, b = (q >> 1) & 1, c = q >> 2;
+        // an even fine index takes its one coarse point (a = 0), an odd one both neighbours
+        if ((a && !(fi & 1)) || (b && !(fj & 1)) || (c && !(fk & 1))) continue;
+        const double w = ((fi & 1) ? 0.5 : 1.0) * ((fj & 1) ? 0.5 : 1.0) * ((fk & 1) ? 0.5 : 1.0);
+        const int64_t cp = ((fi >> 1) + a) + cx * (((fj >> 1) + b) + cy * ((fk >> 1) + c));
+        const int64_t rc = (Cl.rank_of_lat ? int64_t(Cl.rank_of_lat[cp]) : cp) * 3;
+        if (dlast) {
+            s0 += w * (Cl.x[rc + 0] + dlast[rc + 0]);
+            s1 += w * (Cl.x[rc + 1] + dlast[rc + 1]);
+            s2 += w * (Cl.x[rc + 2] + dlast[rc + 2]);
+        } else {
+            s0 += w * Cl.x[rc + 0];
+            s1 += w * Cl.x[rc + 1];
+            s2 += w * Cl.x[rc + 2];
         }
     }
-    const int64_t rf = F.rank_of_lat ? int64_t(F.rank_of_lat[p]) : p;
-#pragma unroll
-    for (int v = 0; v < 3; v++)
-        if (!F.con[rf * 3 + v]) F.x[rf * 3 + v] += s[v];
+    const int64_t rf = (F.rank_of_lat ? int64_t(F.rank_of_lat[p]) : p) * 3;
+    if (!F.con[rf + 0]) F.x[rf + 0] += s0;
+    if (!F.con[rf + 1]) F.x[rf + 1] += s1;
+    if (!F.con[rf + 2]) F.x[rf + 2] += s2;
 }
 
 // ---- coarsest level: dense inverse -------------------------------------------
@@ -695,7 +710,10 @@ void launch_mg_bsr(const MGLevelArgs& L, int mode, int step, const double* din, 
                    hipStream_t s) {
     const int64_t blocks = (L.n_nodes * 8 + 255) / 256;
     const unsigned g = unsigned(blocks < 1 ? 1 : (blocks > 8192 ? 8192 : blocks));
-    hipLaunchKernelGGL(k_mg_bsr, dim3(g), dim3(256), 0, s, L, mode, step, din, dout, done);
+    if (L.val32)
+        hipLaunchKernelGGL(k_mg_bsr<float>, dim3(g), dim3(256), 0, s, L, L.val32, mode, step, din, dout, done);
+    else
+        hipLaunchKernelGGL(k_mg_bsr<double>, dim3(g), dim3(256), 0, s, L, L.val, mode, step, din, dout, done);
 }
 void launch_mg_fine(const CGArgs& c, const MGLevelArgs& L, int mode, int step, const double* din,
                     double* dout, const int* done, hipStream_t s) {
