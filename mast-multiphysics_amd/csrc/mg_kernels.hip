@@ -319,25 +319,30 @@ __global__ __launch_bounds__(256) void k_lattice_apply(CGArgs C, const double* _
     const int64_t pv = valid ? p : 0;
     const int i = int(pv % nxp), j = int((pv / nxp) % nyp), k = int(pv / (nxp * nyp));
     const double* __restrict__ in = KIND == 0 ? xcg : (mode == MG_RESID_INIT ? L.x : din);
-    // ranks of the 27 neighbours (-1 outside the lattice)
+    // Branch-free gathers: indices outside the lattice are clamped into it and
+    // their contributions vanish because every cell holding such a neighbour
+    // is outside too and carries the scale 0.
     int32_t rn[27];
 #pragma unroll
     for (int o = 0; o < 27; o++) {
-        const int mi = i + (o % 3) - 1, mj = j + ((o / 3) % 3) - 1, mk = k + (o / 9) - 1;
-        const bool in_lat = valid && mi >= 0 && mi <= O.nx && mj >= 0 && mj <= O.ny && mk >= 0 && mk <= O.nz;
-        rn[o] = in_lat ? O.rank_of_lat[mi + nxp * (mj + nyp * int64_t(mk))] : -1;
+        const int mi = min(max(i + (o % 3) - 1, 0), O.nx);
+        const int mj = min(max(j + ((o / 3) % 3) - 1, 0), O.ny);
+        const int mk = min(max(k + (o / 9) - 1, 0), O.nz);
+        rn[o] = O.rank_of_lat[mi + nxp * (mj + nyp * int64_t(mk))];
     }
     // cells at n (c bit d: the cell extends towards +d from n - 1) and their scales
-    bool has[8];
     double sc[8];
     int cnt = 0;
 #pragma unroll
     for (int c = 0; c < 8; c++) {
         const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
         const int ci = i - 1 + cx, cj = j - 1 + cy, ck = k - 1 + cz;
-        has[c] = valid && ci >= 0 && ci < O.nx && cj >= 0 && cj < O.ny && ck >= 0 && ck < O.nz;
-        sc[c] = has[c] ? O.sE[O.elem_of_cell[ci + int64_t(O.nx) * (cj + int64_t(O.ny) * ck)]] : 0.0;
-        cnt += has[c] ? 1 : 0;
+        const bool has = valid && ci >= 0 && ci < O.nx && cj >= 0 && cj < O.ny && ck >= 0 && ck < O.nz;
+        const int64_t cell = min(max(ci, 0), O.nx - 1) +
+                             int64_t(O.nx) * (min(max(cj, 0), O.ny - 1) + int64_t(O.ny) * min(max(ck, 0), O.nz - 1));
+        const double sv = O.sE[O.elem_of_cell[cell]];
+        sc[c] = has ? sv : 0.0;
+        cnt += has ? 1 : 0;
     }
     // neighbour-outer: each of the 27 neighbour values is read once and feeds
     // every cell shared with n: t_c += Khat[a_c][b] x_o
@@ -347,7 +352,6 @@ __global__ __launch_bounds__(256) void k_lattice_apply(CGArgs C, const double* _
 #pragma unroll
     for (int o = 0; o < 27; o++) {
         const int ox = (o % 3) - 1, oy = ((o / 3) % 3) - 1, oz = (o / 9) - 1;
-        if (rn[o] < 0) continue;
         const int64_t rm = int64_t(rn[o]) * 3;
         const double xa = in[rm], xb = in[rm + 1], xc = in[rm + 2];
 #pragma unroll
@@ -459,6 +463,8 @@ __global__ __launch_bounds__(1024) void k_mg_invert_lds(int n, double* __restric
 // ---- transfers ---------------------------------------------------------------
 // r_C = sum over the 27 fine points around 2C of w * res_f (w = prod (1 or 1/2)),
 // constrained fine DOFs skipped, constrained coarse DOFs zero (P^T)
+// FR: the fine level is numbered by node rank (level 0); coarse levels are in lattice order
+template <bool FR>
 __global__ __launch_bounds__(256) void k_mg_restrict(MGLevelArgs F, MGLevelArgs Cl, const int* __restrict__ done) {
     if (done && *done) return;
     const int64_t p = int64_t(blockIdx.x) * 256 + threadIdx.x;
@@ -472,21 +478,24 @@ __global__ __launch_bounds__(256) void k_mg_restrict(MGLevelArgs F, MGLevelArgs 
     for (int o = 0; o < 27; o++) {
         const int oi = (o % 3) - 1, oj = ((o / 3) % 3) - 1, ok = (o / 9) - 1;
         const int64_t fi = 2 * I + oi, fj = 2 * J + oj, fk = 2 * K + ok;
-        if (fi < 0 || fi > F.nx || fj < 0 || fj > F.ny || fk < 0 || fk > F.nz) continue;
-        const double w = (ok ? 0.5 : 1.0) * (oj ? 0.5 : 1.0) * (oi ? 0.5 : 1.0);
-        const int64_t fp = fi + fx * (fj + fy * fk);
-        const int64_t rf = (F.rank_of_lat ? int64_t(F.rank_of_lat[fp]) : fp) * 3;
-        s0 += F.con[rf + 0] ? 0.0 : w * F.res[rf + 0];
-        s1 += F.con[rf + 1] ? 0.0 : w * F.res[rf + 1];
-        s2 += F.con[rf + 2] ? 0.0 : w * F.res[rf + 2];
+        const bool in_lat = fi >= 0 && fi <= F.nx && fj >= 0 && fj <= F.ny && fk >= 0 && fk <= F.nz;
+        // out-of-lattice points are clamped (valid address) and weighted 0;
+        // constrained fine DOFs are weighted 0 (P has no row for them)
+        const double w = in_lat ? (ok ? 0.5 : 1.0) * (oj ? 0.5 : 1.0) * (oi ? 0.5 : 1.0) : 0.0;
+        const int64_t fp = (in_lat ? fi : 0) + fx * ((in_lat ? fj : 0) + fy * (in_lat ? fk : 0));
+        const int64_t rf = (FR ? int64_t(F.rank_of_lat[fp]) : fp) * 3;
+        s0 += (w * (1.0 - double(F.con[rf + 0]))) * F.res[rf + 0];
+        s1 += (w * (1.0 - double(F.con[rf + 1]))) * F.res[rf + 1];
+        s2 += (w * (1.0 - double(F.con[rf + 2]))) * F.res[rf + 2];
     }
-    const int64_t rc = (Cl.rank_of_lat ? int64_t(Cl.rank_of_lat[p]) : p) * 3;
+    const int64_t rc = p * 3;
     Cl.r[rc + 0] = Cl.con[rc + 0] ? 0.0 : s0;
     Cl.r[rc + 1] = Cl.con[rc + 1] ? 0.0 : s1;
     Cl.r[rc + 2] = Cl.con[rc + 2] ? 0.0 : s2;
 }
 
 // x_f += P (x_C + dlast_C) at unconstrained fine DOFs (trilinear interpolation)
+template <bool FR>
 __global__ __launch_bounds__(256) void k_mg_prolong(MGLevelArgs F, MGLevelArgs Cl, const double* __restrict__ dlast,
                                                     const int* __restrict__ done) {
     if (done && *done) return;
@@ -499,11 +508,14 @@ __global__ __launch_bounds__(256) void k_mg_prolong(MGLevelArgs F, MGLevelArgs C
 #pragma unroll
     for (int q = 0; q < 8; q++) {
         const int a = q & 1, b = (q >> 1) & 1, c = q >> 2;
-        // an even fine index takes its one coarse point (a = 0), an odd one both neighbours
-        if ((a && !(fi & 1)) || (b && !(fj & 1)) || (c && !(fk & 1))) continue;
-        const double w = ((fi & 1) ? 0.5 : 1.0) * ((fj & 1) ? 0.5 : 1.0) * ((fk & 1) ? 0.5 : 1.0);
-        const int64_t cp = ((fi >> 1) + a) + cx * (((fj >> 1) + b) + cy * ((fk >> 1) + c));
-        const int64_t rc = (Cl.rank_of_lat ? int64_t(Cl.rank_of_lat[cp]) : cp) * 3;
+        // an even fine index takes its one coarse point (a = 0), an odd one both
+        // neighbours; the unused corner of an even index is clamped and weighted 0
+        const bool use = !((a && !(fi & 1)) || (b && !(fj & 1)) || (c && !(fk & 1)));
+        const double w = use ? ((fi & 1) ? 0.5 : 1.0) * ((fj & 1) ? 0.5 : 1.0) * ((fk & 1) ? 0.5 : 1.0) : 0.0;
+        const int64_t CI = min((fi >> 1) + a, int64_t(Cl.nx)), CJ = min((fj >> 1) + b, int64_t(Cl.ny)),
+                      CK = min((fk >> 1) + c, int64_t(Cl.nz));
+        const int64_t cp = CI + cx * (CJ + cy * CK);
+        const int64_t rc = cp * 3;
         if (dlast) {
             s0 += w * (Cl.x[rc + 0] + dlast[rc + 0]);
             s1 += w * (Cl.x[rc + 1] + dlast[rc + 1]);
@@ -514,7 +526,7 @@ __global__ __launch_bounds__(256) void k_mg_prolong(MGLevelArgs F, MGLevelArgs C
             s2 += w * Cl.x[rc + 2];
         }
     }
-    const int64_t rf = (F.rank_of_lat ? int64_t(F.rank_of_lat[p]) : p) * 3;
+    const int64_t rf = (FR ? int64_t(F.rank_of_lat[p]) : p) * 3;
     if (!F.con[rf + 0]) F.x[rf + 0] += s0;
     if (!F.con[rf + 1]) F.x[rf + 1] += s1;
     if (!F.con[rf + 2]) F.x[rf + 2] += s2;
@@ -729,11 +741,17 @@ void launch_mg_fine(const CGArgs& c, const MGLevelArgs& L, int mode, int step, c
                        dout, done);
 }
 void launch_mg_restrict(const MGLevelArgs& F, const MGLevelArgs& C, const int* done, hipStream_t s) {
-    hipLaunchKernelGGL(k_mg_restrict, dim3(mg_blocks(C.n_nodes, 256)), dim3(256), 0, s, F, C, done);
+    if (F.rank_of_lat)
+        hipLaunchKernelGGL(k_mg_restrict<true>, dim3(mg_blocks(C.n_nodes, 256)), dim3(256), 0, s, F, C, done);
+    else
+        hipLaunchKernelGGL(k_mg_restrict<false>, dim3(mg_blocks(C.n_nodes, 256)), dim3(256), 0, s, F, C, done);
 }
 void launch_mg_prolong(const MGLevelArgs& F, const MGLevelArgs& C, const double* dlast, const int* done,
                        hipStream_t s) {
-    hipLaunchKernelGGL(k_mg_prolong, dim3(mg_blocks(F.n_nodes, 256)), dim3(256), 0, s, F, C, dlast, done);
+    if (F.rank_of_lat)
+        hipLaunchKernelGGL(k_mg_prolong<true>, dim3(mg_blocks(F.n_nodes, 256)), dim3(256), 0, s, F, C, dlast, done);
+    else
+        hipLaunchKernelGGL(k_mg_prolong<false>, dim3(mg_blocks(F.n_nodes, 256)), dim3(256), 0, s, F, C, dlast, done);
 }
 void launch_mg_coarse_solve(int n, const double* Ainv, const double* r, double* x, const int* done,
                             hipStream_t s) {
