@@ -319,18 +319,9 @@ __global__ __launch_bounds__(256) void k_lattice_apply(CGArgs C, const double* _
     const int64_t pv = valid ? p : 0;
     const int i = int(pv % nxp), j = int((pv / nxp) % nyp), k = int(pv / (nxp * nyp));
     const double* __restrict__ in = KIND == 0 ? xcg : (mode == MG_RESID_INIT ? L.x : din);
-    // Branch-free gathers: indices outside the lattice are clamped into it and
-    // their contributions vanish because every cell holding such a neighbour
-    // is outside too and carries the scale 0.
-    int32_t rn[27];
-#pragma unroll
-    for (int o = 0; o < 27; o++) {
-        const int mi = min(max(i + (o % 3) - 1, 0), O.nx);
-        const int mj = min(max(j + ((o / 3) % 3) - 1, 0), O.ny);
-        const int mk = min(max(k + (o / 9) - 1, 0), O.nz);
-        rn[o] = O.rank_of_lat[mi + nxp * (mj + nyp * int64_t(mk))];
-    }
-    // cells at n (c bit d: the cell extends towards +d from n - 1) and their scales
+    // cells at n (c bit d: the cell extends towards +d from n - 1) and their
+    // scales; a missing cell gets the scale 0 (its index is clamped into the
+    // lattice so every load stays in bounds and unconditional)
     double sc[8];
     int cnt = 0;
 #pragma unroll
@@ -344,26 +335,42 @@ __global__ __launch_bounds__(256) void k_lattice_apply(CGArgs C, const double* _
         sc[c] = has ? sv : 0.0;
         cnt += has ? 1 : 0;
     }
-    // neighbour-outer: each of the 27 neighbour values is read once and feeds
-    // every cell shared with n: t_c += Khat[a_c][b] x_o
+    // neighbour-outer, one z-layer of 9 neighbours at a time (bounded live
+    // ranges): each neighbour value is read once and feeds every cell shared
+    // with n, t_c += Khat[a_c][b] x_o.  Neighbours outside the lattice are
+    // clamped into it; every cell holding one is missing and scaled by 0.
     double t[8][3];
 #pragma unroll
     for (int c = 0; c < 8; c++) t[c][0] = t[c][1] = t[c][2] = 0.0;
+#pragma unroll 1
+    for (int lz = 0; lz < 3; lz++) {
+        const int mk = min(max(k + lz - 1, 0), O.nz);
+        double xv[9][3];
 #pragma unroll
-    for (int o = 0; o < 27; o++) {
-        const int ox = (o % 3) - 1, oy = ((o / 3) % 3) - 1, oz = (o / 9) - 1;
-        const int64_t rm = int64_t(rn[o]) * 3;
-        const double xa = in[rm], xb = in[rm + 1], xc = in[rm + 2];
+        for (int q = 0; q < 9; q++) {
+            const int mi = min(max(i + (q % 3) - 1, 0), O.nx);
+            const int mj = min(max(j + (q / 3) - 1, 0), O.ny);
+            const int64_t rm = int64_t(O.rank_of_lat[mi + nxp * (mj + nyp * int64_t(mk))]) * 3;
+            xv[q][0] = in[rm];
+            xv[q][1] = in[rm + 1];
+            xv[q][2] = in[rm + 2];
+        }
 #pragma unroll
         for (int c = 0; c < 8; c++) {
             const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
-            const int bx = ox + 1 - cx, by = oy + 1 - cy, bz = oz + 1 - cz;
-            if (bx < 0 || bx > 1 || by < 0 || by > 1 || bz < 0 || bz > 1) continue;   // compile time
-            const int a = lat_corner(1 - cx, 1 - cy, 1 - cz), b = lat_corner(bx, by, bz);
-            const double* K = O.Khat + (a * 8 + b) * 9;
-            t[c][0] += K[0] * xa + K[1] * xb + K[2] * xc;
-            t[c][1] += K[3] * xa + K[4] * xb + K[5] * xc;
-            t[c][2] += K[6] * xa + K[7] * xb + K[8] * xc;
+            const int bz = lz - cz;                      // uniform across the wave
+            if (bz < 0 || bz > 1) continue;
+            const int a = lat_corner(1 - cx, 1 - cy, 1 - cz);
+#pragma unroll
+            for (int q = 0; q < 9; q++) {
+                const int bx = (q % 3) - cx, by = (q / 3) - cy;
+                if (bx < 0 || bx > 1 || by < 0 || by > 1) continue;   // compile time
+                const int b = bz * 4 + lat_corner(bx, by, 0);
+                const double* K = O.Khat + (a * 8 + b) * 9;
+                t[c][0] += K[0] * xv[q][0] + K[1] * xv[q][1] + K[2] * xv[q][2];
+                t[c][1] += K[3] * xv[q][0] + K[4] * xv[q][1] + K[5] * xv[q][2];
+                t[c][2] += K[6] * xv[q][0] + K[7] * xv[q][1] + K[8] * xv[q][2];
+            }
         }
     }
     double y0 = 0.0, y1 = 0.0, y2 = 0.0;
@@ -373,7 +380,7 @@ __global__ __launch_bounds__(256) void k_lattice_apply(CGArgs C, const double* _
         y1 += sc[c] * t[c][1];
         y2 += sc[c] * t[c][2];
     }
-    const int64_t r = rn[13];
+    const int64_t r = valid ? int64_t(O.rank_of_lat[pv]) : 0;
     double dsum = 0.0;
     if (valid) {
         const int m = O.nodecon[r];
