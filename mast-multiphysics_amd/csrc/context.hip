@@ -645,6 +645,9 @@ bool lattice_args(mastg_ctx* c, CGArgs& a) {
     a.lat.sE = c->sE.as<double>();
     a.lat.Khat = c->Khat.as<double>();
     a.lat.nodecon = c->nodecon.as<uint8_t>();
+    // LDS bricks when the partial-sum buffer holds one entry per brick
+    a.lat.tiled = 1;
+    if (lattice_partials(a.lat) > int64_t(c->part0.bytes / sizeof(double))) a.lat.tiled = 0;
     return true;
 }
 
@@ -705,7 +708,10 @@ int mg_solve(mastg_ctx* c, const double* b, const mastg_solve_opts* o, mastg_sol
     a.use_mf = 1;
     a.nparts_op = mf_node_blocks(S.n_nodes);
     // per-node lattice operator for the CG product and the level-0 smoothing
-    if (lattice_args(c, a)) a.use_mf = 3;
+    if (lattice_args(c, a)) {
+        a.use_mf = 3;
+        a.nparts_op = int(lattice_partials(a.lat));
+    }
     c->cg_replacements = 0;
     launch_fill(S.n_active, 0.0, a.x, c->stream);
     HIPC(c, hipMemcpyAsync(a.r, b, S.n_active * sizeof(double), hipMemcpyDeviceToDevice, c->stream));

@@ -88,7 +88,10 @@ struct LatticeOpArgs {
     const double* sE;                  // [n_elems] element scales
     const double* Khat;                // [64*9] the (single) class matrix at E = 1
     const uint8_t* nodecon;            // [n_nodes] bit v = variable v constrained
+    int tiled;                         // 1: LDS-staged 8x8x4 bricks, 0: one thread per node
 };
+// partial sums the lattice operator writes (its grid)
+int64_t lattice_partials(const LatticeOpArgs& l);
 
 struct CGArgs {
     int64_t n;

@@ -403,6 +403,123 @@ __global__ __launch_bounds__(256) void k_lattice_apply(CGArgs C, const double* _
     if (KIND == 0) block_sum_store(dsum, C.part0, blockIdx.x);
 }
 
+// Tiled variant: a block owns an 8x8x4 brick of lattice points; the brick's
+// halo values (10x10x6 points) and cell scales (9x9x5 cells) are staged in
+// LDS once, so every gathered value is read from HBM / L2 once per brick
+// instead of once per neighbouring node.  Same arithmetic order as above.
+constexpr int kLTX = 8, kLTY = 8, kLTZ = 4;
+__host__ __device__ inline int64_t lattice_tiles(int nx, int ny, int nz) {
+    return int64_t((nx + kLTX) / kLTX) * ((ny + kLTY) / kLTY) * ((nz + kLTZ) / kLTZ);
+}
+
+template <int KIND>
+__global__ __launch_bounds__(256) void k_lattice_tiled(CGArgs C, const double* __restrict__ xcg,
+                                                       double* __restrict__ ycg, MGLevelArgs L, int mode, int step,
+                                                       const double* __restrict__ din, double* __restrict__ dout,
+                                                       const int* __restrict__ done) {
+    if (done && *done) return;
+    constexpr int HX = kLTX + 2, HY = kLTY + 2, HZ = kLTZ + 2;   // halo points
+    constexpr int CX = kLTX + 1, CY = kLTY + 1, CZ = kLTZ + 1;   // cells
+    __shared__ double sx[HZ * HY * HX * 3];
+    __shared__ double ssc[CZ * CY * CX];
+    const LatticeOpArgs& O = C.lat;
+    const int64_t nxp = O.nx + 1, nyp = O.ny + 1;
+    const int ntx = (O.nx + kLTX) / kLTX, nty = (O.ny + kLTY) / kLTY;
+    const int64_t t = blockIdx.x;
+    const int i0 = int(t % ntx) * kLTX, j0 = int((t / ntx) % nty) * kLTY, k0 = int(t / (int64_t(ntx) * nty)) * kLTZ;
+    const double* __restrict__ in = KIND == 0 ? xcg : (mode == MG_RESID_INIT ? L.x : din);
+    for (int h = threadIdx.x; h < HZ * HY * HX; h += 256) {
+        const int hx = h % HX, hy = (h / HX) % HY, hz = h / (HX * HY);
+        const int gi = min(max(i0 - 1 + hx, 0), O.nx), gj = min(max(j0 - 1 + hy, 0), O.ny),
+                  gk = min(max(k0 - 1 + hz, 0), O.nz);
+        const int64_t rm = int64_t(O.rank_of_lat[gi + nxp * (gj + nyp * int64_t(gk))]) * 3;
+        sx[h * 3 + 0] = in[rm];
+        sx[h * 3 + 1] = in[rm + 1];
+        sx[h * 3 + 2] = in[rm + 2];
+    }
+    for (int h = threadIdx.x; h < CZ * CY * CX; h += 256) {
+        const int hx = h % CX, hy = (h / CX) % CY, hz = h / (CX * CY);
+        const int ci = i0 - 1 + hx, cj = j0 - 1 + hy, ck = k0 - 1 + hz;
+        const bool has = ci >= 0 && ci < O.nx && cj >= 0 && cj < O.ny && ck >= 0 && ck < O.nz;
+        const int64_t cell = min(max(ci, 0), O.nx - 1) +
+                             int64_t(O.nx) * (min(max(cj, 0), O.ny - 1) + int64_t(O.ny) * min(max(ck, 0), O.nz - 1));
+        const double sv = O.sE[O.elem_of_cell[cell]];
+        ssc[h] = has ? sv : 0.0;
+    }
+    __syncthreads();
+    const int tx = threadIdx.x % kLTX, ty = (threadIdx.x / kLTX) % kLTY, tz = threadIdx.x / (kLTX * kLTY);
+    const int i = i0 + tx, j = j0 + ty, k = k0 + tz;
+    const bool valid = i <= O.nx && j <= O.ny && k <= O.nz;
+    double sc[8];
+    int cnt = 0;
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+        const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
+        sc[c] = ssc[(tz + cz) * (CY * CX) + (ty + cy) * CX + (tx + cx)];
+        const int ci = i - 1 + cx, cj = j - 1 + cy, ck = k - 1 + cz;
+        cnt += (ci >= 0 && ci < O.nx && cj >= 0 && cj < O.ny && ck >= 0 && ck < O.nz) ? 1 : 0;
+    }
+    double tt[8][3];
+#pragma unroll
+    for (int c = 0; c < 8; c++) tt[c][0] = tt[c][1] = tt[c][2] = 0.0;
+#pragma unroll 1
+    for (int lz = 0; lz < 3; lz++) {
+        double xv[9][3];
+#pragma unroll
+        for (int q = 0; q < 9; q++) {
+            const int h = (tz + lz) * (HY * HX) + (ty + q / 3) * HX + (tx + q % 3);
+            xv[q][0] = sx[h * 3 + 0];
+            xv[q][1] = sx[h * 3 + 1];
+            xv[q][2] = sx[h * 3 + 2];
+        }
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+            const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
+            const int bz = lz - cz;
+            if (bz < 0 || bz > 1) continue;
+            const int a = lat_corner(1 - cx, 1 - cy, 1 - cz);
+#pragma unroll
+            for (int q = 0; q < 9; q++) {
+                const int bx = (q % 3) - cx, by = (q / 3) - cy;
+                if (bx < 0 || bx > 1 || by < 0 || by > 1) continue;
+                const int b = bz * 4 + lat_corner(bx, by, 0);
+                const double* K = O.Khat + (a * 8 + b) * 9;
+                tt[c][0] += K[0] * xv[q][0] + K[1] * xv[q][1] + K[2] * xv[q][2];
+                tt[c][1] += K[3] * xv[q][0] + K[4] * xv[q][1] + K[5] * xv[q][2];
+                tt[c][2] += K[6] * xv[q][0] + K[7] * xv[q][1] + K[8] * xv[q][2];
+            }
+        }
+    }
+    double y0 = 0.0, y1 = 0.0, y2 = 0.0;
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+        y0 += sc[c] * tt[c][0];
+        y1 += sc[c] * tt[c][1];
+        y2 += sc[c] * tt[c][2];
+    }
+    double dsum = 0.0;
+    if (valid) {
+        const int64_t r = O.rank_of_lat[i + nxp * (j + nyp * int64_t(k))];
+        const int m = O.nodecon[r];
+        if (m & 1) y0 = cnt * in[r * 3 + 0];
+        if (m & 2) y1 = cnt * in[r * 3 + 1];
+        if (m & 4) y2 = cnt * in[r * 3 + 2];
+        if (KIND == 0) {
+            ycg[r * 3 + 0] = y0;
+            ycg[r * 3 + 1] = y1;
+            ycg[r * 3 + 2] = y2;
+            dsum = y0 * in[r * 3 + 0] + y1 * in[r * 3 + 1] + y2 * in[r * 3 + 2];
+        } else {
+            const double ith = L.cheb[0];
+            const double c1 = step > 0 ? L.cheb[2 * step] : 0.0, c2 = step > 0 ? L.cheb[2 * step + 1] : 0.0;
+            mg_epilogue(L, mode, r * 3 + 0, y0, din, dout, c1, c2, ith);
+            mg_epilogue(L, mode, r * 3 + 1, y1, din, dout, c1, c2, ith);
+            mg_epilogue(L, mode, r * 3 + 2, y2, din, dout, c1, c2, ith);
+        }
+    }
+    if (KIND == 0) block_sum_store(dsum, C.part0, blockIdx.x);
+}
+
 // dinv (when wanted) and the Gershgorin bound over node blocks, 8 lanes per node
 __global__ __launch_bounds__(256) void k_mg_rowinfo_nb(int64_t n_nodes, const int64_t* __restrict__ blk_ptr,
                                                        const int32_t* __restrict__ blk_col,
@@ -716,11 +833,19 @@ void launch_mg_rowinfo_nb(int64_t n_nodes, const int64_t* blk_ptr, const int32_t
     hipLaunchKernelGGL(k_mg_rowinfo_nb, dim3(nb), dim3(256), 0, s, n_nodes, blk_ptr, blk_col, val, dinv, part);
     hipLaunchKernelGGL(k_mg_max, dim3(1), dim3(256), 0, s, part, nb, lam);
 }
+int64_t lattice_partials(const LatticeOpArgs& l) {
+    if (l.tiled) return lattice_tiles(l.nx, l.ny, l.nz);
+    return (int64_t(l.nx + 1) * (l.ny + 1) * (l.nz + 1) + 255) / 256;
+}
 void launch_lattice_apply(const CGArgs& c, const double* x, double* y, hipStream_t s) {
-    const int64_t n_lat = int64_t(c.lat.nx + 1) * (c.lat.ny + 1) * (c.lat.nz + 1);
     MGLevelArgs none{};
-    hipLaunchKernelGGL(k_lattice_apply<0>, dim3(mg_blocks(n_lat, 256)), dim3(256), 0, s, c, x, y, none, 0, 0,
-                       nullptr, nullptr, c.S ? &c.S->done : nullptr);
+    const int* done = c.S ? &c.S->done : nullptr;
+    if (c.lat.tiled)
+        hipLaunchKernelGGL(k_lattice_tiled<0>, dim3(unsigned(lattice_tiles(c.lat.nx, c.lat.ny, c.lat.nz))), dim3(256), 0,
+                           s, c, x, y, none, 0, 0, nullptr, nullptr, done);
+    else
+        hipLaunchKernelGGL(k_lattice_apply<0>, dim3(unsigned(lattice_partials(c.lat))), dim3(256), 0, s, c, x, y, none,
+                           0, 0, nullptr, nullptr, done);
 }
 void launch_mg_pre_init(const MGLevelArgs& L, const int* done, hipStream_t s) {
     hipLaunchKernelGGL(k_mg_pre_init, dim3(mg_grid(L.n_nodes * 3)), dim3(256), 0, s, L, done);
@@ -737,9 +862,12 @@ void launch_mg_bsr(const MGLevelArgs& L, int mode, int step, const double* din, 
 void launch_mg_fine(const CGArgs& c, const MGLevelArgs& L, int mode, int step, const double* din,
                     double* dout, const int* done, hipStream_t s) {
     if (c.lat.rank_of_lat) {
-        const int64_t n_lat = int64_t(c.lat.nx + 1) * (c.lat.ny + 1) * (c.lat.nz + 1);
-        hipLaunchKernelGGL(k_lattice_apply<1>, dim3(mg_blocks(n_lat, 256)), dim3(256), 0, s, c, nullptr, nullptr, L,
-                           mode, step, din, dout, done);
+        if (c.lat.tiled)
+            hipLaunchKernelGGL(k_lattice_tiled<1>, dim3(unsigned(lattice_tiles(c.lat.nx, c.lat.ny, c.lat.nz))), dim3(256),
+                               0, s, c, nullptr, nullptr, L, mode, step, din, dout, done);
+        else
+            hipLaunchKernelGGL(k_lattice_apply<1>, dim3(unsigned(lattice_partials(c.lat))), dim3(256), 0, s, c, nullptr,
+                               nullptr, L, mode, step, din, dout, done);
         return;
     }
     const double* in = mode == MG_RESID_INIT ? L.x : din;
