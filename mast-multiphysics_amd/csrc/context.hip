@@ -57,7 +57,7 @@ struct KTimer {   // accumulated device time of one kernel kind
 struct MGLevelDev {
     MGLevelArgs a{};
     int64_t n_elems = 0;
-    DevBuf rank_of_lat, elem_of_cell;                  // level 0
+    DevBuf rank_of_lat, elem_of_cell, con_lat, nodecon_lat, dinv_lat, b_lat;   // level 0
     DevBuf blk_ptr, blk_col, cptr, cidx, econn, con;   // levels >= 1
     DevBuf Kel, val, val32, dinv, r, x, res, da, db;
 };
@@ -521,8 +521,8 @@ int mg_build(mastg_ctx* c) {
     if (c->mg_tried) return c->mg.empty() ? fail(c, MASTG_ERR_INVALID, "multigrid unavailable: " + c->mg_why)
                                           : MASTG_OK;
     c->mg_tried = true;
-    if (c->d.physics != MASTG_HEX8_ELASTIC || c->S.n_classes == 0) {
-        c->mg_why = "multigrid is built for hex8 elasticity on lattice meshes";
+    if (c->d.physics != MASTG_HEX8_ELASTIC || c->S.n_classes != 1) {
+        c->mg_why = "multigrid is built for hex8 elasticity on lattice meshes (one element shape)";
         return fail(c, MASTG_ERR_INVALID, "multigrid unavailable: " + c->mg_why);
     }
     MGHierarchy H = build_mg_hierarchy(c->S);
@@ -541,11 +541,25 @@ int mg_build(mastg_ctx* c) {
         L->n_elems = h.n_elems;
         const size_t nv = size_t(h.n_nodes) * 3 * sizeof(double);
         if (l == 0) {
+            // the problem level runs in lattice order (x fastest): its vectors
+            // are permuted from / to the caller's node-rank order at the solve
+            std::vector<uint8_t> con_lat(size_t(h.n_nodes) * 3), nodecon_lat(size_t(h.n_nodes));
+            for (int64_t p = 0; p < h.n_nodes; p++) {
+                uint8_t bits = 0;
+                for (int v = 0; v < 3; v++) {
+                    con_lat[p * 3 + v] = c->S.constrained[size_t(h.rank_of_lat[p]) * 3 + v];
+                    if (con_lat[p * 3 + v]) bits |= uint8_t(1 << v);
+                }
+                nodecon_lat[p] = bits;
+            }
             TRY(upload(c, L->rank_of_lat, h.rank_of_lat));
             TRY(upload(c, L->elem_of_cell, h.elem_of_cell));
-            a.rank_of_lat = L->rank_of_lat.as<int32_t>();
-            a.con = c->con.as<uint8_t>();
-            a.dinv = c->dinv.as<double>();
+            TRY(upload(c, L->con_lat, con_lat));
+            TRY(upload(c, L->nodecon_lat, nodecon_lat));
+            TRY(dalloc(c, L->dinv_lat, nv));
+            TRY(dalloc(c, L->b_lat, nv));
+            a.con = L->con_lat.as<uint8_t>();
+            a.dinv = L->dinv_lat.as<double>();
             a.r = c->vr.as<double>();
         } else {
             const Structure& S = h.S;
@@ -560,7 +574,6 @@ int mg_build(mastg_ctx* c) {
             TRY(dalloc(c, L->val32, size_t(S.n_blocks) * 9 * sizeof(float)));
             TRY(dalloc(c, L->dinv, nv));
             TRY(dalloc(c, L->r, nv));
-            a.rank_of_lat = nullptr;    // coarse levels are numbered in lattice order
             a.con = L->con.as<uint8_t>();
             a.blk_ptr = L->blk_ptr.as<int64_t>();
             a.blk_col = L->blk_col.as<int32_t>();
@@ -601,6 +614,8 @@ int mg_setup(mastg_ctx* c) {
     hipStream_t s = c->stream;
     launch_mg_rowinfo_nb(c->S.n_nodes, c->blk_ptr.as<int64_t>(), c->blk_col.as<int32_t>(), c->val.as<double>(),
                          nullptr, c->mg_part.as<double>(), c->mg_lam.as<double>(), s);
+    launch_perm_nodes(c->S.n_nodes, c->mg[0]->rank_of_lat.as<int32_t>(), c->dinv.as<double>(),
+                      c->mg[0]->dinv_lat.as<double>(), 0, s);
     for (int l = 0; l + 1 < nl; l++) {
         MGLevelDev& F = *c->mg[l];
         MGLevelDev& C = *c->mg[l + 1];
@@ -610,7 +625,7 @@ int mg_setup(mastg_ctx* c) {
         g.fnx = F.a.nx; g.fny = F.a.ny;
         g.f_elem_of_cell = l == 0 ? F.elem_of_cell.as<int32_t>() : nullptr;
         g.f_econn = l == 0 ? c->econn.as<int32_t>() : F.econn.as<int32_t>();
-        g.f_con = F.a.con;
+        g.f_con = l == 0 ? c->con.as<uint8_t>() : F.a.con;   // by node rank, as f_econn
         g.c_econn = C.econn.as<int32_t>();
         g.c_con = C.a.con;
         g.Kf = l == 0 ? c->Kel.as<double>() : F.Kel.as<double>();
@@ -637,18 +652,14 @@ int mg_setup(mastg_ctx* c) {
 // the per-node lattice operator's arguments (built multigrid hierarchy, one
 // element class); false when the mesh does not qualify
 bool lattice_args(mastg_ctx* c, CGArgs& a) {
-    if (c->mg.empty() || c->S.n_classes != 1) return false;
+    if (c->mg.empty()) return false;
     const MGLevelDev& L0 = *c->mg[0];
     a.lat.nx = L0.a.nx; a.lat.ny = L0.a.ny; a.lat.nz = L0.a.nz;
-    a.lat.rank_of_lat = L0.rank_of_lat.as<int32_t>();
     a.lat.elem_of_cell = L0.elem_of_cell.as<int32_t>();
     a.lat.sE = c->sE.as<double>();
     a.lat.Khat = c->Khat.as<double>();
-    a.lat.nodecon = c->nodecon.as<uint8_t>();
-    // LDS bricks when the partial-sum buffer holds one entry per brick
-    a.lat.tiled = 1;
-    if (lattice_partials(a.lat) > int64_t(c->part0.bytes / sizeof(double))) a.lat.tiled = 0;
-    return true;
+    a.lat.nodecon = L0.nodecon_lat.as<uint8_t>();
+    return lattice_partials(a.lat) <= int64_t(c->part0.bytes / sizeof(double));
 }
 
 // one V-cycle on a.r; the level-0 output is x0 + dlast0 (formed by the caller)
@@ -704,18 +715,18 @@ int mg_solve(mastg_ctx* c, const double* b, const mastg_solve_opts* o, mastg_sol
     HIPC(c, ev_create(&e1));
     HIPC(c, hipEventRecord(e0, c->stream));
     TRY(mg_setup(c));
+    // the whole PCG runs in lattice order: b is permuted in, x out
     CGArgs a = cg_args(c);
-    a.use_mf = 1;
-    a.nparts_op = mf_node_blocks(S.n_nodes);
-    // per-node lattice operator for the CG product and the level-0 smoothing
-    if (lattice_args(c, a)) {
-        a.use_mf = 3;
-        a.nparts_op = int(lattice_partials(a.lat));
-    }
+    if (!lattice_args(c, a)) return fail(c, MASTG_ERR_INVALID, "partial-sum buffer too small for the lattice");
+    a.use_mf = 3;
+    a.nparts_op = int(lattice_partials(a.lat));
+    const int32_t* lat_map = c->mg[0]->rank_of_lat.as<int32_t>();
+    double* bl = c->mg[0]->b_lat.as<double>();
+    launch_perm_nodes(S.n_nodes, lat_map, b, bl, 0, c->stream);
     c->cg_replacements = 0;
     launch_fill(S.n_active, 0.0, a.x, c->stream);
-    HIPC(c, hipMemcpyAsync(a.r, b, S.n_active * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-    launch_pcg_start(a, b, rtol * rtol, max_it, c->stream);
+    HIPC(c, hipMemcpyAsync(a.r, bl, S.n_active * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    launch_pcg_start(a, bl, rtol * rtol, max_it, c->stream);
     const double* dlast = mg_vcycle(c, a, &a.S->done, c->stream);
     launch_pcg_rz(a, c->mg[0]->a.x, dlast, c->stream);
     launch_pcg_start2(a, c->stream);
@@ -743,6 +754,9 @@ int mg_solve(mastg_ctx* c, const double* b, const mastg_solve_opts* o, mastg_sol
         HIPC(c, hipStreamSynchronize(c->stream));
         done = c->h_scal->done;
     }
+    // back to rank order (x lives in c->vx for the callers)
+    launch_perm_nodes(S.n_nodes, lat_map, a.x, a.z, 1, c->stream);
+    HIPC(c, hipMemcpyAsync(a.x, a.z, S.n_active * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     HIPC(c, hipGetLastError());
     return cg_finish(c, a, b, e0, e1, st, done);
 }
@@ -1332,13 +1346,18 @@ int mastg_jacobian_product(mastg_ctx* c, const double* dX, double* JdX, int wher
     CGArgs a = cg_args(c);
     if (op == 3) {
         if (!lattice_args(c, a)) return fail(c, MASTG_ERR_INVALID, "no lattice operator for this mesh");
+        a.S = nullptr;
         if (!c->sE_valid) {
             launch_elem_scale(c->S.n_elems, c->param.as<double>(), c->d.E, c->d.penalty, c->sE.as<double>(),
                               c->stream);
             c->sE_valid = true;
         }
+        const MGLevelDev& L0 = *c->mg[0];
+        const int32_t* lat_map = L0.rank_of_lat.as<int32_t>();
         launch_mask_constrained(c->S.n_active, c->con.as<uint8_t>(), c->vb.as<double>(), c->stream);
-        launch_lattice_apply(a, c->vb.as<double>(), c->vtmp.as<double>(), c->stream);
+        launch_perm_nodes(c->S.n_nodes, lat_map, c->vb.as<double>(), L0.a.x, 0, c->stream);
+        launch_lattice_apply(a, L0.a.x, L0.a.res, c->stream);
+        launch_perm_nodes(c->S.n_nodes, lat_map, L0.a.res, c->vtmp.as<double>(), 1, c->stream);
     } else if (op == 2) {
         launch_spmv_nb32(a, c->vb.as<double>(), c->vtmp.as<double>(), 0, c->stream);
     } else if (op == 1) {
@@ -1537,7 +1556,7 @@ int mastg_bench_kernel(mastg_ctx* c, int kind, int n_launch, double* avg_ms) {
                 break;
             case 4: launch_sens(c, a.x, a.x, c->vg.as<double>()); break;
             case 5: launch_hex8_K_f32(hex8_args(c, nullptr, false), c->Kel32.as<float>(), c->stream); break;
-            case 6: launch_lattice_apply(a, a.p, a.q, c->stream); break;
+            case 6: launch_lattice_apply(a, c->mg[0]->a.da, c->mg[0]->a.db, c->stream); break;
             default: break;
         }
     };

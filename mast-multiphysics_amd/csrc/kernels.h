@@ -78,17 +78,16 @@ struct MFArgs {
     double* Fel;              // [24][n_elems] element forces (SoA), phase 1 -> phase 2
 };
 
-// matrix-free hex8 operator on a lattice mesh, evaluated per node: y_n =
-// sum over the (up to 8) cells at n of s_e Khat[a_n][b] x_b, constrained rows
-// as diag = cells at the node (inputs must be zero on constrained DOFs)
+// matrix-free hex8 operator on a lattice mesh, evaluated per node with
+// vectors in lattice order: y_n = sum over the (up to 8) cells at n of
+// s_e Khat[a_n][b] x_b, constrained rows as diag = cells at the node (inputs
+// must be zero on constrained DOFs)
 struct LatticeOpArgs {
     int nx, ny, nz;                    // cells per dimension
-    const int32_t* rank_of_lat;        // node rank of lattice point
-    const int32_t* elem_of_cell;       // element of lattice cell
+    const int32_t* elem_of_cell;       // element of lattice cell (null: no lattice operator)
     const double* sE;                  // [n_elems] element scales
     const double* Khat;                // [64*9] the (single) class matrix at E = 1
-    const uint8_t* nodecon;            // [n_nodes] bit v = variable v constrained
-    int tiled;                         // 1: LDS-staged 8x8x4 bricks, 0: one thread per node
+    const uint8_t* nodecon;            // [lattice points] bit v = variable v constrained
 };
 // partial sums the lattice operator writes (its grid)
 int64_t lattice_partials(const LatticeOpArgs& l);
@@ -180,8 +179,7 @@ constexpr int kMGChebStride = 16;      // coefficients per level: 1/theta, (c1_i
 
 struct MGLevelArgs {
     int64_t n_nodes;
-    int nx, ny, nz;                    // lattice cells per dimension
-    const int32_t* rank_of_lat;        // node rank of lattice point; null = identity
+    int nx, ny, nz;                    // lattice cells per dimension (vectors in lattice order)
     const uint8_t* con;                // [3 n_nodes] constrained DOFs
     const int64_t* blk_ptr;            // operator of levels >= 1: node-block CSR, FP64
     const int32_t* blk_col;
@@ -224,10 +222,15 @@ void launch_mg_invert(int n, double* A, hipStream_t s);
 void launch_mg_pre_init(const MGLevelArgs& L, const int* done, hipStream_t s);
 void launch_mg_bsr(const MGLevelArgs& L, int mode, int step, const double* din, double* dout, const int* done,
                    hipStream_t s);
+// level 0: the lattice operator (c.lat) with the smoothing epilogue
 void launch_mg_fine(const CGArgs& c, const MGLevelArgs& L, int mode, int step, const double* din,
                     double* dout, const int* done, hipStream_t s);
-// y = A x with the lattice operator and partial sums of x.y into c.part0
+// y = A x with the lattice operator (lattice-ordered vectors) and partial sums
+// of x.y into c.part0
 void launch_lattice_apply(const CGArgs& c, const double* x, double* y, hipStream_t s);
+// rank order <-> lattice order: gather dst[p] = src[map[p]] or scatter dst[map[p]] = src[p]
+void launch_perm_nodes(int64_t n_nodes, const int32_t* map, const double* src, double* dst, int scatter,
+                       hipStream_t s);
 void launch_mg_rowinfo_nb(int64_t n_nodes, const int64_t* blk_ptr, const int32_t* blk_col, const double* val,
                           double* dinv, double* part, double* lam, hipStream_t s);
 void launch_mg_restrict(const MGLevelArgs& F, const MGLevelArgs& C, const int* done, hipStream_t s);

@@ -258,44 +258,12 @@ __global__ __launch_bounds__(256) void k_mg_bsr(MGLevelArgs L, const T* __restri
     }
 }
 
-// level 0: node phase of the matrix-free operator (element phase ran on `in`)
-// with the fused epilogue; constrained rows act as diag = incident elements
-__global__ __launch_bounds__(256) void k_mg_fine_node(CGArgs C, MGLevelArgs L, int mode, int step,
-                                                      const double* __restrict__ din, double* __restrict__ dout,
-                                                      const int* __restrict__ done) {
-    if (done && *done) return;
-    const MFArgs& A = C.mf;
-    const int64_t r = int64_t(blockIdx.x) * 256 + threadIdx.x;
-    if (r >= A.n_nodes) return;
-    const double* __restrict__ in = mode == MG_RESID_INIT ? L.x : din;
-    const int32_t k0 = A.ninc_ptr[r], k1 = A.ninc_ptr[r + 1];
-    const int64_t ne = A.n_elems;
-    double y[3] = {0.0, 0.0, 0.0};
-    for (int32_t k = k0; k < k1; k++) {
-        const int32_t c = A.ninc[k];
-        const int64_t e = c >> 3;
-        const int la = c & 7;
-        y[0] += A.Fel[(la * 3 + 0) * ne + e];
-        y[1] += A.Fel[(la * 3 + 1) * ne + e];
-        y[2] += A.Fel[(la * 3 + 2) * ne + e];
-    }
-    const int m = A.nodecon[r];
-    const double cnt = double(k1 - k0);
-    const double ith = L.cheb[0];
-    const double c1 = step > 0 ? L.cheb[2 * step] : 0.0, c2 = step > 0 ? L.cheb[2 * step + 1] : 0.0;
-#pragma unroll
-    for (int v = 0; v < 3; v++) {
-        const double yv = (m >> v) & 1 ? cnt * in[r * 3 + v] : y[v];
-        mg_epilogue(L, mode, r * 3 + v, yv, din, dout, c1, c2, ith);
-    }
-}
-
 // ---- level 0 on a lattice: per-node matrix-free operator -------------------
-// Thread per lattice point n; for each existing cell at n (local corner a) the
-// 8 cell nodes' values are gathered (cached; the 27 neighbour ranks are looked
-// up once) and y_n += s_e Khat[a][b] x_b.  Khat indices are compile-time after
-// unrolling, so the class matrix is read with uniform (scalar-cache) loads.
-// Each y_n is a fixed-order sum: results are deterministic.
+// Thread per lattice point n, vectors in lattice order (x fastest), so the
+// neighbour gathers of a wave are contiguous; for each existing cell at n
+// (local corner a) y_n += s_e Khat[a][b] x_b.  Khat indices are uniform
+// across the wave, so the class matrix is read with scalar-cache loads.  Each
+// y_n is a fixed-order sum: results are deterministic.
 __host__ __device__ constexpr int lat_cx(int b) { return (b == 1 || b == 2 || b == 5 || b == 6) ? 1 : 0; }
 __host__ __device__ constexpr int lat_cy(int b) { return (b == 2 || b == 3 || b == 6 || b == 7) ? 1 : 0; }
 __host__ __device__ constexpr int lat_cz(int b) { return b >= 4 ? 1 : 0; }
@@ -337,11 +305,11 @@ __global__ __launch_bounds__(256) void k_lattice_apply(CGArgs C, const double* _
     }
     // neighbour-outer, one z-layer of 9 neighbours at a time (bounded live
     // ranges): each neighbour value is read once and feeds every cell shared
-    // with n, t_c += Khat[a_c][b] x_o.  Neighbours outside the lattice are
-    // clamped into it; every cell holding one is missing and scaled by 0.
-    double t[8][3];
-#pragma unroll
-    for (int c = 0; c < 8; c++) t[c][0] = t[c][1] = t[c][2] = 0.0;
+    // with n, y += s_c Khat[a_c][b] x_o (scaled per block product, so only y
+    // is carried: fewer registers, more waves in flight to hide the gathers).
+    // Neighbours outside the lattice are clamped into it; every cell holding
+    // one is missing and scaled by 0.
+    double y0 = 0.0, y1 = 0.0, y2 = 0.0;
 #pragma unroll 1
     for (int lz = 0; lz < 3; lz++) {
         const int mk = min(max(k + lz - 1, 0), O.nz);
@@ -350,7 +318,7 @@ __global__ __launch_bounds__(256) void k_lattice_apply(CGArgs C, const double* _
         for (int q = 0; q < 9; q++) {
             const int mi = min(max(i + (q % 3) - 1, 0), O.nx);
             const int mj = min(max(j + (q / 3) - 1, 0), O.ny);
-            const int64_t rm = int64_t(O.rank_of_lat[mi + nxp * (mj + nyp * int64_t(mk))]) * 3;
+            const int64_t rm = (mi + nxp * (mj + nyp * int64_t(mk))) * 3;
             xv[q][0] = in[rm];
             xv[q][1] = in[rm + 1];
             xv[q][2] = in[rm + 2];
@@ -361,26 +329,23 @@ __global__ __launch_bounds__(256) void k_lattice_apply(CGArgs C, const double* _
             const int bz = lz - cz;                      // uniform across the wave
             if (bz < 0 || bz > 1) continue;
             const int a = lat_corner(1 - cx, 1 - cy, 1 - cz);
+            double t0 = 0.0, t1 = 0.0, t2 = 0.0;
 #pragma unroll
             for (int q = 0; q < 9; q++) {
                 const int bx = (q % 3) - cx, by = (q / 3) - cy;
                 if (bx < 0 || bx > 1 || by < 0 || by > 1) continue;   // compile time
                 const int b = bz * 4 + lat_corner(bx, by, 0);
                 const double* K = O.Khat + (a * 8 + b) * 9;
-                t[c][0] += K[0] * xv[q][0] + K[1] * xv[q][1] + K[2] * xv[q][2];
-                t[c][1] += K[3] * xv[q][0] + K[4] * xv[q][1] + K[5] * xv[q][2];
-                t[c][2] += K[6] * xv[q][0] + K[7] * xv[q][1] + K[8] * xv[q][2];
+                t0 += K[0] * xv[q][0] + K[1] * xv[q][1] + K[2] * xv[q][2];
+                t1 += K[3] * xv[q][0] + K[4] * xv[q][1] + K[5] * xv[q][2];
+                t2 += K[6] * xv[q][0] + K[7] * xv[q][1] + K[8] * xv[q][2];
             }
+            y0 += sc[c] * t0;
+            y1 += sc[c] * t1;
+            y2 += sc[c] * t2;
         }
     }
-    double y0 = 0.0, y1 = 0.0, y2 = 0.0;
-#pragma unroll
-    for (int c = 0; c < 8; c++) {
-        y0 += sc[c] * t[c][0];
-        y1 += sc[c] * t[c][1];
-        y2 += sc[c] * t[c][2];
-    }
-    const int64_t r = valid ? int64_t(O.rank_of_lat[pv]) : 0;
+    const int64_t r = pv;
     double dsum = 0.0;
     if (valid) {
         const int m = O.nodecon[r];
@@ -403,121 +368,19 @@ __global__ __launch_bounds__(256) void k_lattice_apply(CGArgs C, const double* _
     if (KIND == 0) block_sum_store(dsum, C.part0, blockIdx.x);
 }
 
-// Tiled variant: a block owns an 8x8x4 brick of lattice points; the brick's
-// halo values (10x10x6 points) and cell scales (9x9x5 cells) are staged in
-// LDS once, so every gathered value is read from HBM / L2 once per brick
-// instead of once per neighbouring node.  Same arithmetic order as above.
-constexpr int kLTX = 8, kLTY = 8, kLTZ = 4;
-__host__ __device__ inline int64_t lattice_tiles(int nx, int ny, int nz) {
-    return int64_t((nx + kLTX) / kLTX) * ((ny + kLTY) / kLTY) * ((nz + kLTZ) / kLTZ);
-}
-
-template <int KIND>
-__global__ __launch_bounds__(256) void k_lattice_tiled(CGArgs C, const double* __restrict__ xcg,
-                                                       double* __restrict__ ycg, MGLevelArgs L, int mode, int step,
-                                                       const double* __restrict__ din, double* __restrict__ dout,
-                                                       const int* __restrict__ done) {
-    if (done && *done) return;
-    constexpr int HX = kLTX + 2, HY = kLTY + 2, HZ = kLTZ + 2;   // halo points
-    constexpr int CX = kLTX + 1, CY = kLTY + 1, CZ = kLTZ + 1;   // cells
-    __shared__ double sx[HZ * HY * HX * 3];
-    __shared__ double ssc[CZ * CY * CX];
-    const LatticeOpArgs& O = C.lat;
-    const int64_t nxp = O.nx + 1, nyp = O.ny + 1;
-    const int ntx = (O.nx + kLTX) / kLTX, nty = (O.ny + kLTY) / kLTY;
-    const int64_t t = blockIdx.x;
-    const int i0 = int(t % ntx) * kLTX, j0 = int((t / ntx) % nty) * kLTY, k0 = int(t / (int64_t(ntx) * nty)) * kLTZ;
-    const double* __restrict__ in = KIND == 0 ? xcg : (mode == MG_RESID_INIT ? L.x : din);
-    for (int h = threadIdx.x; h < HZ * HY * HX; h += 256) {
-        const int hx = h % HX, hy = (h / HX) % HY, hz = h / (HX * HY);
-        const int gi = min(max(i0 - 1 + hx, 0), O.nx), gj = min(max(j0 - 1 + hy, 0), O.ny),
-                  gk = min(max(k0 - 1 + hz, 0), O.nz);
-        const int64_t rm = int64_t(O.rank_of_lat[gi + nxp * (gj + nyp * int64_t(gk))]) * 3;
-        sx[h * 3 + 0] = in[rm];
-        sx[h * 3 + 1] = in[rm + 1];
-        sx[h * 3 + 2] = in[rm + 2];
+// node-vector permutation between rank order and lattice order (3 values per
+// node): gather dst[p] = src[map[p]], or scatter dst[map[p]] = src[p]
+__global__ __launch_bounds__(256) void k_perm_nodes(int64_t n_nodes, const int32_t* __restrict__ map,
+                                                    const double* __restrict__ src, double* __restrict__ dst,
+                                                    int scatter) {
+    const int64_t p = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (p >= n_nodes) return;
+    const int64_t m = int64_t(map[p]) * 3, q = p * 3;
+    if (scatter) {
+        dst[m] = src[q]; dst[m + 1] = src[q + 1]; dst[m + 2] = src[q + 2];
+    } else {
+        dst[q] = src[m]; dst[q + 1] = src[m + 1]; dst[q + 2] = src[m + 2];
     }
-    for (int h = threadIdx.x; h < CZ * CY * CX; h += 256) {
-        const int hx = h % CX, hy = (h / CX) % CY, hz = h / (CX * CY);
-        const int ci = i0 - 1 + hx, cj = j0 - 1 + hy, ck = k0 - 1 + hz;
-        const bool has = ci >= 0 && ci < O.nx && cj >= 0 && cj < O.ny && ck >= 0 && ck < O.nz;
-        const int64_t cell = min(max(ci, 0), O.nx - 1) +
-                             int64_t(O.nx) * (min(max(cj, 0), O.ny - 1) + int64_t(O.ny) * min(max(ck, 0), O.nz - 1));
-        const double sv = O.sE[O.elem_of_cell[cell]];
-        ssc[h] = has ? sv : 0.0;
-    }
-    __syncthreads();
-    const int tx = threadIdx.x % kLTX, ty = (threadIdx.x / kLTX) % kLTY, tz = threadIdx.x / (kLTX * kLTY);
-    const int i = i0 + tx, j = j0 + ty, k = k0 + tz;
-    const bool valid = i <= O.nx && j <= O.ny && k <= O.nz;
-    double sc[8];
-    int cnt = 0;
-#pragma unroll
-    for (int c = 0; c < 8; c++) {
-        const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
-        sc[c] = ssc[(tz + cz) * (CY * CX) + (ty + cy) * CX + (tx + cx)];
-        const int ci = i - 1 + cx, cj = j - 1 + cy, ck = k - 1 + cz;
-        cnt += (ci >= 0 && ci < O.nx && cj >= 0 && cj < O.ny && ck >= 0 && ck < O.nz) ? 1 : 0;
-    }
-    double tt[8][3];
-#pragma unroll
-    for (int c = 0; c < 8; c++) tt[c][0] = tt[c][1] = tt[c][2] = 0.0;
-#pragma unroll 1
-    for (int lz = 0; lz < 3; lz++) {
-        double xv[9][3];
-#pragma unroll
-        for (int q = 0; q < 9; q++) {
-            const int h = (tz + lz) * (HY * HX) + (ty + q / 3) * HX + (tx + q % 3);
-            xv[q][0] = sx[h * 3 + 0];
-            xv[q][1] = sx[h * 3 + 1];
-            xv[q][2] = sx[h * 3 + 2];
-        }
-#pragma unroll
-        for (int c = 0; c < 8; c++) {
-            const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
-            const int bz = lz - cz;
-            if (bz < 0 || bz > 1) continue;
-            const int a = lat_corner(1 - cx, 1 - cy, 1 - cz);
-#pragma unroll
-            for (int q = 0; q < 9; q++) {
-                const int bx = (q % 3) - cx, by = (q / 3) - cy;
-                if (bx < 0 || bx > 1 || by < 0 || by > 1) continue;
-                const int b = bz * 4 + lat_corner(bx, by, 0);
-                const double* K = O.Khat + (a * 8 + b) * 9;
-                tt[c][0] += K[0] * xv[q][0] + K[1] * xv[q][1] + K[2] * xv[q][2];
-                tt[c][1] += K[3] * xv[q][0] + K[4] * xv[q][1] + K[5] * xv[q][2];
-                tt[c][2] += K[6] * xv[q][0] + K[7] * xv[q][1] + K[8] * xv[q][2];
-            }
-        }
-    }
-    double y0 = 0.0, y1 = 0.0, y2 = 0.0;
-#pragma unroll
-    for (int c = 0; c < 8; c++) {
-        y0 += sc[c] * tt[c][0];
-        y1 += sc[c] * tt[c][1];
-        y2 += sc[c] * tt[c][2];
-    }
-    double dsum = 0.0;
-    if (valid) {
-        const int64_t r = O.rank_of_lat[i + nxp * (j + nyp * int64_t(k))];
-        const int m = O.nodecon[r];
-        if (m & 1) y0 = cnt * in[r * 3 + 0];
-        if (m & 2) y1 = cnt * in[r * 3 + 1];
-        if (m & 4) y2 = cnt * in[r * 3 + 2];
-        if (KIND == 0) {
-            ycg[r * 3 + 0] = y0;
-            ycg[r * 3 + 1] = y1;
-            ycg[r * 3 + 2] = y2;
-            dsum = y0 * in[r * 3 + 0] + y1 * in[r * 3 + 1] + y2 * in[r * 3 + 2];
-        } else {
-            const double ith = L.cheb[0];
-            const double c1 = step > 0 ? L.cheb[2 * step] : 0.0, c2 = step > 0 ? L.cheb[2 * step + 1] : 0.0;
-            mg_epilogue(L, mode, r * 3 + 0, y0, din, dout, c1, c2, ith);
-            mg_epilogue(L, mode, r * 3 + 1, y1, din, dout, c1, c2, ith);
-            mg_epilogue(L, mode, r * 3 + 2, y2, din, dout, c1, c2, ith);
-        }
-    }
-    if (KIND == 0) block_sum_store(dsum, C.part0, blockIdx.x);
 }
 
 // dinv (when wanted) and the Gershgorin bound over node blocks, 8 lanes per node
@@ -587,8 +450,6 @@ __global__ __launch_bounds__(1024) void k_mg_invert_lds(int n, double* __restric
 // ---- transfers ---------------------------------------------------------------
 // r_C = sum over the 27 fine points around 2C of w * res_f (w = prod (1 or 1/2)),
 // constrained fine DOFs skipped, constrained coarse DOFs zero (P^T)
-// FR: the fine level is numbered by node rank (level 0); coarse levels are in lattice order
-template <bool FR>
 __global__ __launch_bounds__(256) void k_mg_restrict(MGLevelArgs F, MGLevelArgs Cl, const int* __restrict__ done) {
     if (done && *done) return;
     const int64_t p = int64_t(blockIdx.x) * 256 + threadIdx.x;
@@ -607,7 +468,7 @@ __global__ __launch_bounds__(256) void k_mg_restrict(MGLevelArgs F, MGLevelArgs 
         // constrained fine DOFs are weighted 0 (P has no row for them)
         const double w = in_lat ? (ok ? 0.5 : 1.0) * (oj ? 0.5 : 1.0) * (oi ? 0.5 : 1.0) : 0.0;
         const int64_t fp = (in_lat ? fi : 0) + fx * ((in_lat ? fj : 0) + fy * (in_lat ? fk : 0));
-        const int64_t rf = (FR ? int64_t(F.rank_of_lat[fp]) : fp) * 3;
+        const int64_t rf = fp * 3;
         s0 += (w * (1.0 - double(F.con[rf + 0]))) * F.res[rf + 0];
         s1 += (w * (1.0 - double(F.con[rf + 1]))) * F.res[rf + 1];
         s2 += (w * (1.0 - double(F.con[rf + 2]))) * F.res[rf + 2];
@@ -619,7 +480,6 @@ __global__ __launch_bounds__(256) void k_mg_restrict(MGLevelArgs F, MGLevelArgs 
 }
 
 // x_f += P (x_C + dlast_C) at unconstrained fine DOFs (trilinear interpolation)
-template <bool FR>
 __global__ __launch_bounds__(256) void k_mg_prolong(MGLevelArgs F, MGLevelArgs Cl, const double* __restrict__ dlast,
                                                     const int* __restrict__ done) {
     if (done && *done) return;
@@ -650,7 +510,7 @@ __global__ __launch_bounds__(256) void k_mg_prolong(MGLevelArgs F, MGLevelArgs C
             s2 += w * Cl.x[rc + 2];
         }
     }
-    const int64_t rf = (FR ? int64_t(F.rank_of_lat[p]) : p) * 3;
+    const int64_t rf = p * 3;
     if (!F.con[rf + 0]) F.x[rf + 0] += s0;
     if (!F.con[rf + 1]) F.x[rf + 1] += s1;
     if (!F.con[rf + 2]) F.x[rf + 2] += s2;
@@ -834,18 +694,17 @@ void launch_mg_rowinfo_nb(int64_t n_nodes, const int64_t* blk_ptr, const int32_t
     hipLaunchKernelGGL(k_mg_max, dim3(1), dim3(256), 0, s, part, nb, lam);
 }
 int64_t lattice_partials(const LatticeOpArgs& l) {
-    if (l.tiled) return lattice_tiles(l.nx, l.ny, l.nz);
     return (int64_t(l.nx + 1) * (l.ny + 1) * (l.nz + 1) + 255) / 256;
 }
 void launch_lattice_apply(const CGArgs& c, const double* x, double* y, hipStream_t s) {
     MGLevelArgs none{};
     const int* done = c.S ? &c.S->done : nullptr;
-    if (c.lat.tiled)
-        hipLaunchKernelGGL(k_lattice_tiled<0>, dim3(unsigned(lattice_tiles(c.lat.nx, c.lat.ny, c.lat.nz))), dim3(256), 0,
-                           s, c, x, y, none, 0, 0, nullptr, nullptr, done);
-    else
-        hipLaunchKernelGGL(k_lattice_apply<0>, dim3(unsigned(lattice_partials(c.lat))), dim3(256), 0, s, c, x, y, none,
-                           0, 0, nullptr, nullptr, done);
+    hipLaunchKernelGGL(k_lattice_apply<0>, dim3(unsigned(lattice_partials(c.lat))), dim3(256), 0, s, c, x, y, none, 0,
+                       0, nullptr, nullptr, done);
+}
+void launch_perm_nodes(int64_t n_nodes, const int32_t* map, const double* src, double* dst, int scatter,
+                       hipStream_t s) {
+    hipLaunchKernelGGL(k_perm_nodes, dim3(mg_blocks(n_nodes, 256)), dim3(256), 0, s, n_nodes, map, src, dst, scatter);
 }
 void launch_mg_pre_init(const MGLevelArgs& L, const int* done, hipStream_t s) {
     hipLaunchKernelGGL(k_mg_pre_init, dim3(mg_grid(L.n_nodes * 3)), dim3(256), 0, s, L, done);
@@ -861,32 +720,15 @@ void launch_mg_bsr(const MGLevelArgs& L, int mode, int step, const double* din, 
 }
 void launch_mg_fine(const CGArgs& c, const MGLevelArgs& L, int mode, int step, const double* din,
                     double* dout, const int* done, hipStream_t s) {
-    if (c.lat.rank_of_lat) {
-        if (c.lat.tiled)
-            hipLaunchKernelGGL(k_lattice_tiled<1>, dim3(unsigned(lattice_tiles(c.lat.nx, c.lat.ny, c.lat.nz))), dim3(256),
-                               0, s, c, nullptr, nullptr, L, mode, step, din, dout, done);
-        else
-            hipLaunchKernelGGL(k_lattice_apply<1>, dim3(unsigned(lattice_partials(c.lat))), dim3(256), 0, s, c, nullptr,
-                               nullptr, L, mode, step, din, dout, done);
-        return;
-    }
-    const double* in = mode == MG_RESID_INIT ? L.x : din;
-    launch_mf_elem(c, in, s);
-    hipLaunchKernelGGL(k_mg_fine_node, dim3(mg_blocks(c.mf.n_nodes, 256)), dim3(256), 0, s, c, L, mode, step, din,
-                       dout, done);
+    hipLaunchKernelGGL(k_lattice_apply<1>, dim3(unsigned(lattice_partials(c.lat))), dim3(256), 0, s, c, nullptr,
+                       nullptr, L, mode, step, din, dout, done);
 }
 void launch_mg_restrict(const MGLevelArgs& F, const MGLevelArgs& C, const int* done, hipStream_t s) {
-    if (F.rank_of_lat)
-        hipLaunchKernelGGL(k_mg_restrict<true>, dim3(mg_blocks(C.n_nodes, 256)), dim3(256), 0, s, F, C, done);
-    else
-        hipLaunchKernelGGL(k_mg_restrict<false>, dim3(mg_blocks(C.n_nodes, 256)), dim3(256), 0, s, F, C, done);
+    hipLaunchKernelGGL(k_mg_restrict, dim3(mg_blocks(C.n_nodes, 256)), dim3(256), 0, s, F, C, done);
 }
 void launch_mg_prolong(const MGLevelArgs& F, const MGLevelArgs& C, const double* dlast, const int* done,
                        hipStream_t s) {
-    if (F.rank_of_lat)
-        hipLaunchKernelGGL(k_mg_prolong<true>, dim3(mg_blocks(F.n_nodes, 256)), dim3(256), 0, s, F, C, dlast, done);
-    else
-        hipLaunchKernelGGL(k_mg_prolong<false>, dim3(mg_blocks(F.n_nodes, 256)), dim3(256), 0, s, F, C, dlast, done);
+    hipLaunchKernelGGL(k_mg_prolong, dim3(mg_blocks(F.n_nodes, 256)), dim3(256), 0, s, F, C, dlast, done);
 }
 void launch_mg_coarse_solve(int n, const double* Ainv, const double* r, double* x, const int* done,
                             hipStream_t s) {
