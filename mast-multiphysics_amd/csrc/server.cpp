@@ -81,8 +81,10 @@ int64_t footprint(const mastg_request& r) {
         const int64_t nn = int64_t(r.nx + 1) * (r.ny + 1) * (r.nz + 1);
         const int64_t blocks = int64_t(3 * r.nx + 1) * (3 * r.ny + 1) * (3 * r.nz + 1) - 0;
         const int64_t nnz = blocks * 9;
+        // + multigrid levels: coarse element blocks (1/8 + 1/64 + ...), their
+        // node-block values and the lattice-ordered level vectors
         return ne * (64 * 9 * 8 + 64 * 4 + 8 * 4 * 2 + 8 * 3 * 8 + 24 * 8 + 8 * 4) + nnz * 12 +
-               blocks * 8 + nn * 3 * 8 * 12 + (int64_t(64) << 20);
+               blocks * 8 + nn * 3 * 8 * 12 + ne * 700 + nn * 3 * 8 * 8 + (int64_t(64) << 20);
     }
     const int ny = r.ny > 0 ? r.ny : r.nx;
     const int64_t ne = elems_of(r);

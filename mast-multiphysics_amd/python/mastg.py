@@ -324,7 +324,7 @@ class Context:
 class Server:
     """Request server (continuous batching of design evaluations)."""
 
-    def __init__(self, n_slots=4, solver=SOLVER_MATFREE, hbm_budget=0, queue_capacity=0, rtol=1e-10):
+    def __init__(self, n_slots=4, solver=SOLVER_MG, hbm_budget=0, queue_capacity=0, rtol=1e-10):
         h = _P()
         o = ServerOpts(n_slots, solver, hbm_budget, queue_capacity, rtol)
         rc = lib().mastg_server_create(C.byref(o), C.byref(h))

@@ -53,7 +53,7 @@ def test_served_equals_direct_context(oracle, mastg):
     w = mastg.Workload(1, 16, 8, 8, seed=4, lo=0.01, hi=1.0)
     ctx = mastg.Context(w.desc)
     ctx.set_tuning(8, 2048)
-    r = ctx.design_eval(w.params, solver=mastg.SOLVER_MATFREE)
+    r = ctx.design_eval(w.params, solver=mastg.SOLVER_MG)
     ctx.close()
     s = mastg.Server(n_slots=2)
     out = s.wait(s.submit(1, 16, 8, 8, w.params))
