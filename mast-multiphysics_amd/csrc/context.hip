@@ -105,7 +105,8 @@ struct mastg_ctx {
     bool mg_valid = false;     // coarse operators match the last assembly
     int mg_k = 3;              // Chebyshev steps per smoothing
     int64_t mg_n_coarse = 0;   // DOFs of the dense coarsest level
-    DevBuf mg_cheb, mg_lam, mg_part, mg_dense;
+    DevBuf mg_cheb, mg_lam, mg_part, mg_dense, mg_scal;
+    bool mg_power = false;     // Chebyshev bounds from power iteration (plates)
     hipGraphExec_t mg_exec = nullptr;
     int mg_exec_len = 0;
     // launch shapes (autotuned on the first solve)
@@ -512,7 +513,11 @@ int cg_loop(mastg_ctx* c, const CGArgs& a, const double* b, const mastg_solve_op
     return cg_finish(c, a, b, e0, e1, st, done);
 }
 
-// ---- geometric multigrid preconditioned CG (hex8 lattice meshes) ----------
+// ---- geometric multigrid preconditioned CG (lattice meshes) ----------------
+// power iteration for the plates' Chebyshev bounds: steps and safety factor
+constexpr int kMGPowerIters = 20;
+constexpr double kMGPowerSafety = 1.1;
+
 // The hierarchy (lattice detection, coarse meshes, their node-block structure)
 // is built on the first multigrid solve; coarse operators are re-formed after
 // every assembly (Galerkin products of the element matrices, so they follow
@@ -521,8 +526,9 @@ int mg_build(mastg_ctx* c) {
     if (c->mg_tried) return c->mg.empty() ? fail(c, MASTG_ERR_INVALID, "multigrid unavailable: " + c->mg_why)
                                           : MASTG_OK;
     c->mg_tried = true;
-    if (c->d.physics != MASTG_HEX8_ELASTIC || c->S.n_classes != 1) {
-        c->mg_why = "multigrid is built for hex8 elasticity on lattice meshes (one element shape)";
+    const bool plate = c->d.physics == MASTG_QUAD4_MINDLIN;
+    if (!(plate || (c->d.physics == MASTG_HEX8_ELASTIC && c->S.n_classes == 1))) {
+        c->mg_why = "multigrid is built for hex8 elasticity (one element shape) and the Mindlin plate on lattice meshes";
         return fail(c, MASTG_ERR_INVALID, "multigrid unavailable: " + c->mg_why);
     }
     MGHierarchy H = build_mg_hierarchy(c->S);
@@ -537,10 +543,28 @@ int mg_build(mastg_ctx* c) {
         auto L = std::make_unique<MGLevelDev>();
         MGLevelArgs& a = L->a;
         a.n_nodes = h.n_nodes;
+        a.nav = h.nav;
+        a.dim = h.dim;
         a.nx = h.nx; a.ny = h.ny; a.nz = h.nz;
         L->n_elems = h.n_elems;
-        const size_t nv = size_t(h.n_nodes) * 3 * sizeof(double);
-        if (l == 0) {
+        const int nav = h.nav;
+        const size_t nv = size_t(h.n_nodes) * nav * sizeof(double);
+        if (l == 0 && plate) {
+            // the plate's problem level smooths with the assembled node blocks
+            // (FP32 copy) in the caller's node-rank order; the transfers map
+            // lattice points to ranks
+            TRY(upload(c, L->rank_of_lat, h.rank_of_lat));
+            TRY(upload(c, L->elem_of_cell, h.elem_of_cell));
+            TRY(dalloc(c, c->val32, size_t(c->S.nnz) * sizeof(float)));
+            a.rank_of_lat = L->rank_of_lat.as<int32_t>();
+            a.con = c->con.as<uint8_t>();
+            a.blk_ptr = c->blk_ptr.as<int64_t>();
+            a.blk_col = c->blk_col.as<int32_t>();
+            a.val = c->val.as<double>();
+            a.val32 = c->val32.as<float>();
+            a.dinv = c->dinv.as<double>();
+            a.r = c->vr.as<double>();
+        } else if (l == 0) {
             // the problem level runs in lattice order (x fastest): its vectors
             // are permuted from / to the caller's node-rank order at the solve
             std::vector<uint8_t> con_lat(size_t(h.n_nodes) * 3), nodecon_lat(size_t(h.n_nodes));
@@ -569,9 +593,10 @@ int mg_build(mastg_ctx* c) {
             TRY(upload(c, L->cidx, S.cidx));
             TRY(upload(c, L->econn, S.econn));
             TRY(upload(c, L->con, S.constrained));
-            TRY(dalloc(c, L->Kel, size_t(S.n_elems) * 64 * 9 * sizeof(double)));
-            TRY(dalloc(c, L->val, size_t(S.n_blocks) * 9 * sizeof(double)));
-            TRY(dalloc(c, L->val32, size_t(S.n_blocks) * 9 * sizeof(float)));
+            const size_t nb2 = size_t(nav) * nav;
+            TRY(dalloc(c, L->Kel, size_t(S.n_elems) * S.npe * S.npe * nb2 * sizeof(double)));
+            TRY(dalloc(c, L->val, size_t(S.n_blocks) * nb2 * sizeof(double)));
+            TRY(dalloc(c, L->val32, size_t(S.n_blocks) * nb2 * sizeof(float)));
             TRY(dalloc(c, L->dinv, nv));
             TRY(dalloc(c, L->r, nv));
             a.con = L->con.as<uint8_t>();
@@ -600,6 +625,11 @@ int mg_build(mastg_ctx* c) {
     TRY(dalloc(c, c->mg_part, size_t(std::max<int64_t>(8192, (c->S.n_active + 255) / 256 + 1)) * sizeof(double)));
     c->mg_n_coarse = H.coarsest_dofs;
     TRY(dalloc(c, c->mg_dense, size_t(H.coarsest_dofs * H.coarsest_dofs) * sizeof(double)));
+    TRY(dalloc(c, c->mg_scal, 2 * sizeof(double)));
+    // plates: the Gershgorin bound of D^-1 A is loose by ~50x (thin-plate
+    // bending against membrane rows); their Chebyshev intervals come from
+    // power iteration instead
+    c->mg_power = plate;
     for (int l = 0; l < nl; l++) lv[l]->a.cheb = c->mg_cheb.as<double>() + l * kMGChebStride;
     c->mg = std::move(lv);
     return MASTG_OK;
@@ -611,15 +641,25 @@ int mg_setup(mastg_ctx* c) {
     if (c->mg_valid) return MASTG_OK;
     if (!c->kel64_valid) return fail(c, MASTG_ERR_INVALID, "multigrid needs an FP64 assembly");
     const int nl = int(c->mg.size());
+    const int nav = c->S.nav;
     hipStream_t s = c->stream;
-    launch_mg_rowinfo_nb(c->S.n_nodes, c->blk_ptr.as<int64_t>(), c->blk_col.as<int32_t>(), c->val.as<double>(),
+    launch_mg_rowinfo_nb(nav, c->S.n_nodes, c->blk_ptr.as<int64_t>(), c->blk_col.as<int32_t>(), c->val.as<double>(),
                          nullptr, c->mg_part.as<double>(), c->mg_lam.as<double>(), s);
-    launch_perm_nodes(c->S.n_nodes, c->mg[0]->rank_of_lat.as<int32_t>(), c->dinv.as<double>(),
-                      c->mg[0]->dinv_lat.as<double>(), 0, s);
+    if (c->mg_power) {
+        launch_to_f32(c->S.nnz, c->val.as<double>(), c->val32.as<float>(), s);
+        MGLevelDev& L0 = *c->mg[0];
+        launch_mg_power(L0.a, c->val.as<double>(), kMGPowerIters, kMGPowerSafety, L0.a.da, L0.a.db,
+                        c->mg_part.as<double>(), c->mg_scal.as<double>(), c->mg_lam.as<double>(), s);
+    } else {
+        launch_perm_nodes(c->S.n_nodes, c->mg[0]->rank_of_lat.as<int32_t>(), c->dinv.as<double>(),
+                          c->mg[0]->dinv_lat.as<double>(), 0, s);
+    }
     for (int l = 0; l + 1 < nl; l++) {
         MGLevelDev& F = *c->mg[l];
         MGLevelDev& C = *c->mg[l + 1];
         MGGalerkinArgs g{};
+        g.dim = C.a.dim;
+        g.nav = nav;
         g.n_coarse = C.n_elems;
         g.cnx = C.a.nx; g.cny = C.a.ny;
         g.fnx = F.a.nx; g.fny = F.a.ny;
@@ -631,11 +671,14 @@ int mg_setup(mastg_ctx* c) {
         g.Kf = l == 0 ? c->Kel.as<double>() : F.Kel.as<double>();
         g.Kc = C.Kel.as<double>();
         launch_mg_galerkin(g, s);
-        ScatterArgs sa{C.a.n_nodes, 3, C.a.blk_ptr, C.cptr.as<int32_t>(), C.cidx.as<int32_t>(),
+        ScatterArgs sa{C.a.n_nodes, nav, C.a.blk_ptr, C.cptr.as<int32_t>(), C.cidx.as<int32_t>(),
                        C.Kel.as<double>(), C.val.as<double>()};
         launch_scatter(sa, s);
-        launch_mg_rowinfo_nb(C.a.n_nodes, C.a.blk_ptr, C.a.blk_col, C.a.val, C.dinv.as<double>(),
+        launch_mg_rowinfo_nb(nav, C.a.n_nodes, C.a.blk_ptr, C.a.blk_col, C.a.val, C.dinv.as<double>(),
                              c->mg_part.as<double>(), c->mg_lam.as<double>() + l + 1, s);
+        if (c->mg_power)
+            launch_mg_power(C.a, C.a.val, kMGPowerIters, kMGPowerSafety, C.a.da, C.a.db, c->mg_part.as<double>(),
+                            c->mg_scal.as<double>(), c->mg_lam.as<double>() + l + 1, s);
         launch_to_f32(int64_t(C.val32.bytes / sizeof(float)), C.a.val, C.val32.as<float>(), s);
     }
     launch_mg_cheb(c->mg_lam.as<double>(), c->mg_cheb.as<double>(), nl, c->mg_k, 30.0, s);
@@ -652,7 +695,7 @@ int mg_setup(mastg_ctx* c) {
 // the per-node lattice operator's arguments (built multigrid hierarchy, one
 // element class); false when the mesh does not qualify
 bool lattice_args(mastg_ctx* c, CGArgs& a) {
-    if (c->mg.empty()) return false;
+    if (c->mg.empty() || c->mg_power) return false;   // hex8 only
     const MGLevelDev& L0 = *c->mg[0];
     a.lat.nx = L0.a.nx; a.lat.ny = L0.a.ny; a.lat.nz = L0.a.nz;
     a.lat.elem_of_cell = L0.elem_of_cell.as<int32_t>();
@@ -668,7 +711,7 @@ const double* mg_vcycle(mastg_ctx* c, const CGArgs& a, const int* done, hipStrea
     const int k = c->mg_k;
     auto dbuf = [&](int l, int i) { return (i % 2 == 0) ? c->mg[l]->a.da : c->mg[l]->a.db; };
     auto apply = [&](int l, int mode, int step, const double* din, double* dout) {
-        if (l == 0) launch_mg_fine(a, c->mg[0]->a, mode, step, din, dout, done, s);
+        if (l == 0 && !c->mg_power) launch_mg_fine(a, c->mg[0]->a, mode, step, din, dout, done, s);
         else launch_mg_bsr(c->mg[l]->a, mode, step, din, dout, done, s);
     };
     for (int l = 0; l + 1 < nl; l++) {
@@ -705,8 +748,8 @@ int mg_solve(mastg_ctx* c, const double* b, const mastg_solve_opts* o, mastg_sol
     const int max_it = (o && o->max_iters > 0) ? o->max_iters : 100000;
     const int chunk = (o && o->check_every > 0) ? o->check_every : 4;
     TRY(mg_build(c));
-    if (!c->tuned) TRY(autotune(c));   // launch shape of the final CSR residual check
-    if (!c->sE_valid) {
+    if (!c->tuned) TRY(autotune(c));   // launch shape of the CSR products
+    if (!c->mg_power && !c->sE_valid) {
         launch_elem_scale(S.n_elems, c->param.as<double>(), c->d.E, c->d.penalty, c->sE.as<double>(), c->stream);
         c->sE_valid = true;
     }
@@ -715,14 +758,22 @@ int mg_solve(mastg_ctx* c, const double* b, const mastg_solve_opts* o, mastg_sol
     HIPC(c, ev_create(&e1));
     HIPC(c, hipEventRecord(e0, c->stream));
     TRY(mg_setup(c));
-    // the whole PCG runs in lattice order: b is permuted in, x out
     CGArgs a = cg_args(c);
-    if (!lattice_args(c, a)) return fail(c, MASTG_ERR_INVALID, "partial-sum buffer too small for the lattice");
-    a.use_mf = 3;
-    a.nparts_op = int(lattice_partials(a.lat));
     const int32_t* lat_map = c->mg[0]->rank_of_lat.as<int32_t>();
-    double* bl = c->mg[0]->b_lat.as<double>();
-    launch_perm_nodes(S.n_nodes, lat_map, b, bl, 0, c->stream);
+    const double* bl = b;
+    if (!c->mg_power) {
+        // hex8: the whole PCG runs in lattice order with the per-node lattice
+        // operator; b is permuted in, x out
+        if (!lattice_args(c, a)) return fail(c, MASTG_ERR_INVALID, "partial-sum buffer too small for the lattice");
+        a.use_mf = 3;
+        a.nparts_op = int(lattice_partials(a.lat));
+        double* blat = c->mg[0]->b_lat.as<double>();
+        launch_perm_nodes(S.n_nodes, lat_map, b, blat, 0, c->stream);
+        bl = blat;
+    } else {
+        // plate: node-rank order, the FP64 CSR product
+        a.use_mf = 0;
+    }
     c->cg_replacements = 0;
     launch_fill(S.n_active, 0.0, a.x, c->stream);
     HIPC(c, hipMemcpyAsync(a.r, bl, S.n_active * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
@@ -754,9 +805,11 @@ int mg_solve(mastg_ctx* c, const double* b, const mastg_solve_opts* o, mastg_sol
         HIPC(c, hipStreamSynchronize(c->stream));
         done = c->h_scal->done;
     }
-    // back to rank order (x lives in c->vx for the callers)
-    launch_perm_nodes(S.n_nodes, lat_map, a.x, a.z, 1, c->stream);
-    HIPC(c, hipMemcpyAsync(a.x, a.z, S.n_active * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    if (!c->mg_power) {
+        // back to rank order (x lives in c->vx for the callers)
+        launch_perm_nodes(S.n_nodes, lat_map, a.x, a.z, 1, c->stream);
+        HIPC(c, hipMemcpyAsync(a.x, a.z, S.n_active * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    }
     HIPC(c, hipGetLastError());
     return cg_finish(c, a, b, e0, e1, st, done);
 }

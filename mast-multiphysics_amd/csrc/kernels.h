@@ -179,18 +179,21 @@ constexpr int kMGChebStride = 16;      // coefficients per level: 1/theta, (c1_i
 
 struct MGLevelArgs {
     int64_t n_nodes;
-    int nx, ny, nz;                    // lattice cells per dimension (vectors in lattice order)
-    const uint8_t* con;                // [3 n_nodes] constrained DOFs
+    int nav, dim;                      // variables per node (3 hex8, 5 plate), lattice dimension
+    int nx, ny, nz;                    // lattice cells per dimension (nz = 0 in 2D)
+    const int32_t* rank_of_lat;        // vector index of lattice point (null: lattice order)
+    const uint8_t* con;                // [nav n_nodes] constrained DOFs
     const int64_t* blk_ptr;            // operator of levels >= 1: node-block CSR, FP64
     const int32_t* blk_col;
     const double* val;
     const float* val32;                // FP32 copy of val read by the smoother (null = FP64)
-    const double* dinv;                // [3 n_nodes] inverse diagonal
+    const double* dinv;                // [nav n_nodes] inverse diagonal
     double *r, *x, *res, *da, *db;     // right side, iterate, residual, smoothing directions
     const double* cheb;                // this level's Chebyshev coefficients
 };
 
 struct MGGalerkinArgs {                // coarse element matrices = sum_c Q_c^T K_c Q_c
+    int dim, nav;                      // 3, 3 (hex8) or 2, 5 (quad4 plate)
     int64_t n_coarse;                  // coarse elements (lattice cell order)
     int cnx, cny;                      // coarse cells in x, y
     int fnx, fny;                      // fine cells in x, y
@@ -199,8 +202,8 @@ struct MGGalerkinArgs {                // coarse element matrices = sum_c Q_c^T 
     const uint8_t* f_con;              // fine constraints by rank*3+v
     const int32_t* c_econn;
     const uint8_t* c_con;
-    const double* Kf;                  // [n_fine][64][9] fine element blocks
-    double* Kc;                        // [n_coarse][64][9]
+    const double* Kf;                  // [n_fine][npe^2][nav^2] fine element blocks
+    double* Kc;                        // [n_coarse][npe^2][nav^2]
 };
 
 // smoothing modes (row-local epilogue after y = A in):
@@ -212,9 +215,6 @@ enum { MG_STEP = 1, MG_FINAL = 2, MG_RESID_INIT = 3 };
 void launch_mg_galerkin(const MGGalerkinArgs& g, hipStream_t s);
 // dinv (levels >= 1, from the node-block values) and the Gershgorin bound
 // max_i sum_j |a_ij| / a_ii into *lam; level 0 reads the scalar CSR
-void launch_mg_rowinfo_bsr(const MGLevelArgs& L, double* dinv, double* part, double* lam, hipStream_t s);
-void launch_mg_rowinfo_csr(int64_t n, const int64_t* row_ptr, const double* val, const int64_t* diag_pos,
-                           double* part, double* lam, hipStream_t s);
 void launch_mg_cheb(const double* lam, double* cheb, int n_levels, int k, double ratio, hipStream_t s);
 void launch_mg_dense_gather(const MGLevelArgs& L, double* A, hipStream_t s);
 void launch_mg_invert(int n, double* A, hipStream_t s);
@@ -231,8 +231,13 @@ void launch_lattice_apply(const CGArgs& c, const double* x, double* y, hipStream
 // rank order <-> lattice order: gather dst[p] = src[map[p]] or scatter dst[map[p]] = src[p]
 void launch_perm_nodes(int64_t n_nodes, const int32_t* map, const double* src, double* dst, int scatter,
                        hipStream_t s);
-void launch_mg_rowinfo_nb(int64_t n_nodes, const int64_t* blk_ptr, const int32_t* blk_col, const double* val,
-                          double* dinv, double* part, double* lam, hipStream_t s);
+void launch_mg_rowinfo_nb(int nav, int64_t n_nodes, const int64_t* blk_ptr, const int32_t* blk_col,
+                          const double* val, double* dinv, double* part, double* lam, hipStream_t s);
+// estimate of lambda_max(D^-1 A) by power iteration on the FP64 node blocks of
+// L (n_it steps from a fixed positive start vector), times `safety`, into
+// *lam; w0, w1: work vectors of nav n_nodes; scal: 2 device doubles
+void launch_mg_power(const MGLevelArgs& L, const double* val, int n_it, double safety, double* w0, double* w1,
+                     double* part, double* scal, double* lam, hipStream_t s);
 void launch_mg_restrict(const MGLevelArgs& F, const MGLevelArgs& C, const int* done, hipStream_t s);
 void launch_mg_prolong(const MGLevelArgs& F, const MGLevelArgs& C, const double* dlast, const int* done,
                        hipStream_t s);

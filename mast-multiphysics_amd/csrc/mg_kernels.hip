@@ -8,6 +8,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "kernels.h"
 #include "device_util.h"
 
@@ -17,12 +19,13 @@ namespace mastg {
 __constant__ int8_t kMGCorner[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
                                        {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
 
-// trilinear weight of coarse corner A at fine local node a of child cell c
+// tri/bilinear weight of coarse corner A at fine local node a of child cell c
 // (c bit d = the child's offset along dimension d inside the coarse cell)
+template <int DIM>
 __device__ __forceinline__ double mg_q(int c, int a, int A) {
     double w = 1.0;
 #pragma unroll
-    for (int d = 0; d < 3; d++) {
+    for (int d = 0; d < DIM; d++) {
         const int t = ((c >> d) & 1) + kMGCorner[a][d];   // 0, 1, 2 fine cells
         w *= kMGCorner[A][d] ? 0.5 * t : 1.0 - 0.5 * t;
     }
@@ -30,141 +33,106 @@ __device__ __forceinline__ double mg_q(int c, int a, int A) {
 }
 
 // ---- Galerkin coarse element matrices: K_E = sum_c Q_c^T K_c Q_c ---------
-// One wave per coarse element, 4 per block.  Fine blocks are masked (the rows
-// and columns of constrained fine DOFs removed, as P has no row for them), the
-// result is made exactly symmetric and gets the libMesh constraint convention
-// for constrained coarse DOFs (rows / columns zero, unit diagonal).
+// One wave per coarse element, 4 per block; lane (A, B) < NPE^2 owns one
+// NAV x NAV block.  Fine blocks are masked (the rows and columns of
+// constrained fine DOFs removed, as P has no row for them), the result is made
+// exactly symmetric and gets the libMesh constraint convention for
+// constrained coarse DOFs (rows / columns zero, unit diagonal).
+template <int DIM, int NAV>
 __global__ __launch_bounds__(256) void k_mg_galerkin(MGGalerkinArgs G) {
-    __shared__ double sK[4][64 * 9];
-    __shared__ double sT[4][64 * 9];
+    constexpr int NPE = 1 << DIM, NCH = 1 << DIM, NB = NAV * NAV, SZ = NPE * NPE * NB;
+    __shared__ double sK[4][SZ];
+    __shared__ double sT[4][SZ];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int hi = lane >> 3, lo = lane & 7;
+    const bool lane_on = lane < NPE * NPE;
+    const int ln = lane_on ? lane : 0;
+    const int hi = ln / NPE, lo = ln % NPE;
     const int64_t E = int64_t(blockIdx.x) * 4 + wave;
     const bool valid = E < G.n_coarse;
     const int64_t Ev = valid ? E : 0;
     const int64_t I = Ev % G.cnx, J = (Ev / G.cnx) % G.cny, K = Ev / (int64_t(G.cnx) * G.cny);
-    double acc[9];
+    double acc[NB];
 #pragma unroll
-    for (int i = 0; i < 9; i++) acc[i] = 0.0;
-    for (int c = 0; c < 8; c++) {
-        const int64_t fi = 2 * I + (c & 1), fj = 2 * J + ((c >> 1) & 1), fk = 2 * K + ((c >> 2) & 1);
+    for (int i = 0; i < NB; i++) acc[i] = 0.0;
+    for (int c = 0; c < NCH; c++) {
+        const int64_t fi = 2 * I + (c & 1), fj = 2 * J + ((c >> 1) & 1), fk = DIM == 3 ? 2 * K + ((c >> 2) & 1) : 0;
         const int64_t cell = fi + int64_t(G.fnx) * (fj + int64_t(G.fny) * fk);
         const int64_t fe = G.f_elem_of_cell ? int64_t(G.f_elem_of_cell[cell]) : cell;
-        {   // masked fine block (a, b) = (hi, lo)
-            const int32_t ra = G.f_econn[fe * 8 + hi], rb = G.f_econn[fe * 8 + lo];
-            const double* kb = G.Kf + (fe * 64 + lane) * 9;
+        if (lane_on) {   // masked fine block (a, b) = (hi, lo)
+            const int32_t ra = G.f_econn[fe * NPE + hi], rb = G.f_econn[fe * NPE + lo];
+            const double* kb = G.Kf + (fe * NPE * NPE + ln) * NB;
 #pragma unroll
-            for (int v = 0; v < 3; v++)
+            for (int v = 0; v < NAV; v++)
 #pragma unroll
-                for (int w = 0; w < 3; w++) {
-                    double x = kb[v * 3 + w];
-                    if (G.f_con[int64_t(ra) * 3 + v] || G.f_con[int64_t(rb) * 3 + w]) x = 0.0;
-                    sK[wave][lane * 9 + v * 3 + w] = x;
+                for (int w = 0; w < NAV; w++) {
+                    double x = kb[v * NAV + w];
+                    if (G.f_con[int64_t(ra) * NAV + v] || G.f_con[int64_t(rb) * NAV + w]) x = 0.0;
+                    sK[wave][ln * NB + v * NAV + w] = x;
                 }
         }
         __syncthreads();
-        {   // T[a][B] = sum_b K[a][b] Q_c[b][B], lane = (a, B)
-            double t[9];
+        if (lane_on) {   // T[a][B] = sum_b K[a][b] Q_c[b][B], lane = (a, B)
+            double t[NB];
 #pragma unroll
-            for (int i = 0; i < 9; i++) t[i] = 0.0;
-            for (int b = 0; b < 8; b++) {
-                const double q = mg_q(c, b, lo);
+            for (int i = 0; i < NB; i++) t[i] = 0.0;
+            for (int b = 0; b < NPE; b++) {
+                const double q = mg_q<DIM>(c, b, lo);
                 if (q != 0.0) {
-                    const double* kb = &sK[wave][(hi * 8 + b) * 9];
+                    const double* kb = &sK[wave][(hi * NPE + b) * NB];
 #pragma unroll
-                    for (int i = 0; i < 9; i++) t[i] += kb[i] * q;
+                    for (int i = 0; i < NB; i++) t[i] += kb[i] * q;
                 }
             }
 #pragma unroll
-            for (int i = 0; i < 9; i++) sT[wave][lane * 9 + i] = t[i];
+            for (int i = 0; i < NB; i++) sT[wave][ln * NB + i] = t[i];
         }
         __syncthreads();
-        // K_E[A][B] += sum_a Q_c[a][A] T[a][B], lane = (A, B)
-        for (int a = 0; a < 8; a++) {
-            const double q = mg_q(c, a, hi);
-            if (q != 0.0) {
-                const double* tb = &sT[wave][(a * 8 + lo) * 9];
+        if (lane_on)   // K_E[A][B] += sum_a Q_c[a][A] T[a][B], lane = (A, B)
+            for (int a = 0; a < NPE; a++) {
+                const double q = mg_q<DIM>(c, a, hi);
+                if (q != 0.0) {
+                    const double* tb = &sT[wave][(a * NPE + lo) * NB];
 #pragma unroll
-                for (int i = 0; i < 9; i++) acc[i] += q * tb[i];
+                    for (int i = 0; i < NB; i++) acc[i] += q * tb[i];
+                }
             }
-        }
         __syncthreads();
     }
     // exact symmetry: lanes below the diagonal take the transpose of their partner
+    if (lane_on)
 #pragma unroll
-    for (int i = 0; i < 9; i++) sK[wave][lane * 9 + i] = acc[i];
+        for (int i = 0; i < NB; i++) sK[wave][ln * NB + i] = acc[i];
     __syncthreads();
-    double out[9];
+    if (!valid || !lane_on) return;
+    double out[NB];
     if (hi > lo) {
-        const double* tb = &sK[wave][(lo * 8 + hi) * 9];
+        const double* tb = &sK[wave][(lo * NPE + hi) * NB];
 #pragma unroll
-        for (int v = 0; v < 3; v++)
+        for (int v = 0; v < NAV; v++)
 #pragma unroll
-            for (int w = 0; w < 3; w++) out[v * 3 + w] = tb[w * 3 + v];
+            for (int w = 0; w < NAV; w++) out[v * NAV + w] = tb[w * NAV + v];
     } else if (hi == lo) {
 #pragma unroll
-        for (int v = 0; v < 3; v++)
+        for (int v = 0; v < NAV; v++)
 #pragma unroll
-            for (int w = 0; w < 3; w++) out[v * 3 + w] = 0.5 * (acc[v * 3 + w] + acc[w * 3 + v]);
+            for (int w = 0; w < NAV; w++) out[v * NAV + w] = 0.5 * (acc[v * NAV + w] + acc[w * NAV + v]);
     } else {
 #pragma unroll
-        for (int i = 0; i < 9; i++) out[i] = acc[i];
+        for (int i = 0; i < NB; i++) out[i] = acc[i];
     }
-    if (!valid) return;
-    const int32_t cA = G.c_econn[E * 8 + hi], cB = G.c_econn[E * 8 + lo];
-    double* o = G.Kc + (E * 64 + lane) * 9;
+    const int32_t cA = G.c_econn[E * NPE + hi], cB = G.c_econn[E * NPE + lo];
+    double* o = G.Kc + (E * NPE * NPE + ln) * NB;
 #pragma unroll
-    for (int v = 0; v < 3; v++)
+    for (int v = 0; v < NAV; v++)
 #pragma unroll
-        for (int w = 0; w < 3; w++) {
-            double x = out[v * 3 + w];
-            if (G.c_con[int64_t(cA) * 3 + v] || G.c_con[int64_t(cB) * 3 + w]) x = (hi == lo && v == w) ? 1.0 : 0.0;
-            o[v * 3 + w] = x;
+        for (int w = 0; w < NAV; w++) {
+            double x = out[v * NAV + w];
+            if (G.c_con[int64_t(cA) * NAV + v] || G.c_con[int64_t(cB) * NAV + w]) x = (hi == lo && v == w) ? 1.0 : 0.0;
+            o[v * NAV + w] = x;
         }
 }
 
-// ---- row information: dinv and the Gershgorin bound of D^-1 A -------------
-__global__ __launch_bounds__(256) void k_mg_rowinfo_bsr(MGLevelArgs L, double* __restrict__ dinv,
-                                                        double* __restrict__ part) {
-    const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
-    double g = 0.0;
-    if (i < L.n_nodes * 3) {
-        const int64_t r = i / 3;
-        const int v = int(i - r * 3);
-        const int64_t k0 = L.blk_ptr[r];
-        const int64_t deg = L.blk_ptr[r + 1] - k0;
-        const double* row = L.val + k0 * 9 + int64_t(v) * deg * 3;
-        double sum = 0.0, diag = 0.0;
-        for (int64_t k = 0; k < deg; k++) {
-            const int32_t c = L.blk_col[k0 + k];
-#pragma unroll
-            for (int w = 0; w < 3; w++) {
-                const double a = row[k * 3 + w];
-                sum += fabs(a);
-                if (c == r && w == v) diag = a;
-            }
-        }
-        dinv[i] = diag > 0.0 ? 1.0 / diag : 0.0;
-        g = diag > 0.0 ? sum / diag : 0.0;
-    }
-    block_max_store(g, part, blockIdx.x);
-}
-
-__global__ __launch_bounds__(256) void k_mg_rowinfo_csr(int64_t n, const int64_t* __restrict__ row_ptr,
-                                                        const double* __restrict__ val,
-                                                        const int64_t* __restrict__ diag_pos,
-                                                        double* __restrict__ part) {
-    const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
-    double g = 0.0;
-    if (i < n) {
-        double sum = 0.0;
-        for (int64_t k = row_ptr[i]; k < row_ptr[i + 1]; k++) sum += fabs(val[k]);
-        const double diag = val[diag_pos[i]];
-        g = diag > 0.0 ? sum / diag : 0.0;
-    }
-    block_max_store(g, part, blockIdx.x);
-}
-
+// ---- row information and eigenvalue bounds ----------------------------------
 __global__ __launch_bounds__(256) void k_mg_max(const double* __restrict__ part, int n, double* __restrict__ out) {
     double m = 0.0;
     for (int i = threadIdx.x; i < n; i += 256) m = fmax(m, part[i]);
@@ -210,7 +178,7 @@ __device__ __forceinline__ void mg_epilogue(const MGLevelArgs& L, int mode, int6
 __global__ __launch_bounds__(256) void k_mg_pre_init(MGLevelArgs L, const int* __restrict__ done) {
     if (done && *done) return;
     const double ith = L.cheb[0];
-    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < L.n_nodes * 3; i += int64_t(gridDim.x) * 256) {
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < L.n_nodes * L.nav; i += int64_t(gridDim.x) * 256) {
         const double r = L.r[i];
         L.x[i] = 0.0;
         L.res[i] = r;
@@ -218,9 +186,9 @@ __global__ __launch_bounds__(256) void k_mg_pre_init(MGLevelArgs L, const int* _
     }
 }
 
-// node-block operator of a coarse level (values T = double or their FP32
-// copy; products and sums in FP64), 8 lanes per node, fused epilogue
-template <typename T>
+// node-block operator of a level (values T = double or their FP32 copy;
+// products and sums in FP64), 8 lanes per node, fused epilogue
+template <typename T, int NAV>
 __global__ __launch_bounds__(256) void k_mg_bsr(MGLevelArgs L, const T* __restrict__ vals, int mode, int step,
                                                 const double* __restrict__ din, double* __restrict__ dout,
                                                 const int* __restrict__ done) {
@@ -233,27 +201,31 @@ __global__ __launch_bounds__(256) void k_mg_bsr(MGLevelArgs L, const T* __restri
     for (int64_t r = (int64_t(blockIdx.x) * 256 + threadIdx.x) / 8; r < L.n_nodes; r += nodes_per_grid) {
         const int64_t k0 = L.blk_ptr[r];
         const int deg = int(L.blk_ptr[r + 1] - k0);
-        const T* __restrict__ V = vals + k0 * 9;
-        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+        const T* __restrict__ V = vals + k0 * (NAV * NAV);
+        double sv[NAV];
+#pragma unroll
+        for (int v = 0; v < NAV; v++) sv[v] = 0.0;
         for (int k = sub; k < deg; k += 8) {
-            const int64_t cb = int64_t(L.blk_col[k0 + k]) * 3;
-            const double x0 = in[cb], x1 = in[cb + 1], x2 = in[cb + 2];
-            const T* v0 = V + int64_t(k) * 3;
-            const T* v1 = V + (int64_t(deg) + k) * 3;
-            const T* v2 = V + (int64_t(2 * deg) + k) * 3;
-            s0 += double(v0[0]) * x0 + double(v0[1]) * x1 + double(v0[2]) * x2;
-            s1 += double(v1[0]) * x0 + double(v1[1]) * x1 + double(v1[2]) * x2;
-            s2 += double(v2[0]) * x0 + double(v2[1]) * x1 + double(v2[2]) * x2;
+            const int64_t cb = int64_t(L.blk_col[k0 + k]) * NAV;
+            double xw[NAV];
+#pragma unroll
+            for (int w = 0; w < NAV; w++) xw[w] = in[cb + w];
+#pragma unroll
+            for (int v = 0; v < NAV; v++) {
+                const T* vr = V + (int64_t(v) * deg + k) * NAV;
+#pragma unroll
+                for (int w = 0; w < NAV; w++) sv[v] += double(vr[w]) * xw[w];
+            }
         }
 #pragma unroll
-        for (int m = 4; m >= 1; m >>= 1) {
-            s0 += __shfl_xor(s0, m, 8);
-            s1 += __shfl_xor(s1, m, 8);
-            s2 += __shfl_xor(s2, m, 8);
-        }
-        if (sub < 3) {
-            const double y = sub == 0 ? s0 : (sub == 1 ? s1 : s2);
-            mg_epilogue(L, mode, r * 3 + sub, y, din, dout, c1, c2, ith);
+        for (int v = 0; v < NAV; v++)
+#pragma unroll
+            for (int m = 4; m >= 1; m >>= 1) sv[v] += __shfl_xor(sv[v], m, 8);
+        if (sub < NAV) {
+            double y = sv[0];
+#pragma unroll
+            for (int v = 1; v < NAV; v++) y = sub == v ? sv[v] : y;
+            mg_epilogue(L, mode, r * NAV + sub, y, din, dout, c1, c2, ith);
         }
     }
 }
@@ -384,6 +356,7 @@ __global__ __launch_bounds__(256) void k_perm_nodes(int64_t n_nodes, const int32
 }
 
 // dinv (when wanted) and the Gershgorin bound over node blocks, 8 lanes per node
+template <int NAV>
 __global__ __launch_bounds__(256) void k_mg_rowinfo_nb(int64_t n_nodes, const int64_t* __restrict__ blk_ptr,
                                                        const int32_t* __restrict__ blk_col,
                                                        const double* __restrict__ val, double* __restrict__ dinv,
@@ -394,34 +367,111 @@ __global__ __launch_bounds__(256) void k_mg_rowinfo_nb(int64_t n_nodes, const in
     for (int64_t r = (int64_t(blockIdx.x) * 256 + threadIdx.x) / 8; r < n_nodes; r += nodes_per_grid) {
         const int64_t k0 = blk_ptr[r];
         const int deg = int(blk_ptr[r + 1] - k0);
-        const double* __restrict__ V = val + k0 * 9;
-        double s0 = 0.0, s1 = 0.0, s2 = 0.0, d0 = 0.0, d1 = 0.0, d2 = 0.0;
-        for (int kk = sub; kk < deg; kk += 8) {
-            const double* v0 = V + int64_t(kk) * 3;
-            const double* v1 = V + (int64_t(deg) + kk) * 3;
-            const double* v2 = V + (int64_t(2 * deg) + kk) * 3;
-            s0 += fabs(v0[0]) + fabs(v0[1]) + fabs(v0[2]);
-            s1 += fabs(v1[0]) + fabs(v1[1]) + fabs(v1[2]);
-            s2 += fabs(v2[0]) + fabs(v2[1]) + fabs(v2[2]);
-            if (blk_col[k0 + kk] == r) { d0 = v0[0]; d1 = v1[1]; d2 = v2[2]; }
-        }
+        const double* __restrict__ V = val + k0 * (NAV * NAV);
+        double sv[NAV], dv[NAV];
 #pragma unroll
-        for (int m = 4; m >= 1; m >>= 1) {
-            s0 += __shfl_xor(s0, m, 8); s1 += __shfl_xor(s1, m, 8); s2 += __shfl_xor(s2, m, 8);
-            d0 += __shfl_xor(d0, m, 8); d1 += __shfl_xor(d1, m, 8); d2 += __shfl_xor(d2, m, 8);
-        }
-        if (sub == 0) {
-            g = fmax(g, d0 > 0.0 ? s0 / d0 : 0.0);
-            g = fmax(g, d1 > 0.0 ? s1 / d1 : 0.0);
-            g = fmax(g, d2 > 0.0 ? s2 / d2 : 0.0);
-            if (dinv) {
-                dinv[r * 3 + 0] = d0 > 0.0 ? 1.0 / d0 : 0.0;
-                dinv[r * 3 + 1] = d1 > 0.0 ? 1.0 / d1 : 0.0;
-                dinv[r * 3 + 2] = d2 > 0.0 ? 1.0 / d2 : 0.0;
+        for (int v = 0; v < NAV; v++) sv[v] = dv[v] = 0.0;
+        for (int kk = sub; kk < deg; kk += 8) {
+            const bool dg = blk_col[k0 + kk] == r;
+#pragma unroll
+            for (int v = 0; v < NAV; v++) {
+                const double* vr = V + (int64_t(v) * deg + kk) * NAV;
+#pragma unroll
+                for (int w = 0; w < NAV; w++) sv[v] += fabs(vr[w]);
+                if (dg) dv[v] = vr[v];
             }
         }
+#pragma unroll
+        for (int v = 0; v < NAV; v++)
+#pragma unroll
+            for (int m = 4; m >= 1; m >>= 1) {
+                sv[v] += __shfl_xor(sv[v], m, 8);
+                dv[v] += __shfl_xor(dv[v], m, 8);
+            }
+        if (sub == 0)
+#pragma unroll
+            for (int v = 0; v < NAV; v++) {
+                g = fmax(g, dv[v] > 0.0 ? sv[v] / dv[v] : 0.0);
+                if (dinv) dinv[r * NAV + v] = dv[v] > 0.0 ? 1.0 / dv[v] : 0.0;
+            }
     }
     block_max_store(g, part, blockIdx.x);
+}
+
+// power iteration for lambda_max(D^-1 A): out = D^-1 A (s in), s = *scale
+// (1 / ||in||; the start vector is built in place when in == null), with
+// partial sums of out.out
+template <int NAV>
+__global__ __launch_bounds__(256) void k_mg_power(MGLevelArgs L, const double* __restrict__ val,
+                                                  const double* __restrict__ in, const double* __restrict__ scale,
+                                                  double* __restrict__ out, double* __restrict__ part) {
+    const int sub = threadIdx.x & 7;
+    const double sc = in ? *scale : 1.0;
+    double acc = 0.0;
+    const int64_t nodes_per_grid = (int64_t(gridDim.x) * 256) / 8;
+    for (int64_t r = (int64_t(blockIdx.x) * 256 + threadIdx.x) / 8; r < L.n_nodes; r += nodes_per_grid) {
+        const int64_t k0 = L.blk_ptr[r];
+        const int deg = int(L.blk_ptr[r + 1] - k0);
+        const double* __restrict__ V = val + k0 * (NAV * NAV);
+        double sv[NAV];
+#pragma unroll
+        for (int v = 0; v < NAV; v++) sv[v] = 0.0;
+        for (int k = sub; k < deg; k += 8) {
+            const int64_t cb = int64_t(L.blk_col[k0 + k]) * NAV;
+            double xw[NAV];
+#pragma unroll
+            for (int w = 0; w < NAV; w++)   // start vector 1 + ((i * 7919) mod 1000) / 1000
+                xw[w] = in ? sc * in[cb + w] : 1.0 + double(((cb + w) * 7919) % 1000) / 1000.0;
+#pragma unroll
+            for (int v = 0; v < NAV; v++) {
+                const double* vr = V + (int64_t(v) * deg + k) * NAV;
+#pragma unroll
+                for (int w = 0; w < NAV; w++) sv[v] += vr[w] * xw[w];
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < NAV; v++)
+#pragma unroll
+            for (int m = 4; m >= 1; m >>= 1) sv[v] += __shfl_xor(sv[v], m, 8);
+        if (sub == 0)
+#pragma unroll
+            for (int v = 0; v < NAV; v++) {
+                const double o = L.dinv[r * NAV + v] * sv[v];
+                out[r * NAV + v] = o;
+                acc += o * o;
+            }
+    }
+    block_sum_store(acc, part, blockIdx.x);
+}
+
+// norm of the power iterate: scal[0] = 1 / ||out||, *lam = safety * ||out|| / ||start||
+__global__ __launch_bounds__(256) void k_mg_power_norm(const double* __restrict__ part, int n, int first,
+                                                       double safety, double* __restrict__ scal,
+                                                       double* __restrict__ lam) {
+    const double s = reduce_partials(part, n);
+    if (threadIdx.x == 0) {
+        const double nrm = sqrt(s);
+        // the first step multiplies the unnormalised start vector: its norm is scal[1]
+        const double rel = first ? nrm / scal[1] : nrm;
+        scal[0] = nrm > 0.0 ? 1.0 / nrm : 0.0;
+        *lam = safety * rel;
+    }
+}
+
+// squared norm of the start vector (for the first Rayleigh-free estimate)
+__global__ __launch_bounds__(256) void k_mg_power_start(int64_t n, double* __restrict__ part) {
+    double acc = 0.0;
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256) {
+        const double x = 1.0 + double((i * 7919) % 1000) / 1000.0;
+        acc += x * x;
+    }
+    block_sum_store(acc, part, blockIdx.x);
+}
+
+__global__ __launch_bounds__(256) void k_mg_power_start_norm(const double* __restrict__ part, int n,
+                                                             double* __restrict__ scal) {
+    const double s = reduce_partials(part, n);
+    if (threadIdx.x == 0) scal[1] = sqrt(s);
 }
 
 // Gauss-Jordan inverse held in LDS (n <= 136), one block
@@ -448,8 +498,11 @@ __global__ __launch_bounds__(1024) void k_mg_invert_lds(int n, double* __restric
 }
 
 // ---- transfers ---------------------------------------------------------------
-// r_C = sum over the 27 fine points around 2C of w * res_f (w = prod (1 or 1/2)),
-// constrained fine DOFs skipped, constrained coarse DOFs zero (P^T)
+// r_C = sum over the 3^DIM fine points around 2C of w * res_f (w = prod of 1 or
+// 1/2), constrained fine DOFs weighted 0, constrained coarse DOFs zero (P^T).
+// FR: the fine vectors are indexed through rank_of_lat (the plate's problem
+// level); coarse levels are in lattice order.
+template <int DIM, int NAV, bool FR>
 __global__ __launch_bounds__(256) void k_mg_restrict(MGLevelArgs F, MGLevelArgs Cl, const int* __restrict__ done) {
     if (done && *done) return;
     const int64_t p = int64_t(blockIdx.x) * 256 + threadIdx.x;
@@ -457,29 +510,30 @@ __global__ __launch_bounds__(256) void k_mg_restrict(MGLevelArgs F, MGLevelArgs 
     const int64_t cx = Cl.nx + 1, cy = Cl.ny + 1;
     const int64_t I = p % cx, J = (p / cx) % cy, K = p / (cx * cy);
     const int64_t fx = F.nx + 1, fy = F.ny + 1;
-    // all 27 loads are independent (unrolled, predicated) so they are in flight together
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    constexpr int NO = DIM == 3 ? 27 : 9;
+    // all loads are independent (unrolled, predicated) so they are in flight together
+    double sv[NAV];
 #pragma unroll
-    for (int o = 0; o < 27; o++) {
-        const int oi = (o % 3) - 1, oj = ((o / 3) % 3) - 1, ok = (o / 9) - 1;
+    for (int v = 0; v < NAV; v++) sv[v] = 0.0;
+#pragma unroll
+    for (int o = 0; o < NO; o++) {
+        const int oi = (o % 3) - 1, oj = ((o / 3) % 3) - 1, ok = DIM == 3 ? (o / 9) - 1 : 0;
         const int64_t fi = 2 * I + oi, fj = 2 * J + oj, fk = 2 * K + ok;
         const bool in_lat = fi >= 0 && fi <= F.nx && fj >= 0 && fj <= F.ny && fk >= 0 && fk <= F.nz;
-        // out-of-lattice points are clamped (valid address) and weighted 0;
-        // constrained fine DOFs are weighted 0 (P has no row for them)
+        // out-of-lattice points are clamped (valid address) and weighted 0
         const double w = in_lat ? (ok ? 0.5 : 1.0) * (oj ? 0.5 : 1.0) * (oi ? 0.5 : 1.0) : 0.0;
         const int64_t fp = (in_lat ? fi : 0) + fx * ((in_lat ? fj : 0) + fy * (in_lat ? fk : 0));
-        const int64_t rf = fp * 3;
-        s0 += (w * (1.0 - double(F.con[rf + 0]))) * F.res[rf + 0];
-        s1 += (w * (1.0 - double(F.con[rf + 1]))) * F.res[rf + 1];
-        s2 += (w * (1.0 - double(F.con[rf + 2]))) * F.res[rf + 2];
+        const int64_t rf = (FR ? int64_t(F.rank_of_lat[fp]) : fp) * NAV;
+#pragma unroll
+        for (int v = 0; v < NAV; v++) sv[v] += (w * (1.0 - double(F.con[rf + v]))) * F.res[rf + v];
     }
-    const int64_t rc = p * 3;
-    Cl.r[rc + 0] = Cl.con[rc + 0] ? 0.0 : s0;
-    Cl.r[rc + 1] = Cl.con[rc + 1] ? 0.0 : s1;
-    Cl.r[rc + 2] = Cl.con[rc + 2] ? 0.0 : s2;
+    const int64_t rc = p * NAV;
+#pragma unroll
+    for (int v = 0; v < NAV; v++) Cl.r[rc + v] = Cl.con[rc + v] ? 0.0 : sv[v];
 }
 
-// x_f += P (x_C + dlast_C) at unconstrained fine DOFs (trilinear interpolation)
+// x_f += P (x_C + dlast_C) at unconstrained fine DOFs (tri/bilinear interpolation)
+template <int DIM, int NAV, bool FR>
 __global__ __launch_bounds__(256) void k_mg_prolong(MGLevelArgs F, MGLevelArgs Cl, const double* __restrict__ dlast,
                                                     const int* __restrict__ done) {
     if (done && *done) return;
@@ -488,48 +542,44 @@ __global__ __launch_bounds__(256) void k_mg_prolong(MGLevelArgs F, MGLevelArgs C
     const int64_t fx = F.nx + 1, fy = F.ny + 1;
     const int64_t fi = p % fx, fj = (p / fx) % fy, fk = p / (fx * fy);
     const int64_t cx = Cl.nx + 1, cy = Cl.ny + 1;
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    constexpr int NQ = 1 << DIM;
+    double sv[NAV];
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
-        const int a = q & 1, b = (q >> 1) & 1, c = q >> 2;
+    for (int v = 0; v < NAV; v++) sv[v] = 0.0;
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+        const int a = q & 1, b = (q >> 1) & 1, c = DIM == 3 ? (q >> 2) : 0;
         // an even fine index takes its one coarse point (a = 0), an odd one both
         // neighbours; the unused corner of an even index is clamped and weighted 0
         const bool use = !((a && !(fi & 1)) || (b && !(fj & 1)) || (c && !(fk & 1)));
         const double w = use ? ((fi & 1) ? 0.5 : 1.0) * ((fj & 1) ? 0.5 : 1.0) * ((fk & 1) ? 0.5 : 1.0) : 0.0;
         const int64_t CI = min((fi >> 1) + a, int64_t(Cl.nx)), CJ = min((fj >> 1) + b, int64_t(Cl.ny)),
                       CK = min((fk >> 1) + c, int64_t(Cl.nz));
-        const int64_t cp = CI + cx * (CJ + cy * CK);
-        const int64_t rc = cp * 3;
-        if (dlast) {
-            s0 += w * (Cl.x[rc + 0] + dlast[rc + 0]);
-            s1 += w * (Cl.x[rc + 1] + dlast[rc + 1]);
-            s2 += w * (Cl.x[rc + 2] + dlast[rc + 2]);
-        } else {
-            s0 += w * Cl.x[rc + 0];
-            s1 += w * Cl.x[rc + 1];
-            s2 += w * Cl.x[rc + 2];
-        }
+        const int64_t rc = (CI + cx * (CJ + cy * CK)) * NAV;
+#pragma unroll
+        for (int v = 0; v < NAV; v++) sv[v] += w * (dlast ? Cl.x[rc + v] + dlast[rc + v] : Cl.x[rc + v]);
     }
-    const int64_t rf = p * 3;
-    if (!F.con[rf + 0]) F.x[rf + 0] += s0;
-    if (!F.con[rf + 1]) F.x[rf + 1] += s1;
-    if (!F.con[rf + 2]) F.x[rf + 2] += s2;
+    const int64_t rf = (FR ? int64_t(F.rank_of_lat[p]) : p) * NAV;
+#pragma unroll
+    for (int v = 0; v < NAV; v++)
+        if (!F.con[rf + v]) F.x[rf + v] += sv[v];
 }
 
 // ---- coarsest level: dense inverse -------------------------------------------
+template <int NAV>
 __global__ __launch_bounds__(256) void k_mg_dense_gather(MGLevelArgs L, double* __restrict__ A) {
     const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
-    const int64_t n = L.n_nodes * 3;
+    const int64_t n = L.n_nodes * NAV;
     if (i >= n) return;
-    const int64_t r = i / 3;
-    const int v = int(i - r * 3);
+    const int64_t r = i / NAV;
+    const int v = int(i - r * NAV);
     const int64_t k0 = L.blk_ptr[r];
     const int64_t deg = L.blk_ptr[r + 1] - k0;
-    const double* row = L.val + k0 * 9 + int64_t(v) * deg * 3;
+    const double* row = L.val + k0 * (NAV * NAV) + int64_t(v) * deg * NAV;
     for (int64_t k = 0; k < deg; k++) {
         const int64_t c = L.blk_col[k0 + k];
 #pragma unroll
-        for (int w = 0; w < 3; w++) A[i * n + c * 3 + w] = row[k * 3 + w];
+        for (int w = 0; w < NAV; w++) A[i * n + c * NAV + w] = row[k * NAV + w];
     }
 }
 
@@ -661,24 +711,19 @@ static inline unsigned mg_grid(int64_t n) {
 
 void launch_mg_galerkin(const MGGalerkinArgs& g, hipStream_t s) {
     if (g.n_coarse <= 0) return;
-    hipLaunchKernelGGL(k_mg_galerkin, dim3(mg_blocks(g.n_coarse, 4)), dim3(256), 0, s, g);
-}
-void launch_mg_rowinfo_bsr(const MGLevelArgs& L, double* dinv, double* part, double* lam, hipStream_t s) {
-    const unsigned nb = mg_blocks(L.n_nodes * 3, 256);
-    hipLaunchKernelGGL(k_mg_rowinfo_bsr, dim3(nb), dim3(256), 0, s, L, dinv, part);
-    hipLaunchKernelGGL(k_mg_max, dim3(1), dim3(256), 0, s, part, int(nb), lam);
-}
-void launch_mg_rowinfo_csr(int64_t n, const int64_t* row_ptr, const double* val, const int64_t* diag_pos,
-                           double* part, double* lam, hipStream_t s) {
-    const unsigned nb = mg_blocks(n, 256);
-    hipLaunchKernelGGL(k_mg_rowinfo_csr, dim3(nb), dim3(256), 0, s, n, row_ptr, val, diag_pos, part);
-    hipLaunchKernelGGL(k_mg_max, dim3(1), dim3(256), 0, s, part, int(nb), lam);
+    if (g.dim == 2)
+        hipLaunchKernelGGL((k_mg_galerkin<2, 5>), dim3(mg_blocks(g.n_coarse, 4)), dim3(256), 0, s, g);
+    else
+        hipLaunchKernelGGL((k_mg_galerkin<3, 3>), dim3(mg_blocks(g.n_coarse, 4)), dim3(256), 0, s, g);
 }
 void launch_mg_cheb(const double* lam, double* cheb, int n_levels, int k, double ratio, hipStream_t s) {
     hipLaunchKernelGGL(k_mg_cheb, dim3(1), dim3(64), 0, s, lam, cheb, n_levels, k, ratio);
 }
 void launch_mg_dense_gather(const MGLevelArgs& L, double* A, hipStream_t s) {
-    hipLaunchKernelGGL(k_mg_dense_gather, dim3(mg_blocks(L.n_nodes * 3, 256)), dim3(256), 0, s, L, A);
+    if (L.nav == 5)
+        hipLaunchKernelGGL(k_mg_dense_gather<5>, dim3(mg_blocks(L.n_nodes * 5, 256)), dim3(256), 0, s, L, A);
+    else
+        hipLaunchKernelGGL(k_mg_dense_gather<3>, dim3(mg_blocks(L.n_nodes * 3, 256)), dim3(256), 0, s, L, A);
 }
 void launch_mg_invert(int n, double* A, hipStream_t s) {
     if (n <= 136)
@@ -686,12 +731,36 @@ void launch_mg_invert(int n, double* A, hipStream_t s) {
     else
         hipLaunchKernelGGL(k_mg_invert, dim3(1), dim3(1024), 0, s, n, A);
 }
-void launch_mg_rowinfo_nb(int64_t n_nodes, const int64_t* blk_ptr, const int32_t* blk_col, const double* val,
-                          double* dinv, double* part, double* lam, hipStream_t s) {
+static inline int nb_grid(int64_t n_nodes) {
     const int64_t blocks = (n_nodes * 8 + 255) / 256;
-    const int nb = int(blocks < 1 ? 1 : (blocks > 8192 ? 8192 : blocks));
-    hipLaunchKernelGGL(k_mg_rowinfo_nb, dim3(nb), dim3(256), 0, s, n_nodes, blk_ptr, blk_col, val, dinv, part);
+    return int(blocks < 1 ? 1 : (blocks > 8192 ? 8192 : blocks));
+}
+void launch_mg_rowinfo_nb(int nav, int64_t n_nodes, const int64_t* blk_ptr, const int32_t* blk_col,
+                          const double* val, double* dinv, double* part, double* lam, hipStream_t s) {
+    const int nb = nb_grid(n_nodes);
+    if (nav == 5)
+        hipLaunchKernelGGL(k_mg_rowinfo_nb<5>, dim3(nb), dim3(256), 0, s, n_nodes, blk_ptr, blk_col, val, dinv, part);
+    else
+        hipLaunchKernelGGL(k_mg_rowinfo_nb<3>, dim3(nb), dim3(256), 0, s, n_nodes, blk_ptr, blk_col, val, dinv, part);
     hipLaunchKernelGGL(k_mg_max, dim3(1), dim3(256), 0, s, part, nb, lam);
+}
+void launch_mg_power(const MGLevelArgs& L, const double* val, int n_it, double safety, double* w0, double* w1,
+                     double* part, double* scal, double* lam, hipStream_t s) {
+    const int nb = nb_grid(L.n_nodes);
+    const int nbs = int(std::min<int64_t>(mg_blocks(L.n_nodes * L.nav, 256), 8192));
+    hipLaunchKernelGGL(k_mg_power_start, dim3(nbs), dim3(256), 0, s, L.n_nodes * L.nav, part);
+    hipLaunchKernelGGL(k_mg_power_start_norm, dim3(1), dim3(256), 0, s, part, nbs, scal);
+    double* in = nullptr;
+    double* out = w0;
+    for (int it = 0; it < n_it; it++) {
+        if (L.nav == 5)
+            hipLaunchKernelGGL(k_mg_power<5>, dim3(nb), dim3(256), 0, s, L, val, in, scal, out, part);
+        else
+            hipLaunchKernelGGL(k_mg_power<3>, dim3(nb), dim3(256), 0, s, L, val, in, scal, out, part);
+        hipLaunchKernelGGL(k_mg_power_norm, dim3(1), dim3(256), 0, s, part, nb, it == 0 ? 1 : 0, safety, scal, lam);
+        in = out;
+        out = out == w0 ? w1 : w0;
+    }
 }
 int64_t lattice_partials(const LatticeOpArgs& l) {
     return (int64_t(l.nx + 1) * (l.ny + 1) * (l.nz + 1) + 255) / 256;
@@ -707,16 +776,22 @@ void launch_perm_nodes(int64_t n_nodes, const int32_t* map, const double* src, d
     hipLaunchKernelGGL(k_perm_nodes, dim3(mg_blocks(n_nodes, 256)), dim3(256), 0, s, n_nodes, map, src, dst, scatter);
 }
 void launch_mg_pre_init(const MGLevelArgs& L, const int* done, hipStream_t s) {
-    hipLaunchKernelGGL(k_mg_pre_init, dim3(mg_grid(L.n_nodes * 3)), dim3(256), 0, s, L, done);
+    hipLaunchKernelGGL(k_mg_pre_init, dim3(mg_grid(L.n_nodes * L.nav)), dim3(256), 0, s, L, done);
 }
 void launch_mg_bsr(const MGLevelArgs& L, int mode, int step, const double* din, double* dout, const int* done,
                    hipStream_t s) {
-    const int64_t blocks = (L.n_nodes * 8 + 255) / 256;
-    const unsigned g = unsigned(blocks < 1 ? 1 : (blocks > 8192 ? 8192 : blocks));
-    if (L.val32)
-        hipLaunchKernelGGL(k_mg_bsr<float>, dim3(g), dim3(256), 0, s, L, L.val32, mode, step, din, dout, done);
-    else
-        hipLaunchKernelGGL(k_mg_bsr<double>, dim3(g), dim3(256), 0, s, L, L.val, mode, step, din, dout, done);
+    const unsigned g = unsigned(nb_grid(L.n_nodes));
+    if (L.nav == 5) {
+        if (L.val32)
+            hipLaunchKernelGGL((k_mg_bsr<float, 5>), dim3(g), dim3(256), 0, s, L, L.val32, mode, step, din, dout, done);
+        else
+            hipLaunchKernelGGL((k_mg_bsr<double, 5>), dim3(g), dim3(256), 0, s, L, L.val, mode, step, din, dout, done);
+    } else {
+        if (L.val32)
+            hipLaunchKernelGGL((k_mg_bsr<float, 3>), dim3(g), dim3(256), 0, s, L, L.val32, mode, step, din, dout, done);
+        else
+            hipLaunchKernelGGL((k_mg_bsr<double, 3>), dim3(g), dim3(256), 0, s, L, L.val, mode, step, din, dout, done);
+    }
 }
 void launch_mg_fine(const CGArgs& c, const MGLevelArgs& L, int mode, int step, const double* din,
                     double* dout, const int* done, hipStream_t s) {
@@ -724,11 +799,25 @@ void launch_mg_fine(const CGArgs& c, const MGLevelArgs& L, int mode, int step, c
                        nullptr, L, mode, step, din, dout, done);
 }
 void launch_mg_restrict(const MGLevelArgs& F, const MGLevelArgs& C, const int* done, hipStream_t s) {
-    hipLaunchKernelGGL(k_mg_restrict, dim3(mg_blocks(C.n_nodes, 256)), dim3(256), 0, s, F, C, done);
+    const dim3 g(mg_blocks(C.n_nodes, 256));
+    if (F.dim == 2) {
+        if (F.rank_of_lat) hipLaunchKernelGGL((k_mg_restrict<2, 5, true>), g, dim3(256), 0, s, F, C, done);
+        else hipLaunchKernelGGL((k_mg_restrict<2, 5, false>), g, dim3(256), 0, s, F, C, done);
+    } else {
+        if (F.rank_of_lat) hipLaunchKernelGGL((k_mg_restrict<3, 3, true>), g, dim3(256), 0, s, F, C, done);
+        else hipLaunchKernelGGL((k_mg_restrict<3, 3, false>), g, dim3(256), 0, s, F, C, done);
+    }
 }
 void launch_mg_prolong(const MGLevelArgs& F, const MGLevelArgs& C, const double* dlast, const int* done,
                        hipStream_t s) {
-    hipLaunchKernelGGL(k_mg_prolong, dim3(mg_blocks(F.n_nodes, 256)), dim3(256), 0, s, F, C, dlast, done);
+    const dim3 g(mg_blocks(F.n_nodes, 256));
+    if (F.dim == 2) {
+        if (F.rank_of_lat) hipLaunchKernelGGL((k_mg_prolong<2, 5, true>), g, dim3(256), 0, s, F, C, dlast, done);
+        else hipLaunchKernelGGL((k_mg_prolong<2, 5, false>), g, dim3(256), 0, s, F, C, dlast, done);
+    } else {
+        if (F.rank_of_lat) hipLaunchKernelGGL((k_mg_prolong<3, 3, true>), g, dim3(256), 0, s, F, C, dlast, done);
+        else hipLaunchKernelGGL((k_mg_prolong<3, 3, false>), g, dim3(256), 0, s, F, C, dlast, done);
+    }
 }
 void launch_mg_coarse_solve(int n, const double* Ainv, const double* r, double* x, const int* done,
                             hipStream_t s) {

@@ -7,37 +7,40 @@ namespace mastg {
 
 namespace {
 
-// libMesh hex8 local node -> lattice offset of the cell corner
+// libMesh hex8 / quad4 local node -> cell corner offset (quad4: the first four)
 constexpr int kCorner[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
                                {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
 
-bool detect_lattice(const Structure& S, MGLevelHost& L, std::string& why) {
-    if (S.npe != 8 || S.nav != 3) { why = "multigrid is built for hex8 elasticity"; return false; }
+bool detect_lattice(const Structure& S, int dim, MGLevelHost& L, std::string& why) {
+    const int npe = dim == 3 ? 8 : 4;
     const double* X = S.xyz_r.data();
     const int32_t* en = S.econn.data();
-    // element 0 fixes the cell size; it must be an axis-aligned box in libMesh order
+    // element 0 fixes the cell size (corner 0 to the opposite corner); it must
+    // be an axis-aligned box in libMesh order
+    const int opp = dim == 3 ? 6 : 2;
     double h[3];
-    for (int d = 0; d < 3; d++) h[d] = X[int64_t(en[6]) * 3 + d] - X[int64_t(en[0]) * 3 + d];
-    if (!(h[0] > 0 && h[1] > 0 && h[2] > 0)) { why = "elements are not lattice cells"; return false; }
+    for (int d = 0; d < 3; d++) h[d] = X[int64_t(en[opp]) * 3 + d] - X[int64_t(en[0]) * 3 + d];
+    if (!(h[0] > 0 && h[1] > 0 && (dim == 2 || h[2] > 0))) { why = "elements are not lattice cells"; return false; }
     double lo[3] = {1e300, 1e300, 1e300};
     for (int64_t r = 0; r < S.n_nodes; r++)
         for (int d = 0; d < 3; d++) lo[d] = std::min(lo[d], X[r * 3 + d]);
-    std::vector<int32_t> ijk(size_t(S.n_nodes) * 3);
+    std::vector<int32_t> ijk(size_t(S.n_nodes) * 3, 0);
     int n[3] = {0, 0, 0};
     for (int64_t r = 0; r < S.n_nodes; r++)
-        for (int d = 0; d < 3; d++) {
+        for (int d = 0; d < dim; d++) {
             const double t = (X[r * 3 + d] - lo[d]) / h[d];
             const double ti = std::nearbyint(t);
             if (std::fabs(t - ti) > 1e-9 || ti > 2e9) { why = "nodes are not on a lattice"; return false; }
             ijk[r * 3 + d] = int32_t(ti);
             n[d] = std::max(n[d], int32_t(ti));
         }
+    L.dim = dim;
+    L.npe = npe;
+    L.nav = S.nav;
     L.nx = n[0]; L.ny = n[1]; L.nz = n[2];
     const int64_t nlat = int64_t(n[0] + 1) * (n[1] + 1) * (n[2] + 1);
-    if (nlat != S.n_nodes || int64_t(n[0]) * n[1] * n[2] != S.n_elems) {
-        why = "the mesh does not fill its lattice";
-        return false;
-    }
+    const int64_t ncell = int64_t(n[0]) * n[1] * (dim == 3 ? n[2] : 1);
+    if (nlat != S.n_nodes || ncell != S.n_elems) { why = "the mesh does not fill its lattice"; return false; }
     L.n_nodes = S.n_nodes;
     L.n_elems = S.n_elems;
     L.rank_of_lat.assign(size_t(nlat), -1);
@@ -48,10 +51,10 @@ bool detect_lattice(const Structure& S, MGLevelHost& L, std::string& why) {
     }
     L.elem_of_cell.assign(size_t(S.n_elems), -1);
     for (int64_t e = 0; e < S.n_elems; e++) {
-        const int32_t* c = en + e * 8;
+        const int32_t* c = en + e * npe;
         const int32_t i0 = ijk[int64_t(c[0]) * 3], j0 = ijk[int64_t(c[0]) * 3 + 1], k0 = ijk[int64_t(c[0]) * 3 + 2];
-        if (i0 >= n[0] || j0 >= n[1] || k0 >= n[2]) { why = "elements are not lattice cells"; return false; }
-        for (int a = 0; a < 8; a++) {
+        if (i0 >= n[0] || j0 >= n[1] || (dim == 3 && k0 >= n[2])) { why = "elements are not lattice cells"; return false; }
+        for (int a = 0; a < npe; a++) {
             const int64_t p = (i0 + kCorner[a][0]) +
                               int64_t(n[0] + 1) * ((j0 + kCorner[a][1]) + int64_t(n[1] + 1) * (k0 + kCorner[a][2]));
             if (L.rank_of_lat[p] != c[a]) { why = "elements are not lattice cells in libMesh order"; return false; }
@@ -67,25 +70,32 @@ bool detect_lattice(const Structure& S, MGLevelHost& L, std::string& why) {
 
 MGHierarchy build_mg_hierarchy(const Structure& S) {
     MGHierarchy H;
+    int dim = 0, physics = 0;
+    if (S.npe == 8 && S.nav == 3) { dim = 3; physics = 1; }
+    else if (S.npe == 4 && S.nav == 5) { dim = 2; physics = 2; }
+    else { H.why = "multigrid is built for hex8 elasticity and the quad4 Mindlin plate"; return H; }
+    const int nav = S.nav, npe = S.npe;
     MGLevelHost L0;
-    if (!detect_lattice(S, L0, H.why)) return H;
+    if (!detect_lattice(S, dim, L0, H.why)) return H;
     H.levels.push_back(std::move(L0));
     // geometry of the problem mesh's lattice
     double lo[3] = {1e300, 1e300, 1e300}, h[3];
     for (int64_t r = 0; r < S.n_nodes; r++)
         for (int d = 0; d < 3; d++) lo[d] = std::min(lo[d], S.xyz_r[r * 3 + d]);
-    for (int d = 0; d < 3; d++) h[d] = S.xyz_r[int64_t(S.econn[6]) * 3 + d] - S.xyz_r[int64_t(S.econn[0]) * 3 + d];
+    const int opp = dim == 3 ? 6 : 2;
+    for (int d = 0; d < 3; d++)
+        h[d] = S.xyz_r[int64_t(S.econn[opp]) * 3 + d] - S.xyz_r[int64_t(S.econn[0]) * 3 + d];
     // constraint mask of the current finest-so-far level, by rank
     std::vector<uint8_t> con = S.constrained;
     for (int lvl = 1; lvl < 16; lvl++) {
         const MGLevelHost& F = H.levels.back();
-        const int64_t fdofs = F.n_nodes * 3;
-        if (fdofs <= 600) break;
-        if (F.nx % 2 || F.ny % 2 || F.nz % 2) break;
+        if (F.n_nodes * nav <= kMGCoarseTarget) break;
+        if (F.nx % 2 || F.ny % 2 || (dim == 3 && F.nz % 2)) break;
         MGLevelHost C;
+        C.dim = dim; C.nav = nav; C.npe = npe;
         C.nx = F.nx / 2; C.ny = F.ny / 2; C.nz = F.nz / 2;
         C.n_nodes = int64_t(C.nx + 1) * (C.ny + 1) * (C.nz + 1);
-        C.n_elems = int64_t(C.nx) * C.ny * C.nz;
+        C.n_elems = int64_t(C.nx) * C.ny * (dim == 3 ? C.nz : 1);
         const double scale = double(int64_t(1) << lvl);
         std::vector<double> xyz(size_t(C.n_nodes) * 3);
         std::vector<int64_t> base(size_t(C.n_nodes));
@@ -96,24 +106,25 @@ MGHierarchy build_mg_hierarchy(const Structure& S) {
                     const int64_t p = i + int64_t(C.nx + 1) * (j + int64_t(C.ny + 1) * k);
                     xyz[p * 3 + 0] = lo[0] + i * scale * h[0];
                     xyz[p * 3 + 1] = lo[1] + j * scale * h[1];
-                    xyz[p * 3 + 2] = lo[2] + k * scale * h[2];
-                    base[p] = p * 3;
+                    xyz[p * 3 + 2] = dim == 3 ? lo[2] + k * scale * h[2] : lo[2];
+                    base[p] = p * nav;
                     const int64_t fp = 2 * i + int64_t(F.nx + 1) * (2 * j + int64_t(F.ny + 1) * 2 * k);
                     const int32_t fr = F.rank_of_lat[fp];
-                    for (int v = 0; v < 3; v++)
-                        if (con[size_t(fr) * 3 + v]) cdofs.push_back(p * 3 + v);
+                    for (int v = 0; v < nav; v++)
+                        if (con[size_t(fr) * nav + v]) cdofs.push_back(p * nav + v);
                 }
-        std::vector<int32_t> conn(size_t(C.n_elems) * 8);
-        for (int k = 0; k < C.nz; k++)
+        std::vector<int32_t> conn(size_t(C.n_elems) * npe);
+        const int cz = dim == 3 ? C.nz : 1;
+        for (int k = 0; k < cz; k++)
             for (int j = 0; j < C.ny; j++)
                 for (int i = 0; i < C.nx; i++) {
                     const int64_t e = i + int64_t(C.nx) * (j + int64_t(C.ny) * k);
-                    for (int a = 0; a < 8; a++)
-                        conn[e * 8 + a] = int32_t((i + kCorner[a][0]) +
-                                                  int64_t(C.nx + 1) * ((j + kCorner[a][1]) +
-                                                                       int64_t(C.ny + 1) * (k + kCorner[a][2])));
+                    for (int a = 0; a < npe; a++)
+                        conn[e * npe + a] = int32_t((i + kCorner[a][0]) +
+                                                    int64_t(C.nx + 1) * ((j + kCorner[a][1]) +
+                                                                         int64_t(C.ny + 1) * (k + kCorner[a][2])));
                 }
-        std::string err = build_structure(1, C.n_nodes, C.n_elems, xyz.data(), conn.data(), 3, base.data(),
+        std::string err = build_structure(physics, C.n_nodes, C.n_elems, xyz.data(), conn.data(), nav, base.data(),
                                           int64_t(cdofs.size()), cdofs.data(), C.S);
         if (!err.empty()) { H.why = "coarse level: " + err; return H; }
         C.rank_of_lat.resize(size_t(C.n_nodes));
@@ -124,7 +135,7 @@ MGHierarchy build_mg_hierarchy(const Structure& S) {
         H.levels.push_back(std::move(C));
     }
     if (H.levels.size() < 2) { H.why = "the lattice does not coarsen (odd cell counts)"; return H; }
-    H.coarsest_dofs = H.levels.back().n_nodes * 3;
+    H.coarsest_dofs = H.levels.back().n_nodes * nav;
     if (H.coarsest_dofs > kMGMaxCoarseDofs) {
         H.why = "coarsest level too large for the dense solve (" + std::to_string(H.coarsest_dofs) + " DOFs)";
         return H;

@@ -1,13 +1,15 @@
-// Geometric multigrid hierarchy for hex8 elasticity on lattice meshes (host).
+// Geometric multigrid hierarchy on lattice meshes (host): hex8 elasticity (3D,
+// 3 variables per node) and the quad4 Mindlin plate (2D, 5 variables).
 //
-// A mesh qualifies when its nodes form a complete (nx+1)(ny+1)(nz+1) lattice of
-// axis-aligned boxes and every element is one lattice cell in libMesh hex8
-// node order.  Levels halve every dimension while all three stay even; level
-// l+1's nodes are level l's nodes at even lattice indices.  Prolongation is
-// trilinear interpolation (a fine node interpolates from the corners of the
-// coarse cell containing it), coarse operators are Galerkin products
-// P^T A P formed element by element (exact for nested cells), and a coarse
-// DOF is constrained when the fine DOF at the same point is.
+// A mesh qualifies when its nodes form a complete lattice of axis-aligned
+// boxes ((nx+1)(ny+1)(nz+1) points; nz = 0 for the plate) and every element is
+// one lattice cell in libMesh node order.  Levels halve every dimension while
+// all stay even; level l+1's nodes are level l's nodes at even lattice
+// indices.  Prolongation is tri/bilinear interpolation (a fine node
+// interpolates from the corners of the coarse cell containing it), coarse
+// operators are Galerkin products P^T A P formed element by element (exact for
+// nested cells), and a coarse DOF is constrained when the fine DOF at the same
+// point is.
 #pragma once
 #include <cstdint>
 #include <string>
@@ -18,7 +20,8 @@
 namespace mastg {
 
 struct MGLevelHost {
-    int nx = 0, ny = 0, nz = 0;          // cells per dimension
+    int dim = 3, nav = 3, npe = 8;       // lattice dimension, variables and nodes per element
+    int nx = 0, ny = 0, nz = 0;          // cells per dimension (nz = 0 in 2D)
     int64_t n_nodes = 0, n_elems = 0;
     // node rank of lattice point (i + (nx+1)(j + (ny+1) k)); identity on coarse levels
     std::vector<int32_t> rank_of_lat;
@@ -36,8 +39,11 @@ struct MGHierarchy {
 
 // Largest coarsest-level system solved by the dense inverse.
 constexpr int64_t kMGMaxCoarseDofs = 1536;
+// Coarsening continues while a level has more DOFs than this (and its cell
+// counts are even): the coarsest inverse then fits in LDS.
+constexpr int64_t kMGCoarseTarget = 136;
 
-// Detects the lattice and builds the coarse levels (hex8 elasticity, nav = 3).
+// Detects the lattice and builds the coarse levels (hex8: nav 3, quad4: nav 5).
 MGHierarchy build_mg_hierarchy(const Structure& S);
 
 }  // namespace mastg
