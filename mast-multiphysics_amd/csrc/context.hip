@@ -797,6 +797,8 @@ int mg_solve(mastg_ctx* c, const double* b, const mastg_solve_opts* o, mastg_sol
         (void)hipGraphDestroy(g);
         c->mg_exec_len = chunk;
     }
+    double best2 = 1e300;
+    int stalls = 0;
     for (int launched = 0; !done && launched < max_it + chunk; launched += chunk) {
         if (c->use_graphs) HIPC(c, hipGraphLaunch(c->mg_exec, c->stream));
         else
@@ -804,6 +806,31 @@ int mg_solve(mastg_ctx* c, const double* b, const mastg_solve_opts* o, mastg_sol
         HIPC(c, hipMemcpyAsync(c->h_scal, a.S, sizeof(CGScalars), hipMemcpyDeviceToHost, c->stream));
         HIPC(c, hipStreamSynchronize(c->stream));
         done = c->h_scal->done;
+        if (done == 1) {
+            // confirm on the true residual b - A x (the recursion drifts on
+            // ill-conditioned plates); when it misses, PCG restarts from it
+            CGArgs an = a;
+            an.S = nullptr;
+            if (a.use_mf == 3) launch_lattice_apply(an, a.x, c->vtmp.as<double>(), c->stream);
+            else launch_spmv_csr(a, a.x, c->vtmp.as<double>(), 0, c->stream);
+            launch_axpby(S.n_active, 1.0, bl, -1.0, c->vtmp.as<double>(), c->stream);
+            HIPC(c, hipMemcpyAsync(a.r, c->vtmp.p, S.n_active * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+            launch_pcg_recheck(a, c->stream);
+            HIPC(c, hipMemcpyAsync(c->h_scal, a.S, sizeof(CGScalars), hipMemcpyDeviceToHost, c->stream));
+            HIPC(c, hipStreamSynchronize(c->stream));
+            done = c->h_scal->done;
+            if (done == 0) {
+                c->cg_replacements++;
+                const double rr = c->h_scal->rr;
+                if (rr < 0.25 * best2) stalls = 0;
+                else if (++stalls >= 8) { done = 5; break; }
+                best2 = std::min(best2, rr);
+                const double* dl = mg_vcycle(c, a, &a.S->done, c->stream);
+                launch_pcg_rz(a, c->mg[0]->a.x, dl, c->stream);
+                launch_pcg_start2(a, c->stream);
+                HIPC(c, hipMemcpyAsync(a.p, a.z, S.n_active * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+            }
+        }
     }
     if (!c->mg_power) {
         // back to rank order (x lives in c->vx for the callers)

@@ -248,6 +248,8 @@ void launch_pcg_update(const CGArgs& c, hipStream_t s);
 void launch_pcg_rz(const CGArgs& c, const double* x0, const double* dlast, hipStream_t s);
 void launch_pcg_start(const CGArgs& c, const double* b, double tol2, int max_it, hipStream_t s);
 void launch_pcg_start2(const CGArgs& c, hipStream_t s);
+// r.r of the current r (a restart's true residual): done = 1 / 0 / 2 as it meets tol
+void launch_pcg_recheck(const CGArgs& c, hipStream_t s);
 void launch_pcg_beta(const CGArgs& c, hipStream_t s);
 // pieces of the CG iteration shared with the preconditioned variant
 void launch_mf_elem(const CGArgs& c, const double* x, hipStream_t s);   // element phase only

@@ -693,6 +693,19 @@ __global__ __launch_bounds__(256) void k_pcg_start_scalars(CGArgs C, double tol2
     }
 }
 
+// after a restart from the true residual (partials of r.r in part1): converged?
+__global__ __launch_bounds__(256) void k_pcg_recheck(CGArgs C) {
+    const double rr = reduce_partials(C.part1, C.nvec);
+    if (threadIdx.x == 0) {
+        CGScalars* S = C.S;
+        S->rr = rr;
+        if (!isfinite(rr)) S->done = 4;
+        else if (rr <= S->tol2 * S->bb) S->done = 1;
+        else if (S->iter >= S->max_it) S->done = 2;
+        else S->done = 0;
+    }
+}
+
 __global__ __launch_bounds__(256) void k_pcg_start2(CGArgs C) {
     if (C.S->done) return;
     const double rz = reduce_partials(C.part0, C.nvec);
@@ -836,6 +849,10 @@ void launch_pcg_beta(const CGArgs& c, hipStream_t s) {
 void launch_pcg_start(const CGArgs& c, const double* b, double tol2, int max_it, hipStream_t s) {
     hipLaunchKernelGGL(k_pcg_start, dim3(c.nvec), dim3(256), 0, s, c, b);
     hipLaunchKernelGGL(k_pcg_start_scalars, dim3(1), dim3(256), 0, s, c, tol2, max_it);
+}
+void launch_pcg_recheck(const CGArgs& c, hipStream_t s) {
+    hipLaunchKernelGGL(k_pcg_start, dim3(c.nvec), dim3(256), 0, s, c, c.r);
+    hipLaunchKernelGGL(k_pcg_recheck, dim3(1), dim3(256), 0, s, c);
 }
 void launch_pcg_start2(const CGArgs& c, hipStream_t s) {
     hipLaunchKernelGGL(k_pcg_start2, dim3(1), dim3(256), 0, s, c);

@@ -105,7 +105,7 @@ def test_lattice_operator_equals_csr(oracle, mastg, mesh):
     ctx.close()
 
 
-@pytest.mark.parametrize("n", [16, 32])
+@pytest.mark.parametrize("n", [16, 24])
 def test_mg_plate_design_eval(oracle, mastg, n):
     """Mindlin plate: bilinear transfers of all 5 variables, Galerkin 5x5-block
     coarse operators, Chebyshev bounds from power iteration; answers at the
