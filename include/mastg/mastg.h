@@ -248,10 +248,13 @@ int mastg_set_tuning(mastg_ctx* ctx, int spmv_lanes, int grid);
 typedef struct mastg_server mastg_server;
 typedef struct {
     int32_t n_slots;          /* concurrent requests on the device                 */
-    int32_t solver;           /* MASTG_SOLVER_* for hex8 requests (plates use CG)  */
+    int32_t solver;           /* MASTG_SOLVER_*: MG serves hex8 and plates; other
+                                 solvers serve hex8, plates then use CG + Jacobi   */
     int64_t hbm_budget;       /* device bytes the server may hold; 0 = 64 GiB      */
     int32_t queue_capacity;   /* pending requests before submit returns BUSY       */
-    double rtol;              /* solve tolerance, 0 -> 1e-10                       */
+    double rtol;              /* hex8 solve tolerance, 0 -> 1e-10                  */
+    double rtol_plate;        /* plate solve tolerance, 0 -> 1e-7 (thin plates put
+                                 ||b - K u||/||b|| near 1e-8 in FP64 at 384^2)     */
 } mastg_server_opts;
 typedef struct {
     int32_t kind, nx, ny, nz;

@@ -56,7 +56,7 @@ class SolveStats(C.Structure):
 
 class ServerOpts(C.Structure):
     _fields_ = [("n_slots", C.c_int32), ("solver", C.c_int32), ("hbm_budget", C.c_int64),
-                ("queue_capacity", C.c_int32), ("rtol", C.c_double)]
+                ("queue_capacity", C.c_int32), ("rtol", C.c_double), ("rtol_plate", C.c_double)]
 
 
 class Request(C.Structure):
@@ -324,9 +324,9 @@ class Context:
 class Server:
     """Request server (continuous batching of design evaluations)."""
 
-    def __init__(self, n_slots=4, solver=SOLVER_MG, hbm_budget=0, queue_capacity=0, rtol=1e-10):
+    def __init__(self, n_slots=4, solver=SOLVER_MG, hbm_budget=0, queue_capacity=0, rtol=1e-10, rtol_plate=1e-7):
         h = _P()
-        o = ServerOpts(n_slots, solver, hbm_budget, queue_capacity, rtol)
+        o = ServerOpts(n_slots, solver, hbm_budget, queue_capacity, rtol, rtol_plate)
         rc = lib().mastg_server_create(C.byref(o), C.byref(h))
         if rc:
             raise MastgError(rc, "server_create")
