@@ -185,14 +185,9 @@ __constant__ int8_t kBrows[3][3] = {{0, 3, 5}, {1, 3, 4}, {2, 4, 5}};
 __constant__ int8_t kBdirs[3][3] = {{0, 1, 2}, {1, 0, 2}, {2, 1, 0}};
 
 __global__ __launch_bounds__(256) void k_hex8_elastic_K_f32(Hex8Args A, float* __restrict__ Kel32) {
-    // per wave (one element): G and H as FP32 [48][33] (columns 24..32 zero),
-    // then K = G^T H on the matrix cores; K reuses the G area for the store
-    constexpr int LD = 33;
-    __shared__ float sGH[4][2][48 * LD];
     __shared__ float sg[4][8][8][3];   // [wave][q][node][d]
     __shared__ float sc[4][8];
     __shared__ float sD[36];
-    __shared__ uint8_t scon[4][8];     // constrained-variable bits of the element's nodes
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t e = int64_t(blockIdx.x) * 4 + wave;
     const bool valid = e < A.n_elems;
@@ -217,72 +212,47 @@ __global__ __launch_bounds__(256) void k_hex8_elastic_K_f32(Hex8Args A, float* _
         sg[wave][q][n][1] = float(dxi * ixi[1] + deta * ieta[1] + dzeta * izeta[1]);
         sg[wave][q][n][2] = float(dxi * ixi[2] + deta * ieta[2] + dzeta * izeta[2]);
         if (n == 0) sc[wave][q] = float(jac);
-        if (q == 0) {
-            uint8_t bits = 0;
-            if (A.con)
-                for (int v = 0; v < 3; v++) bits |= A.con[int64_t(rn) * 3 + v] ? uint8_t(1 << v) : uint8_t(0);
-            scon[wave][n] = bits;
-        }
-    }
-    __syncthreads();
-    float* G = sGH[wave][0];
-    float* H = sGH[wave][1];
-    // G[s][c] = B_q[row][c], H[s][c] = JxW_q (D B_q)[row][c], s = 6 q + row, c = v*8 + a
-    for (int idx = lane; idx < 48 * 32; idx += 64) {
-        const int srow = idx >> 5, c = idx & 31;
-        float gv = 0.f, hv = 0.f;
-        if (c < 24) {
-            const int q = srow / 6, row = srow - 6 * q, v = c >> 3, a = c & 7;
-            const int d = kBcomp[row][v];
-            if (d >= 0) gv = sg[wave][q][a][d];
-            float h = 0.f;
-#pragma unroll
-            for (int t = 0; t < 3; t++) h += sD[row * 6 + kBrows[v][t]] * sg[wave][q][a][kBdirs[v][t]];
-            hv = sc[wave][q] * h;
-        }
-        G[srow * LD + c] = gv;
-        H[srow * LD + c] = hv;
-    }
-    __syncthreads();
-    const int li = lane & 15, lk = lane >> 4;
-    mastg_floatx4 acc[4];
-#pragma unroll
-    for (int tile = 0; tile < 4; tile++) {
-        const int I = tile >> 1, Jt = tile & 1;
-        acc[tile] = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int ks = 0; ks < 12; ks++) {
-            const int srow = 4 * ks + lk;
-            acc[tile] = __builtin_amdgcn_mfma_f32_16x16x4f32(G[srow * LD + 16 * I + li], H[srow * LD + 16 * Jt + li],
-                                                             acc[tile], 0, 0, 0);
-        }
-    }
-    __syncthreads();
-    // K (24 x 24) into the G area, then one node-pair block per lane
-    float* K = G;
-#pragma unroll
-    for (int tile = 0; tile < 4; tile++) {
-        const int I = tile >> 1, Jt = tile & 1;
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int i = 16 * I + 4 * lk + r, j = 16 * Jt + li;
-            if (i < 24 && j < 24) K[i * LD + j] = acc[tile][r];
-        }
     }
     __syncthreads();
     if (!valid) return;
-    const float Ee = float(A.E0 * pow(A.param[e], A.penalty));
-    const int a = lane >> 3, b = lane & 7;
-    const uint8_t ca = scon[wave][a], cb = scon[wave][b];
-    float* out = Kel32 + (e * 64 + lane) * 9;
+    const int li = lane & 15, lk = lane >> 4;
+    const double Ee = A.E0 * pow(A.param[e], A.penalty);
 #pragma unroll
-    for (int v = 0; v < 3; v++)
+    for (int tile = 0; tile < 4; tile++) {
+        const int I = tile >> 1, Jt = tile & 1;
+        const int ca = 16 * I + li, cb = 16 * Jt + li;     // this lane's A column / B column
+        mastg_floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll 4
+        for (int ks = 0; ks < 12; ks++) {
+            const int sidx = 4 * ks + lk;
+            const int q = sidx / 6, row = sidx - 6 * q;
+            float av = 0.f, bv = 0.f;
+            if (ca < 24) {
+                const int d = kBcomp[row][ca >> 3];
+                if (d >= 0) av = sg[wave][q][ca & 7][d];
+            }
+            if (cb < 24) {
+                const int v = cb >> 3, b = cb & 7;
+                float h = 0.f;
 #pragma unroll
-        for (int w = 0; w < 3; w++) {
-            float kv = Ee * K[(v * 8 + a) * LD + (w * 8 + b)];
-            if (((ca >> v) & 1) || ((cb >> w) & 1)) kv = (a == b && v == w) ? 1.0f : 0.0f;
-            out[v * 3 + w] = kv;
+                for (int t = 0; t < 3; t++) h += sD[row * 6 + kBrows[v][t]] * sg[wave][q][b][kBdirs[v][t]];
+                bv = sc[wave][q] * h;
+            }
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc, 0, 0, 0);
         }
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int i = 16 * I + 4 * lk + r, j = 16 * Jt + li;
+            if (i < 24 && j < 24) {
+                const int a = i & 7, v = i >> 3, b = j & 7, w = j >> 3;
+                const int32_t ra = A.econn[e * 8 + a], rb = A.econn[e * 8 + b];
+                float kv = float(Ee * double(acc[r]));
+                if (A.con && (A.con[int64_t(ra) * 3 + v] || A.con[int64_t(rb) * 3 + w]))
+                    kv = (a == b && v == w) ? 1.0f : 0.0f;
+                Kel32[(e * 64 + a * 8 + b) * 9 + v * 3 + w] = kv;
+            }
+        }
+    }
 }
 
 // segmented reduction of FP32 element blocks into the FP64 CSR values
