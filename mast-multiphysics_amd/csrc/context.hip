@@ -109,6 +109,9 @@ struct mastg_ctx {
     bool mg_power = false;     // Chebyshev bounds from power iteration (plates)
     hipGraphExec_t mg_exec = nullptr;
     int mg_exec_len = 0;
+    hipGraphExec_t mg_bicg_exec = nullptr;
+    int mg_bicg_exec_len = 0;
+    bool mg_bicg_exec_T = false;
     // launch shapes (autotuned on the first solve)
     int spmv_sw = 16;
     bool use_graphs = true;
@@ -526,9 +529,11 @@ int mg_build(mastg_ctx* c) {
     if (c->mg_tried) return c->mg.empty() ? fail(c, MASTG_ERR_INVALID, "multigrid unavailable: " + c->mg_why)
                                           : MASTG_OK;
     c->mg_tried = true;
-    const bool plate = c->d.physics == MASTG_QUAD4_MINDLIN;
-    if (!(plate || (c->d.physics == MASTG_HEX8_ELASTIC && c->S.n_classes == 1))) {
-        c->mg_why = "multigrid is built for hex8 elasticity (one element shape) and the Mindlin plate on lattice meshes";
+    // the plate and heat problem levels smooth with their assembled node blocks
+    // in node-rank order; hex8 elasticity uses the per-node lattice operator
+    const bool plate = c->d.physics != MASTG_HEX8_ELASTIC;
+    if (!(plate || c->S.n_classes == 1)) {
+        c->mg_why = "multigrid is built for lattice meshes (hex8 elasticity needs one element shape)";
         return fail(c, MASTG_ERR_INVALID, "multigrid unavailable: " + c->mg_why);
     }
     MGHierarchy H = build_mg_hierarchy(c->S);
@@ -550,9 +555,9 @@ int mg_build(mastg_ctx* c) {
         const int nav = h.nav;
         const size_t nv = size_t(h.n_nodes) * nav * sizeof(double);
         if (l == 0 && plate) {
-            // the plate's problem level smooths with the assembled node blocks
-            // (FP32 copy) in the caller's node-rank order; the transfers map
-            // lattice points to ranks
+            // plate / heat: the problem level smooths with the assembled node
+            // blocks (FP32 copy) in the caller's node-rank order; the transfers
+            // map lattice points to ranks
             TRY(upload(c, L->rank_of_lat, h.rank_of_lat));
             TRY(upload(c, L->elem_of_cell, h.elem_of_cell));
             TRY(dalloc(c, c->val32, size_t(c->S.nnz) * sizeof(float)));
@@ -628,7 +633,7 @@ int mg_build(mastg_ctx* c) {
     TRY(dalloc(c, c->mg_scal, 2 * sizeof(double)));
     // plates: the Gershgorin bound of D^-1 A is loose by ~50x (thin-plate
     // bending against membrane rows); their Chebyshev intervals come from
-    // power iteration instead
+    // power iteration instead (heat too: its Jacobian is not symmetric)
     c->mg_power = plate;
     for (int l = 0; l < nl; l++) lv[l]->a.cheb = c->mg_cheb.as<double>() + l * kMGChebStride;
     c->mg = std::move(lv);
@@ -705,26 +710,31 @@ bool lattice_args(mastg_ctx* c, CGArgs& a) {
     return lattice_partials(a.lat) <= int64_t(c->part0.bytes / sizeof(double));
 }
 
-// one V-cycle on a.r; the level-0 output is x0 + dlast0 (formed by the caller)
-const double* mg_vcycle(mastg_ctx* c, const CGArgs& a, const int* done, hipStream_t s) {
+// one V-cycle on the problem-level vector r0 (null: the level-0 args' r); the
+// level-0 output is x0 + dlast0 (formed by the caller)
+const double* mg_vcycle(mastg_ctx* c, const CGArgs& a, const int* done, hipStream_t s,
+                        const double* r0 = nullptr) {
     const int nl = int(c->mg.size());
     const int k = c->mg_k;
-    auto dbuf = [&](int l, int i) { return (i % 2 == 0) ? c->mg[l]->a.da : c->mg[l]->a.db; };
+    MGLevelArgs L0 = c->mg[0]->a;
+    if (r0) L0.r = const_cast<double*>(r0);
+    auto lv = [&](int l) -> const MGLevelArgs& { return l == 0 ? L0 : c->mg[l]->a; };
+    auto dbuf = [&](int l, int i) { return (i % 2 == 0) ? lv(l).da : lv(l).db; };
     auto apply = [&](int l, int mode, int step, const double* din, double* dout) {
-        if (l == 0 && !c->mg_power) launch_mg_fine(a, c->mg[0]->a, mode, step, din, dout, done, s);
-        else launch_mg_bsr(c->mg[l]->a, mode, step, din, dout, done, s);
+        if (l == 0 && !c->mg_power) launch_mg_fine(a, L0, mode, step, din, dout, done, s);
+        else launch_mg_bsr(lv(l), mode, step, din, dout, done, s);
     };
     for (int l = 0; l + 1 < nl; l++) {
-        launch_mg_pre_init(c->mg[l]->a, done, s);
+        launch_mg_pre_init(lv(l), done, s);
         for (int i = 1; i < k; i++) apply(l, MG_STEP, i, dbuf(l, i - 1), dbuf(l, i));
         apply(l, MG_FINAL, 0, dbuf(l, k - 1), nullptr);
-        launch_mg_restrict(c->mg[l]->a, c->mg[l + 1]->a, done, s);
+        launch_mg_restrict(lv(l), lv(l + 1), done, s);
     }
-    const MGLevelArgs& Lc = c->mg[nl - 1]->a;
+    const MGLevelArgs& Lc = lv(nl - 1);
     launch_mg_coarse_solve(int(c->mg_n_coarse), c->mg_dense.as<double>(), Lc.r, Lc.x, done, s);
     const double* dlast = nullptr;
     for (int l = nl - 2; l >= 0; l--) {
-        launch_mg_prolong(c->mg[l]->a, c->mg[l + 1]->a, dlast, done, s);
+        launch_mg_prolong(lv(l), lv(l + 1), dlast, done, s);
         apply(l, MG_RESID_INIT, 0, nullptr, dbuf(l, 0));
         for (int i = 1; i < k; i++) apply(l, MG_STEP, i, dbuf(l, i - 1), dbuf(l, i));
         dlast = dbuf(l, k - 1);
@@ -898,8 +908,23 @@ int ensure_bicg(mastg_ctx* c) {
 
 // BiCGStab + Jacobi on the resident matrix (or its transpose): b -> c->vx.
 // The true residual ||b - A x|| / ||b|| is recomputed at the end.
+// BiCGStab iteration right-preconditioned by the multigrid V-cycle: ph = M p,
+// sh = M s replace the Jacobi scalings of the plain iteration
+void mg_bicg_iteration(mastg_ctx* c, const BiCGArgs& a, hipStream_t s) {
+    const int* done = &a.S->done;
+    const CGArgs none{};
+    const int64_t n = c->S.n_active;
+    launch_bicg_part(a, 0, s);
+    const double* dl = mg_vcycle(c, none, done, s, a.p);
+    launch_mg_combine(n, c->mg[0]->a.x, dl, a.ph, done, s);
+    launch_bicg_part(a, 1, s);
+    dl = mg_vcycle(c, none, done, s, a.s);
+    launch_mg_combine(n, c->mg[0]->a.x, dl, a.sh, done, s);
+    launch_bicg_part(a, 2, s);
+}
+
 int bicg_solve(mastg_ctx* c, const double* b, const mastg_solve_opts* o, mastg_solve_stats* st,
-               bool transpose) {
+               bool transpose, bool mg = false) {
     const Structure& S = c->S;
     const double rtol = (o && o->rtol > 0) ? o->rtol : 1e-10;
     const int max_it = (o && o->max_iters > 0) ? o->max_iters : 100000;
@@ -917,28 +942,40 @@ int bicg_solve(mastg_ctx* c, const double* b, const mastg_solve_opts* o, mastg_s
     HIPC(c, ev_create(&e0));
     HIPC(c, ev_create(&e1));
     HIPC(c, hipEventRecord(e0, c->stream));
+    if (mg) {
+        TRY(mg_build(c));
+        TRY(mg_setup(c));
+    }
     launch_bicg_start(a, b, rtol * rtol, max_it, c->stream);
     int done = 0;
+    // graphs: one per (preconditioner, operator) pair
+    hipGraphExec_t& ex = mg ? c->mg_bicg_exec : c->bicg_exec;
+    int& ex_len = mg ? c->mg_bicg_exec_len : c->bicg_exec_len;
+    bool& ex_T = mg ? c->mg_bicg_exec_T : c->bicg_exec_T;
+    auto iteration = [&]() {
+        if (mg) mg_bicg_iteration(c, a, c->stream);
+        else launch_bicg_iteration(a, c->stream);
+    };
     if (c->use_graphs) {
-        if (!c->bicg_exec || c->bicg_exec_len != chunk || c->bicg_exec_T != transpose) {
-            if (c->bicg_exec) (void)hipGraphExecDestroy(c->bicg_exec);
-            c->bicg_exec = nullptr;
+        if (!ex || ex_len != chunk || ex_T != transpose) {
+            if (ex) (void)hipGraphExecDestroy(ex);
+            ex = nullptr;
             HipLock capture_lock(g_hip_mu);
-        hipGraph_t g;
+            hipGraph_t g;
             HIPC(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            for (int i = 0; i < chunk; i++) launch_bicg_iteration(a, c->stream);
+            for (int i = 0; i < chunk; i++) iteration();
             HIPC(c, hipStreamEndCapture(c->stream, &g));
-            HIPC(c, hipGraphInstantiate(&c->bicg_exec, g, nullptr, nullptr, 0));
+            HIPC(c, hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
             (void)hipGraphDestroy(g);
-            c->bicg_exec_len = chunk;
-            c->bicg_exec_T = transpose;
+            ex_len = chunk;
+            ex_T = transpose;
         }
     }
     BiCGScalars hs{};
     for (int launched = 0; launched < max_it + chunk; launched += chunk) {
-        if (c->use_graphs) HIPC(c, hipGraphLaunch(c->bicg_exec, c->stream));
+        if (c->use_graphs) HIPC(c, hipGraphLaunch(ex, c->stream));
         else
-            for (int i = 0; i < chunk; i++) launch_bicg_iteration(a, c->stream);
+            for (int i = 0; i < chunk; i++) iteration();
         HIPC(c, hipMemcpyAsync(c->h_pin, a.S, sizeof(BiCGScalars), hipMemcpyDeviceToHost, c->stream));
         HIPC(c, hipStreamSynchronize(c->stream));
         std::memcpy(&hs, c->h_pin, sizeof(BiCGScalars));
@@ -1009,6 +1046,9 @@ int heat_design_eval(mastg_ctx* c, double* dobj, int where, const mastg_solve_op
                      mastg_design_result* out) {
     const Structure& S = c->S;
     const double newton_rtol = (o && o->rtol > 0) ? o->rtol : 1e-10;
+    // the Newton and adjoint BiCGStab solves are preconditioned by the
+    // multigrid V-cycle (rebuilt from each Newton Jacobian) or by Jacobi
+    const bool heat_mg = o && o->solver == MASTG_SOLVER_MG;
     hipEvent_t e[4];
     for (auto& x : e) HIPC(c, ev_create(&x));
     double t_asm = 0, t_solve = 0;
@@ -1038,7 +1078,7 @@ int heat_design_eval(mastg_ctx* c, double* dobj, int where, const mastg_solve_op
         lo.rtol = std::max(1e-13, std::min(1e-4, 1e-2 * rn / r0));
         lo.max_iters = (o && o->max_iters > 0) ? o->max_iters : 100000;
         mastg_solve_stats st{};
-        rc = bicg_solve(c, c->vb.as<double>(), &lo, &st, false);
+        rc = bicg_solve(c, c->vb.as<double>(), &lo, &st, false, heat_mg);
         t_solve += st.seconds;
         lin_iters += st.iters;
         last_rel = st.rel_res;
@@ -1060,7 +1100,7 @@ int heat_design_eval(mastg_ctx* c, double* dobj, int where, const mastg_solve_op
     ao.rtol = 1e-12;
     ao.max_iters = (o && o->max_iters > 0) ? o->max_iters : 100000;
     mastg_solve_stats ast{};
-    rc = bicg_solve(c, c->vb.as<double>(), &ao, &ast, true);
+    rc = bicg_solve(c, c->vb.as<double>(), &ao, &ast, true, heat_mg);
     if (rc) return rc;
     lin_iters += ast.iters;
     // dJ/dq_e = -lambda . dR/dq_e = lambda_e . int N_e
@@ -1293,6 +1333,7 @@ int mastg_ctx_destroy(mastg_ctx* c) {
         if (c->cg_exec[i]) (void)hipGraphExecDestroy(c->cg_exec[i]);
     if (c->bicg_exec) (void)hipGraphExecDestroy(c->bicg_exec);
     if (c->mg_exec) (void)hipGraphExecDestroy(c->mg_exec);
+    if (c->mg_bicg_exec) (void)hipGraphExecDestroy(c->mg_bicg_exec);
     if (c->h_scal) (void)hipHostFree(c->h_scal);
     if (c->h_pin) (void)hipHostFree(c->h_pin);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -1394,9 +1435,10 @@ int mastg_solve(mastg_ctx* c, const double* rhs, double* x, int where, const mas
     int solver = o ? o->solver : MASTG_SOLVER_CG_JACOBI;
     if (c->d.physics == MASTG_HEX8_HEAT && solver == MASTG_SOLVER_CG_JACOBI)
         solver = MASTG_SOLVER_BICGSTAB_JACOBI;   // the k(T) Jacobian is not symmetric
-    if (solver == MASTG_SOLVER_BICGSTAB_JACOBI) {
+    const bool heat_mg = c->d.physics == MASTG_HEX8_HEAT && solver == MASTG_SOLVER_MG;
+    if (solver == MASTG_SOLVER_BICGSTAB_JACOBI || heat_mg) {
         TRY(to_active(c, rhs, where, c->vb.as<double>()));
-        int rc = bicg_solve(c, c->vb.as<double>(), o, st, o && o->transpose);
+        int rc = bicg_solve(c, c->vb.as<double>(), o, st, o && o->transpose, heat_mg);
         int rc2 = from_active(c, c->vx.as<double>(), x, where);
         return rc ? rc : rc2;
     }

@@ -134,6 +134,9 @@ struct BiCGArgs {
 };
 void launch_bicg_start(const BiCGArgs& a, const double* b, double tol2, int max_it, hipStream_t s);
 void launch_bicg_iteration(const BiCGArgs& a, hipStream_t s);
+// the iteration in three parts (p and ph = dinv p | v = A ph, alpha, s and sh =
+// dinv s | t = A sh, x, r, check) so a preconditioner can replace ph and sh
+void launch_bicg_part(const BiCGArgs& a, int part, hipStream_t s);
 void launch_gather_values(int64_t n, const int32_t* idx, const double* src, double* dst, hipStream_t s);
 
 void launch_to_f32(int64_t n, const double* src, float* dst, hipStream_t s);
@@ -251,6 +254,9 @@ void launch_pcg_start2(const CGArgs& c, hipStream_t s);
 // r.r of the current r (a restart's true residual): done = 1 / 0 / 2 as it meets tol
 void launch_pcg_recheck(const CGArgs& c, hipStream_t s);
 void launch_pcg_beta(const CGArgs& c, hipStream_t s);
+// out = x0 + dlast (a V-cycle's level-0 output), skipped when *done
+void launch_mg_combine(int64_t n, const double* x0, const double* dlast, double* out, const int* done,
+                       hipStream_t s);
 // pieces of the CG iteration shared with the preconditioned variant
 void launch_mf_elem(const CGArgs& c, const double* x, hipStream_t s);   // element phase only
 void launch_cg_op_alpha(const CGArgs& c, hipStream_t s);                // q = A p, p.q, alpha

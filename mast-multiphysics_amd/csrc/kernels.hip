@@ -1359,15 +1359,22 @@ void launch_bicg_start(const BiCGArgs& a, const double* b, double tol2, int max_
     hipLaunchKernelGGL(k_bicg_start, dim3(a.nvec), dim3(256), 0, s, a, b);
     hipLaunchKernelGGL(k_bicg_start_s, dim3(1), dim3(256), 0, s, a, tol2, max_it);
 }
+void launch_bicg_part(const BiCGArgs& a, int part, hipStream_t s) {
+    if (part == 0) {
+        hipLaunchKernelGGL(k_bicg_rho, dim3(a.nvec), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(k_bicg_p, dim3(a.nvec), dim3(256), 0, s, a);
+    } else if (part == 1) {
+        spmv_gen(a, a.ph, a.v, a.r0, 0, s);
+        hipLaunchKernelGGL(k_bicg_alpha, dim3(1), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(k_bicg_s, dim3(a.nvec), dim3(256), 0, s, a);
+    } else {
+        spmv_gen(a, a.sh, a.t, a.s, 1, s);
+        hipLaunchKernelGGL(k_bicg_xr, dim3(a.nvec), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(k_bicg_check, dim3(1), dim3(256), 0, s, a);
+    }
+}
 void launch_bicg_iteration(const BiCGArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_bicg_rho, dim3(a.nvec), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(k_bicg_p, dim3(a.nvec), dim3(256), 0, s, a);
-    spmv_gen(a, a.ph, a.v, a.r0, 0, s);
-    hipLaunchKernelGGL(k_bicg_alpha, dim3(1), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(k_bicg_s, dim3(a.nvec), dim3(256), 0, s, a);
-    spmv_gen(a, a.sh, a.t, a.s, 1, s);
-    hipLaunchKernelGGL(k_bicg_xr, dim3(a.nvec), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(k_bicg_check, dim3(1), dim3(256), 0, s, a);
+    for (int part = 0; part < 3; part++) launch_bicg_part(a, part, s);
 }
 void launch_gather_values(int64_t n, const int32_t* idx, const double* src, double* dst, hipStream_t s) {
     hipLaunchKernelGGL(k_gather_values, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, idx, src, dst);

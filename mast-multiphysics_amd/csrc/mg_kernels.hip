@@ -661,6 +661,14 @@ __global__ __launch_bounds__(256) void k_pcg_rz(CGArgs C, const double* __restri
 }
 
 // beta = rz_new / rz ; a non-positive r.z means the preconditioner is not SPD
+__global__ __launch_bounds__(256) void k_mg_combine(int64_t n, const double* __restrict__ x0,
+                                                    const double* __restrict__ dlast, double* __restrict__ out,
+                                                    const int* __restrict__ done) {
+    if (done && *done) return;
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256)
+        out[i] = dlast ? x0[i] + dlast[i] : x0[i];
+}
+
 __global__ __launch_bounds__(256) void k_pcg_beta(CGArgs C) {
     if (C.S->done) return;
     const double rz = reduce_partials(C.part0, C.nvec);
@@ -726,6 +734,8 @@ void launch_mg_galerkin(const MGGalerkinArgs& g, hipStream_t s) {
     if (g.n_coarse <= 0) return;
     if (g.dim == 2)
         hipLaunchKernelGGL((k_mg_galerkin<2, 5>), dim3(mg_blocks(g.n_coarse, 4)), dim3(256), 0, s, g);
+    else if (g.nav == 1)
+        hipLaunchKernelGGL((k_mg_galerkin<3, 1>), dim3(mg_blocks(g.n_coarse, 4)), dim3(256), 0, s, g);
     else
         hipLaunchKernelGGL((k_mg_galerkin<3, 3>), dim3(mg_blocks(g.n_coarse, 4)), dim3(256), 0, s, g);
 }
@@ -735,6 +745,8 @@ void launch_mg_cheb(const double* lam, double* cheb, int n_levels, int k, double
 void launch_mg_dense_gather(const MGLevelArgs& L, double* A, hipStream_t s) {
     if (L.nav == 5)
         hipLaunchKernelGGL(k_mg_dense_gather<5>, dim3(mg_blocks(L.n_nodes * 5, 256)), dim3(256), 0, s, L, A);
+    else if (L.nav == 1)
+        hipLaunchKernelGGL(k_mg_dense_gather<1>, dim3(mg_blocks(L.n_nodes, 256)), dim3(256), 0, s, L, A);
     else
         hipLaunchKernelGGL(k_mg_dense_gather<3>, dim3(mg_blocks(L.n_nodes * 3, 256)), dim3(256), 0, s, L, A);
 }
@@ -753,6 +765,8 @@ void launch_mg_rowinfo_nb(int nav, int64_t n_nodes, const int64_t* blk_ptr, cons
     const int nb = nb_grid(n_nodes);
     if (nav == 5)
         hipLaunchKernelGGL(k_mg_rowinfo_nb<5>, dim3(nb), dim3(256), 0, s, n_nodes, blk_ptr, blk_col, val, dinv, part);
+    else if (nav == 1)
+        hipLaunchKernelGGL(k_mg_rowinfo_nb<1>, dim3(nb), dim3(256), 0, s, n_nodes, blk_ptr, blk_col, val, dinv, part);
     else
         hipLaunchKernelGGL(k_mg_rowinfo_nb<3>, dim3(nb), dim3(256), 0, s, n_nodes, blk_ptr, blk_col, val, dinv, part);
     hipLaunchKernelGGL(k_mg_max, dim3(1), dim3(256), 0, s, part, nb, lam);
@@ -768,6 +782,8 @@ void launch_mg_power(const MGLevelArgs& L, const double* val, int n_it, double s
     for (int it = 0; it < n_it; it++) {
         if (L.nav == 5)
             hipLaunchKernelGGL(k_mg_power<5>, dim3(nb), dim3(256), 0, s, L, val, in, scal, out, part);
+        else if (L.nav == 1)
+            hipLaunchKernelGGL(k_mg_power<1>, dim3(nb), dim3(256), 0, s, L, val, in, scal, out, part);
         else
             hipLaunchKernelGGL(k_mg_power<3>, dim3(nb), dim3(256), 0, s, L, val, in, scal, out, part);
         hipLaunchKernelGGL(k_mg_power_norm, dim3(1), dim3(256), 0, s, part, nb, it == 0 ? 1 : 0, safety, scal, lam);
@@ -799,6 +815,11 @@ void launch_mg_bsr(const MGLevelArgs& L, int mode, int step, const double* din, 
             hipLaunchKernelGGL((k_mg_bsr<float, 5>), dim3(g), dim3(256), 0, s, L, L.val32, mode, step, din, dout, done);
         else
             hipLaunchKernelGGL((k_mg_bsr<double, 5>), dim3(g), dim3(256), 0, s, L, L.val, mode, step, din, dout, done);
+    } else if (L.nav == 1) {
+        if (L.val32)
+            hipLaunchKernelGGL((k_mg_bsr<float, 1>), dim3(g), dim3(256), 0, s, L, L.val32, mode, step, din, dout, done);
+        else
+            hipLaunchKernelGGL((k_mg_bsr<double, 1>), dim3(g), dim3(256), 0, s, L, L.val, mode, step, din, dout, done);
     } else {
         if (L.val32)
             hipLaunchKernelGGL((k_mg_bsr<float, 3>), dim3(g), dim3(256), 0, s, L, L.val32, mode, step, din, dout, done);
@@ -816,6 +837,9 @@ void launch_mg_restrict(const MGLevelArgs& F, const MGLevelArgs& C, const int* d
     if (F.dim == 2) {
         if (F.rank_of_lat) hipLaunchKernelGGL((k_mg_restrict<2, 5, true>), g, dim3(256), 0, s, F, C, done);
         else hipLaunchKernelGGL((k_mg_restrict<2, 5, false>), g, dim3(256), 0, s, F, C, done);
+    } else if (F.nav == 1) {
+        if (F.rank_of_lat) hipLaunchKernelGGL((k_mg_restrict<3, 1, true>), g, dim3(256), 0, s, F, C, done);
+        else hipLaunchKernelGGL((k_mg_restrict<3, 1, false>), g, dim3(256), 0, s, F, C, done);
     } else {
         if (F.rank_of_lat) hipLaunchKernelGGL((k_mg_restrict<3, 3, true>), g, dim3(256), 0, s, F, C, done);
         else hipLaunchKernelGGL((k_mg_restrict<3, 3, false>), g, dim3(256), 0, s, F, C, done);
@@ -827,6 +851,9 @@ void launch_mg_prolong(const MGLevelArgs& F, const MGLevelArgs& C, const double*
     if (F.dim == 2) {
         if (F.rank_of_lat) hipLaunchKernelGGL((k_mg_prolong<2, 5, true>), g, dim3(256), 0, s, F, C, dlast, done);
         else hipLaunchKernelGGL((k_mg_prolong<2, 5, false>), g, dim3(256), 0, s, F, C, dlast, done);
+    } else if (F.nav == 1) {
+        if (F.rank_of_lat) hipLaunchKernelGGL((k_mg_prolong<3, 1, true>), g, dim3(256), 0, s, F, C, dlast, done);
+        else hipLaunchKernelGGL((k_mg_prolong<3, 1, false>), g, dim3(256), 0, s, F, C, dlast, done);
     } else {
         if (F.rank_of_lat) hipLaunchKernelGGL((k_mg_prolong<3, 3, true>), g, dim3(256), 0, s, F, C, dlast, done);
         else hipLaunchKernelGGL((k_mg_prolong<3, 3, false>), g, dim3(256), 0, s, F, C, dlast, done);
@@ -842,6 +869,10 @@ void launch_pcg_update(const CGArgs& c, hipStream_t s) {
 }
 void launch_pcg_rz(const CGArgs& c, const double* x0, const double* dlast, hipStream_t s) {
     hipLaunchKernelGGL(k_pcg_rz, dim3(c.nvec), dim3(256), 0, s, c, x0, dlast);
+}
+void launch_mg_combine(int64_t n, const double* x0, const double* dlast, double* out, const int* done,
+                       hipStream_t s) {
+    hipLaunchKernelGGL(k_mg_combine, dim3(mg_grid(n)), dim3(256), 0, s, n, x0, dlast, out, done);
 }
 void launch_pcg_beta(const CGArgs& c, hipStream_t s) {
     hipLaunchKernelGGL(k_pcg_beta, dim3(1), dim3(256), 0, s, c);

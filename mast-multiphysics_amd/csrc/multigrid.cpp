@@ -73,7 +73,8 @@ MGHierarchy build_mg_hierarchy(const Structure& S) {
     int dim = 0, physics = 0;
     if (S.npe == 8 && S.nav == 3) { dim = 3; physics = 1; }
     else if (S.npe == 4 && S.nav == 5) { dim = 2; physics = 2; }
-    else { H.why = "multigrid is built for hex8 elasticity and the quad4 Mindlin plate"; return H; }
+    else if (S.npe == 8 && S.nav == 1) { dim = 3; physics = 3; }
+    else { H.why = "multigrid is built for hex8 elasticity, the quad4 Mindlin plate and hex8 heat"; return H; }
     const int nav = S.nav, npe = S.npe;
     MGLevelHost L0;
     if (!detect_lattice(S, dim, L0, H.why)) return H;

@@ -1,5 +1,6 @@
 // Geometric multigrid hierarchy on lattice meshes (host): hex8 elasticity (3D,
-// 3 variables per node) and the quad4 Mindlin plate (2D, 5 variables).
+// 3 variables per node), the quad4 Mindlin plate (2D, 5 variables) and hex8
+// heat conduction (3D, 1 variable).
 //
 // A mesh qualifies when its nodes form a complete lattice of axis-aligned
 // boxes ((nx+1)(ny+1)(nz+1) points; nz = 0 for the plate) and every element is

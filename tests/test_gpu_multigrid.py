@@ -78,7 +78,7 @@ def test_mg_refuses_non_lattice(oracle, mastg):
         ctx.design_eval(P.params(), solver=mastg.SOLVER_MG)
     assert e.value.code == -1
     ctx.close()
-    Ph = oracle.Problem(oracle.HEAT, 4, 4, 4, seed=1)     # heat: not a multigrid physics
+    Ph = oracle.Problem(oracle.HEAT, 4, 4, 4, seed=1)     # 125 DOFs: nothing to coarsen
     ch = ctx_from_oracle(mastg, Ph)
     ch.assemble(np.zeros(Ph.n_dofs))
     with pytest.raises(mastg.MastgError) as e:
@@ -122,4 +122,23 @@ def test_mg_plate_design_eval(oracle, mastg, n):
     # deterministic: a second design evaluation is bitwise equal
     r2 = ctx.design_eval(P.params(), solver=mastg.SOLVER_MG)
     assert r2["objective"] == r["objective"] and np.array_equal(r2["grad"], r["grad"])
+    ctx.close()
+
+
+@pytest.mark.parametrize("n", [8, 16])
+def test_mg_heat_design_eval(oracle, mastg, n):
+    """Nonlinear heat: Newton steps and the adjoint solve with BiCGStab
+    right-preconditioned by the multigrid V-cycle (scalar Galerkin levels,
+    power-iteration Chebyshev bounds, rebuilt from every Newton Jacobian) give
+    the Jacobi path's answers and need far fewer Krylov iterations."""
+    print(f"[mg heat] {n}^3", flush=True)
+    P = oracle.Problem(oracle.HEAT, n, seed=5)
+    ref = P.solve_heat(1e-12, 1e-14)
+    ctx = ctx_from_oracle(mastg, P)
+    rj = ctx.design_eval(P.params(), rtol=1e-12)
+    r = ctx.design_eval(P.params(), rtol=1e-12, solver=mastg.SOLVER_MG)
+    assert 2 <= r["newton_iters"] <= 10
+    assert abs(r["objective"] - ref["mean_T"]) <= 1e-9 * abs(ref["mean_T"])
+    assert rel_err_floor(r["grad"], ref["dJ"], floor=1e-6) <= 1e-7
+    assert r["iters"] * 3 < rj["iters"], (r["iters"], rj["iters"])
     ctx.close()
