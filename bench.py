@@ -52,6 +52,22 @@ def lattice_bytes(n_active, n_elems):
     return 2 * n_active * 8 + n_elems * 8
 
 
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    hardware-counter summary (FETCH_SIZE + WRITE_SIZE, each scaled by what the
+    counter reads for a copy kernel of known size); None when not measured."""
+    import re
+    path = os.path.join(ROOT, "profiles", "c2_lattice_traffic.md")
+    if not os.path.exists(path):
+        return None
+    pat = re.compile(r"^traffic_bytes_per_launch (\S+) (.+): (\d+)\s*$")
+    for line in open(path):
+        m = pat.match(line)
+        if m and m.group(1) == kernel and m.group(2) == workload:
+            return int(m.group(3))
+    return None
+
+
 def cpu_baseline(wl, gpu_iters, budget_s=20.0):
     """Oracle (FP64 C++ restatement of MAST's algorithm, OpenMP over all host
     cores) timed on a bounded sample and scaled to one design of the workload."""
@@ -284,6 +300,9 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch the solver kernels directly instead of replaying hipGraphs")
+    ap.add_argument("--kernel-profile", action="store_true",
+                    help="only assemble the workload and launch the lattice operator and the 256 MiB "
+                         "copy kernel 5 times each (a short program for hardware-counter runs)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the multi-GPU path is not built", file=sys.stderr)
@@ -303,6 +322,17 @@ def main():
     if a.no_graphs:
         ctx.set_option(mastg.OPT_GRAPHS, 0)
     params = w.params
+    if a.kernel_profile:
+        ctx.set_params(params)
+        ctx.assemble(want_R=False, want_J=False)
+        k_ms = ctx.bench_kernel(6, 5)
+        c_ms = ctx.bench_kernel(7, 5)
+        print(json.dumps({"workload": a.workload, "lattice_apply_ms": k_ms, "copy_256MiB_ms": c_ms,
+                          "copy_GBs": 2 * 2**28 / (c_ms * 1e-3) / 1e9,
+                          "lattice_flops": lattice_flops(ctx.n_active // 3),
+                          "lattice_bytes": lattice_bytes(ctx.n_active, w.n_elems)}), flush=True)
+        ctx.close()
+        return
     print(f"# workload {a.workload}: {w.n_elems} elems, {ctx.n_active} active DOF, nnz {ctx.nnz}, "
           f"setup {t_setup:.1f}s", file=sys.stderr, flush=True)
     solvers = {"plain": mastg.SOLVER_CG_JACOBI, "mixed": mastg.SOLVER_MIXED,
@@ -350,7 +380,8 @@ def main():
         achieved = fl / (k_ms * 1e-3) / 1e12
         roof = {"bound": "mfma", "kernel": "k_lattice_apply<0> (FP64 per-node lattice operator)",
                 "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None, "avg_launch_ms": k_ms,
+                "frac": achieved / PEAK_FP64_TFLOPS,
+                "traffic": measured_traffic("k_lattice_apply<0>", f"hex8 {nx}x{ny}x{nz}"), "avg_launch_ms": k_ms,
                 "algorithmic_flops": fl, "algorithmic_bytes": byt,
                 "hbm_view_gbs": byt / (k_ms * 1e-3) / 1e9,
                 "note": "FP64 dense peak of MI355X (78.6 TFLOP/s; VALU FP64 runs at the same rate)"}

@@ -224,8 +224,9 @@ int mastg_enable_kernel_timing(mastg_ctx* ctx, int on);
 /* Replays one kernel of the resident state n_launch times back to back on the
  * context's stream and returns the average duration from HIP events (same
  * kinds as mastg_kernel_time, plus 5 = the FP32 matrix-core element kernel,
- * hex8 only, into a buffer of its own, and 6 = the per-node lattice operator).
- * For roofline measurement. */
+ * hex8 only, into a buffer of its own, 6 = the per-node lattice operator, and
+ * 7 = a plain 256 MiB device copy, the known-traffic kernel that hardware byte
+ * counters are scaled against).  For roofline measurement. */
 int mastg_bench_kernel(mastg_ctx* ctx, int kind, int n_launch, double* avg_ms);
 /* Launch shapes: CSR SpMV lanes per row and grid size (work-groups of the
  * SpMV / CG vector kernels).  Autotuned on the first solve unless set. */

@@ -57,7 +57,7 @@ struct KTimer {   // accumulated device time of one kernel kind
 struct MGLevelDev {
     MGLevelArgs a{};
     int64_t n_elems = 0;
-    DevBuf rank_of_lat, elem_of_cell, con_lat, nodecon_lat, dinv_lat, b_lat;   // level 0
+    DevBuf rank_of_lat, elem_of_cell, con_lat, nodecon_lat, dinv_lat, b_lat, sE_cell;   // level 0
     DevBuf blk_ptr, blk_col, cptr, cidx, econn, con;   // levels >= 1
     DevBuf Kel, val, val32, dinv, r, x, res, da, db;
 };
@@ -117,6 +117,7 @@ struct mastg_ctx {
     bool use_graphs = true;
     bool tuned = false;
     bool sE_valid = false;
+    bool sEc_valid = false;     // the lattice operator's cell-ordered copy of sE
     // timing
     bool timing = false;
     KTimer kt[8];
@@ -416,6 +417,7 @@ int assemble_active(mastg_ctx* c, const double* X_active, bool want_R, bool want
 
 // ---- FP64 CSR CG + Jacobi on the resident matrix: b (active, device) -> c->vx
 int autotune(mastg_ctx* c);
+void ensure_scales(mastg_ctx* c);
 int cg_finish(mastg_ctx* c, const CGArgs& a, const double* b, hipEvent_t e0, hipEvent_t e1,
               mastg_solve_stats* st, int done);
 
@@ -432,11 +434,7 @@ int cg_solve(mastg_ctx* c, const double* b, const mastg_solve_opts* o, mastg_sol
     const int max_it = (o && o->max_iters > 0) ? o->max_iters : 100000;
     const int chunk = (o && o->check_every > 0) ? o->check_every : 32;
     if (!c->tuned) TRY(autotune(c));
-    if (matfree && !c->sE_valid) {
-        launch_elem_scale(S.n_elems, c->param.as<double>(), c->d.E, c->d.penalty, c->sE.as<double>(),
-                          c->stream);
-        c->sE_valid = true;
-    }
+    if (matfree) ensure_scales(c);
     if (op == 2 && !c->val32_valid) {
         TRY(dalloc(c, c->val32, size_t(S.nnz) * sizeof(float)));
         launch_to_f32(S.nnz, c->val.as<double>(), c->val32.as<float>(), c->stream);
@@ -516,6 +514,23 @@ int cg_loop(mastg_ctx* c, const CGArgs& a, const double* b, const mastg_solve_op
     return cg_finish(c, a, b, e0, e1, st, done);
 }
 
+// Element scales E(rho) of the matrix-free operators for the current
+// parameters; once the lattice operator is built, also its copy in
+// lattice-cell order (one independent load per cell in the operator).
+void ensure_scales(mastg_ctx* c) {
+    if (!c->sE_valid) {
+        launch_elem_scale(c->S.n_elems, c->param.as<double>(), c->d.E, c->d.penalty, c->sE.as<double>(), c->stream);
+        c->sE_valid = true;
+        c->sEc_valid = false;
+    }
+    if (!c->sEc_valid && !c->mg.empty() && !c->mg_power) {
+        const MGLevelDev& L0 = *c->mg[0];
+        launch_gather_values(L0.n_elems, L0.elem_of_cell.as<int32_t>(), c->sE.as<double>(),
+                             const_cast<double*>(L0.sE_cell.as<double>()), c->stream);
+        c->sEc_valid = true;
+    }
+}
+
 // ---- geometric multigrid preconditioned CG (lattice meshes) ----------------
 // power iteration for the plates' Chebyshev bounds: steps and safety factor
 constexpr int kMGPowerIters = 20;
@@ -572,12 +587,14 @@ int mg_build(mastg_ctx* c) {
         } else if (l == 0) {
             // the problem level runs in lattice order (x fastest): its vectors
             // are permuted from / to the caller's node-rank order at the solve
+            // (component-major: DOF v of lattice point p at v n_nodes + p)
             std::vector<uint8_t> con_lat(size_t(h.n_nodes) * 3), nodecon_lat(size_t(h.n_nodes));
             for (int64_t p = 0; p < h.n_nodes; p++) {
                 uint8_t bits = 0;
                 for (int v = 0; v < 3; v++) {
-                    con_lat[p * 3 + v] = c->S.constrained[size_t(h.rank_of_lat[p]) * 3 + v];
-                    if (con_lat[p * 3 + v]) bits |= uint8_t(1 << v);
+                    const uint8_t cv = c->S.constrained[size_t(h.rank_of_lat[p]) * 3 + v];
+                    con_lat[size_t(v * h.n_nodes + p)] = cv;
+                    if (cv) bits |= uint8_t(1 << v);
                 }
                 nodecon_lat[p] = bits;
             }
@@ -585,11 +602,13 @@ int mg_build(mastg_ctx* c) {
             TRY(upload(c, L->elem_of_cell, h.elem_of_cell));
             TRY(upload(c, L->con_lat, con_lat));
             TRY(upload(c, L->nodecon_lat, nodecon_lat));
+            TRY(dalloc(c, L->sE_cell, size_t(h.n_elems) * sizeof(double)));
             TRY(dalloc(c, L->dinv_lat, nv));
             TRY(dalloc(c, L->b_lat, nv));
             a.con = L->con_lat.as<uint8_t>();
             a.dinv = L->dinv_lat.as<double>();
             a.r = c->vr.as<double>();
+            a.planar = 1;
         } else {
             const Structure& S = h.S;
             TRY(upload(c, L->blk_ptr, S.blk_ptr));
@@ -702,9 +721,9 @@ int mg_setup(mastg_ctx* c) {
 bool lattice_args(mastg_ctx* c, CGArgs& a) {
     if (c->mg.empty() || c->mg_power) return false;   // hex8 only
     const MGLevelDev& L0 = *c->mg[0];
+    if (L0.a.n_nodes >= (int64_t(1) << 29)) return false;   // 32-bit byte offsets in the operator
     a.lat.nx = L0.a.nx; a.lat.ny = L0.a.ny; a.lat.nz = L0.a.nz;
-    a.lat.elem_of_cell = L0.elem_of_cell.as<int32_t>();
-    a.lat.sE = c->sE.as<double>();
+    a.lat.sE_cell = L0.sE_cell.as<double>();
     a.lat.Khat = c->Khat.as<double>();
     a.lat.nodecon = L0.nodecon_lat.as<uint8_t>();
     return lattice_partials(a.lat) <= int64_t(c->part0.bytes / sizeof(double));
@@ -759,10 +778,7 @@ int mg_solve(mastg_ctx* c, const double* b, const mastg_solve_opts* o, mastg_sol
     const int chunk = (o && o->check_every > 0) ? o->check_every : 4;
     TRY(mg_build(c));
     if (!c->tuned) TRY(autotune(c));   // launch shape of the CSR products
-    if (!c->mg_power && !c->sE_valid) {
-        launch_elem_scale(S.n_elems, c->param.as<double>(), c->d.E, c->d.penalty, c->sE.as<double>(), c->stream);
-        c->sE_valid = true;
-    }
+    if (!c->mg_power) ensure_scales(c);
     hipEvent_t e0, e1;
     HIPC(c, ev_create(&e0));
     HIPC(c, ev_create(&e1));
@@ -1387,6 +1403,7 @@ int mastg_set_params(mastg_ctx* c, const double* p, int where) {
     c->params_set = true;
     c->assembled = false;
     c->sE_valid = false;
+    c->sEc_valid = false;
     return MASTG_OK;
 }
 
@@ -1469,11 +1486,7 @@ int mastg_jacobian_product(mastg_ctx* c, const double* dX, double* JdX, int wher
     if (op == 3) {
         if (!lattice_args(c, a)) return fail(c, MASTG_ERR_INVALID, "no lattice operator for this mesh");
         a.S = nullptr;
-        if (!c->sE_valid) {
-            launch_elem_scale(c->S.n_elems, c->param.as<double>(), c->d.E, c->d.penalty, c->sE.as<double>(),
-                              c->stream);
-            c->sE_valid = true;
-        }
+        ensure_scales(c);
         const MGLevelDev& L0 = *c->mg[0];
         const int32_t* lat_map = L0.rank_of_lat.as<int32_t>();
         launch_mask_constrained(c->S.n_active, c->con.as<uint8_t>(), c->vb.as<double>(), c->stream);
@@ -1483,11 +1496,7 @@ int mastg_jacobian_product(mastg_ctx* c, const double* dX, double* JdX, int wher
     } else if (op == 2) {
         launch_spmv_nb32(a, c->vb.as<double>(), c->vtmp.as<double>(), 0, c->stream);
     } else if (op == 1) {
-        if (!c->sE_valid) {
-            launch_elem_scale(c->S.n_elems, c->param.as<double>(), c->d.E, c->d.penalty, c->sE.as<double>(),
-                              c->stream);
-            c->sE_valid = true;
-        }
+        ensure_scales(c);
         // element phase reads a copy zeroed on constrained DOFs; the node phase
         // applies the constrained rows (diagonal = elements at the node) to the input
         HIPC(c, hipMemcpyAsync(c->vz.p, c->vb.p, c->S.n_active * sizeof(double), hipMemcpyDeviceToDevice,
@@ -1656,6 +1665,9 @@ int mastg_bench_kernel(mastg_ctx* c, int kind, int n_launch, double* avg_ms) {
     if (!c || !avg_ms || n_launch <= 0) return MASTG_ERR_INVALID;
     if (!c->assembled) return fail(c, MASTG_ERR_INVALID, "assemble first");
     CGArgs a = cg_args(c);
+    // kind 7: a 256 MiB device-to-device copy (read 256 MiB, write 256 MiB)
+    constexpr int64_t kCopyDoubles = int64_t(1) << 25;
+    DevBuf copy_src, copy_dst;
     ScatterArgs sa{c->S.n_nodes, c->S.nav, c->blk_ptr.as<int64_t>(), c->cptr.as<int32_t>(),
                    c->cidx.as<int32_t>(), c->Kel.as<double>(), c->val.as<double>()};
     // the CG scalars must not read "done" for the replayed SpMV
@@ -1669,29 +1681,27 @@ int mastg_bench_kernel(mastg_ctx* c, int kind, int n_launch, double* avg_ms) {
             case 1: launch_scatter(sa, c->stream); break;
             case 2: launch_spmv_csr(a, a.p, a.q, 1, c->stream); break;
             case 3:
-                if (!c->sE_valid) {
-                    launch_elem_scale(c->S.n_elems, c->param.as<double>(), c->d.E, c->d.penalty,
-                                      c->sE.as<double>(), c->stream);
-                    c->sE_valid = true;
-                }
+                ensure_scales(c);
                 launch_mf_apply(a, a.p, a.p, a.q, 1, c->stream);
                 break;
             case 4: launch_sens(c, a.x, a.x, c->vg.as<double>()); break;
             case 5: launch_hex8_K_f32(hex8_args(c, nullptr, false), c->Kel32.as<float>(), c->stream); break;
             case 6: launch_lattice_apply(a, c->mg[0]->a.da, c->mg[0]->a.db, c->stream); break;
+            case 7: launch_copy_f64(kCopyDoubles, copy_src.as<double>(), copy_dst.as<double>(), c->stream); break;
             default: break;
         }
     };
-    if (kind < 0 || kind > 6) return fail(c, MASTG_ERR_INVALID, "unknown kernel kind");
+    if (kind < 0 || kind > 7) return fail(c, MASTG_ERR_INVALID, "unknown kernel kind");
+    if (kind == 7) {
+        TRY(dalloc(c, copy_src, kCopyDoubles * sizeof(double)));
+        TRY(dalloc(c, copy_dst, kCopyDoubles * sizeof(double)));
+        HIPC(c, hipMemsetAsync(copy_src.p, 0, kCopyDoubles * sizeof(double), c->stream));
+    }
     if (kind == 6) {
         TRY(mg_build(c));
         a.S = nullptr;
         if (!lattice_args(c, a)) return fail(c, MASTG_ERR_INVALID, "no lattice operator for this mesh");
-        if (!c->sE_valid) {
-            launch_elem_scale(c->S.n_elems, c->param.as<double>(), c->d.E, c->d.penalty, c->sE.as<double>(),
-                              c->stream);
-            c->sE_valid = true;
-        }
+        ensure_scales(c);
     }
     if (kind == 5) {
         if (c->d.physics != MASTG_HEX8_ELASTIC) return fail(c, MASTG_ERR_INVALID, "FP32 element kernel is hex8 only");

@@ -4,6 +4,16 @@
 
 namespace mastg {
 
+// XCD-aware work-group order.  The dispatcher deals work-groups to the 8 XCDs
+// (each with its own L2) round-robin: block b runs on XCD b % 8.  Renumbering
+// block b to this index gives XCD x the contiguous run [x q + min(x, r),
+// (x + 1) q + min(x + 1, r)) of an nb-block grid (nb = 8 q + r), so stencil
+// neighbours gathered by one work-group were mostly fetched into the same L2
+// by the work-groups before it.  A bijection of [0, nb) for every nb >= 1.
+__host__ __device__ constexpr int64_t xcd_block(int64_t b, int64_t nb) {
+    return (b & 7) * (nb >> 3) + ((b & 7) < (nb & 7) ? (b & 7) : (nb & 7)) + (b >> 3);
+}
+
 // block-level FP64 sum into partials[blockIdx.x] (blockDim 256)
 __device__ __forceinline__ void block_sum_store(double v, double* partials, int slot) {
     __shared__ double red[4];

@@ -83,9 +83,9 @@ struct MFArgs {
 // s_e Khat[a_n][b] x_b, constrained rows as diag = cells at the node (inputs
 // must be zero on constrained DOFs)
 struct LatticeOpArgs {
-    int nx, ny, nz;                    // cells per dimension
-    const int32_t* elem_of_cell;       // element of lattice cell (null: no lattice operator)
-    const double* sE;                  // [n_elems] element scales
+    int nx, ny, nz;                    // cells per dimension ((nx+1)(ny+1)(nz+1) < 2^29)
+    const double* sE_cell;             // [cells] element scales in lattice-cell order
+                                       // (cell i + nx (j + ny k))
     const double* Khat;                // [64*9] the (single) class matrix at E = 1
     const uint8_t* nodecon;            // [lattice points] bit v = variable v constrained
 };
@@ -140,6 +140,8 @@ void launch_bicg_part(const BiCGArgs& a, int part, hipStream_t s);
 void launch_gather_values(int64_t n, const int32_t* idx, const double* src, double* dst, hipStream_t s);
 
 void launch_to_f32(int64_t n, const double* src, float* dst, hipStream_t s);
+// dst[0:n) = src[0:n) (n even, 16-byte aligned buffers)
+void launch_copy_f64(int64_t n, const double* src, double* dst, hipStream_t s);
 void launch_spmv_nb32(const CGArgs& c, const double* x, double* y, int want_dot, hipStream_t s);
 // residual replacement: r = rt (a true FP64 residual), z = dinv r, fresh r.z /
 // r.r, p = z when restart; done = 1 if ||r|| meets the tolerance, else 0 (2
@@ -185,6 +187,9 @@ struct MGLevelArgs {
     int nav, dim;                      // variables per node (3 hex8, 5 plate), lattice dimension
     int nx, ny, nz;                    // lattice cells per dimension (nz = 0 in 2D)
     const int32_t* rank_of_lat;        // vector index of lattice point (null: lattice order)
+    int planar;                        // 1: vectors are component-major in lattice order
+                                       // (DOF v of point p at v n_nodes + p; the hex8
+                                       // problem level), 0: node-major (p nav + v)
     const uint8_t* con;                // [nav n_nodes] constrained DOFs
     const int64_t* blk_ptr;            // operator of levels >= 1: node-block CSR, FP64
     const int32_t* blk_col;

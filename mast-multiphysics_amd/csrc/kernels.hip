@@ -1112,6 +1112,14 @@ __global__ void k_to_f32(int64_t n, const double* __restrict__ src, float* __res
         dst[i] = float(src[i]);
 }
 
+// Plain device copy (16-byte lanes): the reference kernel of known traffic that
+// hardware byte counters are scaled against.
+__global__ __launch_bounds__(256) void k_copy_f64(int64_t n2, const double2* __restrict__ src,
+                                                  double2* __restrict__ dst) {
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n2; i += int64_t(gridDim.x) * 256)
+        dst[i] = src[i];
+}
+
 // ------------------------------------------------- BiCGStab (FP64 CSR) ----
 // y = A x with SW lanes per row; optional partials sum(y.w1) -> part0 and
 // sum(y.y) -> part1 (per block).  Stops immediately once the solver is done.
@@ -1336,6 +1344,11 @@ void launch_spmv_nb32(const CGArgs& c, const double* x, double* y, int want_dot,
 void launch_to_f32(int64_t n, const double* src, float* dst, hipStream_t s) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_to_f32, dim3(std::min<int64_t>(blocks_for(n, 256), 8192)), dim3(256), 0, s, n, src, dst);
+}
+void launch_copy_f64(int64_t n, const double* src, double* dst, hipStream_t s) {
+    if (n <= 0 || n % 2) return;
+    hipLaunchKernelGGL(k_copy_f64, dim3(std::min<int64_t>(blocks_for(n / 2, 256), 8192)), dim3(256), 0, s, n / 2,
+                       reinterpret_cast<const double2*>(src), reinterpret_cast<double2*>(dst));
 }
 void launch_cg_replace(const CGArgs& c, const double* rt, int restart, hipStream_t s) {
     hipLaunchKernelGGL(k_cg_replace, dim3(c.nvec), dim3(256), 0, s, c, rt, restart);

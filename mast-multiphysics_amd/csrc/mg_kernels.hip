@@ -231,8 +231,9 @@ __global__ __launch_bounds__(256) void k_mg_bsr(MGLevelArgs L, const T* __restri
 }
 
 // ---- level 0 on a lattice: per-node matrix-free operator -------------------
-// Thread per lattice point n, vectors in lattice order (x fastest), so the
-// neighbour gathers of a wave are contiguous; for each existing cell at n
+// Thread per lattice point n, vectors component-major in lattice order (x
+// fastest), so each neighbour gather of a wave is one contiguous 512-byte run;
+// work-groups in XCD-aware order; for each existing cell at n
 // (local corner a) y_n += s_e Khat[a][b] x_b.  Khat indices are uniform
 // across the wave, so the class matrix is read with scalar-cache loads.  Each
 // y_n is a fixed-order sum: results are deterministic.
@@ -254,10 +255,12 @@ __global__ __launch_bounds__(256) void k_lattice_apply(CGArgs C, const double* _
     const LatticeOpArgs& O = C.lat;
     const int64_t nxp = O.nx + 1, nyp = O.ny + 1;
     const int64_t n_lat = nxp * nyp * (O.nz + 1);
-    const int64_t p = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    const int64_t p = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
     const bool valid = p < n_lat;
     const int64_t pv = valid ? p : 0;
-    const int i = int(pv % nxp), j = int((pv / nxp) % nyp), k = int(pv / (nxp * nyp));
+    // 32-bit index arithmetic (the lattice has fewer than 2^31 points)
+    const uint32_t pu = uint32_t(pv), nxu = uint32_t(nxp), pj = pu / nxu;
+    const int i = int(pu - pj * nxu), j = int(pj % uint32_t(nyp)), k = int(pj / uint32_t(nyp));
     const double* __restrict__ in = KIND == 0 ? xcg : (mode == MG_RESID_INIT ? L.x : din);
     // cells at n (c bit d: the cell extends towards +d from n - 1) and their
     // scales; a missing cell gets the scale 0 (its index is clamped into the
@@ -269,9 +272,8 @@ __global__ __launch_bounds__(256) void k_lattice_apply(CGArgs C, const double* _
         const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
         const int ci = i - 1 + cx, cj = j - 1 + cy, ck = k - 1 + cz;
         const bool has = valid && ci >= 0 && ci < O.nx && cj >= 0 && cj < O.ny && ck >= 0 && ck < O.nz;
-        const int64_t cell = min(max(ci, 0), O.nx - 1) +
-                             int64_t(O.nx) * (min(max(cj, 0), O.ny - 1) + int64_t(O.ny) * min(max(ck, 0), O.nz - 1));
-        const double sv = O.sE[O.elem_of_cell[cell]];
+        const int cell = min(max(ci, 0), O.nx - 1) + O.nx * (min(max(cj, 0), O.ny - 1) + O.ny * min(max(ck, 0), O.nz - 1));
+        const double sv = O.sE_cell[cell];
         sc[c] = has ? sv : 0.0;
         cnt += has ? 1 : 0;
     }
@@ -281,19 +283,26 @@ __global__ __launch_bounds__(256) void k_lattice_apply(CGArgs C, const double* _
     // is carried: fewer registers, more waves in flight to hide the gathers).
     // Neighbours outside the lattice are clamped into it; every cell holding
     // one is missing and scaled by 0.
+    // Gathers address each component array from a uniform base with a 32-bit
+    // byte offset (fewer than 2^29 lattice points), and the products are
+    // explicit FMA chains: 9 FMAs per 3x3 block, 3 per (cell, layer) scaling.
+    const char* __restrict__ b0 = reinterpret_cast<const char*>(in);
+    const char* __restrict__ b1 = reinterpret_cast<const char*>(in + n_lat);
+    const char* __restrict__ b2 = reinterpret_cast<const char*>(in + 2 * n_lat);
+    const uint32_t sx = uint32_t(nxp), sz = uint32_t(nxp * nyp);
     double y0 = 0.0, y1 = 0.0, y2 = 0.0;
 #pragma unroll 1
     for (int lz = 0; lz < 3; lz++) {
-        const int mk = min(max(k + lz - 1, 0), O.nz);
+        const uint32_t mk = uint32_t(min(max(k + lz - 1, 0), O.nz));
         double xv[9][3];
 #pragma unroll
         for (int q = 0; q < 9; q++) {
-            const int mi = min(max(i + (q % 3) - 1, 0), O.nx);
-            const int mj = min(max(j + (q / 3) - 1, 0), O.ny);
-            const int64_t rm = (mi + nxp * (mj + nyp * int64_t(mk))) * 3;
-            xv[q][0] = in[rm];
-            xv[q][1] = in[rm + 1];
-            xv[q][2] = in[rm + 2];
+            const uint32_t mi = uint32_t(min(max(i + (q % 3) - 1, 0), O.nx));
+            const uint32_t mj = uint32_t(min(max(j + (q / 3) - 1, 0), O.ny));
+            const uint32_t off = (mi + sx * mj + sz * mk) * 8u;
+            xv[q][0] = *reinterpret_cast<const double*>(b0 + off);
+            xv[q][1] = *reinterpret_cast<const double*>(b1 + off);
+            xv[q][2] = *reinterpret_cast<const double*>(b2 + off);
         }
 #pragma unroll
         for (int c = 0; c < 8; c++) {
@@ -308,50 +317,52 @@ __global__ __launch_bounds__(256) void k_lattice_apply(CGArgs C, const double* _
                 if (bx < 0 || bx > 1 || by < 0 || by > 1) continue;   // compile time
                 const int b = bz * 4 + lat_corner(bx, by, 0);
                 const double* K = O.Khat + (a * 8 + b) * 9;
-                t0 += K[0] * xv[q][0] + K[1] * xv[q][1] + K[2] * xv[q][2];
-                t1 += K[3] * xv[q][0] + K[4] * xv[q][1] + K[5] * xv[q][2];
-                t2 += K[6] * xv[q][0] + K[7] * xv[q][1] + K[8] * xv[q][2];
+                t0 = fma(K[0], xv[q][0], t0); t0 = fma(K[1], xv[q][1], t0); t0 = fma(K[2], xv[q][2], t0);
+                t1 = fma(K[3], xv[q][0], t1); t1 = fma(K[4], xv[q][1], t1); t1 = fma(K[5], xv[q][2], t1);
+                t2 = fma(K[6], xv[q][0], t2); t2 = fma(K[7], xv[q][1], t2); t2 = fma(K[8], xv[q][2], t2);
             }
-            y0 += sc[c] * t0;
-            y1 += sc[c] * t1;
-            y2 += sc[c] * t2;
+            y0 = fma(sc[c], t0, y0);
+            y1 = fma(sc[c], t1, y1);
+            y2 = fma(sc[c], t2, y2);
         }
     }
     const int64_t r = pv;
     double dsum = 0.0;
     if (valid) {
         const int m = O.nodecon[r];
-        if (m & 1) y0 = cnt * in[r * 3 + 0];
-        if (m & 2) y1 = cnt * in[r * 3 + 1];
-        if (m & 4) y2 = cnt * in[r * 3 + 2];
+        const double x0 = in[r], x1 = in[r + n_lat], x2 = in[r + 2 * n_lat];
+        if (m & 1) y0 = cnt * x0;
+        if (m & 2) y1 = cnt * x1;
+        if (m & 4) y2 = cnt * x2;
         if (KIND == 0) {
-            ycg[r * 3 + 0] = y0;
-            ycg[r * 3 + 1] = y1;
-            ycg[r * 3 + 2] = y2;
-            dsum = y0 * in[r * 3 + 0] + y1 * in[r * 3 + 1] + y2 * in[r * 3 + 2];
+            ycg[r] = y0;
+            ycg[r + n_lat] = y1;
+            ycg[r + 2 * n_lat] = y2;
+            dsum = y0 * x0 + y1 * x1 + y2 * x2;
         } else {
             const double ith = L.cheb[0];
             const double c1 = step > 0 ? L.cheb[2 * step] : 0.0, c2 = step > 0 ? L.cheb[2 * step + 1] : 0.0;
-            mg_epilogue(L, mode, r * 3 + 0, y0, din, dout, c1, c2, ith);
-            mg_epilogue(L, mode, r * 3 + 1, y1, din, dout, c1, c2, ith);
-            mg_epilogue(L, mode, r * 3 + 2, y2, din, dout, c1, c2, ith);
+            mg_epilogue(L, mode, r, y0, din, dout, c1, c2, ith);
+            mg_epilogue(L, mode, r + n_lat, y1, din, dout, c1, c2, ith);
+            mg_epilogue(L, mode, r + 2 * n_lat, y2, din, dout, c1, c2, ith);
         }
     }
     if (KIND == 0) block_sum_store(dsum, C.part0, blockIdx.x);
 }
 
-// node-vector permutation between rank order and lattice order (3 values per
-// node): gather dst[p] = src[map[p]], or scatter dst[map[p]] = src[p]
+// node-vector permutation between rank order (node-major, 3 values per node)
+// and the component-major lattice order: gather dst[v N + p] = src[3 map[p] + v],
+// or scatter src[v N + p] -> dst[3 map[p] + v]
 __global__ __launch_bounds__(256) void k_perm_nodes(int64_t n_nodes, const int32_t* __restrict__ map,
                                                     const double* __restrict__ src, double* __restrict__ dst,
                                                     int scatter) {
     const int64_t p = int64_t(blockIdx.x) * 256 + threadIdx.x;
     if (p >= n_nodes) return;
-    const int64_t m = int64_t(map[p]) * 3, q = p * 3;
+    const int64_t m = int64_t(map[p]) * 3;
     if (scatter) {
-        dst[m] = src[q]; dst[m + 1] = src[q + 1]; dst[m + 2] = src[q + 2];
+        dst[m] = src[p]; dst[m + 1] = src[n_nodes + p]; dst[m + 2] = src[2 * n_nodes + p];
     } else {
-        dst[q] = src[m]; dst[q + 1] = src[m + 1]; dst[q + 2] = src[m + 2];
+        dst[p] = src[m]; dst[n_nodes + p] = src[m + 1]; dst[2 * n_nodes + p] = src[m + 2];
     }
 }
 
@@ -500,9 +511,18 @@ __global__ __launch_bounds__(1024) void k_mg_invert_lds(int n, double* __restric
 // ---- transfers ---------------------------------------------------------------
 // r_C = sum over the 3^DIM fine points around 2C of w * res_f (w = prod of 1 or
 // 1/2), constrained fine DOFs weighted 0, constrained coarse DOFs zero (P^T).
-// FR: the fine vectors are indexed through rank_of_lat (the plate's problem
-// level); coarse levels are in lattice order.
-template <int DIM, int NAV, bool FR>
+// FL: layout of the fine vectors (FL_LAT lattice order, node-major; FL_RANK
+// indexed through rank_of_lat, the plate and heat problem levels; FL_PLANAR
+// lattice order, component-major, the hex8 problem level); coarse levels are in
+// lattice order, node-major.
+enum { FL_LAT = 0, FL_RANK = 1, FL_PLANAR = 2 };
+template <int FL, int NAV>
+__device__ __forceinline__ int64_t mg_fine_index(const MGLevelArgs& F, int64_t p, int v) {
+    if (FL == FL_PLANAR) return int64_t(v) * F.n_nodes + p;
+    return (FL == FL_RANK ? int64_t(F.rank_of_lat[p]) : p) * NAV + v;
+}
+
+template <int DIM, int NAV, int FL>
 __global__ __launch_bounds__(256) void k_mg_restrict(MGLevelArgs F, MGLevelArgs Cl, const int* __restrict__ done) {
     if (done && *done) return;
     const int64_t p = int64_t(blockIdx.x) * 256 + threadIdx.x;
@@ -523,9 +543,11 @@ __global__ __launch_bounds__(256) void k_mg_restrict(MGLevelArgs F, MGLevelArgs 
         // out-of-lattice points are clamped (valid address) and weighted 0
         const double w = in_lat ? (ok ? 0.5 : 1.0) * (oj ? 0.5 : 1.0) * (oi ? 0.5 : 1.0) : 0.0;
         const int64_t fp = (in_lat ? fi : 0) + fx * ((in_lat ? fj : 0) + fy * (in_lat ? fk : 0));
-        const int64_t rf = (FR ? int64_t(F.rank_of_lat[fp]) : fp) * NAV;
 #pragma unroll
-        for (int v = 0; v < NAV; v++) sv[v] += (w * (1.0 - double(F.con[rf + v]))) * F.res[rf + v];
+        for (int v = 0; v < NAV; v++) {
+            const int64_t rf = mg_fine_index<FL, NAV>(F, fp, v);
+            sv[v] += (w * (1.0 - double(F.con[rf]))) * F.res[rf];
+        }
     }
     const int64_t rc = p * NAV;
 #pragma unroll
@@ -533,7 +555,7 @@ __global__ __launch_bounds__(256) void k_mg_restrict(MGLevelArgs F, MGLevelArgs 
 }
 
 // x_f += P (x_C + dlast_C) at unconstrained fine DOFs (tri/bilinear interpolation)
-template <int DIM, int NAV, bool FR>
+template <int DIM, int NAV, int FL>
 __global__ __launch_bounds__(256) void k_mg_prolong(MGLevelArgs F, MGLevelArgs Cl, const double* __restrict__ dlast,
                                                     const int* __restrict__ done) {
     if (done && *done) return;
@@ -559,10 +581,11 @@ __global__ __launch_bounds__(256) void k_mg_prolong(MGLevelArgs F, MGLevelArgs C
 #pragma unroll
         for (int v = 0; v < NAV; v++) sv[v] += w * (dlast ? Cl.x[rc + v] + dlast[rc + v] : Cl.x[rc + v]);
     }
-    const int64_t rf = (FR ? int64_t(F.rank_of_lat[p]) : p) * NAV;
 #pragma unroll
-    for (int v = 0; v < NAV; v++)
-        if (!F.con[rf + v]) F.x[rf + v] += sv[v];
+    for (int v = 0; v < NAV; v++) {
+        const int64_t rf = mg_fine_index<FL, NAV>(F, p, v);
+        if (!F.con[rf]) F.x[rf] += sv[v];
+    }
 }
 
 // ---- coarsest level: dense inverse -------------------------------------------
@@ -834,29 +857,32 @@ void launch_mg_fine(const CGArgs& c, const MGLevelArgs& L, int mode, int step, c
 }
 void launch_mg_restrict(const MGLevelArgs& F, const MGLevelArgs& C, const int* done, hipStream_t s) {
     const dim3 g(mg_blocks(C.n_nodes, 256));
+    const int fl = F.planar ? FL_PLANAR : F.rank_of_lat ? FL_RANK : FL_LAT;
     if (F.dim == 2) {
-        if (F.rank_of_lat) hipLaunchKernelGGL((k_mg_restrict<2, 5, true>), g, dim3(256), 0, s, F, C, done);
-        else hipLaunchKernelGGL((k_mg_restrict<2, 5, false>), g, dim3(256), 0, s, F, C, done);
+        if (fl == FL_RANK) hipLaunchKernelGGL((k_mg_restrict<2, 5, FL_RANK>), g, dim3(256), 0, s, F, C, done);
+        else hipLaunchKernelGGL((k_mg_restrict<2, 5, FL_LAT>), g, dim3(256), 0, s, F, C, done);
     } else if (F.nav == 1) {
-        if (F.rank_of_lat) hipLaunchKernelGGL((k_mg_restrict<3, 1, true>), g, dim3(256), 0, s, F, C, done);
-        else hipLaunchKernelGGL((k_mg_restrict<3, 1, false>), g, dim3(256), 0, s, F, C, done);
+        if (fl == FL_RANK) hipLaunchKernelGGL((k_mg_restrict<3, 1, FL_RANK>), g, dim3(256), 0, s, F, C, done);
+        else hipLaunchKernelGGL((k_mg_restrict<3, 1, FL_LAT>), g, dim3(256), 0, s, F, C, done);
     } else {
-        if (F.rank_of_lat) hipLaunchKernelGGL((k_mg_restrict<3, 3, true>), g, dim3(256), 0, s, F, C, done);
-        else hipLaunchKernelGGL((k_mg_restrict<3, 3, false>), g, dim3(256), 0, s, F, C, done);
+        if (fl == FL_PLANAR) hipLaunchKernelGGL((k_mg_restrict<3, 3, FL_PLANAR>), g, dim3(256), 0, s, F, C, done);
+        else hipLaunchKernelGGL((k_mg_restrict<3, 3, FL_LAT>), g, dim3(256), 0, s, F, C, done);
     }
 }
 void launch_mg_prolong(const MGLevelArgs& F, const MGLevelArgs& C, const double* dlast, const int* done,
                        hipStream_t s) {
     const dim3 g(mg_blocks(F.n_nodes, 256));
+    const int fl = F.planar ? FL_PLANAR : F.rank_of_lat ? FL_RANK : FL_LAT;
     if (F.dim == 2) {
-        if (F.rank_of_lat) hipLaunchKernelGGL((k_mg_prolong<2, 5, true>), g, dim3(256), 0, s, F, C, dlast, done);
-        else hipLaunchKernelGGL((k_mg_prolong<2, 5, false>), g, dim3(256), 0, s, F, C, dlast, done);
+        if (fl == FL_RANK) hipLaunchKernelGGL((k_mg_prolong<2, 5, FL_RANK>), g, dim3(256), 0, s, F, C, dlast, done);
+        else hipLaunchKernelGGL((k_mg_prolong<2, 5, FL_LAT>), g, dim3(256), 0, s, F, C, dlast, done);
     } else if (F.nav == 1) {
-        if (F.rank_of_lat) hipLaunchKernelGGL((k_mg_prolong<3, 1, true>), g, dim3(256), 0, s, F, C, dlast, done);
-        else hipLaunchKernelGGL((k_mg_prolong<3, 1, false>), g, dim3(256), 0, s, F, C, dlast, done);
+        if (fl == FL_RANK) hipLaunchKernelGGL((k_mg_prolong<3, 1, FL_RANK>), g, dim3(256), 0, s, F, C, dlast, done);
+        else hipLaunchKernelGGL((k_mg_prolong<3, 1, FL_LAT>), g, dim3(256), 0, s, F, C, dlast, done);
     } else {
-        if (F.rank_of_lat) hipLaunchKernelGGL((k_mg_prolong<3, 3, true>), g, dim3(256), 0, s, F, C, dlast, done);
-        else hipLaunchKernelGGL((k_mg_prolong<3, 3, false>), g, dim3(256), 0, s, F, C, dlast, done);
+        if (fl == FL_PLANAR)
+            hipLaunchKernelGGL((k_mg_prolong<3, 3, FL_PLANAR>), g, dim3(256), 0, s, F, C, dlast, done);
+        else hipLaunchKernelGGL((k_mg_prolong<3, 3, FL_LAT>), g, dim3(256), 0, s, F, C, dlast, done);
     }
 }
 void launch_mg_coarse_solve(int n, const double* Ainv, const double* r, double* x, const int* done,
