@@ -18,10 +18,22 @@ void spmv(const CSR& A, const double* x, double* y, int n_threads) {
     }
 }
 
+// fixed 4096-element chunks summed in chunk order: the result does not depend
+// on the thread count or on the order threads finish (an OpenMP reduction
+// clause combines per-thread sums in completion order)
 static double dot(const double* a, const double* b, int64_t n, int t) {
+    constexpr int64_t kChunk = 4096;
+    const int64_t nc = (n + kChunk - 1) / kChunk;
+    std::vector<double> part(size_t(std::max<int64_t>(nc, 1)), 0.0);
+#pragma omp parallel for schedule(static) num_threads(nthr(t))
+    for (int64_t c = 0; c < nc; c++) {
+        double s = 0;
+        const int64_t e = std::min(n, (c + 1) * kChunk);
+        for (int64_t i = c * kChunk; i < e; i++) s += a[i] * b[i];
+        part[size_t(c)] = s;
+    }
     double s = 0;
-#pragma omp parallel for reduction(+ : s) schedule(static) num_threads(nthr(t))
-    for (int64_t i = 0; i < n; i++) s += a[i] * b[i];
+    for (int64_t c = 0; c < nc; c++) s += part[size_t(c)];
     return s;
 }
 
