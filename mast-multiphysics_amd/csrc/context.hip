@@ -103,7 +103,7 @@ struct mastg_ctx {
     bool mg_tried = false;
     std::string mg_why;
     bool mg_valid = false;     // coarse operators match the last assembly
-    int mg_k = 3;              // Chebyshev steps per smoothing
+    int mg_k0 = 3, mg_kc = 3;  // Chebyshev steps per smoothing: problem level, coarse levels
     int64_t mg_n_coarse = 0;   // DOFs of the dense coarsest level
     DevBuf mg_cheb, mg_lam, mg_part, mg_dense, mg_scal;
     bool mg_power = false;     // Chebyshev bounds from power iteration (plates)
@@ -705,7 +705,7 @@ int mg_setup(mastg_ctx* c) {
                             c->mg_scal.as<double>(), c->mg_lam.as<double>() + l + 1, s);
         launch_to_f32(int64_t(C.val32.bytes / sizeof(float)), C.a.val, C.val32.as<float>(), s);
     }
-    launch_mg_cheb(c->mg_lam.as<double>(), c->mg_cheb.as<double>(), nl, c->mg_k, 30.0, s);
+    launch_mg_cheb(c->mg_lam.as<double>(), c->mg_cheb.as<double>(), nl, std::max(c->mg_k0, c->mg_kc), 30.0, s);
     const MGLevelDev& Lc = *c->mg[nl - 1];
     const int64_t n = c->mg_n_coarse;
     launch_fill(n * n, 0.0, c->mg_dense.as<double>(), s);
@@ -734,7 +734,6 @@ bool lattice_args(mastg_ctx* c, CGArgs& a) {
 const double* mg_vcycle(mastg_ctx* c, const CGArgs& a, const int* done, hipStream_t s,
                         const double* r0 = nullptr) {
     const int nl = int(c->mg.size());
-    const int k = c->mg_k;
     MGLevelArgs L0 = c->mg[0]->a;
     if (r0) L0.r = const_cast<double*>(r0);
     auto lv = [&](int l) -> const MGLevelArgs& { return l == 0 ? L0 : c->mg[l]->a; };
@@ -743,8 +742,11 @@ const double* mg_vcycle(mastg_ctx* c, const CGArgs& a, const int* done, hipStrea
         if (l == 0 && !c->mg_power) launch_mg_fine(a, L0, mode, step, din, dout, done, s);
         else launch_mg_bsr(lv(l), mode, step, din, dout, done, s);
     };
+    auto steps = [&](int l) { return l == 0 ? c->mg_k0 : c->mg_kc; };
     for (int l = 0; l + 1 < nl; l++) {
-        launch_mg_pre_init(lv(l), done, s);
+        const int k = steps(l);
+        // coarse levels start their pre-smoothing inside the restriction
+        if (l == 0) launch_mg_pre_init(lv(l), done, s);
         for (int i = 1; i < k; i++) apply(l, MG_STEP, i, dbuf(l, i - 1), dbuf(l, i));
         apply(l, MG_FINAL, 0, dbuf(l, k - 1), nullptr);
         launch_mg_restrict(lv(l), lv(l + 1), done, s);
@@ -753,6 +755,7 @@ const double* mg_vcycle(mastg_ctx* c, const CGArgs& a, const int* done, hipStrea
     launch_mg_coarse_solve(int(c->mg_n_coarse), c->mg_dense.as<double>(), Lc.r, Lc.x, done, s);
     const double* dlast = nullptr;
     for (int l = nl - 2; l >= 0; l--) {
+        const int k = steps(l);
         launch_mg_prolong(lv(l), lv(l + 1), dlast, done, s);
         apply(l, MG_RESID_INIT, 0, nullptr, dbuf(l, 0));
         for (int i = 1; i < k; i++) apply(l, MG_STEP, i, dbuf(l, i - 1), dbuf(l, i));
@@ -1165,6 +1168,10 @@ int mastg_ctx_create(const mastg_problem_desc* desc, mastg_ctx** out) {
         c->d.constrained_dofs = nullptr; c->d.side_elem = nullptr; c->d.side_index = nullptr;
         c->d.side_id = nullptr; c->d.point_loads = nullptr;
         if (c->d.penalty == 0) c->d.penalty = 3.0;
+        // smoothing steps measured fastest per design (128x64x64 cantilever,
+        // 512^2 plate, 96^3 heat): the coarse levels are launch-latency bound
+        c->mg_k0 = c->d.physics == MASTG_HEX8_HEAT ? 3 : 4;
+        c->mg_kc = 2;
         std::string m = build_structure(desc->physics, desc->n_nodes, desc->n_elems, desc->xyz, desc->conn,
                                         desc->n_vars, desc->node_dof_base, desc->n_constrained,
                                         desc->constrained_dofs, c->S);
@@ -1638,6 +1645,16 @@ int mastg_set_option(mastg_ctx* c, int option, int value) {
     if (!c) return MASTG_ERR_INVALID;
     switch (option) {
         case MASTG_OPT_GRAPHS: c->use_graphs = value != 0; return MASTG_OK;
+        case MASTG_OPT_MG_STEPS_FINE:
+        case MASTG_OPT_MG_STEPS_COARSE:
+            if (value < 1 || value > kMGMaxCheb) return fail(c, MASTG_ERR_INVALID, "smoothing steps out of range");
+            (option == MASTG_OPT_MG_STEPS_FINE ? c->mg_k0 : c->mg_kc) = value;
+            // the captured V-cycles and the Chebyshev coefficients change
+            if (c->mg_exec) (void)hipGraphExecDestroy(c->mg_exec);
+            if (c->mg_bicg_exec) (void)hipGraphExecDestroy(c->mg_bicg_exec);
+            c->mg_exec = c->mg_bicg_exec = nullptr;
+            c->mg_valid = false;
+            return MASTG_OK;
         default: return fail(c, MASTG_ERR_INVALID, "unknown option");
     }
 }

@@ -510,7 +510,8 @@ __global__ __launch_bounds__(1024) void k_mg_invert_lds(int n, double* __restric
 
 // ---- transfers ---------------------------------------------------------------
 // r_C = sum over the 3^DIM fine points around 2C of w * res_f (w = prod of 1 or
-// 1/2), constrained fine DOFs weighted 0, constrained coarse DOFs zero (P^T).
+// 1/2), constrained fine DOFs weighted 0, constrained coarse DOFs zero (P^T),
+// fused with the start of the coarse level's pre-smoothing.
 // FL: layout of the fine vectors (FL_LAT lattice order, node-major; FL_RANK
 // indexed through rank_of_lat, the plate and heat problem levels; FL_PLANAR
 // lattice order, component-major, the hex8 problem level); coarse levels are in
@@ -549,9 +550,18 @@ __global__ __launch_bounds__(256) void k_mg_restrict(MGLevelArgs F, MGLevelArgs 
             sv[v] += (w * (1.0 - double(F.con[rf]))) * F.res[rf];
         }
     }
+    // and the coarse level's pre-smoothing start (x = 0, res = r, first
+    // Chebyshev direction D^-1 r / theta)
     const int64_t rc = p * NAV;
+    const double ith = Cl.cheb[0];
 #pragma unroll
-    for (int v = 0; v < NAV; v++) Cl.r[rc + v] = Cl.con[rc + v] ? 0.0 : sv[v];
+    for (int v = 0; v < NAV; v++) {
+        const double r = Cl.con[rc + v] ? 0.0 : sv[v];
+        Cl.r[rc + v] = r;
+        Cl.x[rc + v] = 0.0;
+        Cl.res[rc + v] = r;
+        Cl.da[rc + v] = Cl.dinv[rc + v] * r * ith;
+    }
 }
 
 // x_f += P (x_C + dlast_C) at unconstrained fine DOFs (tri/bilinear interpolation)

@@ -17,6 +17,8 @@ ELEM_FP64, ELEM_FP32 = 0, 1
 SOLVER_CG_JACOBI, SOLVER_MIXED, SOLVER_BICGSTAB, SOLVER_MATFREE, SOLVER_MG = 0, 1, 2, 3, 4
 OUTPUT_COMPLIANCE, OUTPUT_MEAN_T = 1, 2
 OPT_GRAPHS = 1
+OPT_MG_STEPS_FINE = 2
+OPT_MG_STEPS_COARSE = 3
 ERRORS = {0: "ok", -1: "invalid", -2: "out of memory", -3: "not converged", -4: "nan",
           -5: "device error", -6: "admission", -7: "busy"}
 

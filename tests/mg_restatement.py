@@ -108,16 +108,18 @@ def chebyshev(A, dinv, r, x, k, lam, ratio=30.0):
 
 
 def vcycle(levels, l, r, k):
+    """k = (steps on the problem level, steps on the coarse levels)."""
     L = levels[l]
     if "inv" in L:
         return L["inv"] @ r
     A, dinv = L["A"], L["dinv"]
-    x = chebyshev(A, dinv, r, np.zeros_like(r), k, L["lam"])
+    kl = k[0] if l == 0 else k[1]
+    x = chebyshev(A, dinv, r, np.zeros_like(r), kl, L["lam"])
     x = x + L["P"] @ vcycle(levels, l + 1, L["P"].T @ (r - A @ x), k)
-    return chebyshev(A, dinv, r, x, k, L["lam"])
+    return chebyshev(A, dinv, r, x, kl, L["lam"])
 
 
-def pcg_iterations(levels, b, k=3, rtol=1e-10, max_it=1000):
+def pcg_iterations(levels, b, k=(4, 2), rtol=1e-10, max_it=1000):
     A = levels[0]["A"]
     x = np.zeros_like(b); r = b.copy()
     z = vcycle(levels, 0, r, k); p = z.copy(); rz = r @ z

@@ -34,16 +34,31 @@ def test_mg_design_eval(oracle, mastg, mesh, lo):
     ctx.close()
 
 
-def test_mg_iterations_match_restatement(oracle, mastg):
-    print("[mg vs scipy restatement] 16x8x8", flush=True)
+@pytest.mark.parametrize("steps", [None, (3, 3), (2, 4)])
+def test_mg_iterations_match_restatement(oracle, mastg, steps):
+    """Default smoothing (4 steps on the problem level, 2 on coarse levels) and
+    two settings through the options."""
+    print(f"[mg vs scipy restatement] 16x8x8 steps={steps}", flush=True)
     mesh = (16, 8, 8)
     P = oracle.Problem(oracle.CANTILEVER, *mesh, seed=3, lo=0.01, hi=1.0)
     levels, b = MR.build_levels(P, mesh, np.array([1.0 / 16, 0.5 / 8, 0.5 / 8]))
     assert [L["A"].shape[0] for L in levels] == [4131, 675, 135]
-    it_ref, _ = MR.pcg_iterations(levels, b, k=3)
+    it_ref, _ = MR.pcg_iterations(levels, b, k=steps or (4, 2))
     ctx = ctx_from_oracle(mastg, P)
+    if steps:
+        ctx.set_option(mastg.OPT_MG_STEPS_FINE, steps[0])
+        ctx.set_option(mastg.OPT_MG_STEPS_COARSE, steps[1])
     r = ctx.design_eval(P.params(), solver=mastg.SOLVER_MG)
     assert abs(r["iters"] - it_ref) <= 1, (r["iters"], it_ref)
+    ctx.close()
+
+
+def test_mg_steps_option_range(oracle, mastg):
+    P = oracle.Problem(oracle.CANTILEVER, 8, 4, 4, seed=3, lo=0.3, hi=1.0)
+    ctx = ctx_from_oracle(mastg, P)
+    for bad in (0, 9):
+        with pytest.raises(mastg.MastgError):
+            ctx.set_option(mastg.OPT_MG_STEPS_FINE, bad)
     ctx.close()
 
 
@@ -118,7 +133,10 @@ def test_mg_plate_design_eval(oracle, mastg, n):
     assert r["rel_res"] <= 1.5e-10
     assert abs(r["objective"] - ref["compliance"]) <= 1e-9 * abs(ref["compliance"])
     assert rel_err_floor(r["grad"], ref["dc"], floor=1e-6) <= 1e-7
-    assert r["iters"] * 8 < ref["iters"], (r["iters"], ref["iters"])
+    # thin plates converge slowly under point-Jacobi Chebyshev smoothing; the
+    # default steps (4 on the problem level, 2 on coarse levels; fastest per
+    # design at 512^2) need ~1/8 of Jacobi-CG's iterations at 16^2
+    assert r["iters"] * 6 < ref["iters"], (r["iters"], ref["iters"])
     # deterministic: a second design evaluation is bitwise equal
     r2 = ctx.design_eval(P.params(), solver=mastg.SOLVER_MG)
     assert r2["objective"] == r["objective"] and np.array_equal(r2["grad"], r["grad"])
