@@ -22,6 +22,7 @@ import mastg  # noqa: E402
 
 PEAK_HBM_GBS = 8000.0
 PEAK_FP64_TFLOPS = 78.6
+PEAK_FP32_TFLOPS = 157.3
 WORKLOADS = {
     # name: (nx, ny, nz, rho_lo, rho_hi)
     "c2": (128, 64, 64, 0.01, 1.0),
@@ -50,6 +51,13 @@ def lattice_flops(n_nodes):
 def lattice_bytes(n_active, n_elems):
     # x read + y write (FP64) and the element scales
     return 2 * n_active * 8 + n_elems * 8
+
+
+def smooth_bytes(n_active, n_elems):
+    # one FP32 Chebyshev step: direction read, x and residual read + written,
+    # inverse diagonal read, next direction written (FP32), element scales
+    # (FP64) and the per-node constraint bits
+    return 7 * n_active * 4 + n_elems * 8 + n_active // 3
 
 
 def measured_traffic(kernel, workload):
@@ -142,9 +150,10 @@ def physics_main(a):
     t_setup = time.perf_counter() - t0
     print(f"# workload {a.workload}: {w.n_elems} elems, {ctx.n_active} active DOF, nnz {ctx.nnz}, "
           f"setup {t_setup:.1f}s", file=sys.stderr, flush=True)
-    # multigrid-preconditioned Krylov solves (plate: CG; heat: Newton with
-    # BiCGStab); the Jacobi-preconditioned path is timed once beside them
-    sv = mastg.SOLVER_MG if a.solver in ("mg", "all") else mastg.SOLVER_CG_JACOBI
+    # heat: Newton with multigrid-preconditioned BiCGStab, the Jacobi-
+    # preconditioned path timed once beside it; plates: CG + Jacobi (multigrid
+    # is switched off for the plate)
+    sv = mastg.SOLVER_MG if a.solver in ("mg", "all") and kind == 3 else mastg.SOLVER_CG_JACOBI
     plain = None
     if sv == mastg.SOLVER_MG:
         t1 = time.perf_counter()
@@ -301,8 +310,9 @@ def main():
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch the solver kernels directly instead of replaying hipGraphs")
     ap.add_argument("--kernel-profile", action="store_true",
-                    help="only assemble the workload and launch the lattice operator and the 256 MiB "
-                         "copy kernel 5 times each (a short program for hardware-counter runs)")
+                    help="only assemble the workload and launch the lattice operator, the FP32 smoothing "
+                         "step and the 256 MiB copy kernel 5 times each (a short program for "
+                         "hardware-counter runs)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the multi-GPU path is not built", file=sys.stderr)
@@ -326,11 +336,14 @@ def main():
         ctx.set_params(params)
         ctx.assemble(want_R=False, want_J=False)
         k_ms = ctx.bench_kernel(6, 5)
+        s_ms = ctx.bench_kernel(8, 5)
         c_ms = ctx.bench_kernel(7, 5)
-        print(json.dumps({"workload": a.workload, "lattice_apply_ms": k_ms, "copy_256MiB_ms": c_ms,
+        print(json.dumps({"workload": a.workload, "lattice_apply_ms": k_ms, "lattice_smooth_ms": s_ms,
+                          "copy_256MiB_ms": c_ms,
                           "copy_GBs": 2 * 2**28 / (c_ms * 1e-3) / 1e9,
                           "lattice_flops": lattice_flops(ctx.n_active // 3),
-                          "lattice_bytes": lattice_bytes(ctx.n_active, w.n_elems)}), flush=True)
+                          "lattice_bytes": lattice_bytes(ctx.n_active, w.n_elems),
+                          "smooth_bytes": smooth_bytes(ctx.n_active, w.n_elems)}), flush=True)
         ctx.close()
         return
     print(f"# workload {a.workload}: {w.n_elems} elems, {ctx.n_active} active DOF, nnz {ctx.nnz}, "
@@ -365,26 +378,37 @@ def main():
     lat = H["lat"]
     elapsed = H["elapsed"]
     designs_per_s = H["dps"]
-    t_elem_path = np.mean([x["t_assemble"] + x["t_sens"] for x in res])
-    elem_rate = w.n_elems / t_elem_path
+    # element K + R + dR/drho rate from the element and sensitivity kernels
+    # (device events): the matrix-free multigrid path forms no element
+    # matrices, so its assembly phase is not the measure
+    elem_rate = w.n_elems / ((ctx.bench_kernel(0, 5) + ctx.bench_kernel(4, 5)) * 1e-3)
     lanes, grid = ctx.tuning()
 
     # roofline of the dominant kernel of the headline path
     if head == "mg":
-        # the lattice operator (CG product; the smoothing steps run the same
-        # kernel with a row-local epilogue): FP64 arithmetic bound
+        # dominant kernel: the FP32 smoothing step on the problem level (2 x 5
+        # launches per V-cycle, one V-cycle per CG iteration), arithmetic
+        # bound; the FP64 CG product (one launch per iteration) beside it
+        wl = f"hex8 {nx}x{ny}x{nz}"
+        s_ms = ctx.bench_kernel(8, 20)
         k_ms = ctx.bench_kernel(6, 20)
         n_nodes = ctx.n_active // 3
         fl = lattice_flops(n_nodes)
-        byt = lattice_bytes(ctx.n_active, w.n_elems)
-        achieved = fl / (k_ms * 1e-3) / 1e12
-        roof = {"bound": "mfma", "kernel": "k_lattice_apply<0> (FP64 per-node lattice operator)",
-                "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / PEAK_FP64_TFLOPS,
-                "traffic": measured_traffic("k_lattice_apply<0>", f"hex8 {nx}x{ny}x{nz}"), "avg_launch_ms": k_ms,
-                "algorithmic_flops": fl, "algorithmic_bytes": byt,
-                "hbm_view_gbs": byt / (k_ms * 1e-3) / 1e9,
-                "note": "FP64 dense peak of MI355X (78.6 TFLOP/s; VALU FP64 runs at the same rate)"}
+        sbyt = smooth_bytes(ctx.n_active, w.n_elems)
+        achieved = fl / (s_ms * 1e-3) / 1e12
+        a64 = fl / (k_ms * 1e-3) / 1e12
+        roof = {"bound": "mfma", "kernel": "k_lattice_smooth (FP32 multigrid smoothing step, lattice operator)",
+                "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / PEAK_FP32_TFLOPS,
+                "traffic": measured_traffic("k_lattice_smooth", wl), "avg_launch_ms": s_ms,
+                "algorithmic_flops": fl, "algorithmic_bytes": sbyt,
+                "hbm_view_gbs": sbyt / (s_ms * 1e-3) / 1e9,
+                "note": "FP32 peak of MI355X (157.3 TFLOP/s); the kernel's FMAs are unpacked VALU FP32",
+                "fp64_cg_operator": {
+                    "kernel": "k_lattice_apply (FP64 per-node lattice operator)", "avg_launch_ms": k_ms,
+                    "achieved": a64, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": a64 / PEAK_FP64_TFLOPS,
+                    "traffic": measured_traffic("k_lattice_apply", wl),
+                    "algorithmic_bytes": lattice_bytes(ctx.n_active, w.n_elems)}}
     else:
         if head in ("plain", "mixed"):
             k_ms = ctx.bench_kernel(2, 20)

@@ -143,13 +143,16 @@ int mastg_assemble(mastg_ctx* ctx, const double* X, double* R, double* J_vals,
 #define MASTG_SOLVER_MATFREE 3          /* FP64 CG + Jacobi on the matrix-free operator
                                            over congruent-element classes (the same
                                            matrix as the CSR; hex8, <= 8 element shapes) */
-#define MASTG_SOLVER_MG 4               /* FP64 CG preconditioned by a geometric multigrid
-                                           V-cycle: trilinear transfers, Galerkin coarse
-                                           operators re-formed per assembly, Chebyshev/
-                                           Jacobi smoothing, dense coarsest solve; the
-                                           matrix-free operator on the problem level.
-                                           hex8 meshes that fill a lattice whose cell
-                                           counts halve to <= 1536 coarse DOFs */
+#define MASTG_SOLVER_MG 4               /* FP64 Krylov solver preconditioned by a geometric
+                                           multigrid V-cycle: trilinear transfers, Galerkin
+                                           coarse operators re-formed per parameter set,
+                                           Chebyshev/Jacobi smoothing, dense coarsest solve.
+                                           hex8 elasticity: CG, matrix-free lattice operator
+                                           on the problem level (FP32 smoothing); hex8
+                                           heat: BiCGStab on the assembled Jacobian.
+                                           Meshes that fill a lattice whose cell counts
+                                           halve to <= 1536 coarse DOFs; refused for the
+                                           Mindlin plate (switched off) */
 
 typedef struct {
     int32_t solver;       /* MASTG_SOLVER_*                                   */
@@ -197,7 +200,11 @@ int mastg_output(mastg_ctx* ctx, int kind, const double* X, int where, double* v
 /* assemble + solve + output + adjoint sensitivities with everything resident
  * on the device: compliance and dC/dp for the elastic physics, mean
  * temperature and d(mean T)/dq for heat (Newton for the k(T) nonlinearity).
- * params / dobj are [n_elems]; `where` applies to both. */
+ * params / dobj are [n_elems]; `where` applies to both.  With MASTG_SOLVER_MG
+ * on a hex8 lattice mesh the evaluation is matrix-free (lattice operator, row
+ * information and Galerkin products from the element class matrix): no
+ * element or global matrix is formed and none is left assembled afterwards
+ * (mastg_assemble forms one). */
 typedef struct {
     double objective;
     int32_t iters;          /* linear iterations (sum over Newton steps)    */
@@ -224,9 +231,10 @@ int mastg_enable_kernel_timing(mastg_ctx* ctx, int on);
 /* Replays one kernel of the resident state n_launch times back to back on the
  * context's stream and returns the average duration from HIP events (same
  * kinds as mastg_kernel_time, plus 5 = the FP32 matrix-core element kernel,
- * hex8 only, into a buffer of its own, 6 = the per-node lattice operator, and
- * 7 = a plain 256 MiB device copy, the known-traffic kernel that hardware byte
- * counters are scaled against).  For roofline measurement. */
+ * hex8 only, into a buffer of its own, 6 = the per-node lattice operator, 7 =
+ * a plain 256 MiB device copy, the known-traffic kernel that hardware byte
+ * counters are scaled against, and 8 = one FP32 multigrid smoothing step on
+ * the lattice, hex8 only).  For roofline measurement. */
 int mastg_bench_kernel(mastg_ctx* ctx, int kind, int n_launch, double* avg_ms);
 /* Launch shapes: CSR SpMV lanes per row and grid size (work-groups of the
  * SpMV / CG vector kernels).  Autotuned on the first solve unless set. */
@@ -236,7 +244,8 @@ int mastg_get_tuning(const mastg_ctx* ctx, int* spmv_lanes, int* grid);
 #define MASTG_OPT_GRAPHS 1
 /* multigrid smoothing: Chebyshev steps per pre-/post-smoothing on the problem
  * level (MASTG_OPT_MG_STEPS_FINE) and on the coarse levels
- * (MASTG_OPT_MG_STEPS_COARSE); 1..8, defaults 4 and 2 (heat: 3 and 2) */
+ * (MASTG_OPT_MG_STEPS_COARSE); 1..8, defaults: hex8 elasticity 5 and 2,
+ * plate 4 and 2, heat 3 and 2 */
 #define MASTG_OPT_MG_STEPS_FINE 2
 #define MASTG_OPT_MG_STEPS_COARSE 3
 int mastg_set_option(mastg_ctx* ctx, int option, int value);
@@ -254,8 +263,8 @@ int mastg_set_tuning(mastg_ctx* ctx, int spmv_lanes, int grid);
 typedef struct mastg_server mastg_server;
 typedef struct {
     int32_t n_slots;          /* concurrent requests on the device                 */
-    int32_t solver;           /* MASTG_SOLVER_*: MG serves hex8 and plates; other
-                                 solvers serve hex8, plates then use CG + Jacobi   */
+    int32_t solver;           /* MASTG_SOLVER_* for hex8 requests; plates are
+                                 served by CG + Jacobi                            */
     int64_t hbm_budget;       /* device bytes the server may hold; 0 = 64 GiB      */
     int32_t queue_capacity;   /* pending requests before submit returns BUSY       */
     double rtol;              /* hex8 solve tolerance, 0 -> 1e-10                  */

@@ -58,6 +58,7 @@ struct MGLevelDev {
     MGLevelArgs a{};
     int64_t n_elems = 0;
     DevBuf rank_of_lat, elem_of_cell, con_lat, nodecon_lat, dinv_lat, b_lat, sE_cell;   // level 0
+    DevBuf x32, res32, da32, db32, dinv32, y_lat;   // hex8 level 0: FP32 smoothing state, FP64 scratch
     DevBuf blk_ptr, blk_col, cptr, cidx, econn, con;   // levels >= 1
     DevBuf Kel, val, val32, dinv, r, x, res, da, db;
 };
@@ -76,6 +77,7 @@ struct mastg_ctx {
     DevBuf ninc_ptr, ninc;
     // matrix-free operator data (hex8 congruence classes)
     DevBuf Khat, sE, eclass, nodecon, Fel;
+    DevBuf Khat32;              // FP32 copy of Khat (lattice smoothing)
     // heat: convective side masks, mean-T weights, BiCGStab work space, A^T values
     DevBuf sidemask, wmean, tpos, valT, vr0, vv, vs, vt, vph, vsh, bscal, vT;
     double volume = 0;
@@ -418,8 +420,11 @@ int assemble_active(mastg_ctx* c, const double* X_active, bool want_R, bool want
 // ---- FP64 CSR CG + Jacobi on the resident matrix: b (active, device) -> c->vx
 int autotune(mastg_ctx* c);
 void ensure_scales(mastg_ctx* c);
+bool lattice_args(mastg_ctx* c, CGArgs& a);
 int cg_finish(mastg_ctx* c, const CGArgs& a, const double* b, hipEvent_t e0, hipEvent_t e1,
               mastg_solve_stats* st, int done);
+int solve_stats(mastg_ctx* c, const double* r, const double* b, hipEvent_t e0, hipEvent_t e1,
+                mastg_solve_stats* st, int done);
 
 int cg_loop(mastg_ctx* c, const CGArgs& a, const double* b, const mastg_solve_opts* o, int op, int chunk,
             hipEvent_t e0, hipEvent_t e1, mastg_solve_stats* st);
@@ -544,8 +549,15 @@ int mg_build(mastg_ctx* c) {
     if (c->mg_tried) return c->mg.empty() ? fail(c, MASTG_ERR_INVALID, "multigrid unavailable: " + c->mg_why)
                                           : MASTG_OK;
     c->mg_tried = true;
-    // the plate and heat problem levels smooth with their assembled node blocks
-    // in node-rank order; hex8 elasticity uses the per-node lattice operator
+    if (c->d.physics == MASTG_QUAD4_MINDLIN) {
+        // switched off: a device memory fault of undetermined cause was seen in a
+        // plate design evaluation; plates are solved by CG + Jacobi until the
+        // cause is shown (the hierarchy code below still handles them)
+        c->mg_why = "multigrid is switched off for the Mindlin plate (use CG + Jacobi)";
+        return fail(c, MASTG_ERR_INVALID, "multigrid unavailable: " + c->mg_why);
+    }
+    // the heat problem level smooths with its assembled node blocks in
+    // node-rank order; hex8 elasticity uses the per-node lattice operator
     const bool plate = c->d.physics != MASTG_HEX8_ELASTIC;
     if (!(plate || c->S.n_classes == 1)) {
         c->mg_why = "multigrid is built for lattice meshes (hex8 elasticity needs one element shape)";
@@ -609,6 +621,21 @@ int mg_build(mastg_ctx* c) {
             a.dinv = L->dinv_lat.as<double>();
             a.r = c->vr.as<double>();
             a.planar = 1;
+            // FP32 smoothing state; y_lat: FP64 lattice-ordered scratch for
+            // operator products outside the solve
+            TRY(dalloc(c, L->x32, nv / 2));
+            TRY(dalloc(c, L->res32, nv / 2));
+            TRY(dalloc(c, L->da32, nv / 2));
+            TRY(dalloc(c, L->db32, nv / 2));
+            TRY(dalloc(c, L->dinv32, nv / 2));
+            TRY(dalloc(c, L->y_lat, nv));
+            a.x32 = L->x32.as<float>();
+            a.res32 = L->res32.as<float>();
+            a.da32 = L->da32.as<float>();
+            a.db32 = L->db32.as<float>();
+            a.dinv32 = L->dinv32.as<float>();
+            TRY(dalloc(c, c->Khat32, 64 * 9 * sizeof(float)));
+            launch_to_f32(64 * 9, c->Khat.as<double>(), c->Khat32.as<float>(), c->stream);
         } else {
             const Structure& S = h.S;
             TRY(upload(c, L->blk_ptr, S.blk_ptr));
@@ -634,14 +661,16 @@ int mg_build(mastg_ctx* c) {
             a.dinv = L->dinv.as<double>();
             a.r = L->r.as<double>();
         }
-        TRY(dalloc(c, L->x, nv));
-        TRY(dalloc(c, L->res, nv));
-        TRY(dalloc(c, L->da, nv));
-        TRY(dalloc(c, L->db, nv));
-        a.x = L->x.as<double>();
-        a.res = L->res.as<double>();
-        a.da = L->da.as<double>();
-        a.db = L->db.as<double>();
+        if (!a.planar) {
+            TRY(dalloc(c, L->x, nv));
+            TRY(dalloc(c, L->res, nv));
+            TRY(dalloc(c, L->da, nv));
+            TRY(dalloc(c, L->db, nv));
+            a.x = L->x.as<double>();
+            a.res = L->res.as<double>();
+            a.da = L->da.as<double>();
+            a.db = L->db.as<double>();
+        }
         lv.push_back(std::move(L));
     }
     TRY(dalloc(c, c->mg_cheb, size_t(nl) * kMGChebStride * sizeof(double)));
@@ -663,20 +692,27 @@ int mg_build(mastg_ctx* c) {
 // assembly (needs the FP64 element blocks and CSR values of that assembly)
 int mg_setup(mastg_ctx* c) {
     if (c->mg_valid) return MASTG_OK;
-    if (!c->kel64_valid) return fail(c, MASTG_ERR_INVALID, "multigrid needs an FP64 assembly");
+    // hex8 on a lattice (one element class): the problem level is matrix-free,
+    // its row information and the first Galerkin product come from the class
+    // matrix and the cell scales; plates and heat read their FP64 assembly
+    const bool lat = !c->mg_power;
+    if (!lat && !c->kel64_valid) return fail(c, MASTG_ERR_INVALID, "multigrid needs an FP64 assembly");
     const int nl = int(c->mg.size());
     const int nav = c->S.nav;
     hipStream_t s = c->stream;
-    launch_mg_rowinfo_nb(nav, c->S.n_nodes, c->blk_ptr.as<int64_t>(), c->blk_col.as<int32_t>(), c->val.as<double>(),
-                         nullptr, c->mg_part.as<double>(), c->mg_lam.as<double>(), s);
-    if (c->mg_power) {
+    MGLevelDev& L0 = *c->mg[0];
+    if (lat) {
+        ensure_scales(c);
+        CGArgs la = cg_args(c);
+        if (!lattice_args(c, la)) return fail(c, MASTG_ERR_INVALID, "no lattice operator for this mesh");
+        launch_lattice_rowinfo(la.lat, L0.dinv_lat.as<double>(), L0.dinv32.as<float>(), c->mg_part.as<double>(),
+                               c->mg_lam.as<double>(), s);
+    } else {
+        launch_mg_rowinfo_nb(nav, c->S.n_nodes, c->blk_ptr.as<int64_t>(), c->blk_col.as<int32_t>(),
+                             c->val.as<double>(), nullptr, c->mg_part.as<double>(), c->mg_lam.as<double>(), s);
         launch_to_f32(c->S.nnz, c->val.as<double>(), c->val32.as<float>(), s);
-        MGLevelDev& L0 = *c->mg[0];
         launch_mg_power(L0.a, c->val.as<double>(), kMGPowerIters, kMGPowerSafety, L0.a.da, L0.a.db,
                         c->mg_part.as<double>(), c->mg_scal.as<double>(), c->mg_lam.as<double>(), s);
-    } else {
-        launch_perm_nodes(c->S.n_nodes, c->mg[0]->rank_of_lat.as<int32_t>(), c->dinv.as<double>(),
-                          c->mg[0]->dinv_lat.as<double>(), 0, s);
     }
     for (int l = 0; l + 1 < nl; l++) {
         MGLevelDev& F = *c->mg[l];
@@ -692,7 +728,9 @@ int mg_setup(mastg_ctx* c) {
         g.f_con = l == 0 ? c->con.as<uint8_t>() : F.a.con;   // by node rank, as f_econn
         g.c_econn = C.econn.as<int32_t>();
         g.c_con = C.a.con;
-        g.Kf = l == 0 ? c->Kel.as<double>() : F.Kel.as<double>();
+        g.Kf = l == 0 ? (lat ? nullptr : c->Kel.as<double>()) : F.Kel.as<double>();
+        g.Khat = l == 0 && lat ? c->Khat.as<double>() : nullptr;
+        g.sE_cell = l == 0 && lat ? L0.sE_cell.as<double>() : nullptr;
         g.Kc = C.Kel.as<double>();
         launch_mg_galerkin(g, s);
         ScatterArgs sa{C.a.n_nodes, nav, C.a.blk_ptr, C.cptr.as<int32_t>(), C.cidx.as<int32_t>(),
@@ -725,51 +763,66 @@ bool lattice_args(mastg_ctx* c, CGArgs& a) {
     a.lat.nx = L0.a.nx; a.lat.ny = L0.a.ny; a.lat.nz = L0.a.nz;
     a.lat.sE_cell = L0.sE_cell.as<double>();
     a.lat.Khat = c->Khat.as<double>();
+    a.lat.Khat32 = c->Khat32.as<float>();
     a.lat.nodecon = L0.nodecon_lat.as<uint8_t>();
     return lattice_partials(a.lat) <= int64_t(c->part0.bytes / sizeof(double));
 }
 
 // one V-cycle on the problem-level vector r0 (null: the level-0 args' r); the
-// level-0 output is x0 + dlast0 (formed by the caller)
-const double* mg_vcycle(mastg_ctx* c, const CGArgs& a, const int* done, hipStream_t s,
-                        const double* r0 = nullptr) {
+// level-0 output is x0 + dlast0 (formed by the caller): the returned dlast0
+// is FP32 on the hex8 problem level (its smoothing state), else FP64
+const void* mg_vcycle(mastg_ctx* c, const CGArgs& a, const int* done, hipStream_t s, const double* r0 = nullptr) {
     const int nl = int(c->mg.size());
     MGLevelArgs L0 = c->mg[0]->a;
     if (r0) L0.r = const_cast<double*>(r0);
+    const bool f32 = L0.planar != 0;
     auto lv = [&](int l) -> const MGLevelArgs& { return l == 0 ? L0 : c->mg[l]->a; };
     auto dbuf = [&](int l, int i) { return (i % 2 == 0) ? lv(l).da : lv(l).db; };
-    auto apply = [&](int l, int mode, int step, const double* din, double* dout) {
-        if (l == 0 && !c->mg_power) launch_mg_fine(a, L0, mode, step, din, dout, done, s);
-        else launch_mg_bsr(lv(l), mode, step, din, dout, done, s);
+    auto dbuf32 = [&](int i) { return (i % 2 == 0) ? L0.da32 : L0.db32; };
+    // smoothing step from direction buffer iin to iout (-1: none)
+    auto apply = [&](int l, int mode, int step, int iin, int iout) {
+        if (l == 0 && f32)
+            launch_mg_fine(a, L0, mode, step, iin < 0 ? nullptr : dbuf32(iin), iout < 0 ? nullptr : dbuf32(iout),
+                           done, s);
+        else
+            launch_mg_bsr(lv(l), mode, step, iin < 0 ? nullptr : dbuf(l, iin), iout < 0 ? nullptr : dbuf(l, iout),
+                          done, s);
     };
     auto steps = [&](int l) { return l == 0 ? c->mg_k0 : c->mg_kc; };
     for (int l = 0; l + 1 < nl; l++) {
         const int k = steps(l);
         // coarse levels start their pre-smoothing inside the restriction
         if (l == 0) launch_mg_pre_init(lv(l), done, s);
-        for (int i = 1; i < k; i++) apply(l, MG_STEP, i, dbuf(l, i - 1), dbuf(l, i));
-        apply(l, MG_FINAL, 0, dbuf(l, k - 1), nullptr);
+        for (int i = 1; i < k; i++) apply(l, MG_STEP, i, i - 1, i);
+        apply(l, MG_FINAL, 0, k - 1, -1);
         launch_mg_restrict(lv(l), lv(l + 1), done, s);
     }
     const MGLevelArgs& Lc = lv(nl - 1);
     launch_mg_coarse_solve(int(c->mg_n_coarse), c->mg_dense.as<double>(), Lc.r, Lc.x, done, s);
-    const double* dlast = nullptr;
+    const double* dlast = nullptr;   // last direction of level l + 1 (coarse levels are FP64)
     for (int l = nl - 2; l >= 0; l--) {
         const int k = steps(l);
         launch_mg_prolong(lv(l), lv(l + 1), dlast, done, s);
-        apply(l, MG_RESID_INIT, 0, nullptr, dbuf(l, 0));
-        for (int i = 1; i < k; i++) apply(l, MG_STEP, i, dbuf(l, i - 1), dbuf(l, i));
-        dlast = dbuf(l, k - 1);
+        apply(l, MG_RESID_INIT, 0, -1, 0);
+        for (int i = 1; i < k; i++) apply(l, MG_STEP, i, i - 1, i);
+        if (l > 0) dlast = dbuf(l, k - 1);
     }
-    return dlast;
+    const int k0 = steps(0);
+    return f32 ? static_cast<const void*>(dbuf32(k0 - 1)) : static_cast<const void*>(dbuf(0, k0 - 1));
+}
+
+// z = x0 + dlast0 on the problem level and the partial sums of r.z
+void mg_rz(mastg_ctx* c, const CGArgs& a, const void* dl, hipStream_t s) {
+    const MGLevelArgs& L0 = c->mg[0]->a;
+    if (L0.planar) launch_pcg_rz32(a, L0.x32, static_cast<const float*>(dl), s);
+    else launch_pcg_rz(a, L0.x, static_cast<const double*>(dl), s);
 }
 
 void mg_pcg_iteration(mastg_ctx* c, const CGArgs& a, hipStream_t s) {
     const int* done = &a.S->done;
     launch_cg_op_alpha(a, s);
     launch_pcg_update(a, s);
-    const double* dlast = mg_vcycle(c, a, done, s);
-    launch_pcg_rz(a, c->mg[0]->a.x, dlast, s);
+    mg_rz(c, a, mg_vcycle(c, a, done, s), s);
     launch_pcg_beta(a, s);
     launch_cg_p(a, s);
 }
@@ -780,7 +833,7 @@ int mg_solve(mastg_ctx* c, const double* b, const mastg_solve_opts* o, mastg_sol
     const int max_it = (o && o->max_iters > 0) ? o->max_iters : 100000;
     const int chunk = (o && o->check_every > 0) ? o->check_every : 4;
     TRY(mg_build(c));
-    if (!c->tuned) TRY(autotune(c));   // launch shape of the CSR products
+    if (c->mg_power && !c->tuned) TRY(autotune(c));   // launch shape of the CSR products
     if (!c->mg_power) ensure_scales(c);
     hipEvent_t e0, e1;
     HIPC(c, ev_create(&e0));
@@ -807,8 +860,7 @@ int mg_solve(mastg_ctx* c, const double* b, const mastg_solve_opts* o, mastg_sol
     launch_fill(S.n_active, 0.0, a.x, c->stream);
     HIPC(c, hipMemcpyAsync(a.r, bl, S.n_active * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     launch_pcg_start(a, bl, rtol * rtol, max_it, c->stream);
-    const double* dlast = mg_vcycle(c, a, &a.S->done, c->stream);
-    launch_pcg_rz(a, c->mg[0]->a.x, dlast, c->stream);
+    mg_rz(c, a, mg_vcycle(c, a, &a.S->done, c->stream), c->stream);
     launch_pcg_start2(a, c->stream);
     HIPC(c, hipMemcpyAsync(a.p, a.z, S.n_active * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     HIPC(c, hipMemcpyAsync(c->h_scal, a.S, sizeof(CGScalars), hipMemcpyDeviceToHost, c->stream));
@@ -854,17 +906,24 @@ int mg_solve(mastg_ctx* c, const double* b, const mastg_solve_opts* o, mastg_sol
                 if (rr < 0.25 * best2) stalls = 0;
                 else if (++stalls >= 8) { done = 5; break; }
                 best2 = std::min(best2, rr);
-                const double* dl = mg_vcycle(c, a, &a.S->done, c->stream);
-                launch_pcg_rz(a, c->mg[0]->a.x, dl, c->stream);
+                mg_rz(c, a, mg_vcycle(c, a, &a.S->done, c->stream), c->stream);
                 launch_pcg_start2(a, c->stream);
                 HIPC(c, hipMemcpyAsync(a.p, a.z, S.n_active * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
             }
         }
     }
     if (!c->mg_power) {
-        // back to rank order (x lives in c->vx for the callers)
+        // the true residual in lattice order with the FP64 lattice operator,
+        // then x back to rank order (x lives in c->vx for the callers)
+        HIPC(c, hipEventRecord(e1, c->stream));
+        CGArgs an = a;
+        an.S = nullptr;
+        launch_lattice_apply(an, a.x, c->vtmp.as<double>(), c->stream);
+        launch_axpby(S.n_active, 1.0, bl, -1.0, c->vtmp.as<double>(), c->stream);
         launch_perm_nodes(S.n_nodes, lat_map, a.x, a.z, 1, c->stream);
         HIPC(c, hipMemcpyAsync(a.x, a.z, S.n_active * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        HIPC(c, hipGetLastError());
+        return solve_stats(c, c->vtmp.as<double>(), bl, e0, e1, st, done);
     }
     HIPC(c, hipGetLastError());
     return cg_finish(c, a, b, e0, e1, st, done);
@@ -878,8 +937,15 @@ int cg_finish(mastg_ctx* c, const CGArgs& a, const double* b, hipEvent_t e0, hip
     // matrix-free operator this also checks its result against the CSR)
     launch_spmv_csr(a, a.x, c->vtmp.as<double>(), 0, c->stream);
     launch_axpby(S.n_active, 1.0, b, -1.0, c->vtmp.as<double>(), c->stream);
+    return solve_stats(c, c->vtmp.as<double>(), b, e0, e1, st, done);
+}
+
+// ||r|| / ||b|| of the true residual r, the solve's device time (e0 .. e1) and
+// the outcome of `done`
+int solve_stats(mastg_ctx* c, const double* r, const double* b, hipEvent_t e0, hipEvent_t e1,
+                mastg_solve_stats* st, int done) {
     double rr = 0, bb = 0;
-    TRY(dot_active(c, c->vtmp.as<double>(), c->vtmp.as<double>(), &rr));
+    TRY(dot_active(c, r, r, &rr));
     TRY(dot_active(c, b, b, &bb));
     float ms = 0;
     HIPC(c, hipEventElapsedTime(&ms, e0, e1));
@@ -934,10 +1000,11 @@ void mg_bicg_iteration(mastg_ctx* c, const BiCGArgs& a, hipStream_t s) {
     const CGArgs none{};
     const int64_t n = c->S.n_active;
     launch_bicg_part(a, 0, s);
-    const double* dl = mg_vcycle(c, none, done, s, a.p);
+    // (plate and heat: FP64 problem-level state)
+    const double* dl = static_cast<const double*>(mg_vcycle(c, none, done, s, a.p));
     launch_mg_combine(n, c->mg[0]->a.x, dl, a.ph, done, s);
     launch_bicg_part(a, 1, s);
-    dl = mg_vcycle(c, none, done, s, a.s);
+    dl = static_cast<const double*>(mg_vcycle(c, none, done, s, a.s));
     launch_mg_combine(n, c->mg[0]->a.x, dl, a.sh, done, s);
     launch_bicg_part(a, 2, s);
 }
@@ -1170,7 +1237,7 @@ int mastg_ctx_create(const mastg_problem_desc* desc, mastg_ctx** out) {
         if (c->d.penalty == 0) c->d.penalty = 3.0;
         // smoothing steps measured fastest per design (128x64x64 cantilever,
         // 512^2 plate, 96^3 heat): the coarse levels are launch-latency bound
-        c->mg_k0 = c->d.physics == MASTG_HEX8_HEAT ? 3 : 4;
+        c->mg_k0 = c->d.physics == MASTG_HEX8_HEAT ? 3 : c->d.physics == MASTG_HEX8_ELASTIC ? 5 : 4;
         c->mg_kc = 2;
         std::string m = build_structure(desc->physics, desc->n_nodes, desc->n_elems, desc->xyz, desc->conn,
                                         desc->n_vars, desc->node_dof_base, desc->n_constrained,
@@ -1411,6 +1478,7 @@ int mastg_set_params(mastg_ctx* c, const double* p, int where) {
     c->assembled = false;
     c->sE_valid = false;
     c->sEc_valid = false;
+    c->mg_valid = false;   // the coarse operators follow the parameters
     return MASTG_OK;
 }
 
@@ -1497,9 +1565,11 @@ int mastg_jacobian_product(mastg_ctx* c, const double* dX, double* JdX, int wher
         const MGLevelDev& L0 = *c->mg[0];
         const int32_t* lat_map = L0.rank_of_lat.as<int32_t>();
         launch_mask_constrained(c->S.n_active, c->con.as<uint8_t>(), c->vb.as<double>(), c->stream);
-        launch_perm_nodes(c->S.n_nodes, lat_map, c->vb.as<double>(), L0.a.x, 0, c->stream);
-        launch_lattice_apply(a, L0.a.x, L0.a.res, c->stream);
-        launch_perm_nodes(c->S.n_nodes, lat_map, L0.a.res, c->vtmp.as<double>(), 1, c->stream);
+        double* xl = L0.b_lat.as<double>();
+        double* yl = L0.y_lat.as<double>();
+        launch_perm_nodes(c->S.n_nodes, lat_map, c->vb.as<double>(), xl, 0, c->stream);
+        launch_lattice_apply(a, xl, yl, c->stream);
+        launch_perm_nodes(c->S.n_nodes, lat_map, yl, c->vtmp.as<double>(), 1, c->stream);
     } else if (op == 2) {
         launch_spmv_nb32(a, c->vb.as<double>(), c->vtmp.as<double>(), 0, c->stream);
     } else if (op == 1) {
@@ -1569,15 +1639,23 @@ int mastg_design_eval(mastg_ctx* c, const double* params, double* dobj, int wher
     if (c->d.physics == MASTG_HEX8_HEAT) return heat_design_eval(c, dobj, where, o, out);
     hipEvent_t e[4];
     for (auto& x : e) HIPC(c, ev_create(&x));
+    const int solver = o ? o->solver : MASTG_SOLVER_CG_JACOBI;
+    TRY(check_cg_solver(c, solver));
+    // hex8 multigrid on a lattice is matrix-free end to end (lattice operator,
+    // row information and Galerkin products from the class matrix): only the
+    // load vector is assembled, and no assembled matrix is left behind
+    bool matrix_free = false;
+    if (solver == MASTG_SOLVER_MG) {
+        TRY(mg_build(c));   // refusals come before any assembly
+        matrix_free = c->d.physics == MASTG_HEX8_ELASTIC && !c->mg_power;
+    }
     HIPC(c, hipEventRecord(e[0], c->stream));
-    TRY(assemble_active(c, nullptr, true, true));   // R(0) = -F -> b = F (constrained)
+    TRY(assemble_active(c, nullptr, true, !matrix_free));   // R(0) = -F -> b = F (constrained)
     launch_axpby(S.n_active, 0.0, c->vtmp.as<double>(), -1.0, c->vtmp.as<double>(), c->stream);
     HIPC(c, hipMemcpyAsync(c->vb.p, c->vtmp.p, S.n_active * sizeof(double), hipMemcpyDeviceToDevice,
                            c->stream));
     HIPC(c, hipEventRecord(e[1], c->stream));
     mastg_solve_stats st{};
-    const int solver = o ? o->solver : MASTG_SOLVER_CG_JACOBI;
-    TRY(check_cg_solver(c, solver));
     int rc = solver == MASTG_SOLVER_MG ? mg_solve(c, c->vb.as<double>(), o, &st)
                                        : cg_solve(c, c->vb.as<double>(), o, &st, cg_op(solver));
     HIPC(c, hipEventRecord(e[2], c->stream));
@@ -1680,7 +1758,8 @@ int mastg_set_tuning(mastg_ctx* c, int spmv_lanes, int grid) {
 
 int mastg_bench_kernel(mastg_ctx* c, int kind, int n_launch, double* avg_ms) {
     if (!c || !avg_ms || n_launch <= 0) return MASTG_ERR_INVALID;
-    if (!c->assembled) return fail(c, MASTG_ERR_INVALID, "assemble first");
+    if (!c->params_set) return fail(c, MASTG_ERR_INVALID, "set the design parameters first");
+    if ((kind == 1 || kind == 2) && !c->assembled) return fail(c, MASTG_ERR_INVALID, "assemble first");
     CGArgs a = cg_args(c);
     // kind 7: a 256 MiB device-to-device copy (read 256 MiB, write 256 MiB)
     constexpr int64_t kCopyDoubles = int64_t(1) << 25;
@@ -1703,12 +1782,29 @@ int mastg_bench_kernel(mastg_ctx* c, int kind, int n_launch, double* avg_ms) {
                 break;
             case 4: launch_sens(c, a.x, a.x, c->vg.as<double>()); break;
             case 5: launch_hex8_K_f32(hex8_args(c, nullptr, false), c->Kel32.as<float>(), c->stream); break;
-            case 6: launch_lattice_apply(a, c->mg[0]->a.da, c->mg[0]->a.db, c->stream); break;
+            case 6: launch_lattice_apply(a, c->mg[0]->b_lat.as<double>(), c->mg[0]->y_lat.as<double>(), c->stream); break;
             case 7: launch_copy_f64(kCopyDoubles, copy_src.as<double>(), copy_dst.as<double>(), c->stream); break;
+            case 8: {
+                const MGLevelArgs& L0 = c->mg[0]->a;
+                launch_mg_fine(a, L0, MG_STEP, 1, L0.da32, L0.db32, nullptr, c->stream);
+                break;
+            }
             default: break;
         }
     };
-    if (kind < 0 || kind > 7) return fail(c, MASTG_ERR_INVALID, "unknown kernel kind");
+    if (kind < 0 || kind > 8) return fail(c, MASTG_ERR_INVALID, "unknown kernel kind");
+    if (kind == 8) {
+        // one Chebyshev step of the hex8 problem level's FP32 smoothing on a
+        // zero state (x, res, directions) with the current assembly's bounds
+        TRY(mg_build(c));
+        a.S = nullptr;
+        if (!lattice_args(c, a)) return fail(c, MASTG_ERR_INVALID, "no lattice operator for this mesh");
+        ensure_scales(c);
+        TRY(mg_setup(c));
+        const MGLevelDev& L0 = *c->mg[0];
+        for (const DevBuf* b : {&L0.x32, &L0.res32, &L0.da32, &L0.db32})
+            HIPC(c, hipMemsetAsync(b->p, 0, b->bytes, c->stream));
+    }
     if (kind == 7) {
         TRY(dalloc(c, copy_src, kCopyDoubles * sizeof(double)));
         TRY(dalloc(c, copy_dst, kCopyDoubles * sizeof(double)));
@@ -1719,6 +1815,7 @@ int mastg_bench_kernel(mastg_ctx* c, int kind, int n_launch, double* avg_ms) {
         a.S = nullptr;
         if (!lattice_args(c, a)) return fail(c, MASTG_ERR_INVALID, "no lattice operator for this mesh");
         ensure_scales(c);
+        launch_fill(c->S.n_active, 1.0, c->mg[0]->b_lat.as<double>(), c->stream);
     }
     if (kind == 5) {
         if (c->d.physics != MASTG_HEX8_ELASTIC) return fail(c, MASTG_ERR_INVALID, "FP32 element kernel is hex8 only");

@@ -87,6 +87,7 @@ struct LatticeOpArgs {
     const double* sE_cell;             // [cells] element scales in lattice-cell order
                                        // (cell i + nx (j + ny k))
     const double* Khat;                // [64*9] the (single) class matrix at E = 1
+    const float* Khat32;               // its FP32 copy (the FP32 smoothing steps)
     const uint8_t* nodecon;            // [lattice points] bit v = variable v constrained
 };
 // partial sums the lattice operator writes (its grid)
@@ -198,6 +199,10 @@ struct MGLevelArgs {
     const double* dinv;                // [nav n_nodes] inverse diagonal
     double *r, *x, *res, *da, *db;     // right side, iterate, residual, smoothing directions
     const double* cheb;                // this level's Chebyshev coefficients
+    // the hex8 problem level (planar) smooths in FP32: iterate, residual,
+    // directions and inverse diagonal (r stays the FP64 CG residual)
+    float *x32, *res32, *da32, *db32;
+    const float* dinv32;
 };
 
 struct MGGalerkinArgs {                // coarse element matrices = sum_c Q_c^T K_c Q_c
@@ -210,7 +215,9 @@ struct MGGalerkinArgs {                // coarse element matrices = sum_c Q_c^T 
     const uint8_t* f_con;              // fine constraints by rank*3+v
     const int32_t* c_econn;
     const uint8_t* c_con;
-    const double* Kf;                  // [n_fine][npe^2][nav^2] fine element blocks
+    const double* Kf;                  // [n_fine][npe^2][nav^2] fine element blocks, or
+    const double* Khat;                // (non-null) fine blocks sE_cell[cell] Khat: one
+    const double* sE_cell;             // element class on a lattice, scales by cell
     double* Kc;                        // [n_coarse][npe^2][nav^2]
 };
 
@@ -230,12 +237,18 @@ void launch_mg_invert(int n, double* A, hipStream_t s);
 void launch_mg_pre_init(const MGLevelArgs& L, const int* done, hipStream_t s);
 void launch_mg_bsr(const MGLevelArgs& L, int mode, int step, const double* din, double* dout, const int* done,
                    hipStream_t s);
-// level 0: the lattice operator (c.lat) with the smoothing epilogue
-void launch_mg_fine(const CGArgs& c, const MGLevelArgs& L, int mode, int step, const double* din,
-                    double* dout, const int* done, hipStream_t s);
+// level 0 (hex8): FP32 smoothing step with the lattice operator (c.lat) and
+// the row-local epilogue on L's FP32 state
+void launch_mg_fine(const CGArgs& c, const MGLevelArgs& L, int mode, int step, const float* din, float* dout,
+                    const int* done, hipStream_t s);
 // y = A x with the lattice operator (lattice-ordered vectors) and partial sums
 // of x.y into c.part0
 void launch_lattice_apply(const CGArgs& c, const double* x, double* y, hipStream_t s);
+// inverse diagonal of the lattice operator (component-major lattice order,
+// FP64 and FP32) and lambda_max(D^-1 A) <= max Gershgorin ratio into *lam, with
+// the assembled entries formed from the class matrix and the cell scales
+void launch_lattice_rowinfo(const LatticeOpArgs& l, double* dinv, float* dinv32, double* part, double* lam,
+                            hipStream_t s);
 // rank order <-> lattice order: gather dst[p] = src[map[p]] or scatter dst[map[p]] = src[p]
 void launch_perm_nodes(int64_t n_nodes, const int32_t* map, const double* src, double* dst, int scatter,
                        hipStream_t s);
@@ -247,6 +260,7 @@ void launch_mg_rowinfo_nb(int nav, int64_t n_nodes, const int64_t* blk_ptr, cons
 void launch_mg_power(const MGLevelArgs& L, const double* val, int n_it, double safety, double* w0, double* w1,
                      double* part, double* scal, double* lam, hipStream_t s);
 void launch_mg_restrict(const MGLevelArgs& F, const MGLevelArgs& C, const int* done, hipStream_t s);
+// (F planar: the fine vectors are F's FP32 state)
 void launch_mg_prolong(const MGLevelArgs& F, const MGLevelArgs& C, const double* dlast, const int* done,
                        hipStream_t s);
 void launch_mg_coarse_solve(int n, const double* Ainv, const double* r, double* x, const int* done,
@@ -254,6 +268,8 @@ void launch_mg_coarse_solve(int n, const double* Ainv, const double* r, double* 
 // preconditioned CG around the V-cycle
 void launch_pcg_update(const CGArgs& c, hipStream_t s);
 void launch_pcg_rz(const CGArgs& c, const double* x0, const double* dlast, hipStream_t s);
+// the same from the FP32 problem-level state of the hex8 V-cycle
+void launch_pcg_rz32(const CGArgs& c, const float* x0, const float* dlast, hipStream_t s);
 void launch_pcg_start(const CGArgs& c, const double* b, double tol2, int max_it, hipStream_t s);
 void launch_pcg_start2(const CGArgs& c, hipStream_t s);
 // r.r of the current r (a restart's true residual): done = 1 / 0 / 2 as it meets tol

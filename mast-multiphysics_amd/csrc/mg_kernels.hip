@@ -60,12 +60,13 @@ __global__ __launch_bounds__(256) void k_mg_galerkin(MGGalerkinArgs G) {
         const int64_t fe = G.f_elem_of_cell ? int64_t(G.f_elem_of_cell[cell]) : cell;
         if (lane_on) {   // masked fine block (a, b) = (hi, lo)
             const int32_t ra = G.f_econn[fe * NPE + hi], rb = G.f_econn[fe * NPE + lo];
-            const double* kb = G.Kf + (fe * NPE * NPE + ln) * NB;
+            const double sc = G.Khat ? G.sE_cell[cell] : 1.0;
+            const double* kb = G.Khat ? G.Khat + ln * NB : G.Kf + (fe * NPE * NPE + ln) * NB;
 #pragma unroll
             for (int v = 0; v < NAV; v++)
 #pragma unroll
                 for (int w = 0; w < NAV; w++) {
-                    double x = kb[v * NAV + w];
+                    double x = sc * kb[v * NAV + w];
                     if (G.f_con[int64_t(ra) * NAV + v] || G.f_con[int64_t(rb) * NAV + w]) x = 0.0;
                     sK[wave][ln * NB + v * NAV + w] = x;
                 }
@@ -186,6 +187,18 @@ __global__ __launch_bounds__(256) void k_mg_pre_init(MGLevelArgs L, const int* _
     }
 }
 
+// the same start on the FP32 state of the hex8 problem level
+__global__ __launch_bounds__(256) void k_mg_pre_init32(MGLevelArgs L, const int* __restrict__ done) {
+    if (done && *done) return;
+    const float ith = float(L.cheb[0]);
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < L.n_nodes * L.nav; i += int64_t(gridDim.x) * 256) {
+        const float r = float(L.r[i]);
+        L.x32[i] = 0.f;
+        L.res32[i] = r;
+        L.da32[i] = L.dinv32[i] * r * ith;
+    }
+}
+
 // node-block operator of a level (values T = double or their FP32 copy;
 // products and sums in FP64), 8 lanes per node, fused epilogue
 template <typename T, int NAV>
@@ -244,37 +257,34 @@ __host__ __device__ constexpr int lat_corner(int dx, int dy, int dz) {
     return dz * 4 + (dy ? (dx ? 2 : 3) : (dx ? 1 : 0));
 }
 
-// KIND 0: y = A x and partial x.y (CG operator); KIND 1: multigrid smoothing
-// step with the row-local epilogue
-template <int KIND>
-__global__ __launch_bounds__(256) void k_lattice_apply(CGArgs C, const double* __restrict__ xcg,
-                                                       double* __restrict__ ycg, MGLevelArgs L, int mode, int step,
-                                                       const double* __restrict__ din, double* __restrict__ dout,
-                                                       const int* __restrict__ done) {
-    if (done && *done) return;
-    const LatticeOpArgs& O = C.lat;
+template <typename T>
+__device__ __forceinline__ T fma_t(T a, T b, T c) {
+    if constexpr (sizeof(T) == 4) return fmaf(a, b, c);
+    else return fma(a, b, c);
+}
+
+// Row n = (i, j, k) of the lattice operator in T (double: the CG product;
+// float: the multigrid smoothing): y = sum over the existing cells at n of
+// s_c Khat[a_c][b] x_b with the class matrix K (Khat or its FP32 copy), and
+// cnt = the number of cells at n.  x is component-major with stride n_lat.
+template <typename T>
+__device__ __forceinline__ void lattice_row(const LatticeOpArgs& O, const T* __restrict__ in,
+                                            const T* __restrict__ Kc, int64_t n_lat, int i, int j, int k, bool valid,
+                                            T& y0, T& y1, T& y2, int& cnt) {
     const int64_t nxp = O.nx + 1, nyp = O.ny + 1;
-    const int64_t n_lat = nxp * nyp * (O.nz + 1);
-    const int64_t p = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
-    const bool valid = p < n_lat;
-    const int64_t pv = valid ? p : 0;
-    // 32-bit index arithmetic (the lattice has fewer than 2^31 points)
-    const uint32_t pu = uint32_t(pv), nxu = uint32_t(nxp), pj = pu / nxu;
-    const int i = int(pu - pj * nxu), j = int(pj % uint32_t(nyp)), k = int(pj / uint32_t(nyp));
-    const double* __restrict__ in = KIND == 0 ? xcg : (mode == MG_RESID_INIT ? L.x : din);
     // cells at n (c bit d: the cell extends towards +d from n - 1) and their
     // scales; a missing cell gets the scale 0 (its index is clamped into the
     // lattice so every load stays in bounds and unconditional)
-    double sc[8];
-    int cnt = 0;
+    T sc[8];
+    cnt = 0;
 #pragma unroll
     for (int c = 0; c < 8; c++) {
         const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
         const int ci = i - 1 + cx, cj = j - 1 + cy, ck = k - 1 + cz;
         const bool has = valid && ci >= 0 && ci < O.nx && cj >= 0 && cj < O.ny && ck >= 0 && ck < O.nz;
         const int cell = min(max(ci, 0), O.nx - 1) + O.nx * (min(max(cj, 0), O.ny - 1) + O.ny * min(max(ck, 0), O.nz - 1));
-        const double sv = O.sE_cell[cell];
-        sc[c] = has ? sv : 0.0;
+        const T sv = T(O.sE_cell[cell]);
+        sc[c] = has ? sv : T(0);
         cnt += has ? 1 : 0;
     }
     // neighbour-outer, one z-layer of 9 neighbours at a time (bounded live
@@ -282,27 +292,27 @@ __global__ __launch_bounds__(256) void k_lattice_apply(CGArgs C, const double* _
     // with n, y += s_c Khat[a_c][b] x_o (scaled per block product, so only y
     // is carried: fewer registers, more waves in flight to hide the gathers).
     // Neighbours outside the lattice are clamped into it; every cell holding
-    // one is missing and scaled by 0.
-    // Gathers address each component array from a uniform base with a 32-bit
-    // byte offset (fewer than 2^29 lattice points), and the products are
-    // explicit FMA chains: 9 FMAs per 3x3 block, 3 per (cell, layer) scaling.
+    // one is missing and scaled by 0.  Gathers address each component array
+    // from a uniform base with a 32-bit byte offset (fewer than 2^29 lattice
+    // points), and the products are explicit FMA chains: 9 FMAs per 3x3
+    // block, 3 per (cell, layer) scaling.
     const char* __restrict__ b0 = reinterpret_cast<const char*>(in);
     const char* __restrict__ b1 = reinterpret_cast<const char*>(in + n_lat);
     const char* __restrict__ b2 = reinterpret_cast<const char*>(in + 2 * n_lat);
     const uint32_t sx = uint32_t(nxp), sz = uint32_t(nxp * nyp);
-    double y0 = 0.0, y1 = 0.0, y2 = 0.0;
+    y0 = y1 = y2 = T(0);
 #pragma unroll 1
     for (int lz = 0; lz < 3; lz++) {
         const uint32_t mk = uint32_t(min(max(k + lz - 1, 0), O.nz));
-        double xv[9][3];
+        T xv[9][3];
 #pragma unroll
         for (int q = 0; q < 9; q++) {
             const uint32_t mi = uint32_t(min(max(i + (q % 3) - 1, 0), O.nx));
             const uint32_t mj = uint32_t(min(max(j + (q / 3) - 1, 0), O.ny));
-            const uint32_t off = (mi + sx * mj + sz * mk) * 8u;
-            xv[q][0] = *reinterpret_cast<const double*>(b0 + off);
-            xv[q][1] = *reinterpret_cast<const double*>(b1 + off);
-            xv[q][2] = *reinterpret_cast<const double*>(b2 + off);
+            const uint32_t off = (mi + sx * mj + sz * mk) * uint32_t(sizeof(T));
+            xv[q][0] = *reinterpret_cast<const T*>(b0 + off);
+            xv[q][1] = *reinterpret_cast<const T*>(b1 + off);
+            xv[q][2] = *reinterpret_cast<const T*>(b2 + off);
         }
 #pragma unroll
         for (int c = 0; c < 8; c++) {
@@ -310,23 +320,45 @@ __global__ __launch_bounds__(256) void k_lattice_apply(CGArgs C, const double* _
             const int bz = lz - cz;                      // uniform across the wave
             if (bz < 0 || bz > 1) continue;
             const int a = lat_corner(1 - cx, 1 - cy, 1 - cz);
-            double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+            T t0 = T(0), t1 = T(0), t2 = T(0);
 #pragma unroll
             for (int q = 0; q < 9; q++) {
                 const int bx = (q % 3) - cx, by = (q / 3) - cy;
                 if (bx < 0 || bx > 1 || by < 0 || by > 1) continue;   // compile time
                 const int b = bz * 4 + lat_corner(bx, by, 0);
-                const double* K = O.Khat + (a * 8 + b) * 9;
-                t0 = fma(K[0], xv[q][0], t0); t0 = fma(K[1], xv[q][1], t0); t0 = fma(K[2], xv[q][2], t0);
-                t1 = fma(K[3], xv[q][0], t1); t1 = fma(K[4], xv[q][1], t1); t1 = fma(K[5], xv[q][2], t1);
-                t2 = fma(K[6], xv[q][0], t2); t2 = fma(K[7], xv[q][1], t2); t2 = fma(K[8], xv[q][2], t2);
+                const T* K = Kc + (a * 8 + b) * 9;
+                t0 = fma_t(K[0], xv[q][0], t0); t0 = fma_t(K[1], xv[q][1], t0); t0 = fma_t(K[2], xv[q][2], t0);
+                t1 = fma_t(K[3], xv[q][0], t1); t1 = fma_t(K[4], xv[q][1], t1); t1 = fma_t(K[5], xv[q][2], t1);
+                t2 = fma_t(K[6], xv[q][0], t2); t2 = fma_t(K[7], xv[q][1], t2); t2 = fma_t(K[8], xv[q][2], t2);
             }
-            y0 = fma(sc[c], t0, y0);
-            y1 = fma(sc[c], t1, y1);
-            y2 = fma(sc[c], t2, y2);
+            y0 = fma_t(sc[c], t0, y0);
+            y1 = fma_t(sc[c], t1, y1);
+            y2 = fma_t(sc[c], t2, y2);
         }
     }
-    const int64_t r = pv;
+}
+
+__device__ __forceinline__ void lattice_point(const LatticeOpArgs& O, int64_t p, int& i, int& j, int& k) {
+    // 32-bit index arithmetic (the lattice has fewer than 2^29 points)
+    const uint32_t pu = uint32_t(p), nxu = uint32_t(O.nx + 1), pj = pu / nxu;
+    i = int(pu - pj * nxu);
+    j = int(pj % uint32_t(O.ny + 1));
+    k = int(pj / uint32_t(O.ny + 1));
+}
+
+// y = A x (FP64 CG operator) and partial sums of x.y into C.part0
+__global__ __launch_bounds__(256) void k_lattice_apply(CGArgs C, const double* __restrict__ in,
+                                                       double* __restrict__ ycg, const int* __restrict__ done) {
+    if (done && *done) return;
+    const LatticeOpArgs& O = C.lat;
+    const int64_t n_lat = int64_t(O.nx + 1) * (O.ny + 1) * (O.nz + 1);
+    const int64_t p = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    const bool valid = p < n_lat;
+    const int64_t r = valid ? p : 0;
+    int i, j, k, cnt;
+    lattice_point(O, r, i, j, k);
+    double y0, y1, y2;
+    lattice_row<double>(O, in, O.Khat, n_lat, i, j, k, valid, y0, y1, y2, cnt);
     double dsum = 0.0;
     if (valid) {
         const int m = O.nodecon[r];
@@ -334,20 +366,127 @@ __global__ __launch_bounds__(256) void k_lattice_apply(CGArgs C, const double* _
         if (m & 1) y0 = cnt * x0;
         if (m & 2) y1 = cnt * x1;
         if (m & 4) y2 = cnt * x2;
-        if (KIND == 0) {
-            ycg[r] = y0;
-            ycg[r + n_lat] = y1;
-            ycg[r + 2 * n_lat] = y2;
-            dsum = y0 * x0 + y1 * x1 + y2 * x2;
-        } else {
-            const double ith = L.cheb[0];
-            const double c1 = step > 0 ? L.cheb[2 * step] : 0.0, c2 = step > 0 ? L.cheb[2 * step + 1] : 0.0;
-            mg_epilogue(L, mode, r, y0, din, dout, c1, c2, ith);
-            mg_epilogue(L, mode, r + n_lat, y1, din, dout, c1, c2, ith);
-            mg_epilogue(L, mode, r + 2 * n_lat, y2, din, dout, c1, c2, ith);
+        ycg[r] = y0;
+        ycg[r + n_lat] = y1;
+        ycg[r + 2 * n_lat] = y2;
+        dsum = y0 * x0 + y1 * x1 + y2 * x2;
+    }
+    block_sum_store(dsum, C.part0, blockIdx.x);
+}
+
+// Chebyshev epilogue of one DOF of the FP32 problem-level state (see mg_epilogue)
+__device__ __forceinline__ void mg_epilogue32(const MGLevelArgs& L, int mode, int64_t i, float y,
+                                              const float* __restrict__ din, float* __restrict__ dout, float c1,
+                                              float c2, float ith) {
+    if (mode == MG_RESID_INIT) {
+        const float rs = float(L.r[i]) - y;
+        L.res32[i] = rs;
+        dout[i] = L.dinv32[i] * rs * ith;
+    } else {
+        const float dv = din[i];
+        L.x32[i] += dv;
+        const float rs = L.res32[i] - y;
+        L.res32[i] = rs;
+        if (mode == MG_STEP) dout[i] = c1 * dv + c2 * L.dinv32[i] * rs;
+    }
+}
+
+// Multigrid smoothing step on the hex8 problem level in FP32: the lattice
+// operator on the FP32 direction (MG_RESID_INIT: on the iterate) with the
+// row-local Chebyshev epilogue.  The smoother is part of the preconditioner;
+// the CG operator, its residuals and the convergence test stay FP64.
+__global__ __launch_bounds__(256) void k_lattice_smooth(LatticeOpArgs O, MGLevelArgs L, int mode, int step,
+                                                        const float* __restrict__ din, float* __restrict__ dout,
+                                                        const int* __restrict__ done) {
+    if (done && *done) return;
+    const int64_t n_lat = int64_t(O.nx + 1) * (O.ny + 1) * (O.nz + 1);
+    const int64_t p = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    if (p >= n_lat) return;   // no block-level synchronisation below
+    const float* __restrict__ in = mode == MG_RESID_INIT ? L.x32 : din;
+    int i, j, k, cnt;
+    lattice_point(O, p, i, j, k);
+    float y0, y1, y2;
+    lattice_row<float>(O, in, O.Khat32, n_lat, i, j, k, true, y0, y1, y2, cnt);
+    const int m = O.nodecon[p];
+    if (m & 1) y0 = float(cnt) * in[p];
+    if (m & 2) y1 = float(cnt) * in[p + n_lat];
+    if (m & 4) y2 = float(cnt) * in[p + 2 * n_lat];
+    const float ith = float(L.cheb[0]);
+    const float c1 = step > 0 ? float(L.cheb[2 * step]) : 0.f, c2 = step > 0 ? float(L.cheb[2 * step + 1]) : 0.f;
+    mg_epilogue32(L, mode, p, y0, din, dout, c1, c2, ith);
+    mg_epilogue32(L, mode, p + n_lat, y1, din, dout, c1, c2, ith);
+    mg_epilogue32(L, mode, p + 2 * n_lat, y2, din, dout, c1, c2, ith);
+}
+
+// Row information of the lattice operator without an assembled matrix: for
+// each row the assembled entries A_rj = sum over the cells shared by the two
+// nodes of s_c Khat[a_c][b_c], so the inverse diagonal and the Gershgorin
+// ratio sum_j |A_rj| / A_rr equal those of the assembled matrix (constrained
+// rows and columns as assembled: zero off the diagonal, cells-at-node on it).
+__global__ __launch_bounds__(256) void k_lattice_rowinfo(LatticeOpArgs O, double* __restrict__ dinv,
+                                                         float* __restrict__ dinv32, double* __restrict__ part) {
+    const int64_t nxp = O.nx + 1, nyp = O.ny + 1;
+    const int64_t n_lat = nxp * nyp * (O.nz + 1);
+    const int64_t p = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    double g = 0.0;
+    if (p < n_lat) {
+        int i, j, k;
+        lattice_point(O, p, i, j, k);
+        double sc[8];
+        int cnt = 0;
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+            const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
+            const int ci = i - 1 + cx, cj = j - 1 + cy, ck = k - 1 + cz;
+            const bool has = ci >= 0 && ci < O.nx && cj >= 0 && cj < O.ny && ck >= 0 && ck < O.nz;
+            const int cell = min(max(ci, 0), O.nx - 1) + O.nx * (min(max(cj, 0), O.ny - 1) + O.ny * min(max(ck, 0), O.nz - 1));
+            sc[c] = has ? O.sE_cell[cell] : 0.0;
+            cnt += has ? 1 : 0;
+        }
+        const int mn = O.nodecon[p];
+        double rs[3] = {0.0, 0.0, 0.0}, dg[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int d = 0; d < 27; d++) {
+            const int dx = d % 3 - 1, dy = (d / 3) % 3 - 1, dz = d / 9 - 1;
+            // neighbour o = n + (dx, dy, dz), clamped into the lattice (a
+            // neighbour outside lies only in missing cells: its block is 0)
+            const int oi = min(max(i + dx, 0), O.nx), oj = min(max(j + dy, 0), O.ny), ok = min(max(k + dz, 0), O.nz);
+            const int mo = O.nodecon[oi + nxp * (oj + nyp * int64_t(ok))];
+            double B[9];
+#pragma unroll
+            for (int e = 0; e < 9; e++) B[e] = 0.0;
+#pragma unroll
+            for (int c = 0; c < 8; c++) {
+                const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
+                // o lies in cell c when each offset is cell_d - 1 or cell_d
+                if (dx < cx - 1 || dx > cx || dy < cy - 1 || dy > cy || dz < cz - 1 || dz > cz) continue;
+                const int a = lat_corner(1 - cx, 1 - cy, 1 - cz);
+                const int b = lat_corner(dx - cx + 1, dy - cy + 1, dz - cz + 1);
+                const double* K = O.Khat + (a * 8 + b) * 9;
+#pragma unroll
+                for (int e = 0; e < 9; e++) B[e] = fma(sc[c], K[e], B[e]);
+            }
+#pragma unroll
+            for (int v = 0; v < 3; v++)
+#pragma unroll
+                for (int w = 0; w < 3; w++) {
+                    const double val = ((mn >> v) & 1) || ((mo >> w) & 1) ? 0.0 : B[v * 3 + w];
+                    rs[v] += fabs(val);
+                    if (d == 13 && v == w) dg[v] = val;
+                }
+        }
+#pragma unroll
+        for (int v = 0; v < 3; v++) {
+            // constrained rows hold only the diagonal (one per cell at the node)
+            const bool cv = (mn >> v) & 1;
+            const double diag = cv ? double(cnt) : dg[v];
+            const double di = diag > 0.0 ? 1.0 / diag : 0.0;
+            g = fmax(g, cv ? 1.0 : (diag > 0.0 ? rs[v] / diag : 0.0));
+            dinv[v * n_lat + p] = di;
+            dinv32[v * n_lat + p] = float(di);
         }
     }
-    if (KIND == 0) block_sum_store(dsum, C.part0, blockIdx.x);
+    block_max_store(g, part, blockIdx.x);
 }
 
 // node-vector permutation between rank order (node-major, 3 values per node)
@@ -488,20 +627,30 @@ __global__ __launch_bounds__(256) void k_mg_power_start_norm(const double* __res
 // Gauss-Jordan inverse held in LDS (n <= 136), one block
 __global__ __launch_bounds__(1024) void k_mg_invert_lds(int n, double* __restrict__ A) {
     __shared__ double sA[136 * 136];
-    __shared__ double colk[136];
+    __shared__ double colk[136], rowk[136];
     const int nn = n * n;
     for (int idx = threadIdx.x; idx < nn; idx += 1024) sA[idx] = A[idx];
+    // this thread's entries idx = t + 1024 m as (i, j), advanced without division
+    const int qi = 1024 / n, qj = 1024 % n;
+    const int i0 = int(threadIdx.x) / n, j0 = int(threadIdx.x) % n;
     __syncthreads();
     for (int k = 0; k < n; k++) {
-        for (int i = threadIdx.x; i < n; i += 1024) colk[i] = sA[i * n + k];
+        if (int(threadIdx.x) < n) colk[threadIdx.x] = sA[threadIdx.x * n + k];
         __syncthreads();
         const double p = 1.0 / colk[k];
-        for (int j = threadIdx.x; j < n; j += 1024) sA[k * n + j] = (j == k) ? p : sA[k * n + j] * p;
+        if (int(threadIdx.x) < n) {
+            const int j = threadIdx.x;
+            const double v = (j == k) ? p : sA[k * n + j] * p;
+            rowk[j] = v;
+            sA[k * n + j] = v;
+        }
         __syncthreads();
+        int i = i0, j = j0;
         for (int idx = threadIdx.x; idx < nn; idx += 1024) {
-            const int i = idx / n, j = idx - i * n;
-            if (i == k) continue;
-            sA[idx] = (j == k) ? -colk[i] * p : sA[idx] - colk[i] * sA[k * n + j];
+            if (i != k) sA[idx] = (j == k) ? -colk[i] * p : sA[idx] - colk[i] * rowk[j];
+            i += qi;
+            j += qj;
+            if (j >= n) { j -= n; i++; }
         }
         __syncthreads();
     }
@@ -547,7 +696,8 @@ __global__ __launch_bounds__(256) void k_mg_restrict(MGLevelArgs F, MGLevelArgs 
 #pragma unroll
         for (int v = 0; v < NAV; v++) {
             const int64_t rf = mg_fine_index<FL, NAV>(F, fp, v);
-            sv[v] += (w * (1.0 - double(F.con[rf]))) * F.res[rf];
+            const double res = FL == FL_PLANAR ? double(F.res32[rf]) : F.res[rf];
+            sv[v] += (w * (1.0 - double(F.con[rf]))) * res;
         }
     }
     // and the coarse level's pre-smoothing start (x = 0, res = r, first
@@ -594,7 +744,11 @@ __global__ __launch_bounds__(256) void k_mg_prolong(MGLevelArgs F, MGLevelArgs C
 #pragma unroll
     for (int v = 0; v < NAV; v++) {
         const int64_t rf = mg_fine_index<FL, NAV>(F, p, v);
-        if (!F.con[rf]) F.x[rf] += sv[v];
+        if (FL == FL_PLANAR) {
+            if (!F.con[rf]) F.x32[rf] += float(sv[v]);
+        } else {
+            if (!F.con[rf]) F.x[rf] += sv[v];
+        }
     }
 }
 
@@ -687,6 +841,18 @@ __global__ __launch_bounds__(256) void k_pcg_rz(CGArgs C, const double* __restri
     double rz = 0.0;
     for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < C.n; i += int64_t(gridDim.x) * 256) {
         const double z = dlast ? x0[i] + dlast[i] : x0[i];
+        C.z[i] = z;
+        rz += C.r[i] * z;
+    }
+    block_sum_store(rz, C.part0, blockIdx.x);
+}
+
+__global__ __launch_bounds__(256) void k_pcg_rz32(CGArgs C, const float* __restrict__ x0,
+                                                  const float* __restrict__ dlast) {
+    if (C.S->done) return;
+    double rz = 0.0;
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < C.n; i += int64_t(gridDim.x) * 256) {
+        const double z = dlast ? double(x0[i] + dlast[i]) : double(x0[i]);
         C.z[i] = z;
         rz += C.r[i] * z;
     }
@@ -828,17 +994,22 @@ int64_t lattice_partials(const LatticeOpArgs& l) {
     return (int64_t(l.nx + 1) * (l.ny + 1) * (l.nz + 1) + 255) / 256;
 }
 void launch_lattice_apply(const CGArgs& c, const double* x, double* y, hipStream_t s) {
-    MGLevelArgs none{};
     const int* done = c.S ? &c.S->done : nullptr;
-    hipLaunchKernelGGL(k_lattice_apply<0>, dim3(unsigned(lattice_partials(c.lat))), dim3(256), 0, s, c, x, y, none, 0,
-                       0, nullptr, nullptr, done);
+    hipLaunchKernelGGL(k_lattice_apply, dim3(unsigned(lattice_partials(c.lat))), dim3(256), 0, s, c, x, y, done);
+}
+void launch_lattice_rowinfo(const LatticeOpArgs& l, double* dinv, float* dinv32, double* part, double* lam,
+                            hipStream_t s) {
+    const int nb = int(lattice_partials(l));
+    hipLaunchKernelGGL(k_lattice_rowinfo, dim3(nb), dim3(256), 0, s, l, dinv, dinv32, part);
+    hipLaunchKernelGGL(k_mg_max, dim3(1), dim3(256), 0, s, part, nb, lam);
 }
 void launch_perm_nodes(int64_t n_nodes, const int32_t* map, const double* src, double* dst, int scatter,
                        hipStream_t s) {
     hipLaunchKernelGGL(k_perm_nodes, dim3(mg_blocks(n_nodes, 256)), dim3(256), 0, s, n_nodes, map, src, dst, scatter);
 }
 void launch_mg_pre_init(const MGLevelArgs& L, const int* done, hipStream_t s) {
-    hipLaunchKernelGGL(k_mg_pre_init, dim3(mg_grid(L.n_nodes * L.nav)), dim3(256), 0, s, L, done);
+    if (L.planar) hipLaunchKernelGGL(k_mg_pre_init32, dim3(mg_grid(L.n_nodes * L.nav)), dim3(256), 0, s, L, done);
+    else hipLaunchKernelGGL(k_mg_pre_init, dim3(mg_grid(L.n_nodes * L.nav)), dim3(256), 0, s, L, done);
 }
 void launch_mg_bsr(const MGLevelArgs& L, int mode, int step, const double* din, double* dout, const int* done,
                    hipStream_t s) {
@@ -860,10 +1031,10 @@ void launch_mg_bsr(const MGLevelArgs& L, int mode, int step, const double* din, 
             hipLaunchKernelGGL((k_mg_bsr<double, 3>), dim3(g), dim3(256), 0, s, L, L.val, mode, step, din, dout, done);
     }
 }
-void launch_mg_fine(const CGArgs& c, const MGLevelArgs& L, int mode, int step, const double* din,
-                    double* dout, const int* done, hipStream_t s) {
-    hipLaunchKernelGGL(k_lattice_apply<1>, dim3(unsigned(lattice_partials(c.lat))), dim3(256), 0, s, c, nullptr,
-                       nullptr, L, mode, step, din, dout, done);
+void launch_mg_fine(const CGArgs& c, const MGLevelArgs& L, int mode, int step, const float* din, float* dout,
+                    const int* done, hipStream_t s) {
+    hipLaunchKernelGGL(k_lattice_smooth, dim3(unsigned(lattice_partials(c.lat))), dim3(256), 0, s, c.lat, L, mode,
+                       step, din, dout, done);
 }
 void launch_mg_restrict(const MGLevelArgs& F, const MGLevelArgs& C, const int* done, hipStream_t s) {
     const dim3 g(mg_blocks(C.n_nodes, 256));
@@ -905,6 +1076,9 @@ void launch_pcg_update(const CGArgs& c, hipStream_t s) {
 }
 void launch_pcg_rz(const CGArgs& c, const double* x0, const double* dlast, hipStream_t s) {
     hipLaunchKernelGGL(k_pcg_rz, dim3(c.nvec), dim3(256), 0, s, c, x0, dlast);
+}
+void launch_pcg_rz32(const CGArgs& c, const float* x0, const float* dlast, hipStream_t s) {
+    hipLaunchKernelGGL(k_pcg_rz32, dim3(c.nvec), dim3(256), 0, s, c, x0, dlast);
 }
 void launch_mg_combine(int64_t n, const double* x0, const double* dlast, double* out, const int* done,
                        hipStream_t s) {

@@ -36,14 +36,15 @@ def test_mg_design_eval(oracle, mastg, mesh, lo):
 
 @pytest.mark.parametrize("steps", [None, (3, 3), (2, 4)])
 def test_mg_iterations_match_restatement(oracle, mastg, steps):
-    """Default smoothing (4 steps on the problem level, 2 on coarse levels) and
-    two settings through the options."""
+    """Default smoothing (5 steps on the problem level, 2 on coarse levels) and
+    two settings through the options.  The problem level smooths in FP32 (the
+    restatement in FP64): iteration counts agree to +-1."""
     print(f"[mg vs scipy restatement] 16x8x8 steps={steps}", flush=True)
     mesh = (16, 8, 8)
     P = oracle.Problem(oracle.CANTILEVER, *mesh, seed=3, lo=0.01, hi=1.0)
     levels, b = MR.build_levels(P, mesh, np.array([1.0 / 16, 0.5 / 8, 0.5 / 8]))
     assert [L["A"].shape[0] for L in levels] == [4131, 675, 135]
-    it_ref, _ = MR.pcg_iterations(levels, b, k=steps or (4, 2))
+    it_ref, _ = MR.pcg_iterations(levels, b, k=steps or (5, 2))
     ctx = ctx_from_oracle(mastg, P)
     if steps:
         ctx.set_option(mastg.OPT_MG_STEPS_FINE, steps[0])
@@ -102,6 +103,29 @@ def test_mg_refuses_non_lattice(oracle, mastg):
     ch.close()
 
 
+def test_mg_design_eval_matrix_free_state(oracle, mastg):
+    """The hex8 multigrid design evaluation forms no assembled matrix: CSR
+    products refuse until mastg_assemble; a later assembly is for the same
+    parameters and the evaluation's results do not depend on it."""
+    print("[mg matrix-free state] 16x8x8", flush=True)
+    P = oracle.Problem(oracle.CANTILEVER, 16, 8, 8, seed=6, lo=0.01, hi=1.0)
+    ctx = ctx_from_oracle(mastg, P)
+    r1 = ctx.design_eval(P.params(), solver=mastg.SOLVER_MG)
+    x = np.random.default_rng(3).standard_normal(P.n_dofs)
+    with pytest.raises(mastg.MastgError):
+        ctx.jacobian_product(x, op=0)
+    ctx.assemble()
+    _, con = P.dofmap()
+    xm = np.where(con == 1, 0.0, x)
+    y0 = ctx.jacobian_product(xm, op=0)
+    y3 = ctx.jacobian_product(x, op=3)
+    assert np.max(np.abs(y3 - y0)) <= 1e-12 * np.max(np.abs(y0))
+    r2 = ctx.design_eval(P.params(), solver=mastg.SOLVER_MG)
+    assert r2["objective"] == r1["objective"] and r2["iters"] == r1["iters"]
+    assert np.array_equal(r2["grad"], r1["grad"])
+    ctx.close()
+
+
 @pytest.mark.parametrize("mesh", [(8, 4, 4), (16, 8, 8)])
 def test_lattice_operator_equals_csr(oracle, mastg, mesh):
     """The per-node lattice operator (multigrid level 0 and its CG product)
@@ -120,6 +144,17 @@ def test_lattice_operator_equals_csr(oracle, mastg, mesh):
     ctx.close()
 
 
+def test_mg_plate_switched_off(oracle, mastg):
+    """Multigrid is refused for the Mindlin plate, before any assembly."""
+    P = oracle.Problem(oracle.PLATE, 8, 8, seed=1)
+    ctx = ctx_from_oracle(mastg, P)
+    with pytest.raises(mastg.MastgError, match="switched off"):
+        ctx.design_eval(P.params(), solver=mastg.SOLVER_MG)
+    ctx.close()
+
+
+@pytest.mark.skip(reason="plate multigrid is switched off: a device memory fault of undetermined "
+                         "cause ended a run of this test; it stays off the GPU until the cause is shown")
 @pytest.mark.parametrize("n", [16, 24])
 def test_mg_plate_design_eval(oracle, mastg, n):
     """Mindlin plate: bilinear transfers of all 5 variables, Galerkin 5x5-block
