@@ -60,6 +60,8 @@ struct MGLevelDev {
     DevBuf rank_of_lat, elem_of_cell, con_lat, nodecon_lat, dinv_lat, b_lat, sE_cell;   // level 0
     DevBuf x32, res32, da32, db32, dinv32, y_lat;   // hex8 level 0: FP32 smoothing state, FP64 scratch
     DevBuf blk_ptr, blk_col, cptr, cidx, econn, con;   // levels >= 1
+    DevBuf gal_dirty, gal_list;   // hex8 level 1: coarse cells touching constraints
+    int64_t n_dirty = 0;
     DevBuf Kel, val, val32, dinv, r, x, res, da, db;
 };
 
@@ -78,6 +80,7 @@ struct mastg_ctx {
     // matrix-free operator data (hex8 congruence classes)
     DevBuf Khat, sE, eclass, nodecon, Fel;
     DevBuf Khat32;              // FP32 copy of Khat (lattice smoothing)
+    DevBuf Gc;                  // [8][576] Q_c^T Khat Q_c (lattice Galerkin products)
     // heat: convective side masks, mean-T weights, BiCGStab work space, A^T values
     DevBuf sidemask, wmean, tpos, valT, vr0, vv, vs, vt, vph, vsh, bscal, vT;
     double volume = 0;
@@ -673,6 +676,23 @@ int mg_build(mastg_ctx* c) {
         }
         lv.push_back(std::move(L));
     }
+    if (!plate) {
+        // one-class lattice: the children's Galerkin products of the class
+        // matrix, and the first coarse level's cells that touch constraints
+        std::vector<double> Kh(64 * 9), G(8 * 576);
+        {
+            HipLock l(g_hip_mu);
+            HIPC(c, hipMemcpy(Kh.data(), c->Khat.p, Kh.size() * sizeof(double), hipMemcpyDeviceToHost));
+        }
+        mg_class_child_products(Kh.data(), G.data());
+        TRY(upload(c, c->Gc, G));
+        std::vector<uint8_t> dirty;
+        std::vector<int32_t> list;
+        mg_dirty_coarse(H.levels[0], H.levels[1], c->S.constrained, dirty, list);
+        TRY(upload(c, lv[1]->gal_dirty, dirty));
+        if (!list.empty()) TRY(upload(c, lv[1]->gal_list, list));
+        lv[1]->n_dirty = int64_t(list.size());
+    }
     TRY(dalloc(c, c->mg_cheb, size_t(nl) * kMGChebStride * sizeof(double)));
     TRY(dalloc(c, c->mg_lam, size_t(nl) * sizeof(double)));
     TRY(dalloc(c, c->mg_part, size_t(std::max<int64_t>(8192, (c->S.n_active + 255) / 256 + 1)) * sizeof(double)));
@@ -732,7 +752,17 @@ int mg_setup(mastg_ctx* c) {
         g.Khat = l == 0 && lat ? c->Khat.as<double>() : nullptr;
         g.sE_cell = l == 0 && lat ? L0.sE_cell.as<double>() : nullptr;
         g.Kc = C.Kel.as<double>();
-        launch_mg_galerkin(g, s);
+        if (l == 0 && lat) {
+            // cells clear of constraints from the class products, the rest masked
+            g.Gc = c->Gc.as<double>();
+            g.dirty = C.gal_dirty.as<uint8_t>();
+            launch_mg_galerkin_class(g, s);
+            g.elems = C.gal_list.as<int32_t>();
+            g.n_elems_list = C.n_dirty;
+            if (C.n_dirty > 0) launch_mg_galerkin(g, s);
+        } else {
+            launch_mg_galerkin(g, s);
+        }
         ScatterArgs sa{C.a.n_nodes, nav, C.a.blk_ptr, C.cptr.as<int32_t>(), C.cidx.as<int32_t>(),
                        C.Kel.as<double>(), C.val.as<double>()};
         launch_scatter(sa, s);
@@ -771,7 +801,9 @@ bool lattice_args(mastg_ctx* c, CGArgs& a) {
 // one V-cycle on the problem-level vector r0 (null: the level-0 args' r); the
 // level-0 output is x0 + dlast0 (formed by the caller): the returned dlast0
 // is FP32 on the hex8 problem level (its smoothing state), else FP64
-const void* mg_vcycle(mastg_ctx* c, const CGArgs& a, const int* done, hipStream_t s, const double* r0 = nullptr) {
+// (started: the caller has written the problem level's pre-smoothing start)
+const void* mg_vcycle(mastg_ctx* c, const CGArgs& a, const int* done, hipStream_t s, const double* r0 = nullptr,
+                      bool started = false) {
     const int nl = int(c->mg.size());
     MGLevelArgs L0 = c->mg[0]->a;
     if (r0) L0.r = const_cast<double*>(r0);
@@ -792,7 +824,7 @@ const void* mg_vcycle(mastg_ctx* c, const CGArgs& a, const int* done, hipStream_
     for (int l = 0; l + 1 < nl; l++) {
         const int k = steps(l);
         // coarse levels start their pre-smoothing inside the restriction
-        if (l == 0) launch_mg_pre_init(lv(l), done, s);
+        if (l == 0 && !started) launch_mg_pre_init(lv(l), done, s);
         for (int i = 1; i < k; i++) apply(l, MG_STEP, i, i - 1, i);
         apply(l, MG_FINAL, 0, k - 1, -1);
         launch_mg_restrict(lv(l), lv(l + 1), done, s);
@@ -821,8 +853,10 @@ void mg_rz(mastg_ctx* c, const CGArgs& a, const void* dl, hipStream_t s) {
 void mg_pcg_iteration(mastg_ctx* c, const CGArgs& a, hipStream_t s) {
     const int* done = &a.S->done;
     launch_cg_op_alpha(a, s);
-    launch_pcg_update(a, s);
-    mg_rz(c, a, mg_vcycle(c, a, done, s), s);
+    const MGLevelArgs& L0 = c->mg[0]->a;
+    if (L0.planar) launch_pcg_update_mg(a, L0, s);   // also starts the level-0 smoothing
+    else launch_pcg_update(a, s);
+    mg_rz(c, a, mg_vcycle(c, a, done, s, nullptr, L0.planar != 0), s);
     launch_pcg_beta(a, s);
     launch_cg_p(a, s);
 }

@@ -219,6 +219,13 @@ struct MGGalerkinArgs {                // coarse element matrices = sum_c Q_c^T 
     const double* Khat;                // (non-null) fine blocks sE_cell[cell] Khat: one
     const double* sE_cell;             // element class on a lattice, scales by cell
     double* Kc;                        // [n_coarse][npe^2][nav^2]
+    // one-class lattice (hex8): Gc[c] = Q_c^T Khat Q_c for the 8 children, so a
+    // coarse element with no constrained DOF among its 27 fine nodes (dirty[E]
+    // = 0) is sum_c s_c Gc[c]; the masked product handles the listed others
+    const double* Gc;                  // [8][576]
+    const uint8_t* dirty;              // [n_coarse]
+    const int32_t* elems;              // (non-null) the masked product runs on these
+    int64_t n_elems_list;              //   n_elems_list coarse elements only
 };
 
 // smoothing modes (row-local epilogue after y = A in):
@@ -228,6 +235,8 @@ struct MGGalerkinArgs {                // coarse element matrices = sum_c Q_c^T 
 enum { MG_STEP = 1, MG_FINAL = 2, MG_RESID_INIT = 3 };
 
 void launch_mg_galerkin(const MGGalerkinArgs& g, hipStream_t s);
+// the coarse elements with dirty[E] = 0 from g.Gc and g.sE_cell (hex8)
+void launch_mg_galerkin_class(const MGGalerkinArgs& g, hipStream_t s);
 // dinv (levels >= 1, from the node-block values) and the Gershgorin bound
 // max_i sum_j |a_ij| / a_ii into *lam; level 0 reads the scalar CSR
 void launch_mg_cheb(const double* lam, double* cheb, int n_levels, int k, double ratio, hipStream_t s);
@@ -267,6 +276,8 @@ void launch_mg_coarse_solve(int n, const double* Ainv, const double* r, double* 
                             hipStream_t s);
 // preconditioned CG around the V-cycle
 void launch_pcg_update(const CGArgs& c, hipStream_t s);
+// the same, also starting the pre-smoothing of the FP32 problem level L (hex8)
+void launch_pcg_update_mg(const CGArgs& c, const MGLevelArgs& L, hipStream_t s);
 void launch_pcg_rz(const CGArgs& c, const double* x0, const double* dlast, hipStream_t s);
 // the same from the FP32 problem-level state of the hex8 V-cycle
 void launch_pcg_rz32(const CGArgs& c, const float* x0, const float* dlast, hipStream_t s);

@@ -47,9 +47,11 @@ __global__ __launch_bounds__(256) void k_mg_galerkin(MGGalerkinArgs G) {
     const bool lane_on = lane < NPE * NPE;
     const int ln = lane_on ? lane : 0;
     const int hi = ln / NPE, lo = ln % NPE;
-    const int64_t E = int64_t(blockIdx.x) * 4 + wave;
-    const bool valid = E < G.n_coarse;
-    const int64_t Ev = valid ? E : 0;
+    // coarse element of this wave: the listed ones, or all of them
+    const int64_t w = int64_t(blockIdx.x) * 4 + wave;
+    const bool valid = G.elems ? w < G.n_elems_list : w < G.n_coarse;
+    const int64_t E = !valid ? 0 : (G.elems ? int64_t(G.elems[w]) : w);
+    const int64_t Ev = E;
     const int64_t I = Ev % G.cnx, J = (Ev / G.cnx) % G.cny, K = Ev / (int64_t(G.cnx) * G.cny);
     double acc[NB];
 #pragma unroll
@@ -131,6 +133,31 @@ __global__ __launch_bounds__(256) void k_mg_galerkin(MGGalerkinArgs G) {
             if (G.c_con[int64_t(cA) * NAV + v] || G.c_con[int64_t(cB) * NAV + w]) x = (hi == lo && v == w) ? 1.0 : 0.0;
             o[v * NAV + w] = x;
         }
+}
+
+// Coarse elements of a one-class hex8 lattice with no constrained DOF among
+// their 27 fine nodes: K_E = sum_c s_c Gc[c] (Gc exactly symmetric, so K_E is);
+// one wave per coarse element, 9 of the 576 entries per lane, coalesced stores.
+__global__ __launch_bounds__(256) void k_mg_galerkin_class(MGGalerkinArgs G) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t E = int64_t(blockIdx.x) * 4 + wave;
+    if (E >= G.n_coarse || G.dirty[E]) return;
+    const int64_t I = E % G.cnx, J = (E / G.cnx) % G.cny, K = E / (int64_t(G.cnx) * G.cny);
+    double sc[8];
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+        const int64_t fi = 2 * I + (c & 1), fj = 2 * J + ((c >> 1) & 1), fk = 2 * K + (c >> 2);
+        sc[c] = G.sE_cell[fi + int64_t(G.fnx) * (fj + int64_t(G.fny) * fk)];
+    }
+    double* __restrict__ o = G.Kc + E * 576;
+#pragma unroll
+    for (int t = 0; t < 9; t++) {
+        const int i = lane + 64 * t;
+        double acc = 0.0;
+#pragma unroll
+        for (int c = 0; c < 8; c++) acc = fma(sc[c], G.Gc[c * 576 + i], acc);
+        o[i] = acc;
+    }
 }
 
 // ---- row information and eigenvalue bounds ----------------------------------
@@ -624,37 +651,61 @@ __global__ __launch_bounds__(256) void k_mg_power_start_norm(const double* __res
     if (threadIdx.x == 0) scal[1] = sqrt(s);
 }
 
-// Gauss-Jordan inverse held in LDS (n <= 136), one block
+// Gauss-Jordan inverse of an SPD matrix (no pivoting), n <= 136, one block of
+// 1024 threads.  Thread t keeps the 9 x 3 entries at rows (t >> 6) + 16 r and
+// columns (t & 63) + 64 c in registers (a 144 x 192 cover of the matrix); per
+// pivot only the pivot column and the scaled pivot row pass through LDS.
 __global__ __launch_bounds__(1024) void k_mg_invert_lds(int n, double* __restrict__ A) {
-    __shared__ double sA[136 * 136];
-    __shared__ double colk[136], rowk[136];
-    const int nn = n * n;
-    for (int idx = threadIdx.x; idx < nn; idx += 1024) sA[idx] = A[idx];
-    // this thread's entries idx = t + 1024 m as (i, j), advanced without division
-    const int qi = 1024 / n, qj = 1024 % n;
-    const int i0 = int(threadIdx.x) / n, j0 = int(threadIdx.x) % n;
-    __syncthreads();
+    __shared__ double colk[144], rowk[192];
+    const int t = threadIdx.x, i0 = t >> 6, j0 = t & 63;
+    double a[9][3];
+#pragma unroll
+    for (int r = 0; r < 9; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const int i = i0 + 16 * r, j = j0 + 64 * c;
+            a[r][c] = (i < n && j < n) ? A[i * n + j] : 0.0;
+        }
     for (int k = 0; k < n; k++) {
-        if (int(threadIdx.x) < n) colk[threadIdx.x] = sA[threadIdx.x * n + k];
+#pragma unroll
+        for (int r = 0; r < 9; r++)
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+                if (j0 + 64 * c == k && i0 + 16 * r < n) colk[i0 + 16 * r] = a[r][c];
         __syncthreads();
         const double p = 1.0 / colk[k];
-        if (int(threadIdx.x) < n) {
-            const int j = threadIdx.x;
-            const double v = (j == k) ? p : sA[k * n + j] * p;
-            rowk[j] = v;
-            sA[k * n + j] = v;
-        }
+#pragma unroll
+        for (int r = 0; r < 9; r++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const int j = j0 + 64 * c;
+                if (i0 + 16 * r == k && j < n) {
+                    a[r][c] = (j == k) ? p : a[r][c] * p;
+                    rowk[j] = a[r][c];
+                }
+            }
         __syncthreads();
-        int i = i0, j = j0;
-        for (int idx = threadIdx.x; idx < nn; idx += 1024) {
-            if (i != k) sA[idx] = (j == k) ? -colk[i] * p : sA[idx] - colk[i] * rowk[j];
-            i += qi;
-            j += qj;
-            if (j >= n) { j -= n; i++; }
-        }
+        double ck[9], rk[3];
+#pragma unroll
+        for (int r = 0; r < 9; r++) ck[r] = i0 + 16 * r < n ? colk[i0 + 16 * r] : 0.0;
+#pragma unroll
+        for (int c = 0; c < 3; c++) rk[c] = j0 + 64 * c < n ? rowk[j0 + 64 * c] : 0.0;
+#pragma unroll
+        for (int r = 0; r < 9; r++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const int i = i0 + 16 * r, j = j0 + 64 * c;
+                if (i < n && j < n && i != k) a[r][c] = (j == k) ? -ck[r] * p : a[r][c] - ck[r] * rk[c];
+            }
         __syncthreads();
     }
-    for (int idx = threadIdx.x; idx < nn; idx += 1024) A[idx] = sA[idx];
+#pragma unroll
+    for (int r = 0; r < 9; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const int i = i0 + 16 * r, j = j0 + 64 * c;
+            if (i < n && j < n) A[i * n + j] = a[r][c];
+        }
 }
 
 // ---- transfers ---------------------------------------------------------------
@@ -677,9 +728,10 @@ __global__ __launch_bounds__(256) void k_mg_restrict(MGLevelArgs F, MGLevelArgs 
     if (done && *done) return;
     const int64_t p = int64_t(blockIdx.x) * 256 + threadIdx.x;
     if (p >= Cl.n_nodes) return;
-    const int64_t cx = Cl.nx + 1, cy = Cl.ny + 1;
-    const int64_t I = p % cx, J = (p / cx) % cy, K = p / (cx * cy);
-    const int64_t fx = F.nx + 1, fy = F.ny + 1;
+    // 32-bit index arithmetic (lattice point counts are below 2^31)
+    const uint32_t cx = uint32_t(Cl.nx + 1), cy = uint32_t(Cl.ny + 1), pu = uint32_t(p), pq = pu / cx;
+    const int I = int(pu - pq * cx), J = int(pq % cy), K = int(pq / cy);
+    const uint32_t fx = uint32_t(F.nx + 1), fy = uint32_t(F.ny + 1);
     constexpr int NO = DIM == 3 ? 27 : 9;
     // all loads are independent (unrolled, predicated) so they are in flight together
     double sv[NAV];
@@ -688,11 +740,11 @@ __global__ __launch_bounds__(256) void k_mg_restrict(MGLevelArgs F, MGLevelArgs 
 #pragma unroll
     for (int o = 0; o < NO; o++) {
         const int oi = (o % 3) - 1, oj = ((o / 3) % 3) - 1, ok = DIM == 3 ? (o / 9) - 1 : 0;
-        const int64_t fi = 2 * I + oi, fj = 2 * J + oj, fk = 2 * K + ok;
+        const int fi = 2 * I + oi, fj = 2 * J + oj, fk = 2 * K + ok;
         const bool in_lat = fi >= 0 && fi <= F.nx && fj >= 0 && fj <= F.ny && fk >= 0 && fk <= F.nz;
         // out-of-lattice points are clamped (valid address) and weighted 0
         const double w = in_lat ? (ok ? 0.5 : 1.0) * (oj ? 0.5 : 1.0) * (oi ? 0.5 : 1.0) : 0.0;
-        const int64_t fp = (in_lat ? fi : 0) + fx * ((in_lat ? fj : 0) + fy * (in_lat ? fk : 0));
+        const uint32_t fp = uint32_t(in_lat ? fi : 0) + fx * (uint32_t(in_lat ? fj : 0) + fy * uint32_t(in_lat ? fk : 0));
 #pragma unroll
         for (int v = 0; v < NAV; v++) {
             const int64_t rf = mg_fine_index<FL, NAV>(F, fp, v);
@@ -721,9 +773,10 @@ __global__ __launch_bounds__(256) void k_mg_prolong(MGLevelArgs F, MGLevelArgs C
     if (done && *done) return;
     const int64_t p = int64_t(blockIdx.x) * 256 + threadIdx.x;
     if (p >= F.n_nodes) return;
-    const int64_t fx = F.nx + 1, fy = F.ny + 1;
-    const int64_t fi = p % fx, fj = (p / fx) % fy, fk = p / (fx * fy);
-    const int64_t cx = Cl.nx + 1, cy = Cl.ny + 1;
+    // 32-bit index arithmetic (lattice point counts are below 2^31)
+    const uint32_t fx = uint32_t(F.nx + 1), fy = uint32_t(F.ny + 1), pu = uint32_t(p), pq = pu / fx;
+    const int fi = int(pu - pq * fx), fj = int(pq % fy), fk = int(pq / fy);
+    const uint32_t cx = uint32_t(Cl.nx + 1), cy = uint32_t(Cl.ny + 1);
     constexpr int NQ = 1 << DIM;
     double sv[NAV];
 #pragma unroll
@@ -735,9 +788,9 @@ __global__ __launch_bounds__(256) void k_mg_prolong(MGLevelArgs F, MGLevelArgs C
         // neighbours; the unused corner of an even index is clamped and weighted 0
         const bool use = !((a && !(fi & 1)) || (b && !(fj & 1)) || (c && !(fk & 1)));
         const double w = use ? ((fi & 1) ? 0.5 : 1.0) * ((fj & 1) ? 0.5 : 1.0) * ((fk & 1) ? 0.5 : 1.0) : 0.0;
-        const int64_t CI = min((fi >> 1) + a, int64_t(Cl.nx)), CJ = min((fj >> 1) + b, int64_t(Cl.ny)),
-                      CK = min((fk >> 1) + c, int64_t(Cl.nz));
-        const int64_t rc = (CI + cx * (CJ + cy * CK)) * NAV;
+        const uint32_t CI = uint32_t(min((fi >> 1) + a, Cl.nx)), CJ = uint32_t(min((fj >> 1) + b, Cl.ny)),
+                       CK = uint32_t(min((fk >> 1) + c, Cl.nz));
+        const int64_t rc = int64_t(CI + cx * (CJ + cy * CK)) * NAV;
 #pragma unroll
         for (int v = 0; v < NAV; v++) sv[v] += w * (dlast ? Cl.x[rc + v] + dlast[rc + v] : Cl.x[rc + v]);
     }
@@ -816,6 +869,27 @@ __global__ __launch_bounds__(256) void k_pcg_update(CGArgs C) {
         const double r = C.r[i] - a * C.q[i];
         C.r[i] = r;
         rr += r * r;
+    }
+    block_sum_store(rr, C.part1, blockIdx.x);
+}
+
+// the same update, with the start of the hex8 problem level's FP32
+// pre-smoothing (x = 0, res = r, first direction D^-1 r / theta) written from
+// the updated residual (the V-cycle's own start kernel is then skipped)
+__global__ __launch_bounds__(256) void k_pcg_update_mg(CGArgs C, MGLevelArgs L) {
+    if (C.S->done) return;
+    const double a = C.S->alpha;
+    const float ith = float(L.cheb[0]);
+    double rr = 0.0;
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < C.n; i += int64_t(gridDim.x) * 256) {
+        C.x[i] += a * C.p[i];
+        const double r = C.r[i] - a * C.q[i];
+        C.r[i] = r;
+        rr += r * r;
+        const float rf = float(r);
+        L.x32[i] = 0.f;
+        L.res32[i] = rf;
+        L.da32[i] = L.dinv32[i] * rf * ith;
     }
     block_sum_store(rr, C.part1, blockIdx.x);
 }
@@ -930,13 +1004,19 @@ static inline unsigned mg_grid(int64_t n) {
 }
 
 void launch_mg_galerkin(const MGGalerkinArgs& g, hipStream_t s) {
-    if (g.n_coarse <= 0) return;
+    const int64_t n = g.elems ? g.n_elems_list : g.n_coarse;
+    if (n <= 0) return;
+    const dim3 grid(mg_blocks(n, 4));
     if (g.dim == 2)
-        hipLaunchKernelGGL((k_mg_galerkin<2, 5>), dim3(mg_blocks(g.n_coarse, 4)), dim3(256), 0, s, g);
+        hipLaunchKernelGGL((k_mg_galerkin<2, 5>), grid, dim3(256), 0, s, g);
     else if (g.nav == 1)
-        hipLaunchKernelGGL((k_mg_galerkin<3, 1>), dim3(mg_blocks(g.n_coarse, 4)), dim3(256), 0, s, g);
+        hipLaunchKernelGGL((k_mg_galerkin<3, 1>), grid, dim3(256), 0, s, g);
     else
-        hipLaunchKernelGGL((k_mg_galerkin<3, 3>), dim3(mg_blocks(g.n_coarse, 4)), dim3(256), 0, s, g);
+        hipLaunchKernelGGL((k_mg_galerkin<3, 3>), grid, dim3(256), 0, s, g);
+}
+void launch_mg_galerkin_class(const MGGalerkinArgs& g, hipStream_t s) {
+    if (g.n_coarse <= 0) return;
+    hipLaunchKernelGGL(k_mg_galerkin_class, dim3(mg_blocks(g.n_coarse, 4)), dim3(256), 0, s, g);
 }
 void launch_mg_cheb(const double* lam, double* cheb, int n_levels, int k, double ratio, hipStream_t s) {
     hipLaunchKernelGGL(k_mg_cheb, dim3(1), dim3(64), 0, s, lam, cheb, n_levels, k, ratio);
@@ -1072,6 +1152,10 @@ void launch_mg_coarse_solve(int n, const double* Ainv, const double* r, double* 
 }
 void launch_pcg_update(const CGArgs& c, hipStream_t s) {
     hipLaunchKernelGGL(k_pcg_update, dim3(c.nvec), dim3(256), 0, s, c);
+    hipLaunchKernelGGL(k_pcg_check, dim3(1), dim3(256), 0, s, c);
+}
+void launch_pcg_update_mg(const CGArgs& c, const MGLevelArgs& L, hipStream_t s) {
+    hipLaunchKernelGGL(k_pcg_update_mg, dim3(c.nvec), dim3(256), 0, s, c, L);
     hipLaunchKernelGGL(k_pcg_check, dim3(1), dim3(256), 0, s, c);
 }
 void launch_pcg_rz(const CGArgs& c, const double* x0, const double* dlast, hipStream_t s) {

@@ -145,4 +145,63 @@ MGHierarchy build_mg_hierarchy(const Structure& S) {
     return H;
 }
 
+void mg_class_child_products(const double* Khat, double* Gc) {
+    // tri-linear weight of coarse corner A at fine local node a of child c
+    auto q = [](int c, int a, int A) {
+        double w = 1.0;
+        for (int d = 0; d < 3; d++) {
+            const int t = ((c >> d) & 1) + kCorner[a][d];
+            w *= kCorner[A][d] ? 0.5 * t : 1.0 - 0.5 * t;
+        }
+        return w;
+    };
+    for (int c = 0; c < 8; c++) {
+        double* G = Gc + c * 576;
+        double Q[8][8];
+        for (int a = 0; a < 8; a++)
+            for (int A = 0; A < 8; A++) Q[a][A] = q(c, a, A);
+        for (int A = 0; A < 8; A++)
+            for (int B = 0; B < 8; B++)
+                for (int v = 0; v < 3; v++)
+                    for (int w = 0; w < 3; w++) {
+                        double s = 0.0;
+                        for (int a = 0; a < 8; a++) {
+                            if (Q[a][A] == 0.0) continue;
+                            double t = 0.0;
+                            for (int b = 0; b < 8; b++) t += Khat[(a * 8 + b) * 9 + v * 3 + w] * Q[b][B];
+                            s += Q[a][A] * t;
+                        }
+                        G[(A * 8 + B) * 9 + v * 3 + w] = s;
+                    }
+        // exact symmetry: row (A, v) > column (B, w) takes its mirror entry
+        for (int A = 0; A < 8; A++)
+            for (int B = 0; B < 8; B++)
+                for (int v = 0; v < 3; v++)
+                    for (int w = 0; w < 3; w++)
+                        if (A * 3 + v > B * 3 + w) G[(A * 8 + B) * 9 + v * 3 + w] = G[(B * 8 + A) * 9 + w * 3 + v];
+    }
+}
+
+void mg_dirty_coarse(const MGLevelHost& F, const MGLevelHost& C, const std::vector<uint8_t>& constrained,
+                     std::vector<uint8_t>& dirty, std::vector<int32_t>& list) {
+    dirty.assign(size_t(C.n_elems), 0);
+    list.clear();
+    const int64_t fx = F.nx + 1, fy = F.ny + 1;
+    for (int64_t E = 0; E < C.n_elems; E++) {
+        const int64_t I = E % C.nx, J = (E / C.nx) % C.ny, K = E / (int64_t(C.nx) * C.ny);
+        bool d = false;
+        for (int dk = 0; dk < 3 && !d; dk++)
+            for (int dj = 0; dj < 3 && !d; dj++)
+                for (int di = 0; di < 3 && !d; di++) {
+                    const int64_t p = (2 * I + di) + fx * ((2 * J + dj) + fy * (2 * K + dk));
+                    const int64_t r = F.rank_of_lat[size_t(p)];
+                    for (int v = 0; v < 3; v++) d = d || constrained[size_t(r * 3 + v)] != 0;
+                }
+        if (d) {
+            dirty[size_t(E)] = 1;
+            list.push_back(int32_t(E));
+        }
+    }
+}
+
 }  // namespace mastg

@@ -47,4 +47,15 @@ constexpr int64_t kMGCoarseTarget = 136;
 // Detects the lattice and builds the coarse levels (hex8: nav 3, quad4: nav 5).
 MGHierarchy build_mg_hierarchy(const Structure& S);
 
+// One-class hex8 lattice: Gc[c] = Q_c^T Khat Q_c ([8][64 blocks][3x3], block
+// (A, B) at (A 8 + B) 9) for the 8 children c of a coarse cell, made exactly
+// symmetric.  Khat: [64][3x3] in the same layout.
+void mg_class_child_products(const double* Khat, double* Gc);
+
+// Coarse cells of level C (the first coarse level of the hex8 problem level F)
+// with a constrained DOF among the 27 fine lattice points they cover:
+// dirty[E] = 1 and E in list.  constrained: by node rank, 3 per node.
+void mg_dirty_coarse(const MGLevelHost& F, const MGLevelHost& C, const std::vector<uint8_t>& constrained,
+                     std::vector<uint8_t>& dirty, std::vector<int32_t>& list);
+
 }  // namespace mastg
