@@ -319,8 +319,10 @@ def main():
         sys.exit(2)
 
     if a.workload == "c5":
-        serving_main(a)
-        return
+        print("bench.py --workload c5 is switched off: the serving-stream run ended with a host "
+              "segmentation fault whose cause has not been found (the server's GPU tests pass)",
+              file=sys.stderr)
+        sys.exit(2)
     if a.workload in ("c3", "c4"):
         physics_main(a)
         return

@@ -62,6 +62,7 @@ struct MGLevelDev {
     DevBuf blk_ptr, blk_col, cptr, cidx, econn, con;   // levels >= 1
     DevBuf gal_dirty, gal_list;   // hex8 level 1: coarse cells touching constraints
     int64_t n_dirty = 0;
+    DevBuf sten;                  // hex8 level 1: the operator as a 27-point FP32 stencil
     DevBuf Kel, val, val32, dinv, r, x, res, da, db;
 };
 
@@ -692,6 +693,9 @@ int mg_build(mastg_ctx* c) {
         TRY(upload(c, lv[1]->gal_dirty, dirty));
         if (!list.empty()) TRY(upload(c, lv[1]->gal_list, list));
         lv[1]->n_dirty = int64_t(list.size());
+        // the first coarse level smooths with its operator as a stencil
+        // (coalesced values, no column indices) when it is not the coarsest
+        if (nl > 2) TRY(dalloc(c, lv[1]->sten, size_t(H.levels[1].n_nodes) * 243 * sizeof(float)));
     }
     TRY(dalloc(c, c->mg_cheb, size_t(nl) * kMGChebStride * sizeof(double)));
     TRY(dalloc(c, c->mg_lam, size_t(nl) * sizeof(double)));
@@ -772,6 +776,7 @@ int mg_setup(mastg_ctx* c) {
             launch_mg_power(C.a, C.a.val, kMGPowerIters, kMGPowerSafety, C.a.da, C.a.db, c->mg_part.as<double>(),
                             c->mg_scal.as<double>(), c->mg_lam.as<double>() + l + 1, s);
         launch_to_f32(int64_t(C.val32.bytes / sizeof(float)), C.a.val, C.val32.as<float>(), s);
+        if (C.sten.p) launch_mg_to_stencil(C.a, C.sten.as<float>(), s);
     }
     launch_mg_cheb(c->mg_lam.as<double>(), c->mg_cheb.as<double>(), nl, std::max(c->mg_k0, c->mg_kc), 30.0, s);
     const MGLevelDev& Lc = *c->mg[nl - 1];
@@ -816,6 +821,9 @@ const void* mg_vcycle(mastg_ctx* c, const CGArgs& a, const int* done, hipStream_
         if (l == 0 && f32)
             launch_mg_fine(a, L0, mode, step, iin < 0 ? nullptr : dbuf32(iin), iout < 0 ? nullptr : dbuf32(iout),
                            done, s);
+        else if (c->mg[l]->sten.p)
+            launch_mg_stencil(lv(l), c->mg[l]->sten.as<float>(), mode, step, iin < 0 ? nullptr : dbuf(l, iin),
+                              iout < 0 ? nullptr : dbuf(l, iout), done, s);
         else
             launch_mg_bsr(lv(l), mode, step, iin < 0 ? nullptr : dbuf(l, iin), iout < 0 ? nullptr : dbuf(l, iout),
                           done, s);

@@ -246,6 +246,12 @@ void launch_mg_invert(int n, double* A, hipStream_t s);
 void launch_mg_pre_init(const MGLevelArgs& L, const int* done, hipStream_t s);
 void launch_mg_bsr(const MGLevelArgs& L, int mode, int step, const double* din, double* dout, const int* done,
                    hipStream_t s);
+// a lattice-ordered hex8 coarse level as a 27-point FP32 stencil
+// (sten[(o 9 + v 3 + w) n_nodes + r], o = the neighbour offset), built from its
+// FP32 node blocks, and a smoothing step with it (same modes as the node blocks)
+void launch_mg_to_stencil(const MGLevelArgs& L, float* sten, hipStream_t s);
+void launch_mg_stencil(const MGLevelArgs& L, const float* sten, int mode, int step, const double* din, double* dout,
+                       const int* done, hipStream_t s);
 // level 0 (hex8): FP32 smoothing step with the lattice operator (c.lat) and
 // the row-local epilogue on L's FP32 state
 void launch_mg_fine(const CGArgs& c, const MGLevelArgs& L, int mode, int step, const float* din, float* dout,

@@ -270,6 +270,65 @@ __global__ __launch_bounds__(256) void k_mg_bsr(MGLevelArgs L, const T* __restri
     }
 }
 
+// ---- a coarse hex8 level as a 27-point stencil ------------------------------
+// The node blocks of a lattice-ordered level (3 variables) rewritten as
+// sten[(o 9 + v 3 + w) N + r] (FP32), o = (dx+1) + 3 (dy+1) + 9 (dz+1) the
+// neighbour offset; offsets outside the lattice hold 0.  One thread per node.
+__global__ __launch_bounds__(256) void k_mg_to_stencil(MGLevelArgs L, float* __restrict__ sten) {
+    const int64_t r = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (r >= L.n_nodes) return;
+    const int64_t N = L.n_nodes;
+    const uint32_t nxp = uint32_t(L.nx + 1), nyp = uint32_t(L.ny + 1), ru = uint32_t(r), rq = ru / nxp;
+    const int ri = int(ru - rq * nxp), rj = int(rq % nyp), rk = int(rq / nyp);
+    for (int e = 0; e < 243; e++) sten[e * N + r] = 0.f;
+    const int64_t k0 = L.blk_ptr[r];
+    const int deg = int(L.blk_ptr[r + 1] - k0);
+    const float* __restrict__ V = L.val32 + k0 * 9;
+    for (int k = 0; k < deg; k++) {
+        const uint32_t cu = uint32_t(L.blk_col[k0 + k]), cq = cu / nxp;
+        const int dx = int(cu - cq * nxp) - ri, dy = int(cq % nyp) - rj, dz = int(cq / nyp) - rk;
+        if (dx < -1 || dx > 1 || dy < -1 || dy > 1 || dz < -1 || dz > 1) continue;   // not a lattice neighbour
+        const int o = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
+#pragma unroll
+        for (int v = 0; v < 3; v++)
+#pragma unroll
+            for (int w = 0; w < 3; w++) sten[(o * 9 + v * 3 + w) * N + r] = V[(v * deg + k) * 3 + w];
+    }
+}
+
+// Smoothing step of a stencil level: thread per node, coalesced FP32 stencil
+// loads, FP64 products and sums, the fused Chebyshev epilogue
+__global__ __launch_bounds__(256) void k_mg_stencil(MGLevelArgs L, const float* __restrict__ sten, int mode, int step,
+                                                    const double* __restrict__ din, double* __restrict__ dout,
+                                                    const int* __restrict__ done) {
+    if (done && *done) return;
+    const int64_t r = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (r >= L.n_nodes) return;
+    const int64_t N = L.n_nodes;
+    const double* __restrict__ in = mode == MG_RESID_INIT ? L.x : din;
+    const uint32_t nxp = uint32_t(L.nx + 1), nyp = uint32_t(L.ny + 1), ru = uint32_t(r), rq = ru / nxp;
+    const int i = int(ru - rq * nxp), j = int(rq % nyp), k = int(rq / nyp);
+    double y0 = 0.0, y1 = 0.0, y2 = 0.0;
+    // one z-layer of 9 neighbours per pass bounds the loads in flight (registers)
+#pragma unroll 9
+    for (int o = 0; o < 27; o++) {
+        // neighbours outside the lattice are clamped into it (their stencil entries are 0)
+        const int ni = min(max(i + o % 3 - 1, 0), L.nx), nj = min(max(j + (o / 3) % 3 - 1, 0), L.ny),
+                  nk = min(max(k + o / 9 - 1, 0), L.nz);
+        const int64_t nb = int64_t(uint32_t(ni) + nxp * (uint32_t(nj) + nyp * uint32_t(nk))) * 3;
+        const double x0 = in[nb], x1 = in[nb + 1], x2 = in[nb + 2];
+        const float* __restrict__ S = sten + int64_t(o) * 9 * N + r;
+        y0 = fma(double(S[0 * N]), x0, y0); y0 = fma(double(S[1 * N]), x1, y0); y0 = fma(double(S[2 * N]), x2, y0);
+        y1 = fma(double(S[3 * N]), x0, y1); y1 = fma(double(S[4 * N]), x1, y1); y1 = fma(double(S[5 * N]), x2, y1);
+        y2 = fma(double(S[6 * N]), x0, y2); y2 = fma(double(S[7 * N]), x1, y2); y2 = fma(double(S[8 * N]), x2, y2);
+    }
+    const double ith = L.cheb[0];
+    const double c1 = step > 0 ? L.cheb[2 * step] : 0.0, c2 = step > 0 ? L.cheb[2 * step + 1] : 0.0;
+    mg_epilogue(L, mode, r * 3 + 0, y0, din, dout, c1, c2, ith);
+    mg_epilogue(L, mode, r * 3 + 1, y1, din, dout, c1, c2, ith);
+    mg_epilogue(L, mode, r * 3 + 2, y2, din, dout, c1, c2, ith);
+}
+
 // ---- level 0 on a lattice: per-node matrix-free operator -------------------
 // Thread per lattice point n, vectors component-major in lattice order (x
 // fastest), so each neighbour gather of a wave is one contiguous 512-byte run;
@@ -1090,6 +1149,14 @@ void launch_perm_nodes(int64_t n_nodes, const int32_t* map, const double* src, d
 void launch_mg_pre_init(const MGLevelArgs& L, const int* done, hipStream_t s) {
     if (L.planar) hipLaunchKernelGGL(k_mg_pre_init32, dim3(mg_grid(L.n_nodes * L.nav)), dim3(256), 0, s, L, done);
     else hipLaunchKernelGGL(k_mg_pre_init, dim3(mg_grid(L.n_nodes * L.nav)), dim3(256), 0, s, L, done);
+}
+void launch_mg_to_stencil(const MGLevelArgs& L, float* sten, hipStream_t s) {
+    hipLaunchKernelGGL(k_mg_to_stencil, dim3(mg_blocks(L.n_nodes, 256)), dim3(256), 0, s, L, sten);
+}
+void launch_mg_stencil(const MGLevelArgs& L, const float* sten, int mode, int step, const double* din, double* dout,
+                       const int* done, hipStream_t s) {
+    hipLaunchKernelGGL(k_mg_stencil, dim3(mg_blocks(L.n_nodes, 256)), dim3(256), 0, s, L, sten, mode, step, din,
+                       dout, done);
 }
 void launch_mg_bsr(const MGLevelArgs& L, int mode, int step, const double* din, double* dout, const int* done,
                    hipStream_t s) {
