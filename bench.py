@@ -151,9 +151,9 @@ def physics_main(a):
     print(f"# workload {a.workload}: {w.n_elems} elems, {ctx.n_active} active DOF, nnz {ctx.nnz}, "
           f"setup {t_setup:.1f}s", file=sys.stderr, flush=True)
     # heat: Newton with multigrid-preconditioned BiCGStab, the Jacobi-
-    # preconditioned path timed once beside it; plates: CG + Jacobi (multigrid
-    # is switched off for the plate)
-    sv = mastg.SOLVER_MG if a.solver in ("mg", "all") and kind == 3 else mastg.SOLVER_CG_JACOBI
+    # preconditioned path timed once beside it; plates: multigrid-preconditioned
+    # CG, the Jacobi-preconditioned CG timed once beside it
+    sv = mastg.SOLVER_MG if a.solver in ("mg", "all") else mastg.SOLVER_CG_JACOBI
     plain = None
     if sv == mastg.SOLVER_MG:
         t1 = time.perf_counter()

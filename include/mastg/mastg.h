@@ -149,10 +149,10 @@ int mastg_assemble(mastg_ctx* ctx, const double* X, double* R, double* J_vals,
                                            Chebyshev/Jacobi smoothing, dense coarsest solve.
                                            hex8 elasticity: CG, matrix-free lattice operator
                                            on the problem level (FP32 smoothing); hex8
-                                           heat: BiCGStab on the assembled Jacobian.
-                                           Meshes that fill a lattice whose cell counts
-                                           halve to <= 1536 coarse DOFs; refused for the
-                                           Mindlin plate (switched off) */
+                                           heat: BiCGStab on the assembled Jacobian;
+                                           Mindlin plate: CG, bilinear transfers, assembled
+                                           problem level.  Meshes that fill a lattice whose
+                                           cell counts halve to <= 1536 coarse DOFs */
 
 typedef struct {
     int32_t solver;       /* MASTG_SOLVER_*                                   */

@@ -553,15 +553,8 @@ int mg_build(mastg_ctx* c) {
     if (c->mg_tried) return c->mg.empty() ? fail(c, MASTG_ERR_INVALID, "multigrid unavailable: " + c->mg_why)
                                           : MASTG_OK;
     c->mg_tried = true;
-    if (c->d.physics == MASTG_QUAD4_MINDLIN) {
-        // switched off: a device memory fault of undetermined cause was seen in a
-        // plate design evaluation; plates are solved by CG + Jacobi until the
-        // cause is shown (the hierarchy code below still handles them)
-        c->mg_why = "multigrid is switched off for the Mindlin plate (use CG + Jacobi)";
-        return fail(c, MASTG_ERR_INVALID, "multigrid unavailable: " + c->mg_why);
-    }
-    // the heat problem level smooths with its assembled node blocks in
-    // node-rank order; hex8 elasticity uses the per-node lattice operator
+    // the plate and heat problem levels smooth with their assembled node blocks
+    // in node-rank order; hex8 elasticity uses the per-node lattice operator
     const bool plate = c->d.physics != MASTG_HEX8_ELASTIC;
     if (!(plate || c->S.n_classes == 1)) {
         c->mg_why = "multigrid is built for lattice meshes (hex8 elasticity needs one element shape)";
@@ -592,6 +585,12 @@ int mg_build(mastg_ctx* c) {
             TRY(upload(c, L->rank_of_lat, h.rank_of_lat));
             TRY(upload(c, L->elem_of_cell, h.elem_of_cell));
             TRY(dalloc(c, c->val32, size_t(c->S.nnz) * sizeof(float)));
+            // the level's arguments hold the context's CSR values and inverse
+            // diagonal: allocate them here at the assembly's sizes (the
+            // assembly then reuses them), since a design evaluation builds the
+            // hierarchy before its first assembly
+            TRY(dalloc(c, c->val, size_t(c->S.nnz) * sizeof(double)));
+            TRY(dalloc(c, c->dinv, size_t(c->S.n_active) * sizeof(double)));
             a.rank_of_lat = L->rank_of_lat.as<int32_t>();
             a.con = c->con.as<uint8_t>();
             a.blk_ptr = c->blk_ptr.as<int64_t>();
@@ -732,6 +731,11 @@ int mg_setup(mastg_ctx* c) {
         launch_lattice_rowinfo(la.lat, L0.dinv_lat.as<double>(), L0.dinv32.as<float>(), c->mg_part.as<double>(),
                                c->mg_lam.as<double>(), s);
     } else {
+        // the problem level reads the context's assembly
+        L0.a.val = c->val.as<double>();
+        L0.a.val32 = c->val32.as<float>();
+        L0.a.dinv = c->dinv.as<double>();
+        if (!L0.a.val || !L0.a.val32 || !L0.a.dinv) return fail(c, MASTG_ERR_INVALID, "multigrid needs an FP64 assembly");
         launch_mg_rowinfo_nb(nav, c->S.n_nodes, c->blk_ptr.as<int64_t>(), c->blk_col.as<int32_t>(),
                              c->val.as<double>(), nullptr, c->mg_part.as<double>(), c->mg_lam.as<double>(), s);
         launch_to_f32(c->S.nnz, c->val.as<double>(), c->val32.as<float>(), s);

@@ -146,8 +146,8 @@ void run_request(mastg_server* s, Req* q, CachedCtx* cc, bool create) {
     if (!rc) {
         mastg_solve_opts so{};
         const bool plate = q->r.kind == 2;
-        // plates: CG + Jacobi (multigrid is switched off for them)
-        so.solver = plate ? MASTG_SOLVER_CG_JACOBI : s->o.solver;
+        // plates take multigrid or CG + Jacobi (they refuse the hex8-only solvers)
+        so.solver = (!plate || s->o.solver == MASTG_SOLVER_MG) ? s->o.solver : MASTG_SOLVER_CG_JACOBI;
         so.rtol = plate ? (s->o.rtol_plate > 0 ? s->o.rtol_plate : 1e-7) : (s->o.rtol > 0 ? s->o.rtol : 1e-10);
         mastg_design_result dr{};
         rc = mastg_design_eval(cc->ctx, q->params.data(), q->r.grad_out, MASTG_HOST, &so, &dr);
