@@ -712,10 +712,18 @@ __global__ __launch_bounds__(256) void k_mg_power_start_norm(const double* __res
 
 // Gauss-Jordan inverse of an SPD matrix (no pivoting), n <= 136, one block of
 // 1024 threads.  Thread t keeps the 9 x 3 entries at rows (t >> 6) + 16 r and
-// columns (t & 63) + 64 c in registers (a 144 x 192 cover of the matrix); per
-// pivot only the pivot column and the scaled pivot row pass through LDS.
+// columns (t & 63) + 64 c in registers (a 144 x 192 cover of the matrix; the
+// entries outside n x n stay 0).  Pivot k's row lives in wave k & 15 (register
+// row k >> 4) and its column in lane k & 63 of every wave (register column
+// k >> 6), so the owners are known without searching: lane k & 63 publishes
+// the column, the pivot row's wave broadcasts the pivot across its lanes,
+// scales the row in registers and publishes it (entry k = 1 / pivot), and
+// every thread updates its entries with selects instead of branches.  The
+// pivot column and row are double-buffered in LDS, so one barrier per pivot
+// suffices: a buffer is rewritten two pivots later, after every thread has
+// passed the barrier that follows its last read.
 __global__ __launch_bounds__(1024) void k_mg_invert_lds(int n, double* __restrict__ A) {
-    __shared__ double colk[144], rowk[192];
+    __shared__ double colk[2][144], rowk[2][192];
     const int t = threadIdx.x, i0 = t >> 6, j0 = t & 63;
     double a[9][3];
 #pragma unroll
@@ -726,37 +734,39 @@ __global__ __launch_bounds__(1024) void k_mg_invert_lds(int n, double* __restric
             a[r][c] = (i < n && j < n) ? A[i * n + j] : 0.0;
         }
     for (int k = 0; k < n; k++) {
+        const int b = k & 1, kr = k >> 4, ki = k & 15, kc = k >> 6, kj = k & 63;
+        if (j0 == kj)   // column k (before this pivot's update)
 #pragma unroll
-        for (int r = 0; r < 9; r++)
+            for (int r = 0; r < 9; r++) colk[b][i0 + 16 * r] = kc == 0 ? a[r][0] : (kc == 1 ? a[r][1] : a[r][2]);
+        if (i0 == ki) {   // the pivot row's wave (uniform branch)
+            double rv[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-            for (int c = 0; c < 3; c++)
-                if (j0 + 64 * c == k && i0 + 16 * r < n) colk[i0 + 16 * r] = a[r][c];
-        __syncthreads();
-        const double p = 1.0 / colk[k];
+            for (int r = 0; r < 9; r++)
 #pragma unroll
-        for (int r = 0; r < 9; r++)
+                for (int c = 0; c < 3; c++) rv[c] = r == kr ? a[r][c] : rv[c];
+            const double piv = __shfl(kc == 0 ? rv[0] : (kc == 1 ? rv[1] : rv[2]), kj);
+            const double p = 1.0 / piv;
 #pragma unroll
             for (int c = 0; c < 3; c++) {
                 const int j = j0 + 64 * c;
-                if (i0 + 16 * r == k && j < n) {
-                    a[r][c] = (j == k) ? p : a[r][c] * p;
-                    rowk[j] = a[r][c];
-                }
+                rowk[b][j] = j == k ? p : rv[c] * p;
             }
+        }
         __syncthreads();
+        const double p = rowk[b][k];
         double ck[9], rk[3];
 #pragma unroll
-        for (int r = 0; r < 9; r++) ck[r] = i0 + 16 * r < n ? colk[i0 + 16 * r] : 0.0;
+        for (int r = 0; r < 9; r++) ck[r] = colk[b][i0 + 16 * r];
 #pragma unroll
-        for (int c = 0; c < 3; c++) rk[c] = j0 + 64 * c < n ? rowk[j0 + 64 * c] : 0.0;
+        for (int c = 0; c < 3; c++) rk[c] = rowk[b][j0 + 64 * c];
 #pragma unroll
         for (int r = 0; r < 9; r++)
 #pragma unroll
             for (int c = 0; c < 3; c++) {
-                const int i = i0 + 16 * r, j = j0 + 64 * c;
-                if (i < n && j < n && i != k) a[r][c] = (j == k) ? -ck[r] * p : a[r][c] - ck[r] * rk[c];
+                const bool is_row = i0 == ki && r == kr, is_col = j0 + 64 * c == k;
+                const double v = is_col ? -ck[r] * p : fma(-ck[r], rk[c], a[r][c]);
+                a[r][c] = is_row ? rk[c] : v;
             }
-        __syncthreads();
     }
 #pragma unroll
     for (int r = 0; r < 9; r++)
