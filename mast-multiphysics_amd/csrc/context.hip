@@ -62,7 +62,7 @@ struct MGLevelDev {
     DevBuf blk_ptr, blk_col, cptr, cidx, econn, con;   // levels >= 1
     DevBuf gal_dirty, gal_list;   // hex8 level 1: coarse cells touching constraints
     int64_t n_dirty = 0;
-    DevBuf sten;                  // hex8 level 1: the operator as a 27-point FP32 stencil
+    DevBuf sten;                  // hex8 level 1: the operator as a symmetric 27-point FP32 stencil
     DevBuf Kel, val, val32, dinv, r, x, res, da, db;
 };
 
@@ -694,7 +694,7 @@ int mg_build(mastg_ctx* c) {
         lv[1]->n_dirty = int64_t(list.size());
         // the first coarse level smooths with its operator as a stencil
         // (coalesced values, no column indices) when it is not the coarsest
-        if (nl > 2) TRY(dalloc(c, lv[1]->sten, size_t(H.levels[1].n_nodes) * 243 * sizeof(float)));
+        if (nl > 2) TRY(dalloc(c, lv[1]->sten, size_t(H.levels[1].n_nodes) * 123 * sizeof(float)));
     }
     TRY(dalloc(c, c->mg_cheb, size_t(nl) * kMGChebStride * sizeof(double)));
     TRY(dalloc(c, c->mg_lam, size_t(nl) * sizeof(double)));

@@ -270,17 +270,24 @@ __global__ __launch_bounds__(256) void k_mg_bsr(MGLevelArgs L, const T* __restri
     }
 }
 
-// ---- a coarse hex8 level as a 27-point stencil ------------------------------
-// The node blocks of a lattice-ordered level (3 variables) rewritten as
-// sten[(o 9 + v 3 + w) N + r] (FP32), o = (dx+1) + 3 (dy+1) + 9 (dz+1) the
-// neighbour offset; offsets outside the lattice hold 0.  One thread per node.
+// ---- a coarse hex8 level as a symmetric 27-point stencil --------------------
+// The level's node blocks are exactly symmetric (B(r, m) = B(m, r)^T), so each
+// node keeps half of them (FP32, 123 entries, entry e of node r at e N + r):
+// the 13 blocks towards the neighbours o = 14..26 (o = (dx+1) + 3 (dy+1) +
+// 9 (dz+1)) at e = (o - 14) 9 + 3 v + w, and the 6 entries v <= w of its own
+// block at e = 117 + kSymIdx[v][w].  The block towards a neighbour m with
+// o < 13 is the transpose of m's block towards r, read at index m (coalesced:
+// m = r + a fixed offset).  Offsets outside the lattice hold 0.
+__constant__ int8_t kSymIdx[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+constexpr int kStenEntries = 123;
+
 __global__ __launch_bounds__(256) void k_mg_to_stencil(MGLevelArgs L, float* __restrict__ sten) {
     const int64_t r = int64_t(blockIdx.x) * 256 + threadIdx.x;
     if (r >= L.n_nodes) return;
     const int64_t N = L.n_nodes;
     const uint32_t nxp = uint32_t(L.nx + 1), nyp = uint32_t(L.ny + 1), ru = uint32_t(r), rq = ru / nxp;
     const int ri = int(ru - rq * nxp), rj = int(rq % nyp), rk = int(rq / nyp);
-    for (int e = 0; e < 243; e++) sten[e * N + r] = 0.f;
+    for (int e = 0; e < kStenEntries; e++) sten[e * N + r] = 0.f;
     const int64_t k0 = L.blk_ptr[r];
     const int deg = int(L.blk_ptr[r + 1] - k0);
     const float* __restrict__ V = L.val32 + k0 * 9;
@@ -289,10 +296,15 @@ __global__ __launch_bounds__(256) void k_mg_to_stencil(MGLevelArgs L, float* __r
         const int dx = int(cu - cq * nxp) - ri, dy = int(cq % nyp) - rj, dz = int(cq / nyp) - rk;
         if (dx < -1 || dx > 1 || dy < -1 || dy > 1 || dz < -1 || dz > 1) continue;   // not a lattice neighbour
         const int o = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
+        if (o < 13) continue;   // kept by the neighbour
 #pragma unroll
         for (int v = 0; v < 3; v++)
 #pragma unroll
-            for (int w = 0; w < 3; w++) sten[(o * 9 + v * 3 + w) * N + r] = V[(v * deg + k) * 3 + w];
+            for (int w = 0; w < 3; w++) {
+                const float x = V[(v * deg + k) * 3 + w];
+                if (o > 13) sten[((o - 14) * 9 + v * 3 + w) * N + r] = x;
+                else if (v <= w) sten[(117 + kSymIdx[v][w]) * N + r] = x;
+            }
     }
 }
 
@@ -308,19 +320,35 @@ __global__ __launch_bounds__(256) void k_mg_stencil(MGLevelArgs L, const float* 
     const double* __restrict__ in = mode == MG_RESID_INIT ? L.x : din;
     const uint32_t nxp = uint32_t(L.nx + 1), nyp = uint32_t(L.ny + 1), ru = uint32_t(r), rq = ru / nxp;
     const int i = int(ru - rq * nxp), j = int(rq % nyp), k = int(rq / nyp);
-    double y0 = 0.0, y1 = 0.0, y2 = 0.0;
-    // one z-layer of 9 neighbours per pass bounds the loads in flight (registers)
+    double y0, y1, y2;
+    {   // own block (symmetric)
+        const float* __restrict__ S = sten + 117 * N + r;
+        const double s00 = S[0], s01 = S[N], s02 = S[2 * N], s11 = S[3 * N], s12 = S[4 * N], s22 = S[5 * N];
+        const double x0 = in[r * 3], x1 = in[r * 3 + 1], x2 = in[r * 3 + 2];
+        y0 = fma(s02, x2, fma(s01, x1, s00 * x0));
+        y1 = fma(s12, x2, fma(s11, x1, s01 * x0));
+        y2 = fma(s22, x2, fma(s12, x1, s02 * x0));
+    }
+    // the 26 neighbours, one z-layer of 9 offsets per pass (bounded loads in flight)
 #pragma unroll 9
     for (int o = 0; o < 27; o++) {
-        // neighbours outside the lattice are clamped into it (their stencil entries are 0)
-        const int ni = min(max(i + o % 3 - 1, 0), L.nx), nj = min(max(j + (o / 3) % 3 - 1, 0), L.ny),
-                  nk = min(max(k + o / 9 - 1, 0), L.nz);
-        const int64_t nb = int64_t(uint32_t(ni) + nxp * (uint32_t(nj) + nyp * uint32_t(nk))) * 3;
-        const double x0 = in[nb], x1 = in[nb + 1], x2 = in[nb + 2];
-        const float* __restrict__ S = sten + int64_t(o) * 9 * N + r;
-        y0 = fma(double(S[0 * N]), x0, y0); y0 = fma(double(S[1 * N]), x1, y0); y0 = fma(double(S[2 * N]), x2, y0);
-        y1 = fma(double(S[3 * N]), x0, y1); y1 = fma(double(S[4 * N]), x1, y1); y1 = fma(double(S[5 * N]), x2, y1);
-        y2 = fma(double(S[6 * N]), x0, y2); y2 = fma(double(S[7 * N]), x1, y2); y2 = fma(double(S[8 * N]), x2, y2);
+        if (o == 13) continue;
+        const int ui = i + o % 3 - 1, uj = j + (o / 3) % 3 - 1, uk = k + o / 9 - 1;
+        const bool inside = ui >= 0 && ui <= L.nx && uj >= 0 && uj <= L.ny && uk >= 0 && uk <= L.nz;
+        const int ni = min(max(ui, 0), L.nx), nj = min(max(uj, 0), L.ny), nk = min(max(uk, 0), L.nz);
+        const int64_t m = int64_t(uint32_t(ni) + nxp * (uint32_t(nj) + nyp * uint32_t(nk)));
+        // a neighbour outside the lattice is clamped into it and weighted 0
+        const double wgt = inside ? 1.0 : 0.0;
+        const double x0 = wgt * in[m * 3], x1 = wgt * in[m * 3 + 1], x2 = wgt * in[m * 3 + 2];
+        // block (v, w) towards m: own entry (o - 14) 9 + 3 v + w at r, or the
+        // neighbour's entry (12 - o) 9 + 3 w + v at m (uniform strides)
+        const bool lower = o < 13;
+        const float* __restrict__ S = lower ? sten + int64_t(12 - o) * 9 * N + m : sten + int64_t(o - 14) * 9 * N + r;
+        const int64_t sv = lower ? N : 3 * N, sw = lower ? 3 * N : N;
+        y0 = fma(double(S[0]), x0, y0); y0 = fma(double(S[sw]), x1, y0); y0 = fma(double(S[2 * sw]), x2, y0);
+        y1 = fma(double(S[sv]), x0, y1); y1 = fma(double(S[sv + sw]), x1, y1); y1 = fma(double(S[sv + 2 * sw]), x2, y1);
+        y2 = fma(double(S[2 * sv]), x0, y2); y2 = fma(double(S[2 * sv + sw]), x1, y2);
+        y2 = fma(double(S[2 * sv + 2 * sw]), x2, y2);
     }
     const double ith = L.cheb[0];
     const double c1 = step > 0 ? L.cheb[2 * step] : 0.0, c2 = step > 0 ? L.cheb[2 * step + 1] : 0.0;
@@ -358,21 +386,6 @@ __device__ __forceinline__ void lattice_row(const LatticeOpArgs& O, const T* __r
                                             const T* __restrict__ Kc, int64_t n_lat, int i, int j, int k, bool valid,
                                             T& y0, T& y1, T& y2, int& cnt) {
     const int64_t nxp = O.nx + 1, nyp = O.ny + 1;
-    // cells at n (c bit d: the cell extends towards +d from n - 1) and their
-    // scales; a missing cell gets the scale 0 (its index is clamped into the
-    // lattice so every load stays in bounds and unconditional)
-    T sc[8];
-    cnt = 0;
-#pragma unroll
-    for (int c = 0; c < 8; c++) {
-        const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
-        const int ci = i - 1 + cx, cj = j - 1 + cy, ck = k - 1 + cz;
-        const bool has = valid && ci >= 0 && ci < O.nx && cj >= 0 && cj < O.ny && ck >= 0 && ck < O.nz;
-        const int cell = min(max(ci, 0), O.nx - 1) + O.nx * (min(max(cj, 0), O.ny - 1) + O.ny * min(max(ck, 0), O.nz - 1));
-        const T sv = T(O.sE_cell[cell]);
-        sc[c] = has ? sv : T(0);
-        cnt += has ? 1 : 0;
-    }
     // neighbour-outer, one z-layer of 9 neighbours at a time (bounded live
     // ranges): each neighbour value is read once and feeds every cell shared
     // with n, y += s_c Khat[a_c][b] x_o (scaled per block product, so only y
@@ -386,11 +399,8 @@ __device__ __forceinline__ void lattice_row(const LatticeOpArgs& O, const T* __r
     const char* __restrict__ b1 = reinterpret_cast<const char*>(in + n_lat);
     const char* __restrict__ b2 = reinterpret_cast<const char*>(in + 2 * n_lat);
     const uint32_t sx = uint32_t(nxp), sz = uint32_t(nxp * nyp);
-    y0 = y1 = y2 = T(0);
-#pragma unroll 1
-    for (int lz = 0; lz < 3; lz++) {
+    auto load_layer = [&](int lz, T (&xv)[9][3]) __attribute__((always_inline)) {
         const uint32_t mk = uint32_t(min(max(k + lz - 1, 0), O.nz));
-        T xv[9][3];
 #pragma unroll
         for (int q = 0; q < 9; q++) {
             const uint32_t mi = uint32_t(min(max(i + (q % 3) - 1, 0), O.nx));
@@ -400,6 +410,27 @@ __device__ __forceinline__ void lattice_row(const LatticeOpArgs& O, const T* __r
             xv[q][1] = *reinterpret_cast<const T*>(b1 + off);
             xv[q][2] = *reinterpret_cast<const T*>(b2 + off);
         }
+    };
+    // the first layer's gathers are issued before the scale loads (both in flight together)
+    T x0v[9][3];
+    load_layer(0, x0v);
+    // cells at n (c bit d: the cell extends towards +d from n - 1) and their
+    // scales; a missing cell gets the scale 0 (its index is clamped into the
+    // lattice so every load stays in bounds)
+    T sc[8];
+    cnt = 0;
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+        const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
+        const int ci = i - 1 + cx, cj = j - 1 + cy, ck = k - 1 + cz;
+        const bool has = valid & (ci >= 0) & (ci < O.nx) & (cj >= 0) & (cj < O.ny) & (ck >= 0) & (ck < O.nz);
+        const int cell = min(max(ci, 0), O.nx - 1) + O.nx * (min(max(cj, 0), O.ny - 1) + O.ny * min(max(ck, 0), O.nz - 1));
+        // unconditional load (the index is clamped) weighted 0 for a missing
+        // cell: used on every path, so it is not sunk into a branch
+        sc[c] = T(O.sE_cell[cell]) * T(has ? 1 : 0);
+        cnt += has ? 1 : 0;
+    }
+    auto compute_layer = [&](int lz, const T (&xv)[9][3]) __attribute__((always_inline)) {
 #pragma unroll
         for (int c = 0; c < 8; c++) {
             const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
@@ -421,6 +452,14 @@ __device__ __forceinline__ void lattice_row(const LatticeOpArgs& O, const T* __r
             y1 = fma_t(sc[c], t1, y1);
             y2 = fma_t(sc[c], t2, y2);
         }
+    };
+    y0 = y1 = y2 = T(0);
+    compute_layer(0, x0v);
+#pragma unroll 1
+    for (int lz = 1; lz < 3; lz++) {
+        T xv[9][3];
+        load_layer(lz, xv);
+        compute_layer(lz, xv);
     }
 }
 
