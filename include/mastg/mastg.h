@@ -263,8 +263,9 @@ int mastg_set_tuning(mastg_ctx* ctx, int spmv_lanes, int grid);
 typedef struct mastg_server mastg_server;
 typedef struct {
     int32_t n_slots;          /* concurrent requests on the device                 */
-    int32_t solver;           /* MASTG_SOLVER_* for hex8 requests; plates are
-                                 served by CG + Jacobi                            */
+    int32_t solver;           /* MASTG_SOLVER_* for hex8 requests; plates take
+                                 MASTG_SOLVER_MG when it is chosen, else CG +
+                                 Jacobi (they refuse the hex8-only solvers)       */
     int64_t hbm_budget;       /* device bytes the server may hold; 0 = 64 GiB      */
     int32_t queue_capacity;   /* pending requests before submit returns BUSY       */
     double rtol;              /* hex8 solve tolerance, 0 -> 1e-10                  */
