@@ -148,14 +148,16 @@ def physics_main(a):
     t0 = time.perf_counter()
     ctx = mastg.Context(w.desc)
     t_setup = time.perf_counter() - t0
+    if a.no_graphs:
+        ctx.set_option(mastg.OPT_GRAPHS, 0)
     print(f"# workload {a.workload}: {w.n_elems} elems, {ctx.n_active} active DOF, nnz {ctx.nnz}, "
           f"setup {t_setup:.1f}s", file=sys.stderr, flush=True)
-    # heat: Newton with multigrid-preconditioned BiCGStab, the Jacobi-
-    # preconditioned path timed once beside it; plates: multigrid-preconditioned
-    # CG, the Jacobi-preconditioned CG timed once beside it
+    # heat: Newton with multigrid-preconditioned BiCGStab; plates: multigrid-
+    # preconditioned CG; with the default --solver all the Jacobi-preconditioned
+    # path is timed once beside it
     sv = mastg.SOLVER_MG if a.solver in ("mg", "all") else mastg.SOLVER_CG_JACOBI
     plain = None
-    if sv == mastg.SOLVER_MG:
+    if sv == mastg.SOLVER_MG and a.solver == "all":
         t1 = time.perf_counter()
         rp = ctx.design_eval(w.params, rtol=rtol)
         plain = dict(s=time.perf_counter() - t1, iters=rp["iters"], objective=rp["objective"], grad=rp["grad"])
