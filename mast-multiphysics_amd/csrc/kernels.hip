@@ -585,7 +585,10 @@ __global__ __launch_bounds__(128) void k_quad4_mindlin_sens(Quad4Args A, const d
 // the element's marked sides (:1099-1154) and the volumetric source (:1709-1745):
 //   R_a = sum_q c k(T_q) g_a.gradT_q + sum_sides int h N_a (T - Tinf) - q_e int N_a
 //   J_ab = sum_q c [k(T_q) g_a.g_b + k' (gradT_q.g_a) N_b] + sum_sides int h N_a N_b
-// One wave per element, lane = (a, b).
+// One wave per element.  Lanes (q, n) form the geometry at quadrature point q;
+// lane q then sums T_q and gradT_q over the 8 nodes (once per point, not per
+// matrix entry, in node order); lanes (a, b) form J_ab and R_a from the
+// per-point values in LDS.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void hex8_side_xi(int side, double s1, double s2, double& xi, double& eta,
                                              double& zeta, int& r0, int& r1) {
@@ -605,6 +608,7 @@ __global__ __launch_bounds__(256) void k_hex8_heat_K(HeatArgs A) {
     __shared__ double sc[4][8];
     __shared__ double sT[4][8];
     __shared__ double sX[4][8][3];
+    __shared__ double sTq[4][8], sgT[4][8][3];   // T and grad T at each quadrature point
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t e = int64_t(blockIdx.x) * 4 + wave;
     const bool valid = e < A.n_elems;
@@ -628,18 +632,18 @@ __global__ __launch_bounds__(256) void k_hex8_heat_K(HeatArgs A) {
             for (int i = 0; i < 9; i++) J[i] += __shfl_xor(J[i], m, 64);
         double ixi[3], ieta[3], izeta[3];
         const double jac = inv3(J, ixi, ieta, izeta);
-        sg[wave][q][n][0] = dxi * ixi[0] + deta * ieta[0] + dzeta * izeta[0];
-        sg[wave][q][n][1] = dxi * ixi[1] + deta * ieta[1] + dzeta * izeta[1];
-        sg[wave][q][n][2] = dxi * ixi[2] + deta * ieta[2] + dzeta * izeta[2];
+        const double g0 = dxi * ixi[0] + deta * ieta[0] + dzeta * izeta[0];
+        const double g1 = dxi * ixi[1] + deta * ieta[1] + dzeta * izeta[1];
+        const double g2 = dxi * ixi[2] + deta * ieta[2] + dzeta * izeta[2];
+        sg[wave][q][n][0] = g0;
+        sg[wave][q][n][1] = g1;
+        sg[wave][q][n][2] = g2;
         sN[wave][q][n] = N;
         if (n == 0) sc[wave][q] = jac * 1.0;
     }
     __syncthreads();
-    if (!valid) return;
-    const int a = lane >> 3, b = lane & 7;
-    double K = 0.0, f = 0.0;
-    const double dk = A.k0 * A.alpha;
-    for (int q = 0; q < 8; q++) {
+    if (lane < 8) {   // T_q = sum_n N_n T_n, gradT_q = sum_n g_n T_n
+        const int q = lane;
         double Tq = 0.0, gT0 = 0.0, gT1 = 0.0, gT2 = 0.0;
 #pragma unroll
         for (int n = 0; n < 8; n++) {
@@ -649,6 +653,19 @@ __global__ __launch_bounds__(256) void k_hex8_heat_K(HeatArgs A) {
             gT1 += sg[wave][q][n][1] * t;
             gT2 += sg[wave][q][n][2] * t;
         }
+        sTq[wave][q] = Tq;
+        sgT[wave][q][0] = gT0;
+        sgT[wave][q][1] = gT1;
+        sgT[wave][q][2] = gT2;
+    }
+    __syncthreads();
+    if (!valid) return;
+    const int a = lane >> 3, b = lane & 7;
+    double K = 0.0, f = 0.0;
+    const double dk = A.k0 * A.alpha;
+#pragma unroll 2
+    for (int q = 0; q < 8; q++) {
+        const double Tq = sTq[wave][q], gT0 = sgT[wave][q][0], gT1 = sgT[wave][q][1], gT2 = sgT[wave][q][2];
         const double c = sc[wave][q];
         const double kk = A.k0 * (1.0 + A.alpha * Tq);
         const double ga0 = sg[wave][q][a][0], ga1 = sg[wave][q][a][1], ga2 = sg[wave][q][a][2];
