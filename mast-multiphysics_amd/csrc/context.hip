@@ -62,7 +62,10 @@ struct MGLevelDev {
     DevBuf blk_ptr, blk_col, cptr, cidx, econn, con;   // levels >= 1
     DevBuf gal_dirty, gal_list;   // hex8 level 1: coarse cells touching constraints
     int64_t n_dirty = 0;
-    DevBuf sten;                  // hex8 level 1: the operator as a symmetric 27-point FP32 stencil
+    DevBuf sten;                  // hex8 level 1: the operator as a symmetric 27-point FP32 stencil;
+                                  // heat level 0: its stencil in lattice order
+    DevBuf lat_of_rank;           // heat level 0: lattice point of each node rank
+    bool lat0 = false;            // heat level 0 smooths in lattice order
     DevBuf Kel, val, val32, dinv, r, x, res, da, db;
 };
 
@@ -591,6 +594,25 @@ int mg_build(mastg_ctx* c) {
             // hierarchy before its first assembly
             TRY(dalloc(c, c->val, size_t(c->S.nnz) * sizeof(double)));
             TRY(dalloc(c, c->dinv, size_t(c->S.n_active) * sizeof(double)));
+            // heat: the V-cycle runs this level in lattice order (node-major)
+            // with a stencil smoother: the inverse map, constraint flags,
+            // inverse diagonal and right side in lattice order, the stencil
+            // (the plate's node-block smoother measured faster than
+            // lattice-ordered stencils: 324 vs 352 and 387 ms per 512^2 design)
+            if (nav == 1) {
+                std::vector<int32_t> lat_of_rank(size_t(h.n_nodes));
+                for (int64_t p = 0; p < h.n_nodes; p++) lat_of_rank[size_t(h.rank_of_lat[p])] = int32_t(p);
+                std::vector<uint8_t> con_lat(size_t(h.n_nodes) * nav);
+                for (int64_t p = 0; p < h.n_nodes; p++)
+                    for (int v = 0; v < nav; v++)
+                        con_lat[size_t(p * nav + v)] = c->S.constrained[size_t(h.rank_of_lat[p]) * nav + v];
+                TRY(upload(c, L->lat_of_rank, lat_of_rank));
+                TRY(upload(c, L->con_lat, con_lat));
+                TRY(dalloc(c, L->dinv_lat, nv));
+                TRY(dalloc(c, L->b_lat, nv));
+                TRY(dalloc(c, L->sten, size_t(h.n_nodes) * 27 * sizeof(float)));
+                L->lat0 = true;
+            }
             a.rank_of_lat = L->rank_of_lat.as<int32_t>();
             a.con = c->con.as<uint8_t>();
             a.blk_ptr = c->blk_ptr.as<int64_t>();
@@ -741,6 +763,11 @@ int mg_setup(mastg_ctx* c) {
         launch_to_f32(c->S.nnz, c->val.as<double>(), c->val32.as<float>(), s);
         launch_mg_power(L0.a, c->val.as<double>(), kMGPowerIters, kMGPowerSafety, L0.a.da, L0.a.db,
                         c->mg_part.as<double>(), c->mg_scal.as<double>(), c->mg_lam.as<double>(), s);
+        if (L0.lat0) {   // the lattice-ordered stencil and inverse diagonal of this assembly
+            launch_mg_to_stencil_rank(L0.a, L0.a.rank_of_lat, L0.lat_of_rank.as<int32_t>(), L0.sten.as<float>(), s);
+            launch_perm_rank_lat(L0.a.n_nodes, nav, L0.a.rank_of_lat, c->dinv.as<double>(), L0.dinv_lat.as<double>(),
+                                 1, nullptr, s);
+        }
     }
     for (int l = 0; l + 1 < nl; l++) {
         MGLevelDev& F = *c->mg[l];
@@ -814,9 +841,19 @@ bool lattice_args(mastg_ctx* c, CGArgs& a) {
 const void* mg_vcycle(mastg_ctx* c, const CGArgs& a, const int* done, hipStream_t s, const double* r0 = nullptr,
                       bool started = false) {
     const int nl = int(c->mg.size());
-    MGLevelArgs L0 = c->mg[0]->a;
+    const MGLevelDev& D0 = *c->mg[0];
+    MGLevelArgs L0 = D0.a;
     if (r0) L0.r = const_cast<double*>(r0);
     const bool f32 = L0.planar != 0;
+    if (D0.lat0) {
+        // heat: the problem level runs in lattice order (its output is mapped
+        // back to rank order by the caller: mg_rz / mg_bicg_iteration)
+        launch_perm_rank_lat(L0.n_nodes, L0.nav, L0.rank_of_lat, L0.r, D0.b_lat.as<double>(), 1, done, s);
+        L0.r = D0.b_lat.as<double>();
+        L0.dinv = D0.dinv_lat.as<double>();
+        L0.con = D0.con_lat.as<uint8_t>();
+        L0.rank_of_lat = nullptr;
+    }
     auto lv = [&](int l) -> const MGLevelArgs& { return l == 0 ? L0 : c->mg[l]->a; };
     auto dbuf = [&](int l, int i) { return (i % 2 == 0) ? lv(l).da : lv(l).db; };
     auto dbuf32 = [&](int i) { return (i % 2 == 0) ? L0.da32 : L0.db32; };
@@ -825,6 +862,9 @@ const void* mg_vcycle(mastg_ctx* c, const CGArgs& a, const int* done, hipStream_
         if (l == 0 && f32)
             launch_mg_fine(a, L0, mode, step, iin < 0 ? nullptr : dbuf32(iin), iout < 0 ? nullptr : dbuf32(iout),
                            done, s);
+        else if (l == 0 && D0.lat0)
+            launch_mg_stencil_rank(L0, D0.sten.as<float>(), mode, step, iin < 0 ? nullptr : dbuf(l, iin),
+                                   iout < 0 ? nullptr : dbuf(l, iout), done, s);
         else if (c->mg[l]->sten.p)
             launch_mg_stencil(lv(l), c->mg[l]->sten.as<float>(), mode, step, iin < 0 ? nullptr : dbuf(l, iin),
                               iout < 0 ? nullptr : dbuf(l, iout), done, s);
@@ -857,9 +897,17 @@ const void* mg_vcycle(mastg_ctx* c, const CGArgs& a, const int* done, hipStream_
 
 // z = x0 + dlast0 on the problem level and the partial sums of r.z
 void mg_rz(mastg_ctx* c, const CGArgs& a, const void* dl, hipStream_t s) {
-    const MGLevelArgs& L0 = c->mg[0]->a;
-    if (L0.planar) launch_pcg_rz32(a, L0.x32, static_cast<const float*>(dl), s);
-    else launch_pcg_rz(a, L0.x, static_cast<const double*>(dl), s);
+    const MGLevelDev& D0 = *c->mg[0];
+    const MGLevelArgs& L0 = D0.a;
+    if (L0.planar) {
+        launch_pcg_rz32(a, L0.x32, static_cast<const float*>(dl), s);
+    } else if (D0.lat0) {   // z = x0 + dlast back in rank order, then r.z
+        launch_mg_combine_rank(L0.n_nodes, L0.nav, L0.rank_of_lat, L0.x, static_cast<const double*>(dl), a.z,
+                               &a.S->done, s);
+        launch_pcg_rz(a, a.z, nullptr, s);
+    } else {
+        launch_pcg_rz(a, L0.x, static_cast<const double*>(dl), s);
+    }
 }
 
 void mg_pcg_iteration(mastg_ctx* c, const CGArgs& a, hipStream_t s) {
@@ -1047,11 +1095,14 @@ void mg_bicg_iteration(mastg_ctx* c, const BiCGArgs& a, hipStream_t s) {
     const int64_t n = c->S.n_active;
     launch_bicg_part(a, 0, s);
     // (plate and heat: FP64 problem-level state)
-    const double* dl = static_cast<const double*>(mg_vcycle(c, none, done, s, a.p));
-    launch_mg_combine(n, c->mg[0]->a.x, dl, a.ph, done, s);
+    const MGLevelDev& D0 = *c->mg[0];
+    auto combine = [&](const double* dl, double* out) {
+        if (D0.lat0) launch_mg_combine_rank(D0.a.n_nodes, D0.a.nav, D0.a.rank_of_lat, D0.a.x, dl, out, done, s);
+        else launch_mg_combine(n, D0.a.x, dl, out, done, s);
+    };
+    combine(static_cast<const double*>(mg_vcycle(c, none, done, s, a.p)), a.ph);
     launch_bicg_part(a, 1, s);
-    dl = static_cast<const double*>(mg_vcycle(c, none, done, s, a.s));
-    launch_mg_combine(n, c->mg[0]->a.x, dl, a.sh, done, s);
+    combine(static_cast<const double*>(mg_vcycle(c, none, done, s, a.s)), a.sh);
     launch_bicg_part(a, 2, s);
 }
 

@@ -249,6 +249,17 @@ void launch_mg_bsr(const MGLevelArgs& L, int mode, int step, const double* din, 
 // a lattice-ordered hex8 coarse level as a 27-point FP32 stencil
 // (sten[(o 9 + v 3 + w) n_nodes + r], o = the neighbour offset), built from its
 // FP32 node blocks, and a smoothing step with it (same modes as the node blocks)
+// heat problem level in lattice order: 27-point FP32 stencil from the
+// rank-ordered rows, its smoothing step, and the rank <-> lattice permutations
+// of node-major vectors
+void launch_mg_to_stencil_rank(const MGLevelArgs& L, const int32_t* rank_of_lat, const int32_t* lat_of_rank,
+                               float* sten, hipStream_t s);
+void launch_mg_stencil_rank(const MGLevelArgs& L, const float* sten, int mode, int step, const double* din,
+                            double* dout, const int* done, hipStream_t s);
+void launch_perm_rank_lat(int64_t n_nodes, int nav, const int32_t* rank_of_lat, const double* src, double* dst,
+                          int to_lat, const int* done, hipStream_t s);
+void launch_mg_combine_rank(int64_t n_nodes, int nav, const int32_t* rank_of_lat, const double* x0,
+                            const double* dlast, double* out, const int* done, hipStream_t s);
 void launch_mg_to_stencil(const MGLevelArgs& L, float* sten, hipStream_t s);
 void launch_mg_stencil(const MGLevelArgs& L, const float* sten, int mode, int step, const double* din, double* dout,
                        const int* done, hipStream_t s);

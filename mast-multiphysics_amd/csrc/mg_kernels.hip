@@ -357,6 +357,88 @@ __global__ __launch_bounds__(256) void k_mg_stencil(MGLevelArgs L, const float* 
     mg_epilogue(L, mode, r * 3 + 2, y2, din, dout, c1, c2, ith);
 }
 
+// ---- the heat problem level as a stencil in lattice order ------------------
+// Its Krylov solver keeps the caller's node-rank order; the V-cycle runs the
+// problem level in lattice order, so its smoother reads a stencil formed once
+// per set-up from the rank-ordered rows instead of values with a column index
+// each: entry o = (dx+1) + 3 (dy+1) + 9 (dz+1) of lattice point p at o N + p
+// (FP32, all 27 neighbours: the Jacobian is not symmetric; offsets outside
+// the lattice hold 0).  (For the plate, 5 variables per node, the node-block
+// smoother measured faster than lattice-ordered stencils.)
+// L: the level in rank order (1 variable); rank_of_lat / lat_of_rank map
+// lattice points to ranks and back
+__global__ __launch_bounds__(256) void k_mg_to_stencil_rank(MGLevelArgs L, const int32_t* __restrict__ rank_of_lat,
+                                                            const int32_t* __restrict__ lat_of_rank,
+                                                            float* __restrict__ sten) {
+    const int64_t p = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (p >= L.n_nodes) return;
+    const int64_t N = L.n_nodes;
+    const uint32_t nxp = uint32_t(L.nx + 1), nyp = uint32_t(L.ny + 1), pu = uint32_t(p), pq = pu / nxp;
+    const int pi = int(pu - pq * nxp), pj = int(pq % nyp), pk = int(pq / nyp);
+    for (int e = 0; e < 27; e++) sten[e * N + p] = 0.f;
+    const int64_t r = rank_of_lat[p];
+    const int64_t k0 = L.blk_ptr[r];
+    const int deg = int(L.blk_ptr[r + 1] - k0);
+    for (int k = 0; k < deg; k++) {
+        const uint32_t qu = uint32_t(lat_of_rank[L.blk_col[k0 + k]]), qq = qu / nxp;
+        const int dx = int(qu - qq * nxp) - pi, dy = int(qq % nyp) - pj, dz = int(qq / nyp) - pk;
+        if (dx < -1 || dx > 1 || dy < -1 || dy > 1 || dz < -1 || dz > 1) continue;   // not a lattice neighbour
+        sten[((dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)) * N + p] = L.val32[k0 + k];
+    }
+}
+
+// smoothing step of the heat problem level in lattice order (FP32 stencil, FP64
+// vectors and sums, fused Chebyshev epilogue)
+__global__ __launch_bounds__(256) void k_mg_stencil_heat(MGLevelArgs L, const float* __restrict__ sten, int mode,
+                                                         int step, const double* __restrict__ din,
+                                                         double* __restrict__ dout, const int* __restrict__ done) {
+    if (done && *done) return;
+    const int64_t p = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (p >= L.n_nodes) return;
+    const int64_t N = L.n_nodes;
+    const double* __restrict__ in = mode == MG_RESID_INIT ? L.x : din;
+    const uint32_t nxp = uint32_t(L.nx + 1), nyp = uint32_t(L.ny + 1), pu = uint32_t(p), pq = pu / nxp;
+    const int i = int(pu - pq * nxp), j = int(pq % nyp), k = int(pq / nyp);
+    double y = 0.0;
+#pragma unroll 9
+    for (int o = 0; o < 27; o++) {
+        // neighbours outside the lattice are clamped into it (their entries are 0)
+        const int ni = min(max(i + o % 3 - 1, 0), L.nx), nj = min(max(j + (o / 3) % 3 - 1, 0), L.ny),
+                  nk = min(max(k + o / 9 - 1, 0), L.nz);
+        const int64_t m = int64_t(uint32_t(ni) + nxp * (uint32_t(nj) + nyp * uint32_t(nk)));
+        y = fma(double(sten[o * N + p]), in[m], y);
+    }
+    const double ith = L.cheb[0];
+    const double c1 = step > 0 ? L.cheb[2 * step] : 0.0, c2 = step > 0 ? L.cheb[2 * step + 1] : 0.0;
+    mg_epilogue(L, mode, p, y, din, dout, c1, c2, ith);
+}
+
+// node-major vectors between rank order and lattice order: to_lat = 1:
+// dst[p nav + v] = src[rank_of_lat[p] nav + v], else the inverse scatter
+__global__ __launch_bounds__(256) void k_perm_rank_lat(int64_t n_nodes, int nav, const int32_t* __restrict__ rank_of_lat,
+                                                       const double* __restrict__ src, double* __restrict__ dst,
+                                                       int to_lat, const int* __restrict__ done) {
+    if (done && *done) return;
+    const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (i >= n_nodes * nav) return;
+    const int64_t p = i / nav, v = i - p * nav, r = int64_t(rank_of_lat[p]) * nav + v;
+    if (to_lat) dst[i] = src[r];
+    else dst[r] = src[i];
+}
+
+// out (rank order) = x0 + dlast (lattice order): the V-cycle's output
+__global__ __launch_bounds__(256) void k_mg_combine_rank(int64_t n_nodes, int nav,
+                                                         const int32_t* __restrict__ rank_of_lat,
+                                                         const double* __restrict__ x0,
+                                                         const double* __restrict__ dlast, double* __restrict__ out,
+                                                         const int* __restrict__ done) {
+    if (done && *done) return;
+    const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (i >= n_nodes * nav) return;
+    const int64_t p = i / nav, v = i - p * nav;
+    out[int64_t(rank_of_lat[p]) * nav + v] = x0[i] + dlast[i];
+}
+
 // ---- level 0 on a lattice: per-node matrix-free operator -------------------
 // Thread per lattice point n, vectors component-major in lattice order (x
 // fastest), so each neighbour gather of a wave is one contiguous 512-byte run;
@@ -1206,6 +1288,26 @@ void launch_mg_stencil(const MGLevelArgs& L, const float* sten, int mode, int st
                        const int* done, hipStream_t s) {
     hipLaunchKernelGGL(k_mg_stencil, dim3(mg_blocks(L.n_nodes, 256)), dim3(256), 0, s, L, sten, mode, step, din,
                        dout, done);
+}
+void launch_mg_to_stencil_rank(const MGLevelArgs& L, const int32_t* rank_of_lat, const int32_t* lat_of_rank,
+                               float* sten, hipStream_t s) {
+    hipLaunchKernelGGL(k_mg_to_stencil_rank, dim3(mg_blocks(L.n_nodes, 256)), dim3(256), 0, s, L, rank_of_lat,
+                       lat_of_rank, sten);
+}
+void launch_mg_stencil_rank(const MGLevelArgs& L, const float* sten, int mode, int step, const double* din,
+                            double* dout, const int* done, hipStream_t s) {
+    hipLaunchKernelGGL(k_mg_stencil_heat, dim3(mg_blocks(L.n_nodes, 256)), dim3(256), 0, s, L, sten, mode, step, din,
+                       dout, done);
+}
+void launch_perm_rank_lat(int64_t n_nodes, int nav, const int32_t* rank_of_lat, const double* src, double* dst,
+                          int to_lat, const int* done, hipStream_t s) {
+    hipLaunchKernelGGL(k_perm_rank_lat, dim3(mg_blocks(n_nodes * nav, 256)), dim3(256), 0, s, n_nodes, nav,
+                       rank_of_lat, src, dst, to_lat, done);
+}
+void launch_mg_combine_rank(int64_t n_nodes, int nav, const int32_t* rank_of_lat, const double* x0,
+                            const double* dlast, double* out, const int* done, hipStream_t s) {
+    hipLaunchKernelGGL(k_mg_combine_rank, dim3(mg_blocks(n_nodes * nav, 256)), dim3(256), 0, s, n_nodes, nav,
+                       rank_of_lat, x0, dlast, out, done);
 }
 void launch_mg_bsr(const MGLevelArgs& L, int mode, int step, const double* din, double* dout, const int* done,
                    hipStream_t s) {
