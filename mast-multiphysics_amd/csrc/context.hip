@@ -63,7 +63,7 @@ struct MGLevelDev {
     DevBuf gal_dirty, gal_list;   // hex8 level 1: coarse cells touching constraints
     int64_t n_dirty = 0;
     DevBuf sten;                  // hex8 level 1: the operator as a symmetric 27-point FP32 stencil;
-                                  // heat level 0: its stencil in lattice order
+                                  // heat levels 0 and 1: 27-point stencils in lattice order
     DevBuf lat_of_rank;           // heat level 0: lattice point of each node rank
     bool lat0 = false;            // heat level 0 smooths in lattice order
     DevBuf Kel, val, val32, dinv, r, x, res, da, db;
@@ -698,6 +698,11 @@ int mg_build(mastg_ctx* c) {
         }
         lv.push_back(std::move(L));
     }
+    if (c->d.physics == MASTG_HEX8_HEAT && nl > 2) {
+        // heat: the first coarse level (lattice order) smooths with a 27-point
+        // stencil too when it is not the coarsest
+        TRY(dalloc(c, lv[1]->sten, size_t(H.levels[1].n_nodes) * 27 * sizeof(float)));
+    }
     if (!plate) {
         // one-class lattice: the children's Galerkin products of the class
         // matrix, and the first coarse level's cells that touch constraints
@@ -807,7 +812,10 @@ int mg_setup(mastg_ctx* c) {
             launch_mg_power(C.a, C.a.val, kMGPowerIters, kMGPowerSafety, C.a.da, C.a.db, c->mg_part.as<double>(),
                             c->mg_scal.as<double>(), c->mg_lam.as<double>() + l + 1, s);
         launch_to_f32(int64_t(C.val32.bytes / sizeof(float)), C.a.val, C.val32.as<float>(), s);
-        if (C.sten.p) launch_mg_to_stencil(C.a, C.sten.as<float>(), s);
+        if (C.sten.p) {
+            if (nav == 1) launch_mg_to_stencil_rank(C.a, nullptr, nullptr, C.sten.as<float>(), s);
+            else launch_mg_to_stencil(C.a, C.sten.as<float>(), s);
+        }
     }
     launch_mg_cheb(c->mg_lam.as<double>(), c->mg_cheb.as<double>(), nl, std::max(c->mg_k0, c->mg_kc), 30.0, s);
     const MGLevelDev& Lc = *c->mg[nl - 1];
@@ -864,6 +872,9 @@ const void* mg_vcycle(mastg_ctx* c, const CGArgs& a, const int* done, hipStream_
                            done, s);
         else if (l == 0 && D0.lat0)
             launch_mg_stencil_rank(L0, D0.sten.as<float>(), mode, step, iin < 0 ? nullptr : dbuf(l, iin),
+                                   iout < 0 ? nullptr : dbuf(l, iout), done, s);
+        else if (c->mg[l]->sten.p && lv(l).nav == 1)
+            launch_mg_stencil_rank(lv(l), c->mg[l]->sten.as<float>(), mode, step, iin < 0 ? nullptr : dbuf(l, iin),
                                    iout < 0 ? nullptr : dbuf(l, iout), done, s);
         else if (c->mg[l]->sten.p)
             launch_mg_stencil(lv(l), c->mg[l]->sten.as<float>(), mode, step, iin < 0 ? nullptr : dbuf(l, iin),

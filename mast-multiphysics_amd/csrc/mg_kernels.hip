@@ -366,7 +366,8 @@ __global__ __launch_bounds__(256) void k_mg_stencil(MGLevelArgs L, const float* 
 // the lattice hold 0).  (For the plate, 5 variables per node, the node-block
 // smoother measured faster than lattice-ordered stencils.)
 // L: the level in rank order (1 variable); rank_of_lat / lat_of_rank map
-// lattice points to ranks and back
+// lattice points to ranks and back (both null: the level is in lattice order,
+// as the coarse levels are)
 __global__ __launch_bounds__(256) void k_mg_to_stencil_rank(MGLevelArgs L, const int32_t* __restrict__ rank_of_lat,
                                                             const int32_t* __restrict__ lat_of_rank,
                                                             float* __restrict__ sten) {
@@ -376,11 +377,12 @@ __global__ __launch_bounds__(256) void k_mg_to_stencil_rank(MGLevelArgs L, const
     const uint32_t nxp = uint32_t(L.nx + 1), nyp = uint32_t(L.ny + 1), pu = uint32_t(p), pq = pu / nxp;
     const int pi = int(pu - pq * nxp), pj = int(pq % nyp), pk = int(pq / nyp);
     for (int e = 0; e < 27; e++) sten[e * N + p] = 0.f;
-    const int64_t r = rank_of_lat[p];
+    const int64_t r = rank_of_lat ? rank_of_lat[p] : p;
     const int64_t k0 = L.blk_ptr[r];
     const int deg = int(L.blk_ptr[r + 1] - k0);
     for (int k = 0; k < deg; k++) {
-        const uint32_t qu = uint32_t(lat_of_rank[L.blk_col[k0 + k]]), qq = qu / nxp;
+        const int32_t col = L.blk_col[k0 + k];
+        const uint32_t qu = uint32_t(lat_of_rank ? lat_of_rank[col] : col), qq = qu / nxp;
         const int dx = int(qu - qq * nxp) - pi, dy = int(qq % nyp) - pj, dz = int(qq / nyp) - pk;
         if (dx < -1 || dx > 1 || dy < -1 || dy > 1 || dz < -1 || dz > 1) continue;   // not a lattice neighbour
         sten[((dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)) * N + p] = L.val32[k0 + k];
