@@ -766,12 +766,19 @@ int mg_setup(mastg_ctx* c) {
         launch_mg_rowinfo_nb(nav, c->S.n_nodes, c->blk_ptr.as<int64_t>(), c->blk_col.as<int32_t>(),
                              c->val.as<double>(), nullptr, c->mg_part.as<double>(), c->mg_lam.as<double>(), s);
         launch_to_f32(c->S.nnz, c->val.as<double>(), c->val32.as<float>(), s);
-        launch_mg_power(L0.a, c->val.as<double>(), kMGPowerIters, kMGPowerSafety, L0.a.da, L0.a.db,
-                        c->mg_part.as<double>(), c->mg_scal.as<double>(), c->mg_lam.as<double>(), s);
-        if (L0.lat0) {   // the lattice-ordered stencil and inverse diagonal of this assembly
+        if (L0.lat0) {
+            // heat: the lattice-ordered stencil and inverse diagonal of this
+            // assembly, and the Chebyshev bound from them
             launch_mg_to_stencil_rank(L0.a, L0.a.rank_of_lat, L0.lat_of_rank.as<int32_t>(), L0.sten.as<float>(), s);
             launch_perm_rank_lat(L0.a.n_nodes, nav, L0.a.rank_of_lat, c->dinv.as<double>(), L0.dinv_lat.as<double>(),
                                  1, nullptr, s);
+            MGLevelArgs Ll = L0.a;
+            Ll.dinv = L0.dinv_lat.as<double>();
+            launch_mg_power_heat(Ll, L0.sten.as<float>(), L0.a.rank_of_lat, kMGPowerIters, kMGPowerSafety, L0.a.da,
+                                 L0.a.db, c->mg_part.as<double>(), c->mg_scal.as<double>(), c->mg_lam.as<double>(), s);
+        } else {
+            launch_mg_power(L0.a, c->val.as<double>(), kMGPowerIters, kMGPowerSafety, L0.a.da, L0.a.db,
+                            c->mg_part.as<double>(), c->mg_scal.as<double>(), c->mg_lam.as<double>(), s);
         }
     }
     for (int l = 0; l + 1 < nl; l++) {

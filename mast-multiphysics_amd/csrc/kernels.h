@@ -258,6 +258,11 @@ void launch_mg_stencil_rank(const MGLevelArgs& L, const float* sten, int mode, i
                             double* dout, const int* done, hipStream_t s);
 void launch_perm_rank_lat(int64_t n_nodes, int nav, const int32_t* rank_of_lat, const double* src, double* dst,
                           int to_lat, const int* done, hipStream_t s);
+// lambda_max(D^-1 A) of the heat problem level by power iteration on its
+// lattice-ordered stencil (L: the level in lattice order, its inverse diagonal)
+void launch_mg_power_heat(const MGLevelArgs& L, const float* sten, const int32_t* rank_of_lat, int n_it,
+                          double safety, double* w0, double* w1, double* part, double* scal, double* lam,
+                          hipStream_t s);
 void launch_mg_combine_rank(int64_t n_nodes, int nav, const int32_t* rank_of_lat, const double* x0,
                             const double* dlast, double* out, const int* done, hipStream_t s);
 void launch_mg_to_stencil(const MGLevelArgs& L, float* sten, hipStream_t s);

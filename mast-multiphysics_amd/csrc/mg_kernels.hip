@@ -415,6 +415,38 @@ __global__ __launch_bounds__(256) void k_mg_stencil_heat(MGLevelArgs L, const fl
     mg_epilogue(L, mode, p, y, din, dout, c1, c2, ith);
 }
 
+// power iteration for lambda_max(D^-1 A) on the heat problem level in lattice
+// order with its stencil: out = D^-1 A (s in), s = *scale; the start vector
+// (in == null) is the rank-ordered one, 1 + ((rank 7919) mod 1000) / 1000, read
+// through rank_of_lat, so the iterates are those of the node-block form
+// permuted; partial sums of out.out per work-group
+__global__ __launch_bounds__(256) void k_mg_power_heat(MGLevelArgs L, const float* __restrict__ sten,
+                                                       const int32_t* __restrict__ rank_of_lat,
+                                                       const double* __restrict__ in, const double* __restrict__ scale,
+                                                       double* __restrict__ out, double* __restrict__ part) {
+    const int64_t p = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    double acc = 0.0;
+    if (p < L.n_nodes) {
+        const int64_t N = L.n_nodes;
+        const double sc = in ? *scale : 1.0;
+        const uint32_t nxp = uint32_t(L.nx + 1), nyp = uint32_t(L.ny + 1), pu = uint32_t(p), pq = pu / nxp;
+        const int i = int(pu - pq * nxp), j = int(pq % nyp), k = int(pq / nyp);
+        double y = 0.0;
+#pragma unroll 9
+        for (int o = 0; o < 27; o++) {
+            const int ni = min(max(i + o % 3 - 1, 0), L.nx), nj = min(max(j + (o / 3) % 3 - 1, 0), L.ny),
+                      nk = min(max(k + o / 9 - 1, 0), L.nz);
+            const int64_t m = int64_t(uint32_t(ni) + nxp * (uint32_t(nj) + nyp * uint32_t(nk)));
+            const double x = in ? sc * in[m] : 1.0 + double((int64_t(rank_of_lat[m]) * 7919) % 1000) / 1000.0;
+            y = fma(double(sten[o * N + p]), x, y);
+        }
+        const double o = L.dinv[p] * y;
+        out[p] = o;
+        acc = o * o;
+    }
+    block_sum_store(acc, part, blockIdx.x);
+}
+
 // node-major vectors between rank order and lattice order: to_lat = 1:
 // dst[p nav + v] = src[rank_of_lat[p] nav + v], else the inverse scatter
 __global__ __launch_bounds__(256) void k_perm_rank_lat(int64_t n_nodes, int nav, const int32_t* __restrict__ rank_of_lat,
@@ -1300,6 +1332,22 @@ void launch_mg_stencil_rank(const MGLevelArgs& L, const float* sten, int mode, i
                             double* dout, const int* done, hipStream_t s) {
     hipLaunchKernelGGL(k_mg_stencil_heat, dim3(mg_blocks(L.n_nodes, 256)), dim3(256), 0, s, L, sten, mode, step, din,
                        dout, done);
+}
+void launch_mg_power_heat(const MGLevelArgs& L, const float* sten, const int32_t* rank_of_lat, int n_it,
+                          double safety, double* w0, double* w1, double* part, double* scal, double* lam,
+                          hipStream_t s) {
+    const int nb = int(mg_blocks(L.n_nodes, 256));
+    const int nbs = int(std::min<int64_t>(mg_blocks(L.n_nodes, 256), 8192));
+    hipLaunchKernelGGL(k_mg_power_start, dim3(nbs), dim3(256), 0, s, L.n_nodes, part);
+    hipLaunchKernelGGL(k_mg_power_start_norm, dim3(1), dim3(256), 0, s, part, nbs, scal);
+    double* in = nullptr;
+    double* out = w0;
+    for (int it = 0; it < n_it; it++) {
+        hipLaunchKernelGGL(k_mg_power_heat, dim3(nb), dim3(256), 0, s, L, sten, rank_of_lat, in, scal, out, part);
+        hipLaunchKernelGGL(k_mg_power_norm, dim3(1), dim3(256), 0, s, part, nb, it == 0 ? 1 : 0, safety, scal, lam);
+        in = out;
+        out = out == w0 ? w1 : w0;
+    }
 }
 void launch_perm_rank_lat(int64_t n_nodes, int nav, const int32_t* rank_of_lat, const double* src, double* dst,
                           int to_lat, const int* done, hipStream_t s) {
