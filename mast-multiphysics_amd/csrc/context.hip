@@ -64,7 +64,7 @@ struct MGLevelDev {
     int64_t n_dirty = 0;
     DevBuf sten;                  // hex8 level 1: the operator as a symmetric 27-point FP32 stencil;
                                   // heat levels 0 and 1: 27-point stencils in lattice order
-    DevBuf lat_of_rank;           // heat level 0: lattice point of each node rank
+    DevBuf kpos;                  // heat level 0: row position of each stencil entry (255: none)
     bool lat0 = false;            // heat level 0 smooths in lattice order
     DevBuf Kel, val, val32, dinv, r, x, res, da, db;
 };
@@ -595,8 +595,8 @@ int mg_build(mastg_ctx* c) {
             TRY(dalloc(c, c->val, size_t(c->S.nnz) * sizeof(double)));
             TRY(dalloc(c, c->dinv, size_t(c->S.n_active) * sizeof(double)));
             // heat: the V-cycle runs this level in lattice order (node-major)
-            // with a stencil smoother: the inverse map, constraint flags,
-            // inverse diagonal and right side in lattice order, the stencil
+            // with a stencil smoother: the stencil's row positions, constraint
+            // flags, inverse diagonal and right side in lattice order, the stencil
             // (the plate's node-block smoother measured faster than
             // lattice-ordered stencils: 324 vs 352 and 387 ms per 512^2 design)
             if (nav == 1) {
@@ -606,7 +606,21 @@ int mg_build(mastg_ctx* c) {
                 for (int64_t p = 0; p < h.n_nodes; p++)
                     for (int v = 0; v < nav; v++)
                         con_lat[size_t(p * nav + v)] = c->S.constrained[size_t(h.rank_of_lat[p]) * nav + v];
-                TRY(upload(c, L->lat_of_rank, lat_of_rank));
+                // position of neighbour o of lattice point p in p's row (the
+                // stencil is gathered through it at every set-up)
+                std::vector<uint8_t> kpos(size_t(h.n_nodes) * 27, uint8_t(255));
+                const int64_t nxp = h.nx + 1, nyp = h.ny + 1;
+                for (int64_t p = 0; p < h.n_nodes; p++) {
+                    const int64_t r = h.rank_of_lat[p], k0 = c->S.blk_ptr[r];
+                    const int64_t pi = p % nxp, pj = (p / nxp) % nyp, pk = p / (nxp * nyp);
+                    for (int64_t k = k0; k < c->S.blk_ptr[r + 1]; k++) {
+                        const int64_t q = lat_of_rank[size_t(c->S.blk_col[k])];
+                        const int64_t dx = q % nxp - pi, dy = (q / nxp) % nyp - pj, dz = q / (nxp * nyp) - pk;
+                        if (dx < -1 || dx > 1 || dy < -1 || dy > 1 || dz < -1 || dz > 1 || k - k0 > 254) continue;
+                        kpos[size_t(p * 27 + (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1))] = uint8_t(k - k0);
+                    }
+                }
+                TRY(upload(c, L->kpos, kpos));
                 TRY(upload(c, L->con_lat, con_lat));
                 TRY(dalloc(c, L->dinv_lat, nv));
                 TRY(dalloc(c, L->b_lat, nv));
@@ -769,7 +783,7 @@ int mg_setup(mastg_ctx* c) {
         if (L0.lat0) {
             // heat: the lattice-ordered stencil and inverse diagonal of this
             // assembly, and the Chebyshev bound from them
-            launch_mg_to_stencil_rank(L0.a, L0.a.rank_of_lat, L0.lat_of_rank.as<int32_t>(), L0.sten.as<float>(), s);
+            launch_mg_gather_stencil(L0.a, L0.a.rank_of_lat, L0.kpos.as<uint8_t>(), L0.sten.as<float>(), s);
             launch_perm_rank_lat(L0.a.n_nodes, nav, L0.a.rank_of_lat, c->dinv.as<double>(), L0.dinv_lat.as<double>(),
                                  1, nullptr, s);
             MGLevelArgs Ll = L0.a;
@@ -820,7 +834,7 @@ int mg_setup(mastg_ctx* c) {
                             c->mg_scal.as<double>(), c->mg_lam.as<double>() + l + 1, s);
         launch_to_f32(int64_t(C.val32.bytes / sizeof(float)), C.a.val, C.val32.as<float>(), s);
         if (C.sten.p) {
-            if (nav == 1) launch_mg_to_stencil_rank(C.a, nullptr, nullptr, C.sten.as<float>(), s);
+            if (nav == 1) launch_mg_to_stencil27(C.a, C.sten.as<float>(), s);
             else launch_mg_to_stencil(C.a, C.sten.as<float>(), s);
         }
     }

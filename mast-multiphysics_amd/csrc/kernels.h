@@ -249,11 +249,13 @@ void launch_mg_bsr(const MGLevelArgs& L, int mode, int step, const double* din, 
 // a lattice-ordered hex8 coarse level as a 27-point FP32 stencil
 // (sten[(o 9 + v 3 + w) n_nodes + r], o = the neighbour offset), built from its
 // FP32 node blocks, and a smoothing step with it (same modes as the node blocks)
-// heat problem level in lattice order: 27-point FP32 stencil from the
-// rank-ordered rows, its smoothing step, and the rank <-> lattice permutations
-// of node-major vectors
-void launch_mg_to_stencil_rank(const MGLevelArgs& L, const int32_t* rank_of_lat, const int32_t* lat_of_rank,
-                               float* sten, hipStream_t s);
+// heat levels in lattice order: 27-point FP32 stencil of a lattice-ordered
+// coarse level from its rows, its smoothing step, and the rank <-> lattice
+// permutations of node-major vectors
+void launch_mg_to_stencil27(const MGLevelArgs& L, float* sten, hipStream_t s);
+// the same stencil from precomputed row positions (kpos[p 27 + o], 255 = none)
+void launch_mg_gather_stencil(const MGLevelArgs& L, const int32_t* rank_of_lat, const uint8_t* kpos, float* sten,
+                              hipStream_t s);
 void launch_mg_stencil_rank(const MGLevelArgs& L, const float* sten, int mode, int step, const double* din,
                             double* dout, const int* done, hipStream_t s);
 void launch_perm_rank_lat(int64_t n_nodes, int nav, const int32_t* rank_of_lat, const double* src, double* dst,

@@ -365,27 +365,40 @@ __global__ __launch_bounds__(256) void k_mg_stencil(MGLevelArgs L, const float* 
 // (FP32, all 27 neighbours: the Jacobian is not symmetric; offsets outside
 // the lattice hold 0).  (For the plate, 5 variables per node, the node-block
 // smoother measured faster than lattice-ordered stencils.)
-// L: the level in rank order (1 variable); rank_of_lat / lat_of_rank map
-// lattice points to ranks and back (both null: the level is in lattice order,
-// as the coarse levels are)
-__global__ __launch_bounds__(256) void k_mg_to_stencil_rank(MGLevelArgs L, const int32_t* __restrict__ rank_of_lat,
-                                                            const int32_t* __restrict__ lat_of_rank,
-                                                            float* __restrict__ sten) {
+// A coarse level (lattice order, 1 variable) converted from its rows; the
+// problem level (rank order) is gathered through precomputed row positions
+// (k_mg_gather_stencil).
+__global__ __launch_bounds__(256) void k_mg_to_stencil27(MGLevelArgs L, float* __restrict__ sten) {
     const int64_t p = int64_t(blockIdx.x) * 256 + threadIdx.x;
     if (p >= L.n_nodes) return;
     const int64_t N = L.n_nodes;
     const uint32_t nxp = uint32_t(L.nx + 1), nyp = uint32_t(L.ny + 1), pu = uint32_t(p), pq = pu / nxp;
     const int pi = int(pu - pq * nxp), pj = int(pq % nyp), pk = int(pq / nyp);
     for (int e = 0; e < 27; e++) sten[e * N + p] = 0.f;
-    const int64_t r = rank_of_lat ? rank_of_lat[p] : p;
-    const int64_t k0 = L.blk_ptr[r];
-    const int deg = int(L.blk_ptr[r + 1] - k0);
+    const int64_t k0 = L.blk_ptr[p];
+    const int deg = int(L.blk_ptr[p + 1] - k0);
     for (int k = 0; k < deg; k++) {
-        const int32_t col = L.blk_col[k0 + k];
-        const uint32_t qu = uint32_t(lat_of_rank ? lat_of_rank[col] : col), qq = qu / nxp;
+        const uint32_t qu = uint32_t(L.blk_col[k0 + k]), qq = qu / nxp;
         const int dx = int(qu - qq * nxp) - pi, dy = int(qq % nyp) - pj, dz = int(qq / nyp) - pk;
         if (dx < -1 || dx > 1 || dy < -1 || dy > 1 || dz < -1 || dz > 1) continue;   // not a lattice neighbour
         sten[((dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)) * N + p] = L.val32[k0 + k];
+    }
+}
+
+// the same conversion with the row positions known: kpos[p 27 + o] is the
+// position in lattice point p's (rank-ordered) row of its neighbour o, 255 for
+// none (the mesh fixes it, so it is formed once on the host)
+__global__ __launch_bounds__(256) void k_mg_gather_stencil(MGLevelArgs L, const int32_t* __restrict__ rank_of_lat,
+                                                           const uint8_t* __restrict__ kpos,
+                                                           float* __restrict__ sten) {
+    const int64_t p = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (p >= L.n_nodes) return;
+    const int64_t N = L.n_nodes;
+    const float* __restrict__ V = L.val32 + L.blk_ptr[rank_of_lat[p]];
+#pragma unroll 9
+    for (int o = 0; o < 27; o++) {
+        const int kk = kpos[p * 27 + o];
+        sten[o * N + p] = kk == 255 ? 0.f : V[kk];
     }
 }
 
@@ -1323,10 +1336,13 @@ void launch_mg_stencil(const MGLevelArgs& L, const float* sten, int mode, int st
     hipLaunchKernelGGL(k_mg_stencil, dim3(mg_blocks(L.n_nodes, 256)), dim3(256), 0, s, L, sten, mode, step, din,
                        dout, done);
 }
-void launch_mg_to_stencil_rank(const MGLevelArgs& L, const int32_t* rank_of_lat, const int32_t* lat_of_rank,
-                               float* sten, hipStream_t s) {
-    hipLaunchKernelGGL(k_mg_to_stencil_rank, dim3(mg_blocks(L.n_nodes, 256)), dim3(256), 0, s, L, rank_of_lat,
-                       lat_of_rank, sten);
+void launch_mg_to_stencil27(const MGLevelArgs& L, float* sten, hipStream_t s) {
+    hipLaunchKernelGGL(k_mg_to_stencil27, dim3(mg_blocks(L.n_nodes, 256)), dim3(256), 0, s, L, sten);
+}
+void launch_mg_gather_stencil(const MGLevelArgs& L, const int32_t* rank_of_lat, const uint8_t* kpos, float* sten,
+                              hipStream_t s) {
+    hipLaunchKernelGGL(k_mg_gather_stencil, dim3(mg_blocks(L.n_nodes, 256)), dim3(256), 0, s, L, rank_of_lat, kpos,
+                       sten);
 }
 void launch_mg_stencil_rank(const MGLevelArgs& L, const float* sten, int mode, int step, const double* din,
                             double* dout, const int* done, hipStream_t s) {
