@@ -2,7 +2,7 @@
 on the oracle's assembled matrix: trilinear transfers with constrained rows /
 columns removed, Galerkin coarse operators (constrained coarse DOFs get a unit
 diagonal per touching element), Chebyshev smoothing on D^-1 A over
-[lam/30, lam] with lam the Gershgorin bound max_i sum_j |a_ij| / a_ii, `k`
+[lam/80, lam] with lam the Gershgorin bound max_i sum_j |a_ij| / a_ii, `k`
 steps before and after the coarse correction, exact coarsest solve.  Used to
 pin the GPU iteration count, which only matches when every piece matches."""
 import numpy as np
@@ -91,7 +91,7 @@ def build_levels(P, dims, h, max_coarse=600):
     return levels, b
 
 
-def chebyshev(A, dinv, r, x, k, lam, ratio=30.0):
+def chebyshev(A, dinv, r, x, k, lam, ratio=80.0):
     a, bnd = lam / ratio, lam
     theta, delta = 0.5 * (bnd + a), 0.5 * (bnd - a)
     sigma = theta / delta
