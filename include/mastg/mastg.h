@@ -244,8 +244,8 @@ int mastg_get_tuning(const mastg_ctx* ctx, int* spmv_lanes, int* grid);
 #define MASTG_OPT_GRAPHS 1
 /* multigrid smoothing: Chebyshev steps per pre-/post-smoothing on the problem
  * level (MASTG_OPT_MG_STEPS_FINE) and on the coarse levels
- * (MASTG_OPT_MG_STEPS_COARSE); 1..8, defaults: hex8 elasticity 5 and 2,
- * plate 4 and 2, heat 3 and 2 */
+ * (MASTG_OPT_MG_STEPS_COARSE); 1..8, defaults: hex8 elasticity 6 and 2,
+ * plate and heat 4 and 2 */
 #define MASTG_OPT_MG_STEPS_FINE 2
 #define MASTG_OPT_MG_STEPS_COARSE 3
 int mastg_set_option(mastg_ctx* ctx, int option, int value);

@@ -1370,9 +1370,11 @@ int mastg_ctx_create(const mastg_problem_desc* desc, mastg_ctx** out) {
         c->d.constrained_dofs = nullptr; c->d.side_elem = nullptr; c->d.side_index = nullptr;
         c->d.side_id = nullptr; c->d.point_loads = nullptr;
         if (c->d.penalty == 0) c->d.penalty = 3.0;
-        // smoothing steps measured fastest per design (128x64x64 cantilever,
-        // 512^2 plate, 96^3 heat): the coarse levels are launch-latency bound
-        c->mg_k0 = c->d.physics == MASTG_HEX8_HEAT ? 3 : c->d.physics == MASTG_HEX8_ELASTIC ? 5 : 4;
+        // smoothing steps measured fastest per design (128x64x64 cantilever:
+        // 6/2 14.8 ms, 5/2 15.7, 7/2 15.8, 5/3 15.1; 512^2 plate: 4/2; 96^3
+        // heat: 4/2 130.8 ms, 3/2 131.7): the coarse levels are launch-latency
+        // bound, so extra coarse steps do not pay
+        c->mg_k0 = c->d.physics == MASTG_HEX8_ELASTIC ? 6 : 4;
         c->mg_kc = 2;
         std::string m = build_structure(desc->physics, desc->n_nodes, desc->n_elems, desc->xyz, desc->conn,
                                         desc->n_vars, desc->node_dof_base, desc->n_constrained,

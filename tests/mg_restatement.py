@@ -119,7 +119,7 @@ def vcycle(levels, l, r, k):
     return chebyshev(A, dinv, r, x, kl, L["lam"])
 
 
-def pcg_iterations(levels, b, k=(5, 2), rtol=1e-10, max_it=1000):
+def pcg_iterations(levels, b, k=(6, 2), rtol=1e-10, max_it=1000):
     A = levels[0]["A"]
     x = np.zeros_like(b); r = b.copy()
     z = vcycle(levels, 0, r, k); p = z.copy(); rz = r @ z

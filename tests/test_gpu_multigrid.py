@@ -36,7 +36,7 @@ def test_mg_design_eval(oracle, mastg, mesh, lo):
 
 @pytest.mark.parametrize("steps", [None, (3, 3), (2, 4)])
 def test_mg_iterations_match_restatement(oracle, mastg, steps):
-    """Default smoothing (5 steps on the problem level, 2 on coarse levels) and
+    """Default smoothing (6 steps on the problem level, 2 on coarse levels) and
     two settings through the options.  The problem level smooths in FP32 (the
     restatement in FP64): iteration counts agree to +-1."""
     print(f"[mg vs scipy restatement] 16x8x8 steps={steps}", flush=True)
@@ -44,7 +44,7 @@ def test_mg_iterations_match_restatement(oracle, mastg, steps):
     P = oracle.Problem(oracle.CANTILEVER, *mesh, seed=3, lo=0.01, hi=1.0)
     levels, b = MR.build_levels(P, mesh, np.array([1.0 / 16, 0.5 / 8, 0.5 / 8]))
     assert [L["A"].shape[0] for L in levels] == [4131, 675, 135]
-    it_ref, _ = MR.pcg_iterations(levels, b, k=steps or (5, 2))
+    it_ref, _ = MR.pcg_iterations(levels, b, k=steps or (6, 2))
     ctx = ctx_from_oracle(mastg, P)
     if steps:
         ctx.set_option(mastg.OPT_MG_STEPS_FINE, steps[0])
