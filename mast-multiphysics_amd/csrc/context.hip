@@ -547,11 +547,14 @@ void ensure_scales(mastg_ctx* c) {
 // power iteration for the plates' Chebyshev bounds: steps and safety factor
 constexpr int kMGPowerIters = 20;
 constexpr double kMGPowerSafety = 1.1;
-// Chebyshev smoothing interval [lam / ratio, lam]: hex8 elasticity (Gershgorin
-// bound) 80, measured fewer PCG iterations than 30 on the scipy restatement
-// (20-21 vs 22 at 32x16x16 and 48x24x24, rho ~ U[0.01, 1)); plates and heat 30
+// Chebyshev smoothing interval [lam / ratio, lam], measured per physics:
+// hex8 elasticity (Gershgorin bound) 80 (PCG iterations on the scipy
+// restatement 20-21 vs 22 with 30 at 32x16x16 and 48x24x24; C2 22 vs 24);
+// the plate (power-iteration bound) 150 (512^2: 292 ms per design vs 297 with
+// 80 and 322 with 30); heat 30 (96^3: 133 ms vs 152 with 80)
 constexpr double kMGChebRatioHex8 = 80.0;
-constexpr double kMGChebRatio = 30.0;
+constexpr double kMGChebRatioPlate = 150.0;
+constexpr double kMGChebRatioHeat = 30.0;
 
 // The hierarchy (lattice detection, coarse meshes, their node-block structure)
 // is built on the first multigrid solve; coarse operators are re-formed after
@@ -844,7 +847,10 @@ int mg_setup(mastg_ctx* c) {
         }
     }
     launch_mg_cheb(c->mg_lam.as<double>(), c->mg_cheb.as<double>(), nl, std::max(c->mg_k0, c->mg_kc),
-                   c->d.physics == MASTG_HEX8_ELASTIC ? kMGChebRatioHex8 : kMGChebRatio, s);
+                   c->d.physics == MASTG_HEX8_ELASTIC    ? kMGChebRatioHex8
+                   : c->d.physics == MASTG_QUAD4_MINDLIN ? kMGChebRatioPlate
+                                                         : kMGChebRatioHeat,
+                   s);
     const MGLevelDev& Lc = *c->mg[nl - 1];
     const int64_t n = c->mg_n_coarse;
     launch_fill(n * n, 0.0, c->mg_dense.as<double>(), s);
