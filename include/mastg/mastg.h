@@ -245,9 +245,14 @@ int mastg_get_tuning(const mastg_ctx* ctx, int* spmv_lanes, int* grid);
 /* multigrid smoothing: Chebyshev steps per pre-/post-smoothing on the problem
  * level (MASTG_OPT_MG_STEPS_FINE) and on the coarse levels
  * (MASTG_OPT_MG_STEPS_COARSE); 1..8, defaults: hex8 elasticity 6 and 2,
- * plate and heat 4 and 2 */
+ * plate 4 and 2, heat 3 and 2 */
 #define MASTG_OPT_MG_STEPS_FINE 2
 #define MASTG_OPT_MG_STEPS_COARSE 3
+/* multigrid smoothing interval [lam / ratio, lam] on D^-1 A (lam: the
+ * Gershgorin bound for hex8 elasticity, 1.1 x a power-iteration estimate
+ * otherwise); 2..10000, 0 = the default: hex8 elasticity 80, plate 150,
+ * heat 7 */
+#define MASTG_OPT_MG_CHEB_RATIO 4
 int mastg_set_option(mastg_ctx* ctx, int option, int value);
 int mastg_set_tuning(mastg_ctx* ctx, int spmv_lanes, int grid);
 

@@ -113,6 +113,7 @@ struct mastg_ctx {
     std::string mg_why;
     bool mg_valid = false;     // coarse operators match the last assembly
     int mg_k0 = 3, mg_kc = 3;  // Chebyshev steps per smoothing: problem level, coarse levels
+    int mg_ratio = 0;          // Chebyshev interval [lam / ratio, lam] (0: the physics' default)
     int64_t mg_n_coarse = 0;   // DOFs of the dense coarsest level
     DevBuf mg_cheb, mg_lam, mg_part, mg_dense, mg_scal;
     bool mg_power = false;     // Chebyshev bounds from power iteration (plates)
@@ -549,12 +550,13 @@ constexpr int kMGPowerIters = 20;
 constexpr double kMGPowerSafety = 1.1;
 // Chebyshev smoothing interval [lam / ratio, lam], measured per physics:
 // hex8 elasticity (Gershgorin bound) 80 (PCG iterations on the scipy
-// restatement 20-21 vs 22 with 30 at 32x16x16 and 48x24x24; C2 22 vs 24);
-// the plate (power-iteration bound) 150 (512^2: 292 ms per design vs 297 with
-// 80 and 322 with 30); heat 30 (96^3: 133 ms vs 152 with 80)
+// restatement 20-21 vs 22 with 30 at 32x16x16 and 48x24x24; C2 22 vs 24;
+// 100 and 120 the same); the plate (power-iteration bound) 150 (512^2: 290 ms
+// per design, the same up to 600, 297 with 80, 322 with 30); heat 7 (96^3
+// with 3/2 steps: 110 ms, 112 with 5 and 10, 122 with 15, 133 with 30 at 4/2)
 constexpr double kMGChebRatioHex8 = 80.0;
 constexpr double kMGChebRatioPlate = 150.0;
-constexpr double kMGChebRatioHeat = 30.0;
+constexpr double kMGChebRatioHeat = 7.0;
 
 // The hierarchy (lattice detection, coarse meshes, their node-block structure)
 // is built on the first multigrid solve; coarse operators are re-formed after
@@ -846,11 +848,11 @@ int mg_setup(mastg_ctx* c) {
             else launch_mg_to_stencil(C.a, C.sten.as<float>(), s);
         }
     }
-    launch_mg_cheb(c->mg_lam.as<double>(), c->mg_cheb.as<double>(), nl, std::max(c->mg_k0, c->mg_kc),
-                   c->d.physics == MASTG_HEX8_ELASTIC    ? kMGChebRatioHex8
-                   : c->d.physics == MASTG_QUAD4_MINDLIN ? kMGChebRatioPlate
-                                                         : kMGChebRatioHeat,
-                   s);
+    const double ratio = c->mg_ratio > 0                        ? double(c->mg_ratio)
+                         : c->d.physics == MASTG_HEX8_ELASTIC    ? kMGChebRatioHex8
+                         : c->d.physics == MASTG_QUAD4_MINDLIN ? kMGChebRatioPlate
+                                                               : kMGChebRatioHeat;
+    launch_mg_cheb(c->mg_lam.as<double>(), c->mg_cheb.as<double>(), nl, std::max(c->mg_k0, c->mg_kc), ratio, s);
     const MGLevelDev& Lc = *c->mg[nl - 1];
     const int64_t n = c->mg_n_coarse;
     launch_fill(n * n, 0.0, c->mg_dense.as<double>(), s);
@@ -1377,10 +1379,11 @@ int mastg_ctx_create(const mastg_problem_desc* desc, mastg_ctx** out) {
         c->d.side_id = nullptr; c->d.point_loads = nullptr;
         if (c->d.penalty == 0) c->d.penalty = 3.0;
         // smoothing steps measured fastest per design (128x64x64 cantilever:
-        // 6/2 14.8 ms, 5/2 15.7, 7/2 15.8, 5/3 15.1; 512^2 plate: 4/2; 96^3
-        // heat: 4/2 130.8 ms, 3/2 131.7): the coarse levels are launch-latency
-        // bound, so extra coarse steps do not pay
-        c->mg_k0 = c->d.physics == MASTG_HEX8_ELASTIC ? 6 : 4;
+        // 6/2 14.8 ms, 5/2 15.7, 7/2 15.8, 5/3 15.1; 512^2 plate: 4/2 290 ms,
+        // 5/2 304; 96^3 heat at its interval: 3/2 110 ms, 4/2 122, 2/2 108
+        // with more iterations): the coarse levels are launch-latency bound,
+        // so extra coarse steps do not pay
+        c->mg_k0 = c->d.physics == MASTG_HEX8_ELASTIC ? 6 : c->d.physics == MASTG_HEX8_HEAT ? 3 : 4;
         c->mg_kc = 2;
         std::string m = build_structure(desc->physics, desc->n_nodes, desc->n_elems, desc->xyz, desc->conn,
                                         desc->n_vars, desc->node_dof_base, desc->n_constrained,
@@ -1868,8 +1871,15 @@ int mastg_set_option(mastg_ctx* c, int option, int value) {
         case MASTG_OPT_GRAPHS: c->use_graphs = value != 0; return MASTG_OK;
         case MASTG_OPT_MG_STEPS_FINE:
         case MASTG_OPT_MG_STEPS_COARSE:
-            if (value < 1 || value > kMGMaxCheb) return fail(c, MASTG_ERR_INVALID, "smoothing steps out of range");
-            (option == MASTG_OPT_MG_STEPS_FINE ? c->mg_k0 : c->mg_kc) = value;
+        case MASTG_OPT_MG_CHEB_RATIO:
+            if (option == MASTG_OPT_MG_CHEB_RATIO) {
+                if (value != 0 && (value < 2 || value > 10000))
+                    return fail(c, MASTG_ERR_INVALID, "Chebyshev interval ratio out of range");
+                c->mg_ratio = value;
+            } else {
+                if (value < 1 || value > kMGMaxCheb) return fail(c, MASTG_ERR_INVALID, "smoothing steps out of range");
+                (option == MASTG_OPT_MG_STEPS_FINE ? c->mg_k0 : c->mg_kc) = value;
+            }
             // the captured V-cycles and the Chebyshev coefficients change
             if (c->mg_exec) (void)hipGraphExecDestroy(c->mg_exec);
             if (c->mg_bicg_exec) (void)hipGraphExecDestroy(c->mg_bicg_exec);

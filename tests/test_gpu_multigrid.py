@@ -60,6 +60,10 @@ def test_mg_steps_option_range(oracle, mastg):
     for bad in (0, 9):
         with pytest.raises(mastg.MastgError):
             ctx.set_option(mastg.OPT_MG_STEPS_FINE, bad)
+    for bad in (1, -3, 10001):
+        with pytest.raises(mastg.MastgError):
+            ctx.set_option(mastg.OPT_MG_CHEB_RATIO, bad)
+    ctx.set_option(mastg.OPT_MG_CHEB_RATIO, 0)
     ctx.close()
 
 
