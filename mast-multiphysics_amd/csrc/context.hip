@@ -65,6 +65,7 @@ struct MGLevelDev {
     DevBuf sten;                  // hex8 level 1: the operator as a symmetric 27-point FP32 stencil;
                                   // heat levels 0 and 1: 27-point stencils in lattice order
     DevBuf kpos;                  // heat level 0: row position of each stencil entry (255: none)
+    DevBuf dinvb;                 // plate level 0: inverses of the 5 x 5 diagonal node blocks
     bool lat0 = false;            // heat level 0 smooths in lattice order
     DevBuf Kel, val, val32, dinv, r, x, res, da, db;
 };
@@ -637,6 +638,10 @@ int mg_build(mastg_ctx* c) {
                 TRY(dalloc(c, L->sten, size_t(h.n_nodes) * 27 * sizeof(float)));
                 L->lat0 = true;
             }
+            if (nav == 5) {   // the plate's problem level smooths with block Jacobi
+                TRY(dalloc(c, L->dinvb, size_t(h.n_nodes) * 25 * sizeof(double)));
+                a.dinvb = L->dinvb.as<double>();
+            }
             a.rank_of_lat = L->rank_of_lat.as<int32_t>();
             a.con = c->con.as<uint8_t>();
             a.blk_ptr = c->blk_ptr.as<int64_t>();
@@ -801,6 +806,7 @@ int mg_setup(mastg_ctx* c) {
             launch_mg_power_heat(Ll, L0.sten.as<float>(), L0.a.rank_of_lat, kMGPowerIters, kMGPowerSafety, L0.a.da,
                                  L0.a.db, c->mg_part.as<double>(), c->mg_scal.as<double>(), c->mg_lam.as<double>(), s);
         } else {
+            if (L0.a.dinvb) launch_mg_block_dinv5(L0.a, L0.dinvb.as<double>(), s);
             launch_mg_power(L0.a, c->val.as<double>(), kMGPowerIters, kMGPowerSafety, L0.a.da, L0.a.db,
                             c->mg_part.as<double>(), c->mg_scal.as<double>(), c->mg_lam.as<double>(), s);
         }

@@ -214,6 +214,65 @@ __global__ __launch_bounds__(256) void k_mg_pre_init(MGLevelArgs L, const int* _
     }
 }
 
+// the same start with block Jacobi (5 variables per node): da = ith Dinv_b r
+__global__ __launch_bounds__(256) void k_mg_pre_init_blk5(MGLevelArgs L, const int* __restrict__ done) {
+    if (done && *done) return;
+    const double ith = L.cheb[0];
+    for (int64_t n = int64_t(blockIdx.x) * 256 + threadIdx.x; n < L.n_nodes; n += int64_t(gridDim.x) * 256) {
+        double r[5];
+#pragma unroll
+        for (int w = 0; w < 5; w++) {
+            r[w] = L.r[n * 5 + w];
+            L.x[n * 5 + w] = 0.0;
+            L.res[n * 5 + w] = r[w];
+        }
+        const double* B = L.dinvb + n * 25;
+#pragma unroll
+        for (int v = 0; v < 5; v++) {
+            double t = 0.0;
+#pragma unroll
+            for (int w = 0; w < 5; w++) t = fma(B[v * 5 + w], r[w], t);
+            L.da[n * 5 + v] = ith * t;
+        }
+    }
+}
+
+// inverse of each node's 5 x 5 diagonal block (Gauss-Jordan without pivoting:
+// the blocks of an SPD matrix are SPD; constrained variables keep their unit
+// diagonal convention), from the FP64 node-block values (layout (v, k, w))
+__global__ __launch_bounds__(256) void k_mg_block_dinv5(MGLevelArgs L, double* __restrict__ dinvb) {
+    const int64_t n = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (n >= L.n_nodes) return;
+    const int64_t k0 = L.blk_ptr[n];
+    const int deg = int(L.blk_ptr[n + 1] - k0);
+    int kd = 0;
+    for (int k = 0; k < deg; k++)
+        if (L.blk_col[k0 + k] == int32_t(n)) kd = k;
+    const double* V = L.val + k0 * 25;
+    double a[5][5];
+#pragma unroll
+    for (int v = 0; v < 5; v++)
+#pragma unroll
+        for (int w = 0; w < 5; w++) a[v][w] = V[(v * deg + kd) * 5 + w];
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+        const double p = 1.0 / a[k][k];
+#pragma unroll
+        for (int w = 0; w < 5; w++) a[k][w] = w == k ? p : a[k][w] * p;
+#pragma unroll
+        for (int v = 0; v < 5; v++) {
+            if (v == k) continue;
+            const double f = a[v][k];
+#pragma unroll
+            for (int w = 0; w < 5; w++) a[v][w] = w == k ? -f * p : fma(-f, a[k][w], a[v][w]);
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < 5; v++)
+#pragma unroll
+        for (int w = 0; w < 5; w++) dinvb[n * 25 + v * 5 + w] = a[v][w];
+}
+
 // the same start on the FP32 state of the hex8 problem level
 __global__ __launch_bounds__(256) void k_mg_pre_init32(MGLevelArgs L, const int* __restrict__ done) {
     if (done && *done) return;
@@ -261,7 +320,34 @@ __global__ __launch_bounds__(256) void k_mg_bsr(MGLevelArgs L, const T* __restri
         for (int v = 0; v < NAV; v++)
 #pragma unroll
             for (int m = 4; m >= 1; m >>= 1) sv[v] += __shfl_xor(sv[v], m, 8);
-        if (sub < NAV) {
+        if (NAV == 5 && L.dinvb) {
+            // block Jacobi: every lane of the node's group holds the 5 rows y;
+            // each forms the 5 new residuals and lane v < 5 its direction entry
+            // Dinv_b[v] . res
+            const int64_t i0 = r * NAV;
+            double rs[NAV];
+#pragma unroll
+            for (int w = 0; w < NAV; w++) rs[w] = (mode == MG_RESID_INIT ? L.r[i0 + w] : L.res[i0 + w]) - sv[w];
+            if (sub < NAV) {
+                const int64_t i = i0 + sub;
+                double rsv = rs[0];
+#pragma unroll
+                for (int w = 1; w < NAV; w++) rsv = sub == w ? rs[w] : rsv;
+                const double* B = L.dinvb + r * 25 + sub * 5;
+                double z = 0.0;
+#pragma unroll
+                for (int w = 0; w < NAV; w++) z = fma(B[w], rs[w], z);
+                if (mode == MG_RESID_INIT) {
+                    L.res[i] = rsv;
+                    dout[i] = z * ith;
+                } else {
+                    const double dv = din[i];
+                    L.x[i] += dv;
+                    L.res[i] = rsv;
+                    if (mode == MG_STEP) dout[i] = c1 * dv + c2 * z;
+                }
+            }
+        } else if (sub < NAV) {
             double y = sv[0];
 #pragma unroll
             for (int v = 1; v < NAV; v++) y = sub == v ? sv[v] : y;
@@ -840,7 +926,12 @@ __global__ __launch_bounds__(256) void k_mg_power(MGLevelArgs L, const double* _
         if (sub == 0)
 #pragma unroll
             for (int v = 0; v < NAV; v++) {
-                const double o = L.dinv[r * NAV + v] * sv[v];
+                double o = L.dinv[r * NAV + v] * sv[v];
+                if (NAV == 5 && L.dinvb) {   // block Jacobi: Dinv_b A
+                    o = 0.0;
+#pragma unroll
+                    for (int w = 0; w < NAV; w++) o = fma(L.dinvb[r * 25 + v * 5 + w], sv[w], o);
+                }
                 out[r * NAV + v] = o;
                 acc += o * o;
             }
@@ -1326,7 +1417,11 @@ void launch_perm_nodes(int64_t n_nodes, const int32_t* map, const double* src, d
 }
 void launch_mg_pre_init(const MGLevelArgs& L, const int* done, hipStream_t s) {
     if (L.planar) hipLaunchKernelGGL(k_mg_pre_init32, dim3(mg_grid(L.n_nodes * L.nav)), dim3(256), 0, s, L, done);
+    else if (L.dinvb) hipLaunchKernelGGL(k_mg_pre_init_blk5, dim3(mg_grid(L.n_nodes)), dim3(256), 0, s, L, done);
     else hipLaunchKernelGGL(k_mg_pre_init, dim3(mg_grid(L.n_nodes * L.nav)), dim3(256), 0, s, L, done);
+}
+void launch_mg_block_dinv5(const MGLevelArgs& L, double* dinvb, hipStream_t s) {
+    hipLaunchKernelGGL(k_mg_block_dinv5, dim3(mg_blocks(L.n_nodes, 256)), dim3(256), 0, s, L, dinvb);
 }
 void launch_mg_to_stencil(const MGLevelArgs& L, float* sten, hipStream_t s) {
     hipLaunchKernelGGL(k_mg_to_stencil, dim3(mg_blocks(L.n_nodes, 256)), dim3(256), 0, s, L, sten);
