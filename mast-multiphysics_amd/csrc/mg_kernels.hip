@@ -214,26 +214,22 @@ __global__ __launch_bounds__(256) void k_mg_pre_init(MGLevelArgs L, const int* _
     }
 }
 
-// the same start with block Jacobi (5 variables per node): da = ith Dinv_b r
+// the same start with block Jacobi (5 variables per node): da = ith Dinv_b r,
+// a thread per DOF (row v of node n's inverse block), so that a wave's loads
+// of the blocks and of r cover contiguous memory
 __global__ __launch_bounds__(256) void k_mg_pre_init_blk5(MGLevelArgs L, const int* __restrict__ done) {
     if (done && *done) return;
     const double ith = L.cheb[0];
-    for (int64_t n = int64_t(blockIdx.x) * 256 + threadIdx.x; n < L.n_nodes; n += int64_t(gridDim.x) * 256) {
-        double r[5];
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < L.n_nodes * 5; i += int64_t(gridDim.x) * 256) {
+        const int64_t n = i / 5;
+        const double* rn = L.r + n * 5;
+        const double* B = L.dinvb + i * 5;   // row v = i - 5 n of node n's block
+        double t = 0.0;
 #pragma unroll
-        for (int w = 0; w < 5; w++) {
-            r[w] = L.r[n * 5 + w];
-            L.x[n * 5 + w] = 0.0;
-            L.res[n * 5 + w] = r[w];
-        }
-        const double* B = L.dinvb + n * 25;
-#pragma unroll
-        for (int v = 0; v < 5; v++) {
-            double t = 0.0;
-#pragma unroll
-            for (int w = 0; w < 5; w++) t = fma(B[v * 5 + w], r[w], t);
-            L.da[n * 5 + v] = ith * t;
-        }
+        for (int w = 0; w < 5; w++) t = fma(B[w], rn[w], t);
+        L.x[i] = 0.0;
+        L.res[i] = rn[i - n * 5];
+        L.da[i] = ith * t;
     }
 }
 
@@ -1417,7 +1413,7 @@ void launch_perm_nodes(int64_t n_nodes, const int32_t* map, const double* src, d
 }
 void launch_mg_pre_init(const MGLevelArgs& L, const int* done, hipStream_t s) {
     if (L.planar) hipLaunchKernelGGL(k_mg_pre_init32, dim3(mg_grid(L.n_nodes * L.nav)), dim3(256), 0, s, L, done);
-    else if (L.dinvb) hipLaunchKernelGGL(k_mg_pre_init_blk5, dim3(mg_grid(L.n_nodes)), dim3(256), 0, s, L, done);
+    else if (L.dinvb) hipLaunchKernelGGL(k_mg_pre_init_blk5, dim3(mg_grid(L.n_nodes * 5)), dim3(256), 0, s, L, done);
     else hipLaunchKernelGGL(k_mg_pre_init, dim3(mg_grid(L.n_nodes * L.nav)), dim3(256), 0, s, L, done);
 }
 void launch_mg_block_dinv5(const MGLevelArgs& L, double* dinvb, hipStream_t s) {
