@@ -639,8 +639,8 @@ int mg_build(mastg_ctx* c) {
                 L->lat0 = true;
             }
             if (nav == 5) {   // the plate's problem level smooths with block Jacobi
-                TRY(dalloc(c, L->dinvb, size_t(h.n_nodes) * 25 * sizeof(double)));
-                a.dinvb = L->dinvb.as<double>();
+                TRY(dalloc(c, L->dinvb, size_t(h.n_nodes) * 25 * sizeof(float)));
+                a.dinvb = L->dinvb.as<float>();
             }
             a.rank_of_lat = L->rank_of_lat.as<int32_t>();
             a.con = c->con.as<uint8_t>();
@@ -806,7 +806,7 @@ int mg_setup(mastg_ctx* c) {
             launch_mg_power_heat(Ll, L0.sten.as<float>(), L0.a.rank_of_lat, kMGPowerIters, kMGPowerSafety, L0.a.da,
                                  L0.a.db, c->mg_part.as<double>(), c->mg_scal.as<double>(), c->mg_lam.as<double>(), s);
         } else {
-            if (L0.a.dinvb) launch_mg_block_dinv5(L0.a, L0.dinvb.as<double>(), s);
+            if (L0.a.dinvb) launch_mg_block_dinv5(L0.a, L0.dinvb.as<float>(), s);
             launch_mg_power(L0.a, c->val.as<double>(), kMGPowerIters, kMGPowerSafety, L0.a.da, L0.a.db,
                             c->mg_part.as<double>(), c->mg_scal.as<double>(), c->mg_lam.as<double>(), s);
         }

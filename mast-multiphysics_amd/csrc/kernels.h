@@ -205,7 +205,7 @@ struct MGLevelArgs {
     const float* dinv32;
     // the plate problem level smooths with block Jacobi: the inverses of its
     // 5 x 5 diagonal node blocks ([n_nodes][25], null: point Jacobi)
-    const double* dinvb;
+    const float* dinvb;
 };
 
 struct MGGalerkinArgs {                // coarse element matrices = sum_c Q_c^T K_c Q_c
@@ -272,7 +272,7 @@ void launch_mg_combine_rank(int64_t n_nodes, int nav, const int32_t* rank_of_lat
                             const double* dlast, double* out, const int* done, hipStream_t s);
 void launch_mg_to_stencil(const MGLevelArgs& L, float* sten, hipStream_t s);
 // inverses of the 5 x 5 diagonal node blocks of a plate level (FP64 values)
-void launch_mg_block_dinv5(const MGLevelArgs& L, double* dinvb, hipStream_t s);
+void launch_mg_block_dinv5(const MGLevelArgs& L, float* dinvb, hipStream_t s);
 void launch_mg_stencil(const MGLevelArgs& L, const float* sten, int mode, int step, const double* din, double* dout,
                        const int* done, hipStream_t s);
 // level 0 (hex8): FP32 smoothing step with the lattice operator (c.lat) and

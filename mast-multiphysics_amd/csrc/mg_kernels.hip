@@ -223,10 +223,10 @@ __global__ __launch_bounds__(256) void k_mg_pre_init_blk5(MGLevelArgs L, const i
     for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < L.n_nodes * 5; i += int64_t(gridDim.x) * 256) {
         const int64_t n = i / 5;
         const double* rn = L.r + n * 5;
-        const double* B = L.dinvb + i * 5;   // row v = i - 5 n of node n's block
+        const float* B = L.dinvb + i * 5;   // row v = i - 5 n of node n's block
         double t = 0.0;
 #pragma unroll
-        for (int w = 0; w < 5; w++) t = fma(B[w], rn[w], t);
+        for (int w = 0; w < 5; w++) t = fma(double(B[w]), rn[w], t);
         L.x[i] = 0.0;
         L.res[i] = rn[i - n * 5];
         L.da[i] = ith * t;
@@ -236,7 +236,7 @@ __global__ __launch_bounds__(256) void k_mg_pre_init_blk5(MGLevelArgs L, const i
 // inverse of each node's 5 x 5 diagonal block (Gauss-Jordan without pivoting:
 // the blocks of an SPD matrix are SPD; constrained variables keep their unit
 // diagonal convention), from the FP64 node-block values (layout (v, k, w))
-__global__ __launch_bounds__(256) void k_mg_block_dinv5(MGLevelArgs L, double* __restrict__ dinvb) {
+__global__ __launch_bounds__(256) void k_mg_block_dinv5(MGLevelArgs L, float* __restrict__ dinvb) {
     const int64_t n = int64_t(blockIdx.x) * 256 + threadIdx.x;
     if (n >= L.n_nodes) return;
     const int64_t k0 = L.blk_ptr[n];
@@ -266,7 +266,7 @@ __global__ __launch_bounds__(256) void k_mg_block_dinv5(MGLevelArgs L, double* _
 #pragma unroll
     for (int v = 0; v < 5; v++)
 #pragma unroll
-        for (int w = 0; w < 5; w++) dinvb[n * 25 + v * 5 + w] = a[v][w];
+        for (int w = 0; w < 5; w++) dinvb[n * 25 + v * 5 + w] = float(a[v][w]);
 }
 
 // the same start on the FP32 state of the hex8 problem level
@@ -329,10 +329,10 @@ __global__ __launch_bounds__(256) void k_mg_bsr(MGLevelArgs L, const T* __restri
                 double rsv = rs[0];
 #pragma unroll
                 for (int w = 1; w < NAV; w++) rsv = sub == w ? rs[w] : rsv;
-                const double* B = L.dinvb + r * 25 + sub * 5;
+                const float* B = L.dinvb + r * 25 + sub * 5;
                 double z = 0.0;
 #pragma unroll
-                for (int w = 0; w < NAV; w++) z = fma(B[w], rs[w], z);
+                for (int w = 0; w < NAV; w++) z = fma(double(B[w]), rs[w], z);
                 if (mode == MG_RESID_INIT) {
                     L.res[i] = rsv;
                     dout[i] = z * ith;
@@ -926,7 +926,7 @@ __global__ __launch_bounds__(256) void k_mg_power(MGLevelArgs L, const double* _
                 if (NAV == 5 && L.dinvb) {   // block Jacobi: Dinv_b A
                     o = 0.0;
 #pragma unroll
-                    for (int w = 0; w < NAV; w++) o = fma(L.dinvb[r * 25 + v * 5 + w], sv[w], o);
+                    for (int w = 0; w < NAV; w++) o = fma(double(L.dinvb[r * 25 + v * 5 + w]), sv[w], o);
                 }
                 out[r * NAV + v] = o;
                 acc += o * o;
@@ -1416,7 +1416,7 @@ void launch_mg_pre_init(const MGLevelArgs& L, const int* done, hipStream_t s) {
     else if (L.dinvb) hipLaunchKernelGGL(k_mg_pre_init_blk5, dim3(mg_grid(L.n_nodes * 5)), dim3(256), 0, s, L, done);
     else hipLaunchKernelGGL(k_mg_pre_init, dim3(mg_grid(L.n_nodes * L.nav)), dim3(256), 0, s, L, done);
 }
-void launch_mg_block_dinv5(const MGLevelArgs& L, double* dinvb, hipStream_t s) {
+void launch_mg_block_dinv5(const MGLevelArgs& L, float* dinvb, hipStream_t s) {
     hipLaunchKernelGGL(k_mg_block_dinv5, dim3(mg_blocks(L.n_nodes, 256)), dim3(256), 0, s, L, dinvb);
 }
 void launch_mg_to_stencil(const MGLevelArgs& L, float* sten, hipStream_t s) {
