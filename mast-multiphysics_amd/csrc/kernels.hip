@@ -687,7 +687,10 @@ __global__ __launch_bounds__(256) void k_hex8_heat_K(HeatArgs A) {
                 int r0, r1;
                 hex8_side_xi(sd, gp_coord(qq & 1), gp_coord(qq >> 1), xi, eta, zeta, r0, r1);
                 double Na = 0, Nb = 0, T = 0, t0[3] = {0, 0, 0}, t1[3] = {0, 0, 0};
-#pragma unroll
+                // not unrolled: the unrolled face loop set the register
+                // budget of the whole kernel (160 VGPRs, 3 waves per SIMD,
+                // against 70 and 7), for the few elements on convecting faces
+#pragma unroll 1
                 for (int n = 0; n < 8; n++) {
                     double d[3], N;
                     hex8_dshape(n, xi, eta, zeta, d[0], d[1], d[2], N);
