@@ -999,8 +999,9 @@ int mg_solve(mastg_ctx* c, const double* b, const mastg_solve_opts* o, mastg_sol
         launch_perm_nodes(S.n_nodes, lat_map, b, blat, 0, c->stream);
         bl = blat;
     } else {
-        // plate: node-rank order, the FP64 CSR product
-        a.use_mf = 0;
+        // plate: node-rank order, the FP64 product over the CSR values in
+        // node blocks (one column index per 5 x 5 block instead of per value)
+        a.use_mf = 4;
     }
     c->cg_replacements = 0;
     launch_fill(S.n_active, 0.0, a.x, c->stream);

@@ -105,7 +105,8 @@ struct CGArgs {
     int nparts_op;            // partial sums written by the operator kernel
     int nvec;                 // grid of the CG vector kernels (and their partials)
     CGScalars* S;
-    int use_mf;               // operator: 0 CSR SpMV, 1 matrix-free, 2 FP32 node-block SpMV
+    int use_mf;               // operator: 0 CSR SpMV, 1 matrix-free, 2 FP32 node-block SpMV,
+                              // 3 hex8 lattice operator, 4 FP64 node-block SpMV
     int spmv_sw;              // CSR SpMV lanes per row (4, 8, 16, 32, 64)
     MFArgs mf;
     // FP32 copy of the CSR values read by node blocks (operator 2): node rank

@@ -1084,13 +1084,14 @@ __global__ __launch_bounds__(256) void k_cg_replace_scalars(CGArgs C) {
     }
 }
 
-// y = A32 x over node blocks: SW lanes per node row, NAV rows per node; values
-// FP32 in the scalar CSR order (row v of node r: deg*NAV entries, block-major),
+// y = A x over node blocks: SW lanes per node row, NAV rows per node; values
+// (the FP64 CSR values or their FP32 copy) in the scalar CSR order (row v of
+// node r: deg*NAV entries, block-major), read with one column index per block;
 // products and sums in FP64; optional partial sums of y.x
-template <int NAV, int SW>
-__global__ __launch_bounds__(256) void k_spmv_nb32_dot(CGArgs C, const double* __restrict__ x,
-                                                       double* __restrict__ y, int want_dot,
-                                                       int in_cg_loop) {
+template <typename T, int NAV, int SW>
+__global__ __launch_bounds__(256) void k_spmv_nb_dot(CGArgs C, const T* __restrict__ vals,
+                                                     const double* __restrict__ x, double* __restrict__ y,
+                                                     int want_dot, int in_cg_loop) {
     if (in_cg_loop && C.S->done) return;
     const int sub = threadIdx.x & (SW - 1);
     double dsum = 0.0;
@@ -1099,7 +1100,7 @@ __global__ __launch_bounds__(256) void k_spmv_nb32_dot(CGArgs C, const double* _
     for (int64_t r = (int64_t(blockIdx.x) * 256 + threadIdx.x) / SW; r < n_nodes; r += nodes_per_grid) {
         const int64_t k0 = C.blk_ptr[r];
         const int deg = int(C.blk_ptr[r + 1] - k0);
-        const float* __restrict__ V = C.val32 + k0 * (NAV * NAV);
+        const T* __restrict__ V = vals + k0 * (NAV * NAV);
         double s[NAV];
 #pragma unroll
         for (int v = 0; v < NAV; v++) s[v] = 0.0;
@@ -1354,9 +1355,21 @@ void launch_spmv_csr(const CGArgs& c, const double* x, double* y, int want_dot, 
 static void nb32_dispatch(const CGArgs& c, const double* x, double* y, int want_dot, int in_loop,
                           hipStream_t s) {
     if (c.nav == 5)
-        hipLaunchKernelGGL((k_spmv_nb32_dot<5, 8>), dim3(c.nparts), dim3(256), 0, s, c, x, y, want_dot, in_loop);
+        hipLaunchKernelGGL((k_spmv_nb_dot<float, 5, 8>), dim3(c.nparts), dim3(256), 0, s, c, c.val32, x, y, want_dot,
+                           in_loop);
     else
-        hipLaunchKernelGGL((k_spmv_nb32_dot<3, 8>), dim3(c.nparts), dim3(256), 0, s, c, x, y, want_dot, in_loop);
+        hipLaunchKernelGGL((k_spmv_nb_dot<float, 3, 8>), dim3(c.nparts), dim3(256), 0, s, c, c.val32, x, y, want_dot,
+                           in_loop);
+}
+// the FP64 operator over node blocks (plates: no column index per value)
+static void nb64_dispatch(const CGArgs& c, const double* x, double* y, int want_dot, int in_loop,
+                          hipStream_t s) {
+    if (c.nav == 5)
+        hipLaunchKernelGGL((k_spmv_nb_dot<double, 5, 8>), dim3(c.nparts), dim3(256), 0, s, c, c.val, x, y, want_dot,
+                           in_loop);
+    else
+        hipLaunchKernelGGL((k_spmv_nb_dot<double, 3, 8>), dim3(c.nparts), dim3(256), 0, s, c, c.val, x, y, want_dot,
+                           in_loop);
 }
 void launch_spmv_nb32(const CGArgs& c, const double* x, double* y, int want_dot, hipStream_t s) {
     nb32_dispatch(c, x, y, want_dot, 0, s);
@@ -1450,6 +1463,8 @@ void launch_cg_op_alpha(const CGArgs& c, hipStream_t s) {
         mf_dispatch(c, c.p, c.p, c.q, 1, s);
     else if (c.use_mf == 2)
         nb32_dispatch(c, c.p, c.q, 1, 1, s);
+    else if (c.use_mf == 4)
+        nb64_dispatch(c, c.p, c.q, 1, 1, s);
     else
         spmv_dispatch(c, c.p, c.q, 1, 1, s);
     hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(256), 0, s, c);
