@@ -233,8 +233,9 @@ int mastg_enable_kernel_timing(mastg_ctx* ctx, int on);
  * kinds as mastg_kernel_time, plus 5 = the FP32 matrix-core element kernel,
  * hex8 only, into a buffer of its own, 6 = the per-node lattice operator, 7 =
  * a plain 256 MiB device copy, the known-traffic kernel that hardware byte
- * counters are scaled against, and 8 = one FP32 multigrid smoothing step on
- * the lattice, hex8 only).  For roofline measurement. */
+ * counters are scaled against, 8 = one FP32 multigrid smoothing step on
+ * the lattice, hex8 only, and 9 = the FP64 operator over node blocks, the
+ * plates' multigrid CG product).  For roofline measurement. */
 int mastg_bench_kernel(mastg_ctx* ctx, int kind, int n_launch, double* avg_ms);
 /* Launch shapes: CSR SpMV lanes per row and grid size (work-groups of the
  * SpMV / CG vector kernels).  Autotuned on the first solve unless set. */

@@ -1944,6 +1944,7 @@ int mastg_bench_kernel(mastg_ctx* c, int kind, int n_launch, double* avg_ms) {
             case 5: launch_hex8_K_f32(hex8_args(c, nullptr, false), c->Kel32.as<float>(), c->stream); break;
             case 6: launch_lattice_apply(a, c->mg[0]->b_lat.as<double>(), c->mg[0]->y_lat.as<double>(), c->stream); break;
             case 7: launch_copy_f64(kCopyDoubles, copy_src.as<double>(), copy_dst.as<double>(), c->stream); break;
+            case 9: launch_spmv_nb64(a, a.p, a.q, 1, c->stream); break;
             case 8: {
                 const MGLevelArgs& L0 = c->mg[0]->a;
                 launch_mg_fine(a, L0, MG_STEP, 1, L0.da32, L0.db32, nullptr, c->stream);
@@ -1952,7 +1953,9 @@ int mastg_bench_kernel(mastg_ctx* c, int kind, int n_launch, double* avg_ms) {
             default: break;
         }
     };
-    if (kind < 0 || kind > 8) return fail(c, MASTG_ERR_INVALID, "unknown kernel kind");
+    if (kind < 0 || kind > 9) return fail(c, MASTG_ERR_INVALID, "unknown kernel kind");
+    if (kind == 9 && (c->S.nav != 5 || !c->assembled))
+        return fail(c, MASTG_ERR_INVALID, "the node-block FP64 product is the plates', after an assembly");
     if (kind == 8) {
         // one Chebyshev step of the hex8 problem level's FP32 smoothing on a
         // zero state (x, res, directions) with the current assembly's bounds

@@ -145,6 +145,7 @@ void launch_to_f32(int64_t n, const double* src, float* dst, hipStream_t s);
 // dst[0:n) = src[0:n) (n even, 16-byte aligned buffers)
 void launch_copy_f64(int64_t n, const double* src, double* dst, hipStream_t s);
 void launch_spmv_nb32(const CGArgs& c, const double* x, double* y, int want_dot, hipStream_t s);
+void launch_spmv_nb64(const CGArgs& c, const double* x, double* y, int want_dot, hipStream_t s);
 // residual replacement: r = rt (a true FP64 residual), z = dinv r, fresh r.z /
 // r.r, p = z when restart; done = 1 if ||r|| meets the tolerance, else 0 (2
 // past max_it)

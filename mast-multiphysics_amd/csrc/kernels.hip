@@ -1374,6 +1374,9 @@ static void nb64_dispatch(const CGArgs& c, const double* x, double* y, int want_
 void launch_spmv_nb32(const CGArgs& c, const double* x, double* y, int want_dot, hipStream_t s) {
     nb32_dispatch(c, x, y, want_dot, 0, s);
 }
+void launch_spmv_nb64(const CGArgs& c, const double* x, double* y, int want_dot, hipStream_t s) {
+    nb64_dispatch(c, x, y, want_dot, 0, s);
+}
 void launch_to_f32(int64_t n, const double* src, float* dst, hipStream_t s) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_to_f32, dim3(std::min<int64_t>(blocks_for(n, 256), 8192)), dim3(256), 0, s, n, src, dst);
