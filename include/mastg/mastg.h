@@ -179,7 +179,9 @@ int mastg_solve(mastg_ctx* ctx, const double* rhs, double* x, int where,
  * op 1 = matrix-free operator (same matrix), op 2 = the FP32-rounded values
  * read by node blocks (the mixed-precision solver's operator), op 3 = the
  * per-node lattice matrix-free operator of the multigrid solver applied to dX
- * with its constrained entries zeroed (hex8 lattice meshes).  Replaces
+ * with its constrained entries zeroed (hex8 lattice meshes), op 4 = the FP64
+ * values read by node blocks, one column index per block (the plates'
+ * multigrid CG operator; structural Jacobians).  Replaces
  * NonlinearImplicitAssembly::linearized_jacobian_solution_product
  * (src/base/nonlinear_implicit_assembly.cpp:271-383) for the linear elements. */
 int mastg_jacobian_product(mastg_ctx* ctx, const double* dX, double* JdX, int where, int op);

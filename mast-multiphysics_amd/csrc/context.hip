@@ -1700,10 +1700,10 @@ int mastg_jacobian_product(mastg_ctx* c, const double* dX, double* JdX, int wher
     if (!c->assembled) return fail(c, MASTG_ERR_INVALID, "no assembled matrix");
     if (op == 1 && (c->S.n_classes == 0 || c->d.physics != MASTG_HEX8_ELASTIC))
         return fail(c, MASTG_ERR_INVALID, "no matrix-free operator");
-    if (op < 0 || op > 3) return fail(c, MASTG_ERR_INVALID, "unknown operator");
+    if (op < 0 || op > 4) return fail(c, MASTG_ERR_INVALID, "unknown operator");
     if (op == 3) TRY(mg_build(c));
-    if (op == 2 && c->d.physics == MASTG_HEX8_HEAT)
-        return fail(c, MASTG_ERR_INVALID, "FP32 node-block operator is for the structural Jacobians");
+    if ((op == 2 || op == 4) && c->d.physics == MASTG_HEX8_HEAT)
+        return fail(c, MASTG_ERR_INVALID, "the node-block operators are for the structural Jacobians");
     TRY(to_active(c, dX, where, c->vb.as<double>()));
     if (op == 2 && !c->val32_valid) {
         TRY(dalloc(c, c->val32, size_t(c->S.nnz) * sizeof(float)));
@@ -1725,6 +1725,8 @@ int mastg_jacobian_product(mastg_ctx* c, const double* dX, double* JdX, int wher
         launch_perm_nodes(c->S.n_nodes, lat_map, yl, c->vtmp.as<double>(), 1, c->stream);
     } else if (op == 2) {
         launch_spmv_nb32(a, c->vb.as<double>(), c->vtmp.as<double>(), 0, c->stream);
+    } else if (op == 4) {
+        launch_spmv_nb64(a, c->vb.as<double>(), c->vtmp.as<double>(), 0, c->stream);
     } else if (op == 1) {
         ensure_scales(c);
         // element phase reads a copy zeroed on constrained DOFs; the node phase

@@ -86,6 +86,32 @@ def test_plate_mixed_refused_fp32_operator_accurate(oracle, mastg):
     ctx.close()
 
 
+def test_plate_node_block_fp64_operator(oracle, mastg):
+    """The FP64 node-block product (the plate multigrid CG's operator: the
+    CSR values read with one column index per 5x5 block) equals the oracle's
+    CSR product and the GPU CSR product up to summation order, including the
+    constrained rows (unit diagonal)."""
+    print("[plate FP64 node-block operator] 20x16", flush=True)
+    P = oracle.Problem(oracle.PLATE, 20, 16, seed=3)
+    ctx = ctx_from_oracle(mastg, P)
+    ctx.assemble()
+    P.assemble()
+    act = ctx.active_map() >= 0
+    x = np.random.default_rng(4).standard_normal(P.n_dofs)
+    y_csr = ctx.jacobian_product(x, op=0)
+    y_nb = ctx.jacobian_product(x, op=4)
+    # the oracle's own product (it keeps the drilling DOFs) on the active entries
+    y_ref = P.spmv(np.where(act, x, 0.0))
+    scale = np.max(np.abs(y_csr))
+    print(f"  vs GPU CSR {np.max(np.abs(y_nb - y_csr)) / scale:.2e}, vs oracle "
+          f"{np.max(np.abs(y_nb - y_ref)[act]) / scale:.2e}", flush=True)
+    assert np.max(np.abs(y_nb - y_csr)) <= 1e-13 * scale
+    assert np.max(np.abs(y_nb - y_ref)[act]) <= 1e-12 * scale
+    with pytest.raises(mastg.MastgError):
+        ctx.jacobian_product(x, op=5)
+    ctx.close()
+
+
 def test_plate_adjoint_vs_fd_on_gpu(mastg):
     print("[plate FD on GPU] 12x12", flush=True)
     # the FP64 floor of ||b - K u|| / ||b|| on this thin plate (cond ~2e9) is
