@@ -51,8 +51,9 @@ __global__ __launch_bounds__(256) void k_mg_galerkin(MGGalerkinArgs G) {
     const int64_t w = int64_t(blockIdx.x) * 4 + wave;
     const bool valid = G.elems ? w < G.n_elems_list : w < G.n_coarse;
     const int64_t E = !valid ? 0 : (G.elems ? int64_t(G.elems[w]) : w);
-    const int64_t Ev = E;
-    const int64_t I = Ev % G.cnx, J = (Ev / G.cnx) % G.cny, K = Ev / (int64_t(G.cnx) * G.cny);
+    // 32-bit index arithmetic (coarse element counts are below 2^31)
+    const uint32_t Eu = uint32_t(E), cnx = uint32_t(G.cnx), Eq = Eu / cnx;
+    const int64_t I = int64_t(Eu - Eq * cnx), J = int64_t(Eq % uint32_t(G.cny)), K = int64_t(Eq / uint32_t(G.cny));
     double acc[NB];
 #pragma unroll
     for (int i = 0; i < NB; i++) acc[i] = 0.0;
