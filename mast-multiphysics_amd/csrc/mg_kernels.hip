@@ -1116,8 +1116,9 @@ __global__ __launch_bounds__(256) void k_mg_prolong(MGLevelArgs F, MGLevelArgs C
         const uint32_t CI = uint32_t(min((fi >> 1) + a, Cl.nx)), CJ = uint32_t(min((fj >> 1) + b, Cl.ny)),
                        CK = uint32_t(min((fk >> 1) + c, Cl.nz));
         const int64_t rc = int64_t(CI + cx * (CJ + cy * CK)) * NAV;
+        if (use)
 #pragma unroll
-        for (int v = 0; v < NAV; v++) sv[v] += w * (dlast ? Cl.x[rc + v] + dlast[rc + v] : Cl.x[rc + v]);
+            for (int v = 0; v < NAV; v++) sv[v] += w * (dlast ? Cl.x[rc + v] + dlast[rc + v] : Cl.x[rc + v]);
     }
 #pragma unroll
     for (int v = 0; v < NAV; v++) {
