@@ -87,6 +87,14 @@ typedef struct {
     double conv_h, conv_tinf;    /* convection coefficient / ambient temperature       */
     uint32_t conv_bid_mask;      /* boundary ids carrying convection (bit mask)        */
     int32_t flags;               /* reserved, 0                                        */
+    /* surface radiation (heat): flux sigma eps ((T + T0)^4 - (T_amb + T0)^4) on the
+     * sides whose boundary id is in rad_bid_mask, with its Jacobian
+     * 4 sigma eps (T + T0)^3 -- HeatConductionElementBase::surface_radiation_residual
+     * (src/heat_conduction/heat_conduction_elem_base.cpp:1402-1463) with the
+     * boundary condition's emissivity, ambient_temperature,
+     * reference_zero_temperature and stefan_bolzmann_constant parameters */
+    double rad_emissivity, rad_tamb, rad_tzero, rad_sigma;
+    uint32_t rad_bid_mask;       /* boundary ids carrying radiation (bit mask)         */
 } mastg_problem_desc;
 
 typedef struct {
@@ -188,15 +196,63 @@ int mastg_jacobian_product(mastg_ctx* ctx, const double* dX, double* JdX, int wh
 
 /* g_e = lambda_e . dR_e/dp_e for every element (constrained), the batched form of
  * sensitivity_assemble + the per-parameter dot of
- * calculate_output_adjoint_sensitivity_multiple_parameters_no_direct. */
+ * calculate_output_adjoint_sensitivity_multiple_parameters_no_direct
+ * (element parameters: rho_e, h_e, and the heat source q_e, whose dR/dq_e =
+ * -int N is volume_heat_source_residual_sensitivity,
+ * src/heat_conduction/heat_conduction_elem_base.cpp:1749-1791). */
 int mastg_sensitivity(mastg_ctx* ctx, const double* X, const double* lambda, double* g,
                       int where);
+
+/* Global parameters: every element's residual depends on them. */
+#define MASTG_PARAM_E 1           /* Young's modulus (hex8 elasticity, plate)           */
+#define MASTG_PARAM_NU 2          /* Poisson's ratio (hex8 elasticity, plate)           */
+#define MASTG_PARAM_PRESSURE 3    /* plate surface pressure                             */
+#define MASTG_PARAM_K0 4          /* conductivity k0 of k(T) = k0 (1 + alpha T) (heat)   */
+#define MASTG_PARAM_CONV_H 5      /* convection coefficient h (heat)                    */
+#define MASTG_PARAM_EMISSIVITY 6  /* radiation emissivity (heat)                        */
+
+/* dR/dp at state X for a global parameter, [n_dofs] caller numbering
+ * (constrained and inactive entries 0): NonlinearImplicitAssembly::
+ * sensitivity_assemble (src/base/nonlinear_implicit_assembly.cpp:515-686) for a
+ * parameter every element depends on, through the elements'
+ * elem_sensitivity_calculations (e.g. StructuralElement3D::internal_residual_sensitivity,
+ * src/elasticity/solid_element_3d.cpp:566-788; HeatConductionElementBase
+ * surface_convection/radiation_residual_sensitivity,
+ * src/heat_conduction/heat_conduction_elem_base.cpp:1160-1275, 1519-1634). */
+int mastg_sensitivity_rhs(mastg_ctx* ctx, int param, const double* X, double* dRdp, int where);
+/* lambda . dR/dp for a global parameter (one value): the per-parameter dot of
+ * AssemblyBase::calculate_output_adjoint_sensitivity_multiple_parameters_no_direct
+ * (src/base/assembly_base.cpp:594-639); the output's total derivative is
+ * dJ/dp = dJ/dp|_direct - lambda . dR/dp. */
+int mastg_sensitivity_global(mastg_ctx* ctx, int param, const double* X, const double* lambda,
+                             int where, double* value);
+/* Forward sensitivity: solve J dX/dp = -dR/dp with the Jacobian of the last
+ * mastg_assemble (assemble it at X first) -- NonlinearSystem::sensitivity_solve
+ * (src/base/nonlinear_system.cpp:742-794).  dXdp: [n_dofs] caller numbering. */
+int mastg_sensitivity_solve(mastg_ctx* ctx, int param, const double* X, double* dXdp, int where,
+                            const mastg_solve_opts* opts, mastg_solve_stats* stats);
 
 /* outputs */
 #define MASTG_OUTPUT_COMPLIANCE 1   /* C = F . X (external loads, incl. point loads) */
 #define MASTG_OUTPUT_MEAN_T 2       /* (1/V) int T dV                                */
+#define MASTG_OUTPUT_KS_STRESS 3    /* hex8 elasticity: (1/k) ln( sum_qp exp(k (svm - s_ref)
+                                       / s_ref) JxW / sum_qp JxW ), svm the von Mises stress of
+                                       E_e D B u at the quadrature points --
+                                       KSStressStrainOutput (src/elasticity/ks_stress_output.cpp:53-243)
+                                       over StructuralElement3D::calculate_stress
+                                       (src/elasticity/solid_element_3d.cpp:1147-1236)  */
 int mastg_output(mastg_ctx* ctx, int kind, const double* X, int where, double* value,
                  double* dvalue_dX /* [n_dofs] or NULL */);
+/* KS stress output's aggregation constant k and reference stress s_ref
+ * (KSStressStrainOutput's _p_norm_stress and _sigma0; defaults 10 and 1) */
+int mastg_set_ks_stress(mastg_ctx* ctx, double k, double sigma_ref);
+/* Direct derivative of an output with respect to every element parameter at
+ * state X, [n_elems] (the partial dJ/dp_e at fixed X; the total derivative
+ * adds -lambda . dR/dp_e from mastg_sensitivity with lambda solving
+ * J^T lambda = dJ/dX): zero for compliance and mean temperature; for the KS
+ * stress the rho_e^p dependence of the stress --
+ * KSStressStrainOutput::functional_sensitivity_for_elem (ks_stress_output.cpp:98-136). */
+int mastg_output_sensitivity(mastg_ctx* ctx, int kind, const double* X, int where, double* dJdp);
 
 /* ------------------------------------------- one design evaluation ---- */
 /* assemble + solve + output + adjoint sensitivities with everything resident
@@ -259,6 +315,36 @@ int mastg_get_tuning(const mastg_ctx* ctx, int* spmv_lanes, int* grid);
 int mastg_set_option(mastg_ctx* ctx, int option, int value);
 int mastg_set_tuning(mastg_ctx* ctx, int spmv_lanes, int grid);
 
+/* ------------------------------------------ element-wise design space --- */
+/* The steps of a SIMP topology-optimisation loop after the design evaluation,
+ * on the device:
+ *   - density filter over the element centroids (the points carrying the
+ *     element-wise design variables): rho~_e = sum_{|c_e - c_f| <= r}
+ *     (r - d_ef) rho_f / sum (r - d_ef) -- MAST::FilterBase::
+ *     compute_filtered_values with the weights of FilterBase::_init / _init2
+ *     (src/level_set/filter_base.cpp:35-142, 244-420), on the element field;
+ *     transpose = 1 applies its transpose (the chain rule of a sensitivity);
+ *   - material volume V = sum_e V_e rho_e and dV/drho_e = V_e (element volume,
+ *     or area for a plate's thickness variable) -- example_6's _evaluate_volume
+ *     (examples/structural/example_6/example_6.cpp:714-791) with element densities;
+ *   - one optimality-criteria update: rho_new = clamp(rho (B_e / lambda)^(1/2),
+ *     max(rho_min, rho - move), min(1, rho + move)), B_e = -dC/drho_e / dV/drho_e,
+ *     lambda such that V(rho_new) = volfrac * V_total (MAST drives external
+ *     optimisers through the src/optimization classes; this is the standard SIMP step).
+ * Built from a problem description's mesh (xyz, conn, physics); vectors are
+ * [n_elems] on the host or device (`where`). */
+typedef struct mastg_design mastg_design;
+int mastg_design_create(const mastg_problem_desc* desc, double filter_radius, mastg_design** out);
+int mastg_design_destroy(mastg_design* d);
+const char* mastg_design_last_error(const mastg_design* d);
+int mastg_design_info(const mastg_design* d, int64_t* n_elems, int64_t* filter_nnz, double* v_total);
+int mastg_design_filter(mastg_design* d, const double* in, double* out, int transpose, int where);
+int mastg_design_volume(mastg_design* d, const double* rho, int where, double* volume,
+                        double* dvol /* [n_elems] or NULL */);
+int mastg_design_oc_update(mastg_design* d, const double* rho, const double* dc, const double* dv,
+                           double volfrac, double move, double rho_min, double* rho_new, int where,
+                           double* lambda /* or NULL */);
+
 /* ---------------------------------------------------- request server --- */
 /* Continuous batching of design evaluations on one GPU (no counterpart in the
  * reference, whose optimizer drives one evaluation at a time through
@@ -301,6 +387,31 @@ int mastg_server_destroy(mastg_server* s);   /* drains the queue first */
 /* message of the most recent failed request ("" if none) */
 const char* mastg_server_last_error(mastg_server* s);
 int mastg_request_footprint(const mastg_request* req, int64_t* bytes);
+
+/* ---------------------------------------------------- Nastran models --- */
+/* Reads a Nastran bulk-data file (.bdf) into a problem description:
+ * MAST::NastranIO::read (src/mesh/nastran_io.cpp:217-255; nodes :120-143,
+ * elements :146-206, SPC sets :209-223), which parses through pyNastran.
+ * Nodes are ordered by Nastran id and elements by EID (the libMesh ids the
+ * reference gives them); DOF bases are handed out first-touch over the
+ * elements, 6 structural variables per node (MAST's
+ * StructuralSystemInitialization).  Cards: GRID (basic system), CHEXA (8
+ * nodes) or CQUAD4 (one element type per model), SPC1 / SPC (homogeneous:
+ * every set is applied), FORCE (basic frame: nodal loads), MAT1 (E, nu),
+ * PSHELL (plate thickness -> the element parameter h_e), PSOLID; small-,
+ * large- and free-field formats.  The element parameter of a hex8 model is
+ * 1 (a solid density).  The description points into the returned object,
+ * which owns its arrays until mastg_nastran_destroy. */
+typedef struct mastg_mesh_file mastg_mesh_file;
+int mastg_nastran_read(const char* path, mastg_mesh_file** out);
+const char* mastg_nastran_last_error(void);   /* of the calling thread's last failed read */
+const mastg_problem_desc* mastg_nastran_desc(const mastg_mesh_file* m);
+const double* mastg_nastran_params(const mastg_mesh_file* m);   /* [n_elems] */
+/* Nastran ids of the nodes [n_nodes] and of the elements and their property
+ * ids [n_elems], in the description's order (any pointer may be NULL) */
+int mastg_nastran_ids(const mastg_mesh_file* m, int64_t* node_ids, int64_t* elem_ids, int32_t* elem_pids);
+int mastg_nastran_spc_sets(const mastg_mesh_file* m);
+int mastg_nastran_destroy(mastg_mesh_file* m);
 
 /* ------------------------------------------------ benchmark workloads -- */
 /* Structured problems of the benchmark configurations, built the way libMesh

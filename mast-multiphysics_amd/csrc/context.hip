@@ -15,10 +15,15 @@
 
 #include "kernels.h"
 #include "mastg/mastg.h"
+#include "hip_lock.h"
 #include "multigrid.h"
 #include "structure.h"
 
 using namespace mastg;
+
+namespace mastg {
+std::recursive_mutex& hip_mutex();
+}
 
 namespace {
 
@@ -29,9 +34,14 @@ thread_local std::string g_create_error;
 // contexts driven from several threads (the request server) must never
 // allocate while another thread captures a solver graph.
 std::recursive_mutex g_hip_mu;
-using HipLock = std::lock_guard<std::recursive_mutex>;
 inline hipError_t ev_create(hipEvent_t* e) { HipLock l(g_hip_mu); return hipEventCreate(e); }
 inline hipError_t ev_destroy(hipEvent_t e) { HipLock l(g_hip_mu); return hipEventDestroy(e); }
+
+}  // namespace
+
+std::recursive_mutex& mastg::hip_mutex() { return g_hip_mu; }
+
+namespace {
 
 struct DevBuf {
     void* p = nullptr;
@@ -87,13 +97,17 @@ struct mastg_ctx {
     DevBuf Khat32;              // FP32 copy of Khat (lattice smoothing)
     DevBuf Gc;                  // [8][576] Q_c^T Khat Q_c (lattice Galerkin products)
     // heat: convective side masks, mean-T weights, BiCGStab work space, A^T values
-    DevBuf sidemask, wmean, tpos, valT, vr0, vv, vs, vt, vph, vsh, bscal, vT;
+    DevBuf sidemask, radmask, wmean, tpos, valT, vr0, vv, vs, vt, vph, vsh, bscal, vT;
+    bool has_rad = false;
+    double ks_k = 10.0, ks_sref = 1.0;   // KS stress output: aggregation constant, reference stress
+    DevBuf ks_part, ks_IV;
     double volume = 0;
     hipGraphExec_t bicg_exec = nullptr;
     int bicg_exec_len = 0;
     bool bicg_exec_T = false;
     // device data
     DevBuf param, Kel, Kel32, Rel, val, dinv, F;
+    DevBuf Fp;                        // plate: load vector of a unit pressure
     DevBuf vx, vr, vz, vp, vq, vb, vtmp, vcaller, vg, part0, part1, scal, dscalar;
     CGScalars* h_scal = nullptr;      // pinned mirror of the CG scalars
     double* h_pin = nullptr;          // pinned scratch for scalar read-back
@@ -198,6 +212,8 @@ void material_D3_unitE(double nu, double D[36]) {
     }
 }
 
+int global_dRdp(mastg_ctx* c, int param, const double* Xa, double* out);
+
 Hex8Args hex8_args(mastg_ctx* c, const double* X, bool want_K) {
     Hex8Args a{};
     a.n_elems = c->S.n_elems;
@@ -242,6 +258,11 @@ HeatArgs heat_args(mastg_ctx* c, const double* X, bool want_R, bool want_K) {
     a.h = c->d.conv_h;
     a.Tinf = c->d.conv_tinf;
     a.sides = c->sidemask.as<uint8_t>();
+    a.rsides = c->has_rad ? c->radmask.as<uint8_t>() : nullptr;
+    a.rad_se = c->d.rad_sigma * c->d.rad_emissivity;
+    a.rad_Tamb = c->d.rad_tamb;
+    a.rad_T0 = c->d.rad_tzero;
+    a.src_scale = 1.0;
     a.con = c->con.as<uint8_t>();
     a.X = X;
     a.Rel = want_R ? c->Rel.as<double>() : nullptr;
@@ -468,6 +489,45 @@ int cg_solve(mastg_ctx* c, const double* b, const mastg_solve_opts* o, mastg_sol
     return cg_loop(c, a, b, o, op, chunk, e0, e1, st);
 }
 
+// Captures `body` (kernel launches on c->stream) into *ex.  The capture is
+// always ended, also when a step fails; *ex is null unless the graph was
+// instantiated.  Capture and the exec's destruction hold the process-wide HIP
+// lock (other threads' contexts may capture or allocate at the same time).
+template <class Body>
+int capture_graph(mastg_ctx* c, hipGraphExec_t* ex, Body&& body) {
+    HipLock capture_lock(g_hip_mu);
+    if (*ex) (void)hipGraphExecDestroy(*ex);
+    *ex = nullptr;
+    HIPC(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    body();
+    hipGraph_t g = nullptr;
+    const hipError_t e_end = hipStreamEndCapture(c->stream, &g);
+    if (e_end != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        return fail(c, MASTG_ERR_DEVICE, std::string("hipStreamEndCapture failed: ") + hipGetErrorString(e_end));
+    }
+    hipGraphExec_t out = nullptr;
+    const hipError_t e_inst = hipGraphInstantiate(&out, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e_inst != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(c, MASTG_ERR_DEVICE, std::string("hipGraphInstantiate failed: ") + hipGetErrorString(e_inst));
+    }
+    *ex = out;
+    return MASTG_OK;
+}
+
+// every captured solver graph of the context (launch shapes changed)
+void drop_graphs(mastg_ctx* c) {
+    HipLock l(g_hip_mu);
+    for (hipGraphExec_t* ex : {&c->cg_exec[0], &c->cg_exec[1], &c->cg_exec[2], &c->bicg_exec, &c->mg_exec,
+                               &c->mg_bicg_exec}) {
+        if (*ex) (void)hipGraphExecDestroy(*ex);
+        *ex = nullptr;
+    }
+}
+
 // Iterations run in chunks (hipGraph replays or direct launches) between host
 // polls of the device scalars.  The recursive residual drifts from b - A x on
 // ill-conditioned systems (and with the FP32 operator), so convergence is
@@ -481,15 +541,9 @@ int cg_loop(mastg_ctx* c, const CGArgs& a, const double* b, const mastg_solve_op
     const Structure& S = c->S;
     const double delta = (o && o->reliable_delta > 0) ? o->reliable_delta : 0.1;
     if (c->use_graphs && (!c->cg_exec[op] || c->cg_exec_len[op] != chunk)) {
-        if (c->cg_exec[op]) (void)hipGraphExecDestroy(c->cg_exec[op]);
-        c->cg_exec[op] = nullptr;
-        HipLock capture_lock(g_hip_mu);
-        hipGraph_t g;
-        HIPC(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        for (int i = 0; i < chunk; i++) launch_cg_iteration(a, c->stream);
-        HIPC(c, hipStreamEndCapture(c->stream, &g));
-        HIPC(c, hipGraphInstantiate(&c->cg_exec[op], g, nullptr, nullptr, 0));
-        (void)hipGraphDestroy(g);
+        TRY(capture_graph(c, &c->cg_exec[op], [&]() {
+            for (int i = 0; i < chunk; i++) launch_cg_iteration(a, c->stream);
+        }));
         c->cg_exec_len[op] = chunk;
     }
     HIPC(c, hipMemcpyAsync(c->h_scal, a.S, sizeof(CGScalars), hipMemcpyDeviceToHost, c->stream));
@@ -567,13 +621,11 @@ int mg_build(mastg_ctx* c) {
     if (c->mg_tried) return c->mg.empty() ? fail(c, MASTG_ERR_INVALID, "multigrid unavailable: " + c->mg_why)
                                           : MASTG_OK;
     c->mg_tried = true;
-    // the plate and heat problem levels smooth with their assembled node blocks
-    // in node-rank order; hex8 elasticity uses the per-node lattice operator
-    const bool plate = c->d.physics != MASTG_HEX8_ELASTIC;
-    if (!(plate || c->S.n_classes == 1)) {
-        c->mg_why = "multigrid is built for lattice meshes (hex8 elasticity needs one element shape)";
-        return fail(c, MASTG_ERR_INVALID, "multigrid unavailable: " + c->mg_why);
-    }
+    // the plate and heat problem levels, and hex8 elasticity on a lattice of
+    // cells that are not all congruent (displaced nodes), smooth with their
+    // assembled node blocks in node-rank order; hex8 elasticity on one element
+    // shape uses the per-node lattice operator (no assembled matrix)
+    const bool plate = c->d.physics != MASTG_HEX8_ELASTIC || c->S.n_classes != 1;
     MGHierarchy H = build_mg_hierarchy(c->S);
     if (!H.ok) {
         c->mg_why = H.why;
@@ -1014,15 +1066,9 @@ int mg_solve(mastg_ctx* c, const double* b, const mastg_solve_opts* o, mastg_sol
     HIPC(c, hipStreamSynchronize(c->stream));
     int done = c->h_scal->done;
     if (c->use_graphs && !done && (!c->mg_exec || c->mg_exec_len != chunk)) {
-        if (c->mg_exec) (void)hipGraphExecDestroy(c->mg_exec);
-        c->mg_exec = nullptr;
-        HipLock capture_lock(g_hip_mu);
-        hipGraph_t g;
-        HIPC(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        for (int i = 0; i < chunk; i++) mg_pcg_iteration(c, a, c->stream);
-        HIPC(c, hipStreamEndCapture(c->stream, &g));
-        HIPC(c, hipGraphInstantiate(&c->mg_exec, g, nullptr, nullptr, 0));
-        (void)hipGraphDestroy(g);
+        TRY(capture_graph(c, &c->mg_exec, [&]() {
+            for (int i = 0; i < chunk; i++) mg_pcg_iteration(c, a, c->stream);
+        }));
         c->mg_exec_len = chunk;
     }
     double best2 = 1e300;
@@ -1194,15 +1240,9 @@ int bicg_solve(mastg_ctx* c, const double* b, const mastg_solve_opts* o, mastg_s
     };
     if (c->use_graphs) {
         if (!ex || ex_len != chunk || ex_T != transpose) {
-            if (ex) (void)hipGraphExecDestroy(ex);
-            ex = nullptr;
-            HipLock capture_lock(g_hip_mu);
-            hipGraph_t g;
-            HIPC(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            for (int i = 0; i < chunk; i++) iteration();
-            HIPC(c, hipStreamEndCapture(c->stream, &g));
-            HIPC(c, hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
-            (void)hipGraphDestroy(g);
+            TRY(capture_graph(c, &ex, [&]() {
+                for (int i = 0; i < chunk; i++) iteration();
+            }));
             ex_len = chunk;
             ex_T = transpose;
         }
@@ -1268,8 +1308,7 @@ int autotune(mastg_ctx* c) {
         }
     c->nparts = bgrid;
     c->spmv_sw = bsw;
-    for (int i = 0; i < 3; i++)
-        if (c->cg_exec[i]) { (void)hipGraphExecDestroy(c->cg_exec[i]); c->cg_exec[i] = nullptr; }
+    drop_graphs(c);   // every captured solver reads the launch shapes
     return MASTG_OK;
 }
 
@@ -1445,8 +1484,10 @@ int mastg_ctx_create(const mastg_problem_desc* desc, mastg_ctx** out) {
                 return bail(MASTG_ERR_INVALID);
             }
         // uniform pressure along local -z: R_w += p int N (surface_pressure_residual,
-        // src/elasticity/structural_element_base.cpp:1167-1235), i.e. F_w = -p int N
-        if (c->d.pressure != 0.0) {
+        // src/elasticity/structural_element_base.cpp:1167-1235), i.e. F_w = -p int N;
+        // the load of a unit pressure is kept for dR/dp
+        std::vector<double> Fp(S.n_active, 0.0);
+        {
             const double gq = 0.57735026918962576451;
             for (int64_t e = 0; e < S.n_elems; e++) {
                 double X[4][2];
@@ -1467,11 +1508,12 @@ int mastg_ctx_create(const mastg_problem_desc* desc, mastg_ctx** out) {
                             j10 += X[n][1] * dxi[n]; j11 += X[n][1] * deta[n];
                         }
                         const double jxw = std::fabs(j00 * j11 - j01 * j10);
-                        for (int n = 0; n < 4; n++)
-                            F[int64_t(S.econn[e * 4 + n]) * 5 + 2] -= jxw * N[n] * c->d.pressure;
+                        for (int n = 0; n < 4; n++) Fp[int64_t(S.econn[e * 4 + n]) * 5 + 2] -= jxw * N[n];
                     }
             }
         }
+        for (int64_t i = 0; i < S.n_active; i++) F[i] += c->d.pressure * Fp[i];
+        if ((rc = upload(c, c->Fp, Fp))) return bail(rc);
     }
     std::vector<int32_t> conv(S.constrained.begin(), S.constrained.end());
     if ((rc = upload(c, c->xyz, S.xyz_r)) || (rc = upload(c, c->econn, S.econn)) ||
@@ -1486,7 +1528,11 @@ int mastg_ctx_create(const mastg_problem_desc* desc, mastg_ctx** out) {
         return bail(rc);
     if (desc->physics == MASTG_HEX8_HEAT) {
         // convective sides per element and the mean-temperature weights
-        std::vector<uint8_t> sm(S.n_elems, 0);
+        std::vector<uint8_t> sm(S.n_elems, 0), rm(S.n_elems, 0);
+        if (c->d.rad_bid_mask && !(c->d.rad_emissivity >= 0 && c->d.rad_sigma >= 0)) {
+            c->err = "radiation needs a non-negative emissivity and Stefan-Boltzmann constant";
+            return bail(MASTG_ERR_INVALID);
+        }
         for (int64_t k = 0; k < desc->n_sides; k++) {
             const int32_t e = desc->side_elem[k], sd = desc->side_index[k], id = desc->side_id[k];
             if (e < 0 || e >= S.n_elems || sd < 0 || sd > 5 || id < 0 || id > 31) {
@@ -1494,7 +1540,12 @@ int mastg_ctx_create(const mastg_problem_desc* desc, mastg_ctx** out) {
                 return bail(MASTG_ERR_INVALID);
             }
             if ((c->d.conv_bid_mask >> id) & 1u) sm[e] |= uint8_t(1u << sd);
+            if ((c->d.rad_bid_mask >> id) & 1u) {
+                rm[e] |= uint8_t(1u << sd);
+                c->has_rad = true;
+            }
         }
+        if (c->has_rad && (rc = upload(c, c->radmask, rm))) return bail(rc);
         std::vector<double> w(S.n_active, 0.0);
         const double gq = 0.57735026918962576451;
         double V = 0;
@@ -1620,9 +1671,13 @@ int mastg_set_params(mastg_ctx* c, const double* p, int where) {
     if (!c || !p) return MASTG_ERR_INVALID;
     const size_t nb = c->S.n_elems * sizeof(double);
     if (where == MASTG_HOST) {
+        // densities and thicknesses must be positive; a heat source may take any
+        // finite value (zero: no source, negative: a sink)
+        const bool heat = c->d.physics == MASTG_HEX8_HEAT;
         for (int64_t e = 0; e < c->S.n_elems; e++)
-            if (!std::isfinite(p[e]) || p[e] <= 0.0)
-                return fail(c, MASTG_ERR_NAN, "design parameter " + std::to_string(e) + " is not a positive finite number");
+            if (!std::isfinite(p[e]) || (!heat && p[e] <= 0.0))
+                return fail(c, MASTG_ERR_NAN, "design parameter " + std::to_string(e) +
+                                                  (heat ? " is not finite" : " is not a positive finite number"));
         HIPC(c, hipMemcpyAsync(c->param.p, p, nb, hipMemcpyHostToDevice, c->stream));
     } else {
         HIPC(c, hipMemcpyAsync(c->param.p, p, nb, hipMemcpyDeviceToDevice, c->stream));
@@ -1672,26 +1727,42 @@ static int check_cg_solver(mastg_ctx* c, int solver) {
     return MASTG_OK;
 }
 
-int mastg_solve(mastg_ctx* c, const double* rhs, double* x, int where, const mastg_solve_opts* o,
-                mastg_solve_stats* st) {
-    if (!c || !rhs || !x) return MASTG_ERR_INVALID;
+// J x = b (b active, in c->vb) with the assembled matrix, x -> c->vx
+static int solve_active(mastg_ctx* c, const mastg_solve_opts* o, mastg_solve_stats* st) {
     if (!c->assembled) return fail(c, MASTG_ERR_INVALID, "no assembled matrix");
     if (o && o->transpose) { /* the elastic Jacobian is symmetric */ }
     int solver = o ? o->solver : MASTG_SOLVER_CG_JACOBI;
     if (c->d.physics == MASTG_HEX8_HEAT && solver == MASTG_SOLVER_CG_JACOBI)
         solver = MASTG_SOLVER_BICGSTAB_JACOBI;   // the k(T) Jacobian is not symmetric
     const bool heat_mg = c->d.physics == MASTG_HEX8_HEAT && solver == MASTG_SOLVER_MG;
-    if (solver == MASTG_SOLVER_BICGSTAB_JACOBI || heat_mg) {
-        TRY(to_active(c, rhs, where, c->vb.as<double>()));
-        int rc = bicg_solve(c, c->vb.as<double>(), o, st, o && o->transpose, heat_mg);
-        int rc2 = from_active(c, c->vx.as<double>(), x, where);
-        return rc ? rc : rc2;
-    }
+    if (solver == MASTG_SOLVER_BICGSTAB_JACOBI || heat_mg)
+        return bicg_solve(c, c->vb.as<double>(), o, st, o && o->transpose, heat_mg);
     TRY(check_cg_solver(c, solver));
+    return solver == MASTG_SOLVER_MG ? mg_solve(c, c->vb.as<double>(), o, st)
+                                     : cg_solve(c, c->vb.as<double>(), o, st, cg_op(solver));
+}
+
+int mastg_solve(mastg_ctx* c, const double* rhs, double* x, int where, const mastg_solve_opts* o,
+                mastg_solve_stats* st) {
+    if (!c || !rhs || !x) return MASTG_ERR_INVALID;
+    if (!c->assembled) return fail(c, MASTG_ERR_INVALID, "no assembled matrix");
     TRY(to_active(c, rhs, where, c->vb.as<double>()));
-    int rc = solver == MASTG_SOLVER_MG ? mg_solve(c, c->vb.as<double>(), o, st)
-                                       : cg_solve(c, c->vb.as<double>(), o, st, cg_op(solver));
+    int rc = solve_active(c, o, st);
     int rc2 = from_active(c, c->vx.as<double>(), x, where);
+    return rc ? rc : rc2;
+}
+
+int mastg_sensitivity_solve(mastg_ctx* c, int param, const double* X, double* dXdp, int where,
+                            const mastg_solve_opts* o, mastg_solve_stats* st) {
+    if (!c || !X || !dXdp) return MASTG_ERR_INVALID;
+    if (!c->assembled) return fail(c, MASTG_ERR_INVALID, "no assembled matrix (assemble the Jacobian at X first)");
+    if (o && o->transpose) return fail(c, MASTG_ERR_INVALID, "the forward sensitivity solve is not transposed");
+    TRY(to_active(c, X, where, c->vb.as<double>()));
+    TRY(global_dRdp(c, param, c->vb.as<double>(), c->vtmp.as<double>()));
+    // b = -dR/dp
+    launch_axpby(c->S.n_active, -1.0, c->vtmp.as<double>(), 0.0, c->vb.as<double>(), c->stream);
+    int rc = solve_active(c, o, st);
+    int rc2 = from_active(c, c->vx.as<double>(), dXdp, where);
     return rc ? rc : rc2;
 }
 
@@ -1744,7 +1815,6 @@ int mastg_jacobian_product(mastg_ctx* c, const double* dX, double* JdX, int wher
 
 int mastg_sensitivity(mastg_ctx* c, const double* X, const double* lam, double* g, int where) {
     if (!c || !X || !lam || !g) return MASTG_ERR_INVALID;
-    if (c->d.physics == MASTG_HEX8_HEAT) return fail(c, MASTG_ERR_INVALID, "physics not available");
     if (!c->params_set) return fail(c, MASTG_ERR_INVALID, "design parameters not set");
     TRY(to_active(c, X, where, c->vb.as<double>()));
     TRY(to_active(c, lam, where, c->vtmp.as<double>()));
@@ -1759,8 +1829,189 @@ int mastg_sensitivity(mastg_ctx* c, const double* X, const double* lam, double* 
     return MASTG_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// d D3(E = 1, nu) / d nu: derivative of the isotropic 3D material matrix at unit
+// modulus (IsotropicMaterialProperty::StiffnessMatrix3D::derivative,
+// src/property_cards/isotropic_material_property_card.cpp:533-567, with dE = 0)
+void material_dD3_dnu_unitE(double nu, double D[36]) {
+    std::memset(D, 0, 36 * sizeof(double));
+    const double d = 1. - nu - 2. * nu * nu, dd = -1. - 4. * nu;
+    const double diag = (-d - (1. - nu) * dd) / (d * d);   // d/dnu (1 - nu) / d
+    const double off = (d - nu * dd) / (d * d);            // d/dnu nu / d
+    for (int i = 0; i < 3; i++) {
+        for (int j = 0; j < 3; j++) D[i * 6 + j] = (i == j) ? diag : off;
+        D[(i + 3) * 6 + i + 3] = -0.5 / ((1. + nu) * (1. + nu));   // d/dnu 1 / (2 (1 + nu))
+    }
+}
+
+// dR/dp at the active state Xa for a global parameter p, into `out` (active,
+// constrained entries 0).  Every residual term is linear in E (elasticity, the
+// plate's sections), in k0, h and the emissivity (heat) and in the pressure
+// (plate loads), so dR/dp is the term's residual at a unit parameter; nu
+// enters through D (and the plate's G), whose nu-derivative replaces it in the
+// element kernel.
+int global_dRdp(mastg_ctx* c, int param, const double* Xa, double* out) {
+    const Structure& S = c->S;
+    TRY(dalloc(c, c->Rel, size_t(S.n_elems) * S.npe * S.nav * sizeof(double)));
+    const int ph = c->d.physics;
+    double scale = 1.0;
+    bool load_term = false;
+    if (ph == MASTG_HEX8_ELASTIC) {
+        Hex8Args a = hex8_args(c, Xa, false);
+        if (param == MASTG_PARAM_E) {
+            if (c->d.E == 0) return fail(c, MASTG_ERR_INVALID, "dR/dE needs E != 0");
+            scale = 1.0 / c->d.E;
+        } else if (param == MASTG_PARAM_NU) {
+            material_dD3_dnu_unitE(c->d.nu, a.Dhat);
+        } else {
+            return fail(c, MASTG_ERR_INVALID, "parameter not defined for hex8 elasticity (E, nu)");
+        }
+        launch_hex8_K(a, c->stream);
+    } else if (ph == MASTG_QUAD4_MINDLIN) {
+        Quad4Args a = quad4_args(c, Xa, false);
+        const double E = c->d.E, nu = c->d.nu;
+        if (param == MASTG_PARAM_E) {
+            if (E == 0) return fail(c, MASTG_ERR_INVALID, "dR/dE needs E != 0");
+            scale = 1.0 / E;
+        } else if (param == MASTG_PARAM_NU) {
+            // d/dnu of the plane-stress D and of G = E / (2 (1 + nu))
+            // (isotropic_material_property_card.cpp:334-345, 431-493 with dE = 0)
+            const double q = 1. - nu * nu;
+            std::memset(a.Dps, 0, sizeof(a.Dps));
+            a.Dps[0] = a.Dps[4] = 2. * E * nu / (q * q);
+            a.Dps[1] = a.Dps[3] = E * (1. + nu * nu) / (q * q);
+            a.Dps[8] = -E / (2. * (1. + nu) * (1. + nu));
+            a.G = -E / (2. * (1. + nu) * (1. + nu));
+        } else if (param == MASTG_PARAM_PRESSURE) {
+            load_term = true;   // dR/dp = -(load of a unit pressure)
+        } else {
+            return fail(c, MASTG_ERR_INVALID, "parameter not defined for the Mindlin plate (E, nu, pressure)");
+        }
+        if (!load_term) launch_quad4_K(a, c->stream);
+    } else {
+        HeatArgs a = heat_args(c, Xa, true, false);
+        a.k0 = 0.0;
+        a.h = 0.0;
+        a.rad_se = 0.0;
+        a.src_scale = 0.0;
+        if (param == MASTG_PARAM_K0) a.k0 = 1.0;
+        else if (param == MASTG_PARAM_CONV_H) a.h = 1.0;
+        else if (param == MASTG_PARAM_EMISSIVITY) {
+            if (!c->has_rad) return fail(c, MASTG_ERR_INVALID, "no radiating sides");
+            a.rad_se = c->d.rad_sigma;
+        } else {
+            return fail(c, MASTG_ERR_INVALID, "parameter not defined for heat conduction (k0, h, emissivity)");
+        }
+        launch_heat_K(a, c->stream);
+    }
+    // -(unit pressure load) for the pressure, else the elements' rows
+    launch_node_residual(S.n_nodes, S.nav, S.npe, c->ninc_ptr.as<int32_t>(), c->ninc.as<int32_t>(),
+                         load_term ? nullptr : c->Rel.as<double>(), load_term ? c->Fp.as<double>() : nullptr,
+                         c->con.as<uint8_t>(), out, c->stream);
+    if (scale != 1.0) launch_axpby(S.n_active, 0.0, out, scale, out, c->stream);
+    HIPC(c, hipGetLastError());
+    return MASTG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mastg_sensitivity_rhs(mastg_ctx* c, int param, const double* X, double* dRdp, int where) {
+    if (!c || !X || !dRdp) return MASTG_ERR_INVALID;
+    TRY(to_active(c, X, where, c->vb.as<double>()));
+    TRY(global_dRdp(c, param, c->vb.as<double>(), c->vtmp.as<double>()));
+    return from_active(c, c->vtmp.as<double>(), dRdp, where);
+}
+
+int mastg_sensitivity_global(mastg_ctx* c, int param, const double* X, const double* lam, int where,
+                             double* value) {
+    if (!c || !X || !lam || !value) return MASTG_ERR_INVALID;
+    TRY(to_active(c, X, where, c->vb.as<double>()));
+    TRY(global_dRdp(c, param, c->vb.as<double>(), c->vtmp.as<double>()));
+    TRY(to_active(c, lam, where, c->vz.as<double>()));
+    return dot_active(c, c->vz.as<double>(), c->vtmp.as<double>(), value);
+}
+
+int mastg_set_ks_stress(mastg_ctx* c, double k, double sigma_ref) {
+    if (!c) return MASTG_ERR_INVALID;
+    if (!(k > 0) || !(sigma_ref > 0) || !std::isfinite(k) || !std::isfinite(sigma_ref))
+        return fail(c, MASTG_ERR_INVALID, "KS aggregation constant and reference stress must be positive");
+    c->ks_k = k;
+    c->ks_sref = sigma_ref;
+    return MASTG_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// KS stress at the active state in c->vb: I and V into c->ks_IV (device),
+// value on the host
+int ks_value(mastg_ctx* c, double* value) {
+    if (c->d.physics != MASTG_HEX8_ELASTIC)
+        return fail(c, MASTG_ERR_INVALID, "the KS stress output is defined for hex8 elasticity");
+    if (!c->params_set) return fail(c, MASTG_ERR_INVALID, "design parameters not set");
+    TRY(dalloc(c, c->ks_part, 2 * 1024 * sizeof(double)));
+    TRY(dalloc(c, c->ks_IV, 2 * sizeof(double)));
+    launch_hex8_ks_sum(hex8_args(c, c->vb.as<double>(), false), c->ks_k, c->ks_sref, c->ks_part.as<double>(),
+                       c->ks_IV.as<double>(), c->stream);
+    HIPC(c, hipMemcpyAsync(c->h_pin, c->ks_IV.p, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    if (!(c->h_pin[0] > 0) || !std::isfinite(c->h_pin[0]))
+        return fail(c, MASTG_ERR_NAN, "KS sum overflowed or vanished: choose the reference stress near the peak stress");
+    *value = std::log(c->h_pin[0] / c->h_pin[1]) / c->ks_k;
+    return MASTG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mastg_output_sensitivity(mastg_ctx* c, int kind, const double* X, int where, double* dJdp) {
+    if (!c || !X || !dJdp) return MASTG_ERR_INVALID;
+    double* gd = where == MASTG_HOST ? c->vg.as<double>() : dJdp;
+    if (kind == MASTG_OUTPUT_COMPLIANCE || kind == MASTG_OUTPUT_MEAN_T) {
+        // the loads and the mean-temperature weights do not depend on the
+        // element parameters
+        launch_fill(c->S.n_elems, 0.0, gd, c->stream);
+    } else if (kind == MASTG_OUTPUT_KS_STRESS) {
+        TRY(to_active(c, X, where, c->vb.as<double>()));
+        double v;
+        TRY(ks_value(c, &v));
+        TRY(dalloc(c, c->Rel, size_t(c->S.n_elems) * 24 * sizeof(double)));
+        launch_hex8_ks_grad(hex8_args(c, c->vb.as<double>(), false), c->ks_k, c->ks_sref, c->ks_IV.as<double>(),
+                            gd, c->stream);
+    } else {
+        return fail(c, MASTG_ERR_INVALID, "output not available");
+    }
+    HIPC(c, hipGetLastError());
+    if (where == MASTG_HOST)
+        HIPC(c, hipMemcpyAsync(dJdp, gd, c->S.n_elems * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    return MASTG_OK;
+}
+
 int mastg_output(mastg_ctx* c, int kind, const double* X, int where, double* value, double* dv) {
     if (!c || !X || !value) return MASTG_ERR_INVALID;
+    if (kind == MASTG_OUTPUT_KS_STRESS) {
+        TRY(to_active(c, X, where, c->vb.as<double>()));
+        TRY(ks_value(c, value));
+        if (dv) {
+            TRY(dalloc(c, c->Rel, size_t(c->S.n_elems) * 24 * sizeof(double)));
+            launch_hex8_ks_grad(hex8_args(c, c->vb.as<double>(), false), c->ks_k, c->ks_sref,
+                                c->ks_IV.as<double>(), nullptr, c->stream);
+            launch_node_residual(c->S.n_nodes, c->S.nav, c->S.npe, c->ninc_ptr.as<int32_t>(),
+                                 c->ninc.as<int32_t>(), c->Rel.as<double>(), nullptr, c->con.as<uint8_t>(),
+                                 c->vtmp.as<double>(), c->stream);
+            TRY(from_active(c, c->vtmp.as<double>(), dv, where));
+        }
+        HIPC(c, hipGetLastError());
+        return MASTG_OK;
+    }
     if (kind == MASTG_OUTPUT_MEAN_T) {
         if (c->d.physics != MASTG_HEX8_HEAT) return fail(c, MASTG_ERR_INVALID, "mean temperature needs heat physics");
         TRY(to_active(c, X, where, c->vb.as<double>()));
@@ -1890,9 +2141,7 @@ int mastg_set_option(mastg_ctx* c, int option, int value) {
                 (option == MASTG_OPT_MG_STEPS_FINE ? c->mg_k0 : c->mg_kc) = value;
             }
             // the captured V-cycles and the Chebyshev coefficients change
-            if (c->mg_exec) (void)hipGraphExecDestroy(c->mg_exec);
-            if (c->mg_bicg_exec) (void)hipGraphExecDestroy(c->mg_bicg_exec);
-            c->mg_exec = c->mg_bicg_exec = nullptr;
+            drop_graphs(c);
             c->mg_valid = false;
             return MASTG_OK;
         default: return fail(c, MASTG_ERR_INVALID, "unknown option");
@@ -1913,8 +2162,7 @@ int mastg_set_tuning(mastg_ctx* c, int spmv_lanes, int grid) {
     c->spmv_sw = spmv_lanes;
     c->nparts = grid;
     c->tuned = true;
-    for (int i = 0; i < 3; i++)
-        if (c->cg_exec[i]) { (void)hipGraphExecDestroy(c->cg_exec[i]); c->cg_exec[i] = nullptr; }
+    drop_graphs(c);   // CG, BiCGStab and multigrid graphs all read the launch shapes
     return MASTG_OK;
 }
 
@@ -1947,6 +2195,7 @@ int mastg_bench_kernel(mastg_ctx* c, int kind, int n_launch, double* avg_ms) {
             case 6: launch_lattice_apply(a, c->mg[0]->b_lat.as<double>(), c->mg[0]->y_lat.as<double>(), c->stream); break;
             case 7: launch_copy_f64(kCopyDoubles, copy_src.as<double>(), copy_dst.as<double>(), c->stream); break;
             case 9: launch_spmv_nb64(a, a.p, a.q, 1, c->stream); break;
+            case 10: (void)assemble_active(c, c->vx.as<double>(), true, true); break;
             case 8: {
                 const MGLevelArgs& L0 = c->mg[0]->a;
                 launch_mg_fine(a, L0, MG_STEP, 1, L0.da32, L0.db32, nullptr, c->stream);
@@ -1955,7 +2204,16 @@ int mastg_bench_kernel(mastg_ctx* c, int kind, int n_launch, double* avg_ms) {
             default: break;
         }
     };
-    if (kind < 0 || kind > 9) return fail(c, MASTG_ERR_INVALID, "unknown kernel kind");
+    if (kind < 0 || kind > 10) return fail(c, MASTG_ERR_INVALID, "unknown kernel kind");
+    if (kind == 0 && c->d.physics != MASTG_HEX8_HEAT && !c->Kel.p)
+        return fail(c, MASTG_ERR_INVALID, "no FP64 element blocks resident (assemble first)");
+    if (kind == 10) {
+        // the whole FP64 assembly at the state in the solution vector:
+        // element K and R, scatter into the CSR, inverse diagonal, node
+        // residual (allocates on the first call only)
+        TRY(ensure_vectors(c));
+        TRY(assemble_active(c, c->vx.as<double>(), true, true));
+    }
     if (kind == 9 && (c->S.nav != 5 || !c->assembled))
         return fail(c, MASTG_ERR_INVALID, "the node-block FP64 product is the plates', after an assembly");
     if (kind == 8) {
@@ -1997,6 +2255,8 @@ int mastg_bench_kernel(mastg_ctx* c, int kind, int n_launch, double* avg_ms) {
     float ms = 0;
     HIPC(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     *avg_ms = double(ms) / n_launch;
+    // the copy kernel's buffers are released with this frame
+    c->device_bytes -= int64_t(copy_src.bytes + copy_dst.bytes);
     HIPC(c, hipGetLastError());
     return MASTG_OK;
 }

@@ -46,6 +46,11 @@ struct HeatArgs {
     const double* X;        // [n_active] temperature (state), required
     double* Rel;            // [n_elems*8] element residual or null
     double* Kel;            // [n_elems*64] element Jacobian or null
+    // surface radiation: flux rad_se ((T + rad_T0)^4 - (rad_Tamb + rad_T0)^4)
+    // on the sides marked in rsides (null: none); rad_se = sigma * emissivity
+    const uint8_t* rsides;  // [n_elems] bit s = side s radiates
+    double rad_se, rad_Tamb, rad_T0;
+    double src_scale;       // multiplies the source q_e (1; 0 leaves the source out)
 };
 
 struct ScatterArgs {
@@ -152,6 +157,12 @@ void launch_spmv_nb64(const CGArgs& c, const double* x, double* y, int want_dot,
 void launch_cg_replace(const CGArgs& c, const double* rt, int restart, hipStream_t s);
 void launch_hex8_K(const Hex8Args& a, hipStream_t s);
 void launch_hex8_sens(const Hex8Args& a, const double* lam, double* g, hipStream_t s);
+// KS aggregate of the von Mises stress (aggregation constant k, reference
+// stress sref) at the state a.X: IV[0] = sum exp(k (svm - sref) / sref) JxW,
+// IV[1] = sum JxW (part: 2 x 1024 doubles of partial sums); then the element
+// rows of dKS/dX into a.Rel and, when dpar is not null, dKS/drho_e|direct
+void launch_hex8_ks_sum(const Hex8Args& a, double k, double sref, double* part, double* IV, hipStream_t s);
+void launch_hex8_ks_grad(const Hex8Args& a, double k, double sref, const double* IV, double* dpar, hipStream_t s);
 void launch_hex8_K_f32(const Hex8Args& a, float* Kel32, hipStream_t s);
 void launch_scatter_f32(const ScatterArgs& a, const float* Kel32, hipStream_t s);
 void launch_scatter(const ScatterArgs& a, hipStream_t s);

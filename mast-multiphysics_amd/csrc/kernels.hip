@@ -674,14 +674,17 @@ __global__ __launch_bounds__(256) void k_hex8_heat_K(HeatArgs A) {
         K += c * (gT0 * ga0 + gT1 * ga1 + gT2 * ga2) * dk * sN[wave][q][b];
         if (b == 0) {
             f += c * kk * (ga0 * gT0 + ga1 * gT1 + ga2 * gT2);
-            f -= c * sN[wave][q][a] * A.param[e];
+            f -= c * sN[wave][q][a] * (A.param[e] * A.src_scale);
         }
     }
-    // convection on marked sides (2x2 Gauss on the face)
-    const int sm = A.sides ? A.sides[e] : 0;
+    // convection and radiation on marked sides (2x2 Gauss on the face)
+    const int cm = A.sides ? A.sides[e] : 0;
+    const int rm = A.rsides ? A.rsides[e] : 0;
+    const int sm = cm | rm;
     if (sm)
         for (int sd = 0; sd < 6; sd++) {
             if (!((sm >> sd) & 1)) continue;
+            const bool conv = (cm >> sd) & 1, rad = (rm >> sd) & 1;
             for (int qq = 0; qq < 4; qq++) {
                 double xi, eta, zeta;
                 int r0, r1;
@@ -707,8 +710,16 @@ __global__ __launch_bounds__(256) void k_hex8_heat_K(HeatArgs A) {
                 const double g12 = t0[0] * t1[0] + t0[1] * t1[1] + t0[2] * t1[2];
                 const double g22 = t1[0] * t1[0] + t1[1] * t1[1] + t1[2] * t1[2];
                 const double jxw = sqrt(g11 * g22 - g12 * g12) * 1.0;
-                K += jxw * (Na * Nb) * A.h;
-                if (b == 0) f += jxw * Na * A.h * (T - A.Tinf);
+                if (conv) {
+                    K += jxw * (Na * Nb) * A.h;
+                    if (b == 0) f += jxw * Na * A.h * (T - A.Tinf);
+                }
+                if (rad) {   // surface_radiation_residual (heat_conduction_elem_base.cpp:1402-1463)
+                    const double ts = T + A.rad_T0, ta = A.rad_Tamb + A.rad_T0;
+                    const double ts3 = ts * ts * ts;
+                    K += jxw * (Na * Nb) * A.rad_se * 4.0 * ts3;
+                    if (b == 0) f += jxw * Na * A.rad_se * (ts3 * ts - ta * ta * ta * ta);
+                }
             }
         }
     const int32_t ra = A.econn[e * 8 + a], rb = A.econn[e * 8 + b];
@@ -841,6 +852,8 @@ __global__ __launch_bounds__(256) void k_reduce_to(const double* __restrict__ p,
     const double s = reduce_partials(p, n);
     if (threadIdx.x == 0) *out = s;
 }
+
+
 
 
 // ---------------------------------------------------------------------------

@@ -1,6 +1,8 @@
 // Multigrid hierarchy construction (see multigrid.h).
 #include "multigrid.h"
 
+#include <algorithm>
+#include <climits>
 #include <cmath>
 
 namespace mastg {
@@ -66,6 +68,96 @@ bool detect_lattice(const Structure& S, int dim, MGLevelHost& L, std::string& wh
     return true;
 }
 
+// The same lattice found from the connectivity alone (node coordinates may be
+// displaced, e.g. a distorted mesh of a block): element 0's corners fix the
+// lattice indices of its nodes, and every element that shares an indexed node
+// places its corners at its base point + the libMesh corner offsets; the
+// indices must agree everywhere and fill a box.
+bool detect_lattice_topology(const Structure& S, int dim, MGLevelHost& L, std::string& why) {
+    const int npe = dim == 3 ? 8 : 4;
+    const int32_t* en = S.econn.data();
+    const int64_t nn = S.n_nodes, ne = S.n_elems;
+    std::vector<int64_t> ijk(size_t(nn) * 3, INT64_MIN);
+    std::vector<std::vector<int32_t>> elems_of(static_cast<size_t>(nn));
+    for (int64_t e = 0; e < ne; e++)
+        for (int a = 0; a < npe; a++) elems_of[size_t(en[e * npe + a])].push_back(int32_t(e));
+    std::vector<uint8_t> done(size_t(ne), 0);
+    std::vector<int32_t> queue;
+    queue.reserve(size_t(ne));
+    queue.push_back(0);
+    done[0] = 1;
+    for (int a = 0; a < npe; a++)
+        for (int d = 0; d < 3; d++) ijk[size_t(en[a]) * 3 + d] = kCorner[a][d];
+    for (size_t qi = 0; qi < queue.size(); qi++) {
+        const int32_t e = queue[qi];
+        const int32_t* c = en + int64_t(e) * npe;
+        // base point from an indexed corner (every queued element has one)
+        int64_t b[3] = {0, 0, 0};
+        int known = -1;
+        for (int a = 0; a < npe && known < 0; a++)
+            if (ijk[size_t(c[a]) * 3] != INT64_MIN) known = a;
+        for (int d = 0; d < 3; d++) b[d] = ijk[size_t(c[known]) * 3 + d] - kCorner[known][d];
+        for (int a = 0; a < npe; a++) {
+            int64_t* t = &ijk[size_t(c[a]) * 3];
+            for (int d = 0; d < 3; d++) {
+                const int64_t v = b[d] + kCorner[a][d];
+                if (t[0] != INT64_MIN && t[d] != v) { why = "elements are not lattice cells in libMesh order"; return false; }
+            }
+            for (int d = 0; d < 3; d++) t[d] = b[d] + kCorner[a][d];
+            for (int32_t f : elems_of[size_t(c[a])])
+                if (!done[size_t(f)]) {
+                    done[size_t(f)] = 1;
+                    queue.push_back(f);
+                }
+        }
+    }
+    if (int64_t(queue.size()) != ne) { why = "the mesh is not connected"; return false; }
+    int64_t lo[3] = {INT64_MAX, INT64_MAX, INT64_MAX}, hi[3] = {INT64_MIN, INT64_MIN, INT64_MIN};
+    for (int64_t r = 0; r < nn; r++) {
+        if (ijk[size_t(r) * 3] == INT64_MIN) { why = "a node belongs to no element"; return false; }
+        for (int d = 0; d < 3; d++) {
+            lo[d] = std::min(lo[d], ijk[size_t(r) * 3 + d]);
+            hi[d] = std::max(hi[d], ijk[size_t(r) * 3 + d]);
+        }
+    }
+    int n[3];
+    for (int d = 0; d < 3; d++) {
+        if (hi[d] - lo[d] > 2000000000LL) { why = "lattice too large"; return false; }
+        n[d] = int(hi[d] - lo[d]);
+    }
+    if (dim == 2) n[2] = 0;
+    L.dim = dim;
+    L.npe = npe;
+    L.nav = S.nav;
+    L.nx = n[0]; L.ny = n[1]; L.nz = n[2];
+    const int64_t nlat = int64_t(n[0] + 1) * (n[1] + 1) * (n[2] + 1);
+    const int64_t ncell = int64_t(n[0]) * n[1] * (dim == 3 ? n[2] : 1);
+    if (nlat != nn || ncell != ne) { why = "the mesh does not fill its lattice"; return false; }
+    L.n_nodes = nn;
+    L.n_elems = ne;
+    L.rank_of_lat.assign(size_t(nlat), -1);
+    for (int64_t r = 0; r < nn; r++) {
+        const int64_t i = ijk[size_t(r) * 3] - lo[0], j = ijk[size_t(r) * 3 + 1] - lo[1],
+                      k = ijk[size_t(r) * 3 + 2] - lo[2];
+        const int64_t p = i + int64_t(n[0] + 1) * (j + int64_t(n[1] + 1) * k);
+        if (L.rank_of_lat[size_t(p)] >= 0) { why = "coincident lattice points"; return false; }
+        L.rank_of_lat[size_t(p)] = int32_t(r);
+    }
+    L.elem_of_cell.assign(size_t(ne), -1);
+    for (int64_t e = 0; e < ne; e++) {
+        const int32_t r0 = en[e * npe];
+        const int64_t i0 = ijk[size_t(r0) * 3] - lo[0], j0 = ijk[size_t(r0) * 3 + 1] - lo[1],
+                      k0 = ijk[size_t(r0) * 3 + 2] - lo[2];
+        const int64_t cell = i0 + int64_t(n[0]) * (j0 + int64_t(n[1]) * k0);
+        if (i0 >= n[0] || j0 >= n[1] || (dim == 3 && k0 >= n[2]) || L.elem_of_cell[size_t(cell)] >= 0) {
+            why = "overlapping elements";
+            return false;
+        }
+        L.elem_of_cell[size_t(cell)] = int32_t(e);
+    }
+    return true;
+}
+
 }  // namespace
 
 MGHierarchy build_mg_hierarchy(const Structure& S) {
@@ -77,7 +169,17 @@ MGHierarchy build_mg_hierarchy(const Structure& S) {
     else { H.why = "multigrid is built for hex8 elasticity, the quad4 Mindlin plate and hex8 heat"; return H; }
     const int nav = S.nav, npe = S.npe;
     MGLevelHost L0;
-    if (!detect_lattice(S, dim, L0, H.why)) return H;
+    if (!detect_lattice(S, dim, L0, H.why)) {
+        // displaced nodes: the lattice from the connectivity
+        std::string why2;
+        MGLevelHost T;
+        if (!detect_lattice_topology(S, dim, T, why2)) {
+            H.why = H.why + "; by connectivity: " + why2;
+            return H;
+        }
+        L0 = std::move(T);
+        H.why.clear();
+    }
     H.levels.push_back(std::move(L0));
     // geometry of the problem mesh's lattice
     double lo[3] = {1e300, 1e300, 1e300}, h[3];

@@ -99,8 +99,31 @@ Key key_of(const mastg_request& r) {
     return Key(r.kind, r.nx, r.kind == 1 ? r.ny : (r.ny > 0 ? r.ny : r.nx), r.kind == 1 ? r.nz : 0);
 }
 
-// under the lock: an idle context for the key, or room to make one
-CachedCtx* acquire_locked(mastg_server* s, Req* q, bool* need_create) {
+// takes cached context `c` out of the server's list (under the lock); the
+// caller destroys it after releasing the lock
+std::unique_ptr<CachedCtx> detach_locked(mastg_server* s, CachedCtx* c) {
+    auto it = std::find_if(s->ctxs.begin(), s->ctxs.end(),
+                           [&](const std::unique_ptr<CachedCtx>& p) { return p.get() == c; });
+    std::unique_ptr<CachedCtx> out = std::move(*it);
+    s->ctxs.erase(it);
+    s->bytes_in_use -= out->bytes;
+    return out;
+}
+
+void destroy_cached(std::vector<std::unique_ptr<CachedCtx>>& v) {
+    for (auto& c : v) {
+        if (c->ctx) mastg_ctx_destroy(c->ctx);
+        if (c->wl) mastg_workload_destroy(c->wl);
+    }
+    v.clear();
+}
+
+// under the lock: an idle context for the key, or room to make one.  Idle
+// contexts of other meshes are evicted (least recently used first) only when
+// that makes the request fit; the evicted ones go to `victims`, destroyed by
+// the caller once the lock is released.
+CachedCtx* acquire_locked(mastg_server* s, Req* q, bool* need_create,
+                          std::vector<std::unique_ptr<CachedCtx>>& victims) {
     const Key k = key_of(q->r);
     for (auto& c : s->ctxs)
         if (!c->busy && c->key == k) {
@@ -109,17 +132,18 @@ CachedCtx* acquire_locked(mastg_server* s, Req* q, bool* need_create) {
             return c.get();
         }
     const int64_t need = q->footprint;
-    // evict idle contexts (least recently used first) until the request fits
-    while (s->bytes_in_use + need > s->o.hbm_budget) {
-        CachedCtx* victim = nullptr;
+    const int64_t short_by = s->bytes_in_use + need - s->o.hbm_budget;
+    if (short_by > 0) {
+        int64_t idle = 0;
         for (auto& c : s->ctxs)
-            if (!c->busy && (!victim || c->last_use < victim->last_use)) victim = c.get();
-        if (!victim) return nullptr;
-        mastg_ctx_destroy(victim->ctx);
-        mastg_workload_destroy(victim->wl);
-        s->bytes_in_use -= victim->bytes;
-        s->ctxs.erase(std::find_if(s->ctxs.begin(), s->ctxs.end(),
-                                   [&](const std::unique_ptr<CachedCtx>& p) { return p.get() == victim; }));
+            if (!c->busy) idle += c->bytes;
+        if (idle < short_by) return nullptr;   // not admissible yet: keep the cache
+        while (s->bytes_in_use + need > s->o.hbm_budget) {
+            CachedCtx* victim = nullptr;
+            for (auto& c : s->ctxs)
+                if (!c->busy && (!victim || c->last_use < victim->last_use)) victim = c.get();
+            victims.push_back(detach_locked(s, victim));
+        }
     }
     auto c = std::make_unique<CachedCtx>();
     c->key = k;
@@ -140,9 +164,11 @@ void run_request(mastg_server* s, Req* q, CachedCtx* cc, bool create) {
         if (!rc) rc = mastg_ctx_create(mastg_workload_desc(cc->wl), &cc->ctx);
         if (!rc) {
             // fixed launch shapes: results independent of co-running work
-            mastg_set_tuning(cc->ctx, 8, 2048);
+            rc = mastg_set_tuning(cc->ctx, 8, 2048);
         }
     }
+    double objective = 0;
+    int iters = 0;
     if (!rc) {
         mastg_solve_opts so{};
         const bool plate = q->r.kind == 2;
@@ -151,30 +177,33 @@ void run_request(mastg_server* s, Req* q, CachedCtx* cc, bool create) {
         so.rtol = plate ? (s->o.rtol_plate > 0 ? s->o.rtol_plate : 1e-7) : (s->o.rtol > 0 ? s->o.rtol : 1e-10);
         mastg_design_result dr{};
         rc = mastg_design_eval(cc->ctx, q->params.data(), q->r.grad_out, MASTG_HOST, &so, &dr);
-        q->res.objective = dr.objective;
-        q->res.iters = dr.iters;
+        objective = dr.objective;
+        iters = dr.iters;
     }
     std::string msg;
     if (rc) msg = "request " + std::to_string(q->id) + ": " + mastg_last_error(cc->ctx);
-    std::lock_guard<std::mutex> lk(s->mu);
-    if (rc) s->err = msg;
-    q->t_end = Clock::now();
-    q->res.status = rc ? rc : 2;
-    q->res.t_queue = std::chrono::duration<double>(q->t_start - q->t_submit).count();
-    q->res.t_run = std::chrono::duration<double>(q->t_end - q->t_start).count();
-    if (rc && create) {
-        // a context that failed to build is dropped
-        if (cc->ctx) mastg_ctx_destroy(cc->ctx);
-        if (cc->wl) mastg_workload_destroy(cc->wl);
-        s->bytes_in_use -= cc->bytes;
-        s->ctxs.erase(std::find_if(s->ctxs.begin(), s->ctxs.end(),
-                                   [&](const std::unique_ptr<CachedCtx>& p) { return p.get() == cc; }));
-    } else {
-        cc->busy = false;
-        cc->last_use = Clock::now();
+    std::vector<std::unique_ptr<CachedCtx>> dead;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (rc) s->err = msg;
+        q->res.objective = objective;
+        q->res.iters = iters;
+        q->t_end = Clock::now();
+        q->res.status = rc ? rc : 2;
+        q->res.t_queue = std::chrono::duration<double>(q->t_start - q->t_submit).count();
+        q->res.t_run = std::chrono::duration<double>(q->t_end - q->t_start).count();
+        if (rc) {
+            // a context whose request failed (at set-up or in the evaluation)
+            // is not reused: its device state may be partial
+            dead.push_back(detach_locked(s, cc));
+        } else {
+            cc->busy = false;
+            cc->last_use = Clock::now();
+        }
+        s->cv_done.notify_all();
+        s->cv_work.notify_all();
     }
-    s->cv_done.notify_all();
-    s->cv_work.notify_all();
+    destroy_cached(dead);
 }
 
 void worker(mastg_server* s) {
@@ -182,6 +211,7 @@ void worker(mastg_server* s) {
         Req* q = nullptr;
         CachedCtx* cc = nullptr;
         bool create = false;
+        std::vector<std::unique_ptr<CachedCtx>> victims;
         {
             std::unique_lock<std::mutex> lk(s->mu);
             for (;;) {
@@ -190,7 +220,7 @@ void worker(mastg_server* s) {
                 const size_t look = std::min<size_t>(s->queue.size(), 64);
                 for (size_t i = 0; i < look && !q; i++) {
                     Req* cand = s->queue[i];
-                    CachedCtx* c = acquire_locked(s, cand, &create);
+                    CachedCtx* c = acquire_locked(s, cand, &create, victims);
                     if (c) {
                         q = cand;
                         cc = c;
@@ -203,6 +233,7 @@ void worker(mastg_server* s) {
             q->t_start = Clock::now();
             q->res.status = 1;
         }
+        destroy_cached(victims);   // evicted contexts, outside the lock
         run_request(s, q, cc, create);
     }
 }
@@ -288,8 +319,11 @@ int mastg_server_stats(mastg_server* s, int64_t* bytes_in_use, int32_t* n_contex
 
 const char* mastg_server_last_error(mastg_server* s) {
     if (!s) return "";
+    // a copy per calling thread: the server's message may change under another worker
+    thread_local std::string copy;
     std::lock_guard<std::mutex> lk(s->mu);
-    return s->err.c_str();
+    copy = s->err;
+    return copy.c_str();
 }
 
 int mastg_server_destroy(mastg_server* s) {

@@ -15,6 +15,9 @@
 //   MAST::NonlinearSystem::solve / adjoint_solve / sensitivity_solve
 //     (src/base/nonlinear_system.h:133-160)
 //   MAST::ComplianceOutput (src/elasticity/compliance_output.h)
+//   MAST::KSStressStrainOutput (src/elasticity/ks_stress_output.h)
+//   MAST::NastranIO::read (src/mesh/nastran_io.h)
+//   MAST::FilterBase (src/level_set/filter_base.h) and example_6's volume
 // Element work is batched on the GPU: the per-element virtual calls of the
 // reference become one device pass over all elements; vectors and the Jacobian
 // live in the caller's DofMap numbering (SerialVector / CsrMatrix below stand
@@ -156,6 +159,23 @@ public:
               "calculate_output_adjoint_sensitivity");
     }
 
+    // sensitivity_assemble(p, rhs) for a parameter every element depends on
+    // (MASTG_PARAM_*): rhs = dR/dp at X
+    void sensitivity_assemble(int param, const RealVector& X, RealVector& dRdp) {
+        ready();
+        dRdp.assign(_sizes.n_dofs, 0.0);
+        check(mastg_sensitivity_rhs(_ctx, param, X.data(), dRdp.data(), MASTG_HOST), _ctx, "sensitivity_assemble");
+    }
+
+    // calculate_output_adjoint_sensitivity for one global parameter: adj . dR/dp
+    double adjoint_sensitivity(int param, const RealVector& X, const RealVector& adj) {
+        ready();
+        double v = 0;
+        check(mastg_sensitivity_global(_ctx, param, X.data(), adj.data(), MASTG_HOST, &v), _ctx,
+              "calculate_output_adjoint_sensitivity");
+        return v;
+    }
+
     int64_t n_elems() const { return _n_elems; }
 
 private:
@@ -185,6 +205,88 @@ public:
         check(mastg_output(a.context(), MASTG_OUTPUT_COMPLIANCE, X.data(), MASTG_HOST, &v, dq_dX.data()),
               a.context(), "ComplianceOutput::output_derivative");
     }
+};
+
+// KS aggregate of the von Mises stress (hex8): output_total is the value,
+// output_derivative dKS/dX, output_sensitivity the direct dKS/drho_e
+class KSStressStrainOutput {
+public:
+    KSStressStrainOutput(double p_norm = 10.0, double sigma0 = 1.0) : _p(p_norm), _s0(sigma0) {}
+    void set_participating_elements_to_all() {}
+    double evaluate(NonlinearImplicitAssembly& a, const RealVector& X) {
+        check(mastg_set_ks_stress(a.context(), _p, _s0), a.context(), "KSStressStrainOutput");
+        double v = 0;
+        check(mastg_output(a.context(), MASTG_OUTPUT_KS_STRESS, X.data(), MASTG_HOST, &v, nullptr), a.context(),
+              "KSStressStrainOutput::evaluate");
+        return v;
+    }
+    void output_derivative(NonlinearImplicitAssembly& a, const RealVector& X, RealVector& dq_dX) {
+        check(mastg_set_ks_stress(a.context(), _p, _s0), a.context(), "KSStressStrainOutput");
+        dq_dX.assign(a.sizes().n_dofs, 0.0);
+        double v = 0;
+        check(mastg_output(a.context(), MASTG_OUTPUT_KS_STRESS, X.data(), MASTG_HOST, &v, dq_dX.data()),
+              a.context(), "KSStressStrainOutput::output_derivative");
+    }
+    void output_sensitivity(NonlinearImplicitAssembly& a, const RealVector& X, RealVector& dq_dp) {
+        check(mastg_set_ks_stress(a.context(), _p, _s0), a.context(), "KSStressStrainOutput");
+        dq_dp.assign(size_t(a.n_elems()), 0.0);
+        check(mastg_output_sensitivity(a.context(), MASTG_OUTPUT_KS_STRESS, X.data(), MASTG_HOST, dq_dp.data()),
+              a.context(), "KSStressStrainOutput::output_sensitivity");
+    }
+private:
+    double _p, _s0;
+};
+
+// ---- mesh input -------------------------------------------------------------
+// NastranIO::read: the model's mesh, DofMap, SPCs, loads and properties as a
+// SystemDescription (owned by this object) and the element parameters
+class NastranIO {
+public:
+    NastranIO() = default;
+    NastranIO(const NastranIO&) = delete;
+    ~NastranIO() { if (_m) mastg_nastran_destroy(_m); }
+    void read(const std::string& filename) {
+        if (_m) mastg_nastran_destroy(_m);
+        _m = nullptr;
+        if (mastg_nastran_read(filename.c_str(), &_m) != MASTG_OK)
+            throw std::runtime_error(std::string("NastranIO::read: ") + mastg_nastran_last_error());
+    }
+    SystemDescription system() const { return SystemDescription(*mastg_nastran_desc(_m)); }
+    RealVector element_parameters() const {
+        const mastg_problem_desc* d = mastg_nastran_desc(_m);
+        const double* p = mastg_nastran_params(_m);
+        return RealVector(p, p + d->n_elems);
+    }
+private:
+    mastg_mesh_file* _m = nullptr;
+};
+
+// ---- design space ---------------------------------------------------------
+// FilterBase::compute_filtered_values over element centroids, and the volume
+// of example_6 (element densities)
+class FilterBase {
+public:
+    FilterBase(const SystemDescription& sys, double radius) {
+        if (mastg_design_create(&sys.desc(), radius, &_d) != MASTG_OK)
+            throw std::runtime_error("FilterBase: mastg_design_create failed");
+    }
+    FilterBase(const FilterBase&) = delete;
+    ~FilterBase() { mastg_design_destroy(_d); }
+    void compute_filtered_values(const RealVector& input, RealVector& output, bool transpose = false) const {
+        output.assign(input.size(), 0.0);
+        if (mastg_design_filter(_d, input.data(), output.data(), transpose ? 1 : 0, MASTG_HOST) != MASTG_OK)
+            throw std::runtime_error(std::string("FilterBase: ") + mastg_design_last_error(_d));
+    }
+    double volume(const RealVector& rho, RealVector* grad) const {
+        double v = 0;
+        if (grad) grad->assign(rho.size(), 0.0);
+        if (mastg_design_volume(_d, rho.data(), MASTG_HOST, &v, grad ? grad->data() : nullptr) != MASTG_OK)
+            throw std::runtime_error(std::string("FilterBase volume: ") + mastg_design_last_error(_d));
+        return v;
+    }
+    mastg_design* handle() const { return _d; }
+private:
+    mastg_design* _d = nullptr;
 };
 
 // ---- system ---------------------------------------------------------------
@@ -218,8 +320,16 @@ public:
         last = st;
     }
 
-    // sensitivity_solve for one element parameter is not provided: the batched
-    // adjoint route above gives every element's sensitivity in one pass
+    // sensitivity_solve(X, p): J dX/dp = -dR/dp for a global parameter with the
+    // Jacobian assembled at X (element parameters take the batched adjoint
+    // route of the assembly, every element's sensitivity in one pass)
+    void sensitivity_solve(const RealVector& X, int param, RealVector& dXdp, const mastg_solve_opts* opts = nullptr) {
+        dXdp.assign(X.size(), 0.0);
+        mastg_solve_stats st{};
+        check(mastg_sensitivity_solve(_a.context(), param, X.data(), dXdp.data(), MASTG_HOST, opts, &st),
+              _a.context(), "NonlinearSystem::sensitivity_solve");
+        last = st;
+    }
     mastg_solve_stats last{};
 
 private:

@@ -15,7 +15,8 @@ HEX8_ELASTIC, QUAD4_MINDLIN, HEX8_HEAT = 1, 2, 3
 HOST, DEVICE = 0, 1
 ELEM_FP64, ELEM_FP32 = 0, 1
 SOLVER_CG_JACOBI, SOLVER_MIXED, SOLVER_BICGSTAB, SOLVER_MATFREE, SOLVER_MG = 0, 1, 2, 3, 4
-OUTPUT_COMPLIANCE, OUTPUT_MEAN_T = 1, 2
+OUTPUT_COMPLIANCE, OUTPUT_MEAN_T, OUTPUT_KS_STRESS = 1, 2, 3
+PARAM_E, PARAM_NU, PARAM_PRESSURE, PARAM_K0, PARAM_CONV_H, PARAM_EMISSIVITY = 1, 2, 3, 4, 5, 6
 OPT_GRAPHS = 1
 OPT_MG_STEPS_FINE = 2
 OPT_MG_STEPS_COARSE = 3
@@ -37,6 +38,8 @@ class ProblemDesc(C.Structure):
         ("kappa", C.c_double), ("offset", C.c_double), ("pressure", C.c_double),
         ("k0", C.c_double), ("alpha", C.c_double), ("conv_h", C.c_double),
         ("conv_tinf", C.c_double), ("conv_bid_mask", C.c_uint32), ("flags", C.c_int32),
+        ("rad_emissivity", C.c_double), ("rad_tamb", C.c_double), ("rad_tzero", C.c_double),
+        ("rad_sigma", C.c_double), ("rad_bid_mask", C.c_uint32),
     ]
 
 
@@ -104,6 +107,15 @@ def lib():
             "mastg_sensitivity": (C.c_int, [_P, _D, _D, _D, C.c_int]),
             "mastg_jacobian_product": (C.c_int, [_P, _D, _D, C.c_int, C.c_int]),
             "mastg_output": (C.c_int, [_P, C.c_int, _D, C.c_int, _D, _D]),
+            "mastg_set_ks_stress": (C.c_int, [_P, C.c_double, C.c_double]),
+            "mastg_output_sensitivity": (C.c_int, [_P, C.c_int, _D, C.c_int, _D]),
+            "mastg_device_alloc": (C.c_int, [_P, C.c_size_t, C.POINTER(C.c_void_p)]),
+            "mastg_device_free": (C.c_int, [_P, C.c_void_p]),
+            "mastg_copy": (C.c_int, [_P, C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]),
+            "mastg_sensitivity_rhs": (C.c_int, [_P, C.c_int, _D, _D, C.c_int]),
+            "mastg_sensitivity_global": (C.c_int, [_P, C.c_int, _D, _D, C.c_int, _D]),
+            "mastg_sensitivity_solve": (C.c_int, [_P, C.c_int, _D, _D, C.c_int, C.POINTER(SolveOpts),
+                                                  C.POINTER(SolveStats)]),
             "mastg_design_eval": (C.c_int, [_P, _D, _D, C.c_int, C.POINTER(SolveOpts),
                                             C.POINTER(DesignResult)]),
             "mastg_enable_kernel_timing": (C.c_int, [_P, C.c_int]),
@@ -127,6 +139,22 @@ def lib():
             "mastg_workload_desc": (C.POINTER(ProblemDesc), [_P]),
             "mastg_workload_params": (_D, [_P]),
             "mastg_workload_destroy": (C.c_int, [_P]),
+            "mastg_nastran_read": (C.c_int, [C.c_char_p, C.POINTER(_P)]),
+            "mastg_nastran_last_error": (C.c_char_p, []),
+            "mastg_nastran_desc": (C.POINTER(ProblemDesc), [_P]),
+            "mastg_nastran_params": (_D, [_P]),
+            "mastg_nastran_ids": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                            C.POINTER(C.c_int32)]),
+            "mastg_nastran_spc_sets": (C.c_int, [_P]),
+            "mastg_nastran_destroy": (C.c_int, [_P]),
+            "mastg_design_create": (C.c_int, [C.POINTER(ProblemDesc), C.c_double, C.POINTER(_P)]),
+            "mastg_design_destroy": (C.c_int, [_P]),
+            "mastg_design_last_error": (C.c_char_p, [_P]),
+            "mastg_design_info": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), _D]),
+            "mastg_design_filter": (C.c_int, [_P, _D, _D, C.c_int, C.c_int]),
+            "mastg_design_volume": (C.c_int, [_P, _D, C.c_int, _D, _D]),
+            "mastg_design_oc_update": (C.c_int, [_P, _D, _D, _D, C.c_double, C.c_double, C.c_double,
+                                                 _D, C.c_int, _D]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -195,6 +223,7 @@ class Workload:
         if rc:
             raise MastgError(rc, "workload_create")
         self.h = h
+        self.nx, self.ny, self.nz = nx, ny, nz
         self.desc = lib().mastg_workload_desc(h)
         self.n_elems = self.desc.contents.n_elems
         self.params = np.ctypeslib.as_array(lib().mastg_workload_params(h), (self.n_elems,)).copy()
@@ -203,6 +232,44 @@ class Workload:
         if getattr(self, "h", None) and _lib is not None:
             _lib.mastg_workload_destroy(self.h)
             self.h = None
+
+
+class NastranModel:
+    """A Nastran bulk-data model read by the library (mesh, DofMap, SPCs,
+    FORCE loads, MAT1, PSHELL thicknesses)."""
+
+    def __init__(self, path):
+        h = _P()
+        rc = lib().mastg_nastran_read(os.fsencode(path), C.byref(h))
+        if rc:
+            raise MastgError(rc, lib().mastg_nastran_last_error().decode())
+        self.h = h
+        self.desc = lib().mastg_nastran_desc(h)
+        d = self.desc.contents
+        self.n_nodes, self.n_elems = d.n_nodes, d.n_elems
+        self.npe = 4 if d.physics == QUAD4_MINDLIN else 8
+        self.params = np.ctypeslib.as_array(lib().mastg_nastran_params(h), (self.n_elems,)).copy()
+        self.xyz = np.ctypeslib.as_array(d.xyz, (self.n_nodes * 3,)).reshape(-1, 3).copy()
+        self.conn = np.ctypeslib.as_array(d.conn, (self.n_elems * self.npe,)).reshape(-1, self.npe).copy()
+        self.node_base = np.ctypeslib.as_array(d.node_dof_base, (self.n_nodes,)).copy()
+        self.n_dofs = int(self.node_base.max()) + d.n_vars if self.n_nodes else 0
+        self.constrained = (np.ctypeslib.as_array(d.constrained_dofs, (d.n_constrained,)).copy()
+                            if d.n_constrained else np.zeros(0, np.int64))
+        self.point_loads = np.ctypeslib.as_array(d.point_loads, (self.n_dofs,)).copy()
+        nid = np.zeros(self.n_nodes, np.int64)
+        eid = np.zeros(self.n_elems, np.int64)
+        pid = np.zeros(self.n_elems, np.int32)
+        lib().mastg_nastran_ids(h, nid.ctypes.data_as(C.POINTER(C.c_int64)),
+                                eid.ctypes.data_as(C.POINTER(C.c_int64)), pid.ctypes.data_as(C.POINTER(C.c_int32)))
+        self.node_ids, self.elem_ids, self.elem_pids = nid, eid, pid
+        self.spc_sets = lib().mastg_nastran_spc_sets(h)
+
+    def close(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.mastg_nastran_destroy(self.h)
+            self.h = None
+
+    __del__ = close
 
 
 class Context:
@@ -278,6 +345,40 @@ class Context:
         self._chk(lib().mastg_sensitivity(self.h, _p(X), _p(lam), _p(g), HOST))
         return g
 
+    def sensitivity_rhs(self, param, X):
+        """dR/dp at X for a global parameter (MASTG_PARAM_*)."""
+        out = np.zeros(self.n_dofs)
+        X = np.ascontiguousarray(X, np.float64)
+        self._chk(lib().mastg_sensitivity_rhs(self.h, param, _p(X), _p(out), HOST))
+        return out
+
+    def sensitivity_global(self, param, X, lam):
+        """lambda . dR/dp for a global parameter."""
+        v = C.c_double()
+        X = np.ascontiguousarray(X, np.float64)
+        lam = np.ascontiguousarray(lam, np.float64)
+        self._chk(lib().mastg_sensitivity_global(self.h, param, _p(X), _p(lam), HOST, C.byref(v)))
+        return v.value
+
+    def sensitivity_solve(self, param, X, solver=SOLVER_CG_JACOBI, rtol=1e-10, max_iters=0):
+        """dX/dp from J dX/dp = -dR/dp (the Jacobian of the last assembly)."""
+        out = np.zeros(self.n_dofs)
+        X = np.ascontiguousarray(X, np.float64)
+        o = SolveOpts(solver, max_iters, rtol, 0.0, 0, 0)
+        st = SolveStats()
+        self._chk(lib().mastg_sensitivity_solve(self.h, param, _p(X), _p(out), HOST, C.byref(o), C.byref(st)))
+        return out, dict(iters=st.iters, rel_res=st.rel_res)
+
+    def set_ks_stress(self, k, sigma_ref):
+        self._chk(lib().mastg_set_ks_stress(self.h, k, sigma_ref))
+
+    def output_sensitivity(self, kind, X):
+        """Direct dJ/dp_e of an output at fixed state X."""
+        X = np.ascontiguousarray(X, np.float64)
+        g = np.zeros(self.n_elems)
+        self._chk(lib().mastg_output_sensitivity(self.h, kind, _p(X), HOST, _p(g)))
+        return g
+
     def output(self, kind, X, want_grad=False):
         v = C.c_double()
         X = np.ascontiguousarray(X, np.float64)
@@ -322,6 +423,55 @@ class Context:
         n = C.c_int64()
         self._chk(lib().mastg_kernel_time(self.h, kind, C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+
+class Design:
+    """Element-wise design space: density filter, volume, optimality-criteria step."""
+
+    def __init__(self, desc, radius):
+        h = _P()
+        dp = desc if isinstance(desc, C.POINTER(ProblemDesc)) else C.byref(desc)
+        rc = lib().mastg_design_create(dp, radius, C.byref(h))
+        if rc:
+            raise MastgError(rc, "design_create")
+        self.h = h
+        n, nnz, vt = C.c_int64(), C.c_int64(), C.c_double()
+        lib().mastg_design_info(h, C.byref(n), C.byref(nnz), C.byref(vt))
+        self.n, self.nnz, self.v_total = n.value, nnz.value, vt.value
+
+    def close(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.mastg_design_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def _chk(self, rc):
+        if rc:
+            raise MastgError(rc, lib().mastg_design_last_error(self.h).decode())
+
+    def filter(self, x, transpose=False):
+        x = np.ascontiguousarray(x, np.float64)
+        out = np.zeros(self.n)
+        self._chk(lib().mastg_design_filter(self.h, _p(x), _p(out), int(transpose), HOST))
+        return out
+
+    def volume(self, rho, want_grad=False):
+        rho = np.ascontiguousarray(rho, np.float64)
+        v = C.c_double()
+        dv = np.zeros(self.n) if want_grad else None
+        self._chk(lib().mastg_design_volume(self.h, _p(rho), HOST, C.byref(v), _p(dv)))
+        return (v.value, dv) if want_grad else v.value
+
+    def oc_update(self, rho, dc, dv, volfrac, move=0.2, rho_min=1e-3):
+        rho = np.ascontiguousarray(rho, np.float64)
+        dc = np.ascontiguousarray(dc, np.float64)
+        dv = np.ascontiguousarray(dv, np.float64)
+        out = np.zeros(self.n)
+        lam = C.c_double()
+        self._chk(lib().mastg_design_oc_update(self.h, _p(rho), _p(dc), _p(dv), volfrac, move, rho_min,
+                                               _p(out), HOST, C.byref(lam)))
+        return out, lam.value
 
 
 class Server:

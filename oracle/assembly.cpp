@@ -101,14 +101,22 @@ static void element_calc(const Problem& P, int64_t e, const double* sol, double*
             const size_t s = size_t(it - m.bs_elem.begin());
             if ((P.conv_bid_mask >> m.bs_id[s]) & 1u)
                 hex8_convection(X, m.bs_side[s], P.conv_h, P.conv_Tinf, sol, f, K);
+            if ((P.rad_bid_mask >> m.bs_id[s]) & 1u)
+                hex8_radiation(X, m.bs_side[s], P.rad_sigma, P.rad_emiss, P.rad_Tamb, P.rad_T0, sol, f, K);
         }
         hex8_source(X, P.param[e], f);
     }
 }
 
 static void constrain(const Problem& P, const int64_t* dofs, int nd, double* f, double* K) {
-    // libMesh DofMap::constrain_element_matrix_and_vector for homogeneous
-    // Dirichlet rows: K <- C^T K C (zero rows and columns), K(i,i) = 1, f(i) = 0.
+    // libMesh DofMap::constrain_element_matrix_and_vector (libMesh 1.5.1,
+    // src/base/dof_map_constraints.C; called by MAST per element at
+    // src/base/nonlinear_implicit_assembly.cpp:174-183).  libMesh builds the
+    // element constraint matrix C (build_constraint_matrix): for a homogeneous
+    // Dirichlet DOF its row is zero, so K <- C^T K C zeroes that row and
+    // column and F <- C^T F zeroes the entry; it then sets K(i,i) = 1 for every
+    // constrained element DOF i.  The global diagonal of a constrained DOF is
+    // therefore the number of elements that contain it.
     for (int i = 0; i < nd; i++) {
         if (!P.dof.constrained[dofs[i]]) continue;
         if (f) f[i] = 0.0;

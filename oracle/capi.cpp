@@ -33,12 +33,39 @@ void* oracle_problem_new(int kind, int nx, int ny, int nz, uint64_t seed, double
     else h->P = make_heat(nx, seed);
     return h;
 }
+// A problem over caller arrays (any mesh and DofMap): kind 1 hex8 elasticity,
+// 2 Mindlin plate, 3 heat; constrained: [n_dofs] 0/1; point_load [n_dofs] or
+// null; material scalars as the structured builders' defaults, changed with
+// oracle_set_scalar.  The plate pressure starts at 0.
+void* oracle_problem_from_arrays(int kind, int64_t n_nodes, const double* xyz, int64_t n_elems,
+                                 const int32_t* conn, int n_vars, const int64_t* node_base, int64_t n_dofs,
+                                 const uint8_t* constrained, const double* params, const double* point_load) {
+    auto* h = new Handle;
+    Problem& P = h->P;
+    P.phys = kind == 1 ? HEX8_ELASTIC : kind == 2 ? QUAD4_MINDLIN : HEX8_HEAT;
+    P.mesh.dim = kind == 2 ? 2 : 3;
+    P.mesh.npe = kind == 2 ? 4 : 8;
+    P.mesh.n_nodes = n_nodes;
+    P.mesh.n_elems = n_elems;
+    P.mesh.xyz.assign(xyz, xyz + n_nodes * 3);
+    P.mesh.conn.assign(conn, conn + n_elems * P.mesh.npe);
+    P.dof.n_vars = n_vars;
+    P.dof.n_dofs = n_dofs;
+    P.dof.node_base.assign(node_base, node_base + n_nodes);
+    P.dof.constrained.assign(constrained, constrained + n_dofs);
+    P.param.assign(params, params + n_elems);
+    if (point_load) P.point_load.assign(point_load, point_load + n_dofs);
+    if (kind == 2) { P.E = 70e9; P.nu = 0.3; P.pressure = 0.0; }
+    P.active_only = (kind == 1 && n_vars == 3) || (kind == 2 && n_vars == 5);
+    return h;
+}
 void oracle_problem_free(void* h) { delete H(h); }
 void oracle_set_threads(void* h, int t) { H(h)->threads = t; }
 int oracle_max_threads() { return omp_get_max_threads(); }
 
 // set a scalar of the problem: E nu penalty kappa off k0 alpha conv_h conv_Tinf
-// conv_mask pressure; returns 0 on success, -1 for an unknown name
+// conv_mask pressure rad_sigma rad_emiss rad_Tamb rad_T0 rad_mask; returns 0 on
+// success, -1 for an unknown name
 int oracle_set_scalar(void* h, const char* name, double v) {
     Problem& P = H(h)->P;
     const std::string n(name);
@@ -53,6 +80,11 @@ int oracle_set_scalar(void* h, const char* name, double v) {
     else if (n == "conv_Tinf") P.conv_Tinf = v;
     else if (n == "conv_mask") P.conv_bid_mask = uint32_t(v);
     else if (n == "pressure") P.pressure = v;
+    else if (n == "rad_sigma") P.rad_sigma = v;
+    else if (n == "rad_emiss") P.rad_emiss = v;
+    else if (n == "rad_Tamb") P.rad_Tamb = v;
+    else if (n == "rad_T0") P.rad_T0 = v;
+    else if (n == "rad_mask") P.rad_bid_mask = uint32_t(v);
     else return -1;
     return 0;
 }
@@ -130,7 +162,7 @@ void oracle_spmv(void* h, const double* x, double* y) { spmv(H(h)->csr(), x, y, 
 
 int oracle_solve_elastic(void* h, double rtol, int max_it, double* u, double* C, double* dc,
                          double* rel_res) {
-    ElasticResult r = solve_elastic(H(h)->P, rtol, max_it, H(h)->threads);
+    ElasticResult r = solve_elastic(H(h)->P, rtol, max_it, H(h)->threads, &H(h)->csr());
     if (u) std::memcpy(u, r.u.data(), r.u.size() * sizeof(double));
     if (C) *C = r.compliance;
     if (dc) std::memcpy(dc, r.dc.data(), r.dc.size() * sizeof(double));
@@ -165,6 +197,16 @@ void oracle_section(double E, double nu, double h, double off, double kappa, dou
     material_G(E, nu, G);
     section_A(Dps, h, A); section_B(Dps, h, off, B); section_Db(Dps, h, off, Db);
     section_S(G, h, kappa, S);
+}
+// stresses [8][6] and JxW [8] of a hex8 at E, nu for the 24 translations u
+void oracle_hex8_stress(const double* X, double E, double nu, const double* u, double* sigma, double* jxw) {
+    double D[36];
+    material_D3(E, nu, D);
+    hex8_stress(reinterpret_cast<const double(*)[3]>(X), D, u, reinterpret_cast<double(*)[6]>(sigma), jxw);
+}
+void oracle_hex8_radiation(const double* X, int side, double sigma, double emiss, double Tamb,
+                           double T0, const double* T, double* f, double* K) {
+    hex8_radiation(reinterpret_cast<const double(*)[3]>(X), side, sigma, emiss, Tamb, T0, T, f, K);
 }
 // X: 8x3, u: 48, f: 48 (+=), K: 48x48 (+=, may be null)
 void oracle_hex8_elastic(const double* X, double E, double nu, const double* u, double* f,

@@ -237,6 +237,30 @@ void hex8_elastic(const double X[8][3], const double D[36], const double* u, dou
         for (int a = 0; a < 48 * 48; a++) K[a] += KK[a];
 }
 
+// StructuralElement3D::calculate_stress (solid_element_3d.cpp:1147-1236), linear
+// strain: at every quadrature point (libMesh QGauss order, xi fastest) the
+// strain B u and the stress D B u; JxW beside it.  u: the 24 translations.
+void hex8_stress(const double X[8][3], const double D[36], const double* u, double sigma[8][6],
+                 double jxw[8]) {
+    int q = 0;
+    for (int k = 0; k < 2; k++)
+        for (int j = 0; j < 2; j++)
+            for (int i = 0; i < 2; i++, q++) {
+                double N[8], dN[8][3], g[8][3], B[6][24];
+                hex8_shape(GP2[i], GP2[j], GP2[k], N, dN);
+                jxw[q] = hex8_map(X, dN, g) * 1.0;
+                hex8_B(g, B);
+                double strain[6] = {0};
+                for (int r = 0; r < 6; r++)
+                    for (int c = 0; c < 24; c++) strain[r] += B[r][c] * u[c];
+                for (int r = 0; r < 6; r++) {
+                    double v = 0;
+                    for (int c = 0; c < 6; c++) v += D[r * 6 + c] * strain[c];
+                    sigma[q][r] = v;
+                }
+            }
+}
+
 void hex8_elastic_sens(const double X[8][3], const double dD[36], const double* u, double* df) {
     double ff[48] = {0};
     hex8_elastic_impl<false>(X, dD, u, ff, nullptr, 48);
@@ -457,6 +481,31 @@ void hex8_convection(const double X[8][3], int side, double h, double Tinf, cons
             if (K)
                 for (int a = 0; a < 8; a++)
                     for (int b = 0; b < 8; b++) K[a * 8 + b] += JxW * (N[a] * N[b]) * th * h;
+        }
+}
+
+// surface_radiation_residual on side s (heat_conduction_elem_base.cpp:1402-1463):
+// f += int N sigma eps th ((T + T0)^4 - (Ta + T0)^4),
+// K += int N N^T sigma eps th 4 (T + T0)^3, th = 1 (a 3D element has no section,
+// src/property_cards/isotropic_element_property_card_3D.h:164-167)
+void hex8_radiation(const double X[8][3], int side, double sigma, double emiss, double Tamb,
+                    double T0, const double* T, double* f, double* K) {
+    for (int j = 0; j < 2; j++)
+        for (int i = 0; i < 2; i++) {
+            double xi[3], N[8], dN[8][3];
+            hex8_side_point(side, GP2[i], GP2[j], xi);
+            hex8_shape(xi[0], xi[1], xi[2], N, dN);
+            const double JxW = hex8_side_jxw(X, side, dN) * 1.0;
+            double temp = 0;
+            for (int a = 0; a < 8; a++) temp += N[a] * T[a];
+            const double th = 1.0;
+            const double flux = sigma * emiss * th * (std::pow(temp + T0, 4.) - std::pow(Tamb + T0, 4.));
+            for (int a = 0; a < 8; a++) f[a] += JxW * N[a] * flux;
+            if (K) {
+                const double dflux = sigma * emiss * th * 4. * std::pow(temp + T0, 3.);
+                for (int a = 0; a < 8; a++)
+                    for (int b = 0; b < 8; b++) K[a * 8 + b] += JxW * (N[a] * N[b]) * dflux;
+            }
         }
 }
 

@@ -110,6 +110,10 @@ void section_S(const double G[4], double h, double kappa, double S[4]);   // :10
 // u: 48 (var-major ux,uy,uz,tx,ty,tz); f: 48; K: 48x48 row-major (may be null).
 void hex8_elastic(const double X[8][3], const double D[36], const double* u,
                   double* f, double* K);
+// calculate_stress (solid_element_3d.cpp:1147-1236): stress D B u at the 8
+// quadrature points (xi fastest) and their JxW; u: 24 translations
+void hex8_stress(const double X[8][3], const double D[36], const double* u, double sigma[8][6],
+                 double jxw[8]);
 // internal_residual_sensitivity (solid_element_3d.cpp:566-788): f' = sum B^T D' B u
 void hex8_elastic_sens(const double X[8][3], const double dD[36], const double* u,
                        double* df);
@@ -136,6 +140,9 @@ void hex8_heat(const double X[8][3], double k0, double alpha, const double* T,
 // surface_convection_residual (:1099-1154) on side s
 void hex8_convection(const double X[8][3], int side, double h, double Tinf,
                      const double* T, double* f, double* K);
+// surface_radiation_residual (:1402-1463) on side s: flux sigma eps ((T+T0)^4 - (Ta+T0)^4)
+void hex8_radiation(const double X[8][3], int side, double sigma, double emiss, double Tamb,
+                    double T0, const double* T, double* f, double* K);
 // volume_heat_source_residual (:1709-1745): f -= int N q ; its d/dq = -int N
 void hex8_source(const double X[8][3], double q, double* f);
 
@@ -154,6 +161,9 @@ struct Problem {
     double k0 = 200.0, alpha = 1e-3;             // heat
     double conv_h = 50.0, conv_Tinf = 300.0;
     uint32_t conv_bid_mask = 0;
+    // surface radiation on the sides whose boundary id is in rad_bid_mask
+    double rad_sigma = 5.670374419e-8, rad_emiss = 0.0, rad_Tamb = 0.0, rad_T0 = 0.0;
+    uint32_t rad_bid_mask = 0;
     double pressure = 0.0;                        // plate
     std::vector<double> param;                    // rho_e / h_e / q_e
     std::vector<double> point_load;               // [n_dofs] nodal loads F (R -= F)
@@ -199,7 +209,9 @@ struct ElasticResult {
 // Static solve + compliance + adjoint sensitivity (compliance is self-adjoint,
 // lambda = u; examples/structural/example_6/example_6.cpp:877-939,
 // src/elasticity/compliance_output.cpp:71-152).
-ElasticResult solve_elastic(const Problem& P, double rtol, int max_it, int n_threads);
+// `pattern`: the mesh's sparsity (build_sparsity), or null to build it here.
+ElasticResult solve_elastic(const Problem& P, double rtol, int max_it, int n_threads,
+                            const CSR* pattern = nullptr);
 
 struct HeatResult {
     std::vector<double> T;

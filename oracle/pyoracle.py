@@ -29,6 +29,8 @@ def lib():
             "oracle_problem_new": (_P, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64,
                                         C.c_double, C.c_double, C.c_int]),
             "oracle_problem_free": (None, [_P]),
+            "oracle_problem_from_arrays": (_P, [C.c_int, C.c_int64, _D, C.c_int64, _I32, C.c_int, _I64,
+                                                C.c_int64, _U8, _D, _D]),
             "oracle_set_threads": (None, [_P, C.c_int]),
             "oracle_max_threads": (C.c_int, []),
             "oracle_set_scalar": (C.c_int, [_P, C.c_char_p, C.c_double]),
@@ -64,6 +66,8 @@ def lib():
             "oracle_hex8_heat": (None, [_D, C.c_double, C.c_double, _D, _D, _D]),
             "oracle_hex8_convection": (None, [_D, C.c_int, C.c_double, C.c_double, _D, _D, _D]),
             "oracle_uniform": (None, [C.c_uint64, C.c_int64, C.c_double, C.c_double, _D]),
+            "oracle_hex8_radiation": (None, [_D, C.c_int] + [C.c_double] * 4 + [_D, _D, _D]),
+            "oracle_hex8_stress": (None, [_D, C.c_double, C.c_double, _D, _D, _D]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -85,10 +89,25 @@ CANTILEVER, PLATE, HEAT = 1, 2, 3
 class Problem:
     """A workload built by the oracle (mesh, DofMap, loads, seeded parameters)."""
 
-    def __init__(self, kind, nx, ny=0, nz=0, seed=1, lo=0.0, hi=1.0, mast_exact=False,
-                 threads=0):
+    def __init__(self, kind, nx=0, ny=0, nz=0, seed=1, lo=0.0, hi=1.0, mast_exact=False,
+                 threads=0, arrays=None):
+        """A structured workload, or (arrays = dict(xyz, conn, n_vars, node_base,
+        constrained, params[, point_load])) a problem over any mesh and DofMap."""
         self.kind = kind
-        self.h = lib().oracle_problem_new(kind, nx, ny, nz, seed, lo, hi, int(mast_exact))
+        if arrays is not None:
+            a = {k: v for k, v in arrays.items()}
+            xyz = np.ascontiguousarray(a["xyz"], np.float64)
+            conn = np.ascontiguousarray(a["conn"], np.int32)
+            base = np.ascontiguousarray(a["node_base"], np.int64)
+            con = np.ascontiguousarray(a["constrained"], np.uint8)
+            par = np.ascontiguousarray(a["params"], np.float64)
+            pl = a.get("point_load")
+            pl = None if pl is None else np.ascontiguousarray(pl, np.float64)
+            self.h = lib().oracle_problem_from_arrays(kind, xyz.shape[0], ptr(xyz), conn.shape[0], ptr(conn, _I32),
+                                                      int(a["n_vars"]), ptr(base, _I64), con.size, ptr(con, _U8),
+                                                      ptr(par), ptr(pl))
+        else:
+            self.h = lib().oracle_problem_new(kind, nx, ny, nz, seed, lo, hi, int(mast_exact))
         if threads:
             lib().oracle_set_threads(self.h, threads)
         L = lib()
@@ -279,3 +298,22 @@ def hex8_convection(X, side, h, Tinf, T):
     K = np.zeros((8, 8))
     lib().oracle_hex8_convection(ptr(X), side, h, Tinf, ptr(T), ptr(f), ptr(K))
     return f, K
+
+
+def hex8_radiation(X, side, sigma, emiss, Tamb, T0, T):
+    X = np.ascontiguousarray(X, np.float64).reshape(8, 3)
+    T = np.ascontiguousarray(T, np.float64)
+    f = np.zeros(8)
+    K = np.zeros((8, 8))
+    lib().oracle_hex8_radiation(ptr(X), side, sigma, emiss, Tamb, T0, ptr(T), ptr(f), ptr(K))
+    return f, K
+
+
+def hex8_stress(X, E, nu, u):
+    """Stresses [8, 6] (xx yy zz xy yz zx) and JxW [8] at the quadrature points."""
+    X = np.ascontiguousarray(X, np.float64).reshape(8, 3)
+    u = np.ascontiguousarray(u, np.float64)
+    sig = np.zeros((8, 6))
+    jxw = np.zeros(8)
+    lib().oracle_hex8_stress(ptr(X), E, nu, ptr(u), ptr(sig), ptr(jxw))
+    return sig, jxw

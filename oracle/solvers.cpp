@@ -133,10 +133,12 @@ CSR transpose(const CSR& A) {
     return T;
 }
 
-ElasticResult solve_elastic(const Problem& P, double rtol, int max_it, int n_threads) {
+ElasticResult solve_elastic(const Problem& P, double rtol, int max_it, int n_threads, const CSR* pattern) {
     ElasticResult res;
     const int64_t n = P.dof.n_dofs;
-    CSR K = build_sparsity(P.mesh, P.dof);
+    // the sparsity pattern is set up once per mesh (NonlinearSystem::init_data)
+    CSR K = pattern ? *pattern : build_sparsity(P.mesh, P.dof);
+    K.val.assign(K.col.size(), 0.0);
     std::vector<double> R(n), b(n);
     assemble(P, nullptr, R.data(), &K, n_threads);
     for (int64_t i = 0; i < n; i++) b[i] = -R[i];   // one Newton step from X = 0

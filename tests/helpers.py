@@ -2,8 +2,10 @@
 import numpy as np
 
 
-def ctx_from_oracle(M, P, physics=None):
-    """A mastg Context over exactly the oracle problem's mesh, DofMap, BCs and loads."""
+def ctx_from_oracle(M, P, physics=None, **extra):
+    """A mastg Context over exactly the oracle problem's mesh, DofMap, BCs and
+    loads (`extra`: further description fields, e.g. radiation, that the
+    oracle problem was given too)."""
     xyz, conn = P.mesh()
     base, con = P.dofmap()
     sides = P.bsides()
@@ -15,6 +17,7 @@ def ctx_from_oracle(M, P, physics=None):
         mat = dict(E=70e9, nu=0.3, kappa=5.0 / 6.0, offset=0.0, pressure=1.0e3)
     else:
         mat = dict(k0=200.0, alpha=1e-3, conv_h=50.0, conv_tinf=300.0, conv_bid_mask=0x3E)
+    mat.update(extra)
     pl = P.point_load() if phys == M.HEX8_ELASTIC else None
     desc = M.desc_from_arrays(phys, xyz, conn, P.n_vars, node_dof_base=base,
                               constrained_dofs=np.nonzero(con)[0], sides=sides,

@@ -68,3 +68,46 @@ def test_heat_design_eval(oracle, mastg, n):
     mt = ctx.output(mastg.OUTPUT_MEAN_T, ref["T"])
     assert abs(mt - ref["mean_T"]) <= 1e-12 * abs(ref["mean_T"])
     ctx.close()
+
+
+# surface radiation on boundary ids 4 and 5 (x = 0 and z = 1; libMesh build_cube
+# ids), on top of the convection of ids 1..5: HeatConductionElementBase::
+# surface_radiation_residual (heat_conduction_elem_base.cpp:1402-1463)
+SB = 5.670374419e-8
+RAD_ORACLE = dict(rad_sigma=SB, rad_emiss=0.8, rad_Tamb=250.0, rad_T0=20.0, rad_mask=0x30)
+RAD_DESC = dict(rad_sigma=SB, rad_emissivity=0.8, rad_tamb=250.0, rad_tzero=20.0, rad_bid_mask=0x30)
+
+
+@pytest.mark.parametrize("n", [1, 2, 4])
+def test_heat_radiation_R_J_at_state(oracle, mastg, n):
+    print(f"[heat radiation K] {n}^3", flush=True)
+    P = oracle.Problem(oracle.HEAT, n, seed=7).set(**RAD_ORACLE)
+    ctx = ctx_from_oracle(mastg, P, **RAD_DESC)
+    rp, _ = ctx.csr_pattern()
+    _, con = P.dofmap()
+    for T in (np.zeros(P.n_dofs), 600 + 200 * np.random.default_rng(n).standard_normal(P.n_dofs)):
+        T[con == 1] = 0.0
+        R0, K0 = P.assemble(T)
+        R, J = ctx.assemble(T)
+        assert rel_err_floor(J, K0, rp, floor=1e-2) <= 1e-12
+        np.testing.assert_allclose(R, R0, rtol=0, atol=1e-12 * np.abs(R0).max())
+    ctx.close()
+
+
+@pytest.mark.parametrize("n,solver", [(4, "bicgstab"), (8, "mg")])
+def test_heat_radiation_design_eval(oracle, mastg, n, solver):
+    print(f"[heat radiation Newton + adjoint] {n}^3 {solver}", flush=True)
+    P = oracle.Problem(oracle.HEAT, n, seed=5).set(**RAD_ORACLE)
+    ref = P.solve_heat(1e-12, 1e-14)
+    ctx = ctx_from_oracle(mastg, P, **RAD_DESC)
+    sv = mastg.SOLVER_MG if solver == "mg" else mastg.SOLVER_CG_JACOBI
+    r = ctx.design_eval(P.params(), rtol=1e-12, solver=sv)
+    print(f"  mean T {r['objective']:.12g} (oracle {ref['mean_T']:.12g}), Newton {r['newton_iters']} "
+          f"(oracle {ref['newton_iters']})", flush=True)
+    assert 2 <= r["newton_iters"] <= 15
+    assert abs(r["objective"] - ref["mean_T"]) <= 1e-9 * abs(ref["mean_T"])
+    assert rel_err_floor(r["grad"], ref["dJ"], floor=1e-6) <= 1e-7
+    # radiation moved the answer: the same problem without it differs
+    P0 = oracle.Problem(oracle.HEAT, n, seed=5)
+    assert abs(P0.solve_heat(1e-12, 1e-14)["mean_T"] - ref["mean_T"]) > 1e-3 * abs(ref["mean_T"])
+    ctx.close()

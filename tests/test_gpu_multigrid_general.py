@@ -1,0 +1,51 @@
+"""Multigrid on hex8 meshes whose cells are not congruent: a block with its
+interior nodes displaced (every element a different shape).  The hierarchy
+is found from the connectivity; the problem level smooths with the assembled
+FP64 matrix's FP32 node blocks, the coarse operators are Galerkin products of
+the FP64 element matrices, and the CG operator is the FP64 node-block product.
+Parity against the oracle on the same arrays (the oracle's CG + Jacobi solve):
+compliance 1e-9 and dC/drho 1e-7 after solves to 1e-10; the multigrid needs
+far fewer iterations than CG + Jacobi.
+"""
+import numpy as np
+import pytest
+
+from helpers import rel_err_floor
+
+pytestmark = pytest.mark.gpu
+
+
+def distorted(oracle, nx, ny, nz, amp, seed):
+    P = oracle.Problem(oracle.CANTILEVER, nx, ny, nz, seed=seed, lo=0.01, hi=1.0)
+    xyz, conn = P.mesh()
+    h = np.array([1.0 / nx, 0.5 / ny, 0.5 / nz])
+    lo, hi = xyz.min(0), xyz.max(0)
+    interior = np.all((xyz > lo + 1e-12) & (xyz < hi - 1e-12), axis=1)
+    rng = np.random.default_rng(seed)
+    xyz = xyz.copy()
+    xyz[interior] += amp * h * rng.uniform(-1.0, 1.0, (int(interior.sum()), 3))
+    base, con = P.dofmap()
+    arrays = dict(xyz=xyz, conn=conn, n_vars=P.n_vars, node_base=base, constrained=con, params=P.params(),
+                  point_load=P.point_load())
+    return P, oracle.Problem(oracle.CANTILEVER, arrays=arrays), arrays
+
+
+@pytest.mark.parametrize("mesh", [(16, 8, 8), (32, 16, 16)])
+def test_distorted_hex8_multigrid(oracle, mastg, mesh):
+    print(f"[multigrid general] distorted hex8 {mesh}", flush=True)
+    P0, P, a = distorted(oracle, *mesh, amp=0.25, seed=5)
+    desc = mastg.desc_from_arrays(mastg.HEX8_ELASTIC, a["xyz"], a["conn"], a["n_vars"], node_dof_base=a["node_base"],
+                                  constrained_dofs=np.nonzero(a["constrained"])[0], sides=P0.bsides(),
+                                  point_loads=a["point_load"], E=72e9, nu=0.33, penalty=3.0)
+    ctx = mastg.Context(desc)
+    rho = a["params"]
+    ref = P.solve_elastic(1e-12)
+    rm = ctx.design_eval(rho, solver=mastg.SOLVER_MG)
+    rp = ctx.design_eval(rho, solver=mastg.SOLVER_CG_JACOBI)
+    for name, r in (("multigrid", rm), ("CG + Jacobi", rp)):
+        ec = abs(r["objective"] - ref["compliance"]) / ref["compliance"]
+        eg = rel_err_floor(r["grad"], ref["dc"], floor=1e-6)
+        print(f"  {name}: {r['iters']} iterations, C rel err {ec:.1e}, grad {eg:.1e}", flush=True)
+        assert r["rel_res"] <= 1.5e-10 and ec <= 1e-9 and eg <= 1e-7
+    assert rm["iters"] * 10 < rp["iters"], (rm["iters"], rp["iters"])
+    ctx.close()

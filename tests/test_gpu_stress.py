@@ -1,0 +1,98 @@
+"""GPU parity of the KS von Mises stress output (MASTG_OUTPUT_KS_STRESS) and
+its adjoint sensitivity against the oracle (oracle/stress.py over the
+oracle's quadrature-point stresses), through the C ABI.
+
+Pins: the value against the oracle's at the same state (1e-12); dKS/dX
+against central differences of the oracle's KS along random directions;
+Euler's identity for the degree-1 homogeneous von Mises stress,
+dKS/dX . u = sum_e rho_e (dKS/drho_e|direct) / p; the direct density
+derivative against central differences of the oracle's KS in rho_e at fixed
+u; the adjoint total derivative dKS/drho_e = direct_e - lambda . dR/drho_e
+(K lambda = dKS/dX) against central differences of the oracle's KS(u(rho))
+through its own solves (KSStressStrainOutput, src/elasticity/
+ks_stress_output.cpp:53-243; AssemblyBase adjoint sensitivities,
+src/base/assembly_base.cpp:594-639).
+"""
+import numpy as np
+import pytest
+
+import stress as S
+from helpers import ctx_from_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _ks_oracle(P, u, k, sref):
+    sig, jxw = S.element_stresses(P, u)
+    return S.ks(S.von_mises(sig), jxw, k, sref)
+
+
+def test_ks_value_and_derivatives(oracle, mastg):
+    print("[KS stress] hex8 6x3x3: value, dKS/dX, direct dKS/drho", flush=True)
+    P = oracle.Problem(oracle.CANTILEVER, 6, 3, 3, seed=3, lo=0.3, hi=1.0)
+    ctx = ctx_from_oracle(mastg, P)
+    ref = P.solve_elastic(1e-12)
+    u = ref["u"]
+    sig, jxw = S.element_stresses(P, u)
+    sref = float(S.von_mises(sig).max())
+    k = 8.0
+    ctx.set_ks_stress(k, sref)
+    v, dv = ctx.output(mastg.OUTPUT_KS_STRESS, u, want_grad=True)
+    v0 = _ks_oracle(P, u, k, sref)
+    print(f"  KS {v:.15g} (oracle {v0:.15g})", flush=True)
+    assert abs(v - v0) <= 1e-12 * max(1.0, abs(v0))
+    rng = np.random.default_rng(1)
+    _, con = P.dofmap()
+    for t in range(3):
+        d = rng.standard_normal(P.n_dofs) * np.abs(u).max()
+        d[con == 1] = 0.0
+        h = 1e-5
+        fd = (_ks_oracle(P, u + h * d, k, sref) - _ks_oracle(P, u - h * d, k, sref)) / (2 * h)
+        err = abs(np.dot(dv, d) - fd) / abs(fd)
+        print(f"  dKS/dX . d vs central difference: rel err {err:.2e}", flush=True)
+        assert err <= 1e-6
+    g_dir = ctx.output_sensitivity(mastg.OUTPUT_KS_STRESS, u)
+    rho = P.params()
+    assert abs(np.dot(dv, u) - np.dot(rho, g_dir) / 3.0) <= 1e-10 * abs(np.dot(dv, u))
+    for e in (0, 17, P.n_elems - 1):
+        dr = 1e-6 * rho[e]
+        rp = rho.copy(); rp[e] += dr
+        rm = rho.copy(); rm[e] -= dr
+        P.set_params(rp); kp = _ks_oracle(P, u, k, sref)
+        P.set_params(rm); km = _ks_oracle(P, u, k, sref)
+        P.set_params(rho)
+        fd = (kp - km) / (2 * dr)
+        assert abs(g_dir[e] - fd) <= 1e-6 * abs(fd) + 1e-12 * np.abs(g_dir).max(), (e, g_dir[e], fd)
+    # compliance and mean temperature have no direct density dependence
+    assert np.all(ctx.output_sensitivity(mastg.OUTPUT_COMPLIANCE, u) == 0.0)
+    ctx.close()
+
+
+def test_ks_adjoint_total_derivative(oracle, mastg):
+    print("[KS stress adjoint] hex8 8x4x4: total dKS/drho vs oracle solves", flush=True)
+    P = oracle.Problem(oracle.CANTILEVER, 8, 4, 4, seed=6, lo=0.3, hi=1.0)
+    ctx = ctx_from_oracle(mastg, P)
+    rho = P.params()
+    R, _ = ctx.assemble()
+    u, _ = ctx.solve(-R, rtol=1e-13)
+    sig, jxw = S.element_stresses(P, u)
+    sref = float(S.von_mises(sig).max())
+    k = 8.0
+    ctx.set_ks_stress(k, sref)
+    _, dKdX = ctx.output(mastg.OUTPUT_KS_STRESS, u, want_grad=True)
+    lam, _ = ctx.solve(dKdX, rtol=1e-13)               # K symmetric: adjoint = forward solve
+    total = ctx.output_sensitivity(mastg.OUTPUT_KS_STRESS, u) - ctx.sensitivity(u, lam)
+    for e in (3, 40, 101, P.n_elems - 1):
+        dr = 1e-5 * rho[e]
+        vals = []
+        for s in (1, -1):
+            rr = rho.copy(); rr[e] += s * dr
+            P.set_params(rr)
+            ue = P.solve_elastic(1e-13)["u"]
+            vals.append(_ks_oracle(P, ue, k, sref))
+        P.set_params(rho)
+        fd = (vals[0] - vals[1]) / (2 * dr)
+        err = abs(total[e] - fd) / abs(fd)
+        print(f"  element {e}: adjoint {total[e]:.10e}, central FD {fd:.10e}, rel err {err:.2e}", flush=True)
+        assert err <= 1e-5
+    ctx.close()
