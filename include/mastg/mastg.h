@@ -312,6 +312,12 @@ int mastg_get_tuning(const mastg_ctx* ctx, int* spmv_lanes, int* grid);
  * otherwise); 2..10000, 0 = the default: hex8 elasticity 80, plate 150,
  * heat 7 */
 #define MASTG_OPT_MG_CHEB_RATIO 4
+/* hex8 elasticity FP64 assembly: 1 (default) = fused kernel by CSR row owner
+ * (element blocks, residual, scatter and inverse diagonal in one pass, no
+ * element matrices in memory); 0 = element kernel + segmented scatter (also
+ * used whenever the element matrices are needed: multigrid on an assembled
+ * problem level, the FP32 element path) */
+#define MASTG_OPT_ROW_ASSEMBLY 5
 int mastg_set_option(mastg_ctx* ctx, int option, int value);
 int mastg_set_tuning(mastg_ctx* ctx, int spmv_lanes, int grid);
 

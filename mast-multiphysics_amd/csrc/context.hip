@@ -99,6 +99,7 @@ struct mastg_ctx {
     // heat: convective side masks, mean-T weights, BiCGStab work space, A^T values
     DevBuf sidemask, radmask, wmean, tpos, valT, vr0, vv, vs, vt, vph, vsh, bscal, vT;
     bool has_rad = false;
+    bool row_fused = true;           // fused row-owner assembly allowed (MASTG_OPT_ROW_ASSEMBLY)
     double ks_k = 10.0, ks_sref = 1.0;   // KS stress output: aggregation constant, reference stress
     DevBuf ks_part, ks_IV;
     double volume = 0;
@@ -395,10 +396,55 @@ int dot_active(mastg_ctx* c, const double* a, const double* b, double* out) {
     return MASTG_OK;
 }
 
-// ---- assembly: element kernel -> segmented scatter -> Jacobi diagonal
+// the fused row-owner assembly applies: hex8 elasticity in FP64, rows of at
+// most the accumulator's block count, and no consumer of the element blocks
+// (the multigrid on an assembled problem level forms its Galerkin products
+// from them)
+bool row_assembly_ok(mastg_ctx* c, bool fp32) {
+    if (c->d.physics != MASTG_HEX8_ELASTIC || fp32 || !c->row_fused) return false;
+    if (!c->mg.empty() && c->mg_power) return false;
+    return c->S.max_blocks_per_row <= hex8_row_assembly_max_blocks();
+}
+
+// ---- assembly: element kernel -> segmented scatter -> Jacobi diagonal, or
+// the fused row-owner kernel (hex8 elasticity)
 int assemble_active(mastg_ctx* c, const double* X_active, bool want_R, bool want_K, bool fp32 = false) {
     if (!c->params_set) return fail(c, MASTG_ERR_INVALID, "design parameters not set");
     const Structure& S = c->S;
+    if (want_K && row_assembly_ok(c, fp32)) {
+        TRY(dalloc(c, c->val, size_t(S.nnz) * sizeof(double)));
+        TRY(dalloc(c, c->dinv, size_t(S.n_active) * sizeof(double)));
+        Hex8RowArgs a{};
+        a.n_nodes = S.n_nodes;
+        a.xyz = c->xyz.as<double>();
+        a.econn = c->econn.as<int32_t>();
+        a.ninc_ptr = c->ninc_ptr.as<int32_t>();
+        a.ninc = c->ninc.as<int32_t>();
+        a.blk_ptr = c->blk_ptr.as<int64_t>();
+        a.blk_col = c->blk_col.as<int32_t>();
+        a.row_ptr = c->row_ptr.as<int64_t>();
+        a.param = c->param.as<double>();
+        a.E0 = c->d.E;
+        a.penalty = c->d.penalty;
+        a.lam = c->Dhat[1];        // D(E = 1) = lam 1 1^T + mu diag(2, 2, 2, 1, 1, 1)
+        a.mu = c->Dhat[21];
+        a.con = c->con.as<uint8_t>();
+        a.X = X_active;
+        a.F = c->F.as<double>();
+        a.val = c->val.as<double>();
+        a.R = want_R ? c->vtmp.as<double>() : nullptr;
+        a.dinv = c->dinv.as<double>();
+        {
+            TimedRegion t(c, 0);
+            launch_hex8_row_assemble(a, c->stream);
+        }
+        c->assembled = true;
+        c->val32_valid = false;
+        c->kel64_valid = false;
+        c->mg_valid = false;
+        HIPC(c, hipGetLastError());
+        return MASTG_OK;
+    }
     if (want_K) {
         const size_t nkel = size_t(S.n_elems) * S.npe * S.npe * S.nav * S.nav;
         if (fp32) TRY(dalloc(c, c->Kel32, nkel * sizeof(float)));
@@ -2129,6 +2175,7 @@ int mastg_set_option(mastg_ctx* c, int option, int value) {
     if (!c) return MASTG_ERR_INVALID;
     switch (option) {
         case MASTG_OPT_GRAPHS: c->use_graphs = value != 0; return MASTG_OK;
+        case MASTG_OPT_ROW_ASSEMBLY: c->row_fused = value != 0; return MASTG_OK;
         case MASTG_OPT_MG_STEPS_FINE:
         case MASTG_OPT_MG_STEPS_COARSE:
         case MASTG_OPT_MG_CHEB_RATIO:
@@ -2205,8 +2252,8 @@ int mastg_bench_kernel(mastg_ctx* c, int kind, int n_launch, double* avg_ms) {
         }
     };
     if (kind < 0 || kind > 10) return fail(c, MASTG_ERR_INVALID, "unknown kernel kind");
-    if (kind == 0 && c->d.physics != MASTG_HEX8_HEAT && !c->Kel.p)
-        return fail(c, MASTG_ERR_INVALID, "no FP64 element blocks resident (assemble first)");
+    if (kind == 0 && c->d.physics != MASTG_HEX8_HEAT)   // the element kernel alone, into its own blocks
+        TRY(dalloc(c, c->Kel, size_t(c->S.n_elems) * c->S.npe * c->S.npe * c->S.nav * c->S.nav * sizeof(double)));
     if (kind == 10) {
         // the whole FP64 assembly at the state in the solution vector:
         // element K and R, scatter into the CSR, inverse diagonal, node

@@ -53,6 +53,30 @@ struct HeatArgs {
     double src_scale;       // multiplies the source q_e (1; 0 leaves the source out)
 };
 
+// fused hex8 assembly by CSR row owner (K, R, scatter, inverse diagonal)
+struct Hex8RowArgs {
+    int64_t n_nodes;
+    const double* xyz;        // [n_nodes*3] by node rank
+    const int32_t* econn;     // [n_elems*8]
+    const int32_t* ninc_ptr;  // [n_nodes+1] node -> (element, local node) incidences
+    const int32_t* ninc;      // e*8 + la
+    const int64_t* blk_ptr;   // [n_nodes+1] node-block CSR
+    const int32_t* blk_col;
+    const int64_t* row_ptr;   // scalar CSR (node r's rows start its value chunk)
+    const double* param;      // rho_e
+    double E0, penalty;
+    double lam, mu;           // Lame constants of the material matrix at E = 1
+    const uint8_t* con;       // [n_active] or null
+    const double* X;          // [n_active] state or null (no residual from the elements)
+    const double* F;          // [n_active] loads or null
+    double* val;              // CSR values
+    double* R;                // [n_active] residual or null
+    double* dinv;             // [n_active] inverse diagonal or null
+};
+// rows with more neighbour blocks than this take the element + scatter path
+int hex8_row_assembly_max_blocks();
+void launch_hex8_row_assemble(const Hex8RowArgs& a, hipStream_t s);
+
 struct ScatterArgs {
     int64_t n_nodes;
     int nav;

@@ -145,6 +145,8 @@ std::string build_structure(int physics, int64_t n_nodes, int64_t n_elems, const
         S.blk_ptr[r + 1] = S.blk_ptr[r] + int64_t(v.size());
     }
     S.n_blocks = S.blk_ptr[n_nodes];
+    for (int64_t r = 0; r < n_nodes; r++)
+        S.max_blocks_per_row = std::max(S.max_blocks_per_row, S.blk_ptr[r + 1] - S.blk_ptr[r]);
     if (S.n_blocks >= (int64_t(1) << 31)) return "too many blocks for 32-bit maps";
     S.blk_col.resize(S.n_blocks);
     for (int64_t r = 0; r < n_nodes; r++) {

@@ -28,6 +28,7 @@ struct Structure {
 
     // node-block CSR: row = node rank, columns = neighbour ranks (sorted)
     std::vector<int64_t> blk_ptr;          // [n_nodes+1]
+    int64_t max_blocks_per_row = 0;        // largest node-row degree
     std::vector<int32_t> blk_col;          // [n_blocks]
     // scalar CSR over active DOFs (rows (rank,v), columns (nbr, w))
     std::vector<int64_t> row_ptr;          // [n_active+1]

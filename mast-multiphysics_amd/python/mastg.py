@@ -21,6 +21,7 @@ OPT_GRAPHS = 1
 OPT_MG_STEPS_FINE = 2
 OPT_MG_STEPS_COARSE = 3
 OPT_MG_CHEB_RATIO = 4
+OPT_ROW_ASSEMBLY = 5
 ERRORS = {0: "ok", -1: "invalid", -2: "out of memory", -3: "not converged", -4: "nan",
           -5: "device error", -6: "admission", -7: "busy"}
 

@@ -22,3 +22,19 @@ def test_mast_call_sequence(nx):
     out = json.loads(p.stdout.strip().splitlines()[-1])
     assert out["compliance_rel_diff"] < 1e-8
     assert out["sens_max_rel_diff"] < 1e-6
+
+
+def test_nastran_ks_forward_adjoint_sequence():
+    """NastranIO::read -> FilterBase -> solve -> KSStressStrainOutput with its
+    adjoint -> NonlinearSystem::sensitivity_solve for E, through the C++
+    layer (mast-multiphysics_amd/host/example_nastran_ks.cpp): the forward and
+    adjoint dC/dE agree, and equal -C/E (K is linear in E)."""
+    print("[C++ layer] Nastran model, KS stress, forward sensitivity", flush=True)
+    exe = os.path.join(ROOT, "mast-multiphysics_amd", "example_nastran_ks")
+    assert os.path.exists(exe), "example_nastran_ks is not built"
+    bdf = os.path.join(ROOT, "tests", "golden", "butterfly_hex.bdf")
+    p = subprocess.run([exe, bdf, "0.15"], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stderr + p.stdout
+    out = json.loads(p.stdout.strip().splitlines()[-1])
+    assert out["dCdE_forward_vs_adjoint"] < 1e-8 and out["dCdE_vs_minus_C_over_E"] < 1e-8
+    assert out["ks_grad_max"] > 0

@@ -314,3 +314,53 @@ def test_design_eval_mixed(oracle, mastg, mesh, lo):
     x, st = ctx.solve(-R, solver=mastg.SOLVER_MIXED)
     assert st["replacements"] >= 1 and st["rel_res"] <= 1.5e-10
     ctx.close()
+
+
+@pytest.mark.parametrize("mesh,exact,distort", [((3, 2, 2), True, 0.0), ((12, 6, 6), False, 0.2),
+                                                ((20, 10, 10), False, 0.0)])
+def test_row_assembly_equals_element_scatter(oracle, mastg, mesh, exact, distort):
+    """The fused row-owner assembly (default) against the element kernel +
+    segmented scatter (MASTG_OPT_ROW_ASSEMBLY 0) and the oracle: J, R and the
+    design evaluation."""
+    print(f"[hex8 row assembly] mesh={mesh} mast_exact={exact} distortion={distort}", flush=True)
+    P0 = oracle.Problem(oracle.CANTILEVER, *mesh, seed=21, lo=0.05, hi=1.0, mast_exact=exact)
+    xyz, conn = P0.mesh()
+    if distort:
+        lo, hi = xyz.min(0), xyz.max(0)
+        inner = np.all((xyz > lo + 1e-12) & (xyz < hi - 1e-12), axis=1)
+        xyz = xyz.copy()
+        xyz[inner] += distort / mesh[0] * np.random.default_rng(2).uniform(-1, 1, (int(inner.sum()), 3))
+    base, con = P0.dofmap()
+    arrays = dict(xyz=xyz, conn=conn, n_vars=P0.n_vars, node_base=base, constrained=con, params=P0.params(),
+                  point_load=P0.point_load())
+    P = oracle.Problem(oracle.CANTILEVER, arrays=arrays)
+    desc = mastg.desc_from_arrays(mastg.HEX8_ELASTIC, xyz, conn, P0.n_vars, node_dof_base=base,
+                                  constrained_dofs=np.nonzero(con)[0], point_loads=P0.point_load(),
+                                  E=72e9, nu=0.33, penalty=3.0)
+    ctx = mastg.Context(desc)
+    ctx.set_params(P0.params())
+    X = np.random.default_rng(4).standard_normal(P.n_dofs) * 1e-4
+    X[con == 1] = 0.0
+    R1, J1 = ctx.assemble(X)
+    ctx.set_option(mastg.OPT_ROW_ASSEMBLY, 0)
+    R0, J0 = ctx.assemble(X)
+    rp, _ = ctx.csr_pattern()
+    assert rel_err_floor(J1, J0, rp, floor=1e-2) <= 1e-12
+    assert np.max(np.abs(R1 - R0)) <= 1e-12 * np.abs(R0).max()
+    # and against the oracle
+    orp, ocol = P.csr()
+    Ro, Ko = P.assemble(X)
+    if exact:
+        keep = np.sort(np.concatenate([base + v for v in range(3)]))
+        orp, ocol, Ko = restrict_csr(orp, ocol, Ko, keep)
+        act = keep
+    else:
+        act = np.arange(P.n_dofs)
+    assert rel_err_floor(J1, Ko, orp, floor=1e-2) <= 1e-12
+    assert np.max(np.abs(R1 - Ro)[act]) <= 1e-12 * np.abs(Ro[act]).max()
+    ctx.set_option(mastg.OPT_ROW_ASSEMBLY, 1)
+    ref = P.solve_elastic(1e-10)
+    r = ctx.design_eval(P0.params())
+    assert abs(r["objective"] - ref["compliance"]) <= 1e-9 * ref["compliance"]
+    assert rel_err_floor(r["grad"], ref["dc"], floor=1e-6) <= 1e-7
+    ctx.close()

@@ -9,7 +9,7 @@ dKS/dX . u = sum_e rho_e (dKS/drho_e|direct) / p; the direct density
 derivative against central differences of the oracle's KS in rho_e at fixed
 u; the adjoint total derivative dKS/drho_e = direct_e - lambda . dR/drho_e
 (K lambda = dKS/dX) against central differences of the oracle's KS(u(rho))
-through its own solves (KSStressStrainOutput, src/elasticity/
+through its own solves (1e-5, floor 1e-7 of the largest entry) (KSStressStrainOutput, src/elasticity/
 ks_stress_output.cpp:53-243; AssemblyBase adjoint sensitivities,
 src/base/assembly_base.cpp:594-639).
 """
@@ -46,7 +46,7 @@ def test_ks_value_and_derivatives(oracle, mastg):
     for t in range(3):
         d = rng.standard_normal(P.n_dofs) * np.abs(u).max()
         d[con == 1] = 0.0
-        h = 1e-5
+        h = 1e-6                      # truncation of the quotient ~3e-8 here (2.9e-6 at 1e-5)
         fd = (_ks_oracle(P, u + h * d, k, sref) - _ks_oracle(P, u - h * d, k, sref)) / (2 * h)
         err = abs(np.dot(dv, d) - fd) / abs(fd)
         print(f"  dKS/dX . d vs central difference: rel err {err:.2e}", flush=True)
@@ -94,5 +94,8 @@ def test_ks_adjoint_total_derivative(oracle, mastg):
         fd = (vals[0] - vals[1]) / (2 * dr)
         err = abs(total[e] - fd) / abs(fd)
         print(f"  element {e}: adjoint {total[e]:.10e}, central FD {fd:.10e}, rel err {err:.2e}", flush=True)
-        assert err <= 1e-5
+        # each oracle solve stops at 1e-13: its error divided by the step
+        # floors the quotient near 1e-9 absolute, which only the smallest
+        # entries (1e-3 of the largest) notice
+        assert abs(total[e] - fd) <= 1e-5 * abs(fd) + 1e-7 * np.abs(total).max()
     ctx.close()
