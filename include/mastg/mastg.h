@@ -254,6 +254,17 @@ int mastg_set_ks_stress(mastg_ctx* ctx, double k, double sigma_ref);
  * KSStressStrainOutput::functional_sensitivity_for_elem (ks_stress_output.cpp:98-136). */
 int mastg_output_sensitivity(mastg_ctx* ctx, int kind, const double* X, int where, double* dJdp);
 
+/* mastg_output for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops): no counterpart in the reference.  X
+ * ([n_dofs], caller numbering) and value (one double) are DEVICE pointers; the
+ * work is enqueued on the context's stream and not waited for (value is ready
+ * after mastg_synchronize).  It calls no hipMalloc / hipFree and no allocating
+ * library routine: the reduction's partial sums (at most 4096) live in a
+ * file-scope device array of 32 KiB per GPU, held until the process ends and
+ * shared by every call on that GPU, so calls that overlap on one GPU race.
+ * Any vector length fits (grid-stride reduction); outputs as mastg_output. */
+int mastg_output_noalloc(mastg_ctx* ctx, int kind, const double* X, double* value);
+
 /* ------------------------------------------- one design evaluation ---- */
 /* assemble + solve + output + adjoint sensitivities with everything resident
  * on the device: compliance and dC/dp for the elastic physics, mean

@@ -37,6 +37,36 @@ std::recursive_mutex g_hip_mu;
 inline hipError_t ev_create(hipEvent_t* e) { HipLock l(g_hip_mu); return hipEventCreate(e); }
 inline hipError_t ev_destroy(hipEvent_t e) { HipLock l(g_hip_mu); return hipEventDestroy(e); }
 
+// Device scratch of the allocation-free output entry point
+// (mastg_output_noalloc): the partial sums of its reduction.  One copy per
+// GPU, held until the process ends; every call on a GPU shares it, so calls
+// that overlap on one GPU race.  Host code never passes `scratch` itself to a
+// kernel (in host code the name is a stand-in, not a device address): it
+// uses the address hipGetSymbolAddress returns for the current device.
+constexpr int kScratchPartials = 4096;
+static __device__ __align__(256) double scratch[kScratchPartials];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// the scratch array's address on the current device (looked up once per device)
+hipError_t scratch_address(double** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> l(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
+}
+
 }  // namespace
 
 std::recursive_mutex& mastg::hip_mutex() { return g_hip_mu; }
@@ -100,6 +130,7 @@ struct mastg_ctx {
     DevBuf sidemask, radmask, wmean, tpos, valT, vr0, vv, vs, vt, vph, vsh, bscal, vT;
     bool has_rad = false;
     bool row_fused = true;           // fused row-owner assembly allowed (MASTG_OPT_ROW_ASSEMBLY)
+    double* scratch_ptr = nullptr;   // this device's `scratch` (allocation-free output)
     double ks_k = 10.0, ks_sref = 1.0;   // KS stress output: aggregation constant, reference stress
     DevBuf ks_part, ks_IV;
     double volume = 0;
@@ -1501,6 +1532,10 @@ int mastg_ctx_create(const mastg_problem_desc* desc, mastg_ctx** out) {
         delete c;
         return MASTG_ERR_DEVICE;
     }
+    if (scratch_address(&c->scratch_ptr) != hipSuccess) {
+        c->err = "hipGetSymbolAddress of the scratch array failed";
+        return bail(MASTG_ERR_DEVICE);
+    }
     if (hipHostMalloc((void**)&c->h_scal, sizeof(CGScalars)) != hipSuccess ||
         hipHostMalloc((void**)&c->h_pin, 64 * sizeof(double)) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
@@ -1980,6 +2015,29 @@ int mastg_sensitivity_global(mastg_ctx* c, int param, const double* X, const dou
     TRY(global_dRdp(c, param, c->vb.as<double>(), c->vtmp.as<double>()));
     TRY(to_active(c, lam, where, c->vz.as<double>()));
     return dot_active(c, c->vz.as<double>(), c->vtmp.as<double>(), value);
+}
+
+int mastg_output_noalloc(mastg_ctx* c, int kind, const double* X, double* value) {
+    if (!c || !X || !value) return MASTG_ERR_INVALID;
+    if (!c->scratch_ptr) return fail(c, MASTG_ERR_DEVICE, "no scratch address for this device");
+    const int64_t n = c->S.n_active;
+    const int nparts = int(std::min<int64_t>(kScratchPartials, std::max<int64_t>(1, (n + 255) / 256)));
+    if (nparts > kScratchPartials)
+        return fail(c, MASTG_ERR_INVALID, "reduction needs more partial sums than the scratch holds");
+    if (kind == MASTG_OUTPUT_MEAN_T) {
+        if (c->d.physics != MASTG_HEX8_HEAT) return fail(c, MASTG_ERR_INVALID, "mean temperature needs heat physics");
+        // J = sum_i w_i T_i, w = (1/V) int N (every node; constrained ones carry T = 0)
+        launch_gather_dot(n, c->wmean.as<double>(), nullptr, c->dof_of_active.as<int64_t>(), X, c->scratch_ptr,
+                          nparts, value, c->stream);
+    } else if (kind == MASTG_OUTPUT_COMPLIANCE) {
+        // C = F . X over the free active DOFs
+        launch_gather_dot(n, c->F.as<double>(), c->con.as<uint8_t>(), c->dof_of_active.as<int64_t>(), X,
+                          c->scratch_ptr, nparts, value, c->stream);
+    } else {
+        return fail(c, MASTG_ERR_INVALID, "output not available");
+    }
+    HIPC(c, hipGetLastError());
+    return MASTG_OK;
 }
 
 int mastg_set_ks_stress(mastg_ctx* c, double k, double sigma_ref) {

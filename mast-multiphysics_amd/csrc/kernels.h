@@ -206,6 +206,10 @@ void launch_jacobi_inv(int64_t n, const int64_t* diag_pos, const double* val, do
 void launch_dot(int64_t n, const double* a, const double* b, double* partials, int nparts,
                 double* out, hipStream_t s);
 void launch_spmv_csr(const CGArgs& c, const double* x, double* y, int want_dot, hipStream_t s);
+// *out = sum_i w_i x[idx[i]] over the i with skip[i] == 0 (skip may be null),
+// partial sums of the nparts work-groups in `partials`, reduced in a fixed order
+void launch_gather_dot(int64_t n, const double* w, const uint8_t* skip, const int64_t* idx, const double* x,
+                       double* partials, int nparts, double* out, hipStream_t s);
 void launch_cg_start(const CGArgs& c, const double* b, double tol2, int max_it, hipStream_t s);
 void launch_cg_iteration(const CGArgs& c, hipStream_t s);
 void launch_elem_scale(int64_t n, const double* rho, double E0, double penalty, double* sE,

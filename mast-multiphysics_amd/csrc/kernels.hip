@@ -853,7 +853,18 @@ __global__ __launch_bounds__(256) void k_reduce_to(const double* __restrict__ p,
     if (threadIdx.x == 0) *out = s;
 }
 
-
+// partial sums of sum_i w_i x[idx[i]] over the i with skip[i] == 0 (skip may
+// be null): x is read through the active -> caller DOF map, so a caller's
+// vector needs no staging copy; fixed grid-stride assignment, deterministic
+__global__ __launch_bounds__(256) void k_gather_dot(int64_t n, const double* __restrict__ w,
+                                                    const uint8_t* __restrict__ skip,
+                                                    const int64_t* __restrict__ idx,
+                                                    const double* __restrict__ x, double* __restrict__ partials) {
+    double s = 0.0;
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256)
+        if (!skip || !skip[i]) s += w[i] * x[idx[i]];
+    block_sum_store(s, partials, blockIdx.x);
+}
 
 
 // ---------------------------------------------------------------------------
@@ -1344,6 +1355,11 @@ void launch_axpby(int64_t n, double a, const double* x, double b, double* y, hip
 void launch_jacobi_inv(int64_t n, const int64_t* diag_pos, const double* val, double* dinv,
                        hipStream_t s) {
     hipLaunchKernelGGL(k_jacobi_inv, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, diag_pos, val, dinv);
+}
+void launch_gather_dot(int64_t n, const double* w, const uint8_t* skip, const int64_t* idx, const double* x,
+                       double* partials, int nparts, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_gather_dot, dim3(nparts), dim3(256), 0, s, n, w, skip, idx, x, partials);
+    hipLaunchKernelGGL(k_reduce_to, dim3(1), dim3(256), 0, s, partials, nparts, out);
 }
 void launch_dot(int64_t n, const double* a, const double* b, double* partials, int nparts,
                 double* out, hipStream_t s) {

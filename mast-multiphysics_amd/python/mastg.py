@@ -108,6 +108,7 @@ def lib():
             "mastg_sensitivity": (C.c_int, [_P, _D, _D, _D, C.c_int]),
             "mastg_jacobian_product": (C.c_int, [_P, _D, _D, C.c_int, C.c_int]),
             "mastg_output": (C.c_int, [_P, C.c_int, _D, C.c_int, _D, _D]),
+            "mastg_output_noalloc": (C.c_int, [_P, C.c_int, C.c_void_p, C.c_void_p]),
             "mastg_set_ks_stress": (C.c_int, [_P, C.c_double, C.c_double]),
             "mastg_output_sensitivity": (C.c_int, [_P, C.c_int, _D, C.c_int, _D]),
             "mastg_device_alloc": (C.c_int, [_P, C.c_size_t, C.POINTER(C.c_void_p)]),
