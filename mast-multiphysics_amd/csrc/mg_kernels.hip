@@ -1038,9 +1038,10 @@ __global__ __launch_bounds__(1024) void k_mg_invert_lds(int n, double* __restric
 // 1/2), constrained fine DOFs weighted 0, constrained coarse DOFs zero (P^T),
 // fused with the start of the coarse level's pre-smoothing.
 // FL: layout of the fine vectors (FL_LAT lattice order, node-major; FL_RANK
-// indexed through rank_of_lat, the plate and heat problem levels; FL_PLANAR
-// lattice order, component-major, the hex8 problem level); coarse levels are in
-// lattice order, node-major.
+// indexed through rank_of_lat, the assembled problem levels: plate, heat and
+// hex8 with cells of several shapes; FL_PLANAR lattice order, component-major,
+// the matrix-free hex8 problem level); coarse levels are in lattice order,
+// node-major.
 enum { FL_LAT = 0, FL_RANK = 1, FL_PLANAR = 2 };
 template <int FL, int NAV>
 __device__ __forceinline__ int64_t mg_fine_index(const MGLevelArgs& F, int64_t p, int v) {
@@ -1504,6 +1505,7 @@ void launch_mg_restrict(const MGLevelArgs& F, const MGLevelArgs& C, const int* d
         else hipLaunchKernelGGL((k_mg_restrict<3, 1, FL_LAT>), g, dim3(256), 0, s, F, C, done);
     } else {
         if (fl == FL_PLANAR) hipLaunchKernelGGL((k_mg_restrict<3, 3, FL_PLANAR>), g, dim3(256), 0, s, F, C, done);
+        else if (fl == FL_RANK) hipLaunchKernelGGL((k_mg_restrict<3, 3, FL_RANK>), g, dim3(256), 0, s, F, C, done);
         else hipLaunchKernelGGL((k_mg_restrict<3, 3, FL_LAT>), g, dim3(256), 0, s, F, C, done);
     }
 }
@@ -1520,6 +1522,8 @@ void launch_mg_prolong(const MGLevelArgs& F, const MGLevelArgs& C, const double*
     } else {
         if (fl == FL_PLANAR)
             hipLaunchKernelGGL((k_mg_prolong<3, 3, FL_PLANAR>), g, dim3(256), 0, s, F, C, dlast, done);
+        else if (fl == FL_RANK)
+            hipLaunchKernelGGL((k_mg_prolong<3, 3, FL_RANK>), g, dim3(256), 0, s, F, C, dlast, done);
         else hipLaunchKernelGGL((k_mg_prolong<3, 3, FL_LAT>), g, dim3(256), 0, s, F, C, dlast, done);
     }
 }
