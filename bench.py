@@ -382,8 +382,8 @@ def main():
                     help="launch the solver kernels directly instead of replaying hipGraphs")
     ap.add_argument("--kernel-profile", action="store_true",
                     help="only assemble the workload and launch the lattice operator, the FP32 smoothing "
-                         "step and the 256 MiB copy kernel 5 times each (a short program for "
-                         "hardware-counter runs)")
+                         "step, the 256 MiB copy kernel and the fused row assembly (on the mesh and on a "
+                         "distorted copy) 5 times each (a short program for hardware-counter runs)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the multi-GPU path is not built", file=sys.stderr)
@@ -411,8 +411,13 @@ def main():
         k_ms = ctx.bench_kernel(6, 5)
         s_ms = ctx.bench_kernel(8, 5)
         c_ms = ctx.bench_kernel(7, 5)
+        asm_ms = ctx.bench_kernel(10, 5)     # fused row assembly (class matrices)
+        dctx = distorted_context(w)
+        dctx.set_params(params)
+        gasm_ms = dctx.bench_kernel(10, 5)   # fused row assembly (geometry per element)
+        dctx.close()
         print(json.dumps({"workload": a.workload, "lattice_apply_ms": k_ms, "lattice_smooth_ms": s_ms,
-                          "copy_256MiB_ms": c_ms,
+                          "copy_256MiB_ms": c_ms, "row_assembly_ms": asm_ms, "row_assembly_distorted_ms": gasm_ms,
                           "copy_GBs": 2 * 2**28 / (c_ms * 1e-3) / 1e9,
                           "lattice_flops": lattice_flops(ctx.n_active // 3),
                           "lattice_bytes": lattice_bytes(ctx.n_active, w.n_elems),

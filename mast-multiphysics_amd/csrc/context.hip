@@ -427,6 +427,8 @@ int dot_active(mastg_ctx* c, const double* a, const double* b, double* out) {
     return MASTG_OK;
 }
 
+void ensure_scales(mastg_ctx* c);
+
 // the fused row-owner assembly applies: hex8 elasticity in FP64, rows of at
 // most the accumulator's block count, and no consumer of the element blocks
 // (the multigrid on an assembled problem level forms its Galerkin products
@@ -445,6 +447,8 @@ int assemble_active(mastg_ctx* c, const double* X_active, bool want_R, bool want
     if (want_K && row_assembly_ok(c, fp32)) {
         TRY(dalloc(c, c->val, size_t(S.nnz) * sizeof(double)));
         TRY(dalloc(c, c->dinv, size_t(S.n_active) * sizeof(double)));
+        if (!c->sE.p) TRY(dalloc(c, c->sE, size_t(S.n_elems) * sizeof(double)));   // meshes without classes
+        ensure_scales(c);
         Hex8RowArgs a{};
         a.n_nodes = S.n_nodes;
         a.xyz = c->xyz.as<double>();
@@ -454,11 +458,13 @@ int assemble_active(mastg_ctx* c, const double* X_active, bool want_R, bool want
         a.blk_ptr = c->blk_ptr.as<int64_t>();
         a.blk_col = c->blk_col.as<int32_t>();
         a.row_ptr = c->row_ptr.as<int64_t>();
-        a.param = c->param.as<double>();
-        a.E0 = c->d.E;
-        a.penalty = c->d.penalty;
+        a.sE = c->sE.as<double>();
         a.lam = c->Dhat[1];        // D(E = 1) = lam 1 1^T + mu diag(2, 2, 2, 1, 1, 1)
         a.mu = c->Dhat[21];
+        // congruence classes: the blocks are scaled class matrices
+        a.Khat = S.n_classes > 0 ? c->Khat.as<double>() : nullptr;
+        a.eclass = S.n_classes > 0 ? c->eclass.as<uint8_t>() : nullptr;
+        a.n_classes = S.n_classes;
         a.con = c->con.as<uint8_t>();
         a.X = X_active;
         a.F = c->F.as<double>();
@@ -467,7 +473,7 @@ int assemble_active(mastg_ctx* c, const double* X_active, bool want_R, bool want
         a.dinv = c->dinv.as<double>();
         {
             TimedRegion t(c, 0);
-            launch_hex8_row_assemble(a, c->stream);
+            launch_hex8_row_assemble(a, int(S.max_blocks_per_row), c->stream);
         }
         c->assembled = true;
         c->val32_valid = false;

@@ -63,9 +63,13 @@ struct Hex8RowArgs {
     const int64_t* blk_ptr;   // [n_nodes+1] node-block CSR
     const int32_t* blk_col;
     const int64_t* row_ptr;   // scalar CSR (node r's rows start its value chunk)
-    const double* param;      // rho_e
-    double E0, penalty;
+    const double* sE;         // [n_elems] element scales E0 rho_e^p
     double lam, mu;           // Lame constants of the material matrix at E = 1
+    // element classes (null: every element's blocks from its own geometry):
+    // K_e = sE[e] Khat[eclass[e]], Khat in node-block layout [class][a][b][3][3]
+    const double* Khat;
+    const uint8_t* eclass;
+    int n_classes;
     const uint8_t* con;       // [n_active] or null
     const double* X;          // [n_active] state or null (no residual from the elements)
     const double* F;          // [n_active] loads or null
@@ -75,7 +79,7 @@ struct Hex8RowArgs {
 };
 // rows with more neighbour blocks than this take the element + scatter path
 int hex8_row_assembly_max_blocks();
-void launch_hex8_row_assemble(const Hex8RowArgs& a, hipStream_t s);
+void launch_hex8_row_assemble(const Hex8RowArgs& a, int max_blocks_per_row, hipStream_t s);
 
 struct ScatterArgs {
     int64_t n_nodes;

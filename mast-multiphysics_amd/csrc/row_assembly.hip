@@ -12,48 +12,91 @@
 //     K_ab[i][j] = E_e sum_q JxW_q (lam g_a,i g_b,j + mu g_a,j g_b,i + mu delta_ij g_a . g_b)
 // (g = shape-function gradients at q) -- the same matrix as the dense
 // B^T D B product, 9 + 3 multiply-adds per (block, point) instead of 81.
+// When the mesh's elements fall into a few congruence classes (the same node
+// coordinates relative to the first node), K_e = E_e Khat_class with Khat the
+// class representative's matrix at E = 1, and a leaner kernel (no geometry,
+// no staged blocks) assembles the rows: k_hex8_row_assemble_class below.
 // NonlinearImplicitAssembly::residual_and_jacobian (src/base/
 // nonlinear_implicit_assembly.cpp:111-186) adds every element block into the
 // rows of its first node; here the owner of a row (one work-group = one wave
-// per node) computes exactly the element blocks of that row -- each
-// (element, a, b) block is computed once in the whole assembly -- and sums
-// them in a fixed order (the node's incidence order), so the assembly is
-// deterministic without atomics.  The row's values are written once,
-// contiguous (rows r*3..r*3+2 of node r are one chunk ordered (v, block, w)).
+// per node) forms exactly the element blocks of that row -- each (element,
+// a, b) block once in the whole assembly -- and sums them in a fixed order (the
+// node's incidence order), so the assembly is deterministic without atomics.
 //
-// Per wave: lanes (s, q) form the geometry of element slot s at quadrature
-// point q (Jacobian, its inverse, the 8 shape gradients) into LDS; lanes
-// (s, b) then form block (a_s, b) of element s, its constraint treatment
-// (zero constrained rows and columns, unit diagonal per element) and its
-// residual contribution K_ab u_b (before constraints, as the reference's
-// element vector), and add the block into the row accumulator in slot
-// order.  A node with more than 8 elements takes several such passes.
+// Per wave and pass over up to 8 incident elements (slots s): for the general
+// form, lanes (s, q) evaluate the geometry of slot s at quadrature point q
+// (Jacobian, its inverse, the 8 shape gradients) into LDS; then lanes (s, b)
+// form block (a_s, b) of slot s, its residual contribution K_ab u_b (before
+// constraints, as the reference's element vector) and its constraint
+// treatment (zero constrained rows and columns, unit diagonal per element),
+// and stage it in LDS with the position of neighbour b in the row.  Each lane
+// then adds, for the row values it owns, the staged blocks of the slots that
+// touch that neighbour, in slot order, into registers.  The row's values
+// (rows r*3..r*3+2 of node r: one chunk ordered (v, block, w)) are written
+// once, coalesced, by their owners.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "device_util.h"
 #include "kernels.h"
+
+#ifndef MASTG_ROW_GRID
+#define MASTG_ROW_GRID (int64_t(32768))
+#endif
 
 namespace mastg {
 
 namespace {
 
-constexpr int kRowMaxBlocks = 64;   // neighbour blocks per row the LDS accumulator holds
+constexpr int kRowMaxBlocks = 64;   // neighbour blocks per row (9 row values per lane)
+constexpr int kRowSmallBlocks = 28;  // rows of up to 28 blocks: 4 values per lane, fewer registers
 
 __device__ __forceinline__ double ra_gp(int i) { return i ? 0.57735026918962576451 : -0.57735026918962576451; }
 
+// a value that is the same in every lane, moved to scalar registers
+__device__ __forceinline__ int uniform_i(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ double uniform_d(double x) {
+    const uint64_t u = uint64_t(__double_as_longlong(x));
+    const uint32_t lo = uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(u))));
+    const uint32_t hi = uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(u >> 32))));
+    return __longlong_as_double((long long)((uint64_t(hi) << 32) | lo));
+}
+
+template <int NV>
 __global__ __launch_bounds__(64) void k_hex8_row_assemble(Hex8RowArgs A) {
-    __shared__ double g[8][8][8][3];          // [slot][q][node][d] shape gradients
-    __shared__ double cq[8][8];               // [slot][q] JxW
-    __shared__ double acc[kRowMaxBlocks * 9];  // [block][v][w] row accumulator
-    __shared__ int32_t bcol[kRowMaxBlocks];
+    __shared__ double g[8][8][8][3];                  // [slot][q][node][d] shape gradients
+    __shared__ double cq[8][8];                       // [slot][q] JxW
+    __shared__ double kst[64 * 9];                    // [slot * 8 + b][v * 3 + w] staged element blocks
+    __shared__ int8_t cont[NV * 64 / 9 * 8];          // [block][slot] local node b, or -1
+    __shared__ int32_t bcol[NV * 64 / 9];
     __shared__ int32_t selem[8], sla[8];
+    __shared__ int sdiag;
     const int lane = threadIdx.x;
     const int64_t r = blockIdx.x;
     const int64_t b0 = A.blk_ptr[r];
     const int deg = int(A.blk_ptr[r + 1] - b0);
     const int32_t i0 = A.ninc_ptr[r], ne = A.ninc_ptr[r + 1] - i0;
-    if (lane < deg) bcol[lane] = A.blk_col[b0 + lane];
-    for (int t = lane; t < deg * 9; t += 64) acc[t] = 0.0;
+    if (lane == 0) sdiag = -1;
+    __syncthreads();
+    if (lane < deg) {
+        const int32_t cb = A.blk_col[b0 + lane];
+        bcol[lane] = cb;
+        if (cb == int32_t(r)) sdiag = lane;
+    }
+    __syncthreads();   // bcol and sdiag are read below (also by a node without elements)
+    // the row values this lane owns: t = lane + 64 j in the chunk's (v, block,
+    // w) order, decoded once into block * 16 + v * 3 + w
+    const int n3 = max(3 * deg, 1), nval = 9 * deg;
+    double acc[NV];
+    int eix[NV];
+#pragma unroll
+    for (int j = 0; j < NV; j++) {
+        const int t = lane + 64 * j;
+        const int v = t / n3, rem = t - v * n3, kb = rem / 3, w = rem - kb * 3;
+        eix[j] = kb * 16 + v * 3 + w;
+        acc[j] = 0.0;
+    }
     double rv0 = 0.0, rv1 = 0.0, rv2 = 0.0;   // residual of row r from this lane's blocks
     const double lam = A.lam, mu = A.mu;
     for (int c0 = 0; c0 < ne; c0 += 8) {
@@ -63,9 +106,10 @@ __global__ __launch_bounds__(64) void k_hex8_row_assemble(Hex8RowArgs A) {
             selem[lane] = inc >> 3;
             sla[lane] = inc & 7;
         }
+        for (int t = lane; t < deg * 8; t += 64) cont[t] = int8_t(-1);
         __syncthreads();
-        // ---- geometry: lane = (s, q)
         {
+            // ---- geometry: lane = (s, q)
             const int s = lane >> 3, q = lane & 7;
             if (s < ns) {
                 const int64_t e = selem[s];
@@ -109,33 +153,34 @@ __global__ __launch_bounds__(64) void k_hex8_row_assemble(Hex8RowArgs A) {
                 }
                 cq[s][q] = det;
             }
+            __syncthreads();
         }
-        __syncthreads();
         // ---- blocks: lane = (s, b)
         const int s = lane >> 3, b = lane & 7;
-        double K[9];
-        int k = -1;
         if (s < ns) {
             const int a = sla[s];
             const int64_t e = selem[s];
-            double P[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, S = 0.0;
-#pragma unroll
-            for (int q = 0; q < 8; q++) {
-                const double c = cq[s][q];
-                const double ga0 = c * g[s][q][a][0], ga1 = c * g[s][q][a][1], ga2 = c * g[s][q][a][2];
-                const double gb0 = g[s][q][b][0], gb1 = g[s][q][b][1], gb2 = g[s][q][b][2];
-                P[0] += ga0 * gb0; P[1] += ga0 * gb1; P[2] += ga0 * gb2;
-                P[3] += ga1 * gb0; P[4] += ga1 * gb1; P[5] += ga1 * gb2;
-                P[6] += ga2 * gb0; P[7] += ga2 * gb1; P[8] += ga2 * gb2;
-                S += ga0 * gb0 + ga1 * gb1 + ga2 * gb2;
-            }
-            const double Ee = A.E0 * pow(A.param[e], A.penalty);
-#pragma unroll
-            for (int i = 0; i < 3; i++)
-#pragma unroll
-                for (int j = 0; j < 3; j++)
-                    K[i * 3 + j] = Ee * (lam * P[i * 3 + j] + mu * P[j * 3 + i] + (i == j ? mu * S : 0.0));
             const int32_t rb = A.econn[e * 8 + b];
+            const double Ee = A.sE[e];
+            double K[9];
+            {
+                double P[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, S = 0.0;
+#pragma unroll
+                for (int q = 0; q < 8; q++) {
+                    const double c = cq[s][q];
+                    const double ga0 = c * g[s][q][a][0], ga1 = c * g[s][q][a][1], ga2 = c * g[s][q][a][2];
+                    const double gb0 = g[s][q][b][0], gb1 = g[s][q][b][1], gb2 = g[s][q][b][2];
+                    P[0] += ga0 * gb0; P[1] += ga0 * gb1; P[2] += ga0 * gb2;
+                    P[3] += ga1 * gb0; P[4] += ga1 * gb1; P[5] += ga1 * gb2;
+                    P[6] += ga2 * gb0; P[7] += ga2 * gb1; P[8] += ga2 * gb2;
+                    S += ga0 * gb0 + ga1 * gb1 + ga2 * gb2;
+                }
+#pragma unroll
+                for (int i = 0; i < 3; i++)
+#pragma unroll
+                    for (int j = 0; j < 3; j++)
+                        K[i * 3 + j] = Ee * (lam * P[i * 3 + j] + mu * P[j * 3 + i] + (i == j ? mu * S : 0.0));
+            }
             if (A.X) {
                 const double u0 = A.X[int64_t(rb) * 3], u1 = A.X[int64_t(rb) * 3 + 1], u2 = A.X[int64_t(rb) * 3 + 2];
                 rv0 += K[0] * u0 + K[1] * u1 + K[2] * u2;
@@ -158,18 +203,27 @@ __global__ __launch_bounds__(64) void k_hex8_row_assemble(Hex8RowArgs A) {
                 if (bcol[mid] < rb) lo = mid + 1;
                 else hi = mid;
             }
-            k = lo;
-        }
-        // add the blocks into the row in slot order (within a slot the 8
-        // blocks are distinct)
-        for (int t = 0; t < ns; t++) {
-            if (s == t)
 #pragma unroll
-                for (int i = 0; i < 9; i++) acc[k * 9 + i] += K[i];
-            __syncthreads();
+            for (int i = 0; i < 9; i++) kst[lane * 9 + i] = K[i];
+            cont[lo * 8 + s] = int8_t(b);   // an element's 8 nodes are distinct: one b per (block, slot)
         }
+        __syncthreads();
+        // ---- the owned row values add the staged blocks in slot order
+#pragma unroll
+        for (int j = 0; j < NV; j++) {
+            if (lane + 64 * j < nval) {
+                const int8_t* cb = cont + (eix[j] >> 4) * 8;
+                const double* kj = kst + (eix[j] & 15);
+                double sum = acc[j];
+                for (int t = 0; t < ns; t++) {
+                    const int bb = cb[t];
+                    if (bb >= 0) sum += kj[(t * 8 + bb) * 9];
+                }
+                acc[j] = sum;
+            }
+        }
+        __syncthreads();   // the next pass rewrites the staged blocks and slots
     }
-    __syncthreads();   // the accumulator is complete (also for a node without elements)
     // residual of row r: fixed-order reduction over the wave
 #pragma unroll
     for (int m = 1; m < 64; m <<= 1) {
@@ -183,17 +237,194 @@ __global__ __launch_bounds__(64) void k_hex8_row_assemble(Hex8RowArgs A) {
         if (A.F && !(A.con && A.con[i])) v -= A.F[i];
         A.R[i] = (A.con && A.con[i]) ? 0.0 : v;
     }
-    // the row's values: (v, block, w) order, one contiguous chunk
+    // the row's values, written by their owners (consecutive lanes, consecutive addresses)
     const int64_t start = A.row_ptr[r * 3];
-    const int n3 = 3 * deg;
-    for (int t = lane; t < 3 * n3; t += 64) {
-        const int v = t / n3, rem = t - v * n3, kb = rem / 3, w = rem - kb * 3;
-        A.val[start + t] = acc[kb * 9 + v * 3 + w];
+    const int kd = sdiag;
+#pragma unroll
+    for (int j = 0; j < NV; j++) {
+        if (lane + 64 * j < nval) {
+            A.val[start + lane + 64 * j] = acc[j];
+            const int iw = eix[j] & 15;
+            if (A.dinv && (eix[j] >> 4) == kd && iw % 4 == 0) A.dinv[r * 3 + iw / 4] = 1.0 / acc[j];
+        }
     }
-    if (A.dinv && lane < 3) {
-        int kd = 0;
-        while (kd < deg && bcol[kd] != int32_t(r)) kd++;
-        A.dinv[r * 3 + lane] = kd < deg ? 1.0 / acc[kd * 9 + lane * 4] : 0.0;
+    if (A.dinv && kd < 0 && lane < 3) A.dinv[r * 3 + lane] = 0.0;
+}
+
+// The same assembly when the elements fall into congruence classes (K_e =
+// sE[e] Khat_class): the constraint convention depends only on the (row,
+// column) variable pair, not on the element, so each owned row value sums the
+// unconstrained contributions sE[e] Khat[class][a_s][b][v][w] of the slots
+// touching its neighbour (slot order), counts them, and is replaced once at
+// the end by the convention's value (the count on the diagonal, 0 elsewhere)
+// when its row or column variable is constrained -- the same value, bit for
+// bit, as masking each element block before the sum.  The residual row is
+// the unconstrained row times the state.  The class matrices are read from
+// LDS: with one class the whole matrix, loaded once per work-group (coalesced);
+// with several, per pass the 8 rows Khat[class_s][a_s][:] of the slots, loaded
+// by the lanes (s, b) that place neighbour b of slot s in the row.  Work-groups
+// run over nodes with a grid stride, so the one-class load is amortised.
+template <int NP, bool ONE>
+__global__ __launch_bounds__(64) void k_hex8_row_assemble_class(Hex8RowArgs A) {
+    // a lane owns NP (row variable v, neighbour block k) pairs p = v deg + k,
+    // i.e. the 3 consecutive row values 3 p .. 3 p + 2 (w = 0, 1, 2)
+    constexpr int MAXB = NP * 64 / 3;
+    __shared__ double kh[576];                   // ONE: Khat[a][b][v][w]; else [slot][b][v][w] rows
+    __shared__ __align__(8) int8_t cont[MAXB * 8];  // [block][slot] local node b, or -1
+    __shared__ int32_t bcol[MAXB];
+    __shared__ double sEe[8];
+    __shared__ int32_t sko[8];                   // kh offset of slot s's row: a_s * 72 (ONE) or s * 72
+    __shared__ int sdiag;
+    const int lane = threadIdx.x;
+    if (ONE)
+        for (int t = lane; t < 576; t += 64) kh[t] = A.Khat[t];
+    // grid-stride over nodes (the launch caps the grid)
+    for (int64_t r = blockIdx.x; r < A.n_nodes; r += gridDim.x) {
+        const int64_t b0 = A.blk_ptr[r];
+        const int deg = int(A.blk_ptr[r + 1] - b0);
+        const int32_t i0 = A.ninc_ptr[r], ne = A.ninc_ptr[r + 1] - i0;
+        if (lane == 0) sdiag = -1;
+        __syncthreads();
+        if (lane < deg) {
+            const int32_t cb = A.blk_col[b0 + lane];
+            bcol[lane] = cb;
+            if (cb == int32_t(r)) sdiag = lane;
+        }
+        const int npair = 3 * deg, dg = max(deg, 1);
+        double acc[NP][3];
+        int pk[NP], pv[NP], cnt[NP];
+#pragma unroll
+        for (int j = 0; j < NP; j++) {
+            const int p = lane + 64 * j;
+            pv[j] = p / dg;
+            pk[j] = p - pv[j] * dg;
+            cnt[j] = 0;
+            acc[j][0] = acc[j][1] = acc[j][2] = 0.0;
+        }
+        __syncthreads();   // bcol and sdiag
+        // the neighbours' state and constraint flags are loaded now, in flight
+        // while the element slots are placed
+        double xs[NP][3];
+        uint8_t cw[NP];
+#pragma unroll
+        for (int j = 0; j < NP; j++) {
+            xs[j][0] = xs[j][1] = xs[j][2] = 0.0;
+            cw[j] = 0;
+            if (lane + 64 * j < npair) {
+                const int64_t col = int64_t(bcol[pk[j]]) * 3;
+                if (A.X) {
+                    xs[j][0] = A.X[col];
+                    xs[j][1] = A.X[col + 1];
+                    xs[j][2] = A.X[col + 2];
+                }
+                if (A.con) cw[j] = uint8_t(A.con[col] | (A.con[col + 1] << 1) | (A.con[col + 2] << 2));
+            }
+        }
+        for (int c0 = 0; c0 < ne; c0 += 8) {
+            const int ns = min(8, ne - c0);
+            for (int t = lane; t < deg * 8; t += 64) cont[t] = int8_t(-1);
+            __syncthreads();   // bcol (first pass) and the cleared slots; the previous pass's readers are done
+            {
+                const int s = lane >> 3, b = lane & 7;
+                if (s < ns) {
+                    const int32_t inc = A.ninc[i0 + c0 + s];
+                    const int64_t e = inc >> 3;
+                    const int a = inc & 7;
+                    const int32_t rb = A.econn[e * 8 + b];
+                    if (b == 0) {
+                        sEe[s] = A.sE[e];
+                        sko[s] = (ONE ? a : s) * 72;
+                    }
+                    if (!ONE) {
+                        const double* __restrict__ src = A.Khat + (int64_t(A.eclass[e]) * 64 + a * 8 + b) * 9;
+#pragma unroll
+                        for (int i = 0; i < 9; i++) kh[(s * 8 + b) * 9 + i] = src[i];
+                    }
+                    int lo = 0, hi = deg - 1;
+                    while (lo < hi) {
+                        const int mid = (lo + hi) >> 1;
+                        if (bcol[mid] < rb) lo = mid + 1;
+                        else hi = mid;
+                    }
+                    cont[lo * 8 + s] = int8_t(b);   // an element's 8 nodes are distinct: one b per (block, slot)
+                }
+            }
+            __syncthreads();
+            // slot scales and row offsets are the same for every lane (scalar registers)
+            double se[8];
+            int so[8];
+#pragma unroll
+            for (int t = 0; t < 8; t++) {
+                se[t] = uniform_d(t < ns ? sEe[t] : 0.0);
+                so[t] = uniform_i(t < ns ? sko[t] : 0);
+            }
+#pragma unroll
+            for (int j = 0; j < NP; j++) {
+                if (lane + 64 * j < npair) {
+                    const uint64_t row = *reinterpret_cast<const uint64_t*>(cont + pk[j] * 8);
+                    const int vo = pv[j] * 3;
+                    double s0 = acc[j][0], s1 = acc[j][1], s2 = acc[j][2];
+                    int n = cnt[j];
+                    // slot order, branch-free: a slot that does not touch the
+                    // neighbour reads an in-bounds row with the scale 0 (adds an
+                    // exact zero)
+#pragma unroll
+                    for (int t = 0; t < 8; t++) {
+                        const int bb = int(int8_t((row >> (8 * t)) & 0xff));
+                        const bool on = t < ns && bb >= 0;
+                        const double* k3 = kh + so[t] + (on ? bb : 0) * 9 + vo;
+                        const double m = on ? se[t] : 0.0;
+                        s0 = fma(m, k3[0], s0);
+                        s1 = fma(m, k3[1], s1);
+                        s2 = fma(m, k3[2], s2);
+                        n += on ? 1 : 0;
+                    }
+                    acc[j][0] = s0;
+                    acc[j][1] = s1;
+                    acc[j][2] = s2;
+                    cnt[j] = n;
+                }
+            }
+            __syncthreads();   // the next pass rewrites the slots
+        }
+        const int kd = sdiag;
+        double rv0 = 0.0, rv1 = 0.0, rv2 = 0.0;
+        const int64_t start = A.row_ptr[r * 3];
+        const uint8_t cr = A.con ? uint8_t(A.con[r * 3] | (A.con[r * 3 + 1] << 1) | (A.con[r * 3 + 2] << 2)) : 0;
+#pragma unroll
+        for (int j = 0; j < NP; j++) {
+            const int p = lane + 64 * j;
+            if (p < npair) {
+                const int k = pk[j], v = pv[j];
+                if (A.X) {
+                    const double q = acc[j][0] * xs[j][0] + acc[j][1] * xs[j][1] + acc[j][2] * xs[j][2];
+                    rv0 += v == 0 ? q : 0.0;
+                    rv1 += v == 1 ? q : 0.0;
+                    rv2 += v == 2 ? q : 0.0;
+                }
+#pragma unroll
+                for (int w = 0; w < 3; w++) {
+                    double val = acc[j][w];
+                    if (A.con && (((cr >> v) & 1) || ((cw[j] >> w) & 1))) val = (k == kd && v == w) ? double(cnt[j]) : 0.0;
+                    A.val[start + 3 * p + w] = val;
+                    if (A.dinv && k == kd && v == w) A.dinv[r * 3 + v] = 1.0 / val;
+                }
+            }
+        }
+        if (A.dinv && kd < 0 && lane < 3) A.dinv[r * 3 + lane] = 0.0;
+#pragma unroll
+        for (int m = 1; m < 64; m <<= 1) {
+            rv0 += __shfl_xor(rv0, m, 64);
+            rv1 += __shfl_xor(rv1, m, 64);
+            rv2 += __shfl_xor(rv2, m, 64);
+        }
+        if (A.R && lane < 3) {
+            const int64_t i = r * 3 + lane;
+            double v = lane == 0 ? rv0 : lane == 1 ? rv1 : rv2;
+            if (A.F && !(A.con && A.con[i])) v -= A.F[i];
+            A.R[i] = (A.con && A.con[i]) ? 0.0 : v;
+        }
+        __syncthreads();   // the shared arrays are rewritten for the next node
     }
 }
 
@@ -201,9 +432,25 @@ __global__ __launch_bounds__(64) void k_hex8_row_assemble(Hex8RowArgs A) {
 
 int hex8_row_assembly_max_blocks() { return kRowMaxBlocks; }
 
-void launch_hex8_row_assemble(const Hex8RowArgs& a, hipStream_t s) {
+void launch_hex8_row_assemble(const Hex8RowArgs& a, int max_blocks_per_row, hipStream_t s) {
     if (a.n_nodes <= 0) return;
-    hipLaunchKernelGGL(k_hex8_row_assemble, dim3(unsigned(a.n_nodes)), dim3(64), 0, s, a);
+    const dim3 g(unsigned(a.n_nodes));
+    const bool small = max_blocks_per_row <= kRowSmallBlocks;
+    if (a.Khat && a.eclass) {
+        const dim3 gc(unsigned(std::min<int64_t>(a.n_nodes, MASTG_ROW_GRID)));
+        // pairs per lane: 2 up to 42 blocks per row, 3 up to 64
+        const bool two = max_blocks_per_row <= 42;
+        if (a.n_classes == 1) {
+            if (two) hipLaunchKernelGGL((k_hex8_row_assemble_class<2, true>), gc, dim3(64), 0, s, a);
+            else hipLaunchKernelGGL((k_hex8_row_assemble_class<3, true>), gc, dim3(64), 0, s, a);
+        } else {
+            if (two) hipLaunchKernelGGL((k_hex8_row_assemble_class<2, false>), gc, dim3(64), 0, s, a);
+            else hipLaunchKernelGGL((k_hex8_row_assemble_class<3, false>), gc, dim3(64), 0, s, a);
+        }
+    } else {
+        if (small) hipLaunchKernelGGL((k_hex8_row_assemble<4>), g, dim3(64), 0, s, a);
+        else hipLaunchKernelGGL((k_hex8_row_assemble<9>), g, dim3(64), 0, s, a);
+    }
 }
 
 }  // namespace mastg
