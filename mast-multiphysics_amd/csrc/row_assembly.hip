@@ -73,7 +73,7 @@ __global__ __launch_bounds__(64) void k_hex8_row_assemble(Hex8RowArgs A) {
     __shared__ int32_t selem[8], sla[8];
     __shared__ int sdiag;
     const int lane = threadIdx.x;
-    const int64_t r = blockIdx.x;
+    const int64_t r = xcd_block(blockIdx.x, gridDim.x);   // consecutive nodes on one XCD (shared L2)
     const int64_t b0 = A.blk_ptr[r];
     const int deg = int(A.blk_ptr[r + 1] - b0);
     const int32_t i0 = A.ninc_ptr[r], ne = A.ninc_ptr[r + 1] - i0;
@@ -262,8 +262,9 @@ __global__ __launch_bounds__(64) void k_hex8_row_assemble(Hex8RowArgs A) {
 // the unconstrained row times the state.  The class matrices are read from
 // LDS: with one class the whole matrix, loaded once per work-group (coalesced);
 // with several, per pass the 8 rows Khat[class_s][a_s][:] of the slots, loaded
-// by the lanes (s, b) that place neighbour b of slot s in the row.  Work-groups
-// run over nodes with a grid stride, so the one-class load is amortised.
+// by the lanes (s, b) that place neighbour b of slot s in the row.  Each
+// work-group runs over a contiguous range of nodes, so the one-class load is
+// amortised.
 template <int NP, bool ONE>
 __global__ __launch_bounds__(64) void k_hex8_row_assemble_class(Hex8RowArgs A) {
     // a lane owns NP (row variable v, neighbour block k) pairs p = v deg + k,
@@ -278,8 +279,12 @@ __global__ __launch_bounds__(64) void k_hex8_row_assemble_class(Hex8RowArgs A) {
     const int lane = threadIdx.x;
     if (ONE)
         for (int t = lane; t < 576; t += 64) kh[t] = A.Khat[t];
-    // grid-stride over nodes (the launch caps the grid)
-    for (int64_t r = blockIdx.x; r < A.n_nodes; r += gridDim.x) {
+    // a contiguous run of nodes per work-group, the runs of one XCD adjacent
+    // (work-groups are dealt to the 8 XCDs in turn): neighbouring rows gather
+    // the same state and flags through the same L2
+    const int64_t nb = gridDim.x, per = (A.n_nodes + nb - 1) / nb;
+    const int64_t rbeg = xcd_block(blockIdx.x, nb) * per, rend = min(A.n_nodes, rbeg + per);
+    for (int64_t r = rbeg; r < rend; r++) {
         const int64_t b0 = A.blk_ptr[r];
         const int deg = int(A.blk_ptr[r + 1] - b0);
         const int32_t i0 = A.ninc_ptr[r], ne = A.ninc_ptr[r + 1] - i0;
