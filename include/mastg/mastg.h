@@ -124,6 +124,20 @@ int mastg_get_csr_pattern(const mastg_ctx* ctx, int64_t* row_ptr, int32_t* col_i
 /* active index of each caller DOF (-1 for inactive variables), [n_dofs] */
 int mastg_get_active_map(const mastg_ctx* ctx, int64_t* active_of_dof);
 
+/* The MAST-exact matrix of the 6-variable structural system after the last
+ * FP64 assembly, as MAST's residual_and_jacobian leaves it in the PETSc
+ * matrix (src/base/nonlinear_implicit_assembly.cpp:111-186): every caller DOF
+ * a row, libMesh's full-coupling pattern over all 6 variables of the nodes,
+ * the assembled values on the active variables, and on the others the small
+ * diagonal MAST adds -- hex8: 1e-20 x each element matrix's largest diagonal
+ * entry on the rotations (src/elasticity/solid_element_3d.cpp:413-417),
+ * plate: 1e-8 per element on theta_z (src/elasticity/structural_element_2d.cpp:
+ * 1215-1219) -- with libMesh's constraint convention on constrained DOFs.
+ * mastg_csr_full_size gives the row count (n_dofs) and nnz; the export
+ * writes host arrays row_ptr [n_rows+1], col_idx [nnz], val [nnz]. */
+int mastg_csr_full_size(mastg_ctx* ctx, int64_t* n_rows, int64_t* nnz);
+int mastg_export_csr_full(mastg_ctx* ctx, int64_t* row_ptr, int32_t* col_idx, double* val);
+
 /* Per-element design parameter (rho_e / h_e / q_e), [n_elems]; `where` =
  * MASTG_HOST or MASTG_DEVICE.  Replaces the per-element FieldFunction
  * (examples/structural/example_6/example_6.cpp:100-137). */

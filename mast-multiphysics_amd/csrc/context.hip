@@ -2081,6 +2081,100 @@ int ks_value(mastg_ctx* c, double* value) {
 
 extern "C" {
 
+int mastg_csr_full_size(mastg_ctx* c, int64_t* n_rows, int64_t* nnz) {
+    if (!c || !n_rows || !nnz) return MASTG_ERR_INVALID;
+    if ((c->d.physics != MASTG_HEX8_ELASTIC && c->d.physics != MASTG_QUAD4_MINDLIN) || c->S.n_vars != 6)
+        return fail(c, MASTG_ERR_INVALID, "the full-variable matrix is the 6-variable structural system's");
+    *n_rows = c->S.n_dofs;
+    *nnz = c->S.n_blocks * 36;
+    return MASTG_OK;
+}
+
+int mastg_export_csr_full(mastg_ctx* c, int64_t* row_ptr, int32_t* col, double* val) {
+    if (!c || !row_ptr || !col || !val) return MASTG_ERR_INVALID;
+    int64_t n_rows = 0, nnz = 0;
+    TRY(mastg_csr_full_size(c, &n_rows, &nnz));
+    if (!c->assembled) return fail(c, MASTG_ERR_INVALID, "no assembly: call mastg_assemble first");
+    const Structure& S = c->S;
+    const int nv = 6, nav = S.nav;
+    const bool hex = c->d.physics == MASTG_HEX8_ELASTIC;
+    // rows in the caller's numbering: node r's 6 rows at base(r) + v, each
+    // with 6 columns per neighbour block
+    std::vector<int64_t> rp(size_t(n_rows) + 1, 0);
+    for (int64_t r = 0; r < S.n_nodes; r++) {
+        const int64_t deg = S.blk_ptr[r + 1] - S.blk_ptr[r];
+        for (int v = 0; v < nv; v++) {
+            const int64_t d = S.node_base_r[r] + v;
+            if (d < 0 || d >= n_rows) return fail(c, MASTG_ERR_INVALID, "node DOFs outside the DOF range");
+            rp[size_t(d) + 1] = deg * nv;
+        }
+    }
+    for (int64_t i = 0; i < n_rows; i++) rp[size_t(i) + 1] += rp[size_t(i)];
+    if (rp[size_t(n_rows)] != nnz) return fail(c, MASTG_ERR_INVALID, "the nodes' DOFs overlap");
+    FullExportArgs a{};
+    a.n_nodes = S.n_nodes;
+    a.nav = nav;
+    a.nv = nv;
+    uint32_t inactive = 0;
+    for (int v = 0; v < nv; v++) {
+        a.act[v] = -1;
+        for (int k = 0; k < nav; k++)
+            if (S.active_var[k] == v) a.act[v] = int8_t(k);
+        if (a.act[v] < 0) inactive |= 1u << v;
+    }
+    DevBuf rpf, colf, valf, diag, base, conv, mxd;
+    auto release_all = [&]() {
+        for (DevBuf* b : {&rpf, &colf, &valf, &diag, &base, &conv, &mxd}) {
+            c->device_bytes -= int64_t(b->bytes);
+            b->release();
+        }
+    };
+    int rc = upload(c, rpf, rp);
+    if (!rc) rc = upload(c, base, S.node_base_r);
+    if (!rc) rc = upload(c, conv, S.constrained_var);
+    if (!rc) rc = dalloc(c, colf, size_t(nnz) * sizeof(int32_t));
+    if (!rc) rc = dalloc(c, valf, size_t(nnz) * sizeof(double));
+    if (!rc) rc = dalloc(c, diag, size_t(S.n_nodes) * nv * sizeof(double));
+    if (!rc && hex) {
+        // the rotations' diagonal needs each element matrix's largest diagonal entry
+        if (!c->sE.p) rc = dalloc(c, c->sE, size_t(S.n_elems) * sizeof(double));
+        if (!rc) rc = dalloc(c, mxd, size_t(S.n_elems) * sizeof(double));
+        if (!rc) {
+            ensure_scales(c);
+            launch_hex8_max_diag(S.n_elems, c->xyz.as<double>(), c->econn.as<int32_t>(), c->sE.as<double>(),
+                                 c->Dhat[1], c->Dhat[21], mxd.as<double>(), c->stream);
+        }
+    }
+    if (rc) {
+        release_all();
+        return rc;
+    }
+    launch_inactive_diag(S.n_nodes, nv, S.npe, inactive, c->ninc_ptr.as<int32_t>(), c->ninc.as<int32_t>(),
+                         hex ? mxd.as<double>() : nullptr, hex ? 1.0e-20 : 1.0e-8, conv.as<uint8_t>(),
+                         diag.as<double>(), c->stream);
+    a.blk_ptr = c->blk_ptr.as<int64_t>();
+    a.blk_col = c->blk_col.as<int32_t>();
+    a.row_ptr_a = c->row_ptr.as<int64_t>();
+    a.val_a = c->val.as<double>();
+    a.diag = diag.as<double>();
+    a.base_r = base.as<int64_t>();
+    a.row_ptr_f = rpf.as<int64_t>();
+    a.col_f = colf.as<int32_t>();
+    a.val_f = valf.as<double>();
+    launch_export_full(a, c->stream);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) {
+        HipLock l(g_hip_mu);
+        e = hipMemcpy(col, colf.p, size_t(nnz) * sizeof(int32_t), hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(val, valf.p, size_t(nnz) * sizeof(double), hipMemcpyDeviceToHost);
+    }
+    release_all();
+    if (e != hipSuccess) return fail(c, MASTG_ERR_DEVICE, std::string("export: ") + hipGetErrorString(e));
+    std::copy(rp.begin(), rp.end(), row_ptr);
+    return MASTG_OK;
+}
+
 int mastg_output_sensitivity(mastg_ctx* c, int kind, const double* X, int where, double* dJdp) {
     if (!c || !X || !dJdp) return MASTG_ERR_INVALID;
     double* gd = where == MASTG_HOST ? c->vg.as<double>() : dJdp;

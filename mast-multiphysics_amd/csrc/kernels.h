@@ -81,6 +81,28 @@ struct Hex8RowArgs {
 int hex8_row_assembly_max_blocks();
 void launch_hex8_row_assemble(const Hex8RowArgs& a, int max_blocks_per_row, hipStream_t s);
 
+// the MAST-exact full-variable matrix from the active assembly (export.hip)
+struct FullExportArgs {
+    int64_t n_nodes;
+    int nav, nv;                     // active and caller variables per node
+    int8_t act[6];                   // caller variable -> active index, -1 inactive
+    const int64_t* blk_ptr;
+    const int32_t* blk_col;
+    const int64_t* row_ptr_a;        // active scalar CSR
+    const double* val_a;
+    const double* diag;              // [n_nodes*nv] inactive-variable diagonal
+    const int64_t* base_r;           // [n_nodes] caller DOF of (rank, variable 0)
+    const int64_t* row_ptr_f;        // [n_dofs+1] full-variable rows, caller numbering
+    int32_t* col_f;
+    double* val_f;
+};
+void launch_hex8_max_diag(int64_t n_elems, const double* xyz, const int32_t* econn, const double* sE, double lam,
+                          double mu, double* out, hipStream_t s);
+void launch_inactive_diag(int64_t n_nodes, int nv, int npe, uint32_t inactive, const int32_t* ninc_ptr,
+                          const int32_t* ninc, const double* per_elem, double coef, const uint8_t* con_var,
+                          double* diag, hipStream_t s);
+void launch_export_full(const FullExportArgs& a, hipStream_t s);
+
 struct ScatterArgs {
     int64_t n_nodes;
     int nav;

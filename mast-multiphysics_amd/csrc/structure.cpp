@@ -202,12 +202,14 @@ std::string build_structure(int physics, int64_t n_nodes, int64_t n_elems, const
             const int32_t* b1 = S.blk_col.data() + S.blk_ptr[c + 1];
             S.blk_transpose[k] = int32_t(std::lower_bound(b0, b1, int32_t(r)) - S.blk_col.data());
         }
-    // ---- Dirichlet mask over active DOFs
+    // ---- Dirichlet mask over active DOFs (and over every caller variable)
     S.constrained.assign(S.n_active, 0);
+    S.node_base_r.resize(n_nodes);
+    for (int64_t r = 0; r < n_nodes; r++) S.node_base_r[r] = base[S.node_of_rank[r]];
+    S.constrained_var.assign(size_t(n_nodes) * n_vars, 0);
     if (n_constrained > 0 && constrained_dofs) {
         // caller dof -> active index via the node ranks
-        std::vector<int64_t> sorted_base(n_nodes);
-        for (int64_t r = 0; r < n_nodes; r++) sorted_base[r] = base[S.node_of_rank[r]];
+        const std::vector<int64_t>& sorted_base = S.node_base_r;
         for (int64_t i = 0; i < n_constrained; i++) {
             const int64_t d = constrained_dofs[i];
             auto it = std::upper_bound(sorted_base.begin(), sorted_base.end(), d);
@@ -215,6 +217,7 @@ std::string build_structure(int physics, int64_t n_nodes, int64_t n_elems, const
             const int64_t r = int64_t(it - sorted_base.begin()) - 1;
             const int64_t var = d - sorted_base[r];
             if (var < 0 || var >= n_vars) return "constrained DOF out of range";
+            S.constrained_var[size_t(r) * n_vars + var] = 1;
             for (int v = 0; v < nav; v++)
                 if (S.active_var[v] == var) S.constrained[r * nav + v] = 1;
         }

@@ -22,6 +22,8 @@ struct Structure {
     std::vector<int32_t> rank_of_node;     // [n_nodes]
     std::vector<int32_t> node_of_rank;     // [n_nodes]
     std::vector<int64_t> caller_dof_of_active;  // [n_active]
+    std::vector<int64_t> node_base_r;      // [n_nodes] caller DOF of (rank, variable 0)
+    std::vector<uint8_t> constrained_var;  // [n_nodes*n_vars] every caller variable, by rank
 
     std::vector<double> xyz_r;             // [n_nodes*3] coordinates by rank
     std::vector<int32_t> econn;            // [n_elems*npe] element node ranks

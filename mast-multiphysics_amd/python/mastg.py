@@ -109,6 +109,8 @@ def lib():
             "mastg_jacobian_product": (C.c_int, [_P, _D, _D, C.c_int, C.c_int]),
             "mastg_output": (C.c_int, [_P, C.c_int, _D, C.c_int, _D, _D]),
             "mastg_output_noalloc": (C.c_int, [_P, C.c_int, C.c_void_p, C.c_void_p]),
+            "mastg_csr_full_size": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+            "mastg_export_csr_full": (C.c_int, [_P, C.c_void_p, C.c_void_p, C.c_void_p]),
             "mastg_set_ks_stress": (C.c_int, [_P, C.c_double, C.c_double]),
             "mastg_output_sensitivity": (C.c_int, [_P, C.c_int, _D, C.c_int, _D]),
             "mastg_device_alloc": (C.c_int, [_P, C.c_size_t, C.POINTER(C.c_void_p)]),
@@ -408,6 +410,17 @@ class Context:
         ms = C.c_double()
         self._chk(lib().mastg_bench_kernel(self.h, kind, n_launch, C.byref(ms)))
         return ms.value
+
+    def export_csr_full(self):
+        """The MAST-exact 6-variable matrix of the last FP64 assembly:
+        (row_ptr, col, val) in the caller's numbering."""
+        n, nnz = C.c_int64(), C.c_int64()
+        self._chk(lib().mastg_csr_full_size(self.h, C.byref(n), C.byref(nnz)))
+        rp = np.zeros(n.value + 1, np.int64)
+        col = np.zeros(nnz.value, np.int32)
+        val = np.zeros(nnz.value, np.float64)
+        self._chk(lib().mastg_export_csr_full(self.h, rp.ctypes.data, col.ctypes.data, val.ctypes.data))
+        return rp, col, val
 
     def set_option(self, option, value):
         self._chk(lib().mastg_set_option(self.h, option, value))
