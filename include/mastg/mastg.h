@@ -82,7 +82,8 @@ typedef struct {
     /* material / section / boundary data */
     double E, nu, penalty;       /* elasticity; SIMP penalty p (E_e = rho^p E)         */
     double kappa, offset;        /* Mindlin section shear factor, offset (0 supported) */
-    double pressure;             /* Mindlin uniform pressure along local -z            */
+    double pressure;             /* Mindlin: uniform pressure along local -z; hex8: on
+                                    the sides in pres_bid_mask (below)               */
     double k0, alpha;            /* conduction k(T) = k0 (1 + alpha T)                 */
     double conv_h, conv_tinf;    /* convection coefficient / ambient temperature       */
     uint32_t conv_bid_mask;      /* boundary ids carrying convection (bit mask)        */
@@ -95,6 +96,12 @@ typedef struct {
      * reference_zero_temperature and stefan_bolzmann_constant parameters */
     double rad_emissivity, rad_tamb, rad_tzero, rad_sigma;
     uint32_t rad_bid_mask;       /* boundary ids carrying radiation (bit mask)         */
+    /* hex8 elasticity: uniform pressure on the sides whose boundary id is in
+     * pres_bid_mask, R -= int N (p n) dA with n the side's outward normal --
+     * StructuralElement3D::surface_pressure_residual
+     * (src/elasticity/solid_element_3d.cpp:815-876); MASTG_PARAM_PRESSURE
+     * is its parameter */
+    uint32_t pres_bid_mask;
 } mastg_problem_desc;
 
 typedef struct {
@@ -220,7 +227,7 @@ int mastg_sensitivity(mastg_ctx* ctx, const double* X, const double* lambda, dou
 /* Global parameters: every element's residual depends on them. */
 #define MASTG_PARAM_E 1           /* Young's modulus (hex8 elasticity, plate)           */
 #define MASTG_PARAM_NU 2          /* Poisson's ratio (hex8 elasticity, plate)           */
-#define MASTG_PARAM_PRESSURE 3    /* plate surface pressure                             */
+#define MASTG_PARAM_PRESSURE 3    /* plate surface pressure; hex8 side pressure         */
 #define MASTG_PARAM_K0 4          /* conductivity k0 of k(T) = k0 (1 + alpha T) (heat)   */
 #define MASTG_PARAM_CONV_H 5      /* convection coefficient h (heat)                    */
 #define MASTG_PARAM_EMISSIVITY 6  /* radiation emissivity (heat)                        */

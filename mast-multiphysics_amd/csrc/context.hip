@@ -139,7 +139,8 @@ struct mastg_ctx {
     bool bicg_exec_T = false;
     // device data
     DevBuf param, Kel, Kel32, Rel, val, dinv, F;
-    DevBuf Fp;                        // plate: load vector of a unit pressure
+    DevBuf Fp;                        // plate, hex8 pressure sides: load vector of a unit pressure
+    bool has_pressure = false;        // hex8 with pressure sides
     DevBuf vx, vr, vz, vp, vq, vb, vtmp, vcaller, vg, part0, part1, scal, dscalar;
     CGScalars* h_scal = nullptr;      // pinned mirror of the CG scalars
     double* h_pin = nullptr;          // pinned scratch for scalar read-back
@@ -1602,6 +1603,72 @@ int mastg_ctx_create(const mastg_problem_desc* desc, mastg_ctx** out) {
         for (int64_t i = 0; i < S.n_active; i++) F[i] += c->d.pressure * Fp[i];
         if ((rc = upload(c, c->Fp, Fp))) return bail(rc);
     }
+    if (desc->physics == MASTG_HEX8_ELASTIC && desc->pres_bid_mask) {
+        // uniform pressure on the marked sides: R -= int N p n dA, n the
+        // outward normal (surface_pressure_residual, src/elasticity/
+        // solid_element_3d.cpp:815-876), i.e. F += p int N n dA; the load of a
+        // unit pressure is kept for dR/dp.  2x2 Gauss on the side; n dA =
+        // dx/ds x dx/dt, turned to point away from the element's centroid.
+        if (desc->n_sides > 0 && (!desc->side_elem || !desc->side_index || !desc->side_id)) {
+            c->err = "pressure sides need side_elem, side_index and side_id";
+            return bail(MASTG_ERR_INVALID);
+        }
+        std::vector<double> Fp(S.n_active, 0.0);
+        const double gq = 0.57735026918962576451;
+        // libMesh hex8 side -> (fixed reference axis, its value)
+        static const int kAxis[6] = {2, 1, 0, 1, 0, 2};
+        static const double kVal[6] = {-1.0, -1.0, 1.0, 1.0, -1.0, 1.0};
+        for (int64_t sd = 0; sd < desc->n_sides; sd++) {
+            if (!((desc->pres_bid_mask >> desc->side_id[sd]) & 1u)) continue;
+            const int64_t e = desc->side_elem[sd];
+            const int side = desc->side_index[sd];
+            if (e < 0 || e >= S.n_elems || side < 0 || side > 5) {
+                c->err = "pressure side out of range";
+                return bail(MASTG_ERR_INVALID);
+            }
+            double X[8][3], cen[3] = {0, 0, 0};
+            for (int n = 0; n < 8; n++)
+                for (int d = 0; d < 3; d++) {
+                    X[n][d] = S.xyz_r[int64_t(S.econn[e * 8 + n]) * 3 + d];
+                    cen[d] += 0.125 * X[n][d];
+                }
+            const int ax = kAxis[side], u0 = ax == 0 ? 1 : 0, u1 = ax == 2 ? 1 : 2;
+            for (int qj = 0; qj < 2; qj++)
+                for (int qi = 0; qi < 2; qi++) {
+                    double xi[3];
+                    xi[ax] = kVal[side];
+                    xi[u0] = qi ? gq : -gq;
+                    xi[u1] = qj ? gq : -gq;
+                    double N[8], dN[8][3];
+                    for (int n = 0; n < 8; n++) {
+                        const double sx = (n == 1 || n == 2 || n == 5 || n == 6) ? 1.0 : -1.0;
+                        const double sy = (n == 2 || n == 3 || n == 6 || n == 7) ? 1.0 : -1.0;
+                        const double sz = (n >= 4) ? 1.0 : -1.0;
+                        const double fx = 0.5 * (1 + sx * xi[0]), fy = 0.5 * (1 + sy * xi[1]),
+                                     fz = 0.5 * (1 + sz * xi[2]);
+                        N[n] = fx * fy * fz;
+                        dN[n][0] = 0.5 * sx * fy * fz;
+                        dN[n][1] = fx * 0.5 * sy * fz;
+                        dN[n][2] = fx * fy * 0.5 * sz;
+                    }
+                    double a[3] = {0, 0, 0}, b[3] = {0, 0, 0}, xq[3] = {0, 0, 0};
+                    for (int n = 0; n < 8; n++)
+                        for (int d = 0; d < 3; d++) {
+                            a[d] += X[n][d] * dN[n][u0];
+                            b[d] += X[n][d] * dN[n][u1];
+                            xq[d] += X[n][d] * N[n];
+                        }
+                    double nda[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+                    if (nda[0] * (xq[0] - cen[0]) + nda[1] * (xq[1] - cen[1]) + nda[2] * (xq[2] - cen[2]) < 0)
+                        for (double& v : nda) v = -v;
+                    for (int n = 0; n < 8; n++)
+                        for (int d = 0; d < 3; d++) Fp[int64_t(S.econn[e * 8 + n]) * 3 + d] += N[n] * nda[d];
+                }
+        }
+        for (int64_t i = 0; i < S.n_active; i++) F[i] += c->d.pressure * Fp[i];
+        if ((rc = upload(c, c->Fp, Fp))) return bail(rc);
+        c->has_pressure = true;
+    }
     std::vector<int32_t> conv(S.constrained.begin(), S.constrained.end());
     if ((rc = upload(c, c->xyz, S.xyz_r)) || (rc = upload(c, c->econn, S.econn)) ||
         (rc = upload(c, c->blk_ptr, S.blk_ptr)) || (rc = upload(c, c->blk_col, S.blk_col)) ||
@@ -1939,7 +2006,7 @@ void material_dD3_dnu_unitE(double nu, double D[36]) {
 // plate's sections), in k0, h and the emissivity (heat) and in the pressure
 // (plate loads), so dR/dp is the term's residual at a unit parameter; nu
 // enters through D (and the plate's G), whose nu-derivative replaces it in the
-// element kernel.
+// element kernel.  The pressure (plate; hex8 sides) enters through the load only.
 int global_dRdp(mastg_ctx* c, int param, const double* Xa, double* out) {
     const Structure& S = c->S;
     TRY(dalloc(c, c->Rel, size_t(S.n_elems) * S.npe * S.nav * sizeof(double)));
@@ -1953,10 +2020,13 @@ int global_dRdp(mastg_ctx* c, int param, const double* Xa, double* out) {
             scale = 1.0 / c->d.E;
         } else if (param == MASTG_PARAM_NU) {
             material_dD3_dnu_unitE(c->d.nu, a.Dhat);
+        } else if (param == MASTG_PARAM_PRESSURE && c->has_pressure) {
+            load_term = true;   // dR/dp = -(load of a unit pressure on the marked sides)
         } else {
-            return fail(c, MASTG_ERR_INVALID, "parameter not defined for hex8 elasticity (E, nu)");
+            return fail(c, MASTG_ERR_INVALID,
+                        "parameter not defined for hex8 elasticity (E, nu; pressure with pressure sides)");
         }
-        launch_hex8_K(a, c->stream);
+        if (!load_term) launch_hex8_K(a, c->stream);
     } else if (ph == MASTG_QUAD4_MINDLIN) {
         Quad4Args a = quad4_args(c, Xa, false);
         const double E = c->d.E, nu = c->d.nu;

@@ -40,7 +40,7 @@ class ProblemDesc(C.Structure):
         ("k0", C.c_double), ("alpha", C.c_double), ("conv_h", C.c_double),
         ("conv_tinf", C.c_double), ("conv_bid_mask", C.c_uint32), ("flags", C.c_int32),
         ("rad_emissivity", C.c_double), ("rad_tamb", C.c_double), ("rad_tzero", C.c_double),
-        ("rad_sigma", C.c_double), ("rad_bid_mask", C.c_uint32),
+        ("rad_sigma", C.c_double), ("rad_bid_mask", C.c_uint32), ("pres_bid_mask", C.c_uint32),
     ]
 
 

@@ -74,6 +74,15 @@ static void element_calc(const Problem& P, int64_t e, const double* sol, double*
         for (int i = 0; i < nv * 8; i++) u[i] = sol[i];
         if (K) std::memset(KK, 0, sizeof(KK));
         hex8_elastic(X, D, u, ff, K ? KK : nullptr);
+        if (P.pres_bid_mask) {
+            // pressure on this element's boundary sides in the mask
+            // (the mesh lists sides in non-decreasing element order)
+            auto lo = std::lower_bound(m.bs_elem.begin(), m.bs_elem.end(), int32_t(e));
+            for (auto it = lo; it != m.bs_elem.end() && *it == e; ++it) {
+                const size_t s = size_t(it - m.bs_elem.begin());
+                if ((P.pres_bid_mask >> m.bs_id[s]) & 1u) hex8_pressure(X, m.bs_side[s], P.pressure, ff);
+            }
+        }
         for (int i = 0; i < nv * 8; i++) {
             f[i] += ff[i];
             if (K)

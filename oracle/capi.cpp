@@ -1,5 +1,6 @@
 // Oracle (test infrastructure only): C entry points for the Python tests and
 // the CPU-baseline leg of bench.py (loaded with ctypes from liboracle.so).
+#include <algorithm>
 #include "oracle.h"
 #include <cstring>
 #include <memory>
@@ -59,12 +60,28 @@ void* oracle_problem_from_arrays(int kind, int64_t n_nodes, const double* xyz, i
     P.active_only = (kind == 1 && n_vars == 3) || (kind == 2 && n_vars == 5);
     return h;
 }
+// boundary sides (element, libMesh side, boundary id) of a problem built from
+// arrays; kept in non-decreasing element order as the element loop expects
+void oracle_set_bsides(void* h, int64_t n, const int32_t* elem, const int32_t* side, const int32_t* id) {
+    Mesh& m = H(h)->P.mesh;
+    std::vector<int64_t> ord(static_cast<size_t>(n));
+    for (int64_t i = 0; i < n; i++) ord[size_t(i)] = i;
+    std::stable_sort(ord.begin(), ord.end(), [&](int64_t a, int64_t b) { return elem[a] < elem[b]; });
+    m.bs_elem.resize(size_t(n));
+    m.bs_side.resize(size_t(n));
+    m.bs_id.resize(size_t(n));
+    for (int64_t i = 0; i < n; i++) {
+        m.bs_elem[size_t(i)] = elem[ord[size_t(i)]];
+        m.bs_side[size_t(i)] = side[ord[size_t(i)]];
+        m.bs_id[size_t(i)] = id[ord[size_t(i)]];
+    }
+}
 void oracle_problem_free(void* h) { delete H(h); }
 void oracle_set_threads(void* h, int t) { H(h)->threads = t; }
 int oracle_max_threads() { return omp_get_max_threads(); }
 
 // set a scalar of the problem: E nu penalty kappa off k0 alpha conv_h conv_Tinf
-// conv_mask pressure rad_sigma rad_emiss rad_Tamb rad_T0 rad_mask; returns 0 on
+// conv_mask pressure pres_mask rad_sigma rad_emiss rad_Tamb rad_T0 rad_mask; returns 0 on
 // success, -1 for an unknown name
 int oracle_set_scalar(void* h, const char* name, double v) {
     Problem& P = H(h)->P;
@@ -80,6 +97,7 @@ int oracle_set_scalar(void* h, const char* name, double v) {
     else if (n == "conv_Tinf") P.conv_Tinf = v;
     else if (n == "conv_mask") P.conv_bid_mask = uint32_t(v);
     else if (n == "pressure") P.pressure = v;
+    else if (n == "pres_mask") P.pres_bid_mask = uint32_t(v);
     else if (n == "rad_sigma") P.rad_sigma = v;
     else if (n == "rad_emiss") P.rad_emiss = v;
     else if (n == "rad_Tamb") P.rad_Tamb = v;
@@ -232,6 +250,9 @@ void oracle_quad4_mindlin(const double* X, double E, double nu, double h, double
 void oracle_hex8_heat(const double* X, double k0, double alpha, const double* T, double* f,
                       double* K) {
     hex8_heat(reinterpret_cast<const double(*)[3]>(X), k0, alpha, T, f, K);
+}
+void oracle_hex8_pressure(const double* X, int side, double p, double* f) {
+    hex8_pressure(reinterpret_cast<const double(*)[3]>(X), side, p, f);
 }
 void oracle_hex8_convection(const double* X, int side, double h, double Tinf, const double* T,
                             double* f, double* K) {

@@ -509,6 +509,37 @@ void hex8_radiation(const double X[8][3], int side, double sigma, double emiss, 
         }
 }
 
+// StructuralElement3D::surface_pressure_residual on side s (solid_element_3d.
+// cpp:815-876): f -= int N (p n) dA over the side (2x2 side Gauss), n the
+// outward unit normal, f in the element's variable-major layout (u0..u7,
+// v0..v7, w0..w7).  n dA = sign (dx/ds x dx/dt) with (s, t) the side's
+// in-face reference directions of hex8_side_point; sign orients the cross
+// product outward for libMesh's side numbering (side 0: zeta = -1, 1: eta =
+// -1, 2: xi = +1, 3: eta = +1, 4: xi = -1, 5: zeta = +1).
+void hex8_pressure(const double X[8][3], int side, double p, double* f) {
+    static const double kSign[6] = {-1.0, 1.0, 1.0, -1.0, -1.0, 1.0};
+    int r0, r1;
+    switch (side) {
+        case 0: case 5: r0 = 0; r1 = 1; break;
+        case 1: case 3: r0 = 0; r1 = 2; break;
+        default: r0 = 1; r1 = 2; break;
+    }
+    for (int j = 0; j < 2; j++)
+        for (int i = 0; i < 2; i++) {
+            double xi[3], N[8], dN[8][3];
+            hex8_side_point(side, GP2[i], GP2[j], xi);
+            hex8_shape(xi[0], xi[1], xi[2], N, dN);
+            double a[3] = {0, 0, 0}, b[3] = {0, 0, 0};
+            for (int n = 0; n < 8; n++)
+                for (int d = 0; d < 3; d++) { a[d] += X[n][d] * dN[n][r0]; b[d] += X[n][d] * dN[n][r1]; }
+            const double nda[3] = {kSign[side] * (a[1] * b[2] - a[2] * b[1]),
+                                   kSign[side] * (a[2] * b[0] - a[0] * b[2]),
+                                   kSign[side] * (a[0] * b[1] - a[1] * b[0])};
+            for (int d = 0; d < 3; d++)
+                for (int n = 0; n < 8; n++) f[d * 8 + n] -= N[n] * p * nda[d];
+        }
+}
+
 void hex8_source(const double X[8][3], double q, double* f) {
     for (int k = 0; k < 2; k++)
         for (int j = 0; j < 2; j++)

@@ -138,6 +138,9 @@ void quad4_pressure(const double X[4][3], double p, double* f);
 void hex8_heat(const double X[8][3], double k0, double alpha, const double* T,
                double* f, double* K);
 // surface_convection_residual (:1099-1154) on side s
+// uniform pressure p on side s of a hex8, StructuralElement3D::surface_pressure_residual:
+// f -= int N p n dA (outward n), variable-major f
+void hex8_pressure(const double X[8][3], int side, double p, double* f);
 void hex8_convection(const double X[8][3], int side, double h, double Tinf,
                      const double* T, double* f, double* K);
 // surface_radiation_residual (:1402-1463) on side s: flux sigma eps ((T+T0)^4 - (Ta+T0)^4)
@@ -164,7 +167,8 @@ struct Problem {
     // surface radiation on the sides whose boundary id is in rad_bid_mask
     double rad_sigma = 5.670374419e-8, rad_emiss = 0.0, rad_Tamb = 0.0, rad_T0 = 0.0;
     uint32_t rad_bid_mask = 0;
-    double pressure = 0.0;                        // plate
+    double pressure = 0.0;                        // plate; hex8 on the pres_bid_mask sides
+    uint32_t pres_bid_mask = 0;                   // hex8: boundary ids carrying the pressure
     std::vector<double> param;                    // rho_e / h_e / q_e
     std::vector<double> point_load;               // [n_dofs] nodal loads F (R -= F)
     bool active_only = false;  // structural: keep only the active variables

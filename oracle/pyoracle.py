@@ -42,6 +42,7 @@ def lib():
             "oracle_n_bsides": (C.c_int64, [_P]),
             "oracle_get_mesh": (None, [_P, _D, _I32]),
             "oracle_get_bsides": (None, [_P, _I32, _I32, _I32]),
+            "oracle_set_bsides": (None, [_P, C.c_int64, _I32, _I32, _I32]),
             "oracle_get_dofmap": (None, [_P, _I64, _U8]),
             "oracle_get_params": (None, [_P, _D]),
             "oracle_set_params": (None, [_P, _D]),
@@ -65,6 +66,7 @@ def lib():
             "oracle_quad4_mindlin": (None, [_D] + [C.c_double] * 5 + [_D, _D, _D]),
             "oracle_hex8_heat": (None, [_D, C.c_double, C.c_double, _D, _D, _D]),
             "oracle_hex8_convection": (None, [_D, C.c_int, C.c_double, C.c_double, _D, _D, _D]),
+            "oracle_hex8_pressure": (None, [_D, C.c_int, C.c_double, _D]),
             "oracle_uniform": (None, [C.c_uint64, C.c_int64, C.c_double, C.c_double, _D]),
             "oracle_hex8_radiation": (None, [_D, C.c_int] + [C.c_double] * 4 + [_D, _D, _D]),
             "oracle_hex8_stress": (None, [_D, C.c_double, C.c_double, _D, _D, _D]),
@@ -92,7 +94,8 @@ class Problem:
     def __init__(self, kind, nx=0, ny=0, nz=0, seed=1, lo=0.0, hi=1.0, mast_exact=False,
                  threads=0, arrays=None):
         """A structured workload, or (arrays = dict(xyz, conn, n_vars, node_base,
-        constrained, params[, point_load])) a problem over any mesh and DofMap."""
+        constrained, params[, point_load][, sides = (elem, side, id)])) a
+        problem over any mesh and DofMap."""
         self.kind = kind
         if arrays is not None:
             a = {k: v for k, v in arrays.items()}
@@ -106,6 +109,9 @@ class Problem:
             self.h = lib().oracle_problem_from_arrays(kind, xyz.shape[0], ptr(xyz), conn.shape[0], ptr(conn, _I32),
                                                       int(a["n_vars"]), ptr(base, _I64), con.size, ptr(con, _U8),
                                                       ptr(par), ptr(pl))
+            if a.get("sides") is not None:
+                se, si, sb = (np.ascontiguousarray(v, np.int32) for v in a["sides"])
+                lib().oracle_set_bsides(self.h, se.size, ptr(se, _I32), ptr(si, _I32), ptr(sb, _I32))
         else:
             self.h = lib().oracle_problem_new(kind, nx, ny, nz, seed, lo, hi, int(mast_exact))
         if threads:
@@ -289,6 +295,15 @@ def hex8_heat(X, k0, alpha, T):
     K = np.zeros((8, 8))
     lib().oracle_hex8_heat(ptr(X), k0, alpha, ptr(T), ptr(f), ptr(K))
     return f, K
+
+
+def hex8_pressure(X, side, p):
+    """Element vector of a uniform pressure p on side `side` (variable-major
+    u0..u7, v0..v7, w0..w7): f = -int N p n dA."""
+    X = np.ascontiguousarray(X, np.float64).reshape(8, 3)
+    f = np.zeros(24)
+    lib().oracle_hex8_pressure(ptr(X), side, p, ptr(f))
+    return f
 
 
 def hex8_convection(X, side, h, Tinf, T):
