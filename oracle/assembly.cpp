@@ -1,11 +1,11 @@
 // Oracle (test infrastructure only): problem set-up for the workloads, the
 // NonlinearImplicitAssembly element loop with libMesh constraint semantics,
 // and per-element adjoint sensitivity dots.
+#include <stdexcept>
 #include "oracle.h"
 #include <cmath>
 #include <cstring>
 #include <algorithm>
-#include <stdexcept>
 #include <omp.h>
 
 namespace oracle {
@@ -94,8 +94,23 @@ static void element_calc(const Problem& P, int64_t e, const double* sol, double*
         for (int i = 0; i < nv * 4; i++) u[i] = sol[i];
         PlateSection s = plate_section(P, P.param[e], false);
         if (K) std::memset(KK, 0, sizeof(KK));
-        quad4_mindlin(X, s, u, ff, K ? KK : nullptr, nv == 6);
-        quad4_pressure(X, P.pressure, ff);
+        double T[9], XL[4][3];
+        if (nv == 6 && plate_local_frame(X, T, XL)) {
+            // out of the x-y plane: the element in its local frame, state in,
+            // vector and matrix out through T (the 6-variable system; the
+            // 5-variable layout is the x-y-plane restriction)
+            double ul[24], fl[24] = {0};
+            static thread_local double KL[24 * 24];
+            plate_to_local(T, u, ul);
+            if (K) std::memset(KL, 0, sizeof(KL));
+            quad4_mindlin(XL, s, ul, fl, K ? KL : nullptr, true);
+            quad4_pressure(XL, P.pressure, fl);
+            plate_to_global(T, fl, ff);
+            if (K) plate_matrix_to_global(T, KL, KK);
+        } else {
+            quad4_mindlin(X, s, u, ff, K ? KK : nullptr, nv == 6);
+            quad4_pressure(X, P.pressure, ff);
+        }
         for (int i = 0; i < nv * 4; i++) {
             f[i] += ff[i];
             if (K)
@@ -205,7 +220,17 @@ void sensitivity_dots(const Problem& P, const double* X, const double* lambda, d
                 double u[24] = {0}, ff[24] = {0};
                 for (int i = 0; i < nd; i++) u[i] = sol[i];
                 PlateSection ds = plate_section(P, P.param[e], true);
-                quad4_mindlin_sens(Xc, ds, u, ff);
+                double Xq[4][3], T[9], XL[4][3];
+                for (int n = 0; n < 4; n++)
+                    for (int d = 0; d < 3; d++) Xq[n][d] = Xc[n][d];
+                if (nd == 24 && plate_local_frame(Xq, T, XL)) {
+                    double ul[24], fl[24] = {0};
+                    plate_to_local(T, u, ul);
+                    quad4_mindlin_sens(XL, ds, ul, fl);
+                    plate_to_global(T, fl, ff);
+                } else {
+                    quad4_mindlin_sens(Xc, ds, u, ff);
+                }
                 for (int i = 0; i < nd; i++) df[i] = ff[i];
             } else {
                 // dR/dq_e = -int N (volume_heat_source_residual_sensitivity)

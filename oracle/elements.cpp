@@ -388,6 +388,82 @@ void quad4_mindlin(const double X[4][3], const PlateSection& s, const double* u,
         for (int a = 0; a < 4; a++) K[(20 + a) * 24 + 20 + a] = 1.0e-8;  // (structural_element_2d.cpp:1215-1221)
 }
 
+// The plate element's local frame (GeomElem::_init_local_elem /
+// _init_local_elem_2d, src/mesh/geom_elem.cpp:330-380, 460-511): none when
+// every node has node 0's z (the element is used in the global x-y plane);
+// else x along node 1 - node 0, z along x cross (node 2 - node 0), y = z cross
+// x, T = [x y z] (columns), local coordinates T^T (X - X0).
+bool plate_local_frame(const double X[4][3], double T[9], double XL[4][3]) {
+    bool use = false;
+    for (int n = 1; n < 4; n++)
+        if (std::fabs(X[0][2] - X[n][2]) != 0.) use = true;
+    if (!use) return false;
+    double v1[3], v2[3], v3[3];
+    for (int i = 0; i < 3; i++) { v1[i] = X[1][i] - X[0][i]; v2[i] = X[2][i] - X[0][i]; }
+    auto unit = [](double* v) {
+        const double n = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        for (int i = 0; i < 3; i++) v[i] /= n;
+    };
+    auto cross = [](const double* a, const double* b, double* c) {
+        c[0] = a[1] * b[2] - a[2] * b[1];
+        c[1] = a[2] * b[0] - a[0] * b[2];
+        c[2] = a[0] * b[1] - a[1] * b[0];
+    };
+    unit(v1);
+    unit(v2);
+    cross(v1, v2, v3);
+    unit(v3);
+    cross(v3, v1, v2);
+    unit(v2);
+    for (int i = 0; i < 3; i++) { T[i * 3 + 0] = v1[i]; T[i * 3 + 1] = v2[i]; T[i * 3 + 2] = v3[i]; }
+    for (int n = 0; n < 4; n++)
+        for (int j = 0; j < 3; j++) {
+            double s = 0;
+            for (int k = 0; k < 3; k++) s += T[k * 3 + j] * (X[n][k] - X[0][k]);
+            XL[n][j] = s;
+        }
+    return true;
+}
+
+// StructuralElementBase::transform_vector_to_local_system / _to_global_system
+// and transform_matrix_to_global_system (src/elasticity/structural_element_base.
+// cpp:1568-1660) for a 4-node element with 6 variables (variable-major,
+// index v*4 + a): the same 3x3 T on (u, v, w) and on (tx, ty, tz) of every node
+void plate_to_local(const double T[9], const double* g, double* l) {
+    for (int a = 0; a < 4; a++)
+        for (int blk = 0; blk < 2; blk++)
+            for (int j = 0; j < 3; j++) {
+                double s = 0;
+                for (int k = 0; k < 3; k++) s += T[k * 3 + j] * g[(blk * 3 + k) * 4 + a];
+                l[(blk * 3 + j) * 4 + a] = s;
+            }
+}
+void plate_to_global(const double T[9], const double* l, double* g) {
+    for (int a = 0; a < 4; a++)
+        for (int blk = 0; blk < 2; blk++)
+            for (int j = 0; j < 3; j++) {
+                double s = 0;
+                for (int k = 0; k < 3; k++) s += T[j * 3 + k] * l[(blk * 3 + k) * 4 + a];
+                g[(blk * 3 + j) * 4 + a] = s;
+            }
+}
+void plate_matrix_to_global(const double T[9], const double* KL, double* KG) {
+    // KG = M KL M^T, M = blockdiag over (node, translation / rotation) of T
+    static thread_local double W[24 * 24];
+    for (int c = 0; c < 24; c++) {           // W = M KL, column by column
+        double col[24], out[24];
+        for (int r = 0; r < 24; r++) col[r] = KL[r * 24 + c];
+        plate_to_global(T, col, out);
+        for (int r = 0; r < 24; r++) W[r * 24 + c] = out[r];
+    }
+    for (int r = 0; r < 24; r++) {           // KG = W M^T = (M W^T)^T
+        double row[24], out[24];
+        for (int c = 0; c < 24; c++) row[c] = W[r * 24 + c];
+        plate_to_global(T, row, out);
+        for (int c = 0; c < 24; c++) KG[r * 24 + c] = out[c];
+    }
+}
+
 void quad4_mindlin_sens(const double X[4][3], const PlateSection& ds, const double* u, double* df) {
     quad4_mindlin_impl(X, ds, u, df, nullptr);
 }

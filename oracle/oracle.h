@@ -126,6 +126,11 @@ struct PlateSection { double A[9], B[9], Db[9], S[4]; };
 void quad4_mindlin(const double X[4][3], const PlateSection& s, const double* u,
                    double* f, double* K, bool add_thetaz_diag);
 // d(internal residual)/dp with the section derivatives in `ds`
+// local frame of a plate element out of the x-y plane (false: none needed)
+bool plate_local_frame(const double X[4][3], double T[9], double XL[4][3]);
+void plate_to_local(const double T[9], const double* g, double* l);        // 24-vectors, 6 variables
+void plate_to_global(const double T[9], const double* l, double* g);
+void plate_matrix_to_global(const double T[9], const double* KL, double* KG);   // 24 x 24
 void quad4_mindlin_sens(const double X[4][3], const PlateSection& ds, const double* u,
                         double* df);
 // uniform surface pressure p along local -z, StructuralElementBase::surface_pressure_residual
