@@ -70,7 +70,11 @@ typedef struct {
     int64_t n_elems;
     const double* xyz;           /* [n_nodes*3]                                        */
     const int32_t* conn;         /* [n_elems*npe] libMesh local node order (npe 8 / 4) */
-    int32_t n_vars;              /* variables per node of the caller's system (6 or 1) */
+    int32_t n_vars;              /* variables per node of the caller's system: 6 (MAST's
+                                    structural system; hex8 also 3, plates in one z = const
+                                    plane also 5) or 1 (heat).  A plate element whose nodes
+                                    do not share one z is computed in its own frame, which
+                                    needs all 6 */
     const int64_t* node_dof_base;/* [n_nodes] or NULL: libMesh var-major first-touch  */
     int64_t n_constrained;       /* homogeneous Dirichlet DOFs (caller numbering)      */
     const int64_t* constrained_dofs;

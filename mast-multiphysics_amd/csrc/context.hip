@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "kernels.h"
+#include "plate_frame.h"
 #include "mastg/mastg.h"
 #include "hip_lock.h"
 #include "multigrid.h"
@@ -266,6 +267,7 @@ Hex8Args hex8_args(mastg_ctx* c, const double* X, bool want_K) {
 Quad4Args quad4_args(mastg_ctx* c, const double* X, bool want_K) {
     Quad4Args a{};
     a.n_elems = c->S.n_elems;
+    a.nav = c->S.nav;
     a.xyz = c->xyz.as<double>();
     a.econn = c->econn.as<int32_t>();
     a.param = c->param.as<double>();
@@ -1564,39 +1566,39 @@ int mastg_ctx_create(const mastg_problem_desc* desc, mastg_ctx** out) {
         c->Dps[1] = c->Dps[3] = E * nu / (1. - nu * nu);
         c->Dps[8] = E / 2. / (1. + nu);
         c->Gmod = E / 2. / (1. + nu);
-        // elements must lie in one z = const plane (no local frame)
-        const double z0 = S.xyz_r[2];
-        for (int64_t r = 0; r < S.n_nodes; r++)
-            if (S.xyz_r[r * 3 + 2] != z0) {
-                c->err = "Mindlin plate elements must lie in one z = const plane";
-                return bail(MASTG_ERR_INVALID);
-            }
-        // uniform pressure along local -z: R_w += p int N (surface_pressure_residual,
-        // src/elasticity/structural_element_base.cpp:1167-1235), i.e. F_w = -p int N;
-        // the load of a unit pressure is kept for dR/dp
+        // uniform pressure along the element's local -z: R_w += p int N
+        // (surface_pressure_residual, src/elasticity/structural_element_base.cpp:
+        // 1167-1235), i.e. F = -p int N v3 with v3 the frame normal ((0, 0, 1)
+        // for an element in a z = const plane); the load of a unit pressure is
+        // kept for dR/dp
+        const int nav = S.nav;
         std::vector<double> Fp(S.n_active, 0.0);
         {
             const double gq = 0.57735026918962576451;
             for (int64_t e = 0; e < S.n_elems; e++) {
-                double X[4][2];
+                double XG[4][3], XL[4][3], T[9];
                 for (int n = 0; n < 4; n++)
-                    for (int i = 0; i < 2; i++) X[n][i] = S.xyz_r[int64_t(S.econn[e * 4 + n]) * 3 + i];
+                    for (int i = 0; i < 3; i++) XG[n][i] = S.xyz_r[int64_t(S.econn[e * 4 + n]) * 3 + i];
+                quad4_frame(XG, T, XL);
                 for (int qj = 0; qj < 2; qj++)
                     for (int qi = 0; qi < 2; qi++) {
                         const double xi = qi ? gq : -gq, eta = qj ? gq : -gq;
-                        double N[4], dxi[4], deta[4];
-                        for (int n = 0; n < 4; n++) {
-                            const double sx = (n == 1 || n == 2) ? 1.0 : -1.0, sy = (n >= 2) ? 1.0 : -1.0;
-                            const double fx = 0.5 * (1.0 + sx * xi), fy = 0.5 * (1.0 + sy * eta);
-                            N[n] = fx * fy; dxi[n] = 0.5 * sx * fy; deta[n] = fx * 0.5 * sy;
+                        double N[4], g[4][2], jxw;
+                        if (nav == 5) {
+                            double dN[4][2];
+                            quad4_shape_all(xi, eta, N, dN);
+                            double j00 = 0, j01 = 0, j10 = 0, j11 = 0;
+                            for (int n = 0; n < 4; n++) {
+                                j00 += XG[n][0] * dN[n][0]; j01 += XG[n][0] * dN[n][1];
+                                j10 += XG[n][1] * dN[n][0]; j11 += XG[n][1] * dN[n][1];
+                            }
+                            jxw = std::fabs(j00 * j11 - j01 * j10);
+                        } else {
+                            jxw = quad4_grad_surface(XL, xi, eta, g, N);
                         }
-                        double j00 = 0, j01 = 0, j10 = 0, j11 = 0;
-                        for (int n = 0; n < 4; n++) {
-                            j00 += X[n][0] * dxi[n]; j01 += X[n][0] * deta[n];
-                            j10 += X[n][1] * dxi[n]; j11 += X[n][1] * deta[n];
-                        }
-                        const double jxw = std::fabs(j00 * j11 - j01 * j10);
-                        for (int n = 0; n < 4; n++) Fp[int64_t(S.econn[e * 4 + n]) * 5 + 2] -= jxw * N[n];
+                        for (int n = 0; n < 4; n++)
+                            for (int i = 0; i < 3; i++)
+                                Fp[int64_t(S.econn[e * 4 + n]) * nav + i] -= jxw * N[n] * T[i * 3 + 2];
                     }
             }
         }

@@ -18,9 +18,11 @@ struct Hex8Args {
     double* Kel;            // [n_elems*64*9] element blocks or null
 };
 
-// quad4 Mindlin plate (elements in a z = const plane), active vars u v w tx ty
+// quad4 Mindlin plate: active vars u v w tx ty (nav 5, every element in a
+// z = const plane) or u v w tx ty tz (nav 6, elements in their own frame)
 struct Quad4Args {
     int64_t n_elems;
+    int nav;                // 5 or 6
     const double* xyz;      // [n_nodes*3] by node rank
     const int32_t* econn;   // [n_elems*4]
     const double* param;    // [n_elems] thickness h_e
@@ -29,8 +31,8 @@ struct Quad4Args {
     double kappa, off;      // shear factor, section offset
     const uint8_t* con;     // [n_active] or null
     const double* X;        // [n_active] state or null
-    double* Rel;            // [n_elems*4*5] element residual rows (when X)
-    double* Kel;            // [n_elems*16*25] element blocks or null
+    double* Rel;            // [n_elems*4*nav] element residual rows (when X)
+    double* Kel;            // [n_elems*16*nav*nav] element blocks or null
 };
 
 // hex8 steady conduction, k(T) = k0 (1 + alpha T)

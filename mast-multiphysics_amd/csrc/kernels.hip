@@ -12,6 +12,7 @@
 
 #include "kernels.h"
 #include "device_util.h"
+#include "plate_frame.h"
 
 #define Structure_kMaxClasses 8
 
@@ -352,14 +353,15 @@ __global__ __launch_bounds__(128) void k_hex8_elastic_sens(Hex8Args A, const dou
 // quad4 Mindlin plate: StructuralElement2D::internal_residual with the
 // MindlinBendingOperator (src/elasticity/structural_element_2d.cpp:1075-1227,
 // 1765-2019; src/elasticity/mindlin_bending_operator.cpp:57-361), linear strain,
-// element in a z = const plane (no local-frame transform, geom_elem.cpp:358-373).
+// in the global x-y plane for an element in a z = const plane, else in the
+// element's own frame (plate_frame.h).
 //   membrane  eps = Bm u  (exx = du/dx, eyy = dv/dy, gxy = du/dy + dv/dx)
 //   bending   kap = Bb u  (kx = d ty/dx, ky = -d tx/dy, kxy = d ty/dy - d tx/dx)
 //   shear     gam = Bs u  (gxz = dw/dx + ty, gyz = dw/dy - tx) on the 1-point rule
 //   K = sum_q c [Bm^T A Bm + Bm^T B Bb + Bb^T B^T Bm + Bb^T Db Bb] + c_s Bs^T S Bs
 // with A = Dps h, B = Dps h off, Db = Dps (h^3/12 + h off^2), S = G h kappa I
 // (Solid2DSectionProperty, solid_2d_section_element_property_card.cpp:105-157, 393-531).
-// One wave = 4 elements x 16 node pairs; lane (a, b) builds the 5x5 block.
+// One wave = 4 elements x 16 node pairs; lane (a, b) builds the NAV x NAV block.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void quad4_shape(int n, double xi, double eta, double& N, double& dxi,
                                             double& deta) {
@@ -394,18 +396,34 @@ __device__ __forceinline__ double quad4_grad(const double X[4][2], double xi, do
     return det;
 }
 
+// NAV 5: every element in a z = const plane, active u v w tx ty.
+// NAV 6: all six variables; an element out of the x-y plane is computed in
+// its own frame on the surface map and its block goes to the global system as
+// T6 K T6^T, T6 = diag(T, T); the Jacobian carries MAST's 1e-8 on the local
+// theta_z diagonal (structural_element_2d.cpp:1215-1221), the residual does not.
+template <int NAV>
 __global__ __launch_bounds__(256) void k_quad4_mindlin_K(Quad4Args A) {
     const int sub = threadIdx.x >> 4, lane = threadIdx.x & 15;   // 16 elements per block
     const int64_t e = int64_t(blockIdx.x) * 16 + sub;
     if (e >= A.n_elems) return;
     const int a = lane >> 2, b = lane & 3;
-    double X[4][2];
+    double X[4][2], XL[4][3], T[9];
+    bool fr = false;
+    {
+        double XG[4][3];
 #pragma unroll
-    for (int n = 0; n < 4; n++) {
-        const int64_t r = A.econn[e * 4 + n];
-        X[n][0] = A.xyz[r * 3 + 0];
-        X[n][1] = A.xyz[r * 3 + 1];
+        for (int n = 0; n < 4; n++) {
+            const int64_t r = A.econn[e * 4 + n];
+            X[n][0] = XG[n][0] = A.xyz[r * 3 + 0];
+            X[n][1] = XG[n][1] = A.xyz[r * 3 + 1];
+            XG[n][2] = A.xyz[r * 3 + 2];
+        }
+        if constexpr (NAV == 6) fr = quad4_frame(XG, T, XL);
     }
+    auto grad = [&](double xi, double eta, double g[4][2], double N[4]) -> double {
+        if constexpr (NAV == 6) return quad4_grad_surface(XL, xi, eta, g, N);
+        else return quad4_grad(X, xi, eta, g, N);
+    };
     // unit-section blocks: membrane (u,v)x(u,v), coupling (u,v)x(tx,ty),
     // bending (tx,ty)x(tx,ty), shear (w,tx,ty)x(w,tx,ty)
     double Km[4] = {0, 0, 0, 0}, Kc[4] = {0, 0, 0, 0}, Kd[4] = {0, 0, 0, 0}, Kt[4] = {0, 0, 0, 0};
@@ -414,7 +432,7 @@ __global__ __launch_bounds__(256) void k_quad4_mindlin_K(Quad4Args A) {
 #pragma unroll
     for (int q = 0; q < 4; q++) {
         double g[4][2], N[4];
-        const double c = quad4_grad(X, gp_coord(q & 1), gp_coord(q >> 1), g, N) * 1.0;
+        const double c = grad(gp_coord(q & 1), gp_coord(q >> 1), g, N) * 1.0;
         const double ax = g[a][0], ay = g[a][1], bx = g[b][0], by = g[b][1];
         // Bm_b columns (u, v): [bx, 0, by], [0, by, bx]; Bb_b columns (tx, ty): [0, -by, -bx], [bx, 0, by]
         const double Bm_b[2][3] = {{bx, 0.0, by}, {0.0, by, bx}};
@@ -451,7 +469,7 @@ __global__ __launch_bounds__(256) void k_quad4_mindlin_K(Quad4Args A) {
     {
         // transverse shear on the 1-point rule (weight 4)
         double g[4][2], N[4];
-        const double c = quad4_grad(X, 0.0, 0.0, g, N) * 4.0;
+        const double c = grad(0.0, 0.0, g, N) * 4.0;
         // Bs columns (w, tx, ty): gxz = [gx, 0, N], gyz = [gy, -N, 0]
         const double Ba[2][3] = {{g[a][0], 0.0, N[a]}, {g[a][1], -N[a], 0.0}};
         const double Bb2[2][3] = {{g[b][0], 0.0, N[b]}, {g[b][1], -N[b], 0.0}};
@@ -462,82 +480,141 @@ __global__ __launch_bounds__(256) void k_quad4_mindlin_K(Quad4Args A) {
     }
     const double h = A.param[e];
     const double fA = h, fB = h * A.off, fD = h * h * h / 12.0 + h * A.off * A.off, fS = A.G * h * A.kappa;
-    double K[25];
+    double K[NAV * NAV];
 #pragma unroll
-    for (int i = 0; i < 25; i++) K[i] = 0.0;
-    // vars: 0 u, 1 v, 2 w, 3 tx, 4 ty
+    for (int i = 0; i < NAV * NAV; i++) K[i] = 0.0;
+    // vars: 0 u, 1 v, 2 w, 3 tx, 4 ty (5 tz: no stiffness in the local frame)
 #pragma unroll
     for (int i = 0; i < 2; i++)
 #pragma unroll
         for (int j = 0; j < 2; j++) {
-            K[i * 5 + j] = fA * Km[i * 2 + j];
-            K[i * 5 + 3 + j] = fB * Kc[i * 2 + j];
-            K[(3 + i) * 5 + j] = fB * Kt[i * 2 + j];
-            K[(3 + i) * 5 + 3 + j] = fD * Kd[i * 2 + j];
+            K[i * NAV + j] = fA * Km[i * 2 + j];
+            K[i * NAV + 3 + j] = fB * Kc[i * 2 + j];
+            K[(3 + i) * NAV + j] = fB * Kt[i * 2 + j];
+            K[(3 + i) * NAV + 3 + j] = fD * Kd[i * 2 + j];
         }
 #pragma unroll
     for (int i = 0; i < 3; i++)
 #pragma unroll
-        for (int j = 0; j < 3; j++) K[(2 + i) * 5 + 2 + j] += fS * Ks[i * 3 + j];
+        for (int j = 0; j < 3; j++) K[(2 + i) * NAV + 2 + j] += fS * Ks[i * 3 + j];
+    if constexpr (NAV == 6) {
+        if (fr) {
+            // K <- T6 K T6^T: translations (0..2) and rotations (3..5) by T
+            double W[36];
+#pragma unroll
+            for (int i = 0; i < 6; i++)
+#pragma unroll
+                for (int j = 0; j < 6; j++) {
+                    const int bi = i / 3 * 3, ii = i % 3;
+                    W[i * 6 + j] = T[ii * 3 + 0] * K[(bi + 0) * 6 + j] + T[ii * 3 + 1] * K[(bi + 1) * 6 + j] +
+                                   T[ii * 3 + 2] * K[(bi + 2) * 6 + j];
+                }
+#pragma unroll
+            for (int i = 0; i < 6; i++)
+#pragma unroll
+                for (int j = 0; j < 6; j++) {
+                    const int bj = j / 3 * 3, jj = j % 3;
+                    K[i * 6 + j] = W[i * 6 + bj + 0] * T[jj * 3 + 0] + W[i * 6 + bj + 1] * T[jj * 3 + 1] +
+                                   W[i * 6 + bj + 2] * T[jj * 3 + 2];
+                }
+        }
+    }
     const int32_t ra = A.econn[e * 4 + a], rb = A.econn[e * 4 + b];
     if (A.X) {
-        double f[5];
+        double f[NAV];
 #pragma unroll
-        for (int v = 0; v < 5; v++) {
+        for (int v = 0; v < NAV; v++) {
             double t = 0.0;
 #pragma unroll
-            for (int w = 0; w < 5; w++) t += K[v * 5 + w] * A.X[int64_t(rb) * 5 + w];
+            for (int w = 0; w < NAV; w++) t += K[v * NAV + w] * A.X[int64_t(rb) * NAV + w];
             f[v] = t;
         }
 #pragma unroll
         for (int m = 1; m < 4; m <<= 1)
 #pragma unroll
-            for (int v = 0; v < 5; v++) f[v] += __shfl_xor(f[v], m, 64);
+            for (int v = 0; v < NAV; v++) f[v] += __shfl_xor(f[v], m, 64);
         if (b == 0)
 #pragma unroll
-            for (int v = 0; v < 5; v++)
-                A.Rel[(e * 4 + a) * 5 + v] = (A.con && A.con[int64_t(ra) * 5 + v]) ? 0.0 : f[v];
+            for (int v = 0; v < NAV; v++)
+                A.Rel[(e * 4 + a) * NAV + v] = (A.con && A.con[int64_t(ra) * NAV + v]) ? 0.0 : f[v];
     }
     if (A.Kel) {
-        double* out = A.Kel + (e * 16 + lane) * 25;
+        if constexpr (NAV == 6) {
+            // 1e-8 on the local theta_z diagonal: 1e-8 v3 v3^T on the rotations
+            if (a == b)
 #pragma unroll
-        for (int v = 0; v < 5; v++)
+                for (int i = 0; i < 3; i++)
 #pragma unroll
-            for (int w = 0; w < 5; w++) {
-                double kv = K[v * 5 + w];
-                if (A.con && (A.con[int64_t(ra) * 5 + v] || A.con[int64_t(rb) * 5 + w]))
+                    for (int j = 0; j < 3; j++) K[(3 + i) * 6 + 3 + j] += 1.0e-8 * T[i * 3 + 2] * T[j * 3 + 2];
+        }
+        double* out = A.Kel + (e * 16 + lane) * (NAV * NAV);
+#pragma unroll
+        for (int v = 0; v < NAV; v++)
+#pragma unroll
+            for (int w = 0; w < NAV; w++) {
+                double kv = K[v * NAV + w];
+                if (A.con && (A.con[int64_t(ra) * NAV + v] || A.con[int64_t(rb) * NAV + w]))
                     kv = (a == b && v == w) ? 1.0 : 0.0;
-                out[v * 5 + w] = kv;
+                out[v * NAV + w] = kv;
             }
     }
 }
 
 // g_e = lambda_e . dR_e/dh_e with the section derivatives
 // dA = Dps, dB = Dps off, dDb = Dps (h^2/4 + off^2), dS = G kappa
-// (..._StiffnessMatrix::derivative, solid_2d_section_element_property_card.cpp:105-157, 393-531)
+// (..._StiffnessMatrix::derivative, solid_2d_section_element_property_card.cpp:105-157, 393-531);
+// NAV 6 out of the x-y plane: u and lambda taken to the element frame by T^T
+template <int NAV>
 __global__ __launch_bounds__(128) void k_quad4_mindlin_sens(Quad4Args A, const double* __restrict__ lam,
                                                             double* __restrict__ gout) {
     const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (e >= A.n_elems) return;
-    double X[4][2], u[4][5], l[4][5];
+    double X[4][2], XL[4][3], u[4][NAV], l[4][NAV];
+    double XG[4][3];
 #pragma unroll
     for (int n = 0; n < 4; n++) {
         const int64_t r = A.econn[e * 4 + n];
-        X[n][0] = A.xyz[r * 3 + 0];
-        X[n][1] = A.xyz[r * 3 + 1];
+        X[n][0] = XG[n][0] = A.xyz[r * 3 + 0];
+        X[n][1] = XG[n][1] = A.xyz[r * 3 + 1];
+        XG[n][2] = A.xyz[r * 3 + 2];
 #pragma unroll
-        for (int v = 0; v < 5; v++) {
-            u[n][v] = A.X[r * 5 + v];
-            l[n][v] = A.con[r * 5 + v] ? 0.0 : lam[r * 5 + v];
+        for (int v = 0; v < NAV; v++) {
+            u[n][v] = A.X[r * NAV + v];
+            l[n][v] = A.con[r * NAV + v] ? 0.0 : lam[r * NAV + v];
         }
     }
+    if constexpr (NAV == 6) {
+        double T[9];
+        if (quad4_frame(XG, T, XL)) {
+#pragma unroll
+            for (int n = 0; n < 4; n++)
+#pragma unroll
+                for (int blk = 0; blk < 6; blk += 3) {
+                    double tu[3], tl[3];
+#pragma unroll
+                    for (int j = 0; j < 3; j++) {
+                        tu[j] = T[0 * 3 + j] * u[n][blk] + T[1 * 3 + j] * u[n][blk + 1] + T[2 * 3 + j] * u[n][blk + 2];
+                        tl[j] = T[0 * 3 + j] * l[n][blk] + T[1 * 3 + j] * l[n][blk + 1] + T[2 * 3 + j] * l[n][blk + 2];
+                    }
+#pragma unroll
+                    for (int j = 0; j < 3; j++) {
+                        u[n][blk + j] = tu[j];
+                        l[n][blk + j] = tl[j];
+                    }
+                }
+        }
+    }
+    auto grad = [&](double xi, double eta, double g[4][2], double N[4]) -> double {
+        if constexpr (NAV == 6) return quad4_grad_surface(XL, xi, eta, g, N);
+        else return quad4_grad(X, xi, eta, g, N);
+    };
     const double h = A.param[e];
     const double dA = 1.0, dB = A.off, dD = h * h / 4.0 + A.off * A.off, dS = A.G * A.kappa;
     double acc = 0.0;
 #pragma unroll
     for (int q = 0; q < 4; q++) {
         double g[4][2], N[4];
-        const double c = quad4_grad(X, gp_coord(q & 1), gp_coord(q >> 1), g, N) * 1.0;
+        const double c = grad(gp_coord(q & 1), gp_coord(q >> 1), g, N) * 1.0;
         double em[3] = {0, 0, 0}, kb[3] = {0, 0, 0}, lm[3] = {0, 0, 0}, lk[3] = {0, 0, 0};
 #pragma unroll
         for (int n = 0; n < 4; n++) {
@@ -562,7 +639,7 @@ __global__ __launch_bounds__(128) void k_quad4_mindlin_sens(Quad4Args A, const d
     }
     {
         double g[4][2], N[4];
-        const double c = quad4_grad(X, 0.0, 0.0, g, N) * 4.0;
+        const double c = grad(0.0, 0.0, g, N) * 4.0;
         double gu[2] = {0, 0}, gl[2] = {0, 0};
 #pragma unroll
         for (int n = 0; n < 4; n++) {
@@ -1320,10 +1397,16 @@ void launch_hex8_sens(const Hex8Args& a, const double* lam, double* g, hipStream
     hipLaunchKernelGGL(k_hex8_elastic_sens, dim3(blocks_for(a.n_elems, 128)), dim3(128), 0, s, a, lam, g);
 }
 void launch_quad4_K(const Quad4Args& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_quad4_mindlin_K, dim3(blocks_for(a.n_elems, 16)), dim3(256), 0, s, a);
+    if (a.nav == 6)
+        hipLaunchKernelGGL(k_quad4_mindlin_K<6>, dim3(blocks_for(a.n_elems, 16)), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_quad4_mindlin_K<5>, dim3(blocks_for(a.n_elems, 16)), dim3(256), 0, s, a);
 }
 void launch_quad4_sens(const Quad4Args& a, const double* lam, double* g, hipStream_t s) {
-    hipLaunchKernelGGL(k_quad4_mindlin_sens, dim3(blocks_for(a.n_elems, 128)), dim3(128), 0, s, a, lam, g);
+    if (a.nav == 6)
+        hipLaunchKernelGGL(k_quad4_mindlin_sens<6>, dim3(blocks_for(a.n_elems, 128)), dim3(128), 0, s, a, lam, g);
+    else
+        hipLaunchKernelGGL(k_quad4_mindlin_sens<5>, dim3(blocks_for(a.n_elems, 128)), dim3(128), 0, s, a, lam, g);
 }
 void launch_heat_K(const HeatArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_hex8_heat_K, dim3(blocks_for(a.n_elems, 4)), dim3(256), 0, s, a);
@@ -1381,24 +1464,28 @@ void launch_spmv_csr(const CGArgs& c, const double* x, double* y, int want_dot, 
 }
 // node-block FP32 SpMV: 8 lanes per node row (a hex8 interior node has 27
 // blocks, a quad4 one 9), grid of the CSR SpMV
+template <typename T>
+static void nb_launch(const CGArgs& c, const T* vals, const double* x, double* y, int want_dot, int in_loop,
+                      hipStream_t s) {
+    switch (c.nav) {
+        case 1: hipLaunchKernelGGL((k_spmv_nb_dot<T, 1, 8>), dim3(c.nparts), dim3(256), 0, s, c, vals, x, y,
+                                   want_dot, in_loop); break;
+        case 5: hipLaunchKernelGGL((k_spmv_nb_dot<T, 5, 8>), dim3(c.nparts), dim3(256), 0, s, c, vals, x, y,
+                                   want_dot, in_loop); break;
+        case 6: hipLaunchKernelGGL((k_spmv_nb_dot<T, 6, 8>), dim3(c.nparts), dim3(256), 0, s, c, vals, x, y,
+                                   want_dot, in_loop); break;
+        default: hipLaunchKernelGGL((k_spmv_nb_dot<T, 3, 8>), dim3(c.nparts), dim3(256), 0, s, c, vals, x, y,
+                                    want_dot, in_loop); break;
+    }
+}
 static void nb32_dispatch(const CGArgs& c, const double* x, double* y, int want_dot, int in_loop,
                           hipStream_t s) {
-    if (c.nav == 5)
-        hipLaunchKernelGGL((k_spmv_nb_dot<float, 5, 8>), dim3(c.nparts), dim3(256), 0, s, c, c.val32, x, y, want_dot,
-                           in_loop);
-    else
-        hipLaunchKernelGGL((k_spmv_nb_dot<float, 3, 8>), dim3(c.nparts), dim3(256), 0, s, c, c.val32, x, y, want_dot,
-                           in_loop);
+    nb_launch<float>(c, c.val32, x, y, want_dot, in_loop, s);
 }
 // the FP64 operator over node blocks (plates: no column index per value)
 static void nb64_dispatch(const CGArgs& c, const double* x, double* y, int want_dot, int in_loop,
                           hipStream_t s) {
-    if (c.nav == 5)
-        hipLaunchKernelGGL((k_spmv_nb_dot<double, 5, 8>), dim3(c.nparts), dim3(256), 0, s, c, c.val, x, y, want_dot,
-                           in_loop);
-    else
-        hipLaunchKernelGGL((k_spmv_nb_dot<double, 3, 8>), dim3(c.nparts), dim3(256), 0, s, c, c.val, x, y, want_dot,
-                           in_loop);
+    nb_launch<double>(c, c.val, x, y, want_dot, in_loop, s);
 }
 void launch_spmv_nb32(const CGArgs& c, const double* x, double* y, int want_dot, hipStream_t s) {
     nb32_dispatch(c, x, y, want_dot, 0, s);

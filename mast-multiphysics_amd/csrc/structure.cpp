@@ -33,8 +33,21 @@ std::string build_structure(int physics, int64_t n_nodes, int64_t n_elems, const
         for (int v = 0; v < 3; v++) S.active_var[v] = v;
     } else if (physics == 2) {
         if (n_vars != 5 && n_vars != 6) return "Mindlin plate needs 5 or 6 variables per node";
-        S.nav = 5;
-        for (int v = 0; v < 5; v++) S.active_var[v] = v;
+        // an element whose nodes do not share one z is computed in its own
+        // frame, which couples theta_z: then all six variables are active
+        bool framed = false;
+        for (int64_t e = 0; e < n_elems && !framed; e++) {
+            const int32_t* en = conn + e * 4;
+            bool in_range = true;
+            for (int a = 0; a < 4; a++) in_range = in_range && en[a] >= 0 && en[a] < n_nodes;
+            if (!in_range) return "connectivity out of range";
+            const double z0 = xyz[int64_t(en[0]) * 3 + 2];
+            for (int a = 1; a < 4; a++) framed = framed || xyz[int64_t(en[a]) * 3 + 2] != z0;
+        }
+        if (framed && n_vars != 6)
+            return "plate elements outside a z = const plane need 6 variables per node (theta_z couples)";
+        S.nav = framed ? 6 : 5;
+        for (int v = 0; v < S.nav; v++) S.active_var[v] = v;
     } else {
         if (n_vars != 1) return "heat conduction needs 1 variable per node";
         S.nav = 1;
