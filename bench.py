@@ -1,12 +1,16 @@
 """Benchmark of the MAST element-assembly + solve + sensitivity path on one GPU.
 
-One step = one design evaluation of the 128x64x64 hex8 SIMP cantilever
-(1.62M DOF, rho_e ~ U[0.01, 1), p = 3): load vector, solve to
-||b - K u|| / ||b|| <= 1e-10, compliance and dC/drho_e for all 524,288
-elements, with the mesh and its structure resident on the device.  The
-headline path is the multigrid-preconditioned FP64 CG, matrix-free on this
-lattice mesh (the element class matrix, scaled per element, stands in for
-the assembled K; no element or global matrix is formed).  The paths that
+A design evaluation of the 128x64x64 hex8 SIMP cantilever (1.62M DOF,
+rho_e ~ U[0.01, 1), p = 3) is: load vector, solve to ||b - K u|| / ||b|| <=
+1e-10, compliance and dC/drho_e for all 524,288 elements, with the mesh and
+its structure resident on the device.  The headline path is the
+multigrid-preconditioned FP64 CG, matrix-free on this lattice mesh (the
+element class matrix, scaled per element, stands in for the assembled K; no
+element or global matrix is formed), evaluating --lanes designs at once
+(mastg_batch_eval: one context, HIP stream and host thread per lane, so one
+design's latency-bound coarse levels overlap another's fine-level kernels):
+one step = one batch of --lanes designs, value = designs/s over the timed
+batches; mg_designs_per_s is the same path one design at a time.  The paths that
 apply to any hex8 mesh are timed beside it: the plain FP64 path (element
 K + R, scatter into the global CSR, CG + Jacobi) and the mixed-precision
 path (the same FP64 assembly, CG on an FP32 copy of the matrix with FP64
@@ -129,6 +133,58 @@ def distorted_context(w, amp=0.25, seed=7):
     return mastg.Context(desc)
 
 
+def batched_mg_bench(w, designs, steps, warmup, rtol=1e-10):
+    """Several designs of the mesh at once (mastg_batch_*): one context, HIP
+    stream and host thread per design of `designs`, multigrid design
+    evaluations; one step = one batch.  Warm-up: at least two batches (the
+    first runs design 0 alone on lane 0 and shares its launch shapes; every
+    lane's solver graphs are captured by the end of the second).  Lanes beyond
+    the process's hardware queues (GPU_MAX_HW_QUEUES, 4 by default) share
+    queues and serialise."""
+    lanes = len(designs)
+    B = mastg.Batch(w.desc, lanes)
+    for _ in range(max(warmup, 2)):
+        B.eval(designs, solver=mastg.SOLVER_MG, rtol=rtol)
+    lat, res = [], []
+    t0 = time.perf_counter()
+    for i in range(steps):
+        t1 = time.perf_counter()
+        res = B.eval(designs, solver=mastg.SOLVER_MG, rtol=rtol)
+        lat += [r["seconds"] for r in res]
+        print(f"# mg batch step {i}: {(time.perf_counter() - t1) * 1e3:.1f} ms for {lanes} designs, "
+              f"iters {[r['iters'] for r in res]}", file=sys.stderr, flush=True)
+    elapsed = time.perf_counter() - t0
+    B.close()
+    return dict(elapsed=elapsed, lat=lat, res=res, dps=steps * lanes / elapsed, lanes=lanes)
+
+
+def batch_designs(params, lanes, seed):
+    """The seeded design, then seeded permutations of it (the same values, so
+    the same distribution)."""
+    rng = np.random.default_rng(seed + 7919)
+    return [params] + [rng.permutation(params) for _ in range(lanes - 1)]
+
+
+def batch_summary(batch, steps, single):
+    """The batched leg's JSON object; `single` = the same path's last
+    one-at-a-time result on design 0."""
+    b0 = batch["res"][0]
+    return {
+        "designs_per_s": batch["dps"], "lanes": batch["lanes"], "designs_per_step": batch["lanes"],
+        "ms_per_step": batch["elapsed"] / steps * 1e3,
+        "iters": [r["iters"] for r in batch["res"]],
+        "design0_objective": b0["objective"],
+        "design0_bitwise_equal_single_context": bool(b0["objective"] == single["objective"] and
+                                                     np.array_equal(b0["grad"], single["grad"])),
+        "design0_rel_diff_single_context": {
+            "objective": abs(b0["objective"] - single["objective"]) / abs(single["objective"]),
+            "grad": float(np.max(np.abs(b0["grad"] - single["grad"])) / np.max(np.abs(single["grad"])))},
+        "note": ("value: designs evaluated at once by mastg_batch_eval, one context, HIP stream and host "
+                 "thread per lane, every design a complete evaluation (assembly, multigrid-preconditioned "
+                 "solve, objective, adjoint sensitivities); design 0 is the seeded design, the others "
+                 "seeded permutations of it; p50/p99 are per-design wall times within a batch")}
+
+
 def general_mesh_bench(w, params, steps):
     """Designs/s on the distorted mesh: multigrid (assembled FP64 matrix, FP32
     node-block smoothing, Galerkin coarse levels from the element matrices)
@@ -225,6 +281,10 @@ def physics_cpu_baseline(kind, n_sub, iters, n_elems, nnz, newton):
 def physics_main(a):
     kind, nx, ny, nz, rtol, desc = PHYSICS[a.workload]
     w = mastg.Workload(kind, nx, ny, nz, seed=a.seed)
+    batch = None
+    if a.solver in ("mg", "all") and a.lanes > 1:
+        # before any other context exists (a hardware queue per lane)
+        batch = batched_mg_bench(w, batch_designs(w.params, a.lanes, a.seed), a.steps, a.warmup, rtol)
     t0 = time.perf_counter()
     ctx = mastg.Context(w.desc)
     t_setup = time.perf_counter() - t0
@@ -266,16 +326,21 @@ def physics_main(a):
         byt = spmv_bytes(ctx.n_active, ctx.nnz)
         kname = "k_spmv_csr_dot (FP64 CSR SpMV)"
     achieved = byt / (k_ms * 1e-3) / 1e9
+    single_dps = a.steps / elapsed
+    if batch:
+        lat = batch["lat"]
+        elapsed = batch["elapsed"]
     out = {
         "metric": "elem K+R+dR/dp assemblies/s; designs/s at rel err<=1e-6; p50/p99 latency",
-        "value": a.steps / elapsed, "unit": "designs/s", "n_gpus": 1, "steps": a.steps,
+        "value": batch["dps"] if batch else single_dps, "unit": "designs/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": elapsed / a.steps * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (seeded SplitMix64 parameters)",
         "config": {"workload": desc, "elements": w.n_elems, "active_dofs": ctx.n_active, "nnz": ctx.nnz,
-                   "rtol": rtol,
+                   "rtol": rtol, "designs_per_step": batch["lanes"] if batch else 1,
                    "solver": ("CG" if kind == 2 else "Newton + BiCGStab, adjoint BiCGStab on J^T") +
                              (" + multigrid V-cycle" if sv == mastg.SOLVER_MG else " + Jacobi")},
+        "single_design_designs_per_s": single_dps,
         "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
         "krylov_iters": res[-1]["iters"], "newton_iters": res[-1]["newton_iters"],
         "rel_res": res[-1]["rel_res"], "objective": res[-1]["objective"],
@@ -285,6 +350,8 @@ def physics_main(a):
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": None,
                      "avg_launch_ms": k_ms, "algorithmic_bytes": byt},
     }
+    if batch:
+        out["mg_batched"] = batch_summary(batch, a.steps, res[-1])
     if plain is not None:
         out["jacobi_preconditioned"] = {"designs_per_s": 1.0 / plain["s"], "iters": plain["iters"],
                                   "speedup_of_value": out["value"] * plain["s"],
@@ -398,6 +465,9 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch the solver kernels directly instead of replaying hipGraphs")
+    ap.add_argument("--lanes", type=int, default=3,
+                    help="multigrid designs evaluated at once (mastg_batch_*: one context, stream and host "
+                         "thread per lane); 1 = one design at a time")
     ap.add_argument("--kernel-profile", action="store_true",
                     help="only assemble the workload and launch the lattice operator, the FP32 smoothing "
                          "step, the 256 MiB copy kernel and the fused row assembly (on the mesh and on a "
@@ -417,6 +487,11 @@ def main():
         return
     nx, ny, nz, lo, hi = WORKLOADS[a.workload]
     w = mastg.Workload(1, nx, ny, nz, seed=a.seed, lo=lo, hi=hi)
+    # the batched leg runs before any other context exists: each lane's
+    # stream then has a hardware queue of its own (4 per process by default)
+    batch = None
+    if not a.kernel_profile and a.solver in ("mg", "all") and a.lanes > 1:
+        batch = batched_mg_bench(w, batch_designs(w.params, a.lanes, a.seed), a.steps, a.warmup)
     t0 = time.perf_counter()
     ctx = mastg.Context(w.desc)
     t_setup = time.perf_counter() - t0
@@ -474,6 +549,10 @@ def main():
     lat = H["lat"]
     elapsed = H["elapsed"]
     designs_per_s = H["dps"]
+    if batch:
+        lat = batch["lat"]
+        elapsed = batch["elapsed"]
+        designs_per_s = batch["dps"]
     # element K + R + dR/drho rate (device events): the whole FP64 assembly at
     # a state (element kernel, scatter into the CSR, inverse diagonal, node
     # residual) plus the sensitivity pass
@@ -545,6 +624,7 @@ def main():
         "config": {"workload": f"hex8 SIMP cantilever {nx}x{ny}x{nz}", "elements": w.n_elems,
                    "active_dofs": ctx.n_active, "nnz": ctx.nnz, "solver": head,
                    "rtol": 1e-10, "rho": [lo, hi], "penalty": 3,
+                   "designs_per_step": a.lanes if batch else 1,
                    "spmv_lanes_per_row": lanes, "grid": grid},
         "elem_assemblies_per_s": elem_rate,
         "assembly_ms": {"K_R_scatter_residual": t_asm_ms, "sensitivity": t_sens_ms},
@@ -556,6 +636,10 @@ def main():
         "compliance": res[-1]["objective"],
         "roofline": roof,
     }
+    if batch:
+        out["mg_batched"] = batch_summary(batch, a.steps, res[-1])
+        if "plain" in runs:
+            out["mg_batched"]["speedup_vs_plain_fp64"] = batch["dps"] / runs["plain"]["dps"]
     for name, R in runs.items():
         out[f"{name}_designs_per_s"] = R["dps"]
         out[f"{name}_cg_iters"] = R["res"][-1]["iters"]

@@ -310,6 +310,31 @@ typedef struct {
 int mastg_design_eval(mastg_ctx* ctx, const double* params, double* dobj, int where,
                       const mastg_solve_opts* opts, mastg_design_result* out);
 
+/* ------------------------------------ batched design evaluations ---- */
+/* Several designs of one mesh at once.  MAST evaluates one design per
+ * FunctionEvaluation::evaluate (src/optimization/function_evaluation.h:107;
+ * examples/structural/example_6/example_6.cpp:450); a batch holds n_lanes
+ * (1..16) contexts over the same description, each with its own HIP stream
+ * and host thread, and the designs of one mastg_batch_eval are taken in turn
+ * by whichever lane is free, so their solves overlap on the device.  The
+ * first design of the first call runs on lane 0 alone and every lane then
+ * keeps lane 0's launch shapes: a design's result is bitwise what one
+ * context alone gives, whatever its lane and whatever runs beside it.
+ * params / dobj are [n_designs][n_elems] (dobj may be null), out and seconds
+ * (each design's wall time on its lane) are [n_designs] or null; `where` as
+ * for mastg_design_eval.  Every design is evaluated; the return value is the
+ * error of the lowest-numbered failing design (its message in
+ * mastg_batch_last_error).  Calls on one batch must not overlap. */
+typedef struct mastg_batch mastg_batch;
+int mastg_batch_create(const mastg_problem_desc* desc, int n_lanes, mastg_batch** out);
+int mastg_batch_destroy(mastg_batch* b);
+int mastg_batch_lanes(const mastg_batch* b);
+int mastg_batch_lane(mastg_batch* b, int lane, mastg_ctx** ctx);   /* a lane's context (read-mostly use) */
+int mastg_batch_set_option(mastg_batch* b, int option, int value);  /* mastg_set_option on every lane */
+int mastg_batch_eval(mastg_batch* b, int n_designs, const double* params, double* dobj, int where,
+                     const mastg_solve_opts* opts, mastg_design_result* out, double* seconds);
+const char* mastg_batch_last_error(mastg_batch* b);
+
 /* device vector access for callers that keep data resident */
 int mastg_device_alloc(mastg_ctx* ctx, size_t bytes, void** ptr);
 int mastg_device_free(mastg_ctx* ctx, void* ptr);

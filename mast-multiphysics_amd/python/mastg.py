@@ -129,6 +129,14 @@ def lib():
             "mastg_get_tuning": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
             "mastg_set_option": (C.c_int, [_P, C.c_int, C.c_int]),
             "mastg_set_tuning": (C.c_int, [_P, C.c_int, C.c_int]),
+            "mastg_batch_create": (C.c_int, [C.POINTER(ProblemDesc), C.c_int, C.POINTER(_P)]),
+            "mastg_batch_destroy": (C.c_int, [_P]),
+            "mastg_batch_lanes": (C.c_int, [_P]),
+            "mastg_batch_lane": (C.c_int, [_P, C.c_int, C.POINTER(_P)]),
+            "mastg_batch_set_option": (C.c_int, [_P, C.c_int, C.c_int]),
+            "mastg_batch_eval": (C.c_int, [_P, C.c_int, _D, _D, C.c_int, C.POINTER(SolveOpts),
+                                           C.POINTER(DesignResult), _D]),
+            "mastg_batch_last_error": (C.c_char_p, [_P]),
             "mastg_server_create": (C.c_int, [C.POINTER(ServerOpts), C.POINTER(_P)]),
             "mastg_server_submit": (C.c_int, [_P, C.POINTER(Request), C.POINTER(C.c_int64)]),
             "mastg_server_poll": (C.c_int, [_P, C.c_int64, C.POINTER(RequestResult)]),
@@ -290,9 +298,24 @@ class Context:
         self.n_dofs, self.n_active, self.nnz = s.n_dofs, s.n_active, s.nnz
         self.n_elems = desc.contents.n_elems if isinstance(desc, C.POINTER(ProblemDesc)) else desc.n_elems
 
+    @classmethod
+    def _borrow(cls, h, n_elems):
+        """A view of a context another object owns (a batch lane): closing it
+        does not destroy the context."""
+        ctx = cls.__new__(cls)
+        ctx.h = h
+        ctx._borrowed = True
+        s = Sizes()
+        ctx._chk(lib().mastg_get_sizes(h, C.byref(s)))
+        ctx.sizes = s
+        ctx.n_dofs, ctx.n_active, ctx.nnz = s.n_dofs, s.n_active, s.nnz
+        ctx.n_elems = n_elems
+        return ctx
+
     def close(self):
         if getattr(self, "h", None) and _lib is not None:
-            _lib.mastg_ctx_destroy(self.h)
+            if not getattr(self, "_borrowed", False):
+                _lib.mastg_ctx_destroy(self.h)
             self.h = None
 
     __del__ = close
@@ -487,6 +510,59 @@ class Design:
         self._chk(lib().mastg_design_oc_update(self.h, _p(rho), _p(dc), _p(dv), volfrac, move, rho_min,
                                                _p(out), HOST, C.byref(lam)))
         return out, lam.value
+
+
+class Batch:
+    """Batched design evaluations (mastg_batch_*): n_lanes contexts over one
+    description, each on its own HIP stream and host thread; the designs of
+    one eval() call are taken in turn by whichever lane is free."""
+
+    def __init__(self, desc, n_lanes=2):
+        h = _P()
+        dp = desc if isinstance(desc, C.POINTER(ProblemDesc)) else C.byref(desc)
+        rc = lib().mastg_batch_create(dp, n_lanes, C.byref(h))
+        if rc:
+            raise MastgError(rc, lib().mastg_last_error(None).decode())
+        self.h = h
+        self.n_lanes = n_lanes
+        self.n_elems = desc.contents.n_elems if isinstance(desc, C.POINTER(ProblemDesc)) else desc.n_elems
+
+    def _chk(self, rc):
+        if rc:
+            raise MastgError(rc, lib().mastg_batch_last_error(self.h).decode())
+        return rc
+
+    def lane(self, i):
+        c = _P()
+        self._chk(lib().mastg_batch_lane(self.h, i, C.byref(c)))
+        return Context._borrow(c, self.n_elems)
+
+    def set_option(self, option, value):
+        self._chk(lib().mastg_batch_set_option(self.h, option, value))
+
+    def eval(self, designs, solver=SOLVER_MG, rtol=1e-10, max_iters=0, reliable_delta=0.0, check_every=0,
+             want_grad=True, raise_on_error=True):
+        """designs: [n][n_elems]; a list of per-design result dicts (as
+        Context.design_eval's, plus 'seconds', the design's wall time)."""
+        P = np.ascontiguousarray(np.asarray(designs, np.float64).reshape(-1, self.n_elems))
+        n = P.shape[0]
+        G = np.zeros_like(P) if want_grad else None
+        o = SolveOpts(solver, max_iters, rtol, reliable_delta, 0, check_every)
+        out = (DesignResult * max(n, 1))()
+        secs = np.zeros(max(n, 1))
+        rc = lib().mastg_batch_eval(self.h, n, _p(P), _p(G), HOST, C.byref(o), out, _p(secs))
+        if raise_on_error:
+            self._chk(rc)
+        return [dict(objective=r.objective, iters=r.iters, newton_iters=r.newton_iters, rel_res=r.rel_res,
+                     t_assemble=r.t_assemble, t_solve=r.t_solve, t_sens=r.t_sens, seconds=float(secs[i]),
+                     grad=None if G is None else G[i], rc=rc) for i, r in enumerate(out[:n])]
+
+    def close(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.mastg_batch_destroy(self.h)
+            self.h = None
+
+    __del__ = close
 
 
 class Server:
