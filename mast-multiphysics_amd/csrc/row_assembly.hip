@@ -265,8 +265,11 @@ __global__ __launch_bounds__(64) void k_hex8_row_assemble(Hex8RowArgs A) {
 // by the lanes (s, b) that place neighbour b of slot s in the row.  Each
 // work-group runs over a contiguous range of nodes, so the one-class load is
 // amortised.
+// NP = 2: 5 waves per SIMD at 93 VGPRs without spills (the compiler's own
+// choice is 99 VGPRs, 4 waves); NP = 3 spills under that bound and keeps its own
 template <int NP, bool ONE>
-__global__ __launch_bounds__(64) void k_hex8_row_assemble_class(Hex8RowArgs A) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NP == 2 ? 5 : 1, 8))) void k_hex8_row_assemble_class(
+    Hex8RowArgs A) {
     // a lane owns NP (row variable v, neighbour block k) pairs p = v deg + k,
     // i.e. the 3 consecutive row values 3 p .. 3 p + 2 (w = 0, 1, 2)
     constexpr int MAXB = NP * 64 / 3;
