@@ -125,6 +125,7 @@ struct mastg_ctx {
     DevBuf ninc_ptr, ninc;
     // matrix-free operator data (hex8 congruence classes)
     DevBuf Khat, sE, eclass, nodecon, Fel;
+    DevBuf heat_geo;            // heat: per congruence class gradients and JxW
     DevBuf Khat32;              // FP32 copy of Khat (lattice smoothing)
     DevBuf Gc;                  // [8][576] Q_c^T Khat Q_c (lattice Galerkin products)
     // heat: convective side masks, mean-T weights, BiCGStab work space, A^T values
@@ -298,6 +299,8 @@ HeatArgs heat_args(mastg_ctx* c, const double* X, bool want_R, bool want_K) {
     a.rad_Tamb = c->d.rad_tamb;
     a.rad_T0 = c->d.rad_tzero;
     a.src_scale = 1.0;
+    a.geo = c->heat_geo.as<double>();
+    a.eclass = c->heat_geo.p ? c->eclass.as<uint8_t>() : nullptr;
     a.con = c->con.as<uint8_t>();
     a.X = X;
     a.Rel = want_R ? c->Rel.as<double>() : nullptr;
@@ -1734,6 +1737,22 @@ int mastg_ctx_create(const mastg_problem_desc* desc, mastg_ctx** out) {
         if ((rc = upload(c, c->sidemask, sm)) || (rc = upload(c, c->wmean, w)) || (rc = upload(c, c->tpos, tp)) ||
             (rc = dalloc(c, c->vT, S.n_active * sizeof(double))) || (rc = ensure_bicg(c)))
             return bail(rc);
+    }
+    if (desc->physics == MASTG_HEX8_HEAT && S.n_classes > 0) {
+        // the geometry of each congruence class once (gradients and JxW at the
+        // 8 points of its representative element): the element kernel then
+        // forms no Jacobian per element
+        DevBuf rep_d;
+        if ((rc = upload(c, rep_d, S.class_rep)) || (rc = upload(c, c->eclass, S.elem_class)) ||
+            (rc = dalloc(c, c->heat_geo, size_t(S.n_classes) * kHeatGeo * sizeof(double))))
+            return bail(rc);
+        launch_heat_geometry(S.n_classes, c->xyz.as<double>(), c->econn.as<int32_t>(), rep_d.as<int32_t>(),
+                             c->heat_geo.as<double>(), c->stream);
+        if (hipStreamSynchronize(c->stream) != hipSuccess) {
+            c->err = "class geometry kernel failed";
+            return bail(MASTG_ERR_DEVICE);
+        }
+        c->device_bytes -= int64_t(rep_d.bytes);
     }
     if (desc->physics == MASTG_HEX8_ELASTIC && S.n_classes > 0) {
         // class reference matrices at E = 1 from the element kernel itself

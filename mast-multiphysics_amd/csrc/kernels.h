@@ -53,7 +53,12 @@ struct HeatArgs {
     const uint8_t* rsides;  // [n_elems] bit s = side s radiates
     double rad_se, rad_Tamb, rad_T0;
     double src_scale;       // multiplies the source q_e (1; 0 leaves the source out)
+    // congruence classes: per class kHeatGeo doubles (gradients [q][n][d],
+    // then JxW [q]); null = geometry per element
+    const double* geo;
+    const uint8_t* eclass;  // [n_elems] class of each element (with geo)
 };
+constexpr int kHeatGeo = 8 * 8 * 3 + 8;
 
 // fused hex8 assembly by CSR row owner (K, R, scatter, inverse diagonal)
 struct Hex8RowArgs {
@@ -220,6 +225,8 @@ void launch_scatter_f32(const ScatterArgs& a, const float* Kel32, hipStream_t s)
 void launch_scatter(const ScatterArgs& a, hipStream_t s);
 void launch_quad4_K(const Quad4Args& a, hipStream_t s);
 void launch_heat_K(const HeatArgs& a, hipStream_t s);
+void launch_heat_geometry(int n_classes, const double* xyz, const int32_t* econn, const int32_t* class_rep,
+                          double* geo, hipStream_t s);
 void launch_heat_source_dot(const HeatArgs& a, const double* lam, double scale, double* g, hipStream_t s);
 void launch_quad4_sens(const Quad4Args& a, const double* lam, double* g, hipStream_t s);
 void launch_node_residual(int64_t n_nodes, int nav, int npe, const int32_t* ninc_ptr,

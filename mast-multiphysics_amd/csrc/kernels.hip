@@ -679,6 +679,48 @@ __device__ __forceinline__ void hex8_side_xi(int side, double s1, double s2, dou
     }
 }
 
+// lane (q, n) of a wave: node n's coordinates, the physical gradient g and
+// value N of N_n at quadrature point q of element ee, and det J at q (the
+// Jacobian summed over the 8 node lanes of q)
+__device__ __forceinline__ void heat_lane_geometry(const double* __restrict__ xyz, const int32_t* __restrict__ econn,
+                                                   int64_t ee, int q, int n, double xn[3], double g[3], double& N,
+                                                   double& jac) {
+    const int32_t rn = econn[ee * 8 + n];
+    xn[0] = xyz[int64_t(rn) * 3 + 0];
+    xn[1] = xyz[int64_t(rn) * 3 + 1];
+    xn[2] = xyz[int64_t(rn) * 3 + 2];
+    double dxi, deta, dzeta;
+    hex8_dshape(n, gp_coord(q & 1), gp_coord((q >> 1) & 1), gp_coord(q >> 2), dxi, deta, dzeta, N);
+    double J[9] = {xn[0] * dxi, xn[0] * deta, xn[0] * dzeta, xn[1] * dxi, xn[1] * deta, xn[1] * dzeta,
+                   xn[2] * dxi, xn[2] * deta, xn[2] * dzeta};
+#pragma unroll
+    for (int m = 1; m < 8; m <<= 1)
+#pragma unroll
+        for (int i = 0; i < 9; i++) J[i] += __shfl_xor(J[i], m, 64);
+    double ixi[3], ieta[3], izeta[3];
+    jac = inv3(J, ixi, ieta, izeta);
+#pragma unroll
+    for (int d = 0; d < 3; d++) g[d] = dxi * ixi[d] + deta * ieta[d] + dzeta * izeta[d];
+}
+
+// the geometry of each congruence class (its representative element): per
+// class 8 x 8 x 3 gradients [q][n][d] then 8 JxW; one wave per class
+__global__ __launch_bounds__(64) void k_hex8_heat_geometry(const double* __restrict__ xyz,
+                                                           const int32_t* __restrict__ econn,
+                                                           const int32_t* __restrict__ class_rep,
+                                                           double* __restrict__ geo) {
+    const int cls = blockIdx.x, lane = threadIdx.x, q = lane >> 3, n = lane & 7;
+    double xn[3], g[3], N, jac;
+    heat_lane_geometry(xyz, econn, class_rep[cls], q, n, xn, g, N, jac);
+    double* out = geo + int64_t(cls) * kHeatGeo;
+#pragma unroll
+    for (int d = 0; d < 3; d++) out[(q * 8 + n) * 3 + d] = g[d];
+    if (n == 0) out[192 + q] = jac * 1.0;
+}
+
+// CLASS: the element's geometry is its congruence class's (A.geo, A.eclass:
+// no Jacobian per element); else formed per element
+template <bool CLASS>
 __global__ __launch_bounds__(256) void k_hex8_heat_K(HeatArgs A) {
     __shared__ double sg[4][8][8][3];
     __shared__ double sN[4][8][8];
@@ -692,31 +734,37 @@ __global__ __launch_bounds__(256) void k_hex8_heat_K(HeatArgs A) {
     const int64_t ee = valid ? e : 0;
     {
         const int q = lane >> 3, n = lane & 7;
-        const int32_t rn = A.econn[ee * 8 + n];
-        const double x = A.xyz[int64_t(rn) * 3 + 0], y = A.xyz[int64_t(rn) * 3 + 1],
-                     z = A.xyz[int64_t(rn) * 3 + 2];
-        if (q == 0) {
-            sT[wave][n] = A.X[rn];
-            sX[wave][n][0] = x; sX[wave][n][1] = y; sX[wave][n][2] = z;
+        if (CLASS) {
+            const double* __restrict__ G = A.geo + int64_t(A.eclass[ee]) * kHeatGeo;
+            sg[wave][q][n][0] = G[lane * 3 + 0];
+            sg[wave][q][n][1] = G[lane * 3 + 1];
+            sg[wave][q][n][2] = G[lane * 3 + 2];
+            if (n == 0) sc[wave][q] = G[192 + q];
+            double d0, d1, d2, N;
+            hex8_dshape(n, gp_coord(q & 1), gp_coord((q >> 1) & 1), gp_coord(q >> 2), d0, d1, d2, N);
+            sN[wave][q][n] = N;
+            if (q == 0) {
+                const int32_t rn = A.econn[ee * 8 + n];
+                sT[wave][n] = A.X[rn];
+                // node coordinates only for the faces of elements on marked sides
+                const bool faces = (A.sides && A.sides[ee]) || (A.rsides && A.rsides[ee]);
+                if (faces)
+#pragma unroll
+                    for (int d = 0; d < 3; d++) sX[wave][n][d] = A.xyz[int64_t(rn) * 3 + d];
+            }
+        } else {
+            double xn[3], g[3], N, jac;
+            heat_lane_geometry(A.xyz, A.econn, ee, q, n, xn, g, N, jac);
+            if (q == 0) {
+                sT[wave][n] = A.X[A.econn[ee * 8 + n]];
+                sX[wave][n][0] = xn[0]; sX[wave][n][1] = xn[1]; sX[wave][n][2] = xn[2];
+            }
+            sg[wave][q][n][0] = g[0];
+            sg[wave][q][n][1] = g[1];
+            sg[wave][q][n][2] = g[2];
+            sN[wave][q][n] = N;
+            if (n == 0) sc[wave][q] = jac * 1.0;
         }
-        double dxi, deta, dzeta, N;
-        hex8_dshape(n, gp_coord(q & 1), gp_coord((q >> 1) & 1), gp_coord(q >> 2), dxi, deta, dzeta, N);
-        double J[9] = {x * dxi, x * deta, x * dzeta, y * dxi, y * deta, y * dzeta,
-                       z * dxi, z * deta, z * dzeta};
-#pragma unroll
-        for (int m = 1; m < 8; m <<= 1)
-#pragma unroll
-            for (int i = 0; i < 9; i++) J[i] += __shfl_xor(J[i], m, 64);
-        double ixi[3], ieta[3], izeta[3];
-        const double jac = inv3(J, ixi, ieta, izeta);
-        const double g0 = dxi * ixi[0] + deta * ieta[0] + dzeta * izeta[0];
-        const double g1 = dxi * ixi[1] + deta * ieta[1] + dzeta * izeta[1];
-        const double g2 = dxi * ixi[2] + deta * ieta[2] + dzeta * izeta[2];
-        sg[wave][q][n][0] = g0;
-        sg[wave][q][n][1] = g1;
-        sg[wave][q][n][2] = g2;
-        sN[wave][q][n] = N;
-        if (n == 0) sc[wave][q] = jac * 1.0;
     }
     __syncthreads();
     if (lane < 8) {   // T_q = sum_n N_n T_n, gradT_q = sum_n g_n T_n
@@ -1409,7 +1457,15 @@ void launch_quad4_sens(const Quad4Args& a, const double* lam, double* g, hipStre
         hipLaunchKernelGGL(k_quad4_mindlin_sens<5>, dim3(blocks_for(a.n_elems, 128)), dim3(128), 0, s, a, lam, g);
 }
 void launch_heat_K(const HeatArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_hex8_heat_K, dim3(blocks_for(a.n_elems, 4)), dim3(256), 0, s, a);
+    if (a.geo)
+        hipLaunchKernelGGL(k_hex8_heat_K<true>, dim3(blocks_for(a.n_elems, 4)), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_hex8_heat_K<false>, dim3(blocks_for(a.n_elems, 4)), dim3(256), 0, s, a);
+}
+void launch_heat_geometry(int n_classes, const double* xyz, const int32_t* econn, const int32_t* class_rep,
+                          double* geo, hipStream_t s) {
+    if (n_classes <= 0) return;
+    hipLaunchKernelGGL(k_hex8_heat_geometry, dim3(n_classes), dim3(64), 0, s, xyz, econn, class_rep, geo);
 }
 void launch_heat_source_dot(const HeatArgs& a, const double* lam, double scale, double* g, hipStream_t s) {
     hipLaunchKernelGGL(k_hex8_source_dot, dim3(blocks_for(a.n_elems, 128)), dim3(128), 0, s, a, lam, scale, g);
