@@ -112,6 +112,14 @@ def measured_traffic(kernel, workload):
 
 
 def distorted_context(w, amp=0.25, seed=7):
+    """A context over distorted_desc(w, amp, seed)."""
+    desc = distorted_desc(w, amp, seed)
+    ctx = mastg.Context(desc)
+    ctx._desc = desc   # keeps the arrays alive with the context
+    return ctx
+
+
+def distorted_desc(w, amp=0.25, seed=7):
     """The workload's mesh with every interior node displaced by up to amp x
     the cell size (seeded): the same topology, DofMap, loads and constraints,
     but no two cells congruent, so the lattice-only fast paths do not apply."""
@@ -130,7 +138,7 @@ def distorted_context(w, amp=0.25, seed=7):
     xyz[inner] += amp * cell * rng.uniform(-1.0, 1.0, (int(inner.sum()), 3))
     desc = mastg.desc_from_arrays(mastg.HEX8_ELASTIC, xyz, conn, d.n_vars, node_dof_base=base,
                                   constrained_dofs=con, point_loads=loads, E=d.E, nu=d.nu, penalty=d.penalty)
-    return mastg.Context(desc)
+    return desc
 
 
 def batched_mg_bench(w, designs, steps, warmup, rtol=1e-10):
@@ -185,12 +193,20 @@ def batch_summary(batch, steps, single):
                  "seeded permutations of it; p50/p99 are per-design wall times within a batch")}
 
 
-def general_mesh_bench(w, params, steps):
+def general_mesh_bench(w, params, steps, lanes=1, seed=1):
     """Designs/s on the distorted mesh: multigrid (assembled FP64 matrix, FP32
     node-block smoothing, Galerkin coarse levels from the element matrices)
-    against the plain FP64 path (CG + Jacobi) and the mixed-precision path."""
-    ctx = distorted_context(w)
+    against the plain FP64 path (CG + Jacobi) and the mixed-precision path;
+    with lanes > 1 also the multigrid path with that many designs at once
+    (run first, while no other context exists)."""
     out = {"workload": f"hex8 {w.nx}x{w.ny}x{w.nz}, interior nodes displaced up to 0.25 of a cell (seeded)"}
+    desc = distorted_desc(w)
+    if lanes > 1:
+        wd = type("W", (), {"desc": desc, "n_elems": w.n_elems})()
+        b = batched_mg_bench(wd, batch_designs(params, lanes, seed), steps, 2)
+        out["mg_batched_designs_per_s"] = b["dps"]
+        out["mg_batched_lanes"] = lanes
+    ctx = mastg.Context(desc)
     # FP64 K + R + scatter + inverse diagonal from each element's own geometry
     # (fused row kernel), and the sensitivity pass
     ctx.set_params(params)
@@ -212,6 +228,8 @@ def general_mesh_bench(w, params, steps):
         print(f"# distorted mesh {name}: {dt*1e3:.1f} ms per design, {r['iters']} iterations, "
               f"C={r['objective']:.10g}", file=sys.stderr, flush=True)
     out["mg_speedup_vs_plain_fp64"] = out["mg_designs_per_s"] / out["plain_fp64_designs_per_s"]
+    if lanes > 1:
+        out["mg_batched_speedup_vs_plain_fp64"] = out["mg_batched_designs_per_s"] / out["plain_fp64_designs_per_s"]
     out["mixed_speedup_vs_plain_fp64"] = out["mixed_designs_per_s"] / out["plain_fp64_designs_per_s"]
     out["compliance_rel_diff_mg_vs_plain"] = abs(out["mg_compliance"] - out["plain_fp64_compliance"]) / \
         abs(out["plain_fp64_compliance"])
@@ -659,9 +677,10 @@ def main():
         out["compliance_rel_diff_paths"] = abs(runs["matfree"]["res"][-1]["objective"] -
                                                runs["plain"]["res"][-1]["objective"]) / \
             abs(runs["plain"]["res"][-1]["objective"])
+    ctx.close()   # the remaining legs build contexts of their own
     if a.solver == "all":
         # the same workload on a mesh whose cells are not congruent
-        out["general_mesh"] = general_mesh_bench(w, params, a.steps)
+        out["general_mesh"] = general_mesh_bench(w, params, a.steps, a.lanes, a.seed)
     if not a.no_cpu_baseline:
         print("# cpu baseline: one complete design on the oracle", file=sys.stderr, flush=True)
         cb = cpu_baseline(a.workload, a.seed)
@@ -682,7 +701,6 @@ def main():
                                                  np.maximum(np.abs(ref["dc"]), 1e-6 * np.abs(ref["dc"]).max()))),
                 "note": f"the {head} path's last timed design against the oracle's on the same inputs"}
     print(json.dumps(out))
-    ctx.close()
 
 
 if __name__ == "__main__":
