@@ -46,11 +46,33 @@ int main(int argc, char** argv) {
             gmax = std::fmax(gmax, std::fabs(g[e]));
         }
         const double ec = std::fabs(C - r.objective) / std::fabs(r.objective);
+        // the same design and two others evaluated side by side on two lanes:
+        // design 0 agrees with the lone evaluation (bitwise when the lanes
+        // run the launch shapes the assembly's context tuned; to rounding else)
+        std::vector<MAST::GPU::RealVector> designs{rho, rho, rho};
+        for (int64_t e = 0; e < d.n_elems; e++) {
+            designs[1][e] = 0.5 * (rho[e] + rho[(e + 1) % d.n_elems]);
+            designs[2][e] = std::fmin(1.0, 1.1 * rho[e]);
+        }
+        MAST::GPU::FunctionEvaluationBatch batch(sys, 2);
+        std::vector<double> obj;
+        std::vector<MAST::GPU::RealVector> obj_grad;
+        batch.evaluate(designs, obj, true, obj_grad);
+        bool same = obj[0] == r.objective;
+        double bmax = 0;
+        for (int64_t e = 0; e < d.n_elems; e++) {
+            same = same && obj_grad[0][e] == g[e];
+            bmax = std::fmax(bmax, std::fabs(obj_grad[0][e] - g[e]));
+        }
+        const double bc = std::fabs(obj[0] - r.objective) / std::fabs(r.objective);
         std::printf("{\"n_elems\": %lld, \"compliance\": %.15g, \"design_eval_compliance\": %.15g, "
-                    "\"compliance_rel_diff\": %.3g, \"sens_max_rel_diff\": %.3g, \"iters\": %d}\n",
-                    (long long)d.n_elems, C, r.objective, ec, emax / gmax, system.last.iters);
+                    "\"compliance_rel_diff\": %.3g, \"sens_max_rel_diff\": %.3g, \"iters\": %d, "
+                    "\"batch_objectives\": [%.15g, %.15g, %.15g], \"batch_design0_rel_diff\": %.3g, "
+                    "\"batch_design0_grad_rel_diff\": %.3g, \"batch_design0_bitwise\": %s}\n",
+                    (long long)d.n_elems, C, r.objective, ec, emax / gmax, system.last.iters, obj[0], obj[1], obj[2],
+                    bc, bmax / gmax, same ? "true" : "false");
         mastg_workload_destroy(w);
-        return (ec < 1e-8 && emax / gmax < 1e-6) ? 0 : 1;
+        return (ec < 1e-8 && emax / gmax < 1e-6 && bc < 1e-12 && bmax / gmax < 1e-10) ? 0 : 1;
     } catch (const std::exception& ex) {
         std::fprintf(stderr, "error: %s\n", ex.what());
         mastg_workload_destroy(w);

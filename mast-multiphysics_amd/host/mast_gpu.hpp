@@ -336,5 +336,55 @@ private:
     NonlinearImplicitAssembly& _a;
 };
 
+// ---- several design evaluations at once ---------------------------------
+// MAST::FunctionEvaluation::evaluate (src/optimization/function_evaluation.h:107;
+// the SIMP driver's evaluate, examples/structural/example_6/example_6.cpp:450)
+// for several design points: each design's objective (compliance, or mean
+// temperature for heat) and, on request, its gradient, evaluated side by side
+// on n_lanes contexts (mastg_batch_*); each result equals one lone evaluation
+// bitwise
+class FunctionEvaluationBatch {
+public:
+    FunctionEvaluationBatch(const SystemDescription& sys, int n_lanes) : _n_elems(sys.desc().n_elems) {
+        const int rc = mastg_batch_create(&sys.desc(), n_lanes, &_b);
+        if (rc != MASTG_OK)
+            throw std::runtime_error(std::string("mastg_batch_create: ") + mastg_last_error(nullptr) +
+                                     " (status " + std::to_string(rc) + ")");
+    }
+    FunctionEvaluationBatch(const FunctionEvaluationBatch&) = delete;
+    ~FunctionEvaluationBatch() { mastg_batch_destroy(_b); }
+
+    // dvars[i]: the element parameters of design i; obj[i], obj_grad[i]
+    void evaluate(const std::vector<RealVector>& dvars, std::vector<double>& obj, bool eval_obj_grad,
+                  std::vector<RealVector>& obj_grad, const mastg_solve_opts* opts = nullptr) {
+        const size_t n = dvars.size();
+        RealVector p(n * size_t(_n_elems)), g(eval_obj_grad ? n * size_t(_n_elems) : 0);
+        for (size_t i = 0; i < n; i++) {
+            if (int64_t(dvars[i].size()) != _n_elems)
+                throw std::runtime_error("FunctionEvaluationBatch::evaluate: design of the wrong length");
+            std::copy(dvars[i].begin(), dvars[i].end(), p.begin() + i * size_t(_n_elems));
+        }
+        std::vector<mastg_design_result> r(n);
+        const int rc = mastg_batch_eval(_b, int(n), p.data(), eval_obj_grad ? g.data() : nullptr, MASTG_HOST, opts,
+                                        r.data(), nullptr);
+        if (rc != MASTG_OK)
+            throw std::runtime_error(std::string("FunctionEvaluationBatch::evaluate: ") + mastg_batch_last_error(_b) +
+                                     " (status " + std::to_string(rc) + ")");
+        obj.resize(n);
+        obj_grad.assign(eval_obj_grad ? n : 0, RealVector());
+        for (size_t i = 0; i < n; i++) {
+            obj[i] = r[i].objective;
+            if (eval_obj_grad)
+                obj_grad[i].assign(g.begin() + i * size_t(_n_elems), g.begin() + (i + 1) * size_t(_n_elems));
+        }
+        last = r;
+    }
+    std::vector<mastg_design_result> last;
+
+private:
+    mastg_batch* _b = nullptr;
+    int64_t _n_elems;
+};
+
 }  // namespace GPU
 }  // namespace MAST

@@ -104,3 +104,36 @@ def test_batch_refusals(mastg):
     for lanes in (0, 17):
         with pytest.raises(mastg.MastgError):
             mastg.Batch(w.desc, lanes)
+
+
+def test_batch_device_pointers(oracle, mastg):
+    """params and gradients in device memory (where = MASTG_DEVICE) give the
+    host-pointer results bitwise."""
+    import ctypes as C
+    print("[batch] device pointers, hex8 8x4x4, 2 lanes, 3 designs", flush=True)
+    P = oracle.Problem(oracle.CANTILEVER, 8, 4, 4, seed=2, lo=0.1, hi=1.0)
+    desc = _desc(oracle, mastg, P)
+    designs = np.ascontiguousarray(np.stack(_designs(P, 3, 0.1, 1.0, 8)))
+    B = mastg.Batch(desc, 2)
+    host = B.eval(designs, solver=mastg.SOLVER_MG)
+    lane = B.lane(0)
+    L = mastg.lib()
+    nbytes = designs.nbytes
+    dp, dg = C.c_void_p(), C.c_void_p()
+    lane._chk(L.mastg_device_alloc(lane.h, nbytes, C.byref(dp)))
+    lane._chk(L.mastg_device_alloc(lane.h, nbytes, C.byref(dg)))
+    try:
+        lane._chk(L.mastg_copy(lane.h, dp, designs.ctypes.data, nbytes, 0))
+        o = mastg.SolveOpts(mastg.SOLVER_MG, 0, 1e-10, 0.0, 0, 0)
+        out = (mastg.DesignResult * 3)()
+        rc = L.mastg_batch_eval(B.h, 3, C.cast(dp, C.POINTER(C.c_double)), C.cast(dg, C.POINTER(C.c_double)),
+                                mastg.DEVICE, C.byref(o), out, None)
+        B._chk(rc)
+        g = np.zeros_like(designs)
+        lane._chk(L.mastg_copy(lane.h, g.ctypes.data, dg, nbytes, 1))
+    finally:
+        L.mastg_device_free(lane.h, dp)
+        L.mastg_device_free(lane.h, dg)
+    for i in range(3):
+        assert out[i].objective == host[i]["objective"] and np.array_equal(g[i], host[i]["grad"]), i
+    B.close()

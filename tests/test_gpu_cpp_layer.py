@@ -1,6 +1,8 @@
 """The MAST-style C++ layer (mast-multiphysics_amd/host/mast_gpu.hpp) drives the
 reference's call sequence (solve, ComplianceOutput, adjoint_solve, adjoint
-sensitivities) through the C ABI and agrees with the fused design evaluation."""
+sensitivities) through the C ABI and agrees with the fused design evaluation;
+FunctionEvaluationBatch evaluates several designs at once and its design 0
+agrees with the lone evaluation."""
 import json
 import os
 import subprocess
@@ -22,6 +24,9 @@ def test_mast_call_sequence(nx):
     out = json.loads(p.stdout.strip().splitlines()[-1])
     assert out["compliance_rel_diff"] < 1e-8
     assert out["sens_max_rel_diff"] < 1e-6
+    # FunctionEvaluationBatch: design 0 of a batch against the lone evaluation
+    assert out["batch_design0_rel_diff"] < 1e-12 and out["batch_design0_grad_rel_diff"] < 1e-10
+    assert out["batch_objectives"][1] != out["batch_objectives"][0]
 
 
 def test_nastran_ks_forward_adjoint_sequence():
