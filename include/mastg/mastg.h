@@ -27,8 +27,9 @@
  * Vectors passed across the boundary use the caller's DofMap numbering
  * (DOF(node, var) = node_dof_base[node] + var, as libMesh's var-major numbering
  * with one variable group gives).  The global matrix is exported over the
- * "active" variables (hex8 elasticity: ux uy uz; Mindlin plate: u v w tx ty;
- * heat: T), with rows and columns in the caller's order: it is the caller's
+ * "active" variables (hex8 elasticity: ux uy uz; Mindlin plate: u v w tx ty,
+ * all six when an element leaves the z = const plane; heat: T), with rows and
+ * columns in the caller's order: it is the caller's
  * full-coupling sparsity pattern restricted to those variables.
  */
 #ifndef MASTG_MASTG_H
@@ -340,6 +341,18 @@ int mastg_device_alloc(mastg_ctx* ctx, size_t bytes, void** ptr);
 int mastg_device_free(mastg_ctx* ctx, void* ptr);
 int mastg_copy(mastg_ctx* ctx, void* dst, const void* src, size_t bytes, int kind /*0 h2d 1 d2h 2 d2d*/);
 int mastg_synchronize(mastg_ctx* ctx);
+
+/* The HIP stream (a hipStream_t, as void*) the context enqueues its work on.
+ * By default a non-blocking stream the context created; mastg_set_stream
+ * moves the context onto the caller's stream (null: back to its own) after
+ * the work already enqueued has finished, so that the caller can order the
+ * context's kernels with its own device work (events, graphs).  Entry points
+ * with host results still synchronise that stream before they return; the
+ * stream must be of default priority and outlive its use by the context.  No
+ * counterpart in the reference (PETSc and libMesh run on the host); for
+ * callers that own the device stream. */
+int mastg_get_stream(const mastg_ctx* ctx, void** stream);
+int mastg_set_stream(mastg_ctx* ctx, void* stream);
 
 /* Device time (milliseconds) of named kernels over the context's lifetime,
  * measured with HIP events on the stream the kernels run on. kind:

@@ -117,7 +117,8 @@ struct mastg_ctx {
     mastg_problem_desc d{};          // scalars of the description (pointers cleared)
     Structure S;
     std::string err;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;      // where the context's work is enqueued
+    hipStream_t own_stream = nullptr;  // the stream the context created
     int nparts = 1024;
     int64_t device_bytes = 0;
     // device structure
@@ -1539,11 +1540,12 @@ int mastg_ctx_create(const mastg_problem_desc* desc, mastg_ctx** out) {
         return r;
     };
     std::unique_lock<std::recursive_mutex> create_lock(g_hip_mu);
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) {
         g_create_error = "hipStreamCreate failed";
         delete c;
         return MASTG_ERR_DEVICE;
     }
+    c->stream = c->own_stream;
     if (scratch_address(&c->scratch_ptr) != hipSuccess) {
         c->err = "hipGetSymbolAddress of the scratch array failed";
         return bail(MASTG_ERR_DEVICE);
@@ -1807,9 +1809,27 @@ int mastg_ctx_destroy(mastg_ctx* c) {
     if (c->h_pin) (void)hipHostFree(c->h_pin);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
-    hipStream_t s = c->stream;
+    if (c->stream && c->stream != c->own_stream) (void)hipStreamSynchronize(c->own_stream);
+    hipStream_t s = c->own_stream;
     delete c;   // DevBuf destructors free device memory
     if (s) (void)hipStreamDestroy(s);
+    return MASTG_OK;
+}
+
+int mastg_get_stream(const mastg_ctx* c, void** stream) {
+    if (!c || !stream) return MASTG_ERR_INVALID;
+    *stream = static_cast<void*>(c->stream);
+    return MASTG_OK;
+}
+
+int mastg_set_stream(mastg_ctx* c, void* stream) {
+    if (!c) return MASTG_ERR_INVALID;
+    hipStream_t next = stream ? static_cast<hipStream_t>(stream) : c->own_stream;
+    if (next == c->stream) return MASTG_OK;
+    // the work already enqueued finishes first; captured solver graphs are not
+    // bound to a stream and replay on the new one
+    HIPC(c, hipStreamSynchronize(c->stream));
+    c->stream = next;
     return MASTG_OK;
 }
 

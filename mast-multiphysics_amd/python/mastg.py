@@ -125,6 +125,8 @@ def lib():
             "mastg_enable_kernel_timing": (C.c_int, [_P, C.c_int]),
             "mastg_kernel_time": (C.c_int, [_P, C.c_int, _D, C.POINTER(C.c_int64)]),
             "mastg_synchronize": (C.c_int, [_P]),
+            "mastg_get_stream": (C.c_int, [_P, C.POINTER(C.c_void_p)]),
+            "mastg_set_stream": (C.c_int, [_P, C.c_void_p]),
             "mastg_bench_kernel": (C.c_int, [_P, C.c_int, C.c_int, _D]),
             "mastg_get_tuning": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
             "mastg_set_option": (C.c_int, [_P, C.c_int, C.c_int]),
@@ -447,6 +449,16 @@ class Context:
 
     def set_option(self, option, value):
         self._chk(lib().mastg_set_option(self.h, option, value))
+
+    def stream(self):
+        """The HIP stream (an integer handle) the context enqueues on."""
+        h = C.c_void_p()
+        self._chk(lib().mastg_get_stream(self.h, C.byref(h)))
+        return h.value or 0
+
+    def set_stream(self, handle):
+        """Enqueue on the caller's HIP stream `handle` (0: the context's own)."""
+        self._chk(lib().mastg_set_stream(self.h, C.c_void_p(handle or None)))
 
     def tuning(self):
         a, b = C.c_int(), C.c_int()
