@@ -67,9 +67,24 @@ void run_design(mastg_batch* b, int lane, int i) {
         if (b->first_bad < 0 || i < b->first_bad) {
             b->first_bad = i;
             b->first_rc = rc;
-            b->err = "design " + std::to_string(i) + ": " + mastg_last_error(b->lanes[size_t(lane)]);
+            try {
+                b->err = "design " + std::to_string(i) + ": " + mastg_last_error(b->lanes[size_t(lane)]);
+            } catch (...) {
+                b->err.clear();   // the status still reports the failure
+            }
         }
     }
+}
+
+// stop and join the lane threads started so far
+void stop_lanes(mastg_batch* b) {
+    {
+        std::lock_guard<std::mutex> lk(b->mu);
+        b->stop = true;
+    }
+    b->cv_work.notify_all();
+    for (auto& t : b->threads) t.join();
+    b->threads.clear();
 }
 
 void lane_loop(mastg_batch* b, int lane) {
@@ -98,20 +113,32 @@ extern "C" {
 int mastg_batch_create(const mastg_problem_desc* desc, int n_lanes, mastg_batch** out) {
     if (!desc || !out || n_lanes < 1 || n_lanes > 16) return MASTG_ERR_INVALID;
     *out = nullptr;
-    auto* b = new mastg_batch;
-    b->n_elems = desc->n_elems;
-    // contexts are created one after another on the calling thread
-    for (int l = 0; l < n_lanes; l++) {
-        mastg_ctx* c = nullptr;
-        const int rc = mastg_ctx_create(desc, &c);
-        if (rc) {
+    mastg_batch* b = nullptr;
+    try {
+        b = new mastg_batch;
+        b->n_elems = desc->n_elems;
+        b->lanes.reserve(size_t(n_lanes));
+        b->threads.reserve(size_t(n_lanes));
+        // contexts are created one after another on the calling thread
+        for (int l = 0; l < n_lanes; l++) {
+            mastg_ctx* c = nullptr;
+            const int rc = mastg_ctx_create(desc, &c);
+            if (rc) {
+                for (mastg_ctx* x : b->lanes) mastg_ctx_destroy(x);
+                delete b;
+                return rc;
+            }
+            b->lanes.push_back(c);
+        }
+        for (int l = 0; l < n_lanes; l++) b->threads.emplace_back(lane_loop, b, l);
+    } catch (...) {   // host allocation or thread creation failed
+        if (b) {
+            stop_lanes(b);
             for (mastg_ctx* x : b->lanes) mastg_ctx_destroy(x);
             delete b;
-            return rc;
         }
-        b->lanes.push_back(c);
+        return MASTG_ERR_OOM;
     }
-    for (int l = 0; l < n_lanes; l++) b->threads.emplace_back(lane_loop, b, l);
     *out = b;
     return MASTG_OK;
 }
@@ -180,12 +207,7 @@ const char* mastg_batch_last_error(mastg_batch* b) {
 
 int mastg_batch_destroy(mastg_batch* b) {
     if (!b) return MASTG_OK;
-    {
-        std::lock_guard<std::mutex> lk(b->mu);
-        b->stop = true;
-    }
-    b->cv_work.notify_all();
-    for (auto& t : b->threads) t.join();
+    stop_lanes(b);
     for (mastg_ctx* c : b->lanes) mastg_ctx_destroy(c);
     delete b;
     return MASTG_OK;
