@@ -165,9 +165,7 @@ __global__ __launch_bounds__(256) void k_hex8_elastic_K(Hex8Args A) {
 
 
 // ---------------------------------------------------------------------------
-// hex8 K_e in FP32 on MFMA (v_mfma_f32_16x16x4f32, FP32 accumulation):
-//   K_e = G^T H,  G[s][c] = B_q[row][c],  H[s][c] = JxW_q (D B_q)[row][c],
-//   s = 6 q + row (K = 48), c = v*8 + a (24 columns, padded to 32: 4 tiles x 12 MFMAs)
+// hex8 K_e in FP32 on MFMA (v_mfma_f32_16x16x4f32, FP32 accumulation).
 // Lane layout of the 16x16x4 instruction: A[i][k] from lane i + 16k, B[k][j]
 // from lane j + 16k, D[4*(l/16)+r][l%16] in accumulator r.  Geometry in FP64,
 // operands rounded to FP32.  Output: FP32 node-pair blocks, constrained,
@@ -178,22 +176,23 @@ __global__ __launch_bounds__(256) void k_hex8_elastic_K(Hex8Args A) {
 // ---------------------------------------------------------------------------
 typedef float mastg_floatx4 __attribute__((ext_vector_type(4)));
 
-// strain row -> component of grad N feeding column variable v (or -1)
-__constant__ int8_t kBcomp[6][3] = {{0, -1, -1}, {-1, 1, -1}, {-1, -1, 2},
-                                    {1, 0, -1}, {-1, 2, 1}, {2, -1, 0}};
-// column variable v -> its three strain rows and the gradient components
-__constant__ int8_t kBrows[3][3] = {{0, 3, 5}, {1, 3, 4}, {2, 4, 5}};
-__constant__ int8_t kBdirs[3][3] = {{0, 1, 2}, {1, 0, 2}, {2, 1, 0}};
-
+// FP32 element matrices on the matrix cores.  With g_qa the physical gradient
+// of N_a at point q and c_q its JxW, P^{kl}[a][b] = sum_q c_q g_qa,k g_qb,l is
+// the 24 x 24 product (W G)^T G over the 8 points (rows (k, a), columns (l,
+// b)): four 16 x 16 tiles of two v_mfma_f32_16x16x4f32 each.  The isotropic
+// B^T D B then follows per entry (the Lame form of D = lam 1 1^T + mu diag(2,
+// 2, 2, 1, 1, 1) in Voigt notation):
+//   K_(a v)(b w) = E_e [lam P^{vw}[a][b] + mu P^{wv}[a][b] + mu delta_vw sum_k P^{kk}[a][b]]
+// One wave per element; lanes (q, n) form the geometry in FP64 and round it.
 __global__ __launch_bounds__(256) void k_hex8_elastic_K_f32(Hex8Args A, float* __restrict__ Kel32) {
     __shared__ float sg[4][8][8][3];   // [wave][q][node][d]
     __shared__ float sc[4][8];
-    __shared__ float sD[36];
+    __shared__ float sP[4][24][25];    // [wave][(k, a)][(l, b)] (one float of padding per row)
+    __shared__ uint8_t scon[4][8];     // constraint bits of the element's nodes
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t e = int64_t(blockIdx.x) * 4 + wave;
     const bool valid = e < A.n_elems;
     const int64_t ee = valid ? e : 0;
-    if (threadIdx.x < 36) sD[threadIdx.x] = float(A.Dhat[threadIdx.x]);
     {
         const int q = lane >> 3, n = lane & 7;
         const int32_t rn = A.econn[ee * 8 + n];
@@ -213,46 +212,48 @@ __global__ __launch_bounds__(256) void k_hex8_elastic_K_f32(Hex8Args A, float* _
         sg[wave][q][n][1] = float(dxi * ixi[1] + deta * ieta[1] + dzeta * izeta[1]);
         sg[wave][q][n][2] = float(dxi * ixi[2] + deta * ieta[2] + dzeta * izeta[2]);
         if (n == 0) sc[wave][q] = float(jac);
+        if (q == 0)
+            scon[wave][n] = A.con ? uint8_t(A.con[int64_t(rn) * 3] | (A.con[int64_t(rn) * 3 + 1] << 1) |
+                                            (A.con[int64_t(rn) * 3 + 2] << 2))
+                                  : uint8_t(0);
+    }
+    __syncthreads();
+    {
+        const int li = lane & 15, lk = lane >> 4;
+#pragma unroll
+        for (int tile = 0; tile < 4; tile++) {
+            const int I = tile >> 1, Jt = tile & 1;
+            const int ra = 16 * I + li, cb = 16 * Jt + li;   // this lane's A row / B column
+            mastg_floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int ks = 0; ks < 2; ks++) {
+                const int q = 4 * ks + lk;
+                const float av = ra < 24 ? sc[wave][q] * sg[wave][q][ra & 7][ra >> 3] : 0.f;
+                const float bv = cb < 24 ? sg[wave][q][cb & 7][cb >> 3] : 0.f;
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc, 0, 0, 0);
+            }
+            // acc[r]: row 16 I + 4 lk + r, column 16 Jt + li
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int i = 16 * I + 4 * lk + r, j = 16 * Jt + li;
+                if (i < 24 && j < 24) sP[wave][i][j] = acc[r];
+            }
+        }
     }
     __syncthreads();
     if (!valid) return;
-    const int li = lane & 15, lk = lane >> 4;
     const double Ee = A.E0 * pow(A.param[e], A.penalty);
+    const float lam = float(Ee * A.Dhat[1]), mu = float(Ee * A.Dhat[21]);
+    float* __restrict__ out = Kel32 + e * 576;
 #pragma unroll
-    for (int tile = 0; tile < 4; tile++) {
-        const int I = tile >> 1, Jt = tile & 1;
-        const int ca = 16 * I + li, cb = 16 * Jt + li;     // this lane's A column / B column
-        mastg_floatx4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll 4
-        for (int ks = 0; ks < 12; ks++) {
-            const int sidx = 4 * ks + lk;
-            const int q = sidx / 6, row = sidx - 6 * q;
-            float av = 0.f, bv = 0.f;
-            if (ca < 24) {
-                const int d = kBcomp[row][ca >> 3];
-                if (d >= 0) av = sg[wave][q][ca & 7][d];
-            }
-            if (cb < 24) {
-                const int v = cb >> 3, b = cb & 7;
-                float h = 0.f;
-#pragma unroll
-                for (int t = 0; t < 3; t++) h += sD[row * 6 + kBrows[v][t]] * sg[wave][q][b][kBdirs[v][t]];
-                bv = sc[wave][q] * h;
-            }
-            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc, 0, 0, 0);
-        }
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int i = 16 * I + 4 * lk + r, j = 16 * Jt + li;
-            if (i < 24 && j < 24) {
-                const int a = i & 7, v = i >> 3, b = j & 7, w = j >> 3;
-                const int32_t ra = A.econn[e * 8 + a], rb = A.econn[e * 8 + b];
-                float kv = float(Ee * double(acc[r]));
-                if (A.con && (A.con[int64_t(ra) * 3 + v] || A.con[int64_t(rb) * 3 + w]))
-                    kv = (a == b && v == w) ? 1.0f : 0.0f;
-                Kel32[(e * 64 + a * 8 + b) * 9 + v * 3 + w] = kv;
-            }
-        }
+    for (int jj = 0; jj < 9; jj++) {
+        // entry t = (a 8 + b) 9 + v 3 + w of the element's blocks
+        const int t = lane + 64 * jj;
+        const int ab = t / 9, vw = t - 9 * ab, a = ab >> 3, b = ab & 7, v = vw / 3, w = vw - 3 * v;
+        const float tr = sP[wave][a][b] + sP[wave][8 + a][8 + b] + sP[wave][16 + a][16 + b];
+        float kv = lam * sP[wave][v * 8 + a][w * 8 + b] + mu * sP[wave][w * 8 + a][v * 8 + b] + (v == w ? mu * tr : 0.f);
+        if (((scon[wave][a] >> v) & 1) || ((scon[wave][b] >> w) & 1)) kv = (a == b && v == w) ? 1.0f : 0.0f;
+        out[t] = kv;
     }
 }
 
