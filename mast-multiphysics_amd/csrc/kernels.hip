@@ -68,7 +68,10 @@ __device__ __forceinline__ double inv3(const double J[9], double ixi[3], double 
 // StructuralElement3D::internal_residual, linear strain
 // (src/elasticity/solid_element_3d.cpp:269-411): K = sum_qp JxW B^T D B.
 // One wave per element, lane = (a, b) node pair: the lane builds the 3x3 block
-// K_ab = sum_q c_q B_a^T (D B_b) from the shape gradients staged in LDS.
+// K_ab = sum_q c_q B_a^T (D B_b) from the shape gradients staged in LDS, in
+// the Lame form of the isotropic D (D = lam 1 1^T + mu diag(2, 2, 2, 1, 1, 1),
+// lam = D[0][1], mu = D[3][3]; so also for dD/dnu):
+//   (K_ab)_vw = sum_q c_q [lam g_a,v g_b,w + mu g_a,w g_b,v + mu delta_vw g_a . g_b]
 // libMesh constraint application is folded in: constrained rows / columns
 // zeroed, unit diagonal (DofMap::constrain_element_matrix_and_vector).
 // ---------------------------------------------------------------------------
@@ -103,28 +106,25 @@ __global__ __launch_bounds__(256) void k_hex8_elastic_K(Hex8Args A) {
     }
     __syncthreads();
     const int a = lane >> 3, b = lane & 7;
-#define D(i) A.Dhat[i]   // 6x6 material matrix at E = 1 (kernel argument)
-    double K[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const double lam = A.Dhat[1], mu = A.Dhat[21];   // at E = 1 (kernel argument)
+    // P_vw = sum_q c_q g_a,v g_b,w and S = trace
+    double P[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll 2
     for (int q = 0; q < 8; q++) {
         const double c = sc[wave][q];
-        const double gax = sg[wave][q][a][0], gay = sg[wave][q][a][1], gaz = sg[wave][q][a][2];
-        const double gbx = sg[wave][q][b][0], gby = sg[wave][q][b][1], gbz = sg[wave][q][b][2];
-        double DB[6][3];
+        const double ga[3] = {c * sg[wave][q][a][0], c * sg[wave][q][a][1], c * sg[wave][q][a][2]};
+        const double gb[3] = {sg[wave][q][b][0], sg[wave][q][b][1], sg[wave][q][b][2]};
 #pragma unroll
-        for (int r = 0; r < 6; r++) {
-            DB[r][0] = D(r * 6 + 0) * gbx + D(r * 6 + 3) * gby + D(r * 6 + 5) * gbz;
-            DB[r][1] = D(r * 6 + 1) * gby + D(r * 6 + 3) * gbx + D(r * 6 + 4) * gbz;
-            DB[r][2] = D(r * 6 + 2) * gbz + D(r * 6 + 4) * gby + D(r * 6 + 5) * gbx;
-        }
+        for (int v = 0; v < 3; v++)
 #pragma unroll
-        for (int w = 0; w < 3; w++) {
-            K[0 * 3 + w] += c * (gax * DB[0][w] + gay * DB[3][w] + gaz * DB[5][w]);
-            K[1 * 3 + w] += c * (gay * DB[1][w] + gax * DB[3][w] + gaz * DB[4][w]);
-            K[2 * 3 + w] += c * (gaz * DB[2][w] + gay * DB[4][w] + gax * DB[5][w]);
-        }
+            for (int w = 0; w < 3; w++) P[v * 3 + w] = fma(ga[v], gb[w], P[v * 3 + w]);
     }
-#undef D
+    const double S = P[0] + P[4] + P[8];
+    double K[9];
+#pragma unroll
+    for (int v = 0; v < 3; v++)
+#pragma unroll
+        for (int w = 0; w < 3; w++) K[v * 3 + w] = lam * P[v * 3 + w] + mu * P[w * 3 + v] + (v == w ? mu * S : 0.0);
     if (!valid) return;
     const double Ee = A.E0 * pow(A.param[e], A.penalty);
 #pragma unroll
