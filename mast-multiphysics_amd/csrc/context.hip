@@ -107,7 +107,7 @@ struct MGLevelDev {
                                   // heat levels 0 and 1: 27-point stencils in lattice order
     DevBuf kpos;                  // heat level 0: row position of each stencil entry (255: none)
     DevBuf dinvb;                 // plate level 0: inverses of the 5 x 5 diagonal node blocks
-    bool lat0 = false;            // heat level 0 smooths in lattice order
+    bool lat0 = false;            // heat / hex8 level 0 smooths in lattice order
     DevBuf Kel, val, val32, dinv, r, x, res, da, db;
 };
 
@@ -747,12 +747,12 @@ int mg_build(mastg_ctx* c) {
             // hierarchy before its first assembly
             TRY(dalloc(c, c->val, size_t(c->S.nnz) * sizeof(double)));
             TRY(dalloc(c, c->dinv, size_t(c->S.n_active) * sizeof(double)));
-            // heat: the V-cycle runs this level in lattice order (node-major)
-            // with a stencil smoother: the stencil's row positions, constraint
+            // heat and hex8 elasticity: the V-cycle runs this level in lattice
+            // order (node-major) with a stencil smoother: the stencil's row positions, constraint
             // flags, inverse diagonal and right side in lattice order, the stencil
             // (the plate's node-block smoother measured faster than
             // lattice-ordered stencils: 324 vs 352 and 387 ms per 512^2 design)
-            if (nav == 1) {
+            if (nav == 1 || nav == 3) {
                 std::vector<int32_t> lat_of_rank(size_t(h.n_nodes));
                 for (int64_t p = 0; p < h.n_nodes; p++) lat_of_rank[size_t(h.rank_of_lat[p])] = int32_t(p);
                 std::vector<uint8_t> con_lat(size_t(h.n_nodes) * nav);
@@ -777,7 +777,9 @@ int mg_build(mastg_ctx* c) {
                 TRY(upload(c, L->con_lat, con_lat));
                 TRY(dalloc(c, L->dinv_lat, nv));
                 TRY(dalloc(c, L->b_lat, nv));
-                TRY(dalloc(c, L->sten, size_t(h.n_nodes) * 27 * sizeof(float)));
+                // heat: all 27 entries (the Jacobian is not symmetric); hex8
+                // elasticity: the symmetric half of the 27 3 x 3 blocks
+                TRY(dalloc(c, L->sten, size_t(h.n_nodes) * (nav == 1 ? 27 : kMGHalfStencil) * sizeof(float)));
                 L->lat0 = true;
             }
             if (nav == 5) {   // the plate's problem level smooths with block Jacobi
@@ -937,7 +939,16 @@ int mg_setup(mastg_ctx* c) {
         launch_mg_rowinfo_nb(nav, c->S.n_nodes, c->blk_ptr.as<int64_t>(), c->blk_col.as<int32_t>(),
                              c->val.as<double>(), nullptr, c->mg_part.as<double>(), c->mg_lam.as<double>(), s);
         launch_to_f32(c->S.nnz, c->val.as<double>(), c->val32.as<float>(), s);
-        if (L0.lat0) {
+        if (L0.lat0 && nav == 3) {
+            // hex8 elasticity on cells of several shapes: the symmetric half
+            // stencil and the inverse diagonal of this assembly in lattice
+            // order; the Chebyshev bound is the Gershgorin bound above (1
+            // PCG iteration more at 128 x 64 x 64 than 1.1 x a 20-step power
+            // iteration, 3 ms less per design)
+            launch_mg_gather_stencil3(L0.a, L0.a.rank_of_lat, L0.kpos.as<uint8_t>(), L0.sten.as<float>(), s);
+            launch_perm_rank_lat(L0.a.n_nodes, nav, L0.a.rank_of_lat, c->dinv.as<double>(), L0.dinv_lat.as<double>(),
+                                 1, nullptr, s);
+        } else if (L0.lat0) {
             // heat: the lattice-ordered stencil and inverse diagonal of this
             // assembly, and the Chebyshev bound from them
             launch_mg_gather_stencil(L0.a, L0.a.rank_of_lat, L0.kpos.as<uint8_t>(), L0.sten.as<float>(), s);
@@ -1053,6 +1064,9 @@ const void* mg_vcycle(mastg_ctx* c, const CGArgs& a, const int* done, hipStream_
         if (l == 0 && f32)
             launch_mg_fine(a, L0, mode, step, iin < 0 ? nullptr : dbuf32(iin), iout < 0 ? nullptr : dbuf32(iout),
                            done, s);
+        else if (l == 0 && D0.lat0 && L0.nav == 3)
+            launch_mg_stencil(L0, D0.sten.as<float>(), mode, step, iin < 0 ? nullptr : dbuf(l, iin),
+                              iout < 0 ? nullptr : dbuf(l, iout), done, s);
         else if (l == 0 && D0.lat0)
             launch_mg_stencil_rank(L0, D0.sten.as<float>(), mode, step, iin < 0 ? nullptr : dbuf(l, iin),
                                    iout < 0 ? nullptr : dbuf(l, iout), done, s);

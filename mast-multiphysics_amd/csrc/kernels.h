@@ -349,6 +349,9 @@ void launch_mg_combine_rank(int64_t n_nodes, int nav, const int32_t* rank_of_lat
 void launch_mg_to_stencil(const MGLevelArgs& L, float* sten, hipStream_t s);
 // inverses of the 5 x 5 diagonal node blocks of a plate level (FP64 values)
 void launch_mg_block_dinv5(const MGLevelArgs& L, float* dinvb, hipStream_t s);
+void launch_mg_gather_stencil3(const MGLevelArgs& L, const int32_t* rank_of_lat, const uint8_t* kpos, float* sten,
+                               hipStream_t s);
+constexpr int kMGHalfStencil = 123;   // FP32 entries per point of the symmetric half 27-point 3x3 stencil
 void launch_mg_stencil(const MGLevelArgs& L, const float* sten, int mode, int step, const double* din, double* dout,
                        const int* done, hipStream_t s);
 // level 0 (hex8): FP32 smoothing step with the lattice operator (c.lat) and

@@ -485,6 +485,34 @@ __global__ __launch_bounds__(256) void k_mg_gather_stencil(MGLevelArgs L, const 
     }
 }
 
+// The hex8 problem level (rank order, cells of several shapes) as the
+// symmetric half stencil of k_mg_stencil in lattice order: lattice point p
+// keeps the blocks towards its 13 forward neighbours and the upper triangle of
+// its own block, read from the FP32 row values of its rank through kpos.
+// Blocks symmetric to rounding become exactly symmetric (the upper ones serve
+// both directions).
+__global__ __launch_bounds__(256) void k_mg_gather_stencil3(MGLevelArgs L, const int32_t* __restrict__ rank_of_lat,
+                                                            const uint8_t* __restrict__ kpos,
+                                                            float* __restrict__ sten) {
+    const int64_t p = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (p >= L.n_nodes) return;
+    const int64_t N = L.n_nodes;
+    const int64_t r = rank_of_lat[p], k0 = L.blk_ptr[r];
+    const int deg = int(L.blk_ptr[r + 1] - k0);
+    const float* __restrict__ V = L.val32 + k0 * 9;
+    for (int o = 13; o < 27; o++) {
+        const int kk = kpos[p * 27 + o];
+#pragma unroll
+        for (int v = 0; v < 3; v++)
+#pragma unroll
+            for (int w = 0; w < 3; w++) {
+                const float x = kk == 255 ? 0.f : V[(v * deg + kk) * 3 + w];
+                if (o > 13) sten[((o - 14) * 9 + v * 3 + w) * N + p] = x;
+                else if (v <= w) sten[(117 + kSymIdx[v][w]) * N + p] = x;
+            }
+    }
+}
+
 // smoothing step of the heat problem level in lattice order (FP32 stencil, FP64
 // vectors and sums, fused Chebyshev epilogue)
 __global__ __launch_bounds__(256) void k_mg_stencil_heat(MGLevelArgs L, const float* __restrict__ sten, int mode,
@@ -1424,6 +1452,11 @@ void launch_mg_block_dinv5(const MGLevelArgs& L, float* dinvb, hipStream_t s) {
 }
 void launch_mg_to_stencil(const MGLevelArgs& L, float* sten, hipStream_t s) {
     hipLaunchKernelGGL(k_mg_to_stencil, dim3(mg_blocks(L.n_nodes, 256)), dim3(256), 0, s, L, sten);
+}
+void launch_mg_gather_stencil3(const MGLevelArgs& L, const int32_t* rank_of_lat, const uint8_t* kpos, float* sten,
+                               hipStream_t s) {
+    hipLaunchKernelGGL(k_mg_gather_stencil3, dim3(mg_blocks(L.n_nodes, 256)), dim3(256), 0, s, L, rank_of_lat, kpos,
+                       sten);
 }
 void launch_mg_stencil(const MGLevelArgs& L, const float* sten, int mode, int step, const double* din, double* dout,
                        const int* done, hipStream_t s) {
