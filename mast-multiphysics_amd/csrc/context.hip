@@ -751,7 +751,8 @@ int mg_build(mastg_ctx* c) {
             // order (node-major) with a stencil smoother: the stencil's row positions, constraint
             // flags, inverse diagonal and right side in lattice order, the stencil
             // (the plate's node-block smoother measured faster than
-            // lattice-ordered stencils: 324 vs 352 and 387 ms per 512^2 design)
+            // lattice-ordered stencils: 324 vs 352 and 387 ms per 512^2 design
+            // with full stencils, 257 vs 284 ms with the symmetric half one)
             if (nav == 1 || nav == 3) {
                 std::vector<int32_t> lat_of_rank(size_t(h.n_nodes));
                 for (int64_t p = 0; p < h.n_nodes; p++) lat_of_rank[size_t(h.rank_of_lat[p])] = int32_t(p);
@@ -1048,8 +1049,9 @@ const void* mg_vcycle(mastg_ctx* c, const CGArgs& a, const int* done, hipStream_
     if (r0) L0.r = const_cast<double*>(r0);
     const bool f32 = L0.planar != 0;
     if (D0.lat0) {
-        // heat: the problem level runs in lattice order (its output is mapped
-        // back to rank order by the caller: mg_rz / mg_bicg_iteration)
+        // heat and hex8 on cells of several shapes: the problem level runs in
+        // lattice order (its output is mapped back to rank order by the
+        // caller: mg_rz / mg_bicg_iteration)
         launch_perm_rank_lat(L0.n_nodes, L0.nav, L0.rank_of_lat, L0.r, D0.b_lat.as<double>(), 1, done, s);
         L0.r = D0.b_lat.as<double>();
         L0.dinv = D0.dinv_lat.as<double>();
