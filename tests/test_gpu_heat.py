@@ -111,3 +111,42 @@ def test_heat_radiation_design_eval(oracle, mastg, n, solver):
     P0 = oracle.Problem(oracle.HEAT, n, seed=5)
     assert abs(P0.solve_heat(1e-12, 1e-14)["mean_T"] - ref["mean_T"]) > 1e-3 * abs(ref["mean_T"])
     ctx.close()
+
+
+@pytest.mark.parametrize("solver", ["bicgstab", "mg"])
+def test_heat_distorted_mesh(oracle, mastg, solver):
+    """Cells of many shapes (interior nodes displaced): the element kernel forms
+    each element's own geometry instead of its congruence class's; R and J at
+    a state and the Newton + adjoint design evaluation against the oracle."""
+    print(f"[heat distorted] 6^3, {solver}", flush=True)
+    P0 = oracle.Problem(oracle.HEAT, 6, seed=4)
+    xyz, conn = P0.mesh()
+    lo, hi = xyz.min(0), xyz.max(0)
+    inner = np.all((xyz > lo + 1e-12) & (xyz < hi - 1e-12), axis=1)
+    xyz = xyz.copy()
+    xyz[inner] += 0.25 / 6 * np.random.default_rng(8).uniform(-1, 1, (int(inner.sum()), 3))
+    base, con = P0.dofmap()
+    sides = P0.bsides()
+    P = oracle.Problem(oracle.HEAT, arrays=dict(xyz=xyz, conn=conn, n_vars=1, node_base=base, constrained=con,
+                                                params=P0.params(), sides=sides))
+    P.set(k0=200.0, alpha=1e-3, conv_h=50.0, conv_Tinf=300.0, conv_mask=0x3E)
+    desc = mastg.desc_from_arrays(mastg.HEX8_HEAT, xyz, conn, 1, node_dof_base=base,
+                                  constrained_dofs=np.nonzero(con)[0], sides=sides, k0=200.0, alpha=1e-3,
+                                  conv_h=50.0, conv_tinf=300.0, conv_bid_mask=0x3E)
+    ctx = mastg.Context(desc)
+    ctx.set_params(P0.params())
+    T = 300 + 50 * np.random.default_rng(3).standard_normal(P.n_dofs)
+    T[con == 1] = 0.0
+    R0, K0 = P.assemble(T)
+    R, J = ctx.assemble(T)
+    orp, _ = P.csr()
+    assert rel_err_floor(J, K0, orp, floor=1e-2) <= 1e-12
+    np.testing.assert_allclose(R, R0, rtol=0, atol=1e-12 * np.abs(R0).max())
+    ref = P.solve_heat(1e-12, 1e-14)
+    sv = mastg.SOLVER_MG if solver == "mg" else mastg.SOLVER_BICGSTAB
+    r = ctx.design_eval(P0.params(), solver=sv, rtol=1e-12)
+    ec = abs(r["objective"] - ref["mean_T"]) / abs(ref["mean_T"])
+    eg = rel_err_floor(r["grad"], ref["dJ"], floor=1e-6)
+    print(f"  mean T {ec:.1e}, grad {eg:.1e}, {r['newton_iters']} Newton steps", flush=True)
+    assert ec <= 1e-9 and eg <= 1e-7
+    ctx.close()
