@@ -388,9 +388,12 @@ int mastg_get_tuning(const mastg_ctx* ctx, int* spmv_lanes, int* grid);
 #define MASTG_OPT_MG_CHEB_RATIO 4
 /* hex8 elasticity FP64 assembly: 1 (default) = fused kernel by CSR row owner
  * (element blocks, residual, scatter and inverse diagonal in one pass, no
- * element matrices in memory); 0 = element kernel + segmented scatter (also
- * used whenever the element matrices are needed: multigrid on an assembled
- * problem level, the FP32 element path) */
+ * element matrices in memory), over a precomputed row plan when the elements
+ * fall into congruence classes and no node has more than 8; 2 = the fused
+ * kernel without the plan (neighbour positions searched per node); 0 =
+ * element kernel + segmented scatter (also used whenever the element matrices
+ * are needed: multigrid on an assembled problem level, the FP32 element
+ * path).  All three give the same matrix. */
 #define MASTG_OPT_ROW_ASSEMBLY 5
 int mastg_set_option(mastg_ctx* ctx, int option, int value);
 int mastg_set_tuning(mastg_ctx* ctx, int spmv_lanes, int grid);

@@ -133,6 +133,8 @@ struct mastg_ctx {
     DevBuf sidemask, radmask, wmean, tpos, valT, vr0, vv, vs, vt, vph, vsh, bscal, vT;
     bool has_rad = false;
     bool row_fused = true;           // fused row-owner assembly allowed (MASTG_OPT_ROW_ASSEMBLY)
+    bool row_plan_ok = true;         // ... over the row plan where it applies (MASTG_OPT_ROW_ASSEMBLY 1)
+    DevBuf row_pos;                  // the class assembly's row plan + node constraint bits (mesh-static)
     double* scratch_ptr = nullptr;   // this device's `scratch` (allocation-free output)
     double ks_k = 10.0, ks_sref = 1.0;   // KS stress output: aggregation constant, reference stress
     DevBuf ks_part, ks_IV;
@@ -478,6 +480,19 @@ int assemble_active(mastg_ctx* c, const double* X_active, bool want_R, bool want
         a.val = c->val.as<double>();
         a.R = want_R ? c->vtmp.as<double>() : nullptr;
         a.dinv = c->dinv.as<double>();
+        // the class kernel's row plan: at most 8 elements per node
+        if (a.Khat && c->row_plan_ok && !c->row_pos.p) {
+            int32_t max_inc = 0;
+            for (int64_t r = 0; r < S.n_nodes; r++)
+                max_inc = std::max(max_inc, S.ninc_ptr[r + 1] - S.ninc_ptr[r]);
+            if (max_inc <= 8) {
+                TRY(dalloc(c, c->row_pos, size_t(S.n_nodes) * 65));
+                launch_hex8_row_plan(a, c->row_pos.as<int8_t>(), c->row_pos.as<uint8_t>() + S.n_nodes * 64,
+                                     c->stream);
+            }
+        }
+        a.row_pos = c->row_plan_ok ? c->row_pos.as<int8_t>() : nullptr;
+        a.cmask = a.row_pos ? c->row_pos.as<uint8_t>() + S.n_nodes * 64 : nullptr;
         {
             TimedRegion t(c, 0);
             launch_hex8_row_assemble(a, int(S.max_blocks_per_row), c->stream);
@@ -2460,7 +2475,11 @@ int mastg_set_option(mastg_ctx* c, int option, int value) {
     if (!c) return MASTG_ERR_INVALID;
     switch (option) {
         case MASTG_OPT_GRAPHS: c->use_graphs = value != 0; return MASTG_OK;
-        case MASTG_OPT_ROW_ASSEMBLY: c->row_fused = value != 0; return MASTG_OK;
+        case MASTG_OPT_ROW_ASSEMBLY:
+            if (value < 0 || value > 2) return fail(c, MASTG_ERR_INVALID, "row assembly option: 0, 1 or 2");
+            c->row_fused = value != 0;
+            c->row_plan_ok = value == 1;
+            return MASTG_OK;
         case MASTG_OPT_MG_STEPS_FINE:
         case MASTG_OPT_MG_STEPS_COARSE:
         case MASTG_OPT_MG_CHEB_RATIO:

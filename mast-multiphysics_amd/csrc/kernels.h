@@ -83,10 +83,17 @@ struct Hex8RowArgs {
     double* val;              // CSR values
     double* R;                // [n_active] residual or null
     double* dinv;             // [n_active] inverse diagonal or null
+    // row plan of the class kernel (null: positions searched per pass):
+    // [n_nodes*64] position in node r's block row of local node b of the
+    // element in incidence slot s, at r*64 + s*8 + b; -1 past the incidences
+    const int8_t* row_pos;
+    const uint8_t* cmask;     // [n_nodes] constraint bits of the node's 3 variables (with row_pos)
 };
 // rows with more neighbour blocks than this take the element + scatter path
 int hex8_row_assembly_max_blocks();
 void launch_hex8_row_assemble(const Hex8RowArgs& a, int max_blocks_per_row, hipStream_t s);
+// the row plan (needs at most 8 incidences per node)
+void launch_hex8_row_plan(const Hex8RowArgs& a, int8_t* pos, uint8_t* cmask, hipStream_t s);
 
 // the MAST-exact full-variable matrix from the active assembly (export.hip)
 struct FullExportArgs {

@@ -436,6 +436,252 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NP == 2 ? 5 
     }
 }
 
+// ---- the class assembly over a row plan ----
+// The row plan row_pos[r * 64 + s * 8 + b] holds, for node r, the position in
+// its sorted block row of local node b of the element in incidence slot s (-1
+// past the node's incidences).  It depends on the mesh only and is built once
+// (k_hex8_row_plan: the searches the kernel above does per pass).  With it a
+// node's data is two loads deep -- its block columns, incidences and plan
+// row, then the neighbours' state and constraint flags and the slots' scales
+// -- and k_hex8_row_assemble_plan keeps both levels ahead of the node it
+// assembles: the gathers of the next node and the index loads of the node
+// after it are in flight while a node is summed and written.  The work-group
+// runs over at most 32 consecutive nodes, whose pointers a lane each loads
+// once up front.  The sums, their order and the constraint treatment are
+// those of k_hex8_row_assemble_class.
+
+__global__ __launch_bounds__(256) void k_hex8_row_plan(Hex8RowArgs A, int8_t* pos, uint8_t* cmask) {
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= A.n_nodes * 64) return;
+    const int64_t r = t >> 6;
+    const int s = int(t >> 3) & 7, b = int(t) & 7;
+    if (s == 0 && b == 0)
+        cmask[r] = A.con ? uint8_t(A.con[r * 3] | (A.con[r * 3 + 1] << 1) | (A.con[r * 3 + 2] << 2)) : 0;
+    const int32_t i0 = A.ninc_ptr[r], ne = A.ninc_ptr[r + 1] - i0;
+    int p = -1;
+    if (s < ne) {
+        const int32_t rb = A.econn[int64_t(A.ninc[i0 + s] >> 3) * 8 + b];
+        const int64_t b0 = A.blk_ptr[r];
+        int64_t lo = b0, hi = A.blk_ptr[r + 1] - 1;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (A.blk_col[mid] < rb) lo = mid + 1;
+            else hi = mid;
+        }
+        p = int(lo - b0);
+    }
+    pos[t] = int8_t(p);
+}
+
+__device__ __forceinline__ int lane_i(int x, int j) { return __builtin_amdgcn_readlane(x, j); }
+__device__ __forceinline__ int64_t lane_i64(int64_t x, int j) {
+    const uint32_t lo = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(uint64_t(x))), j));
+    const uint32_t hi = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(uint64_t(x) >> 32)), j));
+    return int64_t((uint64_t(hi) << 32) | lo);
+}
+__device__ __forceinline__ double lane_d(double x, int j) {
+    return __longlong_as_double((long long)lane_i64(int64_t(__double_as_longlong(x)), j));
+}
+
+// an empty statement that takes v in a vector register: the load that
+// produced it is complete here
+__device__ __forceinline__ void ready(double v) { asm volatile("" ::"v"(v)); }
+__device__ __forceinline__ void ready(int v) { asm volatile("" ::"v"(v)); }
+
+constexpr int kPlanRun = 32;   // nodes per work-group at most (a lane holds each one's pointers)
+
+template <int NP, bool ONE>
+__global__ __launch_bounds__(64) void k_hex8_row_assemble_plan(Hex8RowArgs A) {
+    constexpr int MAXB = NP * 64 / 3;
+    constexpr int kZero = 64;   // kh row of zeros: the slots that do not touch a neighbour
+    __shared__ double kh[65 * 9];                    // ONE: Khat[a][b][v][w]; else [slot][b][v][w]; + zeros
+    __shared__ __align__(8) uint8_t cont[MAXB * 8];  // [block][slot] kh row of the slot's block, or kZero
+    // the run's residual and inverse diagonal, written out once at its end
+    // (node by node, 24-byte pieces of lines that leave L2 in between)
+    __shared__ double sres[kPlanRun * 3], sdinv[kPlanRun * 3];
+    const int lane = threadIdx.x;
+    for (int t = lane; t < 65 * 9; t += 64) kh[t] = (ONE && t < 576) ? A.Khat[t] : 0.0;
+    for (int t = lane; t < MAXB * 8; t += 64) cont[t] = kZero;
+    const int64_t nb = gridDim.x, per = (A.n_nodes + nb - 1) / nb;
+    const int64_t rbeg = xcd_block(blockIdx.x, nb) * per;
+    const int cnt = int(min(A.n_nodes, rbeg + per) - rbeg);
+    if (cnt <= 0) return;
+    // the run's pointers: lane i holds node rbeg + i's
+    int64_t pb = 0, prow = 0;
+    int32_t pn = 0;
+    int pcr = 0;
+    if (lane <= cnt) {
+        pb = A.blk_ptr[rbeg + lane];
+        pn = A.ninc_ptr[rbeg + lane];
+    }
+    if (lane < cnt) {
+        prow = A.row_ptr[(rbeg + lane) * 3];
+        pcr = A.cmask[rbeg + lane];
+    }
+    // level 1 of node jj: block column (lane k), incidence of slot lane / 8,
+    // plan entry (lane = slot * 8 + b), load row value (lanes 0..2)
+    auto load1 = [&](int jj, int& bc, int& inc, int& ps, double& f) {
+        const int64_t b0 = lane_i64(pb, jj);
+        const int deg = int(lane_i64(pb, jj + 1) - b0);
+        const int i0 = lane_i(pn, jj), ne = lane_i(pn, jj + 1) - i0;
+        bc = lane < deg ? A.blk_col[b0 + lane] : -1;
+        inc = (lane >> 3) < ne ? A.ninc[i0 + (lane >> 3)] : -1;
+        ps = A.row_pos[(rbeg + jj) * 64 + lane];
+        f = (A.F && lane < 3) ? A.F[(rbeg + jj) * 3 + lane] : 0.0;
+    };
+    // level 2: the neighbour's state and flags (lane k), the slot's scale and class
+    auto load2 = [&](int bc, int inc, double* x, int& cw, double& se, int& cl) {
+        x[0] = x[1] = x[2] = 0.0;
+        cw = 0;
+        se = 0.0;
+        cl = 0;
+        if (bc >= 0) {
+            const int64_t col = int64_t(bc) * 3;
+            if (A.X) {
+                x[0] = A.X[col];
+                x[1] = A.X[col + 1];
+                x[2] = A.X[col + 2];
+            }
+            cw = A.cmask[bc];
+        }
+        if (inc >= 0) {
+            se = A.sE[inc >> 3];
+            if (!ONE) cl = A.eclass[inc >> 3];
+        }
+    };
+    int bcA, incA, psA, bcB, incB, psB, cwA, clA;
+    double fA, fB, xA[3], seA;
+    load1(0, bcA, incA, psA, fA);
+    load2(bcA, incA, xA, cwA, seA, clA);
+    load1(min(1, cnt - 1), bcB, incB, psB, fB);
+    for (int j = 0; j < cnt; j++) {
+        // in flight while node j is assembled: node j + 1's gathers, node j + 2's indices
+        double xB[3], seB;
+        int cwB, clB, bcC, incC, psC;
+        double fC;
+        load2(bcB, incB, xB, cwB, seB, clB);
+        load1(min(j + 2, cnt - 1), bcC, incC, psC, fC);
+
+        const int64_t r = rbeg + j;
+        const int64_t b0 = lane_i64(pb, j);
+        const int deg = int(lane_i64(pb, j + 1) - b0);
+        const int ne = lane_i(pn, j + 1) - lane_i(pn, j);
+        const int64_t start = lane_i64(prow, j);
+        const int cr = lane_i(pcr, j);
+        const uint64_t dm = __ballot(bcA == int32_t(r));
+        const int kd = dm ? __builtin_ctzll(dm) : -1;
+        const int sl = lane >> 3, b = lane & 7, a = incA & 7;
+        if (incA >= 0) {
+            // an element's 8 nodes are distinct: one entry per (block, slot)
+            cont[psA * 8 + sl] = uint8_t((ONE ? a : sl) * 8 + b);
+            if (!ONE) {
+                const double* __restrict__ src = A.Khat + (int64_t(clA) * 64 + a * 8 + b) * 9;
+#pragma unroll
+                for (int i = 0; i < 9; i++) kh[(sl * 8 + b) * 9 + i] = src[i];
+            }
+        }
+        __syncthreads();
+        // slot scales (scalar registers; 0 past the node's incidences)
+        double se[8];
+#pragma unroll
+        for (int t = 0; t < 8; t++) se[t] = lane_d(seA, t * 8);
+        const int npair = 3 * deg, dg = max(deg, 1);
+        double acc[NP][3];
+        int pk[NP], pv[NP];
+#pragma unroll
+        for (int jp = 0; jp < NP; jp++) {
+            const int p = lane + 64 * jp;
+            pv[jp] = p / dg;
+            pk[jp] = p - pv[jp] * dg;
+            acc[jp][0] = acc[jp][1] = acc[jp][2] = 0.0;
+            if (p < npair) {
+                // slot order; a slot that does not touch the neighbour adds
+                // its scale times the zero row (an exact zero)
+                const uint64_t row = *reinterpret_cast<const uint64_t*>(cont + pk[jp] * 8);
+                const int vo = pv[jp] * 3;
+                double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+#pragma unroll
+                for (int t = 0; t < 8; t++) {
+                    const double* k3 = kh + int((row >> (8 * t)) & 0xff) * 9 + vo;
+                    s0 = fma(se[t], k3[0], s0);
+                    s1 = fma(se[t], k3[1], s1);
+                    s2 = fma(se[t], k3[2], s2);
+                }
+                acc[jp][0] = s0;
+                acc[jp][1] = s1;
+                acc[jp][2] = s2;
+            }
+        }
+        __syncthreads();   // the slots are read; clear this node's entries for the next
+        if (incA >= 0) cont[psA * 8 + sl] = kZero;
+        double rv0 = 0.0, rv1 = 0.0, rv2 = 0.0, vals[NP][3];
+#pragma unroll
+        for (int jp = 0; jp < NP; jp++) {
+            const int p = lane + 64 * jp;
+            const int k = pk[jp], v = pv[jp];
+            // the neighbour's state and flags, from the lane that gathered them
+            const double x0 = __shfl(xA[0], k, 64), x1 = __shfl(xA[1], k, 64), x2 = __shfl(xA[2], k, 64);
+            const int cw = __shfl(cwA, k, 64);
+            if (p < npair && A.X) {
+                const double q = acc[jp][0] * x0 + acc[jp][1] * x1 + acc[jp][2] * x2;
+                rv0 += v == 0 ? q : 0.0;
+                rv1 += v == 1 ? q : 0.0;
+                rv2 += v == 2 ? q : 0.0;
+            }
+            // constrained row or column: 0, and on the diagonal the number
+            // of elements that put a unit there (all ne touch the node itself)
+#pragma unroll
+            for (int w = 0; w < 3; w++) {
+                double val = acc[jp][w];
+                if (A.con && (((cr >> v) & 1) || ((cw >> w) & 1))) val = (k == kd && v == w) ? double(ne) : 0.0;
+                vals[jp][w] = val;
+            }
+        }
+#pragma unroll
+        for (int m = 1; m < 64; m <<= 1) {
+            rv0 += __shfl_xor(rv0, m, 64);
+            rv1 += __shfl_xor(rv1, m, 64);
+            rv2 += __shfl_xor(rv2, m, 64);
+        }
+        const double fr = fA;
+        // the next node's data is waited for here, before this node's stores
+        // are issued: the memory counter is in order, so a wait after them
+        // (where the compiler moves the loop's register copies) would wait for them
+        ready(xB[0]); ready(xB[1]); ready(xB[2]); ready(seB); ready(cwB); ready(clB);
+        ready(bcC); ready(incC); ready(psC); ready(fC);
+        bcA = bcB; incA = incB; psA = psB; fA = fB;
+        xA[0] = xB[0]; xA[1] = xB[1]; xA[2] = xB[2];
+        cwA = cwB; seA = seB; clA = clB;
+        bcB = bcC; incB = incC; psB = psC; fB = fC;
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int jp = 0; jp < NP; jp++) {
+            const int p = lane + 64 * jp;
+            const int k = pk[jp], v = pv[jp];
+            if (p < npair) {
+#pragma unroll
+                for (int w = 0; w < 3; w++) A.val[start + 3 * p + w] = vals[jp][w];
+                if (k == kd) {
+                    const double d = v == 0 ? vals[jp][0] : v == 1 ? vals[jp][1] : vals[jp][2];
+                    sdinv[j * 3 + v] = 1.0 / d;
+                }
+            }
+        }
+        if (kd < 0 && lane < 3) sdinv[j * 3 + lane] = 0.0;
+        if (lane < 3) {
+            const bool ci = (cr >> lane) & 1;
+            double v = lane == 0 ? rv0 : lane == 1 ? rv1 : rv2;
+            if (A.F && !ci) v -= fr;
+            sres[j * 3 + lane] = ci ? 0.0 : v;
+        }
+    }
+    __syncthreads();
+    for (int t = lane; t < cnt * 3; t += 64) {
+        if (A.R) A.R[rbeg * 3 + t] = sres[t];
+        if (A.dinv) A.dinv[rbeg * 3 + t] = sdinv[t];
+    }
+}
+
 }  // namespace
 
 int hex8_row_assembly_max_blocks() { return kRowMaxBlocks; }
@@ -444,6 +690,20 @@ void launch_hex8_row_assemble(const Hex8RowArgs& a, int max_blocks_per_row, hipS
     if (a.n_nodes <= 0) return;
     const dim3 g(unsigned(a.n_nodes));
     const bool small = max_blocks_per_row <= kRowSmallBlocks;
+    if (a.Khat && a.eclass && a.row_pos) {
+        // runs of at most kPlanRun consecutive nodes per work-group
+        const dim3 gp(unsigned(std::max<int64_t>(std::min<int64_t>(a.n_nodes, MASTG_ROW_GRID),
+                                                 (a.n_nodes + kPlanRun - 1) / kPlanRun)));
+        const bool two = max_blocks_per_row <= 42;
+        if (a.n_classes == 1) {
+            if (two) hipLaunchKernelGGL((k_hex8_row_assemble_plan<2, true>), gp, dim3(64), 0, s, a);
+            else hipLaunchKernelGGL((k_hex8_row_assemble_plan<3, true>), gp, dim3(64), 0, s, a);
+        } else {
+            if (two) hipLaunchKernelGGL((k_hex8_row_assemble_plan<2, false>), gp, dim3(64), 0, s, a);
+            else hipLaunchKernelGGL((k_hex8_row_assemble_plan<3, false>), gp, dim3(64), 0, s, a);
+        }
+        return;
+    }
     if (a.Khat && a.eclass) {
         const dim3 gc(unsigned(std::min<int64_t>(a.n_nodes, MASTG_ROW_GRID)));
         // pairs per lane: 2 up to 42 blocks per row, 3 up to 64
@@ -459,6 +719,12 @@ void launch_hex8_row_assemble(const Hex8RowArgs& a, int max_blocks_per_row, hipS
         if (small) hipLaunchKernelGGL((k_hex8_row_assemble<4>), g, dim3(64), 0, s, a);
         else hipLaunchKernelGGL((k_hex8_row_assemble<9>), g, dim3(64), 0, s, a);
     }
+}
+
+void launch_hex8_row_plan(const Hex8RowArgs& a, int8_t* pos, uint8_t* cmask, hipStream_t s) {
+    if (a.n_nodes <= 0) return;
+    const int64_t n = a.n_nodes * 64;
+    hipLaunchKernelGGL(k_hex8_row_plan, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, a, pos, cmask);
 }
 
 }  // namespace mastg

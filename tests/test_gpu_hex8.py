@@ -317,19 +317,29 @@ def test_design_eval_mixed(oracle, mastg, mesh, lo):
 
 
 @pytest.mark.parametrize("mesh,exact,distort", [((3, 2, 2), True, 0.0), ((12, 6, 6), False, 0.2),
-                                                ((20, 10, 10), False, 0.0)])
+                                                ((20, 10, 10), False, 0.0), ((9, 4, 3), False, -1.0)])
 def test_row_assembly_equals_element_scatter(oracle, mastg, mesh, exact, distort):
-    """The fused row-owner assembly (default) against the element kernel +
-    segmented scatter (MASTG_OPT_ROW_ASSEMBLY 0) and the oracle: J, R and the
-    design evaluation."""
+    """The fused row-owner assembly (default: over the row plan where the
+    elements fall into congruence classes) against the same kernel without the
+    plan (MASTG_OPT_ROW_ASSEMBLY 2, bitwise), the element kernel + segmented
+    scatter (MASTG_OPT_ROW_ASSEMBLY 0) and the oracle: J, R and the design
+    evaluation.  distort < 0: cells of alternating widths along x and y (four
+    congruence classes)."""
     print(f"[hex8 row assembly] mesh={mesh} mast_exact={exact} distortion={distort}", flush=True)
     P0 = oracle.Problem(oracle.CANTILEVER, *mesh, seed=21, lo=0.05, hi=1.0, mast_exact=exact)
     xyz, conn = P0.mesh()
-    if distort:
+    if distort > 0:
         lo, hi = xyz.min(0), xyz.max(0)
         inner = np.all((xyz > lo + 1e-12) & (xyz < hi - 1e-12), axis=1)
         xyz = xyz.copy()
         xyz[inner] += distort / mesh[0] * np.random.default_rng(2).uniform(-1, 1, (int(inner.sum()), 3))
+    elif distort < 0:
+        xyz = xyz.copy()
+        for d in (0, 1):
+            u = np.unique(xyz[:, d])
+            i = np.searchsorted(u, xyz[:, d])
+            h = u[1] - u[0]
+            xyz[:, d] = i * h + 0.3 * h * (i % 2)      # widths 1.3 h and 0.7 h in turn
     base, con = P0.dofmap()
     arrays = dict(xyz=xyz, conn=conn, n_vars=P0.n_vars, node_base=base, constrained=con, params=P0.params(),
                   point_load=P0.point_load())
@@ -342,6 +352,10 @@ def test_row_assembly_equals_element_scatter(oracle, mastg, mesh, exact, distort
     X = np.random.default_rng(4).standard_normal(P.n_dofs) * 1e-4
     X[con == 1] = 0.0
     R1, J1 = ctx.assemble(X)
+    ctx.set_option(mastg.OPT_ROW_ASSEMBLY, 2)
+    R2, J2 = ctx.assemble(X)
+    assert np.array_equal(J1, J2)
+    assert np.max(np.abs(R1 - R2)) <= 1e-14 * np.abs(R2).max()
     ctx.set_option(mastg.OPT_ROW_ASSEMBLY, 0)
     R0, J0 = ctx.assemble(X)
     rp, _ = ctx.csr_pattern()
