@@ -267,17 +267,34 @@ int mastg_sensitivity_solve(mastg_ctx* ctx, int param, const double* X, double* 
                                        KSStressStrainOutput (src/elasticity/ks_stress_output.cpp:53-243)
                                        over StructuralElement3D::calculate_stress
                                        (src/elasticity/solid_element_3d.cpp:1147-1236)  */
+#define MASTG_OUTPUT_PNORM_STRESS 4 /* hex8 elasticity: s0 (sum_qp (svm/s0)^p w JxW / sum_qp w JxW)^(1/p),
+                                       w = exp(min(rho ((svm - s0)/s0)^pw, 100)) --
+                                       StressStrainOutputBase::functional_for_all_elems
+                                       (src/elasticity/stress_output_base.cpp:852-908; the stress
+                                       constraint of examples/structural/example_6/example_6.cpp:515-587) */
+#define MASTG_OUTPUT_RAMP_STRESS 5  /* hex8 elasticity: sum_qp ((1 + (svm/s0)^p)^(1/p) - 1) JxW --
+                                       SmoothRampStressStrainOutput::functional_for_all_elems
+                                       (src/elasticity/smooth_ramp_stress_output.cpp:36-87) */
 int mastg_output(mastg_ctx* ctx, int kind, const double* X, int where, double* value,
                  double* dvalue_dX /* [n_dofs] or NULL */);
 /* KS stress output's aggregation constant k and reference stress s_ref
  * (KSStressStrainOutput's _p_norm_stress and _sigma0; defaults 10 and 1) */
 int mastg_set_ks_stress(mastg_ctx* ctx, double k, double sigma_ref);
+/* p-norm and smooth-ramp stress outputs' coefficients: the stress exponent p,
+ * the weight exponent pw, rho and the reference stress s0 --
+ * StressStrainOutputBase::set_aggregation_coefficients(p1, p2, rho, sigma0)
+ * (src/elasticity/stress_output_base.h:246-252; defaults 2, 1, 1, and s0 has
+ * no default: the outputs refuse until it is set).  The ramp uses p and s0. */
+int mastg_set_stress_aggregation(mastg_ctx* ctx, double p, double p_weight, double rho, double sigma0);
 /* Direct derivative of an output with respect to every element parameter at
  * state X, [n_elems] (the partial dJ/dp_e at fixed X; the total derivative
  * adds -lambda . dR/dp_e from mastg_sensitivity with lambda solving
- * J^T lambda = dJ/dX): zero for compliance and mean temperature; for the KS
- * stress the rho_e^p dependence of the stress --
- * KSStressStrainOutput::functional_sensitivity_for_elem (ks_stress_output.cpp:98-136). */
+ * J^T lambda = dJ/dX): zero for compliance and mean temperature; for the
+ * stress outputs the rho_e^p dependence of the stress --
+ * KSStressStrainOutput::functional_sensitivity_for_elem (ks_stress_output.cpp:98-136),
+ * StressStrainOutputBase::functional_sensitivity_for_elem
+ * (stress_output_base.cpp:973-1043), SmoothRampStressStrainOutput::
+ * functional_sensitivity_for_elem (smooth_ramp_stress_output.cpp:90-130). */
 int mastg_output_sensitivity(mastg_ctx* ctx, int kind, const double* X, int where, double* dJdp);
 
 /* mastg_output for callers that cannot allocate (code under HIP graph

@@ -137,6 +137,9 @@ struct mastg_ctx {
     DevBuf row_pos;                  // the class assembly's row plan + node constraint bits (mesh-static)
     double* scratch_ptr = nullptr;   // this device's `scratch` (allocation-free output)
     double ks_k = 10.0, ks_sref = 1.0;   // KS stress output: aggregation constant, reference stress
+    // p-norm and smooth-ramp stress outputs: p, weight exponent, rho, reference stress
+    // (StressStrainOutputBase's defaults 2, 1, 1; the reference stress must be set)
+    double agg_p = 2.0, agg_pw = 1.0, agg_rho = 1.0, agg_s0 = 0.0;
     DevBuf ks_part, ks_IV;
     double volume = 0;
     hipGraphExec_t bicg_exec = nullptr;
@@ -2197,25 +2200,65 @@ int mastg_set_ks_stress(mastg_ctx* c, double k, double sigma_ref) {
     return MASTG_OK;
 }
 
+int mastg_set_stress_aggregation(mastg_ctx* c, double p, double p_weight, double rho, double sigma0) {
+    if (!c) return MASTG_ERR_INVALID;
+    if (!(p > 0) || !(sigma0 > 0) || !std::isfinite(p) || !std::isfinite(p_weight) || !std::isfinite(rho) ||
+        !std::isfinite(sigma0))
+        return fail(c, MASTG_ERR_INVALID, "stress aggregation: p and the reference stress must be positive, all finite");
+    c->agg_p = p;
+    c->agg_pw = p_weight;
+    c->agg_rho = rho;
+    c->agg_s0 = sigma0;
+    return MASTG_OK;
+}
+
 }  // extern "C"
 
 namespace {
 
-// KS stress at the active state in c->vb: I and V into c->ks_IV (device),
-// value on the host
-int ks_value(mastg_ctx* c, double* value) {
+bool is_stress_output(int kind) {
+    return kind == MASTG_OUTPUT_KS_STRESS || kind == MASTG_OUTPUT_PNORM_STRESS || kind == MASTG_OUTPUT_RAMP_STRESS;
+}
+
+StressAgg stress_agg(const mastg_ctx* c, int kind) {
+    StressAgg g{};
+    g.lim = 100.0;   // StressStrainOutputBase::_exp_arg_lim
+    if (kind == MASTG_OUTPUT_KS_STRESS) {
+        g.kind = kAggKS;
+        g.p = c->ks_k;
+        g.s0 = c->ks_sref;
+    } else {
+        g.kind = kind == MASTG_OUTPUT_PNORM_STRESS ? kAggPNorm : kAggRamp;
+        g.p = c->agg_p;
+        g.pw = c->agg_pw;
+        g.rho = c->agg_rho;
+        g.s0 = c->agg_s0;
+    }
+    return g;
+}
+
+// a stress aggregate at the active state in c->vb: I and V into c->ks_IV
+// (device), value on the host
+int ks_value(mastg_ctx* c, int kind, double* value) {
     if (c->d.physics != MASTG_HEX8_ELASTIC)
-        return fail(c, MASTG_ERR_INVALID, "the KS stress output is defined for hex8 elasticity");
+        return fail(c, MASTG_ERR_INVALID, "the stress outputs are defined for hex8 elasticity");
     if (!c->params_set) return fail(c, MASTG_ERR_INVALID, "design parameters not set");
+    if (kind != MASTG_OUTPUT_KS_STRESS && !(c->agg_s0 > 0))
+        return fail(c, MASTG_ERR_INVALID, "set the reference stress first (mastg_set_stress_aggregation)");
+    const StressAgg g = stress_agg(c, kind);
     TRY(dalloc(c, c->ks_part, 2 * 1024 * sizeof(double)));
     TRY(dalloc(c, c->ks_IV, 2 * sizeof(double)));
-    launch_hex8_ks_sum(hex8_args(c, c->vb.as<double>(), false), c->ks_k, c->ks_sref, c->ks_part.as<double>(),
-                       c->ks_IV.as<double>(), c->stream);
+    launch_hex8_ks_sum(hex8_args(c, c->vb.as<double>(), false), g, c->ks_part.as<double>(), c->ks_IV.as<double>(),
+                       c->stream);
     HIPC(c, hipMemcpyAsync(c->h_pin, c->ks_IV.p, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPC(c, hipStreamSynchronize(c->stream));
-    if (!(c->h_pin[0] > 0) || !std::isfinite(c->h_pin[0]))
-        return fail(c, MASTG_ERR_NAN, "KS sum overflowed or vanished: choose the reference stress near the peak stress");
-    *value = std::log(c->h_pin[0] / c->h_pin[1]) / c->ks_k;
+    const double I = c->h_pin[0], V = c->h_pin[1];
+    if (!(I > 0) || !std::isfinite(I) || !(V > 0) || !std::isfinite(V))
+        return fail(c, MASTG_ERR_NAN,
+                    "stress aggregate overflowed or vanished: choose the reference stress near the peak stress");
+    if (g.kind == kAggKS) *value = std::log(I / V) / g.p;
+    else if (g.kind == kAggPNorm) *value = g.s0 * std::pow(I / V, 1.0 / g.p);
+    else *value = I - V;
     return MASTG_OK;
 }
 
@@ -2324,13 +2367,13 @@ int mastg_output_sensitivity(mastg_ctx* c, int kind, const double* X, int where,
         // the loads and the mean-temperature weights do not depend on the
         // element parameters
         launch_fill(c->S.n_elems, 0.0, gd, c->stream);
-    } else if (kind == MASTG_OUTPUT_KS_STRESS) {
+    } else if (is_stress_output(kind)) {
         TRY(to_active(c, X, where, c->vb.as<double>()));
         double v;
-        TRY(ks_value(c, &v));
+        TRY(ks_value(c, kind, &v));
         TRY(dalloc(c, c->Rel, size_t(c->S.n_elems) * 24 * sizeof(double)));
-        launch_hex8_ks_grad(hex8_args(c, c->vb.as<double>(), false), c->ks_k, c->ks_sref, c->ks_IV.as<double>(),
-                            gd, c->stream);
+        launch_hex8_ks_grad(hex8_args(c, c->vb.as<double>(), false), stress_agg(c, kind), c->ks_IV.as<double>(), gd,
+                            c->stream);
     } else {
         return fail(c, MASTG_ERR_INVALID, "output not available");
     }
@@ -2343,13 +2386,13 @@ int mastg_output_sensitivity(mastg_ctx* c, int kind, const double* X, int where,
 
 int mastg_output(mastg_ctx* c, int kind, const double* X, int where, double* value, double* dv) {
     if (!c || !X || !value) return MASTG_ERR_INVALID;
-    if (kind == MASTG_OUTPUT_KS_STRESS) {
+    if (is_stress_output(kind)) {
         TRY(to_active(c, X, where, c->vb.as<double>()));
-        TRY(ks_value(c, value));
+        TRY(ks_value(c, kind, value));
         if (dv) {
             TRY(dalloc(c, c->Rel, size_t(c->S.n_elems) * 24 * sizeof(double)));
-            launch_hex8_ks_grad(hex8_args(c, c->vb.as<double>(), false), c->ks_k, c->ks_sref,
-                                c->ks_IV.as<double>(), nullptr, c->stream);
+            launch_hex8_ks_grad(hex8_args(c, c->vb.as<double>(), false), stress_agg(c, kind), c->ks_IV.as<double>(),
+                                nullptr, c->stream);
             launch_node_residual(c->S.n_nodes, c->S.nav, c->S.npe, c->ninc_ptr.as<int32_t>(),
                                  c->ninc.as<int32_t>(), c->Rel.as<double>(), nullptr, c->con.as<uint8_t>(),
                                  c->vtmp.as<double>(), c->stream);

@@ -221,12 +221,20 @@ void launch_spmv_nb64(const CGArgs& c, const double* x, double* y, int want_dot,
 void launch_cg_replace(const CGArgs& c, const double* rt, int restart, hipStream_t s);
 void launch_hex8_K(const Hex8Args& a, hipStream_t s);
 void launch_hex8_sens(const Hex8Args& a, const double* lam, double* g, hipStream_t s);
-// KS aggregate of the von Mises stress (aggregation constant k, reference
-// stress sref) at the state a.X: IV[0] = sum exp(k (svm - sref) / sref) JxW,
-// IV[1] = sum JxW (part: 2 x 1024 doubles of partial sums); then the element
-// rows of dKS/dX into a.Rel and, when dpar is not null, dKS/drho_e|direct
-void launch_hex8_ks_sum(const Hex8Args& a, double k, double sref, double* part, double* IV, hipStream_t s);
-void launch_hex8_ks_grad(const Hex8Args& a, double k, double sref, const double* IV, double* dpar, hipStream_t s);
+// aggregates of the von Mises stress: KS (p = the aggregation constant k),
+// the weighted p-norm of StressStrainOutputBase (p, weight exponent pw, rho,
+// exponent clamp lim) and the smooth ramp (p); s0 the reference stress
+constexpr int kAggKS = 0, kAggPNorm = 1, kAggRamp = 2;
+struct StressAgg {
+    int kind;
+    double p, pw, rho, s0, lim;
+};
+// at the state a.X: IV[0] = I, IV[1] = V, the aggregate's two sums over the
+// quadrature points (part: 2 x 1024 doubles of partial sums); then the
+// element rows of the state derivative into a.Rel and, when dpar is not
+// null, the direct density derivative
+void launch_hex8_ks_sum(const Hex8Args& a, const StressAgg& g, double* part, double* IV, hipStream_t s);
+void launch_hex8_ks_grad(const Hex8Args& a, const StressAgg& g, const double* IV, double* dpar, hipStream_t s);
 void launch_hex8_K_f32(const Hex8Args& a, float* Kel32, hipStream_t s);
 void launch_scatter_f32(const ScatterArgs& a, const float* Kel32, hipStream_t s);
 void launch_scatter(const ScatterArgs& a, hipStream_t s);

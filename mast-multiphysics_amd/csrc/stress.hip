@@ -1,24 +1,32 @@
-// Stress outputs of hex8 elasticity: the Kreisselmeier-Steinhauser aggregate
-// of the von Mises stress over the quadrature points, its derivative with
-// respect to the state and its direct derivative with respect to the element
-// densities.
+// Stress outputs of hex8 elasticity: aggregates of the von Mises stress over
+// the quadrature points, their derivative with respect to the state and their
+// direct derivative with respect to the element densities.
 //
 //   stress at each quadrature point: sigma = E_e D(nu) B u_e, E_e = E0 rho_e^p
 //     (StructuralElement3D::calculate_stress, src/elasticity/solid_element_3d.cpp:1147-1236)
 //   von Mises: P = ((s0-s1)^2 + (s1-s2)^2 + (s2-s0)^2) / 2 + 3 (s3^2 + s4^2 + s5^2),
 //     svm = P^(1/2) (StressStrainOutputBase::Data::von_Mises_stress,
 //     src/elasticity/stress_output_base.cpp:183-230; derivative 0 where P = 0)
-//   KS = (1/k) ln( I / V ), I = sum_qp exp(k (svm - s_ref) / s_ref) JxW, V = sum_qp JxW
-//     (KSStressStrainOutput::functional_for_all_elems, src/elasticity/ks_stress_output.cpp:53-93)
-//   dKS/dX = sum_qp [exp(k (svm - s_ref) / s_ref) JxW / I] (dsvm/dX) / s_ref
-//     (functional_state_derivartive_for_elem, ks_stress_output.cpp:201-243)
-//   dKS/drho_e|direct = the same weights times dsvm/drho_e = (p / rho_e) svm
-//     (functional_sensitivity_for_elem, ks_stress_output.cpp:98-136)
+// Each aggregate is a function of two sums over the points, I = sum f_I(svm)
+// JxW and V = sum f_V(svm) JxW, with s0 the reference stress:
+//   KS (KSStressStrainOutput, src/elasticity/ks_stress_output.cpp:53-243):
+//     f_I = exp(k (svm - s0) / s0), f_V = 1, value (1/k) ln(I / V)
+//   p-norm (StressStrainOutputBase::functional_for_all_elems and its
+//   derivatives, stress_output_base.cpp:852-1176): weight
+//     w = exp(min(rho ((svm - s0) / s0)^pw, lim)) (lim = 100, the clamp of
+//     _exp_arg_lim), f_I = (svm / s0)^p w, f_V = w, value s0 (I / V)^(1/p)
+//   smooth ramp (SmoothRampStressStrainOutput, src/elasticity/
+//   smooth_ramp_stress_output.cpp:36-241): f_I = (1 + (svm / s0)^p)^(1/p),
+//     f_V = 1, value I - V
+// The derivatives are sums over the points of c_q dsvm_q/d(.), c_q = the
+// value's derivative in svm_q (the reference's num_sens / denom_sens terms);
+// dsvm/drho_e|direct = (p / rho_e) svm.
 //
 // Two passes, a thread per element: the first sums I and V per work-group
 // (fixed grid, fixed-order final reduction: deterministic), the second reads
-// I from device memory and writes the element rows of dKS/dX (constrained
-// rows 0, assembled by the node-residual gather) and the direct derivative.
+// I and V from device memory and writes the element rows of the state
+// derivative (constrained rows 0, assembled by the node-residual gather) and
+// the direct derivative.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -114,7 +122,47 @@ __device__ __forceinline__ void ks_load(const Hex8Args& A, int64_t e, double x[8
     }
 }
 
-__global__ __launch_bounds__(256) void k_hex8_ks_sum(Hex8Args A, double k, double sref, double* __restrict__ part) {
+// the point's contributions to I and V
+__device__ __forceinline__ void agg_terms(const StressAgg& G, double svm, double jxw, double& dI, double& dV) {
+    if (G.kind == kAggPNorm) {
+        const double sp = pow((svm - G.s0) / G.s0, G.pw);
+        const double w = G.rho * sp > G.lim ? exp(G.lim) : exp(G.rho * sp);
+        dI = pow(svm / G.s0, G.p) * w * jxw;
+        dV = w * jxw;
+    } else if (G.kind == kAggRamp) {
+        dI = pow(1.0 + pow(svm / G.s0, G.p), 1.0 / G.p) * jxw;
+        dV = jxw;
+    } else {
+        dI = exp(G.p * (svm - G.s0) / G.s0) * jxw;
+        dV = jxw;
+    }
+}
+
+// c_q: the value's derivative in the point's von Mises stress
+__device__ __forceinline__ double agg_coef(const StressAgg& G, double svm, double jxw, double I, double V) {
+    if (G.kind == kAggPNorm) {
+        const double x = svm / G.s0, y = (svm - G.s0) / G.s0;
+        const double sps = pow(x, G.p), sp1s = pow(x, G.p - 1.0);
+        const double spw = pow(y, G.pw), sp1w = pow(y, G.pw - 1.0);
+        double num, den;
+        if (G.rho * spw > G.lim) {
+            num = G.p * sp1s * exp(G.lim);
+            den = 0.0;
+        } else {
+            const double w = exp(G.rho * spw);
+            num = w * (G.p * sp1s + sps * G.rho * G.pw * sp1w);
+            den = w * G.rho * G.pw * sp1w;
+        }
+        return G.s0 / G.p * pow(I / V, 1.0 / G.p - 1.0) * (num / V - I / (V * V) * den) * jxw / G.s0;
+    }
+    if (G.kind == kAggRamp) {
+        const double x = svm / G.s0;
+        return pow(1.0 + pow(x, G.p), 1.0 / G.p - 1.0) * pow(x, G.p - 1.0) * jxw / G.s0;
+    }
+    return exp(G.p * (svm - G.s0) / G.s0) * jxw * (1.0 / I) / G.s0;
+}
+
+__global__ __launch_bounds__(256) void k_hex8_ks_sum(Hex8Args A, StressAgg G, double* __restrict__ part) {
     double I = 0.0, V = 0.0;
     for (int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x; e < A.n_elems; e += int64_t(gridDim.x) * 256) {
         double x[8][3], u[8][3];
@@ -124,8 +172,10 @@ __global__ __launch_bounds__(256) void k_hex8_ks_sum(Hex8Args A, double k, doubl
             double g[8][3], s[6];
             const double jxw = ks_geometry(x, q, g);
             const double svm = ks_stress(A, g, u, Ee, s);
-            I += exp(k * (svm - sref) / sref) * jxw;
-            V += jxw;
+            double dI, dV;
+            agg_terms(G, svm, jxw, dI, dV);
+            I += dI;
+            V += dV;
         }
     }
     block_sum_store(I, part, blockIdx.x);
@@ -143,15 +193,15 @@ __global__ __launch_bounds__(256) void k_ks_reduce(const double* __restrict__ pa
     }
 }
 
-__global__ __launch_bounds__(128) void k_hex8_ks_grad(Hex8Args A, double k, double sref,
-                                                     const double* __restrict__ IV, double* __restrict__ dpar) {
+__global__ __launch_bounds__(128) void k_hex8_ks_grad(Hex8Args A, StressAgg G, const double* __restrict__ IV,
+                                                     double* __restrict__ dpar) {
     const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (e >= A.n_elems) return;
     double x[8][3], u[8][3];
     ks_load(A, e, x, u);
     const double rho = A.param[e];
     const double Ee = A.E0 * pow(rho, A.penalty);
-    const double invI = 1.0 / IV[0];
+    const double I = IV[0], V = IV[1];
     double f[8][3];
 #pragma unroll
     for (int n = 0; n < 8; n++) f[n][0] = f[n][1] = f[n][2] = 0.0;
@@ -160,7 +210,7 @@ __global__ __launch_bounds__(128) void k_hex8_ks_grad(Hex8Args A, double k, doub
         double g[8][3], s[6];
         const double jxw = ks_geometry(x, q, g);
         const double svm = ks_stress(A, g, u, Ee, s);
-        const double w = exp(k * (svm - sref) / sref) * jxw * invI / sref;
+        const double w = agg_coef(G, svm, jxw, I, V);
         direct += w * svm;
         if (!(svm > 0.0)) continue;
         // dsvm/dsigma = dP/dsigma / (2 svm); d sigma / du = Ee D B (D symmetric)
@@ -193,15 +243,15 @@ __global__ __launch_bounds__(128) void k_hex8_ks_grad(Hex8Args A, double k, doub
 
 }  // namespace
 
-void launch_hex8_ks_sum(const Hex8Args& a, double k, double sref, double* part, double* IV, hipStream_t s) {
+void launch_hex8_ks_sum(const Hex8Args& a, const StressAgg& g, double* part, double* IV, hipStream_t s) {
     const int nb = int(std::min<int64_t>(kKsBlocks, std::max<int64_t>(1, (a.n_elems + 255) / 256)));
-    hipLaunchKernelGGL(k_hex8_ks_sum, dim3(nb), dim3(256), 0, s, a, k, sref, part);
+    hipLaunchKernelGGL(k_hex8_ks_sum, dim3(nb), dim3(256), 0, s, a, g, part);
     hipLaunchKernelGGL(k_ks_reduce, dim3(1), dim3(256), 0, s, part, nb, IV);
 }
 
-void launch_hex8_ks_grad(const Hex8Args& a, double k, double sref, const double* IV, double* dpar, hipStream_t s) {
-    hipLaunchKernelGGL(k_hex8_ks_grad, dim3(std::max<int64_t>(1, (a.n_elems + 127) / 128)), dim3(128), 0, s, a, k,
-                       sref, IV, dpar);
+void launch_hex8_ks_grad(const Hex8Args& a, const StressAgg& g, const double* IV, double* dpar, hipStream_t s) {
+    hipLaunchKernelGGL(k_hex8_ks_grad, dim3(std::max<int64_t>(1, (a.n_elems + 127) / 128)), dim3(128), 0, s, a, g,
+                       IV, dpar);
 }
 
 }  // namespace mastg

@@ -237,6 +237,50 @@ private:
     double _p, _s0;
 };
 
+// The weighted p-norm of the von Mises stress (StressStrainOutputBase, the
+// stress constraint of example_6) and the smooth ramp
+// (SmoothRampStressStrainOutput): set_aggregation_coefficients(p1, p2, rho,
+// sigma0) as in MAST; output_total, output_derivative (d/dX) and
+// output_sensitivity (direct d/drho_e) as for the KS output
+template <int KIND>
+class AggregatedStressOutput {
+public:
+    void set_participating_elements_to_all() {}
+    void set_aggregation_coefficients(double p1, double p2, double rho, double sigma0) {
+        _p = p1;
+        _pw = p2;
+        _rho = rho;
+        _s0 = sigma0;
+    }
+    double evaluate(NonlinearImplicitAssembly& a, const RealVector& X) {
+        configure(a);
+        double v = 0;
+        check(mastg_output(a.context(), KIND, X.data(), MASTG_HOST, &v, nullptr), a.context(), "stress output");
+        return v;
+    }
+    void output_derivative(NonlinearImplicitAssembly& a, const RealVector& X, RealVector& dq_dX) {
+        configure(a);
+        dq_dX.assign(a.sizes().n_dofs, 0.0);
+        double v = 0;
+        check(mastg_output(a.context(), KIND, X.data(), MASTG_HOST, &v, dq_dX.data()), a.context(),
+              "stress output derivative");
+    }
+    void output_sensitivity(NonlinearImplicitAssembly& a, const RealVector& X, RealVector& dq_dp) {
+        configure(a);
+        dq_dp.assign(size_t(a.n_elems()), 0.0);
+        check(mastg_output_sensitivity(a.context(), KIND, X.data(), MASTG_HOST, dq_dp.data()), a.context(),
+              "stress output sensitivity");
+    }
+private:
+    void configure(NonlinearImplicitAssembly& a) {
+        check(mastg_set_stress_aggregation(a.context(), _p, _pw, _rho, _s0), a.context(),
+              "set_aggregation_coefficients");
+    }
+    double _p = 2.0, _pw = 1.0, _rho = 1.0, _s0 = 0.0;   // StressStrainOutputBase's defaults
+};
+using StressStrainOutputBase = AggregatedStressOutput<MASTG_OUTPUT_PNORM_STRESS>;
+using SmoothRampStressStrainOutput = AggregatedStressOutput<MASTG_OUTPUT_RAMP_STRESS>;
+
 // ---- mesh input -------------------------------------------------------------
 // NastranIO::read: the model's mesh, DofMap, SPCs, loads and properties as a
 // SystemDescription (owned by this object) and the element parameters

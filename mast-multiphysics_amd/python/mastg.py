@@ -15,7 +15,7 @@ HEX8_ELASTIC, QUAD4_MINDLIN, HEX8_HEAT = 1, 2, 3
 HOST, DEVICE = 0, 1
 ELEM_FP64, ELEM_FP32 = 0, 1
 SOLVER_CG_JACOBI, SOLVER_MIXED, SOLVER_BICGSTAB, SOLVER_MATFREE, SOLVER_MG = 0, 1, 2, 3, 4
-OUTPUT_COMPLIANCE, OUTPUT_MEAN_T, OUTPUT_KS_STRESS = 1, 2, 3
+OUTPUT_COMPLIANCE, OUTPUT_MEAN_T, OUTPUT_KS_STRESS, OUTPUT_PNORM_STRESS, OUTPUT_RAMP_STRESS = 1, 2, 3, 4, 5
 PARAM_E, PARAM_NU, PARAM_PRESSURE, PARAM_K0, PARAM_CONV_H, PARAM_EMISSIVITY = 1, 2, 3, 4, 5, 6
 OPT_GRAPHS = 1
 OPT_MG_STEPS_FINE = 2
@@ -112,6 +112,7 @@ def lib():
             "mastg_csr_full_size": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
             "mastg_export_csr_full": (C.c_int, [_P, C.c_void_p, C.c_void_p, C.c_void_p]),
             "mastg_set_ks_stress": (C.c_int, [_P, C.c_double, C.c_double]),
+            "mastg_set_stress_aggregation": (C.c_int, [_P, C.c_double, C.c_double, C.c_double, C.c_double]),
             "mastg_output_sensitivity": (C.c_int, [_P, C.c_int, _D, C.c_int, _D]),
             "mastg_device_alloc": (C.c_int, [_P, C.c_size_t, C.POINTER(C.c_void_p)]),
             "mastg_device_free": (C.c_int, [_P, C.c_void_p]),
@@ -400,6 +401,10 @@ class Context:
 
     def set_ks_stress(self, k, sigma_ref):
         self._chk(lib().mastg_set_ks_stress(self.h, k, sigma_ref))
+
+    def set_stress_aggregation(self, p, p_weight, rho, sigma0):
+        """StressStrainOutputBase::set_aggregation_coefficients(p1, p2, rho, sigma0)."""
+        self._chk(lib().mastg_set_stress_aggregation(self.h, p, p_weight, rho, sigma0))
 
     def output_sensitivity(self, kind, X):
         """Direct dJ/dp_e of an output at fixed state X."""

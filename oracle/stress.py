@@ -1,10 +1,13 @@
-"""Oracle (test infrastructure only): the KS von Mises stress output.
+"""Oracle (test infrastructure only): the von Mises stress outputs.
 
 Only tests/ use this module, as the checker.  Restated in numpy from
 StressStrainOutputBase::Data::von_Mises_stress (src/elasticity/
-stress_output_base.cpp:183-195) and KSStressStrainOutput::
-functional_for_all_elems (src/elasticity/ks_stress_output.cpp:53-93), over
-the quadrature-point stresses of the oracle's hex8 element
+stress_output_base.cpp:183-195), KSStressStrainOutput::
+functional_for_all_elems (src/elasticity/ks_stress_output.cpp:53-93),
+StressStrainOutputBase::functional_for_all_elems (the weighted p-norm,
+stress_output_base.cpp:852-908) and SmoothRampStressStrainOutput::
+functional_for_all_elems (src/elasticity/smooth_ramp_stress_output.cpp:
+36-87), over the quadrature-point stresses of the oracle's hex8 element
 (StructuralElement3D::calculate_stress, solid_element_3d.cpp:1147-1236).
 """
 import numpy as np
@@ -36,3 +39,20 @@ def ks(svm, jxw, k, sref):
     """(1/k) ln( sum exp(k (svm - sref) / sref) JxW / sum JxW )."""
     I = np.sum(np.exp(k * (svm - sref) / sref) * jxw)
     return np.log(I / np.sum(jxw)) / k
+
+
+def pnorm(svm, jxw, p, pw, rho, s0, lim=100.0):
+    """s0 (sum (svm/s0)^p w JxW / sum w JxW)^(1/p), w = exp(rho ((svm-s0)/s0)^pw)
+    with the exponent clamped at lim (stress_output_base.cpp:885-907)."""
+    sp = ((svm - s0) / s0) ** pw
+    w = np.where(rho * sp > lim, np.exp(lim), np.exp(np.minimum(rho * sp, lim)))
+    I = np.sum((svm / s0) ** p * w * jxw)
+    V = np.sum(w * jxw)
+    return s0 * (I / V) ** (1.0 / p)
+
+
+def ramp(svm, jxw, p, s0):
+    """sum ((1 + (svm/s0)^p)^(1/p) - 1) JxW, summed as I - V
+    (smooth_ramp_stress_output.cpp:73-86)."""
+    I = np.sum((1.0 + (svm / s0) ** p) ** (1.0 / p) * jxw)
+    return I - np.sum(jxw)
