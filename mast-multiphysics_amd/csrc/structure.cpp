@@ -56,6 +56,12 @@ std::string build_structure(int physics, int64_t n_nodes, int64_t n_elems, const
     if (n_elems * int64_t(npe) * npe >= (int64_t(1) << 31)) return "mesh too large for 32-bit maps";
     for (int64_t i = 0; i < n_elems * npe; i++)
         if (conn[i] < 0 || conn[i] >= n_nodes) return "connectivity out of range";
+    // a node twice in one element: a degenerate element (zero Jacobian), and
+    // the assembly places each of an element's nodes once in a row
+    for (int64_t e = 0; e < n_elems; e++)
+        for (int a = 1; a < npe; a++)
+            for (int b = 0; b < a; b++)
+                if (conn[e * npe + a] == conn[e * npe + b]) return "element with a repeated node";
 
     // ---- caller DOF bases (libMesh first-touch when not given)
     std::vector<int64_t> base(n_nodes, -1);
