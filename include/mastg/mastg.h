@@ -394,14 +394,15 @@ int mastg_get_tuning(const mastg_ctx* ctx, int* spmv_lanes, int* grid);
 #define MASTG_OPT_GRAPHS 1
 /* multigrid smoothing: Chebyshev steps per pre-/post-smoothing on the problem
  * level (MASTG_OPT_MG_STEPS_FINE) and on the coarse levels
- * (MASTG_OPT_MG_STEPS_COARSE); 1..8, defaults: hex8 elasticity 6 and 2,
- * plate 4 and 2, heat 3 and 2 */
+ * (MASTG_OPT_MG_STEPS_COARSE); 1..8, defaults: hex8 elasticity 6 and 2
+ * (3 and 2 on meshes whose cells are not all congruent), plate 4 and 2,
+ * heat 3 and 2 */
 #define MASTG_OPT_MG_STEPS_FINE 2
 #define MASTG_OPT_MG_STEPS_COARSE 3
 /* multigrid smoothing interval [lam / ratio, lam] on D^-1 A (lam: the
  * Gershgorin bound for hex8 elasticity, 1.1 x a power-iteration estimate
- * otherwise); 2..10000, 0 = the default: hex8 elasticity 80, plate 150,
- * heat 7 */
+ * otherwise); 2..10000, 0 = the default: hex8 elasticity 80 (30 on meshes
+ * whose cells are not all congruent), plate 150, heat 7 */
 #define MASTG_OPT_MG_CHEB_RATIO 4
 /* hex8 elasticity FP64 assembly: 1 (default) = fused kernel by CSR row owner
  * (element blocks, residual, scatter and inverse diagonal in one pass, no

@@ -169,6 +169,7 @@ struct mastg_ctx {
     std::string mg_why;
     bool mg_valid = false;     // coarse operators match the last assembly
     int mg_k0 = 3, mg_kc = 3;  // Chebyshev steps per smoothing: problem level, coarse levels
+    bool mg_k0_set = false;    // problem-level steps chosen by the caller (MASTG_OPT_MG_STEPS_FINE)
     int mg_ratio = 0;          // Chebyshev interval [lam / ratio, lam] (0: the physics' default)
     int64_t mg_n_coarse = 0;   // DOFs of the dense coarsest level
     DevBuf mg_cheb, mg_lam, mg_part, mg_dense, mg_scal;
@@ -800,6 +801,10 @@ int mg_build(mastg_ctx* c) {
                 // elasticity: the symmetric half of the 27 3 x 3 blocks
                 TRY(dalloc(c, L->sten, size_t(h.n_nodes) * (nav == 1 ? 27 : kMGHalfStencil) * sizeof(float)));
                 L->lat0 = true;
+                // hex8 cells of several shapes: 3 problem-level steps were
+                // fastest per design (distorted 128x64x64: 3/2 55 ms, 33
+                // iterations; 4/2 56-60; 6/2 64.5, 25 iterations)
+                if (nav == 3 && !c->mg_k0_set) c->mg_k0 = 3;
             }
             if (nav == 5) {   // the plate's problem level smooths with block Jacobi
                 TRY(dalloc(c, L->dinvb, size_t(h.n_nodes) * 25 * sizeof(float)));
@@ -1026,7 +1031,12 @@ int mg_setup(mastg_ctx* c) {
             else launch_mg_to_stencil(C.a, C.sten.as<float>(), s);
         }
     }
+    // hex8 cells of several shapes (assembled problem level): [lam/30, lam]
+    // (33 iterations at 3 steps on the distorted 128x64x64 against 35 at the
+    // lattice's 80; 20 and 45 the same as 30)
+    const bool hex8_assembled = c->d.physics == MASTG_HEX8_ELASTIC && c->mg[0]->lat0;
     const double ratio = c->mg_ratio > 0                        ? double(c->mg_ratio)
+                         : hex8_assembled                         ? 30.0
                          : c->d.physics == MASTG_HEX8_ELASTIC    ? kMGChebRatioHex8
                          : c->d.physics == MASTG_QUAD4_MINDLIN ? kMGChebRatioPlate
                                                                : kMGChebRatioHeat;
@@ -2533,6 +2543,7 @@ int mastg_set_option(mastg_ctx* c, int option, int value) {
             } else {
                 if (value < 1 || value > kMGMaxCheb) return fail(c, MASTG_ERR_INVALID, "smoothing steps out of range");
                 (option == MASTG_OPT_MG_STEPS_FINE ? c->mg_k0 : c->mg_kc) = value;
+                if (option == MASTG_OPT_MG_STEPS_FINE) c->mg_k0_set = true;
             }
             // the captured V-cycles and the Chebyshev coefficients change
             drop_graphs(c);
