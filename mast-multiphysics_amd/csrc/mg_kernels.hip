@@ -397,7 +397,10 @@ __global__ __launch_bounds__(256) void k_mg_stencil(MGLevelArgs L, const float* 
                                                     const double* __restrict__ din, double* __restrict__ dout,
                                                     const int* __restrict__ done) {
     if (done && *done) return;
-    const int64_t r = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    // work-groups in XCD-aware order: the neighbouring rows whose stored
+    // blocks this row reads transposed were read by work-groups of the same
+    // XCD (its L2), not of another
+    const int64_t r = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
     if (r >= L.n_nodes) return;
     const int64_t N = L.n_nodes;
     const double* __restrict__ in = mode == MG_RESID_INIT ? L.x : din;
@@ -519,7 +522,8 @@ __global__ __launch_bounds__(256) void k_mg_stencil_heat(MGLevelArgs L, const fl
                                                          int step, const double* __restrict__ din,
                                                          double* __restrict__ dout, const int* __restrict__ done) {
     if (done && *done) return;
-    const int64_t p = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    // work-groups in XCD-aware order: neighbouring rows' state through one L2
+    const int64_t p = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
     if (p >= L.n_nodes) return;
     const int64_t N = L.n_nodes;
     const double* __restrict__ in = mode == MG_RESID_INIT ? L.x : din;
