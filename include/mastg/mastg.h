@@ -400,9 +400,10 @@ int mastg_get_tuning(const mastg_ctx* ctx, int* spmv_lanes, int* grid);
 #define MASTG_OPT_MG_STEPS_FINE 2
 #define MASTG_OPT_MG_STEPS_COARSE 3
 /* multigrid smoothing interval [lam / ratio, lam] on D^-1 A (lam: the
- * Gershgorin bound for hex8 elasticity, 1.1 x a power-iteration estimate
- * otherwise); 2..10000, 0 = the default: hex8 elasticity 80 (30 on meshes
- * whose cells are not all congruent), plate 150, heat 7 */
+ * Gershgorin bound, except 1.1 x a power-iteration estimate on every plate
+ * level and on the coarse levels of hex8 meshes whose cells are not all
+ * congruent); 2..10000, 0 = the default: hex8 elasticity 80 (30 on meshes
+ * whose cells are not all congruent), plate 150, heat 5 */
 #define MASTG_OPT_MG_CHEB_RATIO 4
 /* hex8 elasticity FP64 assembly: 1 (default) = fused kernel by CSR row owner
  * (element blocks, residual, scatter and inverse diagonal in one pass, no

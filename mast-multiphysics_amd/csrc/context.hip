@@ -716,11 +716,12 @@ constexpr double kMGPowerSafety = 1.1;
 // hex8 elasticity (Gershgorin bound) 80 (PCG iterations on the scipy
 // restatement 20-21 vs 22 with 30 at 32x16x16 and 48x24x24; C2 22 vs 24;
 // 100 and 120 the same); the plate (power-iteration bound) 150 (512^2: 290 ms
-// per design, the same up to 600, 297 with 80, 322 with 30); heat 7 (96^3
-// with 3/2 steps: 110 ms, 112 with 5 and 10, 122 with 15, 133 with 30 at 4/2)
+// per design, the same up to 600, 297 with 80, 322 with 30); heat
+// (Gershgorin bound) 5 (96^3 with 3/2 steps: 92.8 ms, 68 iterations; 96.2
+// with 7, 93.8 with 10, 100.4 with 15, 94.3 with 3)
 constexpr double kMGChebRatioHex8 = 80.0;
 constexpr double kMGChebRatioPlate = 150.0;
-constexpr double kMGChebRatioHeat = 7.0;
+constexpr double kMGChebRatioHeat = 5.0;
 
 // The hierarchy (lattice detection, coarse meshes, their node-block structure)
 // is built on the first multigrid solve; coarse operators are re-formed after
@@ -974,14 +975,10 @@ int mg_setup(mastg_ctx* c) {
                                  1, nullptr, s);
         } else if (L0.lat0) {
             // heat: the lattice-ordered stencil and inverse diagonal of this
-            // assembly, and the Chebyshev bound from them
+            // assembly; the Chebyshev bound is the Gershgorin bound above
             launch_mg_gather_stencil(L0.a, L0.a.rank_of_lat, L0.kpos.as<uint8_t>(), L0.sten.as<float>(), s);
             launch_perm_rank_lat(L0.a.n_nodes, nav, L0.a.rank_of_lat, c->dinv.as<double>(), L0.dinv_lat.as<double>(),
                                  1, nullptr, s);
-            MGLevelArgs Ll = L0.a;
-            Ll.dinv = L0.dinv_lat.as<double>();
-            launch_mg_power_heat(Ll, L0.sten.as<float>(), L0.a.rank_of_lat, kMGPowerIters, kMGPowerSafety, L0.a.da,
-                                 L0.a.db, c->mg_part.as<double>(), c->mg_scal.as<double>(), c->mg_lam.as<double>(), s);
         } else {
             if (L0.a.dinvb) launch_mg_block_dinv5(L0.a, L0.dinvb.as<float>(), s);
             launch_mg_power(L0.a, c->val.as<double>(), kMGPowerIters, kMGPowerSafety, L0.a.da, L0.a.db,
@@ -1022,7 +1019,12 @@ int mg_setup(mastg_ctx* c) {
         launch_scatter(sa, s);
         launch_mg_rowinfo_nb(nav, C.a.n_nodes, C.a.blk_ptr, C.a.blk_col, C.a.val, C.dinv.as<double>(),
                              c->mg_part.as<double>(), c->mg_lam.as<double>() + l + 1, s);
-        if (c->mg_power)
+        // heat: the Gershgorin bound above (C4 96^3: 96 ms per design against
+        // 104 with 20-step power iterations on every level of every Newton
+        // step's hierarchy); plates (a loose Gershgorin bound) and hex8 on
+        // cells of several shapes (2 PCG iterations fewer than with the
+        // Gershgorin bound at 128 x 64 x 64): power iteration
+        if (c->d.physics != MASTG_HEX8_HEAT)
             launch_mg_power(C.a, C.a.val, kMGPowerIters, kMGPowerSafety, C.a.da, C.a.db, c->mg_part.as<double>(),
                             c->mg_scal.as<double>(), c->mg_lam.as<double>() + l + 1, s);
         launch_to_f32(int64_t(C.val32.bytes / sizeof(float)), C.a.val, C.val32.as<float>(), s);
