@@ -52,6 +52,24 @@ int main(int argc, char** argv) {
         double gmax = 0;
         for (int64_t e = 0; e < ne; e++) gmax = std::fmax(gmax, std::fabs(g_dir[e] + g_adj[e]));
 
+        // the weighted p-norm of the von Mises stress (StressStrainOutputBase,
+        // example_6's stress constraint): dF/dX along X against a central
+        // difference of the value
+        MAST::GPU::StressStrainOutputBase pn;
+        pn.set_aggregation_coefficients(6.0, 1.0, 0.5, 2.0e6);
+        const double F0 = pn.evaluate(assembly, X);
+        MAST::GPU::RealVector dF, Xp(X), Xm(X);
+        pn.output_derivative(assembly, X, dF);
+        const double h = 1e-6;
+        double an = 0;
+        for (size_t i = 0; i < X.size(); i++) {
+            Xp[i] *= 1 + h;
+            Xm[i] *= 1 - h;
+            an += dF[i] * X[i];
+        }
+        const double fd = (pn.evaluate(assembly, Xp) - pn.evaluate(assembly, Xm)) / (2 * h);
+        const double e3 = std::fabs(fd - an) / std::fabs(an);
+
         // compliance and its E derivative, forward and adjoint
         MAST::GPU::ComplianceOutput compliance;
         const double C = compliance.evaluate(assembly, X);
@@ -65,10 +83,10 @@ int main(int argc, char** argv) {
         const double e1 = std::fabs(fwd - adj_E) / std::fabs(adj_E);
         const double e2 = std::fabs(adj_E + C / E) / std::fabs(C / E);   // C is homogeneous of degree -1 in E
         std::printf("{\"n_elems\": %lld, \"volume\": %.12g, \"compliance\": %.12g, \"ks\": %.12g, "
-                    "\"ks_grad_max\": %.6g, \"dCdE_forward_vs_adjoint\": %.3g, \"dCdE_vs_minus_C_over_E\": %.3g, "
-                    "\"iters\": %d}\n",
-                    (long long)ne, vol, C, k, gmax, e1, e2, system.last.iters);
-        return (e1 < 1e-8 && e2 < 1e-8 && std::isfinite(k) && gmax > 0) ? 0 : 1;
+                    "\"ks_grad_max\": %.6g, \"pnorm\": %.12g, \"pnorm_dX_vs_fd\": %.3g, "
+                    "\"dCdE_forward_vs_adjoint\": %.3g, \"dCdE_vs_minus_C_over_E\": %.3g, \"iters\": %d}\n",
+                    (long long)ne, vol, C, k, gmax, F0, e3, e1, e2, system.last.iters);
+        return (e1 < 1e-8 && e2 < 1e-8 && e3 < 1e-6 && std::isfinite(k) && gmax > 0 && F0 > 0) ? 0 : 1;
     } catch (const std::exception& ex) {
         std::fprintf(stderr, "error: %s\n", ex.what());
         return 3;

@@ -31,7 +31,8 @@ def test_mast_call_sequence(nx):
 
 def test_nastran_ks_forward_adjoint_sequence():
     """NastranIO::read -> FilterBase -> solve -> KSStressStrainOutput with its
-    adjoint -> NonlinearSystem::sensitivity_solve for E, through the C++
+    adjoint -> StressStrainOutputBase (p-norm) -> NonlinearSystem::
+    sensitivity_solve for E, through the C++
     layer (mast-multiphysics_amd/host/example_nastran_ks.cpp): the forward and
     adjoint dC/dE agree, and equal -C/E (K is linear in E)."""
     print("[C++ layer] Nastran model, KS stress, forward sensitivity", flush=True)
@@ -43,3 +44,6 @@ def test_nastran_ks_forward_adjoint_sequence():
     out = json.loads(p.stdout.strip().splitlines()[-1])
     assert out["dCdE_forward_vs_adjoint"] < 1e-8 and out["dCdE_vs_minus_C_over_E"] < 1e-8
     assert out["ks_grad_max"] > 0
+    # StressStrainOutputBase (weighted p-norm) through the same layer: its
+    # state derivative along X against a central difference of its value
+    assert out["pnorm"] > 0 and out["pnorm_dX_vs_fd"] < 1e-6
