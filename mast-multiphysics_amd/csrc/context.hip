@@ -135,6 +135,7 @@ struct mastg_ctx {
     bool row_fused = true;           // fused row-owner assembly allowed (MASTG_OPT_ROW_ASSEMBLY)
     bool row_plan_ok = true;         // ... over the row plan where it applies (MASTG_OPT_ROW_ASSEMBLY 1)
     DevBuf row_pos;                  // the class assembly's row plan + node constraint bits (mesh-static)
+    bool row_plan_tried = false;     // whether the mesh qualifies for the plan has been decided
     double* scratch_ptr = nullptr;   // this device's `scratch` (allocation-free output)
     double ks_k = 10.0, ks_sref = 1.0;   // KS stress output: aggregation constant, reference stress
     // p-norm and smooth-ramp stress outputs: p, weight exponent, rho, reference stress
@@ -485,7 +486,8 @@ int assemble_active(mastg_ctx* c, const double* X_active, bool want_R, bool want
         a.R = want_R ? c->vtmp.as<double>() : nullptr;
         a.dinv = c->dinv.as<double>();
         // the class kernel's row plan: at most 8 elements per node
-        if (a.Khat && c->row_plan_ok && !c->row_pos.p) {
+        if (a.Khat && c->row_plan_ok && !c->row_pos.p && !c->row_plan_tried) {
+            c->row_plan_tried = true;   // the mesh does not change: decided once
             int32_t max_inc = 0;
             for (int64_t r = 0; r < S.n_nodes; r++)
                 max_inc = std::max(max_inc, S.ninc_ptr[r + 1] - S.ninc_ptr[r]);
