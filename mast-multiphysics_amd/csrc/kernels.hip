@@ -264,14 +264,16 @@ __global__ __launch_bounds__(256) void k_scatter_blocks_f32(ScatterArgs S, const
     if (r >= S.n_nodes) return;
     const int nav = S.nav, nav2 = nav * nav;
     const int64_t k0 = S.blk_ptr[r];
-    const int64_t deg = S.blk_ptr[r + 1] - k0;
-    const int64_t cnt = deg * nav2;
+    // a row's chunk is small (at most a few thousand values): 32-bit index
+    // arithmetic, the divisors uniform across the wave
+    const int deg = int(S.blk_ptr[r + 1] - k0);
+    const int cnt = deg * nav2, dn = deg * nav;
     const int64_t base = k0 * nav2;
-    for (int64_t t = lane; t < cnt; t += 64) {
-        const int64_t v = t / (deg * nav);
-        const int64_t rem = t - v * deg * nav;
-        const int64_t k = rem / nav;
-        const int64_t w = rem - k * nav;
+    for (int t = lane; t < cnt; t += 64) {
+        const int v = t / dn;
+        const int rem = t - v * dn;
+        const int k = rem / nav;
+        const int w = rem - k * nav;
         const int64_t blk = k0 + k;
         double s = 0.0;
         for (int32_t c = S.cptr[blk]; c < S.cptr[blk + 1]; c++)
@@ -904,14 +906,16 @@ __global__ __launch_bounds__(256) void k_scatter_blocks(ScatterArgs S) {
     if (r >= S.n_nodes) return;
     const int nav = S.nav, nav2 = nav * nav;
     const int64_t k0 = S.blk_ptr[r];
-    const int64_t deg = S.blk_ptr[r + 1] - k0;
-    const int64_t cnt = deg * nav2;
+    // a row's chunk is small (at most a few thousand values): 32-bit index
+    // arithmetic, the divisors uniform across the wave
+    const int deg = int(S.blk_ptr[r + 1] - k0);
+    const int cnt = deg * nav2, dn = deg * nav;
     const int64_t base = k0 * nav2;
-    for (int64_t t = lane; t < cnt; t += 64) {
-        const int64_t v = t / (deg * nav);
-        const int64_t rem = t - v * deg * nav;
-        const int64_t k = rem / nav;
-        const int64_t w = rem - k * nav;
+    for (int t = lane; t < cnt; t += 64) {
+        const int v = t / dn;
+        const int rem = t - v * dn;
+        const int k = rem / nav;
+        const int w = rem - k * nav;
         const int64_t blk = k0 + k;
         double s = 0.0;
         for (int32_t c = S.cptr[blk]; c < S.cptr[blk + 1]; c++)
