@@ -161,6 +161,7 @@ struct mastg_ctx {
     hipGraphExec_t cg_exec[3] = {nullptr, nullptr, nullptr};   // CSR, matrix-free, FP32 node-block
     int cg_exec_len[3] = {0, 0, 0};
     DevBuf val32;              // FP32 copy of val for the mixed-precision solver
+    DevBuf val16, rs16;        // plate problem level: row-scaled FP16 copy of val and its row scales
     bool val32_valid = false;
     int cg_replacements = 0;
     bool kel64_valid = false;  // FP64 element blocks of the last assembly are resident
@@ -966,6 +967,16 @@ int mg_setup(mastg_ctx* c) {
         launch_mg_rowinfo_nb(nav, c->S.n_nodes, c->blk_ptr.as<int64_t>(), c->blk_col.as<int32_t>(),
                              c->val.as<double>(), nullptr, c->mg_part.as<double>(), c->mg_lam.as<double>(), s);
         launch_to_f32(c->S.nnz, c->val.as<double>(), c->val32.as<float>(), s);
+        if (nav == 5 && L0.a.dinvb) {
+            // the plate's problem-level smoother reads the values in FP16,
+            // each block row scaled by its largest magnitude
+            TRY(dalloc(c, c->val16, size_t(c->S.nnz) * sizeof(uint16_t)));
+            TRY(dalloc(c, c->rs16, size_t(c->S.n_active) * sizeof(float)));
+            launch_to_f16_rows(c->S.n_nodes, nav, c->blk_ptr.as<int64_t>(), c->val.as<double>(),
+                               c->val16.as<uint16_t>(), c->rs16.as<float>(), s);
+            L0.a.val16 = c->val16.as<uint16_t>();
+            L0.a.rs16 = c->rs16.as<float>();
+        }
         if (L0.lat0 && nav == 3) {
             // hex8 elasticity on cells of several shapes: the symmetric half
             // stencil and the inverse diagonal of this assembly in lattice

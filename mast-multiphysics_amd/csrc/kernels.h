@@ -287,6 +287,10 @@ struct MGLevelArgs {
     const int32_t* blk_col;
     const double* val;
     const float* val32;                // FP32 copy of val read by the smoother (null = FP64)
+    // plate problem level: val in FP16 (raw bits), each block row divided by
+    // its largest magnitude rs16[row] (null = the FP32 copy)
+    const uint16_t* val16;
+    const float* rs16;
     const double* dinv;                // [nav n_nodes] inverse diagonal
     double *r, *x, *res, *da, *db;     // right side, iterate, residual, smoothing directions
     const double* cheb;                // this level's Chebyshev coefficients
@@ -338,6 +342,9 @@ void launch_mg_dense_gather(const MGLevelArgs& L, double* A, hipStream_t s);
 void launch_mg_invert(int n, double* A, hipStream_t s);
 // V-cycle pieces; every kernel returns at once when *done != 0 (done may be null)
 void launch_mg_pre_init(const MGLevelArgs& L, const int* done, hipStream_t s);
+// the FP16 row-scaled copy of a node-block matrix (val16, rs16 as above)
+void launch_to_f16_rows(int64_t n_nodes, int nav, const int64_t* blk_ptr, const double* val, uint16_t* val16,
+                        float* rs16, hipStream_t s);
 void launch_mg_bsr(const MGLevelArgs& L, int mode, int step, const double* din, double* dout, const int* done,
                    hipStream_t s);
 // a lattice-ordered hex8 coarse level as a 27-point FP32 stencil

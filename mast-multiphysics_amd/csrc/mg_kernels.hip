@@ -5,6 +5,7 @@
 //
 // Every reduction is a fixed-order sum (no atomics), so a solve is bitwise
 // reproducible for a given launch configuration.
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -282,8 +283,32 @@ __global__ __launch_bounds__(256) void k_mg_pre_init32(MGLevelArgs L, const int*
     }
 }
 
-// node-block operator of a level (values T = double or their FP32 copy;
-// products and sums in FP64), 8 lanes per node, fused epilogue
+__device__ __forceinline__ double mg_val(double v) { return v; }
+__device__ __forceinline__ double mg_val(float v) { return double(v); }
+__device__ __forceinline__ double mg_val(uint16_t v) { return double(__half2float(__ushort_as_half(v))); }
+
+// FP16 copy of a node-block matrix, each block row (node r, variable v: the
+// deg x NAV values V[(v deg + k) NAV + w]) divided by its largest magnitude
+template <int NAV>
+__global__ __launch_bounds__(256) void k_to_f16_rows(int64_t n_nodes, const int64_t* __restrict__ blk_ptr,
+                                                     const double* __restrict__ val, uint16_t* __restrict__ val16,
+                                                     float* __restrict__ rs16) {
+    const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (i >= n_nodes * NAV) return;
+    const int64_t r = i / NAV, v = i - r * NAV;
+    const int64_t k0 = blk_ptr[r];
+    const int64_t deg = blk_ptr[r + 1] - k0;
+    const int64_t b = k0 * NAV * NAV + v * deg * NAV, n = deg * NAV;
+    double m = 0.0;
+    for (int64_t t = 0; t < n; t++) m = fmax(m, fabs(val[b + t]));
+    const float d = m > 0.0 ? float(m) : 1.0f;
+    rs16[i] = d;
+    for (int64_t t = 0; t < n; t++) val16[b + t] = __half_as_ushort(__float2half(float(val[b + t] / double(d))));
+}
+
+// node-block operator of a level (values T = double, their FP32 copy or the
+// row-scaled FP16 copy; products and sums in FP64), 8 lanes per node, fused
+// epilogue
 template <typename T, int NAV>
 __global__ __launch_bounds__(256) void k_mg_bsr(MGLevelArgs L, const T* __restrict__ vals, int mode, int step,
                                                 const double* __restrict__ din, double* __restrict__ dout,
@@ -310,13 +335,16 @@ __global__ __launch_bounds__(256) void k_mg_bsr(MGLevelArgs L, const T* __restri
             for (int v = 0; v < NAV; v++) {
                 const T* vr = V + (int64_t(v) * deg + k) * NAV;
 #pragma unroll
-                for (int w = 0; w < NAV; w++) sv[v] += double(vr[w]) * xw[w];
+                for (int w = 0; w < NAV; w++) sv[v] += mg_val(vr[w]) * xw[w];
             }
         }
 #pragma unroll
         for (int v = 0; v < NAV; v++)
 #pragma unroll
             for (int m = 4; m >= 1; m >>= 1) sv[v] += __shfl_xor(sv[v], m, 8);
+        if constexpr (sizeof(T) == 2)
+#pragma unroll
+            for (int v = 0; v < NAV; v++) sv[v] *= double(L.rs16[r * NAV + v]);
         if (NAV == 5 && L.dinvb) {
             // block Jacobi: every lane of the node's group holds the 5 rows y;
             // each forms the 5 new residuals and lane v < 5 its direction entry
@@ -1506,11 +1534,20 @@ void launch_mg_combine_rank(int64_t n_nodes, int nav, const int32_t* rank_of_lat
     hipLaunchKernelGGL(k_mg_combine_rank, dim3(mg_blocks(n_nodes * nav, 256)), dim3(256), 0, s, n_nodes, nav,
                        rank_of_lat, x0, dlast, out, done);
 }
+void launch_to_f16_rows(int64_t n_nodes, int nav, const int64_t* blk_ptr, const double* val, uint16_t* val16,
+                        float* rs16, hipStream_t s) {
+    if (nav != 5) return;
+    hipLaunchKernelGGL(k_to_f16_rows<5>, dim3(unsigned((n_nodes * 5 + 255) / 256)), dim3(256), 0, s, n_nodes, blk_ptr,
+                       val, val16, rs16);
+}
 void launch_mg_bsr(const MGLevelArgs& L, int mode, int step, const double* din, double* dout, const int* done,
                    hipStream_t s) {
     const unsigned g = unsigned(nb_grid(L.n_nodes));
     if (L.nav == 5) {
-        if (L.val32)
+        if (L.val16)
+            hipLaunchKernelGGL((k_mg_bsr<uint16_t, 5>), dim3(g), dim3(256), 0, s, L, L.val16, mode, step, din, dout,
+                               done);
+        else if (L.val32)
             hipLaunchKernelGGL((k_mg_bsr<float, 5>), dim3(g), dim3(256), 0, s, L, L.val32, mode, step, din, dout, done);
         else
             hipLaunchKernelGGL((k_mg_bsr<double, 5>), dim3(g), dim3(256), 0, s, L, L.val, mode, step, din, dout, done);
