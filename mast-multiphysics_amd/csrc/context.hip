@@ -107,6 +107,7 @@ struct MGLevelDev {
                                   // heat levels 0 and 1: 27-point stencils in lattice order
     DevBuf kpos;                  // heat level 0: row position of each stencil entry (255: none)
     DevBuf dinvb;                 // plate level 0: inverses of the 5 x 5 diagonal node blocks
+    DevBuf sten16, srow16;        // heat level 0: the stencil in FP16 and its row scales
     bool lat0 = false;            // heat / hex8 level 0 smooths in lattice order
     DevBuf Kel, val, val32, dinv, r, x, res, da, db;
 };
@@ -992,6 +993,13 @@ int mg_setup(mastg_ctx* c) {
             launch_mg_gather_stencil(L0.a, L0.a.rank_of_lat, L0.kpos.as<uint8_t>(), L0.sten.as<float>(), s);
             launch_perm_rank_lat(L0.a.n_nodes, nav, L0.a.rank_of_lat, c->dinv.as<double>(), L0.dinv_lat.as<double>(),
                                  1, nullptr, s);
+            // the smoother reads it in FP16, each point's row scaled by its
+            // largest magnitude
+            TRY(dalloc(c, L0.sten16, size_t(L0.a.n_nodes) * 27 * sizeof(uint16_t)));
+            TRY(dalloc(c, L0.srow16, size_t(L0.a.n_nodes) * sizeof(float)));
+            launch_sten27_to_f16(L0.a.n_nodes, L0.sten.as<float>(), L0.sten16.as<uint16_t>(), L0.srow16.as<float>(), s);
+            L0.a.val16 = L0.sten16.as<uint16_t>();
+            L0.a.rs16 = L0.srow16.as<float>();
         } else {
             if (L0.a.dinvb) launch_mg_block_dinv5(L0.a, L0.dinvb.as<float>(), s);
             launch_mg_power(L0.a, c->val.as<double>(), kMGPowerIters, kMGPowerSafety, L0.a.da, L0.a.db,

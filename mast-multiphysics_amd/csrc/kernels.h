@@ -287,8 +287,9 @@ struct MGLevelArgs {
     const int32_t* blk_col;
     const double* val;
     const float* val32;                // FP32 copy of val read by the smoother (null = FP64)
-    // plate problem level: val in FP16 (raw bits), each block row divided by
-    // its largest magnitude rs16[row] (null = the FP32 copy)
+    // the smoother's values in FP16 (raw bits), each row divided by its
+    // largest magnitude rs16[row] (null = FP32): the plate problem level's
+    // node blocks, the heat problem level's 27-point stencil
     const uint16_t* val16;
     const float* rs16;
     const double* dinv;                // [nav n_nodes] inverse diagonal
@@ -342,6 +343,8 @@ void launch_mg_dense_gather(const MGLevelArgs& L, double* A, hipStream_t s);
 void launch_mg_invert(int n, double* A, hipStream_t s);
 // V-cycle pieces; every kernel returns at once when *done != 0 (done may be null)
 void launch_mg_pre_init(const MGLevelArgs& L, const int* done, hipStream_t s);
+// the FP16 row-scaled copy of a lattice-order 27-point stencil ([27][n])
+void launch_sten27_to_f16(int64_t n, const float* sten, uint16_t* sten16, float* srow, hipStream_t s);
 // the FP16 row-scaled copy of a node-block matrix (val16, rs16 as above)
 void launch_to_f16_rows(int64_t n_nodes, int nav, const int64_t* blk_ptr, const double* val, uint16_t* val16,
                         float* rs16, hipStream_t s);

@@ -546,7 +546,22 @@ __global__ __launch_bounds__(256) void k_mg_gather_stencil3(MGLevelArgs L, const
 
 // smoothing step of the heat problem level in lattice order (FP32 stencil, FP64
 // vectors and sums, fused Chebyshev epilogue)
-__global__ __launch_bounds__(256) void k_mg_stencil_heat(MGLevelArgs L, const float* __restrict__ sten, int mode,
+// the heat problem level's stencil in FP16: each point's 27 entries divided
+// by their largest magnitude (srow[p])
+__global__ __launch_bounds__(256) void k_sten27_to_f16(int64_t N, const float* __restrict__ sten,
+                                                       uint16_t* __restrict__ sten16, float* __restrict__ srow) {
+    const int64_t p = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (p >= N) return;
+    float m = 0.f;
+    for (int o = 0; o < 27; o++) m = fmaxf(m, fabsf(sten[o * N + p]));
+    const float d = m > 0.f ? m : 1.f;
+    srow[p] = d;
+    for (int o = 0; o < 27; o++) sten16[o * N + p] = __half_as_ushort(__float2half(sten[o * N + p] / d));
+}
+
+// T = float: the stencil; T = uint16_t: its FP16 copy, rows scaled by L.rs16
+template <typename T>
+__global__ __launch_bounds__(256) void k_mg_stencil_heat(MGLevelArgs L, const T* __restrict__ sten, int mode,
                                                          int step, const double* __restrict__ din,
                                                          double* __restrict__ dout, const int* __restrict__ done) {
     if (done && *done) return;
@@ -564,8 +579,9 @@ __global__ __launch_bounds__(256) void k_mg_stencil_heat(MGLevelArgs L, const fl
         const int ni = min(max(i + o % 3 - 1, 0), L.nx), nj = min(max(j + (o / 3) % 3 - 1, 0), L.ny),
                   nk = min(max(k + o / 9 - 1, 0), L.nz);
         const int64_t m = int64_t(uint32_t(ni) + nxp * (uint32_t(nj) + nyp * uint32_t(nk)));
-        y = fma(double(sten[o * N + p]), in[m], y);
+        y = fma(mg_val(sten[o * N + p]), in[m], y);
     }
+    if constexpr (sizeof(T) == 2) y *= double(L.rs16[p]);
     const double ith = L.cheb[0];
     const double c1 = step > 0 ? L.cheb[2 * step] : 0.0, c2 = step > 0 ? L.cheb[2 * step + 1] : 0.0;
     mg_epilogue(L, mode, p, y, din, dout, c1, c2, ith);
@@ -1505,8 +1521,15 @@ void launch_mg_gather_stencil(const MGLevelArgs& L, const int32_t* rank_of_lat, 
 }
 void launch_mg_stencil_rank(const MGLevelArgs& L, const float* sten, int mode, int step, const double* din,
                             double* dout, const int* done, hipStream_t s) {
-    hipLaunchKernelGGL(k_mg_stencil_heat, dim3(mg_blocks(L.n_nodes, 256)), dim3(256), 0, s, L, sten, mode, step, din,
-                       dout, done);
+    if (L.val16)
+        hipLaunchKernelGGL(k_mg_stencil_heat<uint16_t>, dim3(mg_blocks(L.n_nodes, 256)), dim3(256), 0, s, L, L.val16,
+                           mode, step, din, dout, done);
+    else
+        hipLaunchKernelGGL(k_mg_stencil_heat<float>, dim3(mg_blocks(L.n_nodes, 256)), dim3(256), 0, s, L, sten, mode,
+                           step, din, dout, done);
+}
+void launch_sten27_to_f16(int64_t n, const float* sten, uint16_t* sten16, float* srow, hipStream_t s) {
+    hipLaunchKernelGGL(k_sten27_to_f16, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, n, sten, sten16, srow);
 }
 void launch_mg_power_heat(const MGLevelArgs& L, const float* sten, const int32_t* rank_of_lat, int n_it,
                           double safety, double* w0, double* w1, double* part, double* scal, double* lam,
