@@ -107,7 +107,8 @@ struct MGLevelDev {
                                   // heat levels 0 and 1: 27-point stencils in lattice order
     DevBuf kpos;                  // heat level 0: row position of each stencil entry (255: none)
     DevBuf dinvb;                 // plate level 0: inverses of the 5 x 5 diagonal node blocks
-    DevBuf sten16, srow16;        // heat level 0: the stencil in FP16 and its row scales
+    DevBuf sten16, srow16;        // heat level 0: the stencil in FP16 and its row scales; plate
+                                  // level 1: its node blocks in FP16 and their row scales
     bool lat0 = false;            // heat / hex8 level 0 smooths in lattice order
     DevBuf Kel, val, val32, dinv, r, x, res, da, db;
 };
@@ -1049,6 +1050,14 @@ int mg_setup(mastg_ctx* c) {
             launch_mg_power(C.a, C.a.val, kMGPowerIters, kMGPowerSafety, C.a.da, C.a.db, c->mg_part.as<double>(),
                             c->mg_scal.as<double>(), c->mg_lam.as<double>() + l + 1, s);
         launch_to_f32(int64_t(C.val32.bytes / sizeof(float)), C.a.val, C.val32.as<float>(), s);
+        if (nav == 5 && l == 0) {
+            // the plate's first coarse level smooths in FP16 too (row scales)
+            TRY(dalloc(c, C.sten16, size_t(C.val32.bytes / sizeof(float)) * sizeof(uint16_t)));
+            TRY(dalloc(c, C.srow16, size_t(C.a.n_nodes) * 5 * sizeof(float)));
+            launch_to_f16_rows(C.a.n_nodes, nav, C.a.blk_ptr, C.a.val, C.sten16.as<uint16_t>(), C.srow16.as<float>(), s);
+            C.a.val16 = C.sten16.as<uint16_t>();
+            C.a.rs16 = C.srow16.as<float>();
+        }
         if (C.sten.p) {
             if (nav == 1) launch_mg_to_stencil27(C.a, C.sten.as<float>(), s);
             else launch_mg_to_stencil(C.a, C.sten.as<float>(), s);
